@@ -1,0 +1,1428 @@
+/*
+ * sbp_oracle.c -- CPU restatement of vtsynergy/SBP's hot path under the
+ * replay contract described in sbp_oracle.h.  Test infrastructure only:
+ * it is the parity checker and the timed CPU baseline.
+ *
+ * Structure follows the reference closely on purpose (a Delta of four maps
+ * with zero-net entries kept, "entries" plus "unchanged cells" loops,
+ * incremental commits), so that the GPU's histogram-based formulation is
+ * checked against an independent derivation.
+ */
+#define _GNU_SOURCE
+#include "sbp_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#include <limits.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as    */
+/* easy as 1, 2, 3", SC'11).                                                 */
+/* ------------------------------------------------------------------------- */
+void orc_philox(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t out[4]) {
+    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    uint32_t x0 = c0, x1 = c1, x2 = c2, x3 = c3;
+    for (int round = 0; round < 10; ++round) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * x0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * x2;
+        uint32_t y0 = (uint32_t)(p1 >> 32) ^ x1 ^ k0;
+        uint32_t y1 = (uint32_t)p1;
+        uint32_t y2 = (uint32_t)(p0 >> 32) ^ x3 ^ k1;
+        uint32_t y3 = (uint32_t)p0;
+        x0 = y0; x1 = y1; x2 = y2; x3 = y3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
+}
+
+uint32_t orc_uniform_below(uint32_t w, uint32_t n) { return (uint32_t)(((uint64_t)w * n) >> 32); }
+
+double orc_unit_double(uint32_t a, uint32_t b) {
+    uint64_t u = (((uint64_t)a << 32) | b) >> 11;
+    return (double)u * 0x1p-53;
+}
+
+static int32_t random_block(int32_t current, int32_t B, uint32_t w) {
+    /* common::propose_random_block (src/common.cpp:200-209): candidates = U[0, B-2] */
+    int32_t proposed = (int32_t)orc_uniform_below(w, (uint32_t)(B - 1));
+    if (proposed >= current) proposed++;
+    return proposed;
+}
+
+/* glibc logf over a contiguous range of float bit patterns (for the device logf check). */
+void orc_logf_range(uint32_t first_bits, uint32_t count, float *out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t bits = first_bits + i;
+        float x;
+        memcpy(&x, &bits, 4);
+        out[i] = logf(x);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* 128-bit fixed point accumulation (scale 2^64, truncation toward zero).    */
+/* ------------------------------------------------------------------------- */
+typedef __int128 fx128;
+
+static fx128 fx_from(double d) {
+    uint64_t bits;
+    memcpy(&bits, &d, 8);
+    int neg = (int)(bits >> 63);
+    int e = (int)((bits >> 52) & 0x7ff);
+    uint64_t m = bits & 0xfffffffffffffull;
+    if (e == 0) return 0; /* zero or subnormal: below the 2^-64 grid */
+    m |= 1ull << 52;
+    int shift = e - 1075 + 64; /* value * 2^64 = m * 2^shift */
+    unsigned __int128 mag;
+    if (shift >= 0) {
+        if (shift > 74) abort(); /* |d| >= 2^63: outside the contract's range */
+        mag = (unsigned __int128)m << shift;
+    } else if (shift > -64) {
+        mag = (unsigned __int128)(m >> (-shift));
+    } else {
+        mag = 0;
+    }
+    fx128 v = (fx128)mag;
+    return neg ? -v : v;
+}
+
+static double fx_to(fx128 v) {
+    int64_t hi = (int64_t)(v >> 64);
+    uint64_t lo = (uint64_t)v;
+    return (double)hi + (double)lo * 0x1p-64;
+}
+
+void orc_fx_from_double(double d, uint64_t out[2]) {
+    fx128 v = fx_from(d);
+    out[0] = (uint64_t)v;
+    out[1] = (uint64_t)(v >> 64);
+}
+double orc_fx_to_double(int64_t hi, uint64_t lo) { return (double)hi + (double)lo * 0x1p-64; }
+
+/* fastlog (include/fastlog.hpp:21-30): logf of the integer, 0 for 0. */
+static inline double fastlog(int64_t x) {
+    if (x == 0) return 0.0;
+    return (double)logf((float)x);
+}
+
+/* common::cell_entropy (include/common.hpp:39-49). */
+static inline double cell_entropy(int64_t value, int64_t d_in, int64_t d_out) {
+    if (value == 0) return 0.0;
+    double e = (double)value * (fastlog(value) - (fastlog(d_in) + fastlog(d_out)));
+    if (isnan(e) || isinf(e)) abort();
+    return e;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Sorted sparse vectors (one per row and one per column of the block matrix) */
+/* ------------------------------------------------------------------------- */
+typedef struct { int32_t *idx; int64_t *val; int32_t len, cap; } svec;
+
+static int32_t sv_find(const svec *v, int32_t key) {
+    int32_t lo = 0, hi = v->len;
+    while (lo < hi) {
+        int32_t mid = (lo + hi) >> 1;
+        if (v->idx[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+static int64_t sv_get(const svec *v, int32_t key) {
+    int32_t p = sv_find(v, key);
+    return (p < v->len && v->idx[p] == key) ? v->val[p] : 0;
+}
+/* Adds delta; erases the cell when it reaches zero (DictTransposeMatrix::sub / update_edge_counts). */
+static void sv_add(svec *v, int32_t key, int64_t delta) {
+    int32_t p = sv_find(v, key);
+    if (p < v->len && v->idx[p] == key) {
+        v->val[p] += delta;
+        if (v->val[p] == 0) {
+            memmove(v->idx + p, v->idx + p + 1, sizeof(int32_t) * (size_t)(v->len - p - 1));
+            memmove(v->val + p, v->val + p + 1, sizeof(int64_t) * (size_t)(v->len - p - 1));
+            v->len--;
+        }
+        return;
+    }
+    if (delta == 0) return;
+    if (v->len == v->cap) {
+        v->cap = v->cap ? v->cap * 2 : 4;
+        v->idx = (int32_t *)realloc(v->idx, sizeof(int32_t) * (size_t)v->cap);
+        v->val = (int64_t *)realloc(v->val, sizeof(int64_t) * (size_t)v->cap);
+    }
+    memmove(v->idx + p + 1, v->idx + p, sizeof(int32_t) * (size_t)(v->len - p));
+    memmove(v->val + p + 1, v->val + p, sizeof(int64_t) * (size_t)(v->len - p));
+    v->idx[p] = key;
+    v->val[p] = delta;
+    v->len++;
+}
+static void sv_copy(svec *dst, const svec *src) {
+    dst->len = src->len;
+    dst->cap = src->len;
+    dst->idx = src->len ? (int32_t *)malloc(sizeof(int32_t) * (size_t)src->len) : NULL;
+    dst->val = src->len ? (int64_t *)malloc(sizeof(int64_t) * (size_t)src->len) : NULL;
+    if (src->len) {
+        memcpy(dst->idx, src->idx, sizeof(int32_t) * (size_t)src->len);
+        memcpy(dst->val, src->val, sizeof(int64_t) * (size_t)src->len);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Blockmodel (include/blockmodel/blockmodel.hpp:46-219)                      */
+/* ------------------------------------------------------------------------- */
+struct orc_bm {
+    int32_t B, N;
+    int32_t *assign;
+    int32_t *size;
+    int64_t *d_out, *d_in, *d_tot;
+    svec *row, *col;
+    double overall_entropy;
+    int64_t num_to_merge;
+    int empty;
+};
+
+static void bm_add(orc_bm *bm, int32_t r, int32_t c, int64_t d) {
+    sv_add(&bm->row[r], c, d);
+    sv_add(&bm->col[c], r, d);
+}
+static inline int64_t bm_get(const orc_bm *bm, int32_t r, int32_t c) {
+    const svec *a = &bm->row[r], *b = &bm->col[c];
+    return a->len <= b->len ? sv_get(a, c) : sv_get(b, r);
+}
+
+static orc_bm *bm_alloc(int32_t B, int32_t N) {
+    orc_bm *bm = (orc_bm *)calloc(1, sizeof(orc_bm));
+    bm->B = B;
+    bm->N = N;
+    bm->assign = (int32_t *)calloc((size_t)N, sizeof(int32_t));
+    bm->size = (int32_t *)calloc((size_t)B, sizeof(int32_t));
+    bm->d_out = (int64_t *)calloc((size_t)B, sizeof(int64_t));
+    bm->d_in = (int64_t *)calloc((size_t)B, sizeof(int64_t));
+    bm->d_tot = (int64_t *)calloc((size_t)B, sizeof(int64_t));
+    bm->row = (svec *)calloc((size_t)B, sizeof(svec));
+    bm->col = (svec *)calloc((size_t)B, sizeof(svec));
+    bm->overall_entropy = DBL_MAX;
+    bm->num_to_merge = (int64_t)((double)B * 0.5);  /* BLOCK_REDUCTION_RATE, blockmodel.hpp:70-76 */
+    return bm;
+}
+
+void orc_bm_free(orc_bm *bm) {
+    if (!bm) return;
+    for (int32_t b = 0; b < bm->B; ++b) {
+        free(bm->row[b].idx); free(bm->row[b].val);
+        free(bm->col[b].idx); free(bm->col[b].val);
+    }
+    free(bm->row); free(bm->col);
+    free(bm->assign); free(bm->size); free(bm->d_out); free(bm->d_in); free(bm->d_tot);
+    free(bm);
+}
+
+/* Blockmodel::initialize_edge_counts (src/blockmodel/blockmodel.cpp:193-276). */
+static void bm_fill(orc_bm *bm, const orc_graph *g) {
+    for (int32_t v = 0; v < g->N; ++v) {
+        int32_t b = bm->assign[v];
+        bm->size[b]++;
+        for (int64_t e = g->out_off[v]; e < g->out_off[v + 1]; ++e) {
+            int32_t nb = bm->assign[g->out_adj[e]];
+            bm_add(bm, b, nb, 1);
+            bm->d_out[b] += 1;
+            bm->d_tot[b] += 1;
+        }
+        for (int64_t e = g->in_off[v]; e < g->in_off[v + 1]; ++e) {
+            int32_t nb = bm->assign[g->in_adj[e]];
+            bm->d_in[b] += 1;
+            if (b != nb) bm->d_tot[b] += 1;
+        }
+    }
+}
+
+orc_bm *orc_bm_new(const orc_graph *g, int32_t B, const int32_t *assignment) {
+    orc_bm *bm = bm_alloc(B, g->N);
+    memcpy(bm->assign, assignment, sizeof(int32_t) * (size_t)g->N);
+    bm_fill(bm, g);
+    return bm;
+}
+
+/* Blockmodel::copy (src/blockmodel/blockmodel.cpp:139-153): num_blocks_to_merge resets to 0. */
+orc_bm *orc_bm_copy(const orc_bm *src) {
+    orc_bm *bm = (orc_bm *)calloc(1, sizeof(orc_bm));
+    *bm = *src;
+    bm->assign = (int32_t *)malloc(sizeof(int32_t) * (size_t)src->N);
+    memcpy(bm->assign, src->assign, sizeof(int32_t) * (size_t)src->N);
+    bm->size = (int32_t *)malloc(sizeof(int32_t) * (size_t)src->B);
+    memcpy(bm->size, src->size, sizeof(int32_t) * (size_t)src->B);
+    bm->d_out = (int64_t *)malloc(sizeof(int64_t) * (size_t)src->B);
+    memcpy(bm->d_out, src->d_out, sizeof(int64_t) * (size_t)src->B);
+    bm->d_in = (int64_t *)malloc(sizeof(int64_t) * (size_t)src->B);
+    memcpy(bm->d_in, src->d_in, sizeof(int64_t) * (size_t)src->B);
+    bm->d_tot = (int64_t *)malloc(sizeof(int64_t) * (size_t)src->B);
+    memcpy(bm->d_tot, src->d_tot, sizeof(int64_t) * (size_t)src->B);
+    bm->row = (svec *)calloc((size_t)src->B, sizeof(svec));
+    bm->col = (svec *)calloc((size_t)src->B, sizeof(svec));
+    for (int32_t b = 0; b < src->B; ++b) {
+        sv_copy(&bm->row[b], &src->row[b]);
+        sv_copy(&bm->col[b], &src->col[b]);
+    }
+    bm->num_to_merge = 0;
+    return bm;
+}
+
+int32_t orc_bm_num_blocks(const orc_bm *bm) { return bm->B; }
+
+int64_t orc_bm_nnz(const orc_bm *bm) {
+    int64_t n = 0;
+    for (int32_t b = 0; b < bm->B; ++b) n += bm->row[b].len;
+    return n;
+}
+
+void orc_bm_export(const orc_bm *bm, int32_t *row_ptr, int32_t *cols, int32_t *cnts, int64_t *d_out,
+                   int64_t *d_in, int64_t *d_tot, int32_t *sizes, int32_t *assignment) {
+    int64_t k = 0;
+    for (int32_t b = 0; b < bm->B; ++b) {
+        if (row_ptr) row_ptr[b] = (int32_t)k;
+        for (int32_t i = 0; i < bm->row[b].len; ++i, ++k) {
+            if (cols) cols[k] = bm->row[b].idx[i];
+            if (cnts) cnts[k] = (int32_t)bm->row[b].val[i];
+        }
+    }
+    if (row_ptr) row_ptr[bm->B] = (int32_t)k;
+    if (d_out) memcpy(d_out, bm->d_out, sizeof(int64_t) * (size_t)bm->B);
+    if (d_in) memcpy(d_in, bm->d_in, sizeof(int64_t) * (size_t)bm->B);
+    if (d_tot) memcpy(d_tot, bm->d_tot, sizeof(int64_t) * (size_t)bm->B);
+    if (sizes) memcpy(sizes, bm->size, sizeof(int32_t) * (size_t)bm->B);
+    if (assignment) memcpy(assignment, bm->assign, sizeof(int32_t) * (size_t)bm->N);
+}
+
+/* Blockmodel::validate (src/blockmodel/blockmodel.cpp:527-632): incremental state == rebuild. */
+int orc_bm_validate(const orc_bm *bm, const orc_graph *g) {
+    orc_bm *fresh = orc_bm_new(g, bm->B, bm->assign);
+    int ok = 1;
+    for (int32_t b = 0; b < bm->B && ok; ++b) {
+        if (fresh->d_out[b] != bm->d_out[b] || fresh->d_in[b] != bm->d_in[b] || fresh->d_tot[b] != bm->d_tot[b] ||
+            fresh->size[b] != bm->size[b] || fresh->row[b].len != bm->row[b].len || fresh->col[b].len != bm->col[b].len)
+            ok = 0;
+        for (int32_t i = 0; ok && i < bm->row[b].len; ++i)
+            if (fresh->row[b].idx[i] != bm->row[b].idx[i] || fresh->row[b].val[i] != bm->row[b].val[i]) ok = 0;
+        for (int32_t i = 0; ok && i < bm->col[b].len; ++i)
+            if (fresh->col[b].idx[i] != bm->col[b].idx[i] || fresh->col[b].val[i] != bm->col[b].val[i]) ok = 0;
+    }
+    orc_bm_free(fresh);
+    return ok;
+}
+
+/* ------------------------------------------------------------------------- */
+/* MDL (src/entropy.cpp:597-604; src/blockmodel/blockmodel.cpp:288-306)       */
+/* ------------------------------------------------------------------------- */
+double orc_log_posterior(const orc_bm *bm) {
+    fx128 acc = 0;
+    for (int32_t r = 0; r < bm->B; ++r) {
+        const svec *row = &bm->row[r];
+        for (int32_t i = 0; i < row->len; ++i) {
+            int32_t c = row->idx[i];
+            double value = (double)row->val[i];
+            double d_out = (double)bm->d_out[r], d_in = (double)bm->d_in[c];
+            if (d_in == 0.0 || d_out == 0.0) abort(); /* reference exits (-1000) */
+            /* values * nat_log(values / (d_out * d_in)); nat_log is logf */
+            double ratio = value / (d_out * d_in);
+            double term = value * (double)logf((float)ratio);
+            acc += fx_from(term);
+        }
+    }
+    return fx_to(acc);
+}
+
+static double model_term(int64_t E, int32_t N, int32_t B) {
+    double x = pow((double)B, 2) / (double)E;
+    double h = ((1 + x) * log(1 + x)) - (x * log(x));
+    return ((double)E * h) + ((double)N * log((double)B));
+}
+
+double orc_mdl(const orc_bm *bm, const orc_graph *g) {
+    return model_term(g->E, g->N, bm->B) - orc_log_posterior(bm);
+}
+
+/* entropy::null_mdl_v1 (src/entropy.cpp:571-585) */
+double orc_null_mdl_v1(int64_t E) {
+    double log_posterior_p = (double)E * log(1.0 / (double)E);
+    double x = 1.0 / (double)E;
+    double h = ((1 + x) * log(1 + x)) - (x * log(x));
+    return ((double)E * h) - log_posterior_p;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Delta (include/blockmodel/sparse/delta.hpp:22-149): four maps; cells that  */
+/* net to zero stay in their map.                                            */
+/* ------------------------------------------------------------------------- */
+typedef struct { int32_t *key; int64_t *val; uint8_t *used; int32_t cap, count; } imap;
+
+static void im_init(imap *m, int32_t expected) {
+    int32_t cap = 16;
+    while (cap < 2 * expected + 2) cap <<= 1;
+    m->cap = cap;
+    m->count = 0;
+    m->key = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap);
+    m->val = (int64_t *)malloc(sizeof(int64_t) * (size_t)cap);
+    m->used = (uint8_t *)calloc((size_t)cap, 1);
+}
+static void im_free(imap *m) { free(m->key); free(m->val); free(m->used); }
+static inline uint32_t im_hash(int32_t k) { return (uint32_t)k * 0x9E3779B1u; }
+static void im_add(imap *m, int32_t k, int64_t d) {
+    if (2 * (m->count + 1) > m->cap) {
+        imap n;
+        im_init(&n, m->cap);
+        for (int32_t i = 0; i < m->cap; ++i)
+            if (m->used[i]) im_add(&n, m->key[i], m->val[i]);
+        im_free(m);
+        *m = n;
+    }
+    uint32_t mask = (uint32_t)m->cap - 1, h = im_hash(k) & mask;
+    while (m->used[h]) {
+        if (m->key[h] == k) { m->val[h] += d; return; }
+        h = (h + 1) & mask;
+    }
+    m->used[h] = 1;
+    m->key[h] = k;
+    m->val[h] = d;
+    m->count++;
+}
+static int64_t im_get(const imap *m, int32_t k) {
+    uint32_t mask = (uint32_t)m->cap - 1, h = im_hash(k) & mask;
+    while (m->used[h]) {
+        if (m->key[h] == k) return m->val[h];
+        h = (h + 1) & mask;
+    }
+    return 0;
+}
+
+typedef struct {
+    imap cur_row, prop_row, cur_col, prop_col;
+    int32_t r, s;
+    int64_t self_edge_weight;
+} delta_t;
+
+static void delta_init(delta_t *d, int32_t r, int32_t s, int32_t expected) {
+    d->r = r; d->s = s; d->self_edge_weight = 0;
+    im_init(&d->cur_row, expected); im_init(&d->prop_row, expected);
+    im_init(&d->cur_col, expected); im_init(&d->prop_col, expected);
+}
+static void delta_free(delta_t *d) {
+    im_free(&d->cur_row); im_free(&d->prop_row); im_free(&d->cur_col); im_free(&d->prop_col);
+}
+static void delta_add(delta_t *d, int32_t row, int32_t col, int64_t v) {
+    if (row == d->r) im_add(&d->cur_row, col, v);
+    else if (row == d->s) im_add(&d->prop_row, col, v);
+    else if (col == d->r) im_add(&d->cur_col, row, v);
+    else if (col == d->s) im_add(&d->prop_col, row, v);
+    else abort(); /* std::logic_error in the reference */
+}
+static int64_t delta_get(const delta_t *d, int32_t row, int32_t col) {
+    if (row == d->r) return im_get(&d->cur_row, col);
+    if (row == d->s) return im_get(&d->prop_row, col);
+    if (col == d->r) return im_get(&d->cur_col, row);
+    if (col == d->s) return im_get(&d->prop_col, row);
+    abort();
+}
+
+/* ------------------------------------------------------------------------- */
+/* Finetune: vertex proposals (src/common.cpp:138-198, src/finetune.cpp)      */
+/* ------------------------------------------------------------------------- */
+typedef struct { int32_t s; int64_t k_out, k_in, k; } proposal_t;
+
+/* Vertex-mode propose_new_block with the canonical multinomial. */
+static proposal_t propose_vertex(const orc_bm *bm, const orc_graph *g, int32_t v, const uint32_t w[4]) {
+    proposal_t p;
+    int32_t r = bm->assign[v];
+    int64_t ko = g->out_off[v + 1] - g->out_off[v];
+    int64_t ki_all = g->in_off[v + 1] - g->in_off[v], self_in = 0;
+    for (int64_t e = g->in_off[v]; e < g->in_off[v + 1]; ++e) self_in += (g->in_adj[e] == v);
+    p.k_out = ko;
+    p.k_in = ki_all - self_in;
+    p.k = p.k_out + p.k_in;
+    if (p.k == 0) {
+        p.s = random_block(r, bm->B, w[1]);
+        return p;
+    }
+    uint32_t idx = orc_uniform_below(w[0], (uint32_t)p.k);
+    int32_t u;
+    if ((int64_t)idx < ko) {
+        u = g->out_adj[g->out_off[v] + idx];
+    } else {
+        int64_t want = (int64_t)idx - ko, seen = 0;
+        u = -1;
+        for (int64_t e = g->in_off[v]; e < g->in_off[v + 1]; ++e) {
+            if (g->in_adj[e] == v) continue;
+            if (seen == want) { u = g->in_adj[e]; break; }
+            seen++;
+        }
+    }
+    int32_t t = bm->assign[u];
+    /* neighbors_weights(t) (dict_transpose_matrix.cpp:176-188): row(t) then col(t) without row t */
+    const svec *row = &bm->row[t], *col = &bm->col[t];
+    int64_t total = 0;
+    for (int32_t i = 0; i < row->len; ++i) total += row->val[i];
+    for (int32_t i = 0; i < col->len; ++i) if (col->idx[i] != t) total += col->val[i];
+    uint32_t target = orc_uniform_below(w[1], (uint32_t)total);
+    int64_t prefix = 0;
+    p.s = -1;
+    for (int32_t i = 0; i < row->len && p.s < 0; ++i) {
+        prefix += row->val[i];
+        if ((int64_t)target < prefix) p.s = row->idx[i];
+    }
+    for (int32_t i = 0; i < col->len && p.s < 0; ++i) {
+        if (col->idx[i] == t) continue;
+        prefix += col->val[i];
+        if ((int64_t)target < prefix) p.s = col->idx[i];
+    }
+    return p;
+}
+
+/* finetune::blockmodel_delta (src/finetune.cpp:201-235) with in-edges excluding self. */
+static void vertex_delta(const orc_bm *bm, const orc_graph *g, int32_t v, int32_t r, int32_t s, delta_t *d) {
+    int64_t k = (g->out_off[v + 1] - g->out_off[v]) + (g->in_off[v + 1] - g->in_off[v]);
+    delta_init(d, r, s, (int32_t)(k < 1000000 ? k : 1000000));
+    for (int64_t e = g->out_off[v]; e < g->out_off[v + 1]; ++e) {
+        int32_t u = g->out_adj[e], ob = bm->assign[u];
+        if (u == v) {
+            delta_add(d, s, s, 1);
+            d->self_edge_weight = 1;
+        } else {
+            delta_add(d, s, ob, 1);
+        }
+        delta_add(d, r, ob, -1);
+    }
+    for (int64_t e = g->in_off[v]; e < g->in_off[v + 1]; ++e) {
+        int32_t u = g->in_adj[e];
+        if (u == v) continue; /* edge_weights(in, ignore_self=true) */
+        int32_t ib = bm->assign[u];
+        delta_add(d, ib, s, 1);
+        delta_add(d, ib, r, -1);
+    }
+}
+
+typedef struct { const orc_bm *bm; int32_t r, s; int64_t shift_in, shift_out; } degfn;
+static inline int64_t new_d_in(const degfn *f, int32_t x) {
+    int64_t v = f->bm->d_in[x];
+    if (x == f->r) v -= f->shift_in; else if (x == f->s) v += f->shift_in;
+    return v;
+}
+static inline int64_t new_d_out(const degfn *f, int32_t x) {
+    int64_t v = f->bm->d_out[x];
+    if (x == f->r) v -= f->shift_out; else if (x == f->s) v += f->shift_out;
+    return v;
+}
+
+static void imap_entries_delta(const orc_bm *bm, const delta_t *d, const imap *m, int which, const degfn *f,
+                               int skip_new_in_r, fx128 *acc) {
+    for (int32_t i = 0; i < m->cap; ++i) {
+        if (!m->used[i]) continue;
+        int32_t row, col;
+        if (which == 0) { row = d->r; col = m->key[i]; }
+        else if (which == 1) { row = d->s; col = m->key[i]; }
+        else if (which == 2) { row = m->key[i]; col = d->r; }
+        else { row = m->key[i]; col = d->s; }
+        int64_t change = m->val[i];
+        int64_t value = bm_get(bm, row, col);
+        *acc += fx_from(cell_entropy(value, bm->d_in[col], bm->d_out[row]));
+        if (skip_new_in_r && (row == d->r || col == d->r)) continue;
+        *acc -= fx_from(cell_entropy(value + change, new_d_in(f, col), new_d_out(f, row)));
+    }
+}
+
+/* entropy::delta_mdl(const Blockmodel&, const Delta&, const ProposalAndEdgeCounts&) (src/entropy.cpp:298-403) */
+static double delta_mdl_from(const orc_bm *bm, const delta_t *d, const proposal_t *p) {
+    degfn f = { bm, d->r, d->s, p->k_in + d->self_edge_weight, p->k_out };
+    fx128 acc = 0;
+    imap_entries_delta(bm, d, &d->cur_row, 0, &f, 0, &acc);
+    imap_entries_delta(bm, d, &d->prop_row, 1, &f, 0, &acc);
+    imap_entries_delta(bm, d, &d->cur_col, 2, &f, 0, &acc);
+    imap_entries_delta(bm, d, &d->prop_col, 3, &f, 0, &acc);
+    int32_t rs[2] = { d->r, d->s };
+    for (int k = 0; k < 2; ++k) {
+        int32_t row = rs[k];
+        const svec *sv = &bm->row[row];
+        for (int32_t i = 0; i < sv->len; ++i) {
+            int32_t col = sv->idx[i];
+            int64_t value = sv->val[i];
+            if (delta_get(d, row, col) != 0) continue;
+            acc += fx_from(cell_entropy(value, bm->d_in[col], bm->d_out[row]));
+            acc -= fx_from(cell_entropy(value, new_d_in(&f, col), new_d_out(&f, row)));
+        }
+    }
+    for (int k = 0; k < 2; ++k) {
+        int32_t col = rs[k];
+        const svec *sv = &bm->col[col];
+        for (int32_t i = 0; i < sv->len; ++i) {
+            int32_t row = sv->idx[i];
+            int64_t value = sv->val[i];
+            if (delta_get(d, row, col) != 0 || row == d->r || row == d->s) continue;
+            acc += fx_from(cell_entropy(value, bm->d_in[col], bm->d_out[row]));
+            acc -= fx_from(cell_entropy(value, new_d_in(&f, col), new_d_out(&f, row)));
+        }
+    }
+    return fx_to(acc);
+}
+
+/* entropy::hastings_correction (src/entropy.cpp:497-561). */
+static double hastings_from(const orc_bm *bm, const orc_graph *g, int32_t v, const delta_t *d, const proposal_t *p) {
+    if (p->k == 0) return 1.0;
+    int32_t r = d->r, s = d->s;
+    /* block_counts in ascending block order */
+    int64_t k = (g->out_off[v + 1] - g->out_off[v]) + (g->in_off[v + 1] - g->in_off[v]);
+    int32_t *blocks = (int32_t *)malloc(sizeof(int32_t) * (size_t)(k + 1));
+    int64_t nb = 0;
+    for (int64_t e = g->out_off[v]; e < g->out_off[v + 1]; ++e) blocks[nb++] = bm->assign[g->out_adj[e]];
+    for (int64_t e = g->in_off[v]; e < g->in_off[v + 1]; ++e)
+        if (g->in_adj[e] != v) blocks[nb++] = bm->assign[g->in_adj[e]];
+    /* insertion-free counting by sorting */
+    for (int64_t i = 1; i < nb; ++i) {
+        int32_t x = blocks[i];
+        int64_t j = i - 1;
+        while (j >= 0 && blocks[j] > x) { blocks[j + 1] = blocks[j]; j--; }
+        blocks[j + 1] = x;
+    }
+    fx128 fwd = 0, bwd = 0;
+    int64_t B = bm->B;
+    for (int64_t i = 0; i < nb;) {
+        int32_t x = blocks[i];
+        int64_t cnt = 0;
+        while (i < nb && blocks[i] == x) { cnt++; i++; }
+        double counts = (double)cnt;
+        double proposal_weights = (double)(bm_get(bm, s, x) + bm_get(bm, x, s)) + 1.0;
+        double block_degrees = (double)(bm->d_tot[x] + B);
+        double block_weights = (double)(bm_get(bm, r, x) + delta_get(d, r, x) + bm_get(bm, x, r) + delta_get(d, x, r)) + 1.0;
+        int64_t new_block_degree = bm->d_tot[x];
+        if (x == r) {
+            int64_t self = bm_get(bm, r, r) + delta_get(d, r, r);
+            new_block_degree = (bm->d_out[r] - p->k_out) + (bm->d_in[r] - p->k_in) - self;
+        } else if (x == s) {
+            int64_t self = bm_get(bm, s, s) + delta_get(d, s, s);
+            new_block_degree = (bm->d_out[s] + p->k_out) + (bm->d_in[s] + p->k_in) - self;
+        }
+        double proposal_degrees = (double)(new_block_degree + B);
+        fwd += fx_from(counts * proposal_weights / block_degrees);
+        bwd += fx_from(counts * block_weights / proposal_degrees);
+    }
+    free(blocks);
+    return fx_to(bwd) / fx_to(fwd);
+}
+
+/* finetune::accept (src/finetune.cpp:22-32). */
+static int accept_move(double dS, double H, int greedy, const uint32_t w[4]) {
+    if (greedy) return dS < 0.0;
+    double u = orc_unit_double(w[2], w[3]);
+    double a = exp(-3.0 * dS) * H;
+    a = (a >= 1.0) ? 1.0 : a;
+    return u <= a;
+}
+
+double orc_vertex_delta_mdl(const orc_bm *bm, const orc_graph *g, int32_t v, int32_t s) {
+    uint32_t w[4] = { 0, 0, 0, 0 };
+    proposal_t p = propose_vertex(bm, g, v, w);
+    p.s = s;
+    delta_t d;
+    vertex_delta(bm, g, v, bm->assign[v], s, &d);
+    double dS = delta_mdl_from(bm, &d, &p);
+    delta_free(&d);
+    return dS;
+}
+
+double orc_vertex_hastings(const orc_bm *bm, const orc_graph *g, int32_t v, int32_t s) {
+    uint32_t w[4] = { 0, 0, 0, 0 };
+    proposal_t p = propose_vertex(bm, g, v, w);
+    p.s = s;
+    delta_t d;
+    vertex_delta(bm, g, v, bm->assign[v], s, &d);
+    double H = hastings_from(bm, g, v, &d, &p);
+    delta_free(&d);
+    return H;
+}
+
+/* propose_gibbs_move_v3 / propose_move + eval (src/finetune.cpp:447-462,842-901), against a fixed bm. */
+static int evaluate_vertex(const orc_bm *bm, const orc_graph *g, uint64_t seed, uint32_t epoch, int32_t v,
+                           int greedy, int32_t *s_out, double *dS_out, delta_t *keep, proposal_t *p_out) {
+    uint32_t w[4];
+    orc_philox(seed, (uint32_t)v, epoch, ORC_TAG_FT, 0, w);
+    int32_t r = bm->assign[v];
+    proposal_t p = propose_vertex(bm, g, v, w);
+    *s_out = p.s;
+    *dS_out = 0.0;
+    if (p_out) *p_out = p;
+    if (p.s == r) return 0;
+    delta_t d;
+    vertex_delta(bm, g, v, r, p.s, &d);
+    double H = greedy ? 1.0 : hastings_from(bm, g, v, &d, &p);
+    double dS = delta_mdl_from(bm, &d, &p);
+    *dS_out = dS;
+    int acc = accept_move(dS, H, greedy, w);
+    if (keep && acc) *keep = d; else delta_free(&d);
+    return acc;
+}
+
+int orc_ft_propose(const orc_bm *bm, const orc_graph *g, uint64_t seed, uint32_t epoch, int32_t v, int32_t greedy,
+                   int32_t *s_out, double *dS_out, int32_t *accept_out) {
+    *accept_out = evaluate_vertex(bm, g, seed, epoch, v, greedy, s_out, dS_out, NULL, NULL);
+    return 0;
+}
+
+/* Blockmodel::move_vertex(const VertexMove_v3&) (src/blockmodel/blockmodel.cpp:437-494). */
+static int commit_v3(orc_bm *bm, const orc_graph *g, int32_t v, int32_t s) {
+    int32_t r = bm->assign[v];
+    if (bm->size[r] == 1) return 0;
+    for (int64_t e = g->out_off[v]; e < g->out_off[v + 1]; ++e) {
+        int32_t u = g->out_adj[e], ob = bm->assign[u];
+        bm_add(bm, r, ob, -1);
+        bm->d_out[r]--;
+        bm->d_out[s]++;
+        if (u == v) {
+            bm_add(bm, s, s, 1);
+            bm->d_in[r]--;
+            bm->d_in[s]++;
+        } else {
+            bm_add(bm, s, ob, 1);
+        }
+    }
+    for (int64_t e = g->in_off[v]; e < g->in_off[v + 1]; ++e) {
+        int32_t u = g->in_adj[e];
+        if (u == v) continue;
+        int32_t ib = bm->assign[u];
+        bm_add(bm, ib, r, -1);
+        bm->d_in[r]--;
+        bm_add(bm, ib, s, 1);
+        bm->d_in[s]++;
+    }
+    bm->d_tot[r] = bm->d_in[r] + bm->d_out[r] - bm_get(bm, r, r);
+    bm->d_tot[s] = bm->d_in[s] + bm->d_out[s] - bm_get(bm, s, s);
+    bm->assign[v] = s;
+    bm->size[r]--;
+    bm->size[s]++;
+    return 1;
+}
+
+/* Blockmodel::move_vertex(Vertex, const Delta&, ProposalAndEdgeCounts&) (src/blockmodel/blockmodel.cpp:408-435). */
+static void commit_delta(orc_bm *bm, int32_t v, const delta_t *d, const proposal_t *p) {
+    int32_t r = d->r, s = d->s;
+    bm->assign[v] = s;
+    const imap *maps[4] = { &d->cur_row, &d->prop_row, &d->cur_col, &d->prop_col };
+    for (int which = 0; which < 4; ++which) {
+        const imap *m = maps[which];
+        for (int32_t i = 0; i < m->cap; ++i) {
+            if (!m->used[i] || m->val[i] == 0) continue;
+            int32_t row, col;
+            if (which == 0) { row = r; col = m->key[i]; }
+            else if (which == 1) { row = s; col = m->key[i]; }
+            else if (which == 2) { row = m->key[i]; col = r; }
+            else { row = m->key[i]; col = s; }
+            bm_add(bm, row, col, m->val[i]);
+        }
+    }
+    bm->d_out[r] -= p->k_out;
+    bm->d_out[s] += p->k_out;
+    bm->d_in[r] -= (p->k_in + d->self_edge_weight);
+    bm->d_in[s] += (p->k_in + d->self_edge_weight);
+    bm->d_tot[r] = bm->d_out[r] + bm->d_in[r] - bm_get(bm, r, r);
+    bm->d_tot[s] = bm->d_out[s] + bm->d_in[s] - bm_get(bm, s, s);
+    bm->size[r]--;
+    bm->size[s]++;
+}
+
+/* Keyed shuffle: ascending Philox keys, ties by vertex id. */
+typedef struct { uint32_t key; int32_t v; } keyed_t;
+static int keyed_cmp(const void *a, const void *b) {
+    const keyed_t *x = (const keyed_t *)a, *y = (const keyed_t *)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return (x->v > y->v) - (x->v < y->v);
+}
+static void keyed_shuffle(int32_t *items, int64_t n, uint64_t seed, uint32_t epoch, uint32_t tag) {
+    keyed_t *k = (keyed_t *)malloc(sizeof(keyed_t) * (size_t)(n ? n : 1));
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t w[4];
+        orc_philox(seed, (uint32_t)items[i], epoch, tag, 0, w);
+        k[i].key = w[0];
+        k[i].v = items[i];
+    }
+    qsort(k, (size_t)n, sizeof(keyed_t), keyed_cmp);
+    for (int64_t i = 0; i < n; ++i) items[i] = k[i].v;
+    free(k);
+}
+
+/* Proposal phase + sequential commit in vertex-id order over one batch of `order[start, end)`. */
+static void gibbs_batch(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t epoch, const int32_t *order,
+                        int64_t start, int64_t end, int32_t *moves_s, int32_t *proposed_s, orc_stats *st) {
+    int32_t N = g->N;
+    for (int32_t v = 0; v < N; ++v) moves_s[v] = -1;
+    int64_t nprop = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : nprop) num_threads(p->threads > 0 ? p->threads : 1)
+    for (int64_t index = start; index < end; ++index) {
+        int32_t v = order ? order[index] : (int32_t)index;
+        int32_t s;
+        double dS;
+        int acc = evaluate_vertex(bm, g, p->seed, epoch, v, p->greedy, &s, &dS, NULL, NULL);
+        nprop++;
+        if (acc) moves_s[v] = s;
+    }
+    if (proposed_s)
+        for (int32_t v = 0; v < N; ++v) if (moves_s[v] >= 0) proposed_s[v] = moves_s[v];
+    int64_t moved = 0, refused = 0;
+    for (int32_t v = 0; v < N; ++v) {
+        if (moves_s[v] < 0) continue;
+        if (commit_v3(bm, g, v, moves_s[v])) moved++; else refused++;
+    }
+    if (st) { st->proposals += nprop; st->moves += moved; st->refused += refused; }
+}
+
+/* One sweep of finetune::asynchronous_gibbs (src/finetune.cpp:50-86). */
+int orc_async_gibbs_sweep(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t epoch, int32_t *proposed_s,
+                          orc_stats *st) {
+    int32_t N = g->N;
+    int32_t *order = NULL;
+    if (!p->ordered) {
+        order = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+        for (int32_t i = 0; i < N; ++i) order[i] = i;
+        keyed_shuffle(order, N, p->seed, epoch, ORC_TAG_SHUFFLE);
+    }
+    if (proposed_s) for (int32_t v = 0; v < N; ++v) proposed_s[v] = -1;
+    int32_t *moves_s = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    int64_t batch_size = (int64_t)ceil((double)N / (double)p->batches);
+    for (int64_t batch = 0; batch < N / batch_size; ++batch) {
+        int64_t start = batch * batch_size;
+        int64_t end = (batch + 1) * batch_size < N ? (batch + 1) * batch_size : N;
+        gibbs_batch(bm, g, p, epoch, order, start, end, moves_s, proposed_s, st);
+    }
+    free(moves_s);
+    free(order);
+    if (st) st->sweeps++;
+    return 0;
+}
+
+/* One sweep of sequential Metropolis-Hastings (src/finetune.cpp:793-803,822-869). */
+int orc_mh_sweep(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t epoch, int32_t ordered,
+                 double *sum_dS, orc_stats *st) {
+    int32_t N = g->N;
+    int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    for (int32_t i = 0; i < N; ++i) order[i] = i;
+    if (!ordered) keyed_shuffle(order, N, p->seed, epoch, ORC_TAG_SHUFFLE);
+    double total = 0.0;
+    for (int32_t index = 0; index < N; ++index) {
+        int32_t v = order[index];
+        int32_t r = bm->assign[v];
+        if (bm->size[r] == 1) continue;
+        int32_t s;
+        double dS;
+        delta_t d;
+        proposal_t prop;
+        int acc = evaluate_vertex(bm, g, p->seed, epoch, v, p->greedy, &s, &dS, &d, &prop);
+        if (st) st->proposals++;
+        if (acc) {
+            commit_delta(bm, v, &d, &prop);
+            delta_free(&d);
+            total += dS;
+            if (st) st->moves++;
+        }
+    }
+    free(order);
+    if (sum_dS) *sum_dS = total;
+    if (st) st->sweeps++;
+    return 0;
+}
+
+/* finetune::early_stop, 4-argument form (src/finetune.cpp:265-283). */
+static int early_stop4(int64_t iteration, int golden_ratio_not_reached, double initial_entropy, const double *de,
+                       int64_t n) {
+    int64_t last = n - 1;
+    if (de[last] == 0.0) return 1;
+    if (iteration < 3) return 0;
+    double average = de[last] + de[last - 1] + de[last - 2];
+    average /= -3.0;
+    double threshold = golden_ratio_not_reached ? 5e-4 * initial_entropy : 1e-4 * initial_entropy;
+    return average < threshold;
+}
+
+/* finetune::early_stop, 3-argument form (src/finetune.cpp:285-294). */
+static int early_stop3(int64_t iteration, double initial_entropy, const double *de, int64_t n) {
+    if (iteration < 3) return 0;
+    int64_t last = n - 1;
+    double average = de[last] + de[last - 1] + de[last - 2];
+    average /= -3.0;
+    return average < 1e-4 * initial_entropy;
+}
+
+/* Stable descending sort by degree (Graph::sort_vertices, src/graph.cpp:235-268). */
+typedef struct { int64_t deg; int32_t v; } degv_t;
+static int degv_cmp(const void *a, const void *b) {
+    const degv_t *x = (const degv_t *)a, *y = (const degv_t *)b;
+    if (x->deg != y->deg) return x->deg > y->deg ? -1 : 1;
+    return (x->v > y->v) - (x->v < y->v);
+}
+static int64_t vertex_degree(const orc_graph *g, int32_t v) {
+    int64_t self = 0;
+    for (int64_t e = g->out_off[v]; e < g->out_off[v + 1]; ++e) if (g->out_adj[e] == v) { self = 1; break; }
+    return (g->out_off[v + 1] - g->out_off[v]) + (g->in_off[v + 1] - g->in_off[v]) - self;
+}
+static void split_by_degree(const orc_graph *g, double mh_percent, int32_t *hdv, int64_t *nh, int32_t *ldv,
+                            int64_t *nl) {
+    degv_t *d = (degv_t *)malloc(sizeof(degv_t) * (size_t)g->N);
+    for (int32_t v = 0; v < g->N; ++v) { d[v].deg = vertex_degree(g, v); d[v].v = v; }
+    qsort(d, (size_t)g->N, sizeof(degv_t), degv_cmp);
+    *nh = 0; *nl = 0;
+    for (int32_t i = 0; i < g->N; ++i) {
+        if ((double)i < mh_percent * (double)g->N) hdv[(*nh)++] = d[i].v; else ldv[(*nl)++] = d[i].v;
+    }
+    free(d);
+}
+
+/* finetune::mcmc dispatch (src/finetune.cpp:754-771) and the sweeps' outer loops. */
+int orc_mcmc(orc_bm *bm, const orc_graph *g, const orc_params *p, int32_t golden_ratio_not_reached, uint32_t *epoch,
+             orc_stats *st) {
+    int32_t N = g->N;
+    if (bm->B == 1) return 0;
+    int algo = p->algorithm;
+    int max_it = p->max_iterations > 0 ? p->max_iterations : 100;
+    double *de = (double *)malloc(sizeof(double) * (size_t)max_it);
+    int64_t nde = 0;
+    if (algo == 0 || algo == 2) {
+        bm->overall_entropy = orc_mdl(bm, g);
+        double initial = bm->overall_entropy, last = initial;
+        int32_t *hdv = NULL, *ldv = NULL;
+        int64_t nh = 0, nl = 0;
+        if (algo == 2) {
+            hdv = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+            ldv = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+            split_by_degree(g, p->mh_percent, hdv, &nh, ldv, &nl);
+        }
+        int32_t *moves_s = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+        for (int64_t it = 0; it < max_it; ++it) {
+            uint32_t ep = (*epoch)++;
+            if (algo == 0) {
+                orc_async_gibbs_sweep(bm, g, p, ep, NULL, st);
+            } else {
+                /* hybrid_mcmc (src/finetune.cpp:563-646) */
+                int32_t *h = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nh ? nh : 1));
+                int32_t *l = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nl ? nl : 1));
+                memcpy(h, hdv, sizeof(int32_t) * (size_t)nh);
+                memcpy(l, ldv, sizeof(int32_t) * (size_t)nl);
+                if (!p->ordered) {
+                    keyed_shuffle(h, nh, p->seed, ep, ORC_TAG_SHUFFLE);
+                    keyed_shuffle(l, nl, p->seed, ep, ORC_TAG_SHUFFLE);
+                }
+                for (int64_t i = 0; i < nh; ++i) {
+                    int32_t v = h[i], s;
+                    if (bm->size[bm->assign[v]] == 1) continue;
+                    double dS;
+                    delta_t d;
+                    proposal_t prop;
+                    int acc = evaluate_vertex(bm, g, p->seed, ep, v, p->greedy, &s, &dS, &d, &prop);
+                    if (st) st->proposals++;
+                    if (acc) {
+                        commit_delta(bm, v, &d, &prop);
+                        delta_free(&d);
+                        if (st) st->moves++;
+                    }
+                }
+                int64_t batch_size = (int64_t)ceil((double)nl / (double)p->batches);
+                for (int64_t batch = 0; batch_size > 0 && batch < nl / batch_size; ++batch) {
+                    int64_t start = batch * batch_size;
+                    int64_t end = (batch + 1) * batch_size < nl ? (batch + 1) * batch_size : nl;
+                    gibbs_batch(bm, g, p, ep, l, start, end, moves_s, NULL, st);
+                }
+                free(h);
+                free(l);
+                if (st) st->sweeps++;
+            }
+            double entropy = orc_mdl(bm, g);
+            de[nde++] = entropy - last;
+            last = entropy;
+            if (early_stop4(it, golden_ratio_not_reached, initial, de, nde)) break;
+        }
+        free(moves_s);
+        free(hdv);
+        free(ldv);
+    } else {
+        /* metropolis_hastings (src/finetune.cpp:773-820) */
+        bm->overall_entropy = orc_mdl(bm, g);
+        for (int64_t it = 0; it < max_it; ++it) {
+            uint32_t ep = (*epoch)++;
+            double sum = 0.0;
+            orc_mh_sweep(bm, g, p, ep, p->ordered, &sum, st);
+            de[nde++] = sum;
+            if (early_stop4(it, golden_ratio_not_reached, bm->overall_entropy, de, nde)) break;
+        }
+    }
+    bm->overall_entropy = orc_mdl(bm, g);
+    free(de);
+    return 0;
+}
+
+/* finetune::finetune_assignment (src/finetune.cpp:903-932): ordered sequential MH over the full graph. */
+int orc_finetune_assignment(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t *epoch, orc_stats *st) {
+    int max_it = p->max_iterations > 0 ? p->max_iterations : 100;
+    double *de = (double *)malloc(sizeof(double) * (size_t)max_it);
+    int64_t nde = 0;
+    bm->overall_entropy = orc_mdl(bm, g);
+    for (int64_t it = 0; it < max_it; ++it) {
+        uint32_t ep = (*epoch)++;
+        double sum = 0.0;
+        orc_mh_sweep(bm, g, p, ep, 1, &sum, st);
+        de[nde++] = sum;
+        if (early_stop3(it, bm->overall_entropy, de, nde)) break;
+    }
+    bm->overall_entropy = orc_mdl(bm, g);
+    free(de);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Block merge (src/block_merge.cpp)                                          */
+/* ------------------------------------------------------------------------- */
+
+/* block_merge::blockmodel_delta (src/block_merge.cpp:14-38). */
+static void merge_delta(const orc_bm *bm, int32_t r, int32_t s, delta_t *d) {
+    delta_init(d, r, s, bm->row[r].len + bm->col[r].len);
+    const svec *row = &bm->row[r], *col = &bm->col[r];
+    for (int32_t i = 0; i < row->len; ++i) {
+        int32_t c = row->idx[i];
+        int64_t v = row->val[i];
+        if (c == r || c == s) delta_add(d, s, s, v); else delta_add(d, s, c, v);
+        delta_add(d, r, c, -v);
+    }
+    for (int32_t i = 0; i < col->len; ++i) {
+        int32_t rw = col->idx[i];
+        if (rw == r) continue;
+        int64_t v = col->val[i];
+        if (rw == s) delta_add(d, s, s, v); else delta_add(d, rw, s, v);
+        delta_add(d, rw, r, -v);
+    }
+}
+
+/* entropy::block_merge_delta_mdl(long, ProposalAndEdgeCounts, const Blockmodel&, const Delta&) (src/entropy.cpp:129-182). */
+static double merge_dS_from(const orc_bm *bm, const delta_t *d, int64_t k_out, int64_t k_in) {
+    degfn f = { bm, d->r, d->s, k_in, k_out };
+    fx128 acc = 0;
+    imap_entries_delta(bm, d, &d->cur_row, 0, &f, 1, &acc);
+    imap_entries_delta(bm, d, &d->prop_row, 1, &f, 1, &acc);
+    imap_entries_delta(bm, d, &d->cur_col, 2, &f, 1, &acc);
+    imap_entries_delta(bm, d, &d->prop_col, 3, &f, 1, &acc);
+    int32_t s = d->s, r = d->r;
+    const svec *row = &bm->row[s];
+    for (int32_t i = 0; i < row->len; ++i) {
+        int32_t col = row->idx[i];
+        int64_t value = row->val[i];
+        if (delta_get(d, s, col) != 0) continue;
+        acc += fx_from(cell_entropy(value, bm->d_in[col], bm->d_out[s]));
+        acc -= fx_from(cell_entropy(value, new_d_in(&f, col), new_d_out(&f, s)));
+    }
+    const svec *col = &bm->col[s];
+    for (int32_t i = 0; i < col->len; ++i) {
+        int32_t rw = col->idx[i];
+        int64_t value = col->val[i];
+        if (delta_get(d, rw, s) != 0 || rw == r || rw == s) continue;
+        acc += fx_from(cell_entropy(value, bm->d_in[s], bm->d_out[rw]));
+        acc -= fx_from(cell_entropy(value, new_d_in(&f, s), new_d_out(&f, rw)));
+    }
+    return fx_to(acc);
+}
+
+double orc_merge_delta_mdl(const orc_bm *bm, int32_t r, int32_t s) {
+    delta_t d;
+    merge_delta(bm, r, s, &d);
+    double dS = merge_dS_from(bm, &d, bm->d_out[r], bm->d_in[r]);
+    delta_free(&d);
+    return dS;
+}
+
+/* propose_merge_sparse (src/block_merge.cpp:251-285) with merge-mode propose_new_block. */
+static int32_t propose_merge(const orc_bm *bm, int32_t r, const uint32_t w[4]) {
+    const svec *row = &bm->row[r], *col = &bm->col[r];
+    int64_t k = bm->d_out[r] + bm->d_in[r];
+    int32_t B = bm->B;
+    if (k == 0) return random_block(r, B, w[1]);
+    /* choose_neighbor over concat(row(r), col(r)) weights */
+    uint32_t target = orc_uniform_below(w[0], (uint32_t)k);
+    int64_t prefix = 0;
+    int32_t t = -1;
+    for (int32_t i = 0; i < row->len && t < 0; ++i) {
+        prefix += row->val[i];
+        if ((int64_t)target < prefix) t = row->idx[i];
+    }
+    for (int32_t i = 0; i < col->len && t < 0; ++i) {
+        prefix += col->val[i];
+        if ((int64_t)target < prefix) t = col->idx[i];
+    }
+    /* neighbors_weights(t) with edges[r] = 0 */
+    const svec *tr = &bm->row[t], *tc = &bm->col[t];
+    int64_t total = 0;
+    for (int32_t i = 0; i < tr->len; ++i) if (tr->idx[i] != r) total += tr->val[i];
+    for (int32_t i = 0; i < tc->len; ++i) if (tc->idx[i] != t && tc->idx[i] != r) total += tc->val[i];
+    if (total == 0) return random_block(r, B, w[1]);
+    uint32_t tg = orc_uniform_below(w[1], (uint32_t)total);
+    prefix = 0;
+    for (int32_t i = 0; i < tr->len; ++i) {
+        if (tr->idx[i] == r) continue;
+        prefix += tr->val[i];
+        if ((int64_t)tg < prefix) return tr->idx[i];
+    }
+    for (int32_t i = 0; i < tc->len; ++i) {
+        if (tc->idx[i] == t || tc->idx[i] == r) continue;
+        prefix += tc->val[i];
+        if ((int64_t)tg < prefix) return tc->idx[i];
+    }
+    abort();
+}
+
+/* block_merge::merge_blocks proposal loop (src/block_merge.cpp:189-217), NUM_AGG_PROPOSALS_PER_BLOCK = 10. */
+int orc_merge_propose(const orc_bm *bm, uint64_t seed, uint32_t epoch, int32_t *best_s, double *best_dS) {
+    int32_t B = bm->B;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int32_t r = 0; r < B; ++r) {
+        int32_t past[10];
+        int32_t npast = 0;
+        best_s[r] = -1;
+        best_dS[r] = DBL_MAX;
+        for (uint32_t i = 0; i < 10; ++i) {
+            uint32_t w[4];
+            orc_philox(seed, (uint32_t)r, epoch, ORC_TAG_MERGE, i, w);
+            int32_t s = propose_merge(bm, r, w);
+            int seen = 0;
+            for (int32_t j = 0; j < npast; ++j) if (past[j] == s) seen = 1;
+            double dS = DBL_MAX;
+            if (!seen) {
+                dS = orc_merge_delta_mdl(bm, r, s);
+                past[npast++] = s;
+            }
+            if (dS < best_dS[r]) {
+                best_dS[r] = dS;
+                best_s[r] = s;
+            }
+        }
+    }
+    return 0;
+}
+
+/* Min-heap on (dS, block); ties by ascending block id. */
+typedef struct { double dS; int32_t r, s; } merge_t;
+static int merge_less(const merge_t *a, const merge_t *b) {
+    if (a->dS != b->dS) return a->dS < b->dS;
+    return a->r < b->r;
+}
+static void heap_push(merge_t *h, int64_t *n, merge_t m) {
+    int64_t i = (*n)++;
+    h[i] = m;
+    while (i > 0) {
+        int64_t p = (i - 1) / 2;
+        if (!merge_less(&h[i], &h[p])) break;
+        merge_t t = h[i]; h[i] = h[p]; h[p] = t;
+        i = p;
+    }
+}
+static merge_t heap_pop(merge_t *h, int64_t *n) {
+    merge_t top = h[0];
+    h[0] = h[--(*n)];
+    int64_t i = 0;
+    for (;;) {
+        int64_t l = 2 * i + 1, rr = l + 1, m = i;
+        if (l < *n && merge_less(&h[l], &h[m])) m = l;
+        if (rr < *n && merge_less(&h[rr], &h[m])) m = rr;
+        if (m == i) break;
+        merge_t t = h[i]; h[i] = h[m]; h[m] = t;
+        i = m;
+    }
+    return top;
+}
+
+/* Stable ascending argsort of doubles (utils::partial_sort_indices, include/utils.hpp:143-153). */
+static const double *g_sort_keys;
+static int idx_cmp(const void *a, const void *b) {
+    int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+    double kx = g_sort_keys[x], ky = g_sort_keys[y];
+    if (kx < ky) return -1;
+    if (kx > ky) return 1;
+    return (x > y) - (x < y);
+}
+
+/* Relabel with block_map, compact labels in ascending order (Blockmodel::build_mapping), rebuild. */
+static orc_bm *finish_merge(orc_bm *bm, const orc_graph *g, const int32_t *block_map, int64_t merged) {
+    int32_t B = bm->B;
+    int32_t *assign = (int32_t *)malloc(sizeof(int32_t) * (size_t)g->N);
+    for (int32_t v = 0; v < g->N; ++v) assign[v] = block_map[bm->assign[v]];
+    int32_t *used = (int32_t *)calloc((size_t)B, sizeof(int32_t));
+    for (int32_t v = 0; v < g->N; ++v) used[assign[v]] = 1;
+    int32_t next = 0;
+    for (int32_t b = 0; b < B; ++b) if (used[b]) used[b] = next++; else used[b] = -1;
+    for (int32_t v = 0; v < g->N; ++v) assign[v] = used[assign[v]];
+    int32_t newB = B - (int32_t)merged;
+    if (next > newB) newB = next;
+    orc_bm *out = orc_bm_new(g, newB, assign);
+    out->overall_entropy = bm->overall_entropy;
+    free(used);
+    free(assign);
+    return out;
+}
+
+orc_bm *orc_merge_apply(orc_bm *bm, const orc_graph *g, const int32_t *best_s, const double *best_dS,
+                        int64_t num_to_merge, int32_t approximate) {
+    int32_t B = bm->B;
+    int32_t *block_map = (int32_t *)malloc(sizeof(int32_t) * (size_t)B);
+    for (int32_t b = 0; b < B; ++b) block_map[b] = b;
+    int64_t merged = 0;
+    if (approximate) {
+        /* Blockmodel::carry_out_best_merges (src/blockmodel/blockmodel.cpp:55-122) */
+        int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)B);
+        for (int32_t b = 0; b < B; ++b) order[b] = b;
+        g_sort_keys = best_dS;
+        qsort(order, (size_t)B, sizeof(int32_t), idx_cmp);
+        int64_t counter = 0;
+        while (merged < num_to_merge && counter < B) {
+            int32_t from = order[counter++];
+            int32_t b = best_s[from];
+            do { b = block_map[b]; } while (block_map[b] != b);
+            if (b != from) {
+                block_map[from] = b;
+                merged++;
+            }
+        }
+        for (int32_t x = 0; x < B; ++x) {
+            int32_t b = x;
+            do { b = block_map[b]; } while (block_map[b] != b);
+            order[x] = b;
+        }
+        memcpy(block_map, order, sizeof(int32_t) * (size_t)B);
+        free(order);
+    } else {
+        /* carry_out_best_merges_advanced (src/block_merge.cpp:40-117) with union-find in place of the
+         * O(B) relabel: block_map values stay roots, so find() returns exactly the relabelled entry. */
+        merge_t *heap = (merge_t *)malloc(sizeof(merge_t) * (size_t)(B + 1));
+        int64_t n = 0;
+        for (int32_t b = 0; b < B; ++b) {
+            merge_t m = { best_dS[b], b, best_s[b] };
+            heap_push(heap, &n, m);
+        }
+        int32_t *parent = block_map;
+        while (merged < num_to_merge && n > 0) {
+            merge_t m = heap_pop(heap, &n);
+            int32_t from = m.r;
+            int32_t to = m.s;
+            while (parent[to] != to) { parent[to] = parent[parent[to]]; to = parent[to]; }
+            if (from == to) continue;
+            int64_t k_out = bm->d_out[from], k_in = bm->d_in[from];
+            delta_t d;
+            merge_delta(bm, from, to, &d);
+            double actual = merge_dS_from(bm, &d, k_out, k_in);
+            if (n > 0 && actual > heap[0].dS) {
+                m.dS = actual;
+                heap_push(heap, &n, m);
+                delta_free(&d);
+                continue;
+            }
+            parent[from] = to;
+            /* Blockmodel::merge_block (src/blockmodel/blockmodel.cpp:337-363) */
+            int64_t self_to = bm_get(bm, to, to) + delta_get(&d, to, to);
+            const imap *maps[4] = { &d.cur_row, &d.prop_row, &d.cur_col, &d.prop_col };
+            for (int which = 0; which < 4; ++which) {
+                const imap *mp = maps[which];
+                for (int32_t i = 0; i < mp->cap; ++i) {
+                    if (!mp->used[i] || mp->val[i] == 0) continue;
+                    int32_t row, col;
+                    if (which == 0) { row = from; col = mp->key[i]; }
+                    else if (which == 1) { row = to; col = mp->key[i]; }
+                    else if (which == 2) { row = mp->key[i]; col = from; }
+                    else { row = mp->key[i]; col = to; }
+                    bm_add(bm, row, col, mp->val[i]);
+                }
+            }
+            delta_free(&d);
+            bm->d_out[from] = 0;
+            bm->d_out[to] += k_out;
+            bm->d_in[from] = 0;
+            bm->d_in[to] += k_in;
+            bm->d_tot[from] = 0;
+            bm->d_tot[to] = bm->d_out[to] + bm->d_in[to] - self_to;
+            bm->size[to] += bm->size[from];
+            bm->size[from] = 0;
+            merged++;
+        }
+        for (int32_t b = 0; b < B; ++b) {
+            int32_t x = b;
+            while (parent[x] != x) x = parent[x];
+            int32_t y = b;
+            while (parent[y] != x) { int32_t nx = parent[y]; parent[y] = x; y = nx; }
+        }
+        free(heap);
+    }
+    orc_bm *out = finish_merge(bm, g, block_map, merged);
+    free(block_map);
+    return out;
+}
+
+/* block_merge::merge_blocks (src/block_merge.cpp:178-228). Consumes bm. */
+orc_bm *orc_merge_blocks(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t epoch, orc_stats *st) {
+    int32_t B = bm->B;
+    int32_t *best_s = (int32_t *)malloc(sizeof(int32_t) * (size_t)B);
+    double *best_dS = (double *)malloc(sizeof(double) * (size_t)B);
+    orc_merge_propose(bm, p->seed, epoch, best_s, best_dS);
+    if (st) { st->merge_proposals += 10 * (int64_t)B; st->merge_phases++; }
+    orc_bm *out = orc_merge_apply(bm, g, best_s, best_dS, bm->num_to_merge, p->approximate);
+    free(best_s);
+    free(best_dS);
+    orc_bm_free(bm);
+    return out;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Driver: BlockmodelTriplet + sbp::stochastic_block_partition               */
+/* (src/blockmodel/blockmodel_triplet.cpp:4-105, src/sbp.cpp:16-86)          */
+/* ------------------------------------------------------------------------- */
+typedef struct { orc_bm *bm[3]; int optimal; } triplet_t;
+
+static int tri_empty(const triplet_t *t, int i) { return t->bm[i] == NULL; }
+static void tri_set(triplet_t *t, int i, orc_bm *b) { if (t->bm[i]) orc_bm_free(t->bm[i]); t->bm[i] = b; }
+
+static void tri_update(triplet_t *t, orc_bm *bm) {
+    int index;
+    if (tri_empty(t, 1)) {
+        index = 1;
+    } else {
+        if (bm->overall_entropy <= t->bm[1]->overall_entropy) {
+            int old_index = (t->bm[1]->B > bm->B) ? 0 : 2;
+            tri_set(t, old_index, orc_bm_copy(t->bm[1]));
+            index = 1;
+        } else {
+            index = (t->bm[1]->B > bm->B) ? 2 : 0;
+        }
+    }
+    tri_set(t, index, orc_bm_copy(bm));
+}
+
+static int tri_is_done(triplet_t *t) {
+    if ((!tri_empty(t, 0) && t->bm[0]->B - t->bm[2]->B == 2) ||
+        (tri_empty(t, 0) && t->bm[1]->B - t->bm[2]->B == 1))
+        t->optimal = 1;
+    return t->optimal;
+}
+
+static orc_bm *tri_next(triplet_t *t, orc_bm *old) {
+    old->num_to_merge = 0;
+    tri_update(t, old);
+    if (tri_empty(t, 2)) {
+        orc_bm *b = orc_bm_copy(t->bm[1]);
+        b->num_to_merge = (int64_t)((double)b->B * 0.5);
+        if (b->num_to_merge == 0) t->optimal = 1;
+        return b;
+    }
+    if (tri_is_done(t)) return orc_bm_copy(t->bm[1]);
+    int index;
+    if (tri_empty(t, 0) && t->bm[1]->B > t->bm[2]->B) {
+        index = 1;
+    } else {
+        int64_t upper = tri_empty(t, 0) ? 0 - (int64_t)t->bm[1]->B : (int64_t)t->bm[0]->B - t->bm[1]->B;
+        int64_t lower = (int64_t)t->bm[1]->B - t->bm[2]->B;
+        index = (upper >= lower) ? 0 : 1;
+    }
+    int64_t next = t->bm[index + 1]->B;
+    next += (int64_t)((double)(t->bm[index]->B - t->bm[index + 1]->B) * 0.618);
+    orc_bm *b = orc_bm_copy(t->bm[index]);
+    b->num_to_merge = b->B - next;
+    return b;
+}
+
+int orc_sbp(const orc_graph *g, const orc_params *p, int32_t *assignment_out, int32_t *num_blocks_out,
+            orc_stats *st) {
+    int32_t N = g->N;
+    int32_t *ident = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    for (int32_t v = 0; v < N; ++v) ident[v] = v;
+    orc_bm *bm = orc_bm_new(g, N, ident);
+    free(ident);
+    triplet_t t = { { NULL, NULL, NULL }, 0 };
+    uint32_t epoch = 0;
+    int64_t iteration = 0;
+    while (!t.optimal) {
+        bm = orc_merge_blocks(bm, g, p, epoch++, st);
+        /* finetune::mcmc: async_gibbs only while iteration < asynciterations, else Metropolis-Hastings */
+        orc_params q = *p;
+        if (p->algorithm == 0 && iteration >= p->asynciterations) q.algorithm = 1;
+        orc_mcmc(bm, g, &q, tri_empty(&t, 2), &epoch, st);
+        iteration++;
+        orc_bm *next = tri_next(&t, bm);
+        orc_bm_free(bm);
+        bm = next;
+    }
+    if (assignment_out) memcpy(assignment_out, bm->assign, sizeof(int32_t) * (size_t)N);
+    if (num_blocks_out) *num_blocks_out = bm->B;
+    if (st) st->mdl = bm->overall_entropy;
+    orc_bm_free(bm);
+    for (int i = 0; i < 3; ++i) if (t.bm[i]) orc_bm_free(t.bm[i]);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Sampling (src/sample.cpp)                                                  */
+/* ------------------------------------------------------------------------- */
+
+/* alg 0 = max_degree (stable descending degree, sample.cpp:219-233), alg 1 = random (keyed shuffle,
+ * sample.cpp:235-247).  Writes mapping[N] (full id -> sample id or -1) and vertices[n] (sample -> full). */
+int64_t orc_sample(const orc_graph *g, int32_t alg, double fraction, uint64_t seed, int32_t *mapping,
+                   int32_t *vertices) {
+    int32_t N = g->N;
+    int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    if (alg == 0) {
+        degv_t *d = (degv_t *)malloc(sizeof(degv_t) * (size_t)N);
+        for (int32_t v = 0; v < N; ++v) { d[v].deg = vertex_degree(g, v); d[v].v = v; }
+        qsort(d, (size_t)N, sizeof(degv_t), degv_cmp);
+        for (int32_t i = 0; i < N; ++i) order[i] = d[i].v;
+        free(d);
+    } else {
+        for (int32_t i = 0; i < N; ++i) order[i] = i;
+        keyed_shuffle(order, N, seed, 0, ORC_TAG_SAMPLE);
+    }
+    int64_t n = (int64_t)(fraction * (double)N);
+    for (int32_t v = 0; v < N; ++v) mapping[v] = -1;
+    for (int64_t i = 0; i < n; ++i) {
+        mapping[order[i]] = (int32_t)i;
+        vertices[i] = order[i];
+    }
+    free(order);
+    return n;
+}
+
+/* sample::extend (src/sample.cpp:157-200): sequential plurality vote; ties to the smallest block. */
+int orc_extend(const orc_graph *g, const int32_t *mapping, const int32_t *sample_assignment, int32_t sample_blocks,
+               uint64_t seed, int32_t *assignment) {
+    int32_t N = g->N;
+    for (int32_t v = 0; v < N; ++v) assignment[v] = mapping[v] >= 0 ? sample_assignment[mapping[v]] : -1;
+    int64_t *counts = (int64_t *)calloc((size_t)sample_blocks, sizeof(int64_t));
+    int32_t *touched = (int32_t *)malloc(sizeof(int32_t) * (size_t)(sample_blocks + 1));
+    for (int32_t v = 0; v < N; ++v) {
+        if (assignment[v] != -1) continue;
+        int32_t nt = 0;
+        for (int64_t e = g->out_off[v]; e < g->out_off[v + 1]; ++e) {
+            int32_t c = assignment[g->out_adj[e]];
+            if (c == -1) continue;
+            if (counts[c]++ == 0) touched[nt++] = c;
+        }
+        for (int64_t e = g->in_off[v]; e < g->in_off[v + 1]; ++e) {
+            int32_t u = g->in_adj[e];
+            int32_t c = assignment[u];
+            if (c == -1 || u == v) continue;
+            if (counts[c]++ == 0) touched[nt++] = c;
+        }
+        if (nt == 0) {
+            uint32_t w[4];
+            orc_philox(seed, (uint32_t)v, 0, ORC_TAG_EXTEND, 0, w);
+            assignment[v] = (int32_t)orc_uniform_below(w[0], (uint32_t)sample_blocks);
+            continue;
+        }
+        int64_t best = 0;
+        int32_t best_c = -1;
+        for (int32_t i = 0; i < nt; ++i) {
+            int32_t c = touched[i];
+            if (counts[c] > best || (counts[c] == best && c < best_c)) { best = counts[c]; best_c = c; }
+            counts[c] = 0;
+        }
+        assignment[v] = best_c;
+    }
+    free(counts);
+    free(touched);
+    return 0;
+}
