@@ -1,0 +1,166 @@
+/*
+ * sbp_amd.h -- C-ABI of the MI355X (gfx950) implementation of SBP's hot path.
+ *
+ * Plain pointers and sizes only.  Every call returns an int status
+ * (SBP_OK = 0, negative on error); sbp_last_error() returns the message of the
+ * last failing call on a context.  No C++ exception crosses this boundary.
+ *
+ * Ownership: the caller owns every handle and frees it; device memory belongs
+ * to the handle.  A context owns one HIP stream; calls on one context are
+ * serialised by the caller; separate contexts may run concurrently.
+ *
+ * Host arrays passed in are read before the call returns; host arrays passed
+ * out are written before it returns.  Vertex ids are int32, graph offsets int64.
+ *
+ * Each entry point names the reference interface it replaces
+ * (vtsynergy/SBP, file:line).  The random draws follow a replay contract:
+ * Philox4x32-10 keyed by the context seed, counter {unit, epoch, tag, sub}.
+ */
+#ifndef SBP_AMD_H
+#define SBP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SBP_OK 0
+#define SBP_E_HIP (-1)
+#define SBP_E_OOM (-2)
+#define SBP_E_ARG (-3)
+#define SBP_E_NAN (-4)
+#define SBP_E_STATE (-5)
+
+typedef struct sbp_ctx_s *sbp_ctx;
+typedef struct sbp_graph_s *sbp_graph;
+typedef struct sbp_bm_s *sbp_bm;
+
+/* Run parameters: the hot-path subset of the reference's Args (include/args.hpp:61-212). */
+typedef struct {
+    int32_t algorithm;        /* -a: 0 async_gibbs, 1 metropolis_hastings, 2 hybrid_mcmc (args.hpp:65-68) */
+    int32_t batches;          /* --batches (args.hpp:75-78), default 1 */
+    int32_t greedy;           /* --greedy (args.hpp:111-112), default 1 */
+    int32_t approximate;      /* --approximate (args.hpp:69-71), default 0 */
+    int32_t ordered;          /* --ordered (args.hpp:130), default 0 */
+    int32_t asynciterations;  /* --asynciterations (args.hpp:72-74), default INT32_MAX */
+    double mh_percent;        /* --mh_percent (args.hpp:115-116), default 0.075 */
+    uint64_t seed;            /* replay seed (the reference seeds from random_device; src/rng.cpp:34-46) */
+    int32_t max_iterations;   /* MAX_NUM_ITERATIONS (include/finetune.hpp:34), default 100 */
+    int32_t threads;          /* --threads: accepted for flag parity; the device ignores it */
+} sbp_params;
+
+/* Counters and phase timers (the reference's timers::*, include/globals.hpp:19-67). */
+typedef struct {
+    int64_t proposals;        /* vertex-move proposals evaluated */
+    int64_t moves;            /* accepted and committed vertex moves */
+    int64_t refused;          /* async-Gibbs moves refused at commit (source block of size 1) */
+    int64_t sweeps;           /* finetune sweeps */
+    int64_t merge_proposals;  /* block-merge proposals evaluated */
+    int64_t merge_phases;
+    double mdl;               /* final description length */
+    double finetune_ms;       /* device time in finetune proposal + commit kernels */
+    double merge_ms;          /* merge proposal kernels */
+    double carry_ms;          /* merge carry-out */
+    double build_ms;          /* blockmodel (re)builds */
+    double mdl_ms;            /* MDL reductions */
+    double total_ms;          /* wall time of the call */
+    uint64_t ft_bytes;        /* algorithmic bytes of the finetune proposal kernels */
+    uint64_t merge_bytes;     /* algorithmic bytes of the merge proposal kernels */
+    int64_t ft_launches;      /* finetune proposal kernel launches */
+    double ft_kernel_ms;      /* device time of the finetune proposal kernels only */
+} sbp_stats;
+
+/* ----------------------------------------------------------------- context */
+
+/* Replaces rng::init_generators (src/rng.cpp:34-46): one context = one device stream + replay seed. */
+int sbp_create(sbp_ctx *out, int device, uint64_t seed);
+int sbp_destroy(sbp_ctx ctx);
+const char *sbp_last_error(sbp_ctx ctx);
+int sbp_set_seed(sbp_ctx ctx, uint64_t seed);
+/* Blocks until all work queued on the context's stream is complete. */
+int sbp_synchronize(sbp_ctx ctx);
+
+/* ----------------------------------------------------------------- graph */
+
+/* Replaces Graph(out_neighbors, in_neighbors, ...) (include/graph.hpp:15-140): CSR with each
+ * vertex's neighbours in file order.  out_off/in_off have N+1 entries. */
+int sbp_graph_upload(sbp_ctx ctx, int32_t N, int64_t E, const int64_t *out_off, const int32_t *out_adj,
+                     const int64_t *in_off, const int32_t *in_adj, sbp_graph *out);
+int sbp_graph_free(sbp_graph g);
+
+/* ----------------------------------------------------------------- blockmodel */
+
+/* Replaces Blockmodel(num_blocks, graph, rate, assignment) + initialize_edge_counts
+ * (include/blockmodel/blockmodel.hpp:82-88; src/blockmodel/blockmodel.cpp:193-276).
+ * assignment: host int32[N] with values in [0, B). */
+int sbp_bm_build(sbp_ctx ctx, sbp_graph g, int32_t B, const int32_t *assignment, sbp_bm *out);
+/* Replaces Blockmodel::copy (src/blockmodel/blockmodel.cpp:139-153). */
+int sbp_bm_copy(sbp_ctx ctx, sbp_bm src, sbp_bm *out);
+int sbp_bm_free(sbp_bm bm);
+int sbp_bm_info(sbp_bm bm, int32_t *num_blocks, int64_t *nnz);
+/* Copies device state to host; any pointer may be NULL.  row_ptr has B+1 entries; cols/cnts nnz entries
+ * (row-major, ascending columns).  Degrees are Blockmodel::degrees_out/_in/degrees. */
+int sbp_bm_download(sbp_ctx ctx, sbp_bm bm, int32_t *assignment, int32_t *sizes, int32_t *d_out, int32_t *d_in,
+                    int32_t *d_tot, int32_t *row_ptr, int32_t *cols, int32_t *cnts);
+/* Replaces Blockmodel::validate (src/blockmodel/blockmodel.cpp:527-632): rebuilds from the assignment on
+ * the device and compares every count; *ok = 1 when equal. */
+int sbp_bm_validate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *ok);
+
+/* ----------------------------------------------------------------- entropy */
+
+/* Replaces entropy::mdl (src/entropy.cpp:597-604) and Blockmodel::log_posterior_probability
+ * (src/blockmodel/blockmodel.cpp:288-306). */
+int sbp_mdl(sbp_ctx ctx, sbp_bm bm, sbp_graph g, double *mdl, double *log_posterior);
+
+/* ----------------------------------------------------------------- finetune */
+
+/* Evaluates one proposal per vertex against the frozen blockmodel, as propose_gibbs_move_v3 +
+ * eval_vertex_move_v3 do (src/finetune.cpp:447-462,871-901).  Host outputs, N entries each:
+ * s_out = proposed block, dS_out = delta-MDL (0 when s == current block), accept_out = 0/1. */
+int sbp_ft_propose_all(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int32_t greedy, int32_t *s_out,
+                       double *dS_out, int32_t *accept_out);
+/* One sweep of finetune::asynchronous_gibbs (src/finetune.cpp:50-86): proposals against the frozen
+ * blockmodel per batch, commit in vertex-id order with refusal of moves that would empty a block
+ * (src/blockmodel/blockmodel.cpp:437-494). */
+int sbp_async_gibbs_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, uint32_t epoch,
+                          sbp_stats *stats);
+/* One sweep of sequential Metropolis-Hastings (src/finetune.cpp:773-820; propose_move :842-869).
+ * ordered = 1 visits vertices in id order (finetune_assignment); sum_dS = sum of accepted delta-MDL. */
+int sbp_mh_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, uint32_t epoch, int32_t ordered,
+                 double *sum_dS, sbp_stats *stats);
+/* Replaces finetune::mcmc (src/finetune.cpp:754-771) incl. early stopping; *epoch is advanced by the
+ * number of sweeps.  *mdl_out receives Blockmodel::getOverall_entropy after the phase. */
+int sbp_mcmc(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, int32_t golden_ratio_not_reached,
+             uint32_t *epoch, double *mdl_out, sbp_stats *stats);
+/* Replaces finetune::finetune_assignment (src/finetune.cpp:903-932). */
+int sbp_finetune_assignment(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, uint32_t *epoch,
+                            double *mdl_out, sbp_stats *stats);
+
+/* ----------------------------------------------------------------- block merge */
+
+/* Replaces the proposal loop of block_merge::merge_blocks (src/block_merge.cpp:189-217): 10 proposals
+ * per block, best (lowest delta-MDL) per block.  Host outputs of B entries. */
+int sbp_merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *best_s, double *best_dS);
+/* Replaces block_merge::merge_blocks (src/block_merge.cpp:178-228): proposals, carry-out
+ * (carry_out_best_merges_advanced, or Blockmodel::carry_out_best_merges when approximate), label
+ * compaction and rebuild.  Writes a new blockmodel with B - num_to_merge blocks to *out. */
+int sbp_merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int64_t num_to_merge,
+                     int32_t approximate, sbp_bm *out, sbp_stats *stats);
+
+/* ----------------------------------------------------------------- driver */
+
+/* Replaces sbp::stochastic_block_partition (src/sbp.cpp:16-72) with the golden-section triplet
+ * (src/blockmodel/blockmodel_triplet.cpp:4-105).  assignment_out: host int32[N]. */
+int sbp_partition(sbp_ctx ctx, sbp_graph g, const sbp_params *p, int32_t *assignment_out,
+                  int32_t *num_blocks_out, sbp_stats *stats);
+
+/* ----------------------------------------------------------------- diagnostics */
+
+/* Device logf over float bit patterns [first_bits, first_bits + count) (checks the restated glibc logf). */
+int sbp_debug_logf(sbp_ctx ctx, uint32_t first_bits, uint32_t count, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SBP_AMD_H */

@@ -805,7 +805,7 @@ int orc_mh_sweep(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t e
     int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
     for (int32_t i = 0; i < N; ++i) order[i] = i;
     if (!ordered) keyed_shuffle(order, N, p->seed, epoch, ORC_TAG_SHUFFLE);
-    double total = 0.0;
+    fx128 total = 0;
     for (int32_t index = 0; index < N; ++index) {
         int32_t v = order[index];
         int32_t r = bm->assign[v];
@@ -819,12 +819,12 @@ int orc_mh_sweep(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t e
         if (acc) {
             commit_delta(bm, v, &d, &prop);
             delta_free(&d);
-            total += dS;
+            total += fx_from(dS);
             if (st) st->moves++;
         }
     }
     free(order);
-    if (sum_dS) *sum_dS = total;
+    if (sum_dS) *sum_dS = fx_to(total);
     if (st) st->sweeps++;
     return 0;
 }

@@ -1,0 +1,193 @@
+// Blockmodel build: M[r][c] = #edges from block r to block c, its transpose, degrees and sizes.
+//
+// Replaces Blockmodel::initialize_edge_counts (src/blockmodel/blockmodel.cpp:193-276), where every
+// OpenMP thread scans all N vertices and inserts into robin_map rows.  Here the edge list is turned
+// into packed (row, col) keys, radix-sorted and run-length encoded; row pointers come from key
+// boundaries, and the transpose from a second sort of the nnz unique cells.  The result is exact
+// integer state, identical to the reference's after any sequence of incremental updates.
+#include <hipcub/hipcub.hpp>
+
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace sbpimpl {
+
+static int bits_for(int64_t x) {  // number of bits to represent values in [0, x)
+    int b = 0;
+    while (b < 62 && (int64_t(1) << b) < x) ++b;
+    return b < 1 ? 1 : b;
+}
+
+__global__ void k_make_keys(int64_t E, const int32_t *__restrict__ src, const int32_t *__restrict__ dst,
+                            const int32_t *__restrict__ assign, int cbits, unsigned long long *__restrict__ keys) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < E; e += (int64_t)gridDim.x * blockDim.x) {
+        unsigned long long r = (unsigned)assign[src[e]], c = (unsigned)assign[dst[e]];
+        keys[e] = (r << cbits) | c;
+    }
+}
+
+// One thread per unique cell: write cells, row pointers at key boundaries, and degree sums.
+__global__ void k_fill_major(int64_t nnz, int32_t B, int cbits, const unsigned long long *__restrict__ uniq,
+                             const int32_t *__restrict__ counts, int32_t *__restrict__ ptr, int2 *__restrict__ cells,
+                             int32_t *__restrict__ deg_major, int32_t *__restrict__ deg_minor) {
+    const unsigned long long mask = (1ull << cbits) - 1;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t a = (int32_t)(uniq[i] >> cbits), b = (int32_t)(uniq[i] & mask), cnt = counts[i];
+        cells[i] = make_int2(b, cnt);
+        if (deg_major) atomicAdd(&deg_major[a], cnt);
+        if (deg_minor) atomicAdd(&deg_minor[b], cnt);
+        int32_t prev = i == 0 ? -1 : (int32_t)(uniq[i - 1] >> cbits);
+        for (int32_t x = prev + 1; x <= a; ++x) ptr[x] = (int32_t)i;
+        if (i == nnz - 1)
+            for (int32_t x = a + 1; x <= B; ++x) ptr[x] = (int32_t)nnz;
+    }
+}
+
+__global__ void k_transpose_keys(int64_t nnz, int cbits, const unsigned long long *__restrict__ uniq,
+                                 unsigned long long *__restrict__ tkeys) {
+    const unsigned long long mask = (1ull << cbits) - 1;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+        unsigned long long r = uniq[i] >> cbits, c = uniq[i] & mask;
+        tkeys[i] = (c << cbits) | r;
+    }
+}
+
+__global__ void k_dtot_sizes(int32_t B, const int32_t *__restrict__ row_ptr, const int2 *__restrict__ row_cells,
+                             const int32_t *__restrict__ d_out, const int32_t *__restrict__ d_in,
+                             int32_t *__restrict__ d_tot) {
+    for (int32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x) {
+        int self = sbpdev::cell_lookup(row_cells, row_ptr[b], row_ptr[b + 1], b);
+        d_tot[b] = d_out[b] + d_in[b] - self;
+    }
+}
+
+__global__ void k_sizes(int32_t N, const int32_t *__restrict__ assign, int32_t *__restrict__ sizes) {
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+        atomicAdd(&sizes[assign[v]], 1);
+}
+
+static int grid_for(int64_t n, int block = 256) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    return (int)g;
+}
+
+void bm_alloc(sbp_bm bm, int32_t B, int32_t N) {
+    if (bm->N != N || bm->assign == nullptr) {
+        if (bm->assign) SBP_HIP(hipFree(bm->assign));
+        SBP_HIP(hipMalloc(&bm->assign, sizeof(int32_t) * (size_t)(N > 0 ? N : 1)));
+        bm->N = N;
+    }
+    if (B + 1 > bm->bcap) {
+        int32_t **arrs[] = {&bm->sizes, &bm->d_out, &bm->d_in, &bm->d_tot, &bm->row_ptr, &bm->col_ptr};
+        for (auto a : arrs) {
+            if (*a) SBP_HIP(hipFree(*a));
+            SBP_HIP(hipMalloc(a, sizeof(int32_t) * (size_t)(B + 2)));
+        }
+        bm->bcap = B + 1;
+    }
+    bm->B = B;
+}
+
+void bm_release(sbp_bm bm) {
+    void *ptrs[] = {bm->assign, bm->sizes, bm->d_out, bm->d_in, bm->d_tot, bm->row_ptr, bm->col_ptr, bm->row_cells,
+                    bm->col_cells};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    bm->assign = bm->sizes = bm->d_out = bm->d_in = bm->d_tot = bm->row_ptr = bm->col_ptr = nullptr;
+    bm->row_cells = bm->col_cells = nullptr;
+    bm->cap = 0;
+    bm->bcap = 0;
+}
+
+static void ensure_cells(sbp_bm bm, int64_t nnz) {
+    if (nnz > bm->cap) {
+        if (bm->row_cells) SBP_HIP(hipFree(bm->row_cells));
+        if (bm->col_cells) SBP_HIP(hipFree(bm->col_cells));
+        int64_t cap = nnz + nnz / 8 + 64;
+        SBP_HIP(hipMalloc(&bm->row_cells, sizeof(int2) * (size_t)cap));
+        SBP_HIP(hipMalloc(&bm->col_cells, sizeof(int2) * (size_t)cap));
+        bm->cap = cap;
+    }
+}
+
+void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm) {
+    hipStream_t st = ctx->stream;
+    const int32_t B = bm->B;
+    const int64_t E = g->E;
+    const int cbits = bits_for(B);
+    const int end_bit = 2 * cbits;
+    SBP_CHECK(end_bit <= 64, SBP_E_ARG, "block count too large for packed keys");
+    SBP_HIP(hipMemsetAsync(bm->d_out, 0, sizeof(int32_t) * B, st));
+    SBP_HIP(hipMemsetAsync(bm->d_in, 0, sizeof(int32_t) * B, st));
+    SBP_HIP(hipMemsetAsync(bm->sizes, 0, sizeof(int32_t) * B, st));
+    k_sizes<<<grid_for(bm->N), 256, 0, st>>>(bm->N, bm->assign, bm->sizes);
+    if (E == 0) {
+        SBP_HIP(hipMemsetAsync(bm->row_ptr, 0, sizeof(int32_t) * (B + 1), st));
+        SBP_HIP(hipMemsetAsync(bm->col_ptr, 0, sizeof(int32_t) * (B + 1), st));
+        SBP_HIP(hipMemsetAsync(bm->d_tot, 0, sizeof(int32_t) * B, st));
+        bm->nnz = 0;
+        return;
+    }
+    auto *keys = ctx->scratch[S_KEYS_A].as<unsigned long long>(E);
+    auto *sorted = ctx->scratch[S_KEYS_B].as<unsigned long long>(E);
+    auto *uniq = ctx->scratch[S_UNIQ].as<unsigned long long>(E);
+    auto *counts = ctx->scratch[S_COUNTS].as<int32_t>(E);
+    auto *num_runs = ctx->scratch[S_MISC].as<int64_t>(4);
+    k_make_keys<<<grid_for(E), 256, 0, st>>>(E, g->edge_src, g->out_adj, bm->assign, cbits, keys);
+    SBP_HIP(hipGetLastError());
+    size_t tmp_a = 0, tmp_b = 0;
+    SBP_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_a, keys, sorted, (int)E, 0, end_bit, st));
+    SBP_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, tmp_b, sorted, uniq, counts, num_runs, (int)E, st));
+    size_t tmp = tmp_a > tmp_b ? tmp_a : tmp_b;
+    void *tmp_ptr = ctx->scratch[S_SORT_TMP].get(tmp);
+    SBP_HIP(hipcub::DeviceRadixSort::SortKeys(tmp_ptr, tmp_a, keys, sorted, (int)E, 0, end_bit, st));
+    SBP_HIP(hipcub::DeviceRunLengthEncode::Encode(tmp_ptr, tmp_b, sorted, uniq, counts, num_runs, (int)E, st));
+    int64_t nnz = 0;
+    SBP_HIP(hipMemcpyAsync(&nnz, num_runs, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    SBP_HIP(hipStreamSynchronize(st));
+    ensure_cells(bm, nnz);
+    bm->nnz = nnz;
+    k_fill_major<<<grid_for(nnz), 256, 0, st>>>(nnz, B, cbits, uniq, counts, bm->row_ptr, bm->row_cells, bm->d_out,
+                                                bm->d_in);
+    SBP_HIP(hipGetLastError());
+    // transpose: sort (col, row) keys carrying the counts
+    auto *tkeys = keys;  // reuse
+    auto *tkeys_sorted = sorted;
+    auto *tvals_sorted = ctx->scratch[S_VALS_B].as<int32_t>(nnz);
+    k_transpose_keys<<<grid_for(nnz), 256, 0, st>>>(nnz, cbits, uniq, tkeys);
+    size_t tmp_c = 0;
+    SBP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_c, tkeys, tkeys_sorted, counts, tvals_sorted, (int)nnz, 0,
+                                               end_bit, st));
+    tmp_ptr = ctx->scratch[S_SORT_TMP].get(tmp_c);
+    SBP_HIP(hipcub::DeviceRadixSort::SortPairs(tmp_ptr, tmp_c, tkeys, tkeys_sorted, counts, tvals_sorted, (int)nnz, 0,
+                                               end_bit, st));
+    k_fill_major<<<grid_for(nnz), 256, 0, st>>>(nnz, B, cbits, tkeys_sorted, tvals_sorted, bm->col_ptr, bm->col_cells,
+                                                nullptr, nullptr);
+    k_dtot_sizes<<<grid_for(B), 256, 0, st>>>(B, bm->row_ptr, bm->row_cells, bm->d_out, bm->d_in, bm->d_tot);
+    SBP_HIP(hipGetLastError());
+}
+
+void bm_copy_into(sbp_ctx ctx, sbp_bm src, sbp_bm dst) {
+    hipStream_t st = ctx->stream;
+    bm_alloc(dst, src->B, src->N);
+    ensure_cells(dst, src->nnz > 0 ? src->nnz : 1);
+    dst->nnz = src->nnz;
+    const int32_t B = src->B;
+    SBP_HIP(hipMemcpyAsync(dst->assign, src->assign, sizeof(int32_t) * src->N, hipMemcpyDeviceToDevice, st));
+    SBP_HIP(hipMemcpyAsync(dst->sizes, src->sizes, sizeof(int32_t) * B, hipMemcpyDeviceToDevice, st));
+    SBP_HIP(hipMemcpyAsync(dst->d_out, src->d_out, sizeof(int32_t) * B, hipMemcpyDeviceToDevice, st));
+    SBP_HIP(hipMemcpyAsync(dst->d_in, src->d_in, sizeof(int32_t) * B, hipMemcpyDeviceToDevice, st));
+    SBP_HIP(hipMemcpyAsync(dst->d_tot, src->d_tot, sizeof(int32_t) * B, hipMemcpyDeviceToDevice, st));
+    SBP_HIP(hipMemcpyAsync(dst->row_ptr, src->row_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToDevice, st));
+    SBP_HIP(hipMemcpyAsync(dst->col_ptr, src->col_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToDevice, st));
+    if (src->nnz > 0) {
+        SBP_HIP(hipMemcpyAsync(dst->row_cells, src->row_cells, sizeof(int2) * src->nnz, hipMemcpyDeviceToDevice, st));
+        SBP_HIP(hipMemcpyAsync(dst->col_cells, src->col_cells, sizeof(int2) * src->nnz, hipMemcpyDeviceToDevice, st));
+    }
+    dst->overall_entropy = src->overall_entropy;
+    dst->num_to_merge = 0;
+}
+
+}  // namespace sbpimpl

@@ -1,0 +1,740 @@
+// C-ABI of the SBP hot path: handles, finetune loops with early stopping, block-merge carry-out, the
+// agglomerative driver with its golden-section triplet, sampling.  Kernels live in build.hip, mdl.hip,
+// finetune.hip, merge.hip and mh.hip.
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <memory>
+#include <numeric>
+#include <queue>
+#include <vector>
+
+#include "device_common.hpp"
+#include "internal.hpp"
+#include "host_blockmodel.hpp"
+
+using namespace sbpimpl;
+
+namespace sbpimpl {
+
+double elapsed_ms(sbp_ctx ctx, hipEvent_t a, hipEvent_t b) {
+    (void)ctx;
+    float ms = 0.f;
+    SBP_HIP(hipEventSynchronize(b));
+    SBP_HIP(hipEventElapsedTime(&ms, a, b));
+    return (double)ms;
+}
+
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static sbp_params default_params() {
+    sbp_params p;
+    p.algorithm = 2;
+    p.batches = 1;
+    p.greedy = 1;
+    p.approximate = 0;
+    p.ordered = 0;
+    p.asynciterations = 2147483647;
+    p.mh_percent = 0.075;
+    p.seed = 0;
+    p.max_iterations = 100;
+    p.threads = 1;
+    return p;
+}
+
+static sbp_bm new_bm(sbp_ctx ctx) {
+    sbp_bm bm = new sbp_bm_s();
+    bm->ctx = ctx;
+    return bm;
+}
+
+static double mdl_of(sbp_ctx ctx, sbp_bm bm, sbp_graph g, sbp_stats *st) {
+    double t0 = now_ms();
+    double lp = log_posterior(ctx, bm);
+    if (st) st->mdl_ms += now_ms() - t0;
+    return model_term(g->E, g->N, bm->B) - lp;
+}
+
+// ------------------------------------------------------------------ finetune
+
+static int32_t *iota_of(sbp_ctx ctx, sbp_graph g);
+
+// One async-Gibbs batch: proposals for items[0, count) against the frozen blockmodel, then commit.
+static void gibbs_batch(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int greedy, uint32_t epoch, const int32_t *d_items,
+                        int64_t count, sbp_stats *st) {
+    hipStream_t s = ctx->stream;
+    int32_t *move_s = ctx->scratch[S_MOVES].as<int32_t>(g->N);
+    SBP_HIP(hipMemsetAsync(move_s, 0xff, sizeof(int32_t) * g->N, s));
+    double t0 = now_ms();
+    FtLaunchInfo info = ft_propose(ctx, bm, g, epoch, greedy, d_items, count, move_s, nullptr, nullptr);
+    int64_t refused = 0;
+    double t1 = now_ms();
+    int64_t moved = ft_commit(ctx, bm, g, move_s, &refused);
+    double t2 = now_ms();
+    if (st) {
+        st->proposals += count;
+        st->moves += moved;
+        st->refused += refused;
+        st->ft_bytes += info.bytes;
+        st->ft_launches += 1;
+        st->ft_kernel_ms += info.kernel_ms;
+        st->finetune_ms += (t1 - t0) + (t2 - t1);
+    }
+}
+
+static void async_gibbs_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t epoch,
+                              sbp_stats *st) {
+    const int64_t N = g->N;
+    int32_t *ident = iota_of(ctx, g);
+    const int32_t *order = ident;
+    if (!p.ordered && p.batches > 1) {
+        // the shuffle only changes which vertices share a batch; with one batch it has no effect
+        int32_t *sh = ctx->scratch[S_ORDER].as<int32_t>(N);
+        shuffle_keys(ctx, ident, N, epoch, sh);
+        order = sh;
+    }
+    int64_t batch_size = (int64_t)ceil((double)N / (double)p.batches);
+    for (int64_t batch = 0; batch_size > 0 && batch < N / batch_size; ++batch) {
+        int64_t start = batch * batch_size;
+        int64_t end = std::min<int64_t>(N, (batch + 1) * batch_size);
+        gibbs_batch(ctx, bm, g, p.greedy, epoch, order + start, end - start, st);
+    }
+    if (st) st->sweeps++;
+}
+
+// finetune::early_stop, 4-argument form (src/finetune.cpp:265-283).
+static bool early_stop4(int64_t iteration, bool golden_ratio_not_reached, double initial, const std::vector<double> &de) {
+    size_t last = de.size() - 1;
+    if (de[last] == 0.0) return true;
+    if (iteration < 3) return false;
+    double average = de[last] + de[last - 1] + de[last - 2];
+    average /= -3.0;
+    double threshold = golden_ratio_not_reached ? 5e-4 * initial : 1e-4 * initial;
+    return average < threshold;
+}
+
+// finetune::early_stop, 3-argument form (src/finetune.cpp:285-294).
+static bool early_stop3(int64_t iteration, double initial, const std::vector<double> &de) {
+    if (iteration < 3) return false;
+    size_t last = de.size() - 1;
+    double average = de[last] + de[last - 1] + de[last - 2];
+    average /= -3.0;
+    return average < 1e-4 * initial;
+}
+
+// Graph::sort_vertices (src/graph.cpp:235-268): stable descending degree; top ceil(mh_percent*N) are high.
+static void split_by_degree(sbp_graph g, double mh_percent, std::vector<int32_t> &hdv, std::vector<int32_t> &ldv) {
+    const int32_t N = g->N;
+    std::vector<int64_t> deg(N);
+    for (int32_t v = 0; v < N; ++v) {
+        int64_t self = 0;
+        for (int64_t e = g->h_out_off[v]; e < g->h_out_off[v + 1]; ++e)
+            if (g->h_out_adj[e] == v) {
+                self = 1;
+                break;
+            }
+        deg[v] = (g->h_out_off[v + 1] - g->h_out_off[v]) + (g->h_in_off[v + 1] - g->h_in_off[v]) - self;
+    }
+    std::vector<int32_t> idx(N);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return deg[a] > deg[b]; });
+    hdv.clear();
+    ldv.clear();
+    for (int32_t i = 0; i < N; ++i) {
+        if ((double)i < mh_percent * (double)N) hdv.push_back(idx[i]);
+        else ldv.push_back(idx[i]);
+    }
+}
+
+void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t epoch, const int32_t *d_order,
+            int64_t count, double *sum_dS, sbp_stats *st);
+
+static void mcmc(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, bool golden_ratio_not_reached,
+                 uint32_t *epoch, sbp_stats *st) {
+    if (bm->B == 1) return;
+    const int max_it = p.max_iterations > 0 ? p.max_iterations : 100;
+    std::vector<double> de;
+    if (p.algorithm == 0 || p.algorithm == 2) {
+        double initial = mdl_of(ctx, bm, g, st), last = initial;
+        bm->overall_entropy = initial;
+        std::vector<int32_t> hdv, ldv;
+        int32_t *d_h = nullptr, *d_l = nullptr;
+        if (p.algorithm == 2) {
+            split_by_degree(g, p.mh_percent, hdv, ldv);
+            SBP_HIP(hipMalloc(&d_h, sizeof(int32_t) * (hdv.size() + 1)));
+            SBP_HIP(hipMalloc(&d_l, sizeof(int32_t) * (ldv.size() + 1)));
+            SBP_HIP(hipMemcpy(d_h, hdv.data(), sizeof(int32_t) * hdv.size(), hipMemcpyHostToDevice));
+            SBP_HIP(hipMemcpy(d_l, ldv.data(), sizeof(int32_t) * ldv.size(), hipMemcpyHostToDevice));
+        }
+        for (int64_t it = 0; it < max_it; ++it) {
+            uint32_t ep = (*epoch)++;
+            if (p.algorithm == 0) {
+                async_gibbs_sweep(ctx, bm, g, p, ep, st);
+            } else {
+                // hybrid_mcmc (src/finetune.cpp:563-646)
+                const int64_t nh = (int64_t)hdv.size(), nl = (int64_t)ldv.size();
+                int32_t *h = ctx->scratch[S_ORDER].as<int32_t>(nh + nl + 2);
+                int32_t *l = h + nh + 1;
+                if (!p.ordered) {
+                    shuffle_keys(ctx, d_h, nh, ep, h);
+                    shuffle_keys(ctx, d_l, nl, ep, l);
+                } else {
+                    SBP_HIP(hipMemcpyAsync(h, d_h, sizeof(int32_t) * nh, hipMemcpyDeviceToDevice, ctx->stream));
+                    SBP_HIP(hipMemcpyAsync(l, d_l, sizeof(int32_t) * nl, hipMemcpyDeviceToDevice, ctx->stream));
+                }
+                double sum = 0.0;
+                mh_run(ctx, bm, g, p, ep, h, nh, &sum, st);
+                int64_t batch_size = (int64_t)ceil((double)nl / (double)p.batches);
+                for (int64_t batch = 0; batch_size > 0 && batch < nl / batch_size; ++batch) {
+                    int64_t start = batch * batch_size;
+                    int64_t end = std::min<int64_t>(nl, (batch + 1) * batch_size);
+                    gibbs_batch(ctx, bm, g, p.greedy, ep, l + start, end - start, st);
+                }
+                if (st) st->sweeps++;
+            }
+            double e = mdl_of(ctx, bm, g, st);
+            de.push_back(e - last);
+            last = e;
+            if (early_stop4(it, golden_ratio_not_reached, initial, de)) break;
+        }
+        if (d_h) (void)hipFree(d_h);
+        if (d_l) (void)hipFree(d_l);
+        bm->overall_entropy = last;
+    } else {
+        // metropolis_hastings (src/finetune.cpp:773-820)
+        bm->overall_entropy = mdl_of(ctx, bm, g, st);
+        int32_t *ident = iota_of(ctx, g);
+        int32_t *order = ctx->scratch[S_ORDER].as<int32_t>(g->N);
+        for (int64_t it = 0; it < max_it; ++it) {
+            uint32_t ep = (*epoch)++;
+            const int32_t *ord = ident;
+            if (!p.ordered) {
+                shuffle_keys(ctx, ident, g->N, ep, order);
+                ord = order;
+            }
+            double sum = 0.0;
+            mh_run(ctx, bm, g, p, ep, ord, g->N, &sum, st);
+            if (st) st->sweeps++;
+            de.push_back(sum);
+            if (early_stop4(it, golden_ratio_not_reached, bm->overall_entropy, de)) break;
+        }
+        bm->overall_entropy = mdl_of(ctx, bm, g, st);
+    }
+}
+
+static void finetune_assignment(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t *epoch,
+                                sbp_stats *st) {
+    const int max_it = p.max_iterations > 0 ? p.max_iterations : 100;
+    std::vector<double> de;
+    bm->overall_entropy = mdl_of(ctx, bm, g, st);
+    int32_t *ident = iota_of(ctx, g);
+    for (int64_t it = 0; it < max_it; ++it) {
+        uint32_t ep = (*epoch)++;
+        double sum = 0.0;
+        mh_run(ctx, bm, g, p, ep, ident, g->N, &sum, st);
+        if (st) st->sweeps++;
+        de.push_back(sum);
+        if (early_stop3(it, bm->overall_entropy, de)) break;
+    }
+    bm->overall_entropy = mdl_of(ctx, bm, g, st);
+}
+
+// ------------------------------------------------------------------ block merge
+
+static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int64_t num_to_merge,
+                           bool approximate, sbp_stats *st) {
+    hipStream_t s = ctx->stream;
+    const int32_t B = bm->B;
+    int32_t *d_best_s = ctx->scratch[S_BEST].as<int32_t>(3 * (size_t)B + 4);
+    double *d_best_dS = reinterpret_cast<double *>(ctx->scratch[S_DS].as<double>((size_t)B + 2));
+    SBP_HIP(hipEventRecord(ctx->ev[2], s));
+    uint64_t bytes = merge_propose(ctx, bm, epoch, d_best_s, d_best_dS);
+    SBP_HIP(hipEventRecord(ctx->ev[3], s));
+    std::vector<int32_t> best_s(B);
+    std::vector<double> best_dS(B);
+    SBP_HIP(hipMemcpyAsync(best_s.data(), d_best_s, sizeof(int32_t) * B, hipMemcpyDeviceToHost, s));
+    SBP_HIP(hipMemcpyAsync(best_dS.data(), d_best_dS, sizeof(double) * B, hipMemcpyDeviceToHost, s));
+    SBP_HIP(hipStreamSynchronize(s));
+    if (st) {
+        st->merge_ms += elapsed_ms(ctx, ctx->ev[2], ctx->ev[3]);
+        st->merge_bytes += bytes;
+        st->merge_proposals += 10 * (int64_t)B;
+        st->merge_phases++;
+    }
+    double t0 = now_ms();
+    std::vector<int32_t> block_map;
+    int32_t new_B = carry_out_merges(ctx, bm, best_s, best_dS, num_to_merge, approximate, block_map);
+    if (st) st->carry_ms += now_ms() - t0;
+    double t1 = now_ms();
+    int32_t *d_map = d_best_s;  // reuse
+    SBP_HIP(hipMemcpyAsync(d_map, block_map.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice, s));
+    sbp_bm out = new_bm(ctx);
+    bm_alloc(out, new_B, bm->N);
+    relabel_assignment(ctx, bm, d_map, new_B, out->assign);
+    bm_rebuild(ctx, g, out);
+    SBP_HIP(hipStreamSynchronize(s));
+    out->overall_entropy = bm->overall_entropy;
+    if (st) st->build_ms += now_ms() - t1;
+    return out;
+}
+
+// ------------------------------------------------------------------ driver: triplet + SBP loop
+
+struct Triplet {
+    sbp_bm bm[3] = {nullptr, nullptr, nullptr};
+    bool optimal = false;
+    sbp_ctx ctx;
+    explicit Triplet(sbp_ctx c) : ctx(c) {}
+    ~Triplet() {
+        for (auto &b : bm) free_bm(b);
+    }
+    static void free_bm(sbp_bm &b) {
+        if (b) {
+            bm_release(b);
+            delete b;
+            b = nullptr;
+        }
+    }
+    bool empty(int i) const { return bm[i] == nullptr; }
+    sbp_bm copy_of(sbp_bm src) {
+        sbp_bm b = new_bm(ctx);
+        bm_copy_into(ctx, src, b);
+        return b;
+    }
+    void set(int i, sbp_bm b) {
+        free_bm(bm[i]);
+        bm[i] = b;
+    }
+    // BlockmodelTriplet::update (src/blockmodel/blockmodel_triplet.cpp:82-105)
+    void update(sbp_bm b) {
+        int index;
+        if (empty(1)) {
+            index = 1;
+        } else if (b->overall_entropy <= bm[1]->overall_entropy) {
+            int old_index = (bm[1]->B > b->B) ? 0 : 2;
+            set(old_index, copy_of(bm[1]));
+            index = 1;
+        } else {
+            index = (bm[1]->B > b->B) ? 2 : 0;
+        }
+        set(index, copy_of(b));
+    }
+    int64_t nb(int i) const { return empty(i) ? 0 : bm[i]->B; }
+    bool is_done() {
+        if ((!empty(0) && nb(0) - nb(2) == 2) || (empty(0) && nb(1) - nb(2) == 1)) optimal = true;
+        return optimal;
+    }
+    // BlockmodelTriplet::get_next_blockmodel (src/blockmodel/blockmodel_triplet.cpp:4-37)
+    sbp_bm next(sbp_bm old) {
+        old->num_to_merge = 0;
+        update(old);
+        if (empty(2)) {
+            sbp_bm b = copy_of(bm[1]);
+            b->num_to_merge = (int64_t)((double)b->B * 0.5);
+            if (b->num_to_merge == 0) optimal = true;
+            return b;
+        }
+        if (is_done()) return copy_of(bm[1]);
+        int index;
+        if (empty(0) && nb(1) > nb(2)) index = 1;
+        else if (nb(0) - nb(1) >= nb(1) - nb(2)) index = 0;
+        else index = 1;
+        int64_t next_nb = nb(index + 1);
+        next_nb += (int64_t)((double)(nb(index) - nb(index + 1)) * 0.618);
+        sbp_bm b = copy_of(bm[index]);
+        b->num_to_merge = b->B - next_nb;
+        return b;
+    }
+};
+
+static void partition(sbp_ctx ctx, sbp_graph g, const sbp_params &p, std::vector<int32_t> &assignment,
+                      int32_t &num_blocks, sbp_stats *st) {
+    const int32_t N = g->N;
+    std::vector<int32_t> ident(N);
+    std::iota(ident.begin(), ident.end(), 0);
+    sbp_bm bm = new_bm(ctx);
+    bm_alloc(bm, N, N);
+    SBP_HIP(hipMemcpy(bm->assign, ident.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
+    double t0 = now_ms();
+    bm_rebuild(ctx, g, bm);
+    SBP_HIP(hipStreamSynchronize(ctx->stream));
+    if (st) st->build_ms += now_ms() - t0;
+    bm->num_to_merge = (int64_t)((double)N * 0.5);
+    Triplet tri(ctx);
+    uint32_t epoch = 0;
+    int64_t iteration = 0;
+    while (!tri.optimal) {
+        sbp_bm merged = merge_blocks(ctx, bm, g, epoch++, bm->num_to_merge, p.approximate != 0, st);
+        Triplet::free_bm(bm);
+        bm = merged;
+        sbp_params q = p;
+        if (p.algorithm == 0 && iteration >= p.asynciterations) q.algorithm = 1;
+        mcmc(ctx, bm, g, q, tri.empty(2), &epoch, st);
+        iteration++;
+        sbp_bm next = tri.next(bm);
+        Triplet::free_bm(bm);
+        bm = next;
+    }
+    assignment.resize(N);
+    SBP_HIP(hipMemcpy(assignment.data(), bm->assign, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    num_blocks = bm->B;
+    if (st) st->mdl = bm->overall_entropy;
+    Triplet::free_bm(bm);
+}
+
+static int32_t *iota_of(sbp_ctx ctx, sbp_graph g) {
+    (void)ctx;
+    return g->edge_src + g->E;  // identity vector stored after the edge sources at upload
+}
+
+}  // namespace sbpimpl
+
+// ====================================================================== C-ABI
+
+#define SBP_API_BEGIN try {
+#define SBP_API_END(ctx)                                  \
+    }                                                     \
+    catch (const sbpimpl::Error &e) {                     \
+        if (ctx) (ctx)->err = e.what();                   \
+        return e.code;                                    \
+    }                                                     \
+    catch (const std::bad_alloc &) {                      \
+        if (ctx) (ctx)->err = "host out of memory";       \
+        return SBP_E_OOM;                                 \
+    }                                                     \
+    catch (const std::exception &e) {                     \
+        if (ctx) (ctx)->err = e.what();                   \
+        return SBP_E_STATE;                               \
+    }                                                     \
+    return SBP_OK;
+
+extern "C" {
+
+int sbp_create(sbp_ctx *out, int device, uint64_t seed) {
+    if (!out) return SBP_E_ARG;
+    *out = nullptr;
+    sbp_ctx ctx = nullptr;
+    try {
+        int n = 0;
+        SBP_HIP(hipGetDeviceCount(&n));
+        SBP_CHECK(device >= 0 && device < n, SBP_E_ARG, "no such HIP device");
+        SBP_HIP(hipSetDevice(device));
+        ctx = new sbp_ctx_s();
+        ctx->device = device;
+        ctx->seed = seed;
+        SBP_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        for (auto &e : ctx->ev) SBP_HIP(hipEventCreate(&e));
+        *out = ctx;
+        return SBP_OK;
+    } catch (const sbpimpl::Error &e) {
+        delete ctx;
+        return e.code;
+    }
+}
+
+int sbp_destroy(sbp_ctx ctx) {
+    if (!ctx) return SBP_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &s : ctx->scratch) {
+        if (s.ptr) (void)hipFree(s.ptr);
+        s.ptr = nullptr;
+        s.bytes = 0;
+    }
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SBP_OK;
+}
+
+const char *sbp_last_error(sbp_ctx ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int sbp_set_seed(sbp_ctx ctx, uint64_t seed) {
+    if (!ctx) return SBP_E_ARG;
+    ctx->seed = seed;
+    return SBP_OK;
+}
+
+int sbp_synchronize(sbp_ctx ctx) {
+    if (!ctx) return SBP_E_ARG;
+    SBP_API_BEGIN
+    SBP_HIP(hipStreamSynchronize(ctx->stream));
+    SBP_API_END(ctx)
+}
+
+int sbp_graph_upload(sbp_ctx ctx, int32_t N, int64_t E, const int64_t *out_off, const int32_t *out_adj,
+                     const int64_t *in_off, const int32_t *in_adj, sbp_graph *out) {
+    if (!ctx || !out || N <= 0 || E < 0 || !out_off || !in_off || (E > 0 && (!out_adj || !in_adj))) return SBP_E_ARG;
+    *out = nullptr;
+    SBP_API_BEGIN
+    SBP_CHECK(out_off[0] == 0 && out_off[N] == E && in_off[0] == 0 && in_off[N] == E, SBP_E_ARG,
+              "graph offsets do not match E");
+    auto g = std::make_unique<sbp_graph_s>();
+    g->ctx = ctx;
+    g->N = N;
+    g->E = E;
+    g->h_out_off.assign(out_off, out_off + N + 1);
+    g->h_in_off.assign(in_off, in_off + N + 1);
+    g->h_out_adj.assign(out_adj, out_adj + E);
+    g->h_in_adj.assign(in_adj, in_adj + E);
+    std::vector<int32_t> src((size_t)E + N);
+    int64_t maxdeg = 0;
+    for (int32_t v = 0; v < N; ++v) {
+        SBP_CHECK(out_off[v + 1] >= out_off[v] && in_off[v + 1] >= in_off[v], SBP_E_ARG, "offsets not monotone");
+        for (int64_t e = out_off[v]; e < out_off[v + 1]; ++e) {
+            SBP_CHECK(out_adj[e] >= 0 && out_adj[e] < N, SBP_E_ARG, "out_adj id out of range");
+            src[e] = v;
+        }
+        for (int64_t e = in_off[v]; e < in_off[v + 1]; ++e)
+            SBP_CHECK(in_adj[e] >= 0 && in_adj[e] < N, SBP_E_ARG, "in_adj id out of range");
+        maxdeg = std::max<int64_t>(maxdeg, (out_off[v + 1] - out_off[v]) + (in_off[v + 1] - in_off[v]));
+        src[(size_t)E + v] = v;
+    }
+    g->max_degree = maxdeg;
+    SBP_HIP(hipMalloc(&g->out_off, sizeof(int64_t) * (N + 1)));
+    SBP_HIP(hipMalloc(&g->in_off, sizeof(int64_t) * (N + 1)));
+    SBP_HIP(hipMalloc(&g->out_adj, sizeof(int32_t) * (E + 1)));
+    SBP_HIP(hipMalloc(&g->in_adj, sizeof(int32_t) * (E + 1)));
+    SBP_HIP(hipMalloc(&g->edge_src, sizeof(int32_t) * (E + N)));
+    SBP_HIP(hipMemcpy(g->out_off, out_off, sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
+    SBP_HIP(hipMemcpy(g->in_off, in_off, sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
+    if (E > 0) {
+        SBP_HIP(hipMemcpy(g->out_adj, out_adj, sizeof(int32_t) * E, hipMemcpyHostToDevice));
+        SBP_HIP(hipMemcpy(g->in_adj, in_adj, sizeof(int32_t) * E, hipMemcpyHostToDevice));
+    }
+    SBP_HIP(hipMemcpy(g->edge_src, src.data(), sizeof(int32_t) * (E + N), hipMemcpyHostToDevice));
+    *out = g.release();
+    SBP_API_END(ctx)
+}
+
+int sbp_graph_free(sbp_graph g) {
+    if (!g) return SBP_OK;
+    void *p[] = {g->out_off, g->in_off, g->out_adj, g->in_adj, g->edge_src};
+    for (void *x : p)
+        if (x) (void)hipFree(x);
+    delete g;
+    return SBP_OK;
+}
+
+int sbp_bm_build(sbp_ctx ctx, sbp_graph g, int32_t B, const int32_t *assignment, sbp_bm *out) {
+    if (!ctx || !g || !assignment || !out || B <= 0) return SBP_E_ARG;
+    *out = nullptr;
+    SBP_API_BEGIN
+    for (int32_t v = 0; v < g->N; ++v)
+        SBP_CHECK(assignment[v] >= 0 && assignment[v] < B, SBP_E_ARG, "assignment value out of [0, B)");
+    sbp_bm bm = new_bm(ctx);
+    try {
+        bm_alloc(bm, B, g->N);
+        SBP_HIP(hipMemcpy(bm->assign, assignment, sizeof(int32_t) * g->N, hipMemcpyHostToDevice));
+        bm_rebuild(ctx, g, bm);
+        SBP_HIP(hipStreamSynchronize(ctx->stream));
+        bm->num_to_merge = (int64_t)((double)B * 0.5);
+    } catch (...) {
+        bm_release(bm);
+        delete bm;
+        throw;
+    }
+    *out = bm;
+    SBP_API_END(ctx)
+}
+
+int sbp_bm_copy(sbp_ctx ctx, sbp_bm src, sbp_bm *out) {
+    if (!ctx || !src || !out) return SBP_E_ARG;
+    *out = nullptr;
+    SBP_API_BEGIN
+    sbp_bm bm = new_bm(ctx);
+    bm_copy_into(ctx, src, bm);
+    SBP_HIP(hipStreamSynchronize(ctx->stream));
+    *out = bm;
+    SBP_API_END(ctx)
+}
+
+int sbp_bm_free(sbp_bm bm) {
+    if (!bm) return SBP_OK;
+    bm_release(bm);
+    delete bm;
+    return SBP_OK;
+}
+
+int sbp_bm_info(sbp_bm bm, int32_t *num_blocks, int64_t *nnz) {
+    if (!bm) return SBP_E_ARG;
+    if (num_blocks) *num_blocks = bm->B;
+    if (nnz) *nnz = bm->nnz;
+    return SBP_OK;
+}
+
+int sbp_bm_download(sbp_ctx ctx, sbp_bm bm, int32_t *assignment, int32_t *sizes, int32_t *d_out, int32_t *d_in,
+                    int32_t *d_tot, int32_t *row_ptr, int32_t *cols, int32_t *cnts) {
+    if (!ctx || !bm) return SBP_E_ARG;
+    SBP_API_BEGIN
+    hipStream_t s = ctx->stream;
+    SBP_HIP(hipStreamSynchronize(s));
+    const int32_t B = bm->B;
+    if (assignment) SBP_HIP(hipMemcpy(assignment, bm->assign, sizeof(int32_t) * bm->N, hipMemcpyDeviceToHost));
+    if (sizes) SBP_HIP(hipMemcpy(sizes, bm->sizes, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+    if (d_out) SBP_HIP(hipMemcpy(d_out, bm->d_out, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+    if (d_in) SBP_HIP(hipMemcpy(d_in, bm->d_in, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+    if (d_tot) SBP_HIP(hipMemcpy(d_tot, bm->d_tot, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+    if (row_ptr) SBP_HIP(hipMemcpy(row_ptr, bm->row_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
+    if ((cols || cnts) && bm->nnz > 0) {
+        std::vector<int2> cells(bm->nnz);
+        SBP_HIP(hipMemcpy(cells.data(), bm->row_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < bm->nnz; ++i) {
+            if (cols) cols[i] = cells[i].x;
+            if (cnts) cnts[i] = cells[i].y;
+        }
+    }
+    SBP_API_END(ctx)
+}
+
+int sbp_bm_validate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *ok) {
+    if (!ctx || !bm || !g || !ok) return SBP_E_ARG;
+    SBP_API_BEGIN
+    sbp_bm fresh = new_bm(ctx);
+    bm_alloc(fresh, bm->B, bm->N);
+    SBP_HIP(hipMemcpyAsync(fresh->assign, bm->assign, sizeof(int32_t) * bm->N, hipMemcpyDeviceToDevice, ctx->stream));
+    bm_rebuild(ctx, g, fresh);
+    SBP_HIP(hipStreamSynchronize(ctx->stream));
+    HostBlockmodel a, b;
+    a.download(ctx, bm);
+    b.download(ctx, fresh);
+    *ok = a.equals(b) ? 1 : 0;
+    bm_release(fresh);
+    delete fresh;
+    SBP_API_END(ctx)
+}
+
+int sbp_mdl(sbp_ctx ctx, sbp_bm bm, sbp_graph g, double *mdl, double *lp) {
+    if (!ctx || !bm || !g) return SBP_E_ARG;
+    SBP_API_BEGIN
+    double l = log_posterior(ctx, bm);
+    if (lp) *lp = l;
+    if (mdl) *mdl = model_term(g->E, g->N, bm->B) - l;
+    SBP_API_END(ctx)
+}
+
+int sbp_ft_propose_all(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int32_t greedy, int32_t *s_out,
+                       double *dS_out, int32_t *accept_out) {
+    if (!ctx || !bm || !g) return SBP_E_ARG;
+    SBP_API_BEGIN
+    hipStream_t s = ctx->stream;
+    const int32_t N = g->N;
+    int32_t *move_s = ctx->scratch[S_MOVES].as<int32_t>(N);
+    double *dS = ctx->scratch[S_DS].as<double>(N);
+    int32_t *sv = ctx->scratch[S_ACCEPT].as<int32_t>(N);
+    SBP_HIP(hipMemsetAsync(move_s, 0xff, sizeof(int32_t) * N, s));
+    SBP_HIP(hipMemsetAsync(dS, 0, sizeof(double) * N, s));
+    ft_propose(ctx, bm, g, epoch, greedy, iota_of(ctx, g), N, move_s, dS, sv);
+    std::vector<int32_t> hm(N);
+    SBP_HIP(hipMemcpy(hm.data(), move_s, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    if (s_out) SBP_HIP(hipMemcpy(s_out, sv, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    if (dS_out) SBP_HIP(hipMemcpy(dS_out, dS, sizeof(double) * N, hipMemcpyDeviceToHost));
+    if (accept_out)
+        for (int32_t v = 0; v < N; ++v) accept_out[v] = hm[v] >= 0 ? 1 : 0;
+    SBP_API_END(ctx)
+}
+
+int sbp_async_gibbs_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, uint32_t epoch,
+                          sbp_stats *stats) {
+    if (!ctx || !bm || !g || !p || p->batches < 1) return SBP_E_ARG;
+    SBP_API_BEGIN
+    async_gibbs_sweep(ctx, bm, g, *p, epoch, stats);
+    SBP_API_END(ctx)
+}
+
+int sbp_mh_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, uint32_t epoch, int32_t ordered,
+                 double *sum_dS, sbp_stats *stats) {
+    if (!ctx || !bm || !g || !p) return SBP_E_ARG;
+    SBP_API_BEGIN
+    int32_t *ident = iota_of(ctx, g);
+    const int32_t *ord = ident;
+    if (!ordered) {
+        int32_t *order = ctx->scratch[S_ORDER].as<int32_t>(g->N);
+        shuffle_keys(ctx, ident, g->N, epoch, order);
+        ord = order;
+    }
+    double sum = 0.0;
+    mh_run(ctx, bm, g, *p, epoch, ord, g->N, &sum, stats);
+    if (stats) stats->sweeps++;
+    if (sum_dS) *sum_dS = sum;
+    SBP_API_END(ctx)
+}
+
+int sbp_mcmc(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, int32_t golden_ratio_not_reached,
+             uint32_t *epoch, double *mdl_out, sbp_stats *stats) {
+    if (!ctx || !bm || !g || !p || !epoch || p->batches < 1) return SBP_E_ARG;
+    SBP_API_BEGIN
+    mcmc(ctx, bm, g, *p, golden_ratio_not_reached != 0, epoch, stats);
+    if (mdl_out) *mdl_out = bm->overall_entropy;
+    SBP_API_END(ctx)
+}
+
+int sbp_finetune_assignment(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, uint32_t *epoch,
+                            double *mdl_out, sbp_stats *stats) {
+    if (!ctx || !bm || !g || !p || !epoch) return SBP_E_ARG;
+    SBP_API_BEGIN
+    finetune_assignment(ctx, bm, g, *p, epoch, stats);
+    if (mdl_out) *mdl_out = bm->overall_entropy;
+    SBP_API_END(ctx)
+}
+
+int sbp_merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *best_s, double *best_dS) {
+    if (!ctx || !bm) return SBP_E_ARG;
+    SBP_API_BEGIN
+    const int32_t B = bm->B;
+    int32_t *d_s = ctx->scratch[S_BEST].as<int32_t>(3 * (size_t)B + 4);
+    double *d_dS = ctx->scratch[S_DS].as<double>((size_t)B + 2);
+    merge_propose(ctx, bm, epoch, d_s, d_dS);
+    if (best_s) SBP_HIP(hipMemcpy(best_s, d_s, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+    if (best_dS) SBP_HIP(hipMemcpy(best_dS, d_dS, sizeof(double) * B, hipMemcpyDeviceToHost));
+    SBP_API_END(ctx)
+}
+
+int sbp_merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int64_t num_to_merge, int32_t approximate,
+                     sbp_bm *out, sbp_stats *stats) {
+    if (!ctx || !bm || !g || !out || num_to_merge < 0 || num_to_merge >= bm->B) return SBP_E_ARG;
+    *out = nullptr;
+    SBP_API_BEGIN
+    *out = merge_blocks(ctx, bm, g, epoch, num_to_merge, approximate != 0, stats);
+    SBP_API_END(ctx)
+}
+
+int sbp_partition(sbp_ctx ctx, sbp_graph g, const sbp_params *p, int32_t *assignment_out, int32_t *num_blocks_out,
+                  sbp_stats *stats) {
+    if (!ctx || !g || !p || p->batches < 1 || p->algorithm < 0 || p->algorithm > 2) return SBP_E_ARG;
+    SBP_API_BEGIN
+    double t0 = now_ms();
+    std::vector<int32_t> a;
+    int32_t nb = 0;
+    uint64_t saved = ctx->seed;
+    ctx->seed = p->seed;
+    try {
+        partition(ctx, g, *p, a, nb, stats);
+    } catch (...) {
+        ctx->seed = saved;
+        throw;
+    }
+    ctx->seed = saved;
+    if (assignment_out) memcpy(assignment_out, a.data(), sizeof(int32_t) * g->N);
+    if (num_blocks_out) *num_blocks_out = nb;
+    if (stats) stats->total_ms += now_ms() - t0;
+    SBP_API_END(ctx)
+}
+
+int sbp_debug_logf(sbp_ctx ctx, uint32_t first_bits, uint32_t count, float *out) {
+    if (!ctx || !out) return SBP_E_ARG;
+    SBP_API_BEGIN
+    float *d = ctx->scratch[S_MISC].as<float>(count + 1);
+    debug_logf(ctx, first_bits, count, d);
+    SBP_HIP(hipMemcpyAsync(out, d, sizeof(float) * count, hipMemcpyDeviceToHost, ctx->stream));
+    SBP_HIP(hipStreamSynchronize(ctx->stream));
+    SBP_API_END(ctx)
+}
+
+}  // extern "C"

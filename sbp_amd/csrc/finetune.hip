@@ -1,0 +1,1023 @@
+// Finetune: vertex-move proposals (one wavefront per vertex), the asynchronous-Gibbs commit, and
+// sequential Metropolis-Hastings.
+//
+// Proposal (propose_gibbs_move_v3 + eval_vertex_move_v3, src/finetune.cpp:447-462,871-901):
+//   1. a uniformly random neighbour position in concat(out(v), in(v) \ {v}) gives neighbour u, t = b(u)
+//      (common::propose_new_block, src/common.cpp:138-198);
+//   2. s is drawn from neighbors_weights(t) = row(t) + col(t) (dict_transpose_matrix.cpp:176-188), whose
+//      total is d_tot[t]; the wave scans the cells with an inclusive prefix sum and a ballot;
+//   3. delta-MDL (entropy::delta_mdl, src/entropy.cpp:298-403): the neighbour-block histogram of v is
+//      built in a hash table (LDS, one per wave; global memory for high-degree vertices); the Delta cells
+//      of finetune::blockmodel_delta (src/finetune.cpp:201-235) follow from it in closed form, including
+//      cells whose adds and subs cancel (they stay in the reference's Delta maps and are counted twice);
+//      rows r, s and columns r, s are scanned once for the unchanged cells;
+//   4. Hastings correction (src/entropy.cpp:497-561) when not greedy; accept (src/finetune.cpp:22-32).
+// Terms are summed in 128-bit fixed point.
+//
+// Async-Gibbs commit (Blockmodel::move_vertex(const VertexMove_v3&), src/blockmodel/blockmodel.cpp:437-494,
+// in vertex-id order, src/finetune.cpp:80-85): the block matrix after the commits is a pure function of
+// the final assignment, so only the refusals (a move out of a block whose size has reached 1) need the
+// sequential order.  Blocks whose proposed out-moves reach their size are "at risk"; for them the
+// refusal pattern is the unique fixed point of a per-block scan over their move events in vertex order,
+// reached by monotone iteration from "all accepted".  The matrix is then rebuilt from the assignment.
+//
+// Metropolis-Hastings (src/finetune.cpp:773-869) is sequential by definition: one persistent wavefront
+// walks the vertex order, evaluates each proposal exactly as above against the current state and applies
+// accepted Deltas in place (Blockmodel::move_vertex(Vertex, const Delta&, ...), blockmodel.cpp:408-435) to
+// a CSR whose rows and columns carry slack.  When a row or column runs out of slack the kernel stops
+// before that vertex; the host re-lays the matrix out with more slack and resumes.
+#include <hipcub/hipcub.hpp>
+#include <math.h>
+
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace sbpimpl {
+using namespace sbpdev;
+
+constexpr int FT_BLOCK = 256;
+constexpr int FT_WAVES = FT_BLOCK / WAVE;
+constexpr int LDS_CAP = 1024;         // hash slots per wave
+constexpr int SMALL_K = LDS_CAP / 2;  // max |out| + |in| for the LDS table
+constexpr int EMPTY = -1;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+struct Table {
+    int *keys;
+    int *outc;
+    int *inc;
+    unsigned mask;
+    int shift;  // 32 - log2(cap)
+    int cap;
+};
+
+__device__ __forceinline__ unsigned hash_slot(const Table &t, int key) {
+    return ((unsigned)key * 0x9E3779B1u) >> t.shift;
+}
+
+__device__ __forceinline__ void table_add(const Table &t, int key, int dout, int din) {
+    unsigned slot = hash_slot(t, key);
+    while (true) {
+        int prev = atomicCAS(&t.keys[slot], EMPTY, key);
+        if (prev == EMPTY || prev == key) {
+            if (dout) atomicAdd(&t.outc[slot], dout);
+            if (din) atomicAdd(&t.inc[slot], din);
+            return;
+        }
+        slot = (slot + 1) & t.mask;
+    }
+}
+
+__device__ __forceinline__ void table_get(const Table &t, int key, int &o, int &i) {
+    unsigned slot = hash_slot(t, key);
+    while (true) {
+        int k = t.keys[slot];
+        if (k == key) {
+            o = t.outc[slot];
+            i = t.inc[slot];
+            return;
+        }
+        if (k == EMPTY) {
+            o = 0;
+            i = 0;
+            return;
+        }
+        slot = (slot + 1) & t.mask;
+    }
+}
+
+// Read-only view of the block matrix.  Compact CSR: row b = [ptr[b], ptr[b+1]).  Slack layout (used by
+// Metropolis-Hastings): row b = [start[b], start[b] + len[b]).
+struct BmView {
+    const int32_t *rbeg, *rlen, *cbeg, *clen;  // rlen/clen null for compact CSR
+    int2 *row_cells, *col_cells;
+    __device__ __forceinline__ int row_begin(int b) const { return rbeg[b]; }
+    __device__ __forceinline__ int row_end(int b) const { return rlen ? rbeg[b] + rlen[b] : rbeg[b + 1]; }
+    __device__ __forceinline__ int col_begin(int b) const { return cbeg[b]; }
+    __device__ __forceinline__ int col_end(int b) const { return clen ? cbeg[b] + clen[b] : cbeg[b + 1]; }
+};
+
+struct FtArgs {
+    int32_t N, B;
+    uint64_t seed;
+    uint32_t epoch;
+    int greedy;
+    const int64_t *out_off, *in_off;
+    const int32_t *out_adj, *in_adj;
+    int32_t *assign, *sizes, *d_out, *d_in, *d_tot;
+    BmView m;
+    const int32_t *items;  // vertices to evaluate
+    int64_t count;
+    int32_t *move_s;       // [N] accepted target or -1
+    double *dS_out;        // optional [N]
+    int32_t *s_out;        // optional [N]
+    unsigned long long *bytes;
+    int *gtable;           // high-degree tables: per wave 3 * 2^gcap_log2 ints
+    int gcap_log2;
+    int *err;              // set when a multinomial runs past its cells (inconsistent degrees)
+};
+
+__device__ __forceinline__ long long get_cell(const FtArgs &a, int row, int col) {
+    int r0 = a.m.row_begin(row), r1 = a.m.row_end(row);
+    int c0 = a.m.col_begin(col), c1 = a.m.col_end(col);
+    if (r1 - r0 <= c1 - c0) return cell_lookup(a.m.row_cells, r0, r1, col);
+    return cell_lookup(a.m.col_cells, c0, c1, row);
+}
+
+struct MoveCtx {
+    int v, r, s, t;
+    int selfc;  // self-loop occurrences in out(v)
+    long long k_out, k_in, shift_in, k_all;
+    int out_r, in_r, out_s, in_s;
+    int64_t o0, o1, i0, i1;
+};
+
+__device__ __forceinline__ long long new_din(const FtArgs &a, const MoveCtx &m, int x) {
+    long long v = a.d_in[x];
+    if (x == m.r) v -= m.shift_in;
+    else if (x == m.s) v += m.shift_in;
+    return v;
+}
+__device__ __forceinline__ long long new_dout(const FtArgs &a, const MoveCtx &m, int x) {
+    long long v = a.d_out[x];
+    if (x == m.r) v -= m.k_out;
+    else if (x == m.s) v += m.k_out;
+    return v;
+}
+
+__device__ __forceinline__ void delta_cell_term(const FtArgs &a, const MoveCtx &m, int row, int col, long long d,
+                                                Fx &acc) {
+    long long M = get_cell(a, row, col);
+    fx_acc(acc, cell_entropy(M, a.d_in[col], a.d_out[row]));
+    fx_sub_acc(acc, cell_entropy(M + d, new_din(a, m, col), new_dout(a, m, row)));
+}
+
+__device__ __forceinline__ void unchanged_term(const FtArgs &a, const MoveCtx &m, int row, int col, long long M,
+                                               Fx &acc) {
+    fx_acc(acc, cell_entropy(M, a.d_in[col], a.d_out[row]));
+    fx_sub_acc(acc, cell_entropy(M, new_din(a, m, col), new_dout(a, m, row)));
+}
+
+// Delta of row r / row s at column c from the histogram (0 when the cell is not in the Delta).
+__device__ __forceinline__ long long delta_row_r(const MoveCtx &m, int c, int oc) {
+    return -(long long)oc + (c == m.r ? -(long long)m.selfc - m.in_r : 0) + (c == m.s ? (long long)m.in_r : 0);
+}
+__device__ __forceinline__ long long delta_row_s(const MoveCtx &m, int c, int oc) {
+    return (long long)oc + (c == m.s ? (long long)m.selfc + m.in_s : 0) + (c == m.r ? -(long long)m.in_s : 0);
+}
+
+__device__ __forceinline__ Table make_table(const FtArgs &a, bool big, int *lds_keys, int *lds_out, int *lds_in,
+                                            int64_t gwave) {
+    Table t;
+    if (big) {
+        t.cap = 1 << a.gcap_log2;
+        t.keys = a.gtable + gwave * 3 * (int64_t)t.cap;
+        t.outc = t.keys + t.cap;
+        t.inc = t.outc + t.cap;
+        t.shift = 32 - a.gcap_log2;
+    } else {
+        t.cap = LDS_CAP;
+        t.keys = lds_keys;
+        t.outc = lds_out;
+        t.inc = lds_in;
+        t.shift = 32 - 10;
+    }
+    t.mask = (unsigned)t.cap - 1;
+    return t;
+}
+
+// Proposal of vertex v against the current state.  Returns s (== r for "no move").  Fills m.
+__device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w, unsigned long long &bytes) {
+    const int lane = lane_id();
+    m.v = v;
+    m.o0 = a.out_off[v];
+    m.o1 = a.out_off[v + 1];
+    m.i0 = a.in_off[v];
+    m.i1 = a.in_off[v + 1];
+    const long long ko = m.o1 - m.o0, kia = m.i1 - m.i0;
+    m.r = a.assign[v];
+    int self_out = 0, self_in = 0;
+    for (int64_t e = m.o0 + lane; e < m.o1; e += WAVE) self_out += (a.out_adj[e] == v);
+    for (int64_t e = m.i0 + lane; e < m.i1; e += WAVE) self_in += (a.in_adj[e] == v);
+    self_out = wave_sum(self_out);
+    self_in = wave_sum(self_in);
+    m.selfc = self_out;
+    m.k_out = ko;
+    m.k_in = kia - self_in;
+    m.k_all = ko + kia;
+    m.shift_in = m.k_in + (self_out > 0 ? 1 : 0);
+    const long long k = ko + m.k_in;
+    w = philox(a.seed, (uint32_t)v, a.epoch, TAG_FT, 0);
+    int s = -1;
+    m.t = -1;
+    if (k == 0) {
+        s = random_block(m.r, a.B, w.y);
+    } else {
+        const uint32_t idx = uniform_below(w.x, (uint32_t)k);
+        int u;
+        if ((long long)idx < ko) {
+            u = a.out_adj[m.o0 + idx];
+        } else if (self_in == 0) {
+            u = a.in_adj[m.i0 + (idx - ko)];
+        } else {
+            long long want = (long long)idx - ko;
+            u = -1;
+            for (int64_t base = m.i0; base < m.i1 && u < 0; base += WAVE) {
+                int64_t e = base + lane;
+                int x = e < m.i1 ? a.in_adj[e] : v;
+                bool ok = (e < m.i1) && (x != v);
+                unsigned long long msk = ballot(ok);
+                int cnt = __popcll(msk);
+                if (want < cnt) {
+                    int before = __popcll(msk & ((1ull << lane) - 1ull));
+                    unsigned long long hit = ballot(ok && before == want);
+                    u = __shfl(x, __ffsll((long long)hit) - 1, WAVE);
+                } else {
+                    want -= cnt;
+                }
+            }
+        }
+        const int t = a.assign[u];
+        m.t = t;
+        const long long total = a.d_tot[t];
+        const long long target = (long long)uniform_below(w.y, (uint32_t)total);
+        long long base_sum = 0;
+        const int rp0 = a.m.row_begin(t), rp1 = a.m.row_end(t);
+        for (int c0 = rp0; c0 < rp1 && s < 0; c0 += WAVE) {
+            int ci = c0 + lane;
+            int2 cell = ci < rp1 ? a.m.row_cells[ci] : make_int2(0, 0);
+            long long incl = wave_inclusive_scan((long long)cell.y) + base_sum;
+            unsigned long long hit = ballot(ci < rp1 && target < incl);
+            if (hit) s = __shfl(cell.x, __ffsll((long long)hit) - 1, WAVE);
+            else base_sum = __shfl(incl, WAVE - 1, WAVE);
+        }
+        const int cp0 = a.m.col_begin(t), cp1 = a.m.col_end(t);
+        for (int c0 = cp0; c0 < cp1 && s < 0; c0 += WAVE) {
+            int ci = c0 + lane;
+            int2 cell = ci < cp1 ? a.m.col_cells[ci] : make_int2(0, 0);
+            bool ok = ci < cp1 && cell.x != t;
+            long long incl = wave_inclusive_scan(ok ? (long long)cell.y : 0ll) + base_sum;
+            unsigned long long hit = ballot(ok && target < incl);
+            if (hit) s = __shfl(cell.x, __ffsll((long long)hit) - 1, WAVE);
+            else base_sum = __shfl(incl, WAVE - 1, WAVE);
+        }
+        bytes += 8ull * (unsigned long long)((rp1 - rp0) + (cp1 - cp0));
+        if (s < 0) {  // d_tot[t] disagrees with the cells of row/column t: report, propose no move
+            if (lane == 0) atomicExch(a.err, 1);
+            s = m.r;
+        }
+    }
+    m.s = s;
+    bytes += 8ull * (unsigned long long)(ko + kia) + 24;
+    return s;
+}
+
+// Builds the neighbour-block histogram and returns delta-MDL; sets *accept.  Table stays populated.
+__device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept,
+                                unsigned long long &bytes) {
+    const int lane = lane_id();
+    const int v = m.v, r = m.r, s = m.s;
+    for (int i = lane; i < t.cap; i += WAVE) {
+        t.keys[i] = EMPTY;
+        t.outc[i] = 0;
+        t.inc[i] = 0;
+    }
+    wave_sync();
+    for (int64_t e = m.o0 + lane; e < m.o1; e += WAVE) {
+        int u = a.out_adj[e];
+        if (u != v) table_add(t, a.assign[u], 1, 0);
+    }
+    for (int64_t e = m.i0 + lane; e < m.i1; e += WAVE) {
+        int u = a.in_adj[e];
+        if (u != v) table_add(t, a.assign[u], 0, 1);
+    }
+    wave_sync();
+    table_get(t, r, m.out_r, m.in_r);
+    table_get(t, s, m.out_s, m.in_s);
+    Fx acc = fx_zero();
+    int nb = 0;
+    for (int i = lane; i < t.cap; i += WAVE) {
+        int c = t.keys[i];
+        if (c == EMPTY) continue;
+        nb++;
+        int oc = t.outc[i], ic = t.inc[i];
+        if (oc > 0) {
+            delta_cell_term(a, m, r, c, delta_row_r(m, c, oc), acc);
+            delta_cell_term(a, m, s, c, delta_row_s(m, c, oc), acc);
+        }
+        if (ic > 0 && c != r && c != s) {
+            delta_cell_term(a, m, c, r, -(long long)ic, acc);
+            delta_cell_term(a, m, c, s, (long long)ic, acc);
+        }
+    }
+    if (lane == 0) {
+        if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0)) delta_cell_term(a, m, r, r, -(long long)m.selfc - m.in_r, acc);
+        if (m.out_s == 0 && m.in_r > 0) delta_cell_term(a, m, r, s, (long long)m.in_r, acc);
+        if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) delta_cell_term(a, m, s, s, (long long)m.selfc + m.in_s, acc);
+        if (m.out_r == 0 && m.in_s > 0) delta_cell_term(a, m, s, r, -(long long)m.in_s, acc);
+    }
+    const int rr0 = a.m.row_begin(r), rr1 = a.m.row_end(r), rs0 = a.m.row_begin(s), rs1 = a.m.row_end(s);
+    const int cr0 = a.m.col_begin(r), cr1 = a.m.col_end(r), cs0 = a.m.col_begin(s), cs1 = a.m.col_end(s);
+    for (int i = rr0 + lane; i < rr1; i += WAVE) {
+        int2 cell = a.m.row_cells[i];
+        int oc, ic;
+        table_get(t, cell.x, oc, ic);
+        if (delta_row_r(m, cell.x, oc) != 0) continue;
+        unchanged_term(a, m, r, cell.x, cell.y, acc);
+    }
+    for (int i = rs0 + lane; i < rs1; i += WAVE) {
+        int2 cell = a.m.row_cells[i];
+        int oc, ic;
+        table_get(t, cell.x, oc, ic);
+        if (delta_row_s(m, cell.x, oc) != 0) continue;
+        unchanged_term(a, m, s, cell.x, cell.y, acc);
+    }
+    for (int i = cr0 + lane; i < cr1; i += WAVE) {
+        int2 cell = a.m.col_cells[i];
+        if (cell.x == r || cell.x == s) continue;
+        int oc, ic;
+        table_get(t, cell.x, oc, ic);
+        if (ic != 0) continue;
+        unchanged_term(a, m, cell.x, r, cell.y, acc);
+    }
+    for (int i = cs0 + lane; i < cs1; i += WAVE) {
+        int2 cell = a.m.col_cells[i];
+        if (cell.x == r || cell.x == s) continue;
+        int oc, ic;
+        table_get(t, cell.x, oc, ic);
+        if (ic != 0) continue;
+        unchanged_term(a, m, cell.x, s, cell.y, acc);
+    }
+    acc = wave_sum_fx(acc);
+    nb = wave_sum(nb);
+    const double dS = fx_to(acc);
+    const long long rc_r = (rr1 - rr0) + (cr1 - cr0), rc_s = (rs1 - rs0) + (cs1 - cs0);
+    bytes += 12ull * (unsigned long long)(rc_r + rc_s) + 16ull * (unsigned long long)nb;
+    if (a.greedy) {
+        accept = dS < 0.0;
+        return dS;
+    }
+    // Hastings correction over the neighbour blocks (entropy.cpp:497-561)
+    Fx fwd = fx_zero(), bwd = fx_zero();
+    const long long Bn = a.B;
+    for (int i = lane; i < t.cap + 1; i += WAVE) {
+        int x, cntx;
+        if (i < t.cap) {
+            x = t.keys[i];
+            if (x == EMPTY) continue;
+            cntx = t.outc[i] + t.inc[i] + (x == r ? m.selfc : 0);
+        } else {
+            if (!(m.selfc > 0 && m.out_r == 0 && m.in_r == 0)) continue;  // self-loop is the only link to r
+            x = r;
+            cntx = m.selfc;
+        }
+        int oc, ic;
+        table_get(t, x, oc, ic);
+        double counts = (double)cntx;
+        double pw = (double)(get_cell(a, s, x) + get_cell(a, x, s)) + 1.0;
+        double bd = (double)((long long)a.d_tot[x] + Bn);
+        long long d_rx = delta_row_r(m, x, oc);
+        long long d_xr = (x == r) ? delta_row_r(m, r, m.out_r) : (x == s ? delta_row_s(m, r, m.out_r) : -(long long)ic);
+        double bw = (double)(get_cell(a, r, x) + d_rx + get_cell(a, x, r) + d_xr) + 1.0;
+        long long nbd = a.d_tot[x];
+        if (x == r) {
+            long long self = get_cell(a, r, r) + delta_row_r(m, r, m.out_r);
+            nbd = (a.d_out[r] - m.k_out) + (a.d_in[r] - m.k_in) - self;
+        } else if (x == s) {
+            long long self = get_cell(a, s, s) + delta_row_s(m, s, m.out_s);
+            nbd = (a.d_out[s] + m.k_out) + (a.d_in[s] + m.k_in) - self;
+        }
+        double pd = (double)(nbd + Bn);
+        fx_acc(fwd, counts * pw / bd);
+        fx_acc(bwd, counts * bw / pd);
+    }
+    fwd = wave_sum_fx(fwd);
+    bwd = wave_sum_fx(bwd);
+    double H = (m.k_out + m.k_in == 0) ? 1.0 : fx_to(bwd) / fx_to(fwd);
+    double u = unit_double(w.z, w.w);
+    double p = exp(-3.0 * dS) * H;
+    p = p >= 1.0 ? 1.0 : p;
+    accept = u <= p;
+    bytes += 8ull * (unsigned long long)rc_s + 8ull * (unsigned long long)nb;
+    return dS;
+}
+
+template <bool BIG>
+__global__ void __launch_bounds__(FT_BLOCK) k_ft_propose(FtArgs a) {
+    __shared__ int lds_keys[BIG ? 1 : FT_WAVES * LDS_CAP];
+    __shared__ int lds_out[BIG ? 1 : FT_WAVES * LDS_CAP];
+    __shared__ int lds_in[BIG ? 1 : FT_WAVES * LDS_CAP];
+    const int lane = lane_id(), wid = threadIdx.x / WAVE;
+    const int64_t gwave = (int64_t)blockIdx.x * FT_WAVES + wid;
+    const int64_t nwaves = (int64_t)gridDim.x * FT_WAVES;
+    const Table t = make_table(a, BIG, lds_keys + (BIG ? 0 : wid * LDS_CAP), lds_out + (BIG ? 0 : wid * LDS_CAP),
+                               lds_in + (BIG ? 0 : wid * LDS_CAP), gwave);
+    unsigned long long bytes = 0;
+    for (int64_t item = gwave; item < a.count; item += nwaves) {
+        const int v = a.items[item];
+        const long long kall = (a.out_off[v + 1] - a.out_off[v]) + (a.in_off[v + 1] - a.in_off[v]);
+        if ((kall > SMALL_K) != BIG) continue;
+        MoveCtx m;
+        U4 w;
+        const int s = propose_block(a, v, m, w, bytes);
+        if (a.s_out && lane == 0) a.s_out[v] = s;
+        if (s == m.r) {
+            if (lane == 0) {
+                a.move_s[v] = -1;
+                if (a.dS_out) a.dS_out[v] = 0.0;
+            }
+            continue;
+        }
+        bool accept;
+        const double dS = evaluate_move(a, m, t, w, accept, bytes);
+        if (lane == 0) {
+            a.move_s[v] = accept ? s : -1;
+            if (a.dS_out) a.dS_out[v] = dS;
+        }
+        wave_sync();
+    }
+    if (lane == 0 && bytes) atomicAdd(a.bytes, bytes);
+}
+
+static int log2_ceil(int64_t x) {
+    int b = 0;
+    while ((int64_t(1) << b) < x) ++b;
+    return b;
+}
+
+static FtArgs base_args(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int greedy) {
+    FtArgs a;
+    a.N = g->N;
+    a.B = bm->B;
+    a.seed = ctx->seed;
+    a.epoch = epoch;
+    a.greedy = greedy;
+    a.out_off = g->out_off;
+    a.in_off = g->in_off;
+    a.out_adj = g->out_adj;
+    a.in_adj = g->in_adj;
+    a.assign = bm->assign;
+    a.sizes = bm->sizes;
+    a.d_out = bm->d_out;
+    a.d_in = bm->d_in;
+    a.d_tot = bm->d_tot;
+    a.m.rbeg = bm->row_ptr;
+    a.m.rlen = nullptr;
+    a.m.cbeg = bm->col_ptr;
+    a.m.clen = nullptr;
+    a.m.row_cells = bm->row_cells;
+    a.m.col_cells = bm->col_cells;
+    a.items = nullptr;
+    a.count = 0;
+    a.move_s = nullptr;
+    a.dS_out = nullptr;
+    a.s_out = nullptr;
+    a.bytes = nullptr;
+    a.gtable = nullptr;
+    a.gcap_log2 = 0;
+    a.err = nullptr;
+    return a;
+}
+
+FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int greedy, const int32_t *d_vertices,
+                        int64_t count, int32_t *d_move_s, double *d_dS, int32_t *d_s_out) {
+    hipStream_t st = ctx->stream;
+    FtLaunchInfo info{0, count, 0.f};
+    if (count == 0) return info;
+    auto *bytes = ctx->scratch[S_COUNTER].as<unsigned long long>(4);
+    SBP_HIP(hipMemsetAsync(bytes, 0, 4 * sizeof(unsigned long long), st));
+    FtArgs a = base_args(ctx, bm, g, epoch, greedy);
+    a.err = reinterpret_cast<int *>(bytes + 1);
+    a.items = d_vertices;
+    a.count = count;
+    a.move_s = d_move_s;
+    a.dS_out = d_dS;
+    a.s_out = d_s_out;
+    a.bytes = bytes;
+    int64_t blocks = (count + FT_WAVES - 1) / FT_WAVES;
+    if (blocks > 8192) blocks = 8192;
+    SBP_HIP(hipEventRecord(ctx->ev[0], st));
+    k_ft_propose<false><<<(int)blocks, FT_BLOCK, 0, st>>>(a);
+    SBP_HIP(hipGetLastError());
+    if (g->max_degree > SMALL_K) {
+        int lg = log2_ceil(2 * g->max_degree + 2);
+        if (lg < 11) lg = 11;
+        const int64_t big_blocks = 64;
+        a.gtable = ctx->scratch[S_HASH].as<int>((size_t)big_blocks * FT_WAVES * 3 * ((size_t)1 << lg));
+        a.gcap_log2 = lg;
+        k_ft_propose<true><<<(int)big_blocks, FT_BLOCK, 0, st>>>(a);
+        SBP_HIP(hipGetLastError());
+    }
+    SBP_HIP(hipEventRecord(ctx->ev[1], st));
+    unsigned long long hb[2] = {0, 0};
+    SBP_HIP(hipMemcpyAsync(hb, bytes, sizeof(hb), hipMemcpyDeviceToHost, st));
+    SBP_HIP(hipStreamSynchronize(st));
+    SBP_CHECK(hb[1] == 0, SBP_E_STATE, "finetune proposal: block degrees inconsistent with the block matrix");
+    info.bytes = hb[0];
+    info.kernel_ms = (float)elapsed_ms(ctx, ctx->ev[0], ctx->ev[1]);
+    return info;
+}
+
+// ------------------------------------------------------------------ async-Gibbs commit
+
+__global__ void k_count_out(int32_t N, const int32_t *__restrict__ move_s, const int32_t *__restrict__ assign,
+                            int32_t *__restrict__ out_count) {
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+        if (move_s[v] >= 0) atomicAdd(&out_count[assign[v]], 1);
+}
+
+__global__ void k_mark_risk(int32_t B, const int32_t *__restrict__ out_count, const int32_t *__restrict__ sizes,
+                            int32_t *__restrict__ risk, int32_t *__restrict__ any,
+                            unsigned long long *__restrict__ proposed) {
+    unsigned long long local = 0;
+    for (int32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x) {
+        int x = out_count[b] > 0 && out_count[b] >= sizes[b];
+        risk[b] = x;
+        local += (unsigned long long)out_count[b];
+        if (x) atomicOr(any, 1);
+    }
+    local = wave_sum(local);
+    if (lane_id() == 0 && local) atomicAdd(proposed, local);
+}
+
+// Events of moves touching at-risk blocks, keyed (block << 32 | vertex).
+__global__ void k_make_events(int32_t N, const int32_t *__restrict__ move_s, const int32_t *__restrict__ assign,
+                              const int32_t *__restrict__ risk, unsigned long long *__restrict__ events,
+                              unsigned long long *__restrict__ n_events, int32_t *__restrict__ acc_a,
+                              int32_t *__restrict__ acc_b) {
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        int s = move_s[v];
+        acc_a[v] = 1;
+        acc_b[v] = 1;
+        if (s < 0) continue;
+        int r = assign[v];
+        if (risk[r]) events[atomicAdd(n_events, 1ull)] = ((unsigned long long)(unsigned)r << 32) | (unsigned)v;
+        if (risk[s]) events[atomicAdd(n_events, 1ull)] = ((unsigned long long)(unsigned)s << 32) | (unsigned)v;
+    }
+}
+
+__global__ void k_segments(int64_t n, const unsigned long long *__restrict__ ev, int32_t *__restrict__ seg_start,
+                           int32_t *__restrict__ seg_end) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int b = (int)(ev[i] >> 32);
+        if (i == 0 || (int)(ev[i - 1] >> 32) != b) seg_start[b] = (int)i;
+        if (i == n - 1 || (int)(ev[i + 1] >> 32) != b) seg_end[b] = (int)i + 1;
+    }
+}
+
+// One thread per at-risk block walks its events in vertex order with the block's running size.
+__global__ void k_resolve(int32_t B, const int32_t *__restrict__ risk, const int32_t *__restrict__ seg_start,
+                          const int32_t *__restrict__ seg_end, const unsigned long long *__restrict__ ev,
+                          const int32_t *__restrict__ assign, const int32_t *__restrict__ sizes,
+                          const int32_t *__restrict__ acc_prev, int32_t *__restrict__ acc_next,
+                          int32_t *__restrict__ changed) {
+    for (int32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x) {
+        if (!risk[b]) continue;
+        int size = sizes[b];
+        for (int i = seg_start[b]; i < seg_end[b]; ++i) {
+            int v = (int)(ev[i] & 0xffffffffull);
+            if (assign[v] == b) {  // move out of b
+                int ok = size > 1;
+                if (ok) size--;
+                acc_next[v] = ok;
+                if (ok != acc_prev[v]) *changed = 1;
+            } else if (acc_prev[v]) {  // move into b
+                size++;
+            }
+        }
+    }
+}
+
+__global__ void k_apply(int32_t N, const int32_t *__restrict__ move_s, const int32_t *__restrict__ acc,
+                        int32_t *__restrict__ assign, unsigned long long *__restrict__ moved) {
+    unsigned long long local = 0;
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        int s = move_s[v];
+        if (s >= 0 && (acc == nullptr || acc[v])) {
+            assign[v] = s;
+            local++;
+        }
+    }
+    local = wave_sum(local);
+    if (lane_id() == 0 && local) atomicAdd(moved, local);
+}
+
+static int grid_of(int64_t n) {
+    int64_t gsz = (n + 255) / 256;
+    if (gsz < 1) gsz = 1;
+    if (gsz > 16384) gsz = 16384;
+    return (int)gsz;
+}
+
+int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_t *refused_out) {
+    hipStream_t st = ctx->stream;
+    const int32_t N = bm->N, B = bm->B;
+    int32_t *out_count = ctx->scratch[S_RISK].as<int32_t>(2 * (size_t)B + 8);
+    int32_t *risk = out_count + B;
+    int32_t *flags = ctx->scratch[S_FLAGS].as<int32_t>(16);
+    unsigned long long *counters = reinterpret_cast<unsigned long long *>(flags + 4);  // [0] moved [1] events [2] proposed
+    SBP_HIP(hipMemsetAsync(out_count, 0, sizeof(int32_t) * B, st));
+    SBP_HIP(hipMemsetAsync(flags, 0, 16 * sizeof(int32_t), st));
+    k_count_out<<<grid_of(N), 256, 0, st>>>(N, d_move_s, bm->assign, out_count);
+    k_mark_risk<<<grid_of(B), 256, 0, st>>>(B, out_count, bm->sizes, risk, flags, counters + 2);
+    int32_t any = 0;
+    SBP_HIP(hipMemcpyAsync(&any, flags, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    SBP_HIP(hipStreamSynchronize(st));
+    const int32_t *acc_final = nullptr;
+    if (any) {
+        auto *events = ctx->scratch[S_EVENTS_A].as<unsigned long long>(2 * (size_t)N + 2);
+        auto *events_sorted = ctx->scratch[S_EVENTS_B].as<unsigned long long>(2 * (size_t)N + 2);
+        int32_t *acc = ctx->scratch[S_ACCEPT].as<int32_t>(2 * (size_t)N + 2);
+        int32_t *seg = ctx->scratch[S_MISC].as<int32_t>(2 * (size_t)B + 2);
+        int32_t *seg_start = seg, *seg_end = seg + B;
+        SBP_HIP(hipMemsetAsync(seg, 0, sizeof(int32_t) * 2 * (size_t)B, st));
+        k_make_events<<<grid_of(N), 256, 0, st>>>(N, d_move_s, bm->assign, risk, events, counters + 1, acc, acc + N);
+        unsigned long long n_ev = 0;
+        SBP_HIP(hipMemcpyAsync(&n_ev, counters + 1, sizeof(n_ev), hipMemcpyDeviceToHost, st));
+        SBP_HIP(hipStreamSynchronize(st));
+        size_t tmp = 0;
+        SBP_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, events, events_sorted, (int)n_ev, 0, 64, st));
+        void *tp = ctx->scratch[S_SORT_TMP].get(tmp);
+        SBP_HIP(hipcub::DeviceRadixSort::SortKeys(tp, tmp, events, events_sorted, (int)n_ev, 0, 64, st));
+        k_segments<<<grid_of((int64_t)n_ev), 256, 0, st>>>((int64_t)n_ev, events_sorted, seg_start, seg_end);
+        int32_t *prev = acc, *next = acc + N;
+        // monotone iteration; every round fixes at least the earliest undecided move, so it terminates
+        for (int64_t it = 0; it <= (int64_t)n_ev + 1; ++it) {
+            SBP_HIP(hipMemsetAsync(flags + 1, 0, sizeof(int32_t), st));
+            SBP_HIP(hipMemcpyAsync(next, prev, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, st));
+            k_resolve<<<grid_of(B), 256, 0, st>>>(B, risk, seg_start, seg_end, events_sorted, bm->assign, bm->sizes,
+                                                   prev, next, flags + 1);
+            int32_t changed = 0;
+            SBP_HIP(hipMemcpyAsync(&changed, flags + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            SBP_HIP(hipStreamSynchronize(st));
+            std::swap(prev, next);
+            if (!changed) break;
+        }
+        acc_final = prev;
+    }
+    k_apply<<<grid_of(N), 256, 0, st>>>(N, d_move_s, acc_final, bm->assign, counters);
+    SBP_HIP(hipGetLastError());
+    unsigned long long c[2] = {0, 0};
+    SBP_HIP(hipMemcpyAsync(&c[0], counters, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    SBP_HIP(hipMemcpyAsync(&c[1], counters + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    SBP_HIP(hipStreamSynchronize(st));
+    if (refused_out) *refused_out = (int64_t)c[1] - (int64_t)c[0];
+    bm_rebuild(ctx, g, bm);
+    return (int64_t)c[0];
+}
+
+// ------------------------------------------------------------------ shuffle
+
+__global__ void k_shuffle_keys(int64_t n, const int32_t *__restrict__ items, uint64_t seed, uint32_t epoch,
+                               unsigned long long *__restrict__ keys) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int v = items[i];
+        U4 w = philox(seed, (uint32_t)v, epoch, TAG_SHUFFLE, 0);
+        keys[i] = ((unsigned long long)w.x << 32) | (unsigned)v;
+    }
+}
+
+__global__ void k_low_word(int64_t n, const unsigned long long *__restrict__ keys, int32_t *__restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (int32_t)(keys[i] & 0xffffffffull);
+}
+
+// Ascending Philox sort key, ties by vertex id.
+void shuffle_keys(sbp_ctx ctx, const int32_t *d_items, int64_t n, uint32_t epoch, int32_t *d_out) {
+    if (n == 0) return;
+    hipStream_t st = ctx->stream;
+    auto *ka = ctx->scratch[S_KEYS_A].as<unsigned long long>(n);
+    auto *kb = ctx->scratch[S_KEYS_B].as<unsigned long long>(n);
+    k_shuffle_keys<<<grid_of(n), 256, 0, st>>>(n, d_items, ctx->seed, epoch, ka);
+    size_t tmp = 0;
+    SBP_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, ka, kb, (int)n, 0, 64, st));
+    void *tp = ctx->scratch[S_SORT_TMP].get(tmp);
+    SBP_HIP(hipcub::DeviceRadixSort::SortKeys(tp, tmp, ka, kb, (int)n, 0, 64, st));
+    k_low_word<<<grid_of(n), 256, 0, st>>>(n, kb, d_out);
+    SBP_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ Metropolis-Hastings
+
+struct MhState {
+    int32_t *rstart, *rlen, *rcap, *cstart, *clen, *ccap;  // [B]
+    int32_t *stop_at;   // first order index not processed (== count when done)
+    int32_t *status;    // 0 done, 1 out of slack
+    unsigned long long *moves;
+    Fx *sum;            // fixed-point sum of accepted delta-MDL
+};
+
+// Inserts/updates/erases one cell in a sorted slack row: cells[beg, beg+len), capacity cap.
+__device__ void slack_add(int2 *cells, int beg, int32_t *len_p, int key, int delta) {
+    const int lane = lane_id();
+    const int len = *len_p;
+    int lo = beg, hi = beg + len;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cells[mid].x < key) lo = mid + 1;
+        else hi = mid;
+    }
+    const int pos = lo;
+    const bool found = pos < beg + len && cells[pos].x == key;
+    if (found) {
+        const int nv = cells[pos].y + delta;
+        wave_sync();
+        if (nv != 0) {
+            if (lane == 0) cells[pos].y = nv;
+        } else {
+            // shift left by one: ascending chunks read before they write
+            for (int i0 = pos; i0 < beg + len - 1; i0 += WAVE) {
+                int i = i0 + lane;
+                int2 x = (i < beg + len - 1) ? cells[i + 1] : make_int2(0, 0);
+                wave_sync();
+                if (i < beg + len - 1) cells[i] = x;
+                wave_sync();
+            }
+            if (lane == 0) *len_p = len - 1;
+        }
+    } else if (delta != 0) {
+        // shift right by one: descending chunks
+        const int end = beg + len;
+        for (int hi_i = end; hi_i > pos; hi_i -= WAVE) {
+            int i = hi_i - 1 - lane;  // destination i+1 <- source i
+            int2 x = (i >= pos) ? cells[i] : make_int2(0, 0);
+            wave_sync();
+            if (i >= pos) cells[i + 1] = x;
+            wave_sync();
+        }
+        if (lane == 0) {
+            cells[pos] = make_int2(key, delta);
+            *len_p = len + 1;
+        }
+    }
+    wave_sync();
+}
+
+__device__ __forceinline__ void apply_cell(const FtArgs &a, const MhState &h, int row, int col, int d) {
+    slack_add(a.m.row_cells, h.rstart[row], &h.rlen[row], col, d);
+    slack_add(a.m.col_cells, h.cstart[col], &h.clen[col], row, d);
+}
+
+// Would adding cell (row, col) need a new slot that row or column lacks?
+__device__ __forceinline__ bool lacks_room(const FtArgs &a, const MhState &h, int row, int col, long long d) {
+    if (d <= 0) return false;
+    if (get_cell(a, row, col) != 0) return false;
+    return h.rlen[row] >= h.rcap[row] || h.clen[col] >= h.ccap[col];
+}
+
+__global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start) {
+    __shared__ int lds_keys[LDS_CAP];
+    __shared__ int lds_out[LDS_CAP];
+    __shared__ int lds_in[LDS_CAP];
+    const int lane = lane_id();
+    unsigned long long bytes = 0, moves = 0;
+    Fx sum = fx_zero();
+    int32_t status = 0, stop = (int32_t)a.count;
+    for (int64_t item = start; item < a.count; ++item) {
+        const int v = a.items[item];
+        const int r0 = a.assign[v];
+        if (a.sizes[r0] == 1) continue;  // propose_move skips singleton blocks (src/finetune.cpp:845-847)
+        const long long kall = (a.out_off[v + 1] - a.out_off[v]) + (a.in_off[v + 1] - a.in_off[v]);
+        const bool big = kall > SMALL_K;
+        const Table t = make_table(a, big, lds_keys, lds_out, lds_in, 0);
+        MoveCtx m;
+        U4 w;
+        const int s = propose_block(a, v, m, w, bytes);
+        if (s == m.r) continue;
+        bool accept;
+        const double dS = evaluate_move(a, m, t, w, accept, bytes);
+        if (!accept) continue;
+        const int r = m.r;
+        // ---- room check for every Delta cell that would create a new entry
+        bool short_room = false;
+        for (int i = lane; i < t.cap; i += WAVE) {
+            int c = t.keys[i];
+            if (c == EMPTY) continue;
+            int oc = t.outc[i], ic = t.inc[i];
+            if (oc > 0) short_room |= lacks_room(a, h, s, c, delta_row_s(m, c, oc));
+            if (ic > 0 && c != r && c != s) short_room |= lacks_room(a, h, c, s, ic);
+        }
+        if (lane == 0) {
+            if (m.out_s == 0 && m.in_r > 0) short_room |= lacks_room(a, h, r, s, m.in_r);
+            if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) short_room |= lacks_room(a, h, s, s, m.selfc + m.in_s);
+        }
+        // row s and column s may receive several new cells: check their totals
+        int new_row_s = 0, new_col_s = 0;
+        for (int i = lane; i < t.cap; i += WAVE) {
+            int c = t.keys[i];
+            if (c == EMPTY) continue;
+            int oc = t.outc[i], ic = t.inc[i];
+            if (oc > 0 && delta_row_s(m, c, oc) > 0 && get_cell(a, s, c) == 0) new_row_s++;
+            if (ic > 0 && c != r && c != s && get_cell(a, c, s) == 0) new_col_s++;
+        }
+        new_row_s = wave_sum(new_row_s) + 2;
+        new_col_s = wave_sum(new_col_s) + 2;
+        short_room |= (h.rlen[s] + new_row_s > h.rcap[s]) || (h.clen[s] + new_col_s > h.ccap[s]);
+        if (ballot(short_room)) {
+            status = 1;
+            stop = (int32_t)item;
+            break;
+        }
+        // ---- apply the Delta in place (update_edge_counts(delta), dict_transpose_matrix.cpp:430-446)
+        for (int i = 0; i < t.cap; ++i) {  // sequential over slots: each cell update is wave-cooperative
+            int c = t.keys[i];
+            if (c == EMPTY) continue;
+            int oc = t.outc[i], ic = t.inc[i];
+            if (oc > 0) {
+                long long dr = delta_row_r(m, c, oc), ds = delta_row_s(m, c, oc);
+                if (dr) apply_cell(a, h, r, c, (int)dr);
+                if (ds) apply_cell(a, h, s, c, (int)ds);
+            }
+            if (ic > 0 && c != r && c != s) {
+                apply_cell(a, h, c, r, -ic);
+                apply_cell(a, h, c, s, ic);
+            }
+        }
+        if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0)) apply_cell(a, h, r, r, -m.selfc - m.in_r);
+        if (m.out_s == 0 && m.in_r > 0) apply_cell(a, h, r, s, m.in_r);
+        if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) apply_cell(a, h, s, s, m.selfc + m.in_s);
+        if (m.out_r == 0 && m.in_s > 0) apply_cell(a, h, s, r, -m.in_s);
+        wave_sync();
+        if (lane == 0) {
+            a.d_out[r] -= (int)m.k_out;
+            a.d_out[s] += (int)m.k_out;
+            a.d_in[r] -= (int)m.shift_in;
+            a.d_in[s] += (int)m.shift_in;
+            a.sizes[r] -= 1;
+            a.sizes[s] += 1;
+            a.assign[v] = s;
+        }
+        wave_sync();
+        if (lane == 0) {
+            a.d_tot[r] = a.d_out[r] + a.d_in[r] - (int)get_cell(a, r, r);
+            a.d_tot[s] = a.d_out[s] + a.d_in[s] - (int)get_cell(a, s, s);
+        }
+        wave_sync();
+        fx_add(sum, fx_from(dS));
+        moves++;
+        bytes += 12ull * (unsigned long long)kall;
+    }
+    if (lane == 0) {
+        *h.status = status;
+        *h.stop_at = stop;
+        atomicAdd(h.moves, moves);
+        Fx old = *h.sum;
+        fx_add(old, sum);
+        *h.sum = old;
+        if (bytes) atomicAdd(a.bytes, bytes);
+    }
+}
+
+// Slack layout: capacity = len + max(8, len / 2) (times `grow`), for rows and columns.
+__global__ void k_slack_caps(int32_t B, const int32_t *__restrict__ ptr, int32_t *__restrict__ len,
+                             int32_t *__restrict__ cap, int grow) {
+    for (int32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x) {
+        int l = ptr[b + 1] - ptr[b];
+        len[b] = l;
+        int extra = l / 2 > 8 ? l / 2 : 8;
+        cap[b] = l + extra * grow;
+    }
+}
+
+__global__ void k_slack_copy(int32_t B, const int32_t *__restrict__ ptr, const int2 *__restrict__ src,
+                             const int32_t *__restrict__ start, int2 *__restrict__ dst) {
+    for (int32_t b = blockIdx.x; b < B; b += gridDim.x)
+        for (int i = ptr[b] + threadIdx.x; i < ptr[b + 1]; i += blockDim.x) dst[start[b] + (i - ptr[b])] = src[i];
+}
+
+// Re-lays slack rows back out (used when growing): copies [start, start+len) to a compact CSR.
+__global__ void k_compact_copy(int32_t B, const int32_t *__restrict__ start, const int32_t *__restrict__ len,
+                               const int2 *__restrict__ src, const int32_t *__restrict__ ptr, int2 *__restrict__ dst) {
+    for (int32_t b = blockIdx.x; b < B; b += gridDim.x)
+        for (int i = threadIdx.x; i < len[b]; i += blockDim.x) dst[ptr[b] + i] = src[start[b] + i];
+}
+
+void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t epoch, const int32_t *d_order,
+            int64_t count, double *sum_dS, sbp_stats *stats) {
+    hipStream_t st = ctx->stream;
+    if (count == 0) {
+        if (sum_dS) *sum_dS = 0.0;
+        return;
+    }
+    const int32_t B = bm->B;
+    // per-block arrays: rstart rlen rcap cstart clen ccap + bookkeeping
+    int32_t *arr = ctx->scratch[S_EVENTS_A].as<int32_t>(6 * ((size_t)B + 1) + 64);
+    MhState h;
+    h.rstart = arr;
+    h.rlen = arr + (B + 1);
+    h.rcap = arr + 2 * (B + 1);
+    h.cstart = arr + 3 * (B + 1);
+    h.clen = arr + 4 * (B + 1);
+    h.ccap = arr + 5 * (B + 1);
+    int32_t *book = arr + 6 * (B + 1);
+    h.stop_at = book;
+    h.status = book + 1;
+    h.moves = reinterpret_cast<unsigned long long *>(book + 4);
+    h.sum = reinterpret_cast<Fx *>(book + 8);
+    auto *bytes = ctx->scratch[S_COUNTER].as<unsigned long long>(4);
+    SBP_HIP(hipMemsetAsync(book, 0, 64 * sizeof(int32_t), st));
+    SBP_HIP(hipMemsetAsync(bytes, 0, 4 * sizeof(unsigned long long), st));
+    int grow = 1;
+    int32_t start = 0;
+    double t0 = 0;
+    float kernel_ms = 0.f;
+    (void)t0;
+    while (true) {
+        // lay out rows and columns with slack
+        k_slack_caps<<<grid_of(B), 256, 0, st>>>(B, bm->row_ptr, h.rlen, h.rcap, grow);
+        k_slack_caps<<<grid_of(B), 256, 0, st>>>(B, bm->col_ptr, h.clen, h.ccap, grow);
+        size_t tmp = 0;
+        SBP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, h.rcap, h.rstart, B + 1, st));
+        void *tp = ctx->scratch[S_SORT_TMP].get(tmp);
+        // capacities have B entries; the (B+1)-th scan input is the zeroed slot past the end
+        SBP_HIP(hipMemsetAsync(h.rcap + B, 0, sizeof(int32_t), st));
+        SBP_HIP(hipMemsetAsync(h.ccap + B, 0, sizeof(int32_t), st));
+        SBP_HIP(hipcub::DeviceScan::ExclusiveSum(tp, tmp, h.rcap, h.rstart, B + 1, st));
+        SBP_HIP(hipcub::DeviceScan::ExclusiveSum(tp, tmp, h.ccap, h.cstart, B + 1, st));
+        int32_t tot[2] = {0, 0};
+        SBP_HIP(hipMemcpyAsync(&tot[0], h.rstart + B, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        SBP_HIP(hipMemcpyAsync(&tot[1], h.cstart + B, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        SBP_HIP(hipStreamSynchronize(st));
+        int2 *rc = ctx->scratch[S_KEYS_A].as<int2>((size_t)tot[0] + 1);
+        int2 *cc = ctx->scratch[S_KEYS_B].as<int2>((size_t)tot[1] + 1);
+        k_slack_copy<<<grid_of((int64_t)B * 64), 64, 0, st>>>(B, bm->row_ptr, bm->row_cells, h.rstart, rc);
+        k_slack_copy<<<grid_of((int64_t)B * 64), 64, 0, st>>>(B, bm->col_ptr, bm->col_cells, h.cstart, cc);
+        FtArgs a = base_args(ctx, bm, g, epoch, p.greedy);
+        a.m.rbeg = h.rstart;
+        a.m.rlen = h.rlen;
+        a.m.cbeg = h.cstart;
+        a.m.clen = h.clen;
+        a.m.row_cells = rc;
+        a.m.col_cells = cc;
+        a.items = d_order;
+        a.count = count;
+        a.bytes = bytes;
+        a.err = reinterpret_cast<int *>(bytes + 1);
+        int lg = log2_ceil(2 * g->max_degree + 2);
+        if (lg < 11) lg = 11;
+        a.gtable = ctx->scratch[S_HASH].as<int>(3 * ((size_t)1 << lg));
+        a.gcap_log2 = lg;
+        SBP_HIP(hipEventRecord(ctx->ev[4], st));
+        k_mh<<<1, WAVE, 0, st>>>(a, h, start);
+        SBP_HIP(hipGetLastError());
+        SBP_HIP(hipEventRecord(ctx->ev[5], st));
+        // compact the slack layout back into the blockmodel's CSR (row and column counts changed)
+        size_t tmp2 = 0;
+        int32_t *rp = bm->row_ptr, *cp = bm->col_ptr;
+        SBP_HIP(hipMemsetAsync(h.rlen + B, 0, sizeof(int32_t), st));
+        SBP_HIP(hipMemsetAsync(h.clen + B, 0, sizeof(int32_t), st));
+        SBP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, h.rlen, rp, B + 1, st));
+        tp = ctx->scratch[S_SORT_TMP].get(tmp2);
+        SBP_HIP(hipcub::DeviceScan::ExclusiveSum(tp, tmp2, h.rlen, rp, B + 1, st));
+        SBP_HIP(hipcub::DeviceScan::ExclusiveSum(tp, tmp2, h.clen, cp, B + 1, st));
+        int32_t nnz[2] = {0, 0}, hs[2] = {0, 0};
+        SBP_HIP(hipMemcpyAsync(&nnz[0], rp + B, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        SBP_HIP(hipMemcpyAsync(&nnz[1], cp + B, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        SBP_HIP(hipMemcpyAsync(hs, book, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        SBP_HIP(hipStreamSynchronize(st));
+        kernel_ms += (float)elapsed_ms(ctx, ctx->ev[4], ctx->ev[5]);
+        SBP_CHECK(nnz[0] == nnz[1], SBP_E_STATE, "Metropolis-Hastings: row and column cell counts diverged");
+        if (nnz[0] > bm->cap) {
+            // more cells than the compact arrays hold: grow them
+            (void)hipFree(bm->row_cells);
+            (void)hipFree(bm->col_cells);
+            bm->cap = nnz[0] + nnz[0] / 4 + 64;
+            SBP_HIP(hipMalloc(&bm->row_cells, sizeof(int2) * bm->cap));
+            SBP_HIP(hipMalloc(&bm->col_cells, sizeof(int2) * bm->cap));
+        }
+        k_compact_copy<<<grid_of((int64_t)B * 64), 64, 0, st>>>(B, h.rstart, h.rlen, rc, rp, bm->row_cells);
+        k_compact_copy<<<grid_of((int64_t)B * 64), 64, 0, st>>>(B, h.cstart, h.clen, cc, cp, bm->col_cells);
+        bm->nnz = nnz[0];
+        SBP_HIP(hipStreamSynchronize(st));
+        if (hs[1] == 0) break;
+        start = hs[0];  // resume at the vertex that ran out of room, with more slack
+        grow *= 2;
+        SBP_CHECK(grow <= (1 << 24), SBP_E_STATE, "Metropolis-Hastings: slack growth did not converge");
+        SBP_HIP(hipMemsetAsync(book, 0, 2 * sizeof(int32_t), st));
+    }
+    struct {
+        unsigned long long moves;
+        unsigned long long pad;
+        Fx sum;
+    } res;
+    unsigned long long hb[2] = {0, 0};
+    SBP_HIP(hipMemcpyAsync(&res, book + 4, sizeof(res), hipMemcpyDeviceToHost, st));
+    SBP_HIP(hipMemcpyAsync(hb, bytes, sizeof(hb), hipMemcpyDeviceToHost, st));
+    SBP_HIP(hipStreamSynchronize(st));
+    SBP_CHECK(hb[1] == 0, SBP_E_STATE, "Metropolis-Hastings: block degrees inconsistent with the block matrix");
+    if (sum_dS) *sum_dS = (double)res.sum.hi + (double)res.sum.lo * 0x1p-64;
+    if (stats) {
+        stats->proposals += count;
+        stats->moves += (int64_t)res.moves;
+        stats->ft_bytes += hb[0];
+        stats->ft_kernel_ms += kernel_ms;
+        stats->finetune_ms += kernel_ms;
+        stats->ft_launches += 1;
+    }
+}
+
+}  // namespace sbpimpl

@@ -1,0 +1,134 @@
+// Host-side handle layouts and helpers shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sbp_amd.h"
+
+namespace sbpimpl {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define SBP_HIP(expr)                                                                                     \
+    do {                                                                                                  \
+        hipError_t _e = (expr);                                                                           \
+        if (_e != hipSuccess)                                                                             \
+            throw ::sbpimpl::Error(SBP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e) + " at " + \
+                                                  __FILE__ + ":" + std::to_string(__LINE__));             \
+    } while (0)
+
+#define SBP_CHECK(cond, code, msg)                                  \
+    do {                                                            \
+        if (!(cond)) throw ::sbpimpl::Error((code), std::string(msg)); \
+    } while (0)
+
+// Device buffer that only grows; contents are not preserved across growth.
+struct Scratch {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    void *get(size_t need) {
+        if (need > bytes) {
+            // queued kernels may still read the old buffer
+            if (ptr) {
+                SBP_HIP(hipDeviceSynchronize());
+                SBP_HIP(hipFree(ptr));
+            }
+            ptr = nullptr;
+            size_t n = need + need / 4 + 256;
+            SBP_HIP(hipMalloc(&ptr, n));
+            bytes = n;
+        }
+        return ptr;
+    }
+    template <typename T>
+    T *as(size_t count) {
+        return static_cast<T *>(get(count * sizeof(T) + 16));
+    }
+    ~Scratch() {
+        if (ptr) (void)hipFree(ptr);
+    }
+};
+
+enum ScratchSlot {
+    S_KEYS_A, S_KEYS_B, S_VALS_A, S_VALS_B, S_UNIQ, S_COUNTS, S_SORT_TMP, S_MISC, S_PARTIAL,
+    S_MOVES, S_DS, S_ACCEPT, S_COUNTER, S_HASH, S_RISK, S_EVENTS_A, S_EVENTS_B, S_ORDER, S_BEST,
+    S_ASSIGN_TMP, S_FLAGS, S_NUM_SLOTS
+};
+
+}  // namespace sbpimpl
+
+struct sbp_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t seed = 0;
+    std::string err;
+    sbpimpl::Scratch scratch[sbpimpl::S_NUM_SLOTS];
+    hipEvent_t ev[8] = {};
+    void *pinned = nullptr;  // small pinned staging area for scalars
+};
+
+struct sbp_graph_s {
+    sbp_ctx ctx = nullptr;
+    int32_t N = 0;
+    int64_t E = 0;
+    int64_t *out_off = nullptr, *in_off = nullptr;
+    int32_t *out_adj = nullptr, *in_adj = nullptr;
+    int32_t *edge_src = nullptr;    // source vertex of each out-edge (CSR order)
+    int64_t max_degree = 0;         // max over v of |out(v)| + |in(v)|
+    std::vector<int64_t> h_out_off, h_in_off;  // host copies for launch planning
+    std::vector<int32_t> h_out_adj, h_in_adj;
+};
+
+struct sbp_bm_s {
+    sbp_ctx ctx = nullptr;
+    int32_t B = 0, N = 0;
+    int64_t nnz = 0, cap = 0;
+    int32_t *assign = nullptr;   // [N]
+    int32_t *sizes = nullptr;    // [B]
+    int32_t *d_out = nullptr;    // [B]
+    int32_t *d_in = nullptr;     // [B]
+    int32_t *d_tot = nullptr;    // [B]
+    int32_t *row_ptr = nullptr;  // [B+1]
+    int32_t *col_ptr = nullptr;  // [B+1]
+    int2 *row_cells = nullptr;   // [cap] {col, count}, ascending col per row
+    int2 *col_cells = nullptr;   // [cap] {row, count}, ascending row per column
+    int32_t bcap = 0;            // capacity of the per-block arrays
+    double overall_entropy = 1.7976931348623157e308;
+    int64_t num_to_merge = 0;
+};
+
+namespace sbpimpl {
+
+// build.hip
+void bm_alloc(sbp_bm bm, int32_t B, int32_t N);
+void bm_release(sbp_bm bm);
+void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm);  // from bm->assign (device), bm->B set
+void bm_copy_into(sbp_ctx ctx, sbp_bm src, sbp_bm dst);
+// mdl.hip
+double log_posterior(sbp_ctx ctx, sbp_bm bm);
+double model_term(int64_t E, int32_t N, int32_t B);
+// finetune.hip
+struct FtLaunchInfo {
+    uint64_t bytes;
+    int64_t proposals;
+    float kernel_ms;
+};
+FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int greedy, const int32_t *d_vertices,
+                        int64_t count, int32_t *d_move_s, double *d_dS, int32_t *d_accept);
+int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_t *refused_out);
+void shuffle_keys(sbp_ctx ctx, const int32_t *d_items, int64_t n, uint32_t epoch, int32_t *d_out);
+// merge.hip
+uint64_t merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *d_best_s, double *d_best_dS);
+void relabel_assignment(sbp_ctx ctx, sbp_bm bm, const int32_t *d_block_map, int32_t new_B, int32_t *d_out);
+void debug_logf(sbp_ctx ctx, uint32_t first, uint32_t count, float *d_out);
+
+double elapsed_ms(sbp_ctx ctx, hipEvent_t a, hipEvent_t b);
+
+}  // namespace sbpimpl

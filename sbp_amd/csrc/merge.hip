@@ -1,0 +1,228 @@
+// Block-merge proposals: one wavefront per block, NUM_AGG_PROPOSALS_PER_BLOCK = 10 proposals each.
+//
+// Replaces the OpenMP loop of block_merge::merge_blocks (src/block_merge.cpp:189-217) with
+// propose_merge_sparse (:251-285), merge-mode common::propose_new_block (src/common.cpp:138-198) and
+// entropy::block_merge_delta_mdl (src/entropy.cpp:129-182).  The merge Delta of
+// block_merge::blockmodel_delta (src/block_merge.cpp:14-38) moves all of row r and column r into s; its
+// cells are enumerated straight from the sorted row/column cells of r, with sorted lookups into row s /
+// column s, so no per-proposal map is built.  Repeated proposals of the same s are skipped as
+// past_proposals does.  Terms are summed in 128-bit fixed point.
+#include <float.h>
+
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace sbpimpl {
+using namespace sbpdev;
+
+constexpr int MG_BLOCK = 256;
+constexpr int MG_WAVES = MG_BLOCK / WAVE;
+
+struct MgArgs {
+    int32_t B;
+    uint64_t seed;
+    uint32_t epoch;
+    const int32_t *d_out, *d_in;
+    const int32_t *row_ptr, *col_ptr;
+    const int2 *row_cells, *col_cells;
+    int32_t *best_s;
+    double *best_dS;
+    unsigned long long *bytes;
+    int *err;
+};
+
+__device__ __forceinline__ long long mg_get(const MgArgs &a, int row, int col) {
+    int r0 = a.row_ptr[row], r1 = a.row_ptr[row + 1];
+    int c0 = a.col_ptr[col], c1 = a.col_ptr[col + 1];
+    if (r1 - r0 <= c1 - c0) return cell_lookup(a.row_cells, r0, r1, col);
+    return cell_lookup(a.col_cells, c0, c1, row);
+}
+
+// Wave-cooperative selection: first cell (in list order, skipping excluded ids) whose inclusive prefix of
+// counts exceeds target.  Returns -1 when the list is exhausted; base carries the running prefix.
+__device__ __forceinline__ int wave_select(const int2 *cells, int begin, int end, int ex1, int ex2,
+                                           long long target, long long &base) {
+    const int lane = lane_id();
+    for (int c0 = begin; c0 < end; c0 += WAVE) {
+        int ci = c0 + lane;
+        int2 cell = ci < end ? cells[ci] : make_int2(-1, 0);
+        bool ok = ci < end && cell.x != ex1 && cell.x != ex2;
+        long long incl = wave_inclusive_scan(ok ? (long long)cell.y : 0ll) + base;
+        unsigned long long hit = ballot(ok && target < incl);
+        if (hit) return __shfl(cell.x, __ffsll((long long)hit) - 1, WAVE);
+        base = __shfl(incl, WAVE - 1, WAVE);
+    }
+    return -1;
+}
+
+// delta-MDL of merging block r into block s (entropy::block_merge_delta_mdl, src/entropy.cpp:129-182).
+__device__ double merge_delta_mdl(const MgArgs &a, int r, int s) {
+    const int lane = lane_id();
+    const long long k_out = a.d_out[r], k_in = a.d_in[r];
+    const int rr0 = a.row_ptr[r], rr1 = a.row_ptr[r + 1], rs0 = a.row_ptr[s], rs1 = a.row_ptr[s + 1];
+    const int cr0 = a.col_ptr[r], cr1 = a.col_ptr[r + 1], cs0 = a.col_ptr[s], cs1 = a.col_ptr[s + 1];
+    const long long din_s_new = (long long)a.d_in[s] + k_in, dout_s_new = (long long)a.d_out[s] + k_out;
+    Fx acc = fx_zero();
+    // Delta entries in row r: (r, c) old terms; (s, c) for c not in {r, s}
+    for (int i = rr0 + lane; i < rr1; i += WAVE) {
+        int2 cell = a.row_cells[i];
+        int c = cell.x;
+        fx_acc(acc, cell_entropy(cell.y, a.d_in[c], a.d_out[r]));
+        if (c != r && c != s) {
+            long long msc = cell_lookup(a.row_cells, rs0, rs1, c);
+            fx_acc(acc, cell_entropy(msc, a.d_in[c], a.d_out[s]));
+            fx_sub_acc(acc, cell_entropy(msc + cell.y, a.d_in[c], dout_s_new));
+        }
+    }
+    // Delta entries in column r: (x, r) old terms; (x, s) for x not in {r, s}
+    for (int i = cr0 + lane; i < cr1; i += WAVE) {
+        int2 cell = a.col_cells[i];
+        int x = cell.x;
+        if (x == r) continue;
+        fx_acc(acc, cell_entropy(cell.y, a.d_in[r], a.d_out[x]));
+        if (x != s) {
+            long long mxs = cell_lookup(a.col_cells, cs0, cs1, x);
+            fx_acc(acc, cell_entropy(mxs, a.d_in[s], a.d_out[x]));
+            fx_sub_acc(acc, cell_entropy(mxs + cell.y, din_s_new, a.d_out[x]));
+        }
+    }
+    // (s, s): receives M[r][r] + M[r][s] + M[s][r]
+    const long long mrr = mg_get(a, r, r), mrs = mg_get(a, r, s), msr = mg_get(a, s, r), mss = mg_get(a, s, s);
+    const long long dss = mrr + mrs + msr;
+    if (lane == 0 && dss > 0) {
+        fx_acc(acc, cell_entropy(mss, a.d_in[s], a.d_out[s]));
+        fx_sub_acc(acc, cell_entropy(mss + dss, din_s_new, dout_s_new));
+    }
+    // unchanged cells of row s
+    for (int i = rs0 + lane; i < rs1; i += WAVE) {
+        int2 cell = a.row_cells[i];
+        int c = cell.x;
+        if (c == r) continue;                 // delta (s, r) = -M[s][r] != 0
+        if (c == s && dss > 0) continue;      // (s, s) is a delta entry
+        if (c != s && cell_lookup(a.row_cells, rr0, rr1, c) != 0) continue;  // (s, c) receives M[r][c]
+        long long din_new = (c == s) ? din_s_new : (long long)a.d_in[c];
+        fx_acc(acc, cell_entropy(cell.y, a.d_in[c], a.d_out[s]));
+        fx_sub_acc(acc, cell_entropy(cell.y, din_new, dout_s_new));
+    }
+    // unchanged cells of column s (rows other than r, s)
+    for (int i = cs0 + lane; i < cs1; i += WAVE) {
+        int2 cell = a.col_cells[i];
+        int x = cell.x;
+        if (x == r || x == s) continue;
+        if (cell_lookup(a.col_cells, cr0, cr1, x) != 0) continue;  // (x, s) receives M[x][r]
+        fx_acc(acc, cell_entropy(cell.y, a.d_in[s], a.d_out[x]));
+        fx_sub_acc(acc, cell_entropy(cell.y, din_s_new, a.d_out[x]));
+    }
+    acc = wave_sum_fx(acc);
+    return fx_to(acc);
+}
+
+__global__ void __launch_bounds__(MG_BLOCK) k_merge_propose(MgArgs a) {
+    const int lane = lane_id(), wid = threadIdx.x / WAVE;
+    const int64_t nwaves = (int64_t)gridDim.x * MG_WAVES;
+    unsigned long long bytes = 0;
+    for (int64_t rb = (int64_t)blockIdx.x * MG_WAVES + wid; rb < a.B; rb += nwaves) {
+        const int r = (int)rb;
+        const long long k = (long long)a.d_out[r] + a.d_in[r];
+        const int rr0 = a.row_ptr[r], rr1 = a.row_ptr[r + 1], cr0 = a.col_ptr[r], cr1 = a.col_ptr[r + 1];
+        int past[10];
+        int npast = 0;
+        double best = DBL_MAX;
+        int best_s = -1;
+        for (uint32_t i = 0; i < 10; ++i) {
+            const U4 w = philox(a.seed, (uint32_t)r, a.epoch, TAG_MERGE, i);
+            int s = -1, t = -1;
+            if (k == 0) {
+                s = random_block(r, a.B, w.y);
+            } else {
+                long long target = (long long)uniform_below(w.x, (uint32_t)k), base = 0;
+                t = wave_select(a.row_cells, rr0, rr1, -2, -2, target, base);
+                if (t < 0) t = wave_select(a.col_cells, cr0, cr1, -2, -2, target, base);
+                if (t < 0) {  // degrees disagree with the cells: report and stop proposing for r
+                    if (lane == 0) atomicExch(a.err, 1);
+                    break;
+                }
+                // neighbors_weights(t) with edges[r] = 0 (src/common.cpp:169-178)
+                long long mtr = mg_get(a, t, r), mtt = mg_get(a, t, t), mrt = (r != t) ? mg_get(a, r, t) : 0;
+                long long total = (long long)a.d_out[t] - mtr + (long long)a.d_in[t] - mtt - mrt;
+                if (total == 0) {
+                    s = random_block(r, a.B, w.y);
+                } else {
+                    long long tg = (long long)uniform_below(w.y, (uint32_t)total), b2 = 0;
+                    s = wave_select(a.row_cells, a.row_ptr[t], a.row_ptr[t + 1], r, -2, tg, b2);
+                    if (s < 0) s = wave_select(a.col_cells, a.col_ptr[t], a.col_ptr[t + 1], t, r, tg, b2);
+                    if (s < 0) {
+                        if (lane == 0) atomicExch(a.err, 1);
+                        break;
+                    }
+                }
+            }
+            bool seen = false;
+            for (int j = 0; j < npast; ++j) seen |= (past[j] == s);
+            double dS = DBL_MAX;
+            if (!seen) {
+                dS = merge_delta_mdl(a, r, s);
+                past[npast++] = s;
+                long long rc_r = (rr1 - rr0) + (cr1 - cr0);
+                long long rc_s = (a.row_ptr[s + 1] - a.row_ptr[s]) + (a.col_ptr[s + 1] - a.col_ptr[s]);
+                long long rc_t = t >= 0 ? (a.row_ptr[t + 1] - a.row_ptr[t]) + (a.col_ptr[t + 1] - a.col_ptr[t]) : 0;
+                bytes += 16ull * rc_r + 8ull * rc_t + 12ull * (rc_r + rc_s);
+            }
+            if (dS < best) {
+                best = dS;
+                best_s = s;
+            }
+        }
+        if (lane == 0) {
+            a.best_s[r] = best_s;
+            a.best_dS[r] = best;
+        }
+    }
+    if (lane == 0 && bytes) atomicAdd(a.bytes, bytes);
+}
+
+uint64_t merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *d_best_s, double *d_best_dS) {
+    hipStream_t st = ctx->stream;
+    auto *bytes = ctx->scratch[S_COUNTER].as<unsigned long long>(4);
+    SBP_HIP(hipMemsetAsync(bytes, 0, 4 * sizeof(unsigned long long), st));
+    MgArgs a;
+    a.err = reinterpret_cast<int *>(bytes + 1);
+    a.B = bm->B;
+    a.seed = ctx->seed;
+    a.epoch = epoch;
+    a.d_out = bm->d_out;
+    a.d_in = bm->d_in;
+    a.row_ptr = bm->row_ptr;
+    a.col_ptr = bm->col_ptr;
+    a.row_cells = bm->row_cells;
+    a.col_cells = bm->col_cells;
+    a.best_s = d_best_s;
+    a.best_dS = d_best_dS;
+    a.bytes = bytes;
+    int64_t blocks = (bm->B + MG_WAVES - 1) / MG_WAVES;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    k_merge_propose<<<(int)blocks, MG_BLOCK, 0, st>>>(a);
+    SBP_HIP(hipGetLastError());
+    unsigned long long hb[2] = {0, 0};
+    SBP_HIP(hipMemcpyAsync(hb, bytes, sizeof(hb), hipMemcpyDeviceToHost, st));
+    SBP_HIP(hipStreamSynchronize(st));
+    SBP_CHECK(hb[1] == 0, SBP_E_STATE, "merge proposal: block degrees inconsistent with the block matrix");
+    return hb[0];
+}
+
+__global__ void k_relabel(int32_t N, const int32_t *__restrict__ map, const int32_t *__restrict__ in,
+                          int32_t *__restrict__ out) {
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) out[v] = map[in[v]];
+}
+
+void relabel_assignment(sbp_ctx ctx, sbp_bm bm, const int32_t *d_block_map, int32_t new_B, int32_t *d_out) {
+    (void)new_B;
+    int64_t g = (bm->N + 255) / 256;
+    if (g > 16384) g = 16384;
+    if (g < 1) g = 1;
+    k_relabel<<<(int)g, 256, 0, ctx->stream>>>(bm->N, d_block_map, bm->assign, d_out);
+    SBP_HIP(hipGetLastError());
+}
+
+}  // namespace sbpimpl

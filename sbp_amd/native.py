@@ -1,0 +1,281 @@
+"""ctypes binding of ``libsbp_amd.so`` (the C-ABI declared in ``include/sbp_amd.h``).
+
+The library must exist (built by ``sbp_amd/build.py``); there is no fallback path: every call runs the
+HIP kernels, and a missing library or device raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsbp_amd.so")
+
+ALGORITHMS = {"async_gibbs": 0, "metropolis_hastings": 1, "hybrid_mcmc": 2}
+
+# every symbol the header declares
+EXPORTS = [
+    "sbp_create", "sbp_destroy", "sbp_last_error", "sbp_set_seed", "sbp_synchronize",
+    "sbp_graph_upload", "sbp_graph_free",
+    "sbp_bm_build", "sbp_bm_copy", "sbp_bm_free", "sbp_bm_info", "sbp_bm_download", "sbp_bm_validate",
+    "sbp_mdl",
+    "sbp_ft_propose_all", "sbp_async_gibbs_sweep", "sbp_mh_sweep", "sbp_mcmc", "sbp_finetune_assignment",
+    "sbp_merge_propose", "sbp_merge_blocks",
+    "sbp_partition",
+    "sbp_debug_logf",
+]
+
+
+class SbpError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    """sbp_params: the hot-path subset of the reference's Args (include/args.hpp:61-212)."""
+    _fields_ = [("algorithm", C.c_int32), ("batches", C.c_int32), ("greedy", C.c_int32),
+                ("approximate", C.c_int32), ("ordered", C.c_int32), ("asynciterations", C.c_int32),
+                ("mh_percent", C.c_double), ("seed", C.c_uint64), ("max_iterations", C.c_int32),
+                ("threads", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("proposals", C.c_int64), ("moves", C.c_int64), ("refused", C.c_int64), ("sweeps", C.c_int64),
+                ("merge_proposals", C.c_int64), ("merge_phases", C.c_int64), ("mdl", C.c_double),
+                ("finetune_ms", C.c_double), ("merge_ms", C.c_double), ("carry_ms", C.c_double),
+                ("build_ms", C.c_double), ("mdl_ms", C.c_double), ("total_ms", C.c_double),
+                ("ft_bytes", C.c_uint64), ("merge_bytes", C.c_uint64), ("ft_launches", C.c_int64),
+                ("ft_kernel_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def params(algorithm="hybrid_mcmc", batches=1, greedy=True, approximate=False, ordered=False,
+           asynciterations=2 ** 31 - 1, mh_percent=0.075, seed=0, max_iterations=100, threads=1):
+    """Defaults follow include/args.hpp (algorithm hybrid_mcmc, greedy on, one batch, mh_percent 0.075)."""
+    alg = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
+    return Params(alg, batches, int(greedy), int(approximate), int(ordered), asynciterations, mh_percent, seed,
+                  max_iterations, threads)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SbpError(f"{LIB_PATH} is missing: build it with `python sbp_amd/build.py` "
+                           "(or __graft_entry__.build()) before using the GPU path")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, u32, u64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_double
+        P, S = C.POINTER(Params), C.POINTER(Stats)
+        sig = {
+            "sbp_create": [vp, C.c_int, u64],
+            "sbp_destroy": [vp],
+            "sbp_set_seed": [vp, u64],
+            "sbp_synchronize": [vp],
+            "sbp_graph_upload": [vp, i32, i64, vp, vp, vp, vp, vp],
+            "sbp_graph_free": [vp],
+            "sbp_bm_build": [vp, vp, i32, vp, vp],
+            "sbp_bm_copy": [vp, vp, vp],
+            "sbp_bm_free": [vp],
+            "sbp_bm_info": [vp, vp, vp],
+            "sbp_bm_download": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+            "sbp_bm_validate": [vp, vp, vp, vp],
+            "sbp_mdl": [vp, vp, vp, vp, vp],
+            "sbp_ft_propose_all": [vp, vp, vp, u32, i32, vp, vp, vp],
+            "sbp_async_gibbs_sweep": [vp, vp, vp, P, u32, S],
+            "sbp_mh_sweep": [vp, vp, vp, P, u32, i32, vp, S],
+            "sbp_mcmc": [vp, vp, vp, P, i32, vp, vp, S],
+            "sbp_finetune_assignment": [vp, vp, vp, P, vp, vp, S],
+            "sbp_merge_propose": [vp, vp, u32, vp, vp],
+            "sbp_merge_blocks": [vp, vp, vp, u32, i64, i32, vp, S],
+            "sbp_partition": [vp, vp, P, vp, vp, S],
+            "sbp_debug_logf": [vp, u32, u32, vp],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.restype = C.c_int
+            f.argtypes = args
+        L.sbp_last_error.restype = C.c_char_p
+        L.sbp_last_error.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One HIP stream on one device plus the replay seed (sbp_create)."""
+
+    def __init__(self, device=0, seed=0):
+        self.h = C.c_void_p()
+        rc = lib().sbp_create(C.byref(self.h), device, seed)
+        if rc != 0:
+            raise SbpError(f"sbp_create failed with status {rc} (no HIP device?)")
+
+    def check(self, rc, what):
+        if rc != 0:
+            msg = lib().sbp_last_error(self.h)
+            raise SbpError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            lib().sbp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_seed(self, seed):
+        self.check(lib().sbp_set_seed(self.h, seed), "sbp_set_seed")
+
+    def synchronize(self):
+        self.check(lib().sbp_synchronize(self.h), "sbp_synchronize")
+
+    def upload(self, g):
+        return DeviceGraph(self, g)
+
+    def debug_logf(self, first_bits, count):
+        out = np.zeros(count, np.float32)
+        self.check(lib().sbp_debug_logf(self.h, first_bits, count, _p(out)), "sbp_debug_logf")
+        return out
+
+
+class DeviceGraph:
+    def __init__(self, ctx, g):
+        self.ctx = ctx
+        self.N, self.E = g.N, g.E
+        self.h = C.c_void_p()
+        arrs = [np.ascontiguousarray(g.out_off, np.int64), np.ascontiguousarray(g.out_adj, np.int32),
+                np.ascontiguousarray(g.in_off, np.int64), np.ascontiguousarray(g.in_adj, np.int32)]
+        ctx.check(lib().sbp_graph_upload(ctx.h, g.N, g.E, *[_p(a) for a in arrs], C.byref(self.h)),
+                  "sbp_graph_upload")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().sbp_graph_free(self.h)
+            self.h = None
+
+    def partition(self, prm):
+        assign = np.zeros(self.N, np.int32)
+        nb = C.c_int32()
+        st = Stats()
+        self.ctx.check(lib().sbp_partition(self.ctx.h, self.h, C.byref(prm), _p(assign), C.byref(nb), C.byref(st)),
+                       "sbp_partition")
+        return assign, nb.value, st
+
+
+class DeviceBlockmodel:
+    def __init__(self, graph, handle):
+        self.g = graph
+        self.ctx = graph.ctx
+        self.h = handle
+
+    @classmethod
+    def build(cls, graph, B, assignment):
+        a = np.ascontiguousarray(assignment, np.int32)
+        h = C.c_void_p()
+        graph.ctx.check(lib().sbp_bm_build(graph.ctx.h, graph.h, int(B), _p(a), C.byref(h)), "sbp_bm_build")
+        return cls(graph, h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().sbp_bm_free(self.h)
+            self.h = None
+
+    def copy(self):
+        h = C.c_void_p()
+        self.ctx.check(lib().sbp_bm_copy(self.ctx.h, self.h, C.byref(h)), "sbp_bm_copy")
+        return DeviceBlockmodel(self.g, h)
+
+    def info(self):
+        B, nnz = C.c_int32(), C.c_int64()
+        self.ctx.check(lib().sbp_bm_info(self.h, C.byref(B), C.byref(nnz)), "sbp_bm_info")
+        return B.value, nnz.value
+
+    @property
+    def B(self):
+        return self.info()[0]
+
+    def export(self):
+        B, nnz = self.info()
+        out = dict(B=B, assignment=np.zeros(self.g.N, np.int32), sizes=np.zeros(B, np.int32),
+                   d_out=np.zeros(B, np.int32), d_in=np.zeros(B, np.int32), d_tot=np.zeros(B, np.int32),
+                   row_ptr=np.zeros(B + 1, np.int32), cols=np.zeros(max(nnz, 1), np.int32),
+                   cnts=np.zeros(max(nnz, 1), np.int32))
+        self.ctx.check(lib().sbp_bm_download(self.ctx.h, self.h, _p(out["assignment"]), _p(out["sizes"]),
+                                             _p(out["d_out"]), _p(out["d_in"]), _p(out["d_tot"]),
+                                             _p(out["row_ptr"]), _p(out["cols"]), _p(out["cnts"])),
+                       "sbp_bm_download")
+        out["cols"] = out["cols"][:nnz]
+        out["cnts"] = out["cnts"][:nnz]
+        return out
+
+    def validate(self):
+        ok = C.c_int32()
+        self.ctx.check(lib().sbp_bm_validate(self.ctx.h, self.h, self.g.h, C.byref(ok)), "sbp_bm_validate")
+        return bool(ok.value)
+
+    def mdl(self):
+        m, lp = C.c_double(), C.c_double()
+        self.ctx.check(lib().sbp_mdl(self.ctx.h, self.h, self.g.h, C.byref(m), C.byref(lp)), "sbp_mdl")
+        return m.value, lp.value
+
+    def propose_all(self, epoch, greedy=True):
+        N = self.g.N
+        s = np.zeros(N, np.int32)
+        dS = np.zeros(N, np.float64)
+        acc = np.zeros(N, np.int32)
+        self.ctx.check(lib().sbp_ft_propose_all(self.ctx.h, self.h, self.g.h, epoch, int(greedy), _p(s), _p(dS),
+                                                _p(acc)), "sbp_ft_propose_all")
+        return s, dS, acc.astype(bool)
+
+    def async_gibbs_sweep(self, prm, epoch):
+        st = Stats()
+        self.ctx.check(lib().sbp_async_gibbs_sweep(self.ctx.h, self.h, self.g.h, C.byref(prm), epoch, C.byref(st)),
+                       "sbp_async_gibbs_sweep")
+        return st
+
+    def mh_sweep(self, prm, epoch, ordered=False):
+        st = Stats()
+        tot = C.c_double()
+        self.ctx.check(lib().sbp_mh_sweep(self.ctx.h, self.h, self.g.h, C.byref(prm), epoch, int(ordered),
+                                          C.byref(tot), C.byref(st)), "sbp_mh_sweep")
+        return tot.value, st
+
+    def mcmc(self, prm, golden_ratio_not_reached, epoch):
+        ep = C.c_uint32(epoch)
+        mdl = C.c_double()
+        st = Stats()
+        self.ctx.check(lib().sbp_mcmc(self.ctx.h, self.h, self.g.h, C.byref(prm), int(golden_ratio_not_reached),
+                                      C.byref(ep), C.byref(mdl), C.byref(st)), "sbp_mcmc")
+        return ep.value, mdl.value, st
+
+    def finetune_assignment(self, prm, epoch):
+        ep = C.c_uint32(epoch)
+        mdl = C.c_double()
+        st = Stats()
+        self.ctx.check(lib().sbp_finetune_assignment(self.ctx.h, self.h, self.g.h, C.byref(prm), C.byref(ep),
+                                                     C.byref(mdl), C.byref(st)), "sbp_finetune_assignment")
+        return ep.value, mdl.value, st
+
+    def merge_propose(self, epoch):
+        B = self.B
+        s = np.zeros(B, np.int32)
+        dS = np.zeros(B, np.float64)
+        self.ctx.check(lib().sbp_merge_propose(self.ctx.h, self.h, epoch, _p(s), _p(dS)), "sbp_merge_propose")
+        return s, dS
+
+    def merge_blocks(self, epoch, num_to_merge, approximate=False):
+        h = C.c_void_p()
+        st = Stats()
+        self.ctx.check(lib().sbp_merge_blocks(self.ctx.h, self.h, self.g.h, epoch, int(num_to_merge),
+                                              int(approximate), C.byref(h), C.byref(st)), "sbp_merge_blocks")
+        return DeviceBlockmodel(self.g, h), st

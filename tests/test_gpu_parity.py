@@ -1,0 +1,231 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle on the same seeded inputs.
+
+Integer state (block matrix, degrees, sizes, proposals, assignments) must be bit-exact; delta-MDL and
+MDL are FP64 sums of glibc-logf terms accumulated in 128-bit fixed point on both sides, so they must be
+bit-identical too (the stated tolerance, 1e-12 relative, is checked as well for the record).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from sbp_amd import native
+from sbp_amd.graph import CSRGraph, dcsbm
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+TOY = json.load(open(os.path.join(HERE, "golden", "toy_example.json")))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = native.Context(0, seed=1234)
+    yield c
+    c.close()
+
+
+def toy_graph():
+    e = np.array(TOY["edges"])
+    return CSRGraph.from_edges(e[:, 0], e[:, 1], TOY["num_vertices"])
+
+
+def small_graph(seed=3, N=500, B=8, kmax=60, mean=8):
+    return dcsbm(N, B, gamma=2.5, kmin=1, kmax=kmax, mean_degree=mean, p_in=0.7, seed=seed)
+
+
+def both(ctx, g, B, assign):
+    og = O.OGraph.from_csr(g)
+    dg = ctx.upload(g)
+    return og, O.OBlockmodel.build(og, B, assign), dg, native.DeviceBlockmodel.build(dg, B, assign)
+
+
+def same_state(ob, db):
+    x, y = ob.export(), db.export()
+    assert x["B"] == y["B"]
+    for k in ("row_ptr", "cols", "cnts", "sizes", "assignment"):
+        assert np.array_equal(x[k], y[k]), k
+    for k in ("d_out", "d_in", "d_tot"):
+        assert np.array_equal(x[k], y[k].astype(np.int64)), k
+
+
+# ------------------------------------------------------------------ primitives
+
+def test_device_logf_matches_glibc(ctx):
+    """Restated glibc logf on the device: every integer count up to 2^25 and, for every exponent of the
+    positive floats (subnormals included), three windows of 65536 consecutive mantissas."""
+    ints = np.arange(1, 1 << 25, dtype=np.float32).view(np.uint32)
+    lo, hi = int(ints[0]), int(ints[-1]) + 1
+    for start in range(lo, hi, 1 << 24):
+        n = min(1 << 24, hi - start)
+        assert np.array_equal(ctx.debug_logf(start, n).view(np.uint32), O.logf_range(start, n).view(np.uint32))
+    for exp in range(0, 255):
+        for off in (0, 0x2AAAAA, 0x7F0000):
+            start = (exp << 23) | off
+            if start == 0:
+                start = 1
+            n = 65536
+            dev = ctx.debug_logf(start, n).view(np.uint32)
+            ref = O.logf_range(start, n).view(np.uint32)
+            assert np.array_equal(dev, ref), hex(start)
+
+
+# ------------------------------------------------------------------ blockmodel + MDL
+
+def test_toy_build_and_mdl(ctx):
+    g = toy_graph()
+    og, ob, dg, db = both(ctx, g, 3, TOY["assignment"])
+    same_state(ob, db)
+    m, lp = db.mdl()
+    assert m == ob.mdl()
+    assert abs(m - TOY["entropy"]) < 4 * 7.63e-6  # 4 float ULPs at 92.6, as the reference's test
+
+
+@pytest.mark.parametrize("seed,B", [(1, 5), (2, 40), (3, 500)])
+def test_build_and_mdl_random(ctx, seed, B):
+    g = small_graph(seed=seed)
+    rng = np.random.default_rng(seed)
+    assign = rng.integers(0, B, g.N).astype(np.int32)
+    assign[:B] = np.arange(B)
+    og, ob, dg, db = both(ctx, g, B, assign)
+    same_state(ob, db)
+    assert db.mdl()[0] == ob.mdl()
+    assert db.validate()
+
+
+# ------------------------------------------------------------------ finetune proposals
+
+@pytest.mark.parametrize("greedy", [True, False])
+@pytest.mark.parametrize("B", [3, 30, 300])
+def test_propose_all_bit_exact(ctx, greedy, B):
+    g = small_graph(seed=B, N=600, kmax=600, mean=10)  # includes vertices above the LDS-table threshold
+    rng = np.random.default_rng(B)
+    assign = rng.integers(0, B, g.N).astype(np.int32)
+    assign[:B] = np.arange(B)
+    og, ob, dg, db = both(ctx, g, B, assign)
+    ctx.set_seed(99)
+    s, dS, acc = db.propose_all(epoch=7, greedy=greedy)
+    mismatches = 0
+    for v in range(g.N):
+        os_, odS, oacc = ob.ft_propose(99, 7, v, greedy)
+        assert s[v] == os_, v
+        if s[v] != assign[v]:
+            assert dS[v] == odS, (v, dS[v], odS)
+            assert abs(dS[v] - odS) <= 1e-12 * max(1.0, abs(odS))
+        mismatches += int(acc[v] != oacc)
+    assert mismatches == 0
+
+
+def test_async_gibbs_sweeps_match_with_refusals(ctx):
+    g = small_graph(seed=21, N=800, mean=6)
+    B = 400  # blocks of about two vertices: many commits would empty a block
+    rng = np.random.default_rng(5)
+    assign = np.concatenate([np.arange(B), rng.integers(0, B, g.N - B)]).astype(np.int32)
+    og, ob, dg, db = both(ctx, g, B, assign)
+    ctx.set_seed(5)
+    prm_o = O.params("async_gibbs", seed=5, threads=4)
+    prm_d = native.params("async_gibbs", seed=5)
+    refused = 0
+    for ep in range(4):
+        _, st_o = ob.async_gibbs_sweep(prm_o, ep)
+        st_d = db.async_gibbs_sweep(prm_d, ep)
+        same_state(ob, db)
+        assert st_d.moves == st_o.moves and st_d.refused == st_o.refused
+        refused += st_o.refused
+    assert refused > 0
+
+
+@pytest.mark.parametrize("batches", [1, 3])
+def test_async_gibbs_batches_and_shuffle(ctx, batches):
+    g = small_graph(seed=8, N=700)
+    B = 12
+    assign = (np.arange(g.N) % B).astype(np.int32)
+    og, ob, dg, db = both(ctx, g, B, assign)
+    ctx.set_seed(3)
+    prm_o = O.params("async_gibbs", seed=3, batches=batches)
+    prm_d = native.params("async_gibbs", seed=3, batches=batches)
+    for ep in range(3):
+        ob.async_gibbs_sweep(prm_o, ep)
+        db.async_gibbs_sweep(prm_d, ep)
+        same_state(ob, db)
+
+
+@pytest.mark.parametrize("greedy", [True, False])
+def test_mh_sweeps_match(ctx, greedy):
+    g = small_graph(seed=31, N=400, kmax=600)
+    B = 25
+    rng = np.random.default_rng(1)
+    assign = rng.integers(0, B, g.N).astype(np.int32)
+    assign[:B] = np.arange(B)
+    og, ob, dg, db = both(ctx, g, B, assign)
+    ctx.set_seed(11)
+    prm_o = O.params("metropolis_hastings", seed=11, greedy=greedy)
+    prm_d = native.params("metropolis_hastings", seed=11, greedy=greedy)
+    for ep in range(3):
+        so, _ = ob.mh_sweep(prm_o, ep, ordered=(ep == 2))
+        sd, _ = db.mh_sweep(prm_d, ep, ordered=(ep == 2))
+        same_state(ob, db)
+        assert so == sd
+    assert db.validate()
+
+
+@pytest.mark.parametrize("alg", ["async_gibbs", "metropolis_hastings", "hybrid_mcmc"])
+def test_mcmc_phase_matches(ctx, alg):
+    g = small_graph(seed=41, N=900, kmax=700)
+    B = 40
+    rng = np.random.default_rng(2)
+    assign = rng.integers(0, B, g.N).astype(np.int32)
+    assign[:B] = np.arange(B)
+    og, ob, dg, db = both(ctx, g, B, assign)
+    ctx.set_seed(17)
+    ep_o, st_o = ob.mcmc(O.params(alg, seed=17, mh_percent=0.1), True, 0)
+    ep_d, mdl_d, st_d = db.mcmc(native.params(alg, seed=17, mh_percent=0.1), True, 0)
+    assert ep_o == ep_d
+    same_state(ob, db)
+    assert mdl_d == ob.mdl()
+
+
+# ------------------------------------------------------------------ block merge
+
+@pytest.mark.parametrize("B", [50, 900])
+def test_merge_propose_bit_exact(ctx, B):
+    g = small_graph(seed=B, N=900)
+    rng = np.random.default_rng(B)
+    assign = rng.integers(0, B, g.N).astype(np.int32)
+    assign[:B] = np.arange(B)
+    og, ob, dg, db = both(ctx, g, B, assign)
+    ctx.set_seed(42)
+    s_d, dS_d = db.merge_propose(epoch=3)
+    s_o, dS_o = ob.merge_propose(42, 3)
+    assert np.array_equal(s_d, s_o)
+    assert np.array_equal(dS_d, dS_o)
+
+
+@pytest.mark.parametrize("approximate", [False, True])
+def test_merge_blocks_matches(ctx, approximate):
+    g = small_graph(seed=77, N=1000)
+    B = g.N
+    assign = np.arange(B, dtype=np.int32)
+    og, ob, dg, db = both(ctx, g, B, assign)
+    ctx.set_seed(6)
+    out_d, st = db.merge_blocks(epoch=0, num_to_merge=B // 2, approximate=approximate)
+    s_o, dS_o = ob.merge_propose(6, 0)
+    out_o = ob.merge_apply(s_o, dS_o, B // 2, approximate)
+    same_state(out_o, out_d)
+
+
+# ------------------------------------------------------------------ end to end
+
+@pytest.mark.parametrize("alg", ["metropolis_hastings", "async_gibbs", "hybrid_mcmc"])
+def test_full_partition_bit_exact(ctx, alg):
+    g = dcsbm(1000, 16, gamma=2.5, kmin=1, kmax=100, mean_degree=10, p_in=0.75, seed=0)
+    og = O.OGraph.from_csr(g)
+    a_o, nb_o, st_o = O.sbp(og, O.params(alg, seed=0, threads=8))
+    dg = ctx.upload(g)
+    a_d, nb_d, st_d = dg.partition(native.params(alg, seed=0))
+    assert nb_d == nb_o
+    assert np.array_equal(a_d, a_o)
+    assert st_d.mdl == st_o.mdl
+    from sklearn.metrics import normalized_mutual_info_score as nmi
+    assert nmi(g.truth, a_d) > 0.9
