@@ -1,0 +1,164 @@
+// C++ drop-in surface of the SBP hot path, implemented over the C-ABI in include/sbp_amd.h.
+//
+// Mirrors the reference's public functions and classes (vtsynergy/SBP):
+//   sbp::stochastic_block_partition / done_blockmodeling      include/sbp.hpp:18-21
+//   Blockmodel                                                include/blockmodel/blockmodel.hpp:46-219
+//   BlockmodelTriplet                                         include/blockmodel/blockmodel_triplet.hpp:12-55
+//   block_merge::merge_blocks                                 include/block_merge.hpp:48
+//   finetune::mcmc / asynchronous_gibbs / metropolis_hastings /
+//            hybrid_mcmc / finetune_assignment                include/finetune.hpp:38-123
+//   entropy::mdl / null_mdl_v1 / normalize_mdl_v1             include/entropy.hpp:71-80
+//   sample::sample / extend                                   include/sample.hpp:53-71
+// Vertex and block ids are `long` at this surface, as in the reference; the device stores int32.
+// Errors from the device path are thrown as std::runtime_error (std::invalid_argument for bad input).
+#ifndef SBP_AMD_SBP_HPP
+#define SBP_AMD_SBP_HPP
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../sbp_amd.h"
+#include "args.hpp"
+
+namespace sbp_amd {
+/// Process-wide device context (one HIP stream), created on first use from the Args' device and seed.
+sbp_ctx context(const Args &args);
+void throw_on(int status, sbp_ctx ctx, const char *what);
+}  // namespace sbp_amd
+
+/// Directed graph with per-vertex neighbour lists in file order (include/graph.hpp:15-140).
+class Graph {
+  public:
+    Graph() = default;
+    Graph(long num_vertices, std::vector<int64_t> out_off, std::vector<int32_t> out_adj, std::vector<int64_t> in_off,
+          std::vector<int32_t> in_adj);
+    /// Graph::load (src/graph.cpp:33-72): <filepath>.tsv (1-indexed "from to" lines) or .mtx, plus
+    /// <filepath>_truePartition.tsv when present.
+    static Graph load(const Args &args);
+    /// Builds from an edge list in file order (0-indexed).
+    static Graph from_edges(long num_vertices, const std::vector<std::pair<long, long>> &edges);
+    long num_vertices() const { return N_; }
+    long num_edges() const { return (long)out_adj_.size(); }
+    std::vector<long> out_neighbors(long v) const;
+    std::vector<long> in_neighbors(long v) const;
+    /// Graph::degrees (src/graph.cpp:24-31)
+    std::vector<long> degrees() const;
+    const std::vector<long> &assignment() const { return truth_; }
+    void assignment(std::vector<long> a) { truth_ = std::move(a); }
+    /// Device copy on the context (uploaded on first use).
+    sbp_graph device(sbp_ctx ctx) const;
+    const std::vector<int64_t> &out_off() const { return out_off_; }
+    const std::vector<int32_t> &out_adj() const { return out_adj_; }
+    const std::vector<int64_t> &in_off() const { return in_off_; }
+    const std::vector<int32_t> &in_adj() const { return in_adj_; }
+
+  private:
+    long N_ = 0;
+    std::vector<int64_t> out_off_{0}, in_off_{0};
+    std::vector<int32_t> out_adj_, in_adj_;
+    std::vector<long> truth_;
+    mutable std::shared_ptr<struct sbp_graph_s> dev_;
+};
+
+/// Device-resident blockmodel (include/blockmodel/blockmodel.hpp:46-219).  Copy-assignment shares the
+/// device state, as the reference's copy-assignment shares its block matrix; copy() deep-copies.
+class Blockmodel {
+  public:
+    Blockmodel() = default;  // empty
+    /// Blockmodel(num_blocks, graph, rate): every vertex in its own block (num_blocks == N).
+    Blockmodel(long num_blocks, const Graph &graph, double block_reduction_rate);
+    /// Blockmodel(num_blocks, graph, rate, assignment) + initialize_edge_counts.
+    Blockmodel(long num_blocks, const Graph &graph, double block_reduction_rate, std::vector<long> &assignment);
+    Blockmodel copy() const;
+    long num_blocks() const;
+    long num_vertices() const { return N_; }
+    std::vector<long> block_assignment() const;
+    long block_assignment(long vertex) const { return block_assignment()[vertex]; }
+    std::vector<long> block_sizes() const;
+    std::vector<long> degrees() const;
+    std::vector<long> degrees_in() const;
+    std::vector<long> degrees_out() const;
+    double getOverall_entropy() const { return overall_entropy_; }
+    void setOverall_entropy(double e) { overall_entropy_ = e; }
+    long &getNum_blocks_to_merge() { return num_blocks_to_merge_; }
+    void setNum_blocks_to_merge(long n) { num_blocks_to_merge_ = n; }
+    /// Blockmodel::validate: rebuild from the assignment and compare every count.
+    bool validate(const Graph &graph) const;
+    /// log_posterior_probability (src/blockmodel/blockmodel.cpp:288-306)
+    double log_posterior_probability(const Graph &graph) const;
+    sbp_bm handle() const { return bm_.get(); }
+    sbp_ctx ctx() const { return ctx_; }
+    /// Takes ownership of a device blockmodel handle.
+    static Blockmodel adopt(sbp_ctx ctx, sbp_bm bm, long num_vertices, double overall_entropy,
+                            long num_blocks_to_merge);
+    bool empty = true;
+
+  private:
+    std::shared_ptr<struct sbp_bm_s> bm_;
+    sbp_ctx ctx_ = nullptr;
+    long N_ = 0;
+    double overall_entropy_ = 1.7976931348623157e308;
+    long num_blocks_to_merge_ = 0;
+};
+
+/// Golden-section bracket over block counts (src/blockmodel/blockmodel_triplet.cpp:4-105).
+class BlockmodelTriplet {
+  public:
+    bool optimal_num_blocks_found = false;
+    void update(Blockmodel &blockmodel);
+    void status() const;
+    Blockmodel &get(long i) { return blockmodels_[i]; }
+    bool golden_ratio_not_reached() const { return blockmodels_[2].empty; }
+    bool is_done();
+    Blockmodel get_next_blockmodel(Blockmodel &old_blockmodel);
+
+  private:
+    long nb(long i) const { return blockmodels_[i].empty ? 0 : blockmodels_[i].num_blocks(); }
+    Blockmodel blockmodels_[3];
+};
+
+namespace sbp {
+/// src/sbp.cpp:16-72.  The epoch counter of the replay contract starts at 0 for every call.
+Blockmodel stochastic_block_partition(Graph &graph, Args &args, bool divide_and_conquer = false);
+bool done_blockmodeling(Blockmodel &blockmodel, BlockmodelTriplet &blockmodel_triplet, long min_num_blocks = 0);
+/// Statistics of the last stochastic_block_partition call on this thread (timers::* equivalents).
+const sbp_stats &last_stats();
+}  // namespace sbp
+
+namespace block_merge {
+/// src/block_merge.cpp:178-228.  Consumes one epoch of the replay counter.
+Blockmodel &merge_blocks(Blockmodel &blockmodel, const Graph &graph, long num_edges);
+}  // namespace block_merge
+
+namespace finetune {
+/// src/finetune.cpp:754-771.  Dispatches on args.algorithm (and args.asynciterations).
+Blockmodel &mcmc(int iteration, const Graph &graph, Blockmodel &blockmodel, BlockmodelTriplet &blockmodel_triplet);
+Blockmodel &asynchronous_gibbs(Blockmodel &blockmodel, const Graph &graph, bool golden_ratio_not_reached);
+Blockmodel &metropolis_hastings(Blockmodel &blockmodel, const Graph &graph, bool golden_ratio_not_reached);
+Blockmodel &hybrid_mcmc(Blockmodel &blockmodel, const Graph &graph, bool golden_ratio_not_reached);
+Blockmodel &finetune_assignment(Blockmodel &blockmodel, Graph &graph);
+}  // namespace finetune
+
+namespace entropy {
+/// src/entropy.cpp:597-604
+double mdl(const Blockmodel &blockmodel, const Graph &graph);
+/// src/entropy.cpp:571-585
+double null_mdl_v1(const Graph &graph);
+/// src/entropy.cpp:563-565
+double normalize_mdl_v1(double mdl, const Graph &graph);
+}  // namespace entropy
+
+namespace sbp_amd {
+/// Arguments used by the namespaced entry points above (the reference reads the global `args`).
+Args &args();
+/// Replay epoch counter shared by merge_blocks and the finetune sweeps.
+uint32_t &epoch();
+/// Translates the flags into the C-ABI parameter block.
+sbp_params to_params(const Args &a);
+/// The agglomerative driver over a context and a device graph (shared by sbp::stochastic_block_partition
+/// and the C-ABI's sbp_partition).
+Blockmodel drive(sbp_ctx ctx, sbp_graph graph, int32_t num_vertices, const sbp_params &p, sbp_stats *stats);
+}  // namespace sbp_amd
+
+#endif  // SBP_AMD_SBP_HPP

@@ -1,4 +1,5 @@
-"""Builds the gfx950 shared library ``sbp_amd/libsbp_amd.so`` from ``sbp_amd/csrc/*.hip``.
+"""Builds the gfx950 shared library ``sbp_amd/libsbp_amd.so`` (HIP kernels + C-ABI + C++ drop-in API) and the
+command-line driver ``sbp_amd/sbp_cli``.
 
 Every translation unit is compiled with FP contraction off, so each FP64 operation in the delta-MDL and
 MDL terms is a single IEEE rounding (the fixed-point sums then make them bit-identical to a CPU
@@ -13,7 +14,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsbp_amd.so")
-SOURCES = ["build.hip", "mdl.hip", "finetune.hip", "merge.hip", "capi.hip"]
+SOURCES = ["csrc/build.hip", "csrc/mdl.hip", "csrc/finetune.hip", "csrc/merge.hip", "csrc/capi.hip",
+           "host/sbp_api.cpp", "host/args.cpp"]
+CLI = os.path.join(HERE, "sbp_cli")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SBP_OFFLOAD_ARCH", "gfx950")
@@ -25,8 +28,8 @@ FLAGS = [
 
 
 def _compile(src):
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    obj = os.path.join(BUILD, os.path.basename(os.path.splitext(src)[0]) + ".o")
+    cmd = [HIPCC, *FLAGS, "-c", os.path.join(HERE, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
@@ -43,8 +46,14 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    cmd = [HIPCC, *FLAGS, "-o", CLI + ".tmp", os.path.join(HERE, "host", "main.cpp"), "-L" + HERE, "-lsbp_amd",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"CLI link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(CLI + ".tmp", CLI)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CLI}")
     return LIB
 
 

@@ -15,6 +15,7 @@
 #include "device_common.hpp"
 #include "internal.hpp"
 #include "host_blockmodel.hpp"
+#include "../../include/sbp/sbp.hpp"
 
 using namespace sbpimpl;
 
@@ -281,110 +282,6 @@ static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, 
     out->overall_entropy = bm->overall_entropy;
     if (st) st->build_ms += now_ms() - t1;
     return out;
-}
-
-// ------------------------------------------------------------------ driver: triplet + SBP loop
-
-struct Triplet {
-    sbp_bm bm[3] = {nullptr, nullptr, nullptr};
-    bool optimal = false;
-    sbp_ctx ctx;
-    explicit Triplet(sbp_ctx c) : ctx(c) {}
-    ~Triplet() {
-        for (auto &b : bm) free_bm(b);
-    }
-    static void free_bm(sbp_bm &b) {
-        if (b) {
-            bm_release(b);
-            delete b;
-            b = nullptr;
-        }
-    }
-    bool empty(int i) const { return bm[i] == nullptr; }
-    sbp_bm copy_of(sbp_bm src) {
-        sbp_bm b = new_bm(ctx);
-        bm_copy_into(ctx, src, b);
-        return b;
-    }
-    void set(int i, sbp_bm b) {
-        free_bm(bm[i]);
-        bm[i] = b;
-    }
-    // BlockmodelTriplet::update (src/blockmodel/blockmodel_triplet.cpp:82-105)
-    void update(sbp_bm b) {
-        int index;
-        if (empty(1)) {
-            index = 1;
-        } else if (b->overall_entropy <= bm[1]->overall_entropy) {
-            int old_index = (bm[1]->B > b->B) ? 0 : 2;
-            set(old_index, copy_of(bm[1]));
-            index = 1;
-        } else {
-            index = (bm[1]->B > b->B) ? 2 : 0;
-        }
-        set(index, copy_of(b));
-    }
-    int64_t nb(int i) const { return empty(i) ? 0 : bm[i]->B; }
-    bool is_done() {
-        if ((!empty(0) && nb(0) - nb(2) == 2) || (empty(0) && nb(1) - nb(2) == 1)) optimal = true;
-        return optimal;
-    }
-    // BlockmodelTriplet::get_next_blockmodel (src/blockmodel/blockmodel_triplet.cpp:4-37)
-    sbp_bm next(sbp_bm old) {
-        old->num_to_merge = 0;
-        update(old);
-        if (empty(2)) {
-            sbp_bm b = copy_of(bm[1]);
-            b->num_to_merge = (int64_t)((double)b->B * 0.5);
-            if (b->num_to_merge == 0) optimal = true;
-            return b;
-        }
-        if (is_done()) return copy_of(bm[1]);
-        int index;
-        if (empty(0) && nb(1) > nb(2)) index = 1;
-        else if (nb(0) - nb(1) >= nb(1) - nb(2)) index = 0;
-        else index = 1;
-        int64_t next_nb = nb(index + 1);
-        next_nb += (int64_t)((double)(nb(index) - nb(index + 1)) * 0.618);
-        sbp_bm b = copy_of(bm[index]);
-        b->num_to_merge = b->B - next_nb;
-        return b;
-    }
-};
-
-static void partition(sbp_ctx ctx, sbp_graph g, const sbp_params &p, std::vector<int32_t> &assignment,
-                      int32_t &num_blocks, sbp_stats *st) {
-    const int32_t N = g->N;
-    std::vector<int32_t> ident(N);
-    std::iota(ident.begin(), ident.end(), 0);
-    sbp_bm bm = new_bm(ctx);
-    bm_alloc(bm, N, N);
-    SBP_HIP(hipMemcpy(bm->assign, ident.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
-    double t0 = now_ms();
-    bm_rebuild(ctx, g, bm);
-    SBP_HIP(hipStreamSynchronize(ctx->stream));
-    if (st) st->build_ms += now_ms() - t0;
-    bm->num_to_merge = (int64_t)((double)N * 0.5);
-    Triplet tri(ctx);
-    uint32_t epoch = 0;
-    int64_t iteration = 0;
-    while (!tri.optimal) {
-        sbp_bm merged = merge_blocks(ctx, bm, g, epoch++, bm->num_to_merge, p.approximate != 0, st);
-        Triplet::free_bm(bm);
-        bm = merged;
-        sbp_params q = p;
-        if (p.algorithm == 0 && iteration >= p.asynciterations) q.algorithm = 1;
-        mcmc(ctx, bm, g, q, tri.empty(2), &epoch, st);
-        iteration++;
-        sbp_bm next = tri.next(bm);
-        Triplet::free_bm(bm);
-        bm = next;
-    }
-    assignment.resize(N);
-    SBP_HIP(hipMemcpy(assignment.data(), bm->assign, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
-    num_blocks = bm->B;
-    if (st) st->mdl = bm->overall_entropy;
-    Triplet::free_bm(bm);
 }
 
 static int32_t *iota_of(sbp_ctx ctx, sbp_graph g) {
@@ -710,19 +607,19 @@ int sbp_partition(sbp_ctx ctx, sbp_graph g, const sbp_params *p, int32_t *assign
     if (!ctx || !g || !p || p->batches < 1 || p->algorithm < 0 || p->algorithm > 2) return SBP_E_ARG;
     SBP_API_BEGIN
     double t0 = now_ms();
-    std::vector<int32_t> a;
-    int32_t nb = 0;
-    uint64_t saved = ctx->seed;
+    const uint64_t saved = ctx->seed;
     ctx->seed = p->seed;
     try {
-        partition(ctx, g, *p, a, nb, stats);
+        Blockmodel bm = sbp_amd::drive(ctx, g, g->N, *p, stats);
+        SBP_HIP(hipStreamSynchronize(ctx->stream));
+        if (assignment_out)
+            SBP_HIP(hipMemcpy(assignment_out, bm.handle()->assign, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
+        if (num_blocks_out) *num_blocks_out = bm.handle()->B;
     } catch (...) {
         ctx->seed = saved;
         throw;
     }
     ctx->seed = saved;
-    if (assignment_out) memcpy(assignment_out, a.data(), sizeof(int32_t) * g->N);
-    if (num_blocks_out) *num_blocks_out = nb;
     if (stats) stats->total_ms += now_ms() - t0;
     SBP_API_END(ctx)
 }
