@@ -65,6 +65,8 @@ def lib():
             "orc_vertex_delta_mdl": (dbl, [vp, G, i32, i32]),
             "orc_vertex_hastings": (dbl, [vp, G, i32, i32]),
             "orc_merge_delta_mdl": (dbl, [vp, i32, i32]),
+            "orc_vertex_delta_mdl_cells": (dbl, [vp, G, i32, i32, vp]),
+            "orc_merge_delta_mdl_cells": (dbl, [vp, i32, i32, vp]),
             "orc_ft_propose": (C.c_int, [vp, G, u64, u32, i32, i32, vp, vp, vp]),
             "orc_async_gibbs_sweep": (C.c_int, [vp, G, P, u32, vp, S]),
             "orc_mh_sweep": (C.c_int, [vp, G, P, u32, i32, vp, S]),
@@ -180,6 +182,17 @@ class OBlockmodel:
 
     def merge_delta_mdl(self, r, s):
         return lib().orc_merge_delta_mdl(self.h, r, s)
+
+    def vertex_delta_mdl_cells(self, v, s):
+        """Per-cell (reference order) delta-MDL and the sum of |cell terms|."""
+        a = C.c_double()
+        d = lib().orc_vertex_delta_mdl_cells(self.h, self.g.ref(), v, s, C.byref(a))
+        return d, a.value
+
+    def merge_delta_mdl_cells(self, r, s):
+        a = C.c_double()
+        d = lib().orc_merge_delta_mdl_cells(self.h, r, s, C.byref(a))
+        return d, a.value
 
     def ft_propose(self, seed, epoch, v, greedy=True):
         s, dS, acc = C.c_int32(), C.c_double(), C.c_int32()

@@ -503,6 +503,9 @@ static void vertex_delta(const orc_bm *bm, const orc_graph *g, int32_t v, int32_
     }
 }
 
+/* Sum of |cell entropy| terms a per-cell evaluation touches (the scale of its rounding error). */
+static __thread double g_abs_terms;
+
 typedef struct { const orc_bm *bm; int32_t r, s; int64_t shift_in, shift_out; } degfn;
 static inline int64_t new_d_in(const degfn *f, int32_t x) {
     int64_t v = f->bm->d_in[x];
@@ -526,10 +529,54 @@ static void imap_entries_delta(const orc_bm *bm, const delta_t *d, const imap *m
         else { row = m->key[i]; col = d->s; }
         int64_t change = m->val[i];
         int64_t value = bm_get(bm, row, col);
-        *acc += fx_from(cell_entropy(value, bm->d_in[col], bm->d_out[row]));
+        double t_old = cell_entropy(value, bm->d_in[col], bm->d_out[row]);
+        *acc += fx_from(t_old);
+        g_abs_terms += fabs(t_old);
         if (skip_new_in_r && (row == d->r || col == d->r)) continue;
-        *acc -= fx_from(cell_entropy(value + change, new_d_in(f, col), new_d_out(f, row)));
+        double t_new = cell_entropy(value + change, new_d_in(f, col), new_d_out(f, row));
+        *acc -= fx_from(t_new);
+        g_abs_terms += fabs(t_new);
     }
+}
+
+/* Replay-contract delta-MDL of a vertex move: the Delta entries term by term, the four cells at
+ * rows/columns {r, s} term by term, and the unchanged cells of rows r, s and columns r, s in bulk.  An
+ * unchanged cell (row, c) with c outside {r, s} changes only through d_out[row], so its old-minus-new
+ * entropy is M * (fastlog(d_out'[row]) - fastlog(d_out[row])); summing M over those cells gives the row
+ * total minus the Delta and special cells.  Equal to entropy::delta_mdl (src/entropy.cpp:298-403) up to
+ * rounding: the per-cell form is kept below as delta_mdl_from and the two are compared in the tests. */
+static double delta_mdl_factored(const orc_bm *bm, const delta_t *d, const proposal_t *p) {
+    degfn f = { bm, d->r, d->s, p->k_in + d->self_edge_weight, p->k_out };
+    const int32_t r = d->r, s = d->s;
+    fx128 acc = 0;
+    imap_entries_delta(bm, d, &d->cur_row, 0, &f, 0, &acc);
+    imap_entries_delta(bm, d, &d->prop_row, 1, &f, 0, &acc);
+    imap_entries_delta(bm, d, &d->cur_col, 2, &f, 0, &acc);
+    imap_entries_delta(bm, d, &d->prop_col, 3, &f, 0, &acc);
+    const int32_t sp[4][2] = { { r, r }, { r, s }, { s, r }, { s, s } };
+    for (int k = 0; k < 4; ++k) {
+        int32_t row = sp[k][0], col = sp[k][1];
+        int64_t m = bm_get(bm, row, col);
+        if (m == 0 || delta_get(d, row, col) != 0) continue;
+        acc += fx_from(cell_entropy(m, bm->d_in[col], bm->d_out[row]));
+        acc -= fx_from(cell_entropy(m, new_d_in(&f, col), new_d_out(&f, row)));
+    }
+    const int64_t mrr = bm_get(bm, r, r), mrs = bm_get(bm, r, s), msr = bm_get(bm, s, r), mss = bm_get(bm, s, s);
+    int64_t S_rrow = bm->d_out[r] - mrr - mrs, S_srow = bm->d_out[s] - mss - msr;
+    int64_t S_rcol = bm->d_in[r] - mrr - msr, S_scol = bm->d_in[s] - mss - mrs;
+    for (int32_t i = 0; i < d->cur_row.cap; ++i)
+        if (d->cur_row.used[i] && d->cur_row.key[i] != r && d->cur_row.key[i] != s) S_rrow -= bm_get(bm, r, d->cur_row.key[i]);
+    for (int32_t i = 0; i < d->prop_row.cap; ++i)
+        if (d->prop_row.used[i] && d->prop_row.key[i] != r && d->prop_row.key[i] != s) S_srow -= bm_get(bm, s, d->prop_row.key[i]);
+    for (int32_t i = 0; i < d->cur_col.cap; ++i)
+        if (d->cur_col.used[i]) S_rcol -= bm_get(bm, d->cur_col.key[i], r);
+    for (int32_t i = 0; i < d->prop_col.cap; ++i)
+        if (d->prop_col.used[i]) S_scol -= bm_get(bm, d->prop_col.key[i], s);
+    acc += fx_from((double)S_rrow * (fastlog(new_d_out(&f, r)) - fastlog(bm->d_out[r])));
+    acc += fx_from((double)S_srow * (fastlog(new_d_out(&f, s)) - fastlog(bm->d_out[s])));
+    acc += fx_from((double)S_rcol * (fastlog(new_d_in(&f, r)) - fastlog(bm->d_in[r])));
+    acc += fx_from((double)S_scol * (fastlog(new_d_in(&f, s)) - fastlog(bm->d_in[s])));
+    return fx_to(acc);
 }
 
 /* entropy::delta_mdl(const Blockmodel&, const Delta&, const ProposalAndEdgeCounts&) (src/entropy.cpp:298-403) */
@@ -548,8 +595,11 @@ static double delta_mdl_from(const orc_bm *bm, const delta_t *d, const proposal_
             int32_t col = sv->idx[i];
             int64_t value = sv->val[i];
             if (delta_get(d, row, col) != 0) continue;
-            acc += fx_from(cell_entropy(value, bm->d_in[col], bm->d_out[row]));
-            acc -= fx_from(cell_entropy(value, new_d_in(&f, col), new_d_out(&f, row)));
+            double a = cell_entropy(value, bm->d_in[col], bm->d_out[row]);
+            double b = cell_entropy(value, new_d_in(&f, col), new_d_out(&f, row));
+            acc += fx_from(a);
+            acc -= fx_from(b);
+            g_abs_terms += fabs(a) + fabs(b);
         }
     }
     for (int k = 0; k < 2; ++k) {
@@ -559,8 +609,11 @@ static double delta_mdl_from(const orc_bm *bm, const delta_t *d, const proposal_
             int32_t row = sv->idx[i];
             int64_t value = sv->val[i];
             if (delta_get(d, row, col) != 0 || row == d->r || row == d->s) continue;
-            acc += fx_from(cell_entropy(value, bm->d_in[col], bm->d_out[row]));
-            acc -= fx_from(cell_entropy(value, new_d_in(&f, col), new_d_out(&f, row)));
+            double a = cell_entropy(value, bm->d_in[col], bm->d_out[row]);
+            double b = cell_entropy(value, new_d_in(&f, col), new_d_out(&f, row));
+            acc += fx_from(a);
+            acc -= fx_from(b);
+            g_abs_terms += fabs(a) + fabs(b);
         }
     }
     return fx_to(acc);
@@ -625,7 +678,21 @@ double orc_vertex_delta_mdl(const orc_bm *bm, const orc_graph *g, int32_t v, int
     p.s = s;
     delta_t d;
     vertex_delta(bm, g, v, bm->assign[v], s, &d);
+    double dS = delta_mdl_factored(bm, &d, &p);
+    delta_free(&d);
+    return dS;
+}
+
+/* entropy::delta_mdl cell by cell, as the reference evaluates it; *abs_terms = sum of |cell terms|. */
+double orc_vertex_delta_mdl_cells(const orc_bm *bm, const orc_graph *g, int32_t v, int32_t s, double *abs_terms) {
+    uint32_t w[4] = { 0, 0, 0, 0 };
+    proposal_t p = propose_vertex(bm, g, v, w);
+    p.s = s;
+    delta_t d;
+    vertex_delta(bm, g, v, bm->assign[v], s, &d);
+    g_abs_terms = 0.0;
     double dS = delta_mdl_from(bm, &d, &p);
+    if (abs_terms) *abs_terms = g_abs_terms;
     delta_free(&d);
     return dS;
 }
@@ -655,7 +722,7 @@ static int evaluate_vertex(const orc_bm *bm, const orc_graph *g, uint64_t seed, 
     delta_t d;
     vertex_delta(bm, g, v, r, p.s, &d);
     double H = greedy ? 1.0 : hastings_from(bm, g, v, &d, &p);
-    double dS = delta_mdl_from(bm, &d, &p);
+    double dS = delta_mdl_factored(bm, &d, &p);
     *dS_out = dS;
     int acc = accept_move(dS, H, greedy, w);
     if (keep && acc) *keep = d; else delta_free(&d);
@@ -1011,24 +1078,69 @@ static double merge_dS_from(const orc_bm *bm, const delta_t *d, int64_t k_out, i
         int32_t col = row->idx[i];
         int64_t value = row->val[i];
         if (delta_get(d, s, col) != 0) continue;
-        acc += fx_from(cell_entropy(value, bm->d_in[col], bm->d_out[s]));
-        acc -= fx_from(cell_entropy(value, new_d_in(&f, col), new_d_out(&f, s)));
+        double a = cell_entropy(value, bm->d_in[col], bm->d_out[s]);
+        double b = cell_entropy(value, new_d_in(&f, col), new_d_out(&f, s));
+        acc += fx_from(a);
+        acc -= fx_from(b);
+        g_abs_terms += fabs(a) + fabs(b);
     }
     const svec *col = &bm->col[s];
     for (int32_t i = 0; i < col->len; ++i) {
         int32_t rw = col->idx[i];
         int64_t value = col->val[i];
         if (delta_get(d, rw, s) != 0 || rw == r || rw == s) continue;
-        acc += fx_from(cell_entropy(value, bm->d_in[s], bm->d_out[rw]));
-        acc -= fx_from(cell_entropy(value, new_d_in(&f, s), new_d_out(&f, rw)));
+        double a = cell_entropy(value, bm->d_in[s], bm->d_out[rw]);
+        double b = cell_entropy(value, new_d_in(&f, s), new_d_out(&f, rw));
+        acc += fx_from(a);
+        acc -= fx_from(b);
+        g_abs_terms += fabs(a) + fabs(b);
     }
+    return fx_to(acc);
+}
+
+/* Replay-contract delta-MDL of merging r into s: Delta entries term by term, cell (s, s) when it is
+ * unchanged, and the unchanged cells of row s / column s in bulk (their d_in / d_out do not change, so
+ * each contributes M * (fastlog(d') - fastlog(d)) of the s degree).  Equal to
+ * entropy::block_merge_delta_mdl (src/entropy.cpp:129-182) up to rounding; the per-cell form is
+ * merge_dS_from and the tests compare the two. */
+static double merge_dS_factored(const orc_bm *bm, const delta_t *d, int64_t k_out, int64_t k_in) {
+    degfn f = { bm, d->r, d->s, k_in, k_out };
+    const int32_t r = d->r, s = d->s;
+    fx128 acc = 0;
+    imap_entries_delta(bm, d, &d->cur_row, 0, &f, 1, &acc);
+    imap_entries_delta(bm, d, &d->prop_row, 1, &f, 1, &acc);
+    imap_entries_delta(bm, d, &d->cur_col, 2, &f, 1, &acc);
+    imap_entries_delta(bm, d, &d->prop_col, 3, &f, 1, &acc);
+    const int64_t mss = bm_get(bm, s, s), msr = bm_get(bm, s, r), mrs = bm_get(bm, r, s);
+    if (mss != 0 && delta_get(d, s, s) == 0) {
+        acc += fx_from(cell_entropy(mss, bm->d_in[s], bm->d_out[s]));
+        acc -= fx_from(cell_entropy(mss, new_d_in(&f, s), new_d_out(&f, s)));
+    }
+    int64_t S_srow = bm->d_out[s] - mss - msr, S_scol = bm->d_in[s] - mss - mrs;
+    for (int32_t i = 0; i < d->prop_row.cap; ++i)
+        if (d->prop_row.used[i] && d->prop_row.key[i] != r && d->prop_row.key[i] != s) S_srow -= bm_get(bm, s, d->prop_row.key[i]);
+    for (int32_t i = 0; i < d->prop_col.cap; ++i)
+        if (d->prop_col.used[i]) S_scol -= bm_get(bm, d->prop_col.key[i], s);
+    acc += fx_from((double)S_srow * (fastlog(new_d_out(&f, s)) - fastlog(bm->d_out[s])));
+    acc += fx_from((double)S_scol * (fastlog(new_d_in(&f, s)) - fastlog(bm->d_in[s])));
     return fx_to(acc);
 }
 
 double orc_merge_delta_mdl(const orc_bm *bm, int32_t r, int32_t s) {
     delta_t d;
     merge_delta(bm, r, s, &d);
+    double dS = merge_dS_factored(bm, &d, bm->d_out[r], bm->d_in[r]);
+    delta_free(&d);
+    return dS;
+}
+
+/* entropy::block_merge_delta_mdl cell by cell; *abs_terms = sum of |cell terms|. */
+double orc_merge_delta_mdl_cells(const orc_bm *bm, int32_t r, int32_t s, double *abs_terms) {
+    delta_t d;
+    merge_delta(bm, r, s, &d);
+    g_abs_terms = 0.0;
     double dS = merge_dS_from(bm, &d, bm->d_out[r], bm->d_in[r]);
+    if (abs_terms) *abs_terms = g_abs_terms;
     delta_free(&d);
     return dS;
 }
@@ -1209,7 +1321,7 @@ orc_bm *orc_merge_apply(orc_bm *bm, const orc_graph *g, const int32_t *best_s, c
             int64_t k_out = bm->d_out[from], k_in = bm->d_in[from];
             delta_t d;
             merge_delta(bm, from, to, &d);
-            double actual = merge_dS_from(bm, &d, k_out, k_in);
+            double actual = merge_dS_factored(bm, &d, k_out, k_in);
             if (n > 0 && actual > heap[0].dS) {
                 m.dS = actual;
                 heap_push(heap, &n, m);

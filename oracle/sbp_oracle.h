@@ -24,6 +24,11 @@
  *     (double)hi + (double)lo * 2^-64.  Integer addition is associative, so
  *     the sum does not depend on the order of the terms.
  *   - logf is glibc's (the reference's fastlog / nat_log are logf).
+ *   - delta-MDL sums the Delta cells and the cells at rows/columns {r, s} term by term, and the remaining
+ *     unchanged cells of rows r, s / columns r, s in bulk: such a cell changes only through one block
+ *     degree d -> d', so its old-minus-new entropy is M * (fastlog(d') - fastlog(d)), and M summed over
+ *     them is a row/column total minus the other cells.  The per-cell form of the reference is kept as
+ *     orc_*_delta_mdl_cells and the tests bound the difference by 1e-12 of the sum of |cell terms|.
  *   - Heap ties in the merge carry-out are broken by ascending block id;
  *     the sample extension breaks plurality ties by the smallest block id;
  *     the vertex shuffle is the ascending order of Philox sort keys.
@@ -107,6 +112,9 @@ double   orc_null_mdl_v1(int64_t E);
 double   orc_vertex_delta_mdl(const orc_bm *bm, const orc_graph *g, int32_t v, int32_t s);
 double   orc_vertex_hastings(const orc_bm *bm, const orc_graph *g, int32_t v, int32_t s);
 double   orc_merge_delta_mdl(const orc_bm *bm, int32_t r, int32_t s);
+/* The reference's per-cell evaluation order (entropy.cpp); abs_terms = sum of |cell terms|. */
+double   orc_vertex_delta_mdl_cells(const orc_bm *bm, const orc_graph *g, int32_t v, int32_t s, double *abs_terms);
+double   orc_merge_delta_mdl_cells(const orc_bm *bm, int32_t r, int32_t s, double *abs_terms);
 
 /* Finetune */
 int      orc_ft_propose(const orc_bm *bm, const orc_graph *g, uint64_t seed, uint32_t epoch, int32_t v,

@@ -17,6 +17,7 @@ LIB = os.path.join(HERE, "libsbp_amd.so")
 SOURCES = ["csrc/build.hip", "csrc/mdl.hip", "csrc/finetune.hip", "csrc/merge.hip", "csrc/capi.hip",
            "host/sbp_api.cpp", "host/args.cpp"]
 CLI = os.path.join(HERE, "sbp_cli")
+CARRY_LIB = os.path.join(HERE, "libsbp_carry.so")  # host-only carry-out, for CPU-side checks
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SBP_OFFLOAD_ARCH", "gfx950")
@@ -52,6 +53,12 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"CLI link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(CLI + ".tmp", CLI)
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-o", CARRY_LIB + ".tmp",
+           os.path.join(HERE, "host", "carry_capi.cpp")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host carry-out build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(CARRY_LIB + ".tmp", CARRY_LIB)
     if verbose:
         print(f"built {LIB} and {CLI}")
     return LIB

@@ -92,24 +92,59 @@ void bm_alloc(sbp_bm bm, int32_t B, int32_t N) {
 
 void bm_release(sbp_bm bm) {
     void *ptrs[] = {bm->assign, bm->sizes, bm->d_out, bm->d_in, bm->d_tot, bm->row_ptr, bm->col_ptr, bm->row_cells,
-                    bm->col_cells};
+                    bm->col_cells, bm->row_cum, bm->col_cum};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     bm->assign = bm->sizes = bm->d_out = bm->d_in = bm->d_tot = bm->row_ptr = bm->col_ptr = nullptr;
     bm->row_cells = bm->col_cells = nullptr;
+    bm->row_cum = bm->col_cum = nullptr;
     bm->cap = 0;
     bm->bcap = 0;
 }
 
-static void ensure_cells(sbp_bm bm, int64_t nnz) {
+void bm_ensure_cells(sbp_bm bm, int64_t nnz) {
     if (nnz > bm->cap) {
-        if (bm->row_cells) SBP_HIP(hipFree(bm->row_cells));
-        if (bm->col_cells) SBP_HIP(hipFree(bm->col_cells));
+        void *old[] = {bm->row_cells, bm->col_cells, bm->row_cum, bm->col_cum};
+        for (void *p : old)
+            if (p) SBP_HIP(hipFree(p));
         int64_t cap = nnz + nnz / 8 + 64;
         SBP_HIP(hipMalloc(&bm->row_cells, sizeof(int2) * (size_t)cap));
         SBP_HIP(hipMalloc(&bm->col_cells, sizeof(int2) * (size_t)cap));
+        SBP_HIP(hipMalloc(&bm->row_cum, sizeof(int32_t) * (size_t)cap));
+        SBP_HIP(hipMalloc(&bm->col_cum, sizeof(int32_t) * (size_t)cap));
         bm->cap = cap;
     }
+}
+static void ensure_cells(sbp_bm bm, int64_t nnz) { bm_ensure_cells(bm, nnz); }
+
+// Inclusive cumulative counts per row (and per column without the diagonal cell): the multinomial over
+// neighbors_weights(t) then selects by binary search instead of a scan.  One wavefront per row.
+__global__ void k_cum(int32_t B, const int32_t *__restrict__ ptr, const int2 *__restrict__ cells, int skip_diag,
+                      int32_t *__restrict__ cum) {
+    const int lane = sbpdev::lane_id();
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / sbpdev::WAVE);
+    for (int64_t b = (int64_t)blockIdx.x * (blockDim.x / sbpdev::WAVE) + threadIdx.x / sbpdev::WAVE; b < B;
+         b += waves) {
+        int base = 0;
+        for (int c0 = ptr[b]; c0 < ptr[b + 1]; c0 += sbpdev::WAVE) {
+            int i = c0 + lane;
+            int x = 0;
+            if (i < ptr[b + 1]) x = (skip_diag && cells[i].x == (int)b) ? 0 : cells[i].y;
+            int incl = sbpdev::wave_inclusive_scan(x) + base;
+            if (i < ptr[b + 1]) cum[i] = incl;
+            base = __shfl(incl, sbpdev::WAVE - 1, sbpdev::WAVE);
+        }
+    }
+}
+
+void bm_compute_cum(sbp_ctx ctx, sbp_bm bm) {
+    if (bm->nnz == 0) return;
+    int64_t blocks = (bm->B + 3) / 4;
+    if (blocks > 16384) blocks = 16384;
+    if (blocks < 1) blocks = 1;
+    k_cum<<<(int)blocks, 256, 0, ctx->stream>>>(bm->B, bm->row_ptr, bm->row_cells, 0, bm->row_cum);
+    k_cum<<<(int)blocks, 256, 0, ctx->stream>>>(bm->B, bm->col_ptr, bm->col_cells, 1, bm->col_cum);
+    SBP_HIP(hipGetLastError());
 }
 
 void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm) {
@@ -167,6 +202,7 @@ void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm) {
                                                 nullptr, nullptr);
     k_dtot_sizes<<<grid_for(B), 256, 0, st>>>(B, bm->row_ptr, bm->row_cells, bm->d_out, bm->d_in, bm->d_tot);
     SBP_HIP(hipGetLastError());
+    bm_compute_cum(ctx, bm);
 }
 
 void bm_copy_into(sbp_ctx ctx, sbp_bm src, sbp_bm dst) {
@@ -185,6 +221,8 @@ void bm_copy_into(sbp_ctx ctx, sbp_bm src, sbp_bm dst) {
     if (src->nnz > 0) {
         SBP_HIP(hipMemcpyAsync(dst->row_cells, src->row_cells, sizeof(int2) * src->nnz, hipMemcpyDeviceToDevice, st));
         SBP_HIP(hipMemcpyAsync(dst->col_cells, src->col_cells, sizeof(int2) * src->nnz, hipMemcpyDeviceToDevice, st));
+        SBP_HIP(hipMemcpyAsync(dst->row_cum, src->row_cum, sizeof(int32_t) * src->nnz, hipMemcpyDeviceToDevice, st));
+        SBP_HIP(hipMemcpyAsync(dst->col_cum, src->col_cum, sizeof(int32_t) * src->nnz, hipMemcpyDeviceToDevice, st));
     }
     dst->overall_entropy = src->overall_entropy;
     dst->num_to_merge = 0;

@@ -37,7 +37,7 @@ using namespace sbpdev;
 
 constexpr int FT_BLOCK = 256;
 constexpr int FT_WAVES = FT_BLOCK / WAVE;
-constexpr int LDS_CAP = 1024;         // hash slots per wave
+constexpr int LDS_CAP = 512;          // hash slots per wave
 constexpr int SMALL_K = LDS_CAP / 2;  // max |out| + |in| for the LDS table
 constexpr int EMPTY = -1;
 
@@ -46,10 +46,14 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Open-addressing neighbour-block histogram.  Slots claimed for the current vertex are listed in `used`,
+// so clearing costs O(distinct neighbour blocks) rather than O(capacity).
 struct Table {
     int *keys;
     int *outc;
     int *inc;
+    int *used;
+    int *n_used;
     unsigned mask;
     int shift;  // 32 - log2(cap)
     int cap;
@@ -63,6 +67,7 @@ __device__ __forceinline__ void table_add(const Table &t, int key, int dout, int
     unsigned slot = hash_slot(t, key);
     while (true) {
         int prev = atomicCAS(&t.keys[slot], EMPTY, key);
+        if (prev == EMPTY) t.used[atomicAdd(t.n_used, 1)] = (int)slot;
         if (prev == EMPTY || prev == key) {
             if (dout) atomicAdd(&t.outc[slot], dout);
             if (din) atomicAdd(&t.inc[slot], din);
@@ -90,11 +95,37 @@ __device__ __forceinline__ void table_get(const Table &t, int key, int &o, int &
     }
 }
 
-// Read-only view of the block matrix.  Compact CSR: row b = [ptr[b], ptr[b+1]).  Slack layout (used by
-// Metropolis-Hastings): row b = [start[b], start[b] + len[b]).
+__device__ __forceinline__ void table_clear_all(const Table &t) {
+    for (int i = lane_id(); i < t.cap; i += WAVE) {
+        t.keys[i] = EMPTY;
+        t.outc[i] = 0;
+        t.inc[i] = 0;
+    }
+    if (lane_id() == 0) *t.n_used = 0;
+    wave_sync();
+}
+
+__device__ __forceinline__ void table_clear_used(const Table &t) {
+    wave_sync();
+    const int n = *t.n_used;
+    for (int i = lane_id(); i < n; i += WAVE) {
+        int slot = t.used[i];
+        t.keys[slot] = EMPTY;
+        t.outc[slot] = 0;
+        t.inc[slot] = 0;
+    }
+    wave_sync();
+    if (lane_id() == 0) *t.n_used = 0;
+    wave_sync();
+}
+
+// Read-only view of the block matrix.  Compact CSR: row b = [ptr[b], ptr[b+1]), with per-row inclusive
+// cumulative counts (columns: excluding the diagonal cell) for multinomial selection by binary search.
+// Slack layout (Metropolis-Hastings): row b = [start[b], start[b] + len[b]), no cumulative counts.
 struct BmView {
     const int32_t *rbeg, *rlen, *cbeg, *clen;  // rlen/clen null for compact CSR
     int2 *row_cells, *col_cells;
+    const int32_t *row_cum, *col_cum;          // null for the slack layout
     __device__ __forceinline__ int row_begin(int b) const { return rbeg[b]; }
     __device__ __forceinline__ int row_end(int b) const { return rlen ? rbeg[b] + rlen[b] : rbeg[b + 1]; }
     __device__ __forceinline__ int col_begin(int b) const { return cbeg[b]; }
@@ -116,9 +147,11 @@ struct FtArgs {
     double *dS_out;        // optional [N]
     int32_t *s_out;        // optional [N]
     unsigned long long *bytes;
-    int *gtable;           // high-degree tables: per wave 3 * 2^gcap_log2 ints
+    int *gtable;           // high-degree tables: per wave 4 * 2^gcap_log2 + 1 ints
     int gcap_log2;
     int *err;              // set when a multinomial runs past its cells (inconsistent degrees)
+    int32_t *big_queue;    // high-degree vertices deferred by the LDS-table launch
+    int32_t *big_count;
 };
 
 __device__ __forceinline__ long long get_cell(const FtArgs &a, int row, int col) {
@@ -149,17 +182,11 @@ __device__ __forceinline__ long long new_dout(const FtArgs &a, const MoveCtx &m,
     return v;
 }
 
-__device__ __forceinline__ void delta_cell_term(const FtArgs &a, const MoveCtx &m, int row, int col, long long d,
-                                                Fx &acc) {
-    long long M = get_cell(a, row, col);
+// old-minus-new entropy of one cell with current value M and change d
+__device__ __forceinline__ void cell_term(const FtArgs &a, const MoveCtx &m, int row, int col, long long M, long long d,
+                                          Fx &acc) {
     fx_acc(acc, cell_entropy(M, a.d_in[col], a.d_out[row]));
     fx_sub_acc(acc, cell_entropy(M + d, new_din(a, m, col), new_dout(a, m, row)));
-}
-
-__device__ __forceinline__ void unchanged_term(const FtArgs &a, const MoveCtx &m, int row, int col, long long M,
-                                               Fx &acc) {
-    fx_acc(acc, cell_entropy(M, a.d_in[col], a.d_out[row]));
-    fx_sub_acc(acc, cell_entropy(M, new_din(a, m, col), new_dout(a, m, row)));
 }
 
 // Delta of row r / row s at column c from the histogram (0 when the cell is not in the Delta).
@@ -170,24 +197,33 @@ __device__ __forceinline__ long long delta_row_s(const MoveCtx &m, int c, int oc
     return (long long)oc + (c == m.s ? (long long)m.selfc + m.in_s : 0) + (c == m.r ? -(long long)m.in_s : 0);
 }
 
-__device__ __forceinline__ Table make_table(const FtArgs &a, bool big, int *lds_keys, int *lds_out, int *lds_in,
-                                            int64_t gwave) {
+__device__ __forceinline__ Table make_table(const FtArgs &a, bool big, int *lds, int64_t gwave) {
     Table t;
     if (big) {
         t.cap = 1 << a.gcap_log2;
-        t.keys = a.gtable + gwave * 3 * (int64_t)t.cap;
-        t.outc = t.keys + t.cap;
-        t.inc = t.outc + t.cap;
+        t.keys = a.gtable + gwave * (4 * (int64_t)t.cap + 1);
         t.shift = 32 - a.gcap_log2;
     } else {
         t.cap = LDS_CAP;
-        t.keys = lds_keys;
-        t.outc = lds_out;
-        t.inc = lds_in;
-        t.shift = 32 - 10;
+        t.keys = lds;
+        t.shift = 32 - 9;
     }
+    t.outc = t.keys + t.cap;
+    t.inc = t.outc + t.cap;
+    t.used = t.inc + t.cap;
+    t.n_used = t.used + t.cap;
     t.mask = (unsigned)t.cap - 1;
     return t;
+}
+
+// First index in [lo, hi) whose inclusive cumulative count exceeds target.
+__device__ __forceinline__ int cum_search(const int32_t *cum, int lo, int hi, long long target) {
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if ((long long)cum[mid] <= target) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
 }
 
 // Proposal of vertex v against the current state.  Returns s (== r for "no move").  Fills m.
@@ -243,29 +279,41 @@ __device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w, unsigned
         }
         const int t = a.assign[u];
         m.t = t;
-        const long long total = a.d_tot[t];
-        const long long target = (long long)uniform_below(w.y, (uint32_t)total);
-        long long base_sum = 0;
+        // neighbors_weights(t): row(t) then column(t) without the diagonal; total = d_tot[t]
+        const long long target = (long long)uniform_below(w.y, (uint32_t)a.d_tot[t]);
         const int rp0 = a.m.row_begin(t), rp1 = a.m.row_end(t);
-        for (int c0 = rp0; c0 < rp1 && s < 0; c0 += WAVE) {
-            int ci = c0 + lane;
-            int2 cell = ci < rp1 ? a.m.row_cells[ci] : make_int2(0, 0);
-            long long incl = wave_inclusive_scan((long long)cell.y) + base_sum;
-            unsigned long long hit = ballot(ci < rp1 && target < incl);
-            if (hit) s = __shfl(cell.x, __ffsll((long long)hit) - 1, WAVE);
-            else base_sum = __shfl(incl, WAVE - 1, WAVE);
-        }
         const int cp0 = a.m.col_begin(t), cp1 = a.m.col_end(t);
-        for (int c0 = cp0; c0 < cp1 && s < 0; c0 += WAVE) {
-            int ci = c0 + lane;
-            int2 cell = ci < cp1 ? a.m.col_cells[ci] : make_int2(0, 0);
-            bool ok = ci < cp1 && cell.x != t;
-            long long incl = wave_inclusive_scan(ok ? (long long)cell.y : 0ll) + base_sum;
-            unsigned long long hit = ballot(ok && target < incl);
-            if (hit) s = __shfl(cell.x, __ffsll((long long)hit) - 1, WAVE);
-            else base_sum = __shfl(incl, WAVE - 1, WAVE);
+        if (a.m.row_cum) {
+            const long long dout_t = a.d_out[t];
+            if (target < dout_t) {
+                int i = cum_search(a.m.row_cum, rp0, rp1, target);
+                if (i < rp1) s = a.m.row_cells[i].x;
+            } else {
+                int i = cum_search(a.m.col_cum, cp0, cp1, target - dout_t);
+                if (i < cp1) s = a.m.col_cells[i].x;
+            }
+            bytes += 8ull * 2 * 12;  // two binary searches over cumulative counts
+        } else {
+            long long base_sum = 0;
+            for (int c0 = rp0; c0 < rp1 && s < 0; c0 += WAVE) {
+                int ci = c0 + lane;
+                int2 cell = ci < rp1 ? a.m.row_cells[ci] : make_int2(0, 0);
+                long long incl = wave_inclusive_scan((long long)cell.y) + base_sum;
+                unsigned long long hit = ballot(ci < rp1 && target < incl);
+                if (hit) s = __shfl(cell.x, __ffsll((long long)hit) - 1, WAVE);
+                else base_sum = __shfl(incl, WAVE - 1, WAVE);
+            }
+            for (int c0 = cp0; c0 < cp1 && s < 0; c0 += WAVE) {
+                int ci = c0 + lane;
+                int2 cell = ci < cp1 ? a.m.col_cells[ci] : make_int2(0, 0);
+                bool ok = ci < cp1 && cell.x != t;
+                long long incl = wave_inclusive_scan(ok ? (long long)cell.y : 0ll) + base_sum;
+                unsigned long long hit = ballot(ok && target < incl);
+                if (hit) s = __shfl(cell.x, __ffsll((long long)hit) - 1, WAVE);
+                else base_sum = __shfl(incl, WAVE - 1, WAVE);
+            }
+            bytes += 8ull * (unsigned long long)((rp1 - rp0) + (cp1 - cp0));
         }
-        bytes += 8ull * (unsigned long long)((rp1 - rp0) + (cp1 - cp0));
         if (s < 0) {  // d_tot[t] disagrees with the cells of row/column t: report, propose no move
             if (lane == 0) atomicExch(a.err, 1);
             s = m.r;
@@ -276,17 +324,18 @@ __device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w, unsigned
     return s;
 }
 
-// Builds the neighbour-block histogram and returns delta-MDL; sets *accept.  Table stays populated.
+// Builds the neighbour-block histogram and returns delta-MDL; sets accept.  The table stays populated
+// for the caller (Metropolis-Hastings applies the Delta from it), which clears it.
+//
+// delta-MDL = the Delta entries term by term + the cells (r,r), (r,s), (s,r), (s,s) that are unchanged,
+// term by term + the other unchanged cells of rows r, s and columns r, s in bulk: each changes only
+// through one block degree d -> d', contributing M * (fastlog(d') - fastlog(d)), so the four bulk terms
+// are (row or column total - Delta and special cells) * (fastlog(d') - fastlog(d)).
+template <bool GREEDY>
 __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept,
                                 unsigned long long &bytes) {
     const int lane = lane_id();
     const int v = m.v, r = m.r, s = m.s;
-    for (int i = lane; i < t.cap; i += WAVE) {
-        t.keys[i] = EMPTY;
-        t.outc[i] = 0;
-        t.inc[i] = 0;
-    }
-    wave_sync();
     for (int64_t e = m.o0 + lane; e < m.o1; e += WAVE) {
         int u = a.out_adj[e];
         if (u != v) table_add(t, a.assign[u], 1, 0);
@@ -298,85 +347,84 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
     wave_sync();
     table_get(t, r, m.out_r, m.in_r);
     table_get(t, s, m.out_s, m.in_s);
+    const long long mrr = get_cell(a, r, r), mrs = get_cell(a, r, s), msr = get_cell(a, s, r),
+                    mss = get_cell(a, s, s);
     Fx acc = fx_zero();
-    int nb = 0;
-    for (int i = lane; i < t.cap; i += WAVE) {
-        int c = t.keys[i];
-        if (c == EMPTY) continue;
-        nb++;
-        int oc = t.outc[i], ic = t.inc[i];
+    long long sub_rrow = 0, sub_srow = 0, sub_rcol = 0, sub_scol = 0;
+    const int nb = *t.n_used;
+    for (int i = lane; i < nb; i += WAVE) {
+        const int slot = t.used[i];
+        const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
+        const bool special = (c == r) || (c == s);
         if (oc > 0) {
-            delta_cell_term(a, m, r, c, delta_row_r(m, c, oc), acc);
-            delta_cell_term(a, m, s, c, delta_row_s(m, c, oc), acc);
+            long long mrc = c == r ? mrr : (c == s ? mrs : get_cell(a, r, c));
+            long long msc = c == r ? msr : (c == s ? mss : get_cell(a, s, c));
+            cell_term(a, m, r, c, mrc, delta_row_r(m, c, oc), acc);
+            cell_term(a, m, s, c, msc, delta_row_s(m, c, oc), acc);
+            if (!special) {
+                sub_rrow += mrc;
+                sub_srow += msc;
+            }
         }
-        if (ic > 0 && c != r && c != s) {
-            delta_cell_term(a, m, c, r, -(long long)ic, acc);
-            delta_cell_term(a, m, c, s, (long long)ic, acc);
+        if (ic > 0 && !special) {
+            long long mcr = get_cell(a, c, r), mcs = get_cell(a, c, s);
+            cell_term(a, m, c, r, mcr, -(long long)ic, acc);
+            cell_term(a, m, c, s, mcs, (long long)ic, acc);
+            sub_rcol += mcr;
+            sub_scol += mcs;
         }
     }
     if (lane == 0) {
-        if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0)) delta_cell_term(a, m, r, r, -(long long)m.selfc - m.in_r, acc);
-        if (m.out_s == 0 && m.in_r > 0) delta_cell_term(a, m, r, s, (long long)m.in_r, acc);
-        if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) delta_cell_term(a, m, s, s, (long long)m.selfc + m.in_s, acc);
-        if (m.out_r == 0 && m.in_s > 0) delta_cell_term(a, m, s, r, -(long long)m.in_s, acc);
-    }
-    const int rr0 = a.m.row_begin(r), rr1 = a.m.row_end(r), rs0 = a.m.row_begin(s), rs1 = a.m.row_end(s);
-    const int cr0 = a.m.col_begin(r), cr1 = a.m.col_end(r), cs0 = a.m.col_begin(s), cs1 = a.m.col_end(s);
-    for (int i = rr0 + lane; i < rr1; i += WAVE) {
-        int2 cell = a.m.row_cells[i];
-        int oc, ic;
-        table_get(t, cell.x, oc, ic);
-        if (delta_row_r(m, cell.x, oc) != 0) continue;
-        unchanged_term(a, m, r, cell.x, cell.y, acc);
-    }
-    for (int i = rs0 + lane; i < rs1; i += WAVE) {
-        int2 cell = a.m.row_cells[i];
-        int oc, ic;
-        table_get(t, cell.x, oc, ic);
-        if (delta_row_s(m, cell.x, oc) != 0) continue;
-        unchanged_term(a, m, s, cell.x, cell.y, acc);
-    }
-    for (int i = cr0 + lane; i < cr1; i += WAVE) {
-        int2 cell = a.m.col_cells[i];
-        if (cell.x == r || cell.x == s) continue;
-        int oc, ic;
-        table_get(t, cell.x, oc, ic);
-        if (ic != 0) continue;
-        unchanged_term(a, m, cell.x, r, cell.y, acc);
-    }
-    for (int i = cs0 + lane; i < cs1; i += WAVE) {
-        int2 cell = a.m.col_cells[i];
-        if (cell.x == r || cell.x == s) continue;
-        int oc, ic;
-        table_get(t, cell.x, oc, ic);
-        if (ic != 0) continue;
-        unchanged_term(a, m, cell.x, s, cell.y, acc);
+        // Delta entries not produced by an out-neighbour block
+        if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0)) cell_term(a, m, r, r, mrr, -(long long)m.selfc - m.in_r, acc);
+        if (m.out_s == 0 && m.in_r > 0) cell_term(a, m, r, s, mrs, (long long)m.in_r, acc);
+        if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) cell_term(a, m, s, s, mss, (long long)m.selfc + m.in_s, acc);
+        if (m.out_r == 0 && m.in_s > 0) cell_term(a, m, s, r, msr, -(long long)m.in_s, acc);
+        // unchanged cells at rows/columns {r, s} (a zero-net Delta entry counts here as well)
+        if (mrr && delta_row_r(m, r, m.out_r) == 0) cell_term(a, m, r, r, mrr, 0, acc);
+        if (mrs && delta_row_r(m, s, m.out_s) == 0) cell_term(a, m, r, s, mrs, 0, acc);
+        if (msr && delta_row_s(m, r, m.out_r) == 0) cell_term(a, m, s, r, msr, 0, acc);
+        if (mss && delta_row_s(m, s, m.out_s) == 0) cell_term(a, m, s, s, mss, 0, acc);
     }
     acc = wave_sum_fx(acc);
-    nb = wave_sum(nb);
+    sub_rrow = wave_sum(sub_rrow);
+    sub_srow = wave_sum(sub_srow);
+    sub_rcol = wave_sum(sub_rcol);
+    sub_scol = wave_sum(sub_scol);
+    const long long S_rrow = (long long)a.d_out[r] - mrr - mrs - sub_rrow;
+    const long long S_srow = (long long)a.d_out[s] - mss - msr - sub_srow;
+    const long long S_rcol = (long long)a.d_in[r] - mrr - msr - sub_rcol;
+    const long long S_scol = (long long)a.d_in[s] - mss - mrs - sub_scol;
+    fx_acc(acc, (double)S_rrow * (fastlog(new_dout(a, m, r)) - fastlog(a.d_out[r])));
+    fx_acc(acc, (double)S_srow * (fastlog(new_dout(a, m, s)) - fastlog(a.d_out[s])));
+    fx_acc(acc, (double)S_rcol * (fastlog(new_din(a, m, r)) - fastlog(a.d_in[r])));
+    fx_acc(acc, (double)S_scol * (fastlog(new_din(a, m, s)) - fastlog(a.d_in[s])));
     const double dS = fx_to(acc);
-    const long long rc_r = (rr1 - rr0) + (cr1 - cr0), rc_s = (rs1 - rs0) + (cs1 - cs0);
-    bytes += 12ull * (unsigned long long)(rc_r + rc_s) + 16ull * (unsigned long long)nb;
-    if (a.greedy) {
+    // 2 neighbour-block keys + 4 lookups of ~log2(row) probes per Delta cell, 8 per lookup probe
+    bytes += 16ull * (unsigned long long)nb + 8ull * 4 * 10 * (unsigned long long)nb + 64;
+    if (GREEDY) {
         accept = dS < 0.0;
         return dS;
     }
     // Hastings correction over the neighbour blocks (entropy.cpp:497-561)
     Fx fwd = fx_zero(), bwd = fx_zero();
     const long long Bn = a.B;
-    for (int i = lane; i < t.cap + 1; i += WAVE) {
-        int x, cntx;
-        if (i < t.cap) {
-            x = t.keys[i];
-            if (x == EMPTY) continue;
-            cntx = t.outc[i] + t.inc[i] + (x == r ? m.selfc : 0);
+    const bool self_only_r = m.selfc > 0 && m.out_r == 0 && m.in_r == 0;
+    for (int i = lane; i < nb + 1; i += WAVE) {
+        int x, cntx, oc, ic;
+        if (i < nb) {
+            const int slot = t.used[i];
+            x = t.keys[slot];
+            oc = t.outc[slot];
+            ic = t.inc[slot];
+            cntx = oc + ic + (x == r ? m.selfc : 0);
         } else {
-            if (!(m.selfc > 0 && m.out_r == 0 && m.in_r == 0)) continue;  // self-loop is the only link to r
+            if (!self_only_r) continue;  // the self-loop is the only link to block r
             x = r;
+            oc = 0;
+            ic = 0;
             cntx = m.selfc;
         }
-        int oc, ic;
-        table_get(t, x, oc, ic);
         double counts = (double)cntx;
         double pw = (double)(get_cell(a, s, x) + get_cell(a, x, s)) + 1.0;
         double bd = (double)((long long)a.d_tot[x] + Bn);
@@ -385,10 +433,10 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
         double bw = (double)(get_cell(a, r, x) + d_rx + get_cell(a, x, r) + d_xr) + 1.0;
         long long nbd = a.d_tot[x];
         if (x == r) {
-            long long self = get_cell(a, r, r) + delta_row_r(m, r, m.out_r);
+            long long self = mrr + delta_row_r(m, r, m.out_r);
             nbd = (a.d_out[r] - m.k_out) + (a.d_in[r] - m.k_in) - self;
         } else if (x == s) {
-            long long self = get_cell(a, s, s) + delta_row_s(m, s, m.out_s);
+            long long self = mss + delta_row_s(m, s, m.out_s);
             nbd = (a.d_out[s] + m.k_out) + (a.d_in[s] + m.k_in) - self;
         }
         double pd = (double)(nbd + Bn);
@@ -402,25 +450,29 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
     double p = exp(-3.0 * dS) * H;
     p = p >= 1.0 ? 1.0 : p;
     accept = u <= p;
-    bytes += 8ull * (unsigned long long)rc_s + 8ull * (unsigned long long)nb;
+    bytes += 8ull * 6 * 10 * (unsigned long long)nb;
     return dS;
 }
 
-template <bool BIG>
+template <bool BIG, bool GREEDY>
 __global__ void __launch_bounds__(FT_BLOCK) k_ft_propose(FtArgs a) {
-    __shared__ int lds_keys[BIG ? 1 : FT_WAVES * LDS_CAP];
-    __shared__ int lds_out[BIG ? 1 : FT_WAVES * LDS_CAP];
-    __shared__ int lds_in[BIG ? 1 : FT_WAVES * LDS_CAP];
+    __shared__ int lds[BIG ? 1 : FT_WAVES * (4 * LDS_CAP + 1)];
     const int lane = lane_id(), wid = threadIdx.x / WAVE;
     const int64_t gwave = (int64_t)blockIdx.x * FT_WAVES + wid;
     const int64_t nwaves = (int64_t)gridDim.x * FT_WAVES;
-    const Table t = make_table(a, BIG, lds_keys + (BIG ? 0 : wid * LDS_CAP), lds_out + (BIG ? 0 : wid * LDS_CAP),
-                               lds_in + (BIG ? 0 : wid * LDS_CAP), gwave);
+    const Table t = make_table(a, BIG, lds + (BIG ? 0 : wid * (4 * LDS_CAP + 1)), gwave);
+    table_clear_all(t);
+    const int64_t count = BIG ? (int64_t)*a.big_count : a.count;
     unsigned long long bytes = 0;
-    for (int64_t item = gwave; item < a.count; item += nwaves) {
-        const int v = a.items[item];
-        const long long kall = (a.out_off[v + 1] - a.out_off[v]) + (a.in_off[v + 1] - a.in_off[v]);
-        if ((kall > SMALL_K) != BIG) continue;
+    for (int64_t item = gwave; item < count; item += nwaves) {
+        const int v = BIG ? a.big_queue[item] : a.items[item];
+        if (!BIG) {
+            const long long kall = (a.out_off[v + 1] - a.out_off[v]) + (a.in_off[v + 1] - a.in_off[v]);
+            if (kall > SMALL_K) {  // deferred to the global-memory-table launch
+                if (lane == 0) a.big_queue[atomicAdd(a.big_count, 1)] = v;
+                continue;
+            }
+        }
         MoveCtx m;
         U4 w;
         const int s = propose_block(a, v, m, w, bytes);
@@ -433,12 +485,12 @@ __global__ void __launch_bounds__(FT_BLOCK) k_ft_propose(FtArgs a) {
             continue;
         }
         bool accept;
-        const double dS = evaluate_move(a, m, t, w, accept, bytes);
+        const double dS = evaluate_move<GREEDY>(a, m, t, w, accept, bytes);
         if (lane == 0) {
             a.move_s[v] = accept ? s : -1;
             if (a.dS_out) a.dS_out[v] = dS;
         }
-        wave_sync();
+        table_clear_used(t);
     }
     if (lane == 0 && bytes) atomicAdd(a.bytes, bytes);
 }
@@ -471,6 +523,8 @@ static FtArgs base_args(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     a.m.clen = nullptr;
     a.m.row_cells = bm->row_cells;
     a.m.col_cells = bm->col_cells;
+    a.m.row_cum = bm->row_cum;
+    a.m.col_cum = bm->col_cum;
     a.items = nullptr;
     a.count = 0;
     a.move_s = nullptr;
@@ -480,6 +534,8 @@ static FtArgs base_args(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     a.gtable = nullptr;
     a.gcap_log2 = 0;
     a.err = nullptr;
+    a.big_queue = nullptr;
+    a.big_count = nullptr;
     return a;
 }
 
@@ -488,33 +544,37 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     hipStream_t st = ctx->stream;
     FtLaunchInfo info{0, count, 0.f};
     if (count == 0) return info;
-    auto *bytes = ctx->scratch[S_COUNTER].as<unsigned long long>(4);
-    SBP_HIP(hipMemsetAsync(bytes, 0, 4 * sizeof(unsigned long long), st));
+    auto *counters = ctx->scratch[S_COUNTER].as<unsigned long long>(4);
+    SBP_HIP(hipMemsetAsync(counters, 0, 4 * sizeof(unsigned long long), st));
     FtArgs a = base_args(ctx, bm, g, epoch, greedy);
-    a.err = reinterpret_cast<int *>(bytes + 1);
+    a.bytes = counters;
+    a.err = reinterpret_cast<int *>(counters + 1);
+    a.big_count = reinterpret_cast<int32_t *>(counters + 2);
+    a.big_queue = ctx->scratch[S_EVENTS_B].as<int32_t>((size_t)g->N + 1);
     a.items = d_vertices;
     a.count = count;
     a.move_s = d_move_s;
     a.dS_out = d_dS;
     a.s_out = d_s_out;
-    a.bytes = bytes;
     int64_t blocks = (count + FT_WAVES - 1) / FT_WAVES;
-    if (blocks > 8192) blocks = 8192;
+    if (blocks > 4096) blocks = 4096;
     SBP_HIP(hipEventRecord(ctx->ev[0], st));
-    k_ft_propose<false><<<(int)blocks, FT_BLOCK, 0, st>>>(a);
+    if (greedy) k_ft_propose<false, true><<<(int)blocks, FT_BLOCK, 0, st>>>(a);
+    else k_ft_propose<false, false><<<(int)blocks, FT_BLOCK, 0, st>>>(a);
     SBP_HIP(hipGetLastError());
     if (g->max_degree > SMALL_K) {
         int lg = log2_ceil(2 * g->max_degree + 2);
-        if (lg < 11) lg = 11;
-        const int64_t big_blocks = 64;
-        a.gtable = ctx->scratch[S_HASH].as<int>((size_t)big_blocks * FT_WAVES * 3 * ((size_t)1 << lg));
+        if (lg < 10) lg = 10;
+        const int64_t big_blocks = 32;
+        a.gtable = ctx->scratch[S_HASH].as<int>((size_t)big_blocks * FT_WAVES * (4 * ((size_t)1 << lg) + 1));
         a.gcap_log2 = lg;
-        k_ft_propose<true><<<(int)big_blocks, FT_BLOCK, 0, st>>>(a);
+        if (greedy) k_ft_propose<true, true><<<(int)big_blocks, FT_BLOCK, 0, st>>>(a);
+        else k_ft_propose<true, false><<<(int)big_blocks, FT_BLOCK, 0, st>>>(a);
         SBP_HIP(hipGetLastError());
     }
     SBP_HIP(hipEventRecord(ctx->ev[1], st));
     unsigned long long hb[2] = {0, 0};
-    SBP_HIP(hipMemcpyAsync(hb, bytes, sizeof(hb), hipMemcpyDeviceToHost, st));
+    SBP_HIP(hipMemcpyAsync(hb, counters, sizeof(hb), hipMemcpyDeviceToHost, st));
     SBP_HIP(hipStreamSynchronize(st));
     SBP_CHECK(hb[1] == 0, SBP_E_STATE, "finetune proposal: block degrees inconsistent with the block matrix");
     info.bytes = hb[0];
@@ -769,64 +829,68 @@ __device__ __forceinline__ bool lacks_room(const FtArgs &a, const MhState &h, in
     return h.rlen[row] >= h.rcap[row] || h.clen[col] >= h.ccap[col];
 }
 
+template <bool GREEDY>
 __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start) {
-    __shared__ int lds_keys[LDS_CAP];
-    __shared__ int lds_out[LDS_CAP];
-    __shared__ int lds_in[LDS_CAP];
+    __shared__ int lds[4 * LDS_CAP + 1];
     const int lane = lane_id();
     unsigned long long bytes = 0, moves = 0;
     Fx sum = fx_zero();
     int32_t status = 0, stop = (int32_t)a.count;
+    const Table small_t = make_table(a, false, lds, 0);
+    const Table big_t = make_table(a, true, lds, 0);
+    table_clear_all(small_t);
+    if (a.gtable) table_clear_all(big_t);
     for (int64_t item = start; item < a.count; ++item) {
         const int v = a.items[item];
         const int r0 = a.assign[v];
         if (a.sizes[r0] == 1) continue;  // propose_move skips singleton blocks (src/finetune.cpp:845-847)
         const long long kall = (a.out_off[v + 1] - a.out_off[v]) + (a.in_off[v + 1] - a.in_off[v]);
-        const bool big = kall > SMALL_K;
-        const Table t = make_table(a, big, lds_keys, lds_out, lds_in, 0);
+        const Table &t = kall > SMALL_K ? big_t : small_t;
         MoveCtx m;
         U4 w;
         const int s = propose_block(a, v, m, w, bytes);
         if (s == m.r) continue;
         bool accept;
-        const double dS = evaluate_move(a, m, t, w, accept, bytes);
-        if (!accept) continue;
+        const double dS = evaluate_move<GREEDY>(a, m, t, w, accept, bytes);
+        if (!accept) {
+            table_clear_used(t);
+            continue;
+        }
         const int r = m.r;
+        const int nb = *t.n_used;
         // ---- room check for every Delta cell that would create a new entry
         bool short_room = false;
-        for (int i = lane; i < t.cap; i += WAVE) {
-            int c = t.keys[i];
-            if (c == EMPTY) continue;
-            int oc = t.outc[i], ic = t.inc[i];
-            if (oc > 0) short_room |= lacks_room(a, h, s, c, delta_row_s(m, c, oc));
-            if (ic > 0 && c != r && c != s) short_room |= lacks_room(a, h, c, s, ic);
+        int new_row_s = 0, new_col_s = 0;
+        for (int i = lane; i < nb; i += WAVE) {
+            const int slot = t.used[i];
+            const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
+            if (oc > 0) {
+                short_room |= lacks_room(a, h, s, c, delta_row_s(m, c, oc));
+                if (delta_row_s(m, c, oc) > 0 && get_cell(a, s, c) == 0) new_row_s++;
+            }
+            if (ic > 0 && c != r && c != s) {
+                short_room |= lacks_room(a, h, c, s, ic);
+                if (get_cell(a, c, s) == 0) new_col_s++;
+            }
         }
         if (lane == 0) {
             if (m.out_s == 0 && m.in_r > 0) short_room |= lacks_room(a, h, r, s, m.in_r);
             if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) short_room |= lacks_room(a, h, s, s, m.selfc + m.in_s);
         }
         // row s and column s may receive several new cells: check their totals
-        int new_row_s = 0, new_col_s = 0;
-        for (int i = lane; i < t.cap; i += WAVE) {
-            int c = t.keys[i];
-            if (c == EMPTY) continue;
-            int oc = t.outc[i], ic = t.inc[i];
-            if (oc > 0 && delta_row_s(m, c, oc) > 0 && get_cell(a, s, c) == 0) new_row_s++;
-            if (ic > 0 && c != r && c != s && get_cell(a, c, s) == 0) new_col_s++;
-        }
         new_row_s = wave_sum(new_row_s) + 2;
         new_col_s = wave_sum(new_col_s) + 2;
         short_room |= (h.rlen[s] + new_row_s > h.rcap[s]) || (h.clen[s] + new_col_s > h.ccap[s]);
         if (ballot(short_room)) {
+            table_clear_used(t);
             status = 1;
             stop = (int32_t)item;
             break;
         }
         // ---- apply the Delta in place (update_edge_counts(delta), dict_transpose_matrix.cpp:430-446)
-        for (int i = 0; i < t.cap; ++i) {  // sequential over slots: each cell update is wave-cooperative
-            int c = t.keys[i];
-            if (c == EMPTY) continue;
-            int oc = t.outc[i], ic = t.inc[i];
+        for (int i = 0; i < nb; ++i) {  // one cell at a time: each update is wave-cooperative
+            const int slot = t.used[i];
+            const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
             if (oc > 0) {
                 long long dr = delta_row_r(m, c, oc), ds = delta_row_s(m, c, oc);
                 if (dr) apply_cell(a, h, r, c, (int)dr);
@@ -841,7 +905,7 @@ __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start)
         if (m.out_s == 0 && m.in_r > 0) apply_cell(a, h, r, s, m.in_r);
         if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) apply_cell(a, h, s, s, m.selfc + m.in_s);
         if (m.out_r == 0 && m.in_s > 0) apply_cell(a, h, s, r, -m.in_s);
-        wave_sync();
+        table_clear_used(t);
         if (lane == 0) {
             a.d_out[r] -= (int)m.k_out;
             a.d_out[s] += (int)m.k_out;
@@ -958,11 +1022,14 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         a.bytes = bytes;
         a.err = reinterpret_cast<int *>(bytes + 1);
         int lg = log2_ceil(2 * g->max_degree + 2);
-        if (lg < 11) lg = 11;
-        a.gtable = ctx->scratch[S_HASH].as<int>(3 * ((size_t)1 << lg));
+        if (lg < 10) lg = 10;
+        a.gtable = g->max_degree > SMALL_K ? ctx->scratch[S_HASH].as<int>(4 * ((size_t)1 << lg) + 1) : nullptr;
         a.gcap_log2 = lg;
+        a.m.row_cum = nullptr;
+        a.m.col_cum = nullptr;
         SBP_HIP(hipEventRecord(ctx->ev[4], st));
-        k_mh<<<1, WAVE, 0, st>>>(a, h, start);
+        if (p.greedy) k_mh<true><<<1, WAVE, 0, st>>>(a, h, start);
+        else k_mh<false><<<1, WAVE, 0, st>>>(a, h, start);
         SBP_HIP(hipGetLastError());
         SBP_HIP(hipEventRecord(ctx->ev[5], st));
         // compact the slack layout back into the blockmodel's CSR (row and column counts changed)
@@ -981,17 +1048,11 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         SBP_HIP(hipStreamSynchronize(st));
         kernel_ms += (float)elapsed_ms(ctx, ctx->ev[4], ctx->ev[5]);
         SBP_CHECK(nnz[0] == nnz[1], SBP_E_STATE, "Metropolis-Hastings: row and column cell counts diverged");
-        if (nnz[0] > bm->cap) {
-            // more cells than the compact arrays hold: grow them
-            (void)hipFree(bm->row_cells);
-            (void)hipFree(bm->col_cells);
-            bm->cap = nnz[0] + nnz[0] / 4 + 64;
-            SBP_HIP(hipMalloc(&bm->row_cells, sizeof(int2) * bm->cap));
-            SBP_HIP(hipMalloc(&bm->col_cells, sizeof(int2) * bm->cap));
-        }
+        bm_ensure_cells(bm, nnz[0]);  // more cells than the compact arrays hold: grow them
         k_compact_copy<<<grid_of((int64_t)B * 64), 64, 0, st>>>(B, h.rstart, h.rlen, rc, rp, bm->row_cells);
         k_compact_copy<<<grid_of((int64_t)B * 64), 64, 0, st>>>(B, h.cstart, h.clen, cc, cp, bm->col_cells);
         bm->nnz = nnz[0];
+        bm_compute_cum(ctx, bm);
         SBP_HIP(hipStreamSynchronize(st));
         if (hs[1] == 0) break;
         start = hs[0];  // resume at the vertex that ran out of room, with more slack

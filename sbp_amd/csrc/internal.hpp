@@ -99,6 +99,8 @@ struct sbp_bm_s {
     int32_t *col_ptr = nullptr;  // [B+1]
     int2 *row_cells = nullptr;   // [cap] {col, count}, ascending col per row
     int2 *col_cells = nullptr;   // [cap] {row, count}, ascending row per column
+    int32_t *row_cum = nullptr;  // [cap] inclusive cumulative counts within each row
+    int32_t *col_cum = nullptr;  // [cap] inclusive cumulative counts within each column, diagonal excluded
     int32_t bcap = 0;            // capacity of the per-block arrays
     double overall_entropy = 1.7976931348623157e308;
     int64_t num_to_merge = 0;
@@ -111,6 +113,8 @@ void bm_alloc(sbp_bm bm, int32_t B, int32_t N);
 void bm_release(sbp_bm bm);
 void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm);  // from bm->assign (device), bm->B set
 void bm_copy_into(sbp_ctx ctx, sbp_bm src, sbp_bm dst);
+void bm_compute_cum(sbp_ctx ctx, sbp_bm bm);
+void bm_ensure_cells(sbp_bm bm, int64_t nnz);
 // mdl.hip
 double log_posterior(sbp_ctx ctx, sbp_bm bm);
 double model_term(int64_t E, int32_t N, int32_t B);

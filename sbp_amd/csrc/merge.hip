@@ -55,7 +55,11 @@ __device__ __forceinline__ int wave_select(const int2 *cells, int begin, int end
     return -1;
 }
 
-// delta-MDL of merging block r into block s (entropy::block_merge_delta_mdl, src/entropy.cpp:129-182).
+// delta-MDL of merging block r into block s (entropy::block_merge_delta_mdl, src/entropy.cpp:129-182):
+// the Delta entries term by term (row r and column r move into s), cell (s, s) when it is unchanged, and
+// the other unchanged cells of row s / column s in bulk -- such a cell changes only through d_out[s] or
+// d_in[s], contributing M * (fastlog(d') - fastlog(d)), and M summed over them is the row/column total minus
+// the cells the merge touches.  Work is O(|row r| + |col r|) lookups; row s and column s are not scanned.
 __device__ double merge_delta_mdl(const MgArgs &a, int r, int s) {
     const int lane = lane_id();
     const long long k_out = a.d_out[r], k_in = a.d_in[r];
@@ -63,8 +67,8 @@ __device__ double merge_delta_mdl(const MgArgs &a, int r, int s) {
     const int cr0 = a.col_ptr[r], cr1 = a.col_ptr[r + 1], cs0 = a.col_ptr[s], cs1 = a.col_ptr[s + 1];
     const long long din_s_new = (long long)a.d_in[s] + k_in, dout_s_new = (long long)a.d_out[s] + k_out;
     Fx acc = fx_zero();
-    // Delta entries in row r: (r, c) old terms; (s, c) for c not in {r, s}
-    for (int i = rr0 + lane; i < rr1; i += WAVE) {
+    long long sub_row = 0, sub_col = 0;
+    for (int i = rr0 + lane; i < rr1; i += WAVE) {  // (r, c) old terms; (s, c) for c outside {r, s}
         int2 cell = a.row_cells[i];
         int c = cell.x;
         fx_acc(acc, cell_entropy(cell.y, a.d_in[c], a.d_out[r]));
@@ -72,10 +76,10 @@ __device__ double merge_delta_mdl(const MgArgs &a, int r, int s) {
             long long msc = cell_lookup(a.row_cells, rs0, rs1, c);
             fx_acc(acc, cell_entropy(msc, a.d_in[c], a.d_out[s]));
             fx_sub_acc(acc, cell_entropy(msc + cell.y, a.d_in[c], dout_s_new));
+            sub_row += msc;
         }
     }
-    // Delta entries in column r: (x, r) old terms; (x, s) for x not in {r, s}
-    for (int i = cr0 + lane; i < cr1; i += WAVE) {
+    for (int i = cr0 + lane; i < cr1; i += WAVE) {  // (x, r) old terms; (x, s) for x outside {r, s}
         int2 cell = a.col_cells[i];
         int x = cell.x;
         if (x == r) continue;
@@ -84,36 +88,26 @@ __device__ double merge_delta_mdl(const MgArgs &a, int r, int s) {
             long long mxs = cell_lookup(a.col_cells, cs0, cs1, x);
             fx_acc(acc, cell_entropy(mxs, a.d_in[s], a.d_out[x]));
             fx_sub_acc(acc, cell_entropy(mxs + cell.y, din_s_new, a.d_out[x]));
+            sub_col += mxs;
         }
     }
-    // (s, s): receives M[r][r] + M[r][s] + M[s][r]
     const long long mrr = mg_get(a, r, r), mrs = mg_get(a, r, s), msr = mg_get(a, s, r), mss = mg_get(a, s, s);
-    const long long dss = mrr + mrs + msr;
-    if (lane == 0 && dss > 0) {
+    const long long dss = mrr + mrs + msr;  // what (s, s) receives
+    acc = wave_sum_fx(acc);
+    sub_row = wave_sum(sub_row);
+    sub_col = wave_sum(sub_col);
+    if (dss > 0) {
         fx_acc(acc, cell_entropy(mss, a.d_in[s], a.d_out[s]));
         fx_sub_acc(acc, cell_entropy(mss + dss, din_s_new, dout_s_new));
+    } else if (mss > 0) {
+        fx_acc(acc, cell_entropy(mss, a.d_in[s], a.d_out[s]));
+        fx_sub_acc(acc, cell_entropy(mss, din_s_new, dout_s_new));
     }
-    // unchanged cells of row s
-    for (int i = rs0 + lane; i < rs1; i += WAVE) {
-        int2 cell = a.row_cells[i];
-        int c = cell.x;
-        if (c == r) continue;                 // delta (s, r) = -M[s][r] != 0
-        if (c == s && dss > 0) continue;      // (s, s) is a delta entry
-        if (c != s && cell_lookup(a.row_cells, rr0, rr1, c) != 0) continue;  // (s, c) receives M[r][c]
-        long long din_new = (c == s) ? din_s_new : (long long)a.d_in[c];
-        fx_acc(acc, cell_entropy(cell.y, a.d_in[c], a.d_out[s]));
-        fx_sub_acc(acc, cell_entropy(cell.y, din_new, dout_s_new));
-    }
-    // unchanged cells of column s (rows other than r, s)
-    for (int i = cs0 + lane; i < cs1; i += WAVE) {
-        int2 cell = a.col_cells[i];
-        int x = cell.x;
-        if (x == r || x == s) continue;
-        if (cell_lookup(a.col_cells, cr0, cr1, x) != 0) continue;  // (x, s) receives M[x][r]
-        fx_acc(acc, cell_entropy(cell.y, a.d_in[s], a.d_out[x]));
-        fx_sub_acc(acc, cell_entropy(cell.y, din_s_new, a.d_out[x]));
-    }
-    acc = wave_sum_fx(acc);
+    const long long S_row = (long long)a.d_out[s] - mss - msr - sub_row;
+    const long long S_col = (long long)a.d_in[s] - mss - mrs - sub_col;
+    fx_acc(acc, (double)S_row * (fastlog(dout_s_new) - fastlog(a.d_out[s])));
+    fx_acc(acc, (double)S_col * (fastlog(din_s_new) - fastlog(a.d_in[s])));
+    (void)lane;
     return fx_to(acc);
 }
 
@@ -166,7 +160,7 @@ __global__ void __launch_bounds__(MG_BLOCK) k_merge_propose(MgArgs a) {
                 long long rc_r = (rr1 - rr0) + (cr1 - cr0);
                 long long rc_s = (a.row_ptr[s + 1] - a.row_ptr[s]) + (a.col_ptr[s + 1] - a.col_ptr[s]);
                 long long rc_t = t >= 0 ? (a.row_ptr[t + 1] - a.row_ptr[t]) + (a.col_ptr[t + 1] - a.col_ptr[t]) : 0;
-                bytes += 16ull * rc_r + 8ull * rc_t + 12ull * (rc_r + rc_s);
+                bytes += 16ull * rc_r + 8ull * rc_t + 12ull * (rc_r + rc_s);  // per-proposal model of the reference
             }
             if (dS < best) {
                 best = dS;

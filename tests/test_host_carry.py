@@ -1,0 +1,60 @@
+"""The library's host-side merge carry-out (sbp_amd/csrc/carry.hpp, built without HIP as
+libsbp_carry.so) against the oracle's carry_out_best_merges_advanced / carry_out_best_merges
+restatement, up to the first merge phase of the 200,000-vertex benchmark graph."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from sbp_amd.graph import dcsbm
+
+LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sbp_amd", "libsbp_carry.so")
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def host_carry(bm, best_s, best_dS, num_to_merge, approximate):
+    L = C.CDLL(LIB)
+    L.sbp_host_carry_out.restype = C.c_int32
+    x = bm.export()
+    B = x["B"]
+    rp, cols, cnts = x["row_ptr"], x["cols"], x["cnts"]
+    rows = np.repeat(np.arange(B), np.diff(rp))
+    order = np.lexsort((rows, cols))
+    cp = np.zeros(B + 1, np.int32)
+    np.cumsum(np.bincount(cols, minlength=B), out=cp[1:])
+    crow, ccnt = rows[order].astype(np.int32), cnts[order].astype(np.int32)
+    dout, din = x["d_out"].astype(np.int32), x["d_in"].astype(np.int32)
+    s = np.ascontiguousarray(best_s, np.int32)
+    d = np.ascontiguousarray(best_dS, np.float64)
+    bmap = np.zeros(B, np.int32)
+    st = np.zeros(4, np.int64)
+    nb = L.sbp_host_carry_out(B, _p(rp), _p(cols), _p(cnts), _p(cp), _p(crow), _p(ccnt), _p(dout), _p(din),
+                              _p(x["sizes"]), _p(s), _p(d), C.c_int64(num_to_merge), int(approximate), _p(bmap),
+                              _p(st))
+    return nb, bmap, st
+
+
+@pytest.mark.parametrize("approximate", [False, True])
+@pytest.mark.parametrize("N,B", [(3000, 3000), (3000, 400), (200000, 200000)])
+def test_host_carry_matches_oracle(N, B, approximate):
+    if not os.path.exists(LIB):
+        pytest.fail("libsbp_carry.so not built (run __graft_entry__.build())")
+    if N == 200000:
+        g = dcsbm(N, 500, gamma=2.1, kmin=1, kmax=2000, mean_degree=20, p_in=0.6, sigma=0.5, seed=0)
+    else:
+        g = dcsbm(N, 20, gamma=2.3, kmin=1, kmax=150, mean_degree=12, p_in=0.65, seed=N + B)
+    og = O.OGraph.from_csr(g)
+    rng = np.random.default_rng(B)
+    assign = np.arange(N, dtype=np.int32) if B == N else np.concatenate(
+        [np.arange(B), rng.integers(0, B, N - B)]).astype(np.int32)
+    bm = O.OBlockmodel.build(og, B, assign)
+    best_s, best_dS = bm.merge_propose(seed=3, epoch=1)
+    nb, bmap, st = host_carry(bm, best_s, best_dS, B // 2, approximate)
+    out = bm.copy().merge_apply(best_s, best_dS, B // 2, approximate)
+    assert nb == out.B == B - B // 2
+    assert np.array_equal(bmap[assign], out.export()["assignment"])

@@ -370,3 +370,46 @@ def test_full_sbp_recovers_planted_partition():
     assert nmi(g.truth, a) > 0.9
     a2, nb2, _ = O.sbp(og, O.params("async_gibbs", seed=0, threads=1))
     assert (a == a2).all() and nb == nb2  # deterministic regardless of thread count
+
+
+# ---------------------------------------------------------------- factored vs per-cell delta-MDL
+
+@pytest.mark.parametrize("B", [4, 30, 200])
+def test_factored_vertex_delta_mdl_matches_per_cell(B):
+    """The replay contract's bulk form equals the reference's cell-by-cell delta_mdl within 1e-12 of the
+    sum of |cell terms|, and never flips the sign of a delta that is not a rounding-level tie."""
+    g = dcsbm(600, 6, gamma=2.3, kmin=1, kmax=120, mean_degree=9, p_in=0.6, seed=B)
+    og = O.OGraph.from_csr(g)
+    rng = np.random.default_rng(B)
+    assign = rng.integers(0, B, g.N).astype(np.int32)
+    assign[:B] = np.arange(B)
+    bm = O.OBlockmodel.build(og, B, assign)
+    n = 0
+    for v in rng.choice(g.N, 150, replace=False):
+        for s in rng.choice(B, 4, replace=False):
+            if s == assign[v]:
+                continue
+            f = bm.vertex_delta_mdl(int(v), int(s))
+            c, scale = bm.vertex_delta_mdl_cells(int(v), int(s))
+            assert abs(f - c) <= 1e-12 * max(scale, 1.0), (v, s, f, c, scale)
+            if abs(c) > 1e-9 * max(scale, 1.0):
+                assert (f < 0) == (c < 0)
+            n += 1
+    assert n > 300
+
+
+@pytest.mark.parametrize("B", [10, 120])
+def test_factored_merge_delta_mdl_matches_per_cell(B):
+    g = dcsbm(800, 8, gamma=2.3, kmin=1, kmax=120, mean_degree=9, p_in=0.6, seed=100 + B)
+    og = O.OGraph.from_csr(g)
+    rng = np.random.default_rng(B)
+    assign = rng.integers(0, B, g.N).astype(np.int32)
+    assign[:B] = np.arange(B)
+    bm = O.OBlockmodel.build(og, B, assign)
+    for r in range(min(B, 40)):
+        for s in rng.choice(B, 5, replace=False):
+            if s == r:
+                continue
+            f = bm.merge_delta_mdl(r, int(s))
+            c, scale = bm.merge_delta_mdl_cells(r, int(s))
+            assert abs(f - c) <= 1e-12 * max(scale, 1.0), (r, s, f, c, scale)
