@@ -53,7 +53,7 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"CLI link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(CLI + ".tmp", CLI)
-    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-o", CARRY_LIB + ".tmp",
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-ffp-contract=off", "-o", CARRY_LIB + ".tmp",
            os.path.join(HERE, "host", "carry_capi.cpp")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -14,6 +14,7 @@
 
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "carry.hpp"
 #include "host_blockmodel.hpp"
 #include "../../include/sbp/sbp.hpp"
 
@@ -268,8 +269,28 @@ static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, 
         st->merge_phases++;
     }
     double t0 = now_ms();
-    std::vector<int32_t> block_map;
-    int32_t new_B = carry_out_merges(ctx, bm, best_s, best_dS, num_to_merge, approximate, block_map);
+    std::vector<int32_t> block_map, sizes(B);
+    SBP_HIP(hipMemcpy(sizes.data(), bm->sizes, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+    sbpcarry::Model model;
+    std::unique_ptr<sbpcarry::Pool> pool;
+    if (!approximate) {  // the advanced carry-out re-evaluates merges against the changing blockmodel
+        std::vector<int32_t> rp(B + 1), cp(B + 1), dout(B), din(B);
+        std::vector<int2> rc(bm->nnz), cc(bm->nnz);
+        SBP_HIP(hipMemcpy(rp.data(), bm->row_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
+        SBP_HIP(hipMemcpy(cp.data(), bm->col_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
+        SBP_HIP(hipMemcpy(dout.data(), bm->d_out, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+        SBP_HIP(hipMemcpy(din.data(), bm->d_in, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+        if (bm->nnz) {
+            SBP_HIP(hipMemcpy(rc.data(), bm->row_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
+            SBP_HIP(hipMemcpy(cc.data(), bm->col_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
+        }
+        pool.reset(new sbpcarry::Pool(sbpcarry::carry_threads()));
+        const int32_t *rcp = reinterpret_cast<const int32_t *>(rc.data());
+        const int32_t *ccp = reinterpret_cast<const int32_t *>(cc.data());
+        model.load(B, rp.data(), rcp, rcp + 1, cp.data(), ccp, ccp + 1, 2, dout.data(), din.data(), pool.get());
+    }
+    int32_t new_B = sbpcarry::carry_out(approximate ? nullptr : &model, B, best_s.data(), best_dS.data(),
+                                        sizes.data(), num_to_merge, approximate, block_map, nullptr, pool.get());
     if (st) st->carry_ms += now_ms() - t0;
     double t1 = now_ms();
     int32_t *d_map = d_best_s;  // reuse
@@ -381,7 +402,7 @@ int sbp_graph_upload(sbp_ctx ctx, int32_t N, int64_t E, const int64_t *out_off, 
     g->h_out_adj.assign(out_adj, out_adj + E);
     g->h_in_adj.assign(in_adj, in_adj + E);
     std::vector<int32_t> src((size_t)E + N);
-    int64_t maxdeg = 0;
+    int64_t maxdeg = 0, nhigh = 0;
     for (int32_t v = 0; v < N; ++v) {
         SBP_CHECK(out_off[v + 1] >= out_off[v] && in_off[v + 1] >= in_off[v], SBP_E_ARG, "offsets not monotone");
         for (int64_t e = out_off[v]; e < out_off[v + 1]; ++e) {
@@ -390,10 +411,13 @@ int sbp_graph_upload(sbp_ctx ctx, int32_t N, int64_t E, const int64_t *out_off, 
         }
         for (int64_t e = in_off[v]; e < in_off[v + 1]; ++e)
             SBP_CHECK(in_adj[e] >= 0 && in_adj[e] < N, SBP_E_ARG, "in_adj id out of range");
-        maxdeg = std::max<int64_t>(maxdeg, (out_off[v + 1] - out_off[v]) + (in_off[v + 1] - in_off[v]));
+        const int64_t kall = (out_off[v + 1] - out_off[v]) + (in_off[v + 1] - in_off[v]);
+        maxdeg = std::max<int64_t>(maxdeg, kall);
+        nhigh += kall > FT_SMALL_DEGREE;
         src[(size_t)E + v] = v;
     }
     g->max_degree = maxdeg;
+    g->n_high_degree = nhigh;
     SBP_HIP(hipMalloc(&g->out_off, sizeof(int64_t) * (N + 1)));
     SBP_HIP(hipMalloc(&g->in_off, sizeof(int64_t) * (N + 1)));
     SBP_HIP(hipMalloc(&g->out_adj, sizeof(int32_t) * (E + 1)));

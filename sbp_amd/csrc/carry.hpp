@@ -15,104 +15,126 @@
 #pragma once
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <memory>
+#include <mutex>
 #include <queue>
+#include <thread>
 #include <vector>
 
 namespace sbpcarry {
 
-// Linear-probing map int32 -> int64 with backward-shift deletion.
+// Linear-probing map int32 -> int32 (block id -> cell count) with backward-shift deletion.  Keys and
+// counts share one slot array so a probe touches one cache line; the slot index is the top log2(cap) bits
+// of a multiplicative hash.
 struct FlatMap {
-    std::vector<int32_t> keys;  // -1 = empty
-    std::vector<int64_t> vals;
-    uint32_t n = 0;
+    struct Slot {
+        int32_t key;  // -1 = empty
+        int32_t val;
+    };
+    Slot *s = nullptr;
+    uint32_t cap = 0, n = 0;
+    int shift = 32;
 
-    static uint32_t h(int32_t k) { return (uint32_t)k * 0x9E3779B1u; }
+    FlatMap() = default;
+    FlatMap(const FlatMap &) = delete;
+    FlatMap &operator=(const FlatMap &) = delete;
+    FlatMap(FlatMap &&o) noexcept : s(o.s), cap(o.cap), n(o.n), shift(o.shift) { o.s = nullptr, o.cap = o.n = 0; }
+    FlatMap &operator=(FlatMap &&o) noexcept {
+        std::swap(s, o.s);
+        std::swap(cap, o.cap);
+        std::swap(n, o.n);
+        std::swap(shift, o.shift);
+        return *this;
+    }
+    ~FlatMap() { free(s); }
+
+    uint32_t slot(int32_t k) const { return ((uint32_t)k * 0x9E3779B1u) >> shift; }
     void init(uint32_t expected) {
-        uint32_t cap = 4;
-        while (cap < 2 * expected + 2) cap <<= 1;
-        keys.assign(cap, -1);
-        vals.assign(cap, 0);
+        uint32_t c = 4;
+        int lg = 2;
+        while (c < 2 * expected + 2) c <<= 1, ++lg;
+        free(s);
+        s = static_cast<Slot *>(malloc(sizeof(Slot) * c));
+        for (uint32_t i = 0; i < c; ++i) s[i] = Slot{-1, 0};
+        cap = c;
+        shift = 32 - lg;
         n = 0;
     }
-    uint32_t mask() const { return (uint32_t)keys.size() - 1; }
     int64_t get(int32_t k) const {
-        if (keys.empty()) return 0;
-        uint32_t m = mask(), i = h(k) & m;
-        while (keys[i] != -1) {
-            if (keys[i] == k) return vals[i];
-            i = (i + 1) & m;
+        if (!cap) return 0;
+        const uint32_t m = cap - 1;
+        for (uint32_t i = slot(k);; i = (i + 1) & m) {
+            if (s[i].key == k) return s[i].val;
+            if (s[i].key == -1) return 0;
         }
-        return 0;
     }
     void grow() {
-        std::vector<int32_t> ok;
-        std::vector<int64_t> ov;
-        ok.swap(keys);
-        ov.swap(vals);
+        Slot *old = s;
+        const uint32_t oc = cap;
+        s = nullptr;
         init(n * 2 + 4);
-        for (size_t i = 0; i < ok.size(); ++i)
-            if (ok[i] != -1) add(ok[i], ov[i]);
+        for (uint32_t i = 0; i < oc; ++i)
+            if (old[i].key != -1) add(old[i].key, old[i].val);
+        free(old);
     }
     void erase_at(uint32_t i) {
-        uint32_t m = mask();
-        keys[i] = -1;
-        vals[i] = 0;
+        const uint32_t m = cap - 1;
+        s[i] = Slot{-1, 0};
         n--;
-        uint32_t j = (i + 1) & m;
-        while (keys[j] != -1) {  // backward shift: re-place entries whose probe path crossed i
-            uint32_t home = h(keys[j]) & m;
-            bool move = (j > i) ? (home <= i || home > j) : (home <= i && home > j);
+        for (uint32_t j = (i + 1) & m; s[j].key != -1; j = (j + 1) & m) {
+            // backward shift: re-place entries whose probe path crossed i
+            const uint32_t home = slot(s[j].key);
+            const bool move = (j > i) ? (home <= i || home > j) : (home <= i && home > j);
             if (move) {
-                keys[i] = keys[j];
-                vals[i] = vals[j];
-                keys[j] = -1;
-                vals[j] = 0;
+                s[i] = s[j];
+                s[j] = Slot{-1, 0};
                 i = j;
             }
-            j = (j + 1) & m;
         }
     }
     // adds d; erases the entry when it reaches 0
     void add(int32_t k, int64_t d) {
         if (d == 0) return;
-        if (keys.empty()) init(4);
-        uint32_t m = mask(), i = h(k) & m;
-        while (keys[i] != -1) {
-            if (keys[i] == k) {
-                vals[i] += d;
-                if (vals[i] == 0) erase_at(i);
+        if (!cap) init(4);
+        const uint32_t m = cap - 1;
+        uint32_t i = slot(k);
+        for (; s[i].key != -1; i = (i + 1) & m) {
+            if (s[i].key == k) {
+                s[i].val += (int32_t)d;
+                if (s[i].val == 0) erase_at(i);
                 return;
             }
-            i = (i + 1) & m;
         }
-        keys[i] = k;
-        vals[i] = d;
+        s[i] = Slot{k, (int32_t)d};
         n++;
-        if (2 * n + 2 > keys.size()) grow();
+        if (2 * n + 2 > cap) grow();
     }
     void erase(int32_t k) {
-        if (keys.empty()) return;
-        uint32_t m = mask(), i = h(k) & m;
-        while (keys[i] != -1) {
-            if (keys[i] == k) {
+        if (!cap) return;
+        const uint32_t m = cap - 1;
+        for (uint32_t i = slot(k); s[i].key != -1; i = (i + 1) & m)
+            if (s[i].key == k) {
                 erase_at(i);
                 return;
             }
-            i = (i + 1) & m;
-        }
     }
     template <typename F>
     void each(F f) const {
-        for (size_t i = 0; i < keys.size(); ++i)
-            if (keys[i] != -1) f(keys[i], vals[i]);
+        for (uint32_t i = 0; i < cap; ++i)
+            if (s[i].key != -1) f(s[i].key, (int64_t)s[i].val);
     }
     void clear() {
-        keys.clear();
-        vals.clear();
-        n = 0;
+        free(s);
+        s = nullptr;
+        cap = n = 0;
+        shift = 32;
     }
 };
 
@@ -140,7 +162,7 @@ struct Fx128 {
 // fastlog (include/fastlog.hpp:21-30) with a table of logf(i) for small counts and degrees, like the
 // reference's fastlog_cache (src/fastlog.cpp:13-23) -- the table holds logf's own values.
 struct LogfTable {
-    static constexpr int64_t SIZE = 1 << 22;
+    static constexpr int64_t SIZE = 1 << 16;
     std::vector<float> v;
     LogfTable() : v(SIZE) {
         v[0] = 0.0f;
@@ -161,26 +183,110 @@ inline double cell(int64_t value, int64_t din, int64_t dout) {
     return (double)value * (fastlog(value) - (fastlog(din) + fastlog(dout)));
 }
 
+// Fork-join helper for the speculative evaluations: fn(i) for i in [0, n) on up to `threads` threads.
+class Pool {
+  public:
+    explicit Pool(int threads) : nthreads_(std::max(1, threads)) {
+        for (int t = 1; t < nthreads_; ++t) workers_.emplace_back([this] { work(); });
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+            ++gen_;
+        }
+        cv_.notify_all();
+        for (auto &w : workers_) w.join();
+    }
+    template <typename F>
+    void run(int64_t n, F fn) {
+        if (n <= 0) return;
+        if (nthreads_ == 1 || n < 8) {
+            for (int64_t i = 0; i < n; ++i) fn(i);
+            return;
+        }
+        std::function<void(int64_t)> f = fn;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            task_ = &f;
+            n_ = n;
+            next_.store(0);
+            busy_ = nthreads_ - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        drain(f);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [this] { return busy_ == 0; });
+        task_ = nullptr;
+    }
+
+  private:
+    void drain(const std::function<void(int64_t)> &f) {
+        for (int64_t i; (i = next_.fetch_add(1)) < n_;) f(i);
+    }
+    void work() {
+        uint64_t seen = 0;
+        while (true) {
+            const std::function<void(int64_t)> *f;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (stop_) return;
+                f = task_;
+            }
+            drain(*f);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--busy_ == 0) done_cv_.notify_one();
+        }
+    }
+    int nthreads_;
+    std::vector<std::thread> workers_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+    const std::function<void(int64_t)> *task_ = nullptr;
+    int64_t n_ = 0;
+    std::atomic<int64_t> next_{0};
+    int busy_ = 0;
+};
+
+inline int carry_threads() {
+    unsigned hw = std::thread::hardware_concurrency();
+    return (int)std::min<unsigned>(16, hw ? hw : 1);
+}
+
 struct Model {
     int32_t B = 0;
     std::vector<FlatMap> row, col;
     std::vector<int64_t> d_out, d_in;
 
-    // row-major cells (col, count) per row and column-major cells (row, count) per column
-    void load(int32_t nb, const int32_t *row_ptr, const int32_t *row_cols, const int32_t *row_cnts,
-              const int32_t *col_ptr, const int32_t *col_rows, const int32_t *col_cnts, const int32_t *dout,
-              const int32_t *din) {
+    // row-major cells (col, count) per row and column-major cells (row, count) per column, as separate
+    // index/count arrays (stride 1) or interleaved (index, count) pairs (stride 2)
+    void load(int32_t nb, const int32_t *row_ptr, const int32_t *row_idx, const int32_t *row_cnt,
+              const int32_t *col_ptr, const int32_t *col_idx, const int32_t *col_cnt, int stride,
+              const int32_t *dout, const int32_t *din, Pool *pool = nullptr) {
         B = nb;
-        row.assign(B, FlatMap());
-        col.assign(B, FlatMap());
+        row.clear();
+        col.clear();
+        row.resize(B);
+        col.resize(B);
         d_out.assign(dout, dout + B);
         d_in.assign(din, din + B);
-        for (int32_t b = 0; b < B; ++b) {
-            row[b].init(row_ptr[b + 1] - row_ptr[b]);
-            for (int32_t i = row_ptr[b]; i < row_ptr[b + 1]; ++i) row[b].add(row_cols[i], row_cnts[i]);
-            col[b].init(col_ptr[b + 1] - col_ptr[b]);
-            for (int32_t i = col_ptr[b]; i < col_ptr[b + 1]; ++i) col[b].add(col_rows[i], col_cnts[i]);
-        }
+        auto fill = [&](int64_t chunk) {
+            const int32_t b0 = (int32_t)(chunk * 1024), b1 = std::min<int32_t>(B, b0 + 1024);
+            for (int32_t b = b0; b < b1; ++b) {
+                row[b].init(row_ptr[b + 1] - row_ptr[b]);
+                for (int64_t i = row_ptr[b]; i < row_ptr[b + 1]; ++i) row[b].add(row_idx[stride * i], row_cnt[stride * i]);
+                col[b].init(col_ptr[b + 1] - col_ptr[b]);
+                for (int64_t i = col_ptr[b]; i < col_ptr[b + 1]; ++i) col[b].add(col_idx[stride * i], col_cnt[stride * i]);
+            }
+        };
+        const int64_t chunks = ((int64_t)B + 1023) / 1024;
+        if (pool) pool->run(chunks, fill);
+        else for (int64_t c = 0; c < chunks; ++c) fill(c);
     }
     int64_t get(int32_t r, int32_t c) const { return row[r].n <= col[c].n ? row[r].get(c) : col[c].get(r); }
 };
@@ -257,15 +363,34 @@ inline void merge_apply(Model &m, int32_t r, int32_t s) {
 }
 
 struct CarryStats {
-    int64_t pops = 0, evaluations = 0, requeues = 0, merges = 0;
+    int64_t pops = 0;          // items taken from the queue
+    int64_t evaluations = 0;   // delta-MDL evaluations on the sequential path
+    int64_t speculative = 0;   // delta-MDL evaluations run ahead in parallel
+    int64_t reused = 0;        // pops answered from a still-valid earlier evaluation
+    int64_t requeues = 0, merges = 0;
 };
 
 // Returns the block count after the merges; block_map[b] = new label of old block b (labels compacted in
 // ascending order of the surviving roots, as Blockmodel::build_mapping does).  `m` may be null when
 // approximate.
+//
+// The advanced carry-out takes exactly the decisions of the sequential loop (pop the smallest (dS, r),
+// re-evaluate it on the current state, re-queue it if it got worse than the next item, else merge), but
+// avoids most sequential evaluations:
+//   * delta-MDL(r -> t) reads rows/columns r and t, the degrees of r, t and of N(r) = the blocks sharing a
+//     cell with r, and the cells of t keyed by N(r) + {r, t}.  A merge a -> b changes rows/columns a and b,
+//     the entries keyed a (which become b) of the rows/columns of N(a), and the degrees of a and b.  So
+//     it leaves delta-MDL(r -> t) unchanged unless a or b is r, t or (then or now) in N(r); a and b get
+//     stamped with the merge count, and an evaluation is still exact while no block of N(r) + {r, t} has
+//     a newer stamp;
+//   * the proposal values (computed on the same state with the same terms and fixed-point sum) are the
+//     first evaluations;
+//   * items are taken from the heap in batches and the ones without a valid evaluation are evaluated in
+//     parallel against the current state before the sequential pass consumes them.  The batch size adapts
+//     to the fraction of those evaluations that a merge inside the batch invalidates.
 inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const double *best_dS, const int32_t *sizes,
                          int64_t num_to_merge, bool approximate, std::vector<int32_t> &block_map,
-                         CarryStats *cs = nullptr) {
+                         CarryStats *cs = nullptr, Pool *pool = nullptr, bool seed_from_proposals = true) {
     std::vector<int32_t> parent(B);
     for (int32_t b = 0; b < B; ++b) parent[b] = b;
     auto find = [&](int32_t x) {
@@ -284,6 +409,7 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
         int64_t counter = 0;
         while (merged < num_to_merge && counter < B) {
             int32_t from = order[counter++];
+            if (best_s[from] < 0) continue;
             int32_t to = find(best_s[from]);
             if (to != from) {
                 parent[from] = to;
@@ -295,29 +421,107 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
             double dS;
             int32_t r, s;
         };
-        auto worse = [](const Item &a, const Item &b) {
-            if (a.dS != b.dS) return a.dS > b.dS;
-            return a.r > b.r;
-        };
+        auto before = [](const Item &a, const Item &b) { return a.dS != b.dS ? a.dS < b.dS : a.r < b.r; };
+        auto worse = [&](const Item &a, const Item &b) { return before(b, a); };
         std::vector<Item> init;
         init.reserve(B);
-        for (int32_t b = 0; b < B; ++b) init.push_back(Item{best_dS[b], b, best_s[b]});
+        for (int32_t b = 0; b < B; ++b)
+            if (best_s[b] >= 0) init.push_back(Item{best_dS[b], b, best_s[b]});
         std::priority_queue<Item, std::vector<Item>, decltype(worse)> heap(worse, std::move(init));
-        while (merged < num_to_merge && !heap.empty()) {
-            Item it = heap.top();
-            heap.pop();
+        // evaluation cache per source block: target root, merge count when evaluated, value
+        std::vector<int32_t> c_to(B, -1);
+        std::vector<int64_t> c_time(B, -1), stamp(B, 0);
+        std::vector<double> c_val(B, 0.0);
+        if (seed_from_proposals)
+            for (int32_t b = 0; b < B; ++b)
+                if (best_s[b] >= 0 && best_s[b] != b) {
+                    c_to[b] = best_s[b];
+                    c_time[b] = 0;
+                    c_val[b] = best_dS[b];
+                }
+        int64_t now = 0;
+        auto valid = [&](int32_t r, int32_t t) {
+            const int64_t T = c_time[r];
+            if (c_to[r] != t || T < stamp[r] || T < stamp[t]) return false;
+            bool ok = true;
+            m->row[r].each([&](int32_t c, int64_t) { ok = ok && stamp[c] <= T; });
+            if (ok) m->col[r].each([&](int32_t x, int64_t) { ok = ok && stamp[x] <= T; });
+            return ok;
+        };
+        auto root_ro = [&](int32_t x) {
+            while (parent[x] != x) x = parent[x];
+            return x;
+        };
+        std::vector<Item> batch;
+        std::atomic<int64_t> spec_count{0};
+        size_t bi = 0;
+        int64_t K = 256, batch_spec = 0, batch_lost = 0, spec_time = -1;
+        while (merged < num_to_merge) {
+            if (bi == batch.size()) {
+                if (heap.empty()) break;
+                if (batch_spec >= 32) {  // adapt the batch size to the invalidated fraction of the last batch
+                    if (4 * batch_lost > batch_spec) K = std::max<int64_t>(32, K / 2);
+                    else if (10 * batch_lost < batch_spec) K = std::min<int64_t>(16384, K * 2);
+                }
+                batch.clear();
+                bi = 0;
+                batch_spec = batch_lost = 0;
+                const int64_t want = std::min<int64_t>(K, 2 * (num_to_merge - merged) + 32);
+                while ((int64_t)batch.size() < want && !heap.empty()) {
+                    batch.push_back(heap.top());
+                    heap.pop();
+                }
+                if (pool && batch.size() >= 16) {
+                    pool->run((int64_t)batch.size(), [&](int64_t i) {
+                        const Item &it = batch[i];
+                        const int32_t t = root_ro(it.s);
+                        if (t == it.r || valid(it.r, t)) return;
+                        c_val[it.r] = merge_dS(*m, it.r, t);
+                        c_to[it.r] = t;
+                        c_time[it.r] = now;
+                        spec_count.fetch_add(1, std::memory_order_relaxed);
+                    });
+                    batch_spec = spec_count.exchange(0);
+                    st.speculative += batch_spec;
+                    spec_time = now;
+                }
+            }
+            Item it;
+            if (!heap.empty() && before(heap.top(), batch[bi])) {
+                it = heap.top();
+                heap.pop();
+            } else {
+                it = batch[bi++];
+            }
             st.pops++;
             const int32_t from = it.r;
             const int32_t to = find(it.s);
             if (from == to) continue;
-            double actual = merge_dS(*m, from, to);
-            st.evaluations++;
-            if (!heap.empty() && actual > heap.top().dS) {
-                it.dS = actual;
-                heap.push(it);
-                st.requeues++;
-                continue;
+            double actual;
+            if (valid(from, to)) {
+                actual = c_val[from];
+                st.reused++;
+            } else {
+                if (c_to[from] == to && c_time[from] == spec_time) batch_lost++;  // speculated, then invalidated
+                actual = merge_dS(*m, from, to);
+                c_to[from] = to;
+                c_time[from] = now;
+                c_val[from] = actual;
+                st.evaluations++;
             }
+            const bool has_b = bi < batch.size();
+            if (has_b || !heap.empty()) {
+                const double next = (has_b && (heap.empty() || before(batch[bi], heap.top()))) ? batch[bi].dS
+                                                                                                : heap.top().dS;
+                if (actual > next) {
+                    it.dS = actual;
+                    heap.push(it);
+                    st.requeues++;
+                    continue;
+                }
+            }
+            ++now;
+            stamp[from] = stamp[to] = now;
             parent[from] = to;
             merge_apply(*m, from, to);
             merged++;

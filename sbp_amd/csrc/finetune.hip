@@ -29,6 +29,8 @@
 #include <hipcub/hipcub.hpp>
 #include <math.h>
 
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -38,7 +40,8 @@ using namespace sbpdev;
 constexpr int FT_BLOCK = 256;
 constexpr int FT_WAVES = FT_BLOCK / WAVE;
 constexpr int LDS_CAP = 512;          // hash slots per wave
-constexpr int SMALL_K = LDS_CAP / 2;  // max |out| + |in| for the LDS table
+constexpr int SMALL_K = FT_SMALL_DEGREE;  // max |out| + |in| for the LDS table
+static_assert(2 * SMALL_K <= LDS_CAP, "LDS table load factor above 1/2");
 constexpr int EMPTY = -1;
 
 __device__ __forceinline__ void wave_sync() {
@@ -183,8 +186,9 @@ __device__ __forceinline__ long long new_dout(const FtArgs &a, const MoveCtx &m,
 }
 
 // old-minus-new entropy of one cell with current value M and change d
-__device__ __forceinline__ void cell_term(const FtArgs &a, const MoveCtx &m, int row, int col, long long M, long long d,
-                                          Fx &acc) {
+// Kept out of line: its glibc-logf restatement is large and is used at a dozen call sites.
+__device__ __noinline__ void cell_term(const FtArgs &a, const MoveCtx &m, int row, int col, long long M, long long d,
+                                      Fx &acc) {
     fx_acc(acc, cell_entropy(M, a.d_in[col], a.d_out[row]));
     fx_sub_acc(acc, cell_entropy(M + d, new_din(a, m, col), new_dout(a, m, row)));
 }
@@ -324,6 +328,57 @@ __device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w, unsigned
     return s;
 }
 
+// Hastings correction over the neighbour blocks (src/entropy.cpp:497-561) and the accept test
+// (src/finetune.cpp:22-32); out of line so the greedy path keeps its registers.
+__device__ __noinline__ bool hastings_accept(const FtArgs &a, const MoveCtx &m, const Table &t, const U4 &w,
+                                             double dS, int nb, long long mrr, long long mss) {
+    const int lane = lane_id();
+    const int r = m.r, s = m.s;
+    Fx fwd = fx_zero(), bwd = fx_zero();
+    const long long Bn = a.B;
+    const bool self_only_r = m.selfc > 0 && m.out_r == 0 && m.in_r == 0;
+    for (int i = lane; i < nb + 1; i += WAVE) {
+        int x, cntx, oc, ic;
+        if (i < nb) {
+            const int slot = t.used[i];
+            x = t.keys[slot];
+            oc = t.outc[slot];
+            ic = t.inc[slot];
+            cntx = oc + ic + (x == r ? m.selfc : 0);
+        } else {
+            if (!self_only_r) continue;  // the self-loop is the only link to block r
+            x = r;
+            oc = 0;
+            ic = 0;
+            cntx = m.selfc;
+        }
+        double counts = (double)cntx;
+        double pw = (double)(get_cell(a, s, x) + get_cell(a, x, s)) + 1.0;
+        double bd = (double)((long long)a.d_tot[x] + Bn);
+        long long d_rx = delta_row_r(m, x, oc);
+        long long d_xr = (x == r) ? delta_row_r(m, r, m.out_r) : (x == s ? delta_row_s(m, r, m.out_r) : -(long long)ic);
+        double bw = (double)(get_cell(a, r, x) + d_rx + get_cell(a, x, r) + d_xr) + 1.0;
+        long long nbd = a.d_tot[x];
+        if (x == r) {
+            long long self = mrr + delta_row_r(m, r, m.out_r);
+            nbd = (a.d_out[r] - m.k_out) + (a.d_in[r] - m.k_in) - self;
+        } else if (x == s) {
+            long long self = mss + delta_row_s(m, s, m.out_s);
+            nbd = (a.d_out[s] + m.k_out) + (a.d_in[s] + m.k_in) - self;
+        }
+        double pd = (double)(nbd + Bn);
+        fx_acc(fwd, counts * pw / bd);
+        fx_acc(bwd, counts * bw / pd);
+    }
+    fwd = wave_sum_fx(fwd);
+    bwd = wave_sum_fx(bwd);
+    double H = (m.k_out + m.k_in == 0) ? 1.0 : fx_to(bwd) / fx_to(fwd);
+    double u = unit_double(w.z, w.w);
+    double p = exp(-3.0 * dS) * H;
+    p = p >= 1.0 ? 1.0 : p;
+    return u <= p;
+}
+
 // Builds the neighbour-block histogram and returns delta-MDL; sets accept.  The table stays populated
 // for the caller (Metropolis-Hastings applies the Delta from it), which clears it.
 //
@@ -406,50 +461,7 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
         accept = dS < 0.0;
         return dS;
     }
-    // Hastings correction over the neighbour blocks (entropy.cpp:497-561)
-    Fx fwd = fx_zero(), bwd = fx_zero();
-    const long long Bn = a.B;
-    const bool self_only_r = m.selfc > 0 && m.out_r == 0 && m.in_r == 0;
-    for (int i = lane; i < nb + 1; i += WAVE) {
-        int x, cntx, oc, ic;
-        if (i < nb) {
-            const int slot = t.used[i];
-            x = t.keys[slot];
-            oc = t.outc[slot];
-            ic = t.inc[slot];
-            cntx = oc + ic + (x == r ? m.selfc : 0);
-        } else {
-            if (!self_only_r) continue;  // the self-loop is the only link to block r
-            x = r;
-            oc = 0;
-            ic = 0;
-            cntx = m.selfc;
-        }
-        double counts = (double)cntx;
-        double pw = (double)(get_cell(a, s, x) + get_cell(a, x, s)) + 1.0;
-        double bd = (double)((long long)a.d_tot[x] + Bn);
-        long long d_rx = delta_row_r(m, x, oc);
-        long long d_xr = (x == r) ? delta_row_r(m, r, m.out_r) : (x == s ? delta_row_s(m, r, m.out_r) : -(long long)ic);
-        double bw = (double)(get_cell(a, r, x) + d_rx + get_cell(a, x, r) + d_xr) + 1.0;
-        long long nbd = a.d_tot[x];
-        if (x == r) {
-            long long self = mrr + delta_row_r(m, r, m.out_r);
-            nbd = (a.d_out[r] - m.k_out) + (a.d_in[r] - m.k_in) - self;
-        } else if (x == s) {
-            long long self = mss + delta_row_s(m, s, m.out_s);
-            nbd = (a.d_out[s] + m.k_out) + (a.d_in[s] + m.k_in) - self;
-        }
-        double pd = (double)(nbd + Bn);
-        fx_acc(fwd, counts * pw / bd);
-        fx_acc(bwd, counts * bw / pd);
-    }
-    fwd = wave_sum_fx(fwd);
-    bwd = wave_sum_fx(bwd);
-    double H = (m.k_out + m.k_in == 0) ? 1.0 : fx_to(bwd) / fx_to(fwd);
-    double u = unit_double(w.z, w.w);
-    double p = exp(-3.0 * dS) * H;
-    p = p >= 1.0 ? 1.0 : p;
-    accept = u <= p;
+    accept = hastings_accept(a, m, t, w, dS, nb, mrr, mss);
     bytes += 8ull * 6 * 10 * (unsigned long long)nb;
     return dS;
 }
@@ -563,10 +575,15 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     else k_ft_propose<false, false><<<(int)blocks, FT_BLOCK, 0, st>>>(a);
     SBP_HIP(hipGetLastError());
     if (g->max_degree > SMALL_K) {
-        int lg = log2_ceil(2 * g->max_degree + 2);
-        if (lg < 10) lg = 10;
-        const int64_t big_blocks = 32;
-        a.gtable = ctx->scratch[S_HASH].as<int>((size_t)big_blocks * FT_WAVES * (4 * ((size_t)1 << lg) + 1));
+        // one wave per deferred vertex where the tables fit in kMaxTableBytes; a table holds the distinct
+        // neighbour blocks of one vertex at load factor <= 1/2
+        const int lg = std::max(10, log2_ceil(2 * std::min<int64_t>(g->max_degree, bm->B) + 2));
+        const size_t table_ints = 4 * ((size_t)1 << lg) + 1;
+        constexpr size_t kMaxTableBytes = size_t(1) << 30;
+        int64_t waves = std::min<int64_t>(g->n_high_degree, count);
+        waves = std::max<int64_t>(1, std::min<int64_t>(waves, kMaxTableBytes / (table_ints * sizeof(int))));
+        const int64_t big_blocks = (waves + FT_WAVES - 1) / FT_WAVES;
+        a.gtable = ctx->scratch[S_HASH].as<int>((size_t)big_blocks * FT_WAVES * table_ints);
         a.gcap_log2 = lg;
         if (greedy) k_ft_propose<true, true><<<(int)big_blocks, FT_BLOCK, 0, st>>>(a);
         else k_ft_propose<true, false><<<(int)big_blocks, FT_BLOCK, 0, st>>>(a);
@@ -1021,8 +1038,7 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         a.count = count;
         a.bytes = bytes;
         a.err = reinterpret_cast<int *>(bytes + 1);
-        int lg = log2_ceil(2 * g->max_degree + 2);
-        if (lg < 10) lg = 10;
+        const int lg = std::max(10, log2_ceil(2 * std::min<int64_t>(g->max_degree, bm->B) + 2));
         a.gtable = g->max_degree > SMALL_K ? ctx->scratch[S_HASH].as<int>(4 * ((size_t)1 << lg) + 1) : nullptr;
         a.gcap_log2 = lg;
         a.m.row_cum = nullptr;

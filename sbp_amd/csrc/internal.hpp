@@ -82,9 +82,14 @@ struct sbp_graph_s {
     int32_t *out_adj = nullptr, *in_adj = nullptr;
     int32_t *edge_src = nullptr;    // source vertex of each out-edge (CSR order)
     int64_t max_degree = 0;         // max over v of |out(v)| + |in(v)|
+    int64_t n_high_degree = 0;      // vertices with |out(v)| + |in(v)| > FT_SMALL_DEGREE
     std::vector<int64_t> h_out_off, h_in_off;  // host copies for launch planning
     std::vector<int32_t> h_out_adj, h_in_adj;
 };
+
+// Vertices up to this total degree keep their neighbour-block histogram in LDS during finetune proposals;
+// the others use a per-wave table in global memory.
+constexpr int FT_SMALL_DEGREE = 256;
 
 struct sbp_bm_s {
     sbp_ctx ctx = nullptr;

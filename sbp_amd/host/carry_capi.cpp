@@ -1,26 +1,46 @@
 // Host-only entry to the merge carry-out (sbp_amd/csrc/carry.hpp), built without HIP so the CPU test
 // suite can check it against the oracle at full size.  Not part of the device C-ABI.
+#include <chrono>
+#include <memory>
 #include <vector>
 
 #include "../csrc/carry.hpp"
 
+// stats_out (10 entries): pops, sequential evaluations, speculative evaluations, reused evaluations,
+// requeues, merges, model load microseconds, carry-out loop microseconds, threads, proposal values that
+// differ from a host re-evaluation (checked only when verify_proposals != 0).
 extern "C" int sbp_host_carry_out(int32_t B, const int32_t *row_ptr, const int32_t *row_cols, const int32_t *row_cnts,
                                   const int32_t *col_ptr, const int32_t *col_rows, const int32_t *col_cnts,
                                   const int32_t *d_out, const int32_t *d_in, const int32_t *sizes,
                                   const int32_t *best_s, const double *best_dS, int64_t num_to_merge,
-                                  int32_t approximate, int32_t *block_map_out, int64_t *stats_out) {
+                                  int32_t approximate, int32_t threads, int32_t seed_from_proposals,
+                                  int32_t verify_proposals, int32_t *block_map_out, int64_t *stats_out) {
+    using clk = std::chrono::steady_clock;
+    auto us = [](clk::time_point a, clk::time_point b) {
+        return (int64_t)std::chrono::duration_cast<std::chrono::microseconds>(b - a).count();
+    };
+    const int nt = threads > 0 ? threads : sbpcarry::carry_threads();
+    std::unique_ptr<sbpcarry::Pool> pool(nt > 1 ? new sbpcarry::Pool(nt) : nullptr);
     sbpcarry::Model m;
-    if (!approximate) m.load(B, row_ptr, row_cols, row_cnts, col_ptr, col_rows, col_cnts, d_out, d_in);
+    auto t0 = clk::now();
+    if (!approximate)
+        m.load(B, row_ptr, row_cols, row_cnts, col_ptr, col_rows, col_cnts, 1, d_out, d_in, pool.get());
+    auto t1 = clk::now();
+    int64_t mismatches = 0;
+    if (verify_proposals && !approximate)
+        for (int32_t b = 0; b < B; ++b)
+            if (best_s[b] >= 0 && best_s[b] != b && sbpcarry::merge_dS(m, b, best_s[b]) != best_dS[b]) ++mismatches;
+    auto t2 = clk::now();
     std::vector<int32_t> map;
     sbpcarry::CarryStats cs;
     int32_t nb = sbpcarry::carry_out(approximate ? nullptr : &m, B, best_s, best_dS, sizes, num_to_merge,
-                                     approximate != 0, map, &cs);
+                                     approximate != 0, map, &cs, pool.get(), seed_from_proposals != 0);
+    auto t3 = clk::now();
     for (int32_t b = 0; b < B; ++b) block_map_out[b] = map[b];
     if (stats_out) {
-        stats_out[0] = cs.pops;
-        stats_out[1] = cs.evaluations;
-        stats_out[2] = cs.requeues;
-        stats_out[3] = cs.merges;
+        const int64_t v[10] = {cs.pops, cs.evaluations, cs.speculative, cs.reused, cs.requeues, cs.merges,
+                               us(t0, t1), us(t2, t3), nt, mismatches};
+        for (int i = 0; i < 10; ++i) stats_out[i] = v[i];
     }
     return nb;
 }

@@ -17,7 +17,7 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def host_carry(bm, best_s, best_dS, num_to_merge, approximate):
+def host_carry(bm, best_s, best_dS, num_to_merge, approximate, threads=0, seed=True, verify=False):
     L = C.CDLL(LIB)
     L.sbp_host_carry_out.restype = C.c_int32
     x = bm.export()
@@ -32,10 +32,10 @@ def host_carry(bm, best_s, best_dS, num_to_merge, approximate):
     s = np.ascontiguousarray(best_s, np.int32)
     d = np.ascontiguousarray(best_dS, np.float64)
     bmap = np.zeros(B, np.int32)
-    st = np.zeros(4, np.int64)
+    st = np.zeros(10, np.int64)
     nb = L.sbp_host_carry_out(B, _p(rp), _p(cols), _p(cnts), _p(cp), _p(crow), _p(ccnt), _p(dout), _p(din),
-                              _p(x["sizes"]), _p(s), _p(d), C.c_int64(num_to_merge), int(approximate), _p(bmap),
-                              _p(st))
+                              _p(x["sizes"]), _p(s), _p(d), C.c_int64(num_to_merge), int(approximate), int(threads),
+                              int(seed), int(verify), _p(bmap), _p(st))
     return nb, bmap, st
 
 
@@ -58,3 +58,29 @@ def test_host_carry_matches_oracle(N, B, approximate):
     out = bm.copy().merge_apply(best_s, best_dS, B // 2, approximate)
     assert nb == out.B == B - B // 2
     assert np.array_equal(bmap[assign], out.export()["assignment"])
+
+
+@pytest.mark.parametrize("N,B", [(3000, 400), (60000, 60000)])
+def test_host_carry_speculation_is_exact(N, B):
+    """The speculative, parallel carry-out takes the sequential loop's decisions: seeded from the proposal
+    values or not, on 1 thread or several, the block map is the same, and every proposal value equals the
+    host re-evaluation on the pre-merge state bit for bit (what makes the seeding exact)."""
+    g = dcsbm(N, 40, gamma=2.1, kmin=1, kmax=600, mean_degree=16, p_in=0.6, sigma=0.5, seed=7)
+    og = O.OGraph.from_csr(g)
+    rng = np.random.default_rng(5)
+    assign = np.arange(N, dtype=np.int32) if B == N else np.concatenate(
+        [np.arange(B), rng.integers(0, B, N - B)]).astype(np.int32)
+    bm = O.OBlockmodel.build(og, B, assign)
+    best_s, best_dS = bm.merge_propose(seed=11, epoch=2)
+    ref = None
+    for threads, seed in [(1, False), (1, True), (4, True), (8, False)]:
+        nb, bmap, st = host_carry(bm, best_s, best_dS, B // 2, False, threads=threads, seed=seed, verify=True)
+        assert st[9] == 0, "proposal value differs from the host re-evaluation"
+        assert st[5] == B // 2
+        if seed:
+            assert st[3] > 0
+        if ref is None:
+            ref = (nb, bmap)
+        assert nb == ref[0] and np.array_equal(bmap, ref[1]), (threads, seed)
+    out = bm.copy().merge_apply(best_s, best_dS, B // 2, False)
+    assert np.array_equal(ref[1][assign], out.export()["assignment"])
