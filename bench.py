@@ -76,15 +76,75 @@ def gpu_sweeps_at_truth(ctx, dg, g, algorithm, sweeps):
     from sbp_amd import native
     db = native.DeviceBlockmodel.build(dg, int(g.truth.max()) + 1, g.truth)
     prm = native.params(algorithm, seed=0)
-    props, ft_ms, t0 = 0, 0.0, time.perf_counter()
+    props, t0 = 0, time.perf_counter()
     for ep in range(sweeps):
         if algorithm == "metropolis_hastings":
             _, st = db.mh_sweep(prm, ep)
         else:
             st = db.async_gibbs_sweep(prm, ep)
         props += st.proposals
-        ft_ms += st.ft_kernel_ms
     return props / (time.perf_counter() - t0)
+
+
+KERNEL_NAMES = {  # rocprofv3 names of the finetune proposal launches (greedy / non-greedy)
+    "ft": ("k_ft_propose<false, true>", "k_ft_propose<false, false>"),
+    "ft_high": ("k_ft_propose<true, true>", "k_ft_propose<true, false>"),
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ft_propose_c2_traffic.txt")
+
+
+def committed_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE,
+    scaled by a copy-kernel calibration), or None when there is none for this kernel."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            lines = f.read().splitlines()
+    except OSError:
+        return None
+    fields = dict(l.split(":", 1) for l in lines if ":" in l and not l.startswith("#"))
+    if fields.get("kernel", "").strip() != kernel:
+        return None
+    try:
+        return float(fields["traffic_bytes_per_launch"].split()[0])
+    except (KeyError, ValueError, IndexError):
+        return None
+
+
+def roofline(tot, greedy=True):
+    """Roofline of the dominant finetune proposal launch: algorithmic bytes per launch over its average
+    device time (HIP events around that launch alone, on the library's stream)."""
+    legs = {}
+    for key, prefix in (("ft", "ft"), ("ft_high", "ft_high")):
+        n = tot[f"{prefix}_launches"]
+        if n:
+            legs[key] = dict(kernel=KERNEL_NAMES[key][0 if greedy else 1], launches=n,
+                             avg_launch_ms=tot[f"{prefix}_kernel_ms"] / n,
+                             algorithmic_bytes_per_launch=tot[f"{prefix}_bytes"] / n,
+                             total_ms=tot[f"{prefix}_kernel_ms"])
+    if not legs:
+        return None
+    for leg in legs.values():
+        leg["achieved_GBps"] = leg["algorithmic_bytes_per_launch"] / (leg["avg_launch_ms"] * 1e-3) / 1e9
+    dom = max(legs.values(), key=lambda l: l["total_ms"])
+    traffic = committed_traffic(dom["kernel"])
+    return {"bound": "hbm", "achieved": dom["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": dom["achieved_GBps"] / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": dom["kernel"], "avg_launch_ms": dom["avg_launch_ms"],
+            "algorithmic_bytes_per_launch": dom["algorithmic_bytes_per_launch"],
+            "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic is not None else None,
+            "launches": {k: {kk: vv for kk, vv in v.items() if kk != "total_ms"} for k, v in legs.items()}}
+
+
+def calibrate_copy(mib):
+    """A plain device-to-device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20)
+    src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "copies": 3}))
 
 
 def main():
@@ -95,7 +155,12 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
+                    help="only copy MIB MiB device-to-device 3 times (counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the path is single-device, "
               "see the sharding note in DESIGN.md)", file=sys.stderr)
@@ -134,9 +199,6 @@ def main():
 
     a, nb, mdl = results[-1]
     value = tot["proposals"] / wall
-    launch_ms = tot["ft_kernel_ms"] / max(tot["ft_launches"], 1)
-    bytes_per_launch = tot["ft_bytes"] / max(tot["ft_launches"], 1)
-    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
 
     gpu_truth_rate = gpu_sweeps_at_truth(ctx, dg, g, w["algorithm"], 10)
     cpu = None
@@ -170,12 +232,13 @@ def main():
         "mdl": mdl,
         "nmi": nmi(g.truth, a),
         "phase_ms_per_step": {k: tot[k] / args.steps for k in
-                              ("finetune_ms", "merge_ms", "carry_ms", "build_ms", "mdl_ms", "ft_kernel_ms")},
+                              ("finetune_ms", "merge_ms", "carry_ms", "build_ms", "mdl_ms", "ft_kernel_ms",
+                               "ft_high_kernel_ms", "mh_kernel_ms")},
         "sweeps_per_step": tot["sweeps"] / args.steps,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_ft_propose (finetune proposals)", "avg_launch_ms": launch_ms,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "carry_per_step": {k: tot[k] / args.steps for k in
+                           ("carry_load_ms", "carry_evaluations", "carry_speculative", "carry_reused",
+                            "carry_requeues")},
+        "roofline": roofline(tot, greedy=True),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

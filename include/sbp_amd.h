@@ -65,10 +65,22 @@ typedef struct {
     double build_ms;          /* blockmodel (re)builds */
     double mdl_ms;            /* MDL reductions */
     double total_ms;          /* wall time of the call */
-    uint64_t ft_bytes;        /* algorithmic bytes of the finetune proposal kernels */
+    uint64_t ft_bytes;        /* algorithmic bytes of the finetune proposal launch for vertices of total
+                               * degree <= 256 (neighbour-block histogram in LDS) */
     uint64_t merge_bytes;     /* algorithmic bytes of the merge proposal kernels */
-    int64_t ft_launches;      /* finetune proposal kernel launches */
-    double ft_kernel_ms;      /* device time of the finetune proposal kernels only */
+    int64_t ft_launches;      /* launches of that finetune proposal kernel */
+    double ft_kernel_ms;      /* device time of that finetune proposal kernel */
+    int64_t carry_evaluations;   /* merge carry-out: delta-MDL re-evaluations on the sequential path */
+    int64_t carry_speculative;   /* merge carry-out: re-evaluations run ahead in parallel */
+    int64_t carry_reused;        /* merge carry-out: pops answered by a still-exact earlier evaluation */
+    int64_t carry_requeues;      /* merge carry-out: merges pushed back because they got worse */
+    double carry_load_ms;        /* merge carry-out: device-to-host copy and hash-map build */
+    uint64_t ft_high_bytes;      /* bytes, launches, device time of the high-degree finetune proposal launch */
+    int64_t ft_high_launches;
+    double ft_high_kernel_ms;
+    uint64_t mh_bytes;           /* bytes, launches, device time of the Metropolis-Hastings kernel */
+    int64_t mh_launches;
+    double mh_kernel_ms;
 } sbp_stats;
 
 /* ----------------------------------------------------------------- context */

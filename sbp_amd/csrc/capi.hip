@@ -2,6 +2,8 @@
 // agglomerative driver with its golden-section triplet, sampling.  Kernels live in build.hip, mdl.hip,
 // finetune.hip, merge.hip and mh.hip.
 #include <float.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <math.h>
 #include <string.h>
 
@@ -85,6 +87,11 @@ static void gibbs_batch(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int greedy, uint32_
         st->ft_bytes += info.bytes;
         st->ft_launches += 1;
         st->ft_kernel_ms += info.kernel_ms;
+        if (info.high_launched) {
+            st->ft_high_bytes += info.high_bytes;
+            st->ft_high_launches += 1;
+            st->ft_high_kernel_ms += info.high_kernel_ms;
+        }
         st->finetune_ms += (t1 - t0) + (t2 - t1);
     }
 }
@@ -289,9 +296,23 @@ static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, 
         const int32_t *ccp = reinterpret_cast<const int32_t *>(cc.data());
         model.load(B, rp.data(), rcp, rcp + 1, cp.data(), ccp, ccp + 1, 2, dout.data(), din.data(), pool.get());
     }
+    const double t_loaded = now_ms();
+    sbpcarry::CarryStats cs;
     int32_t new_B = sbpcarry::carry_out(approximate ? nullptr : &model, B, best_s.data(), best_dS.data(),
-                                        sizes.data(), num_to_merge, approximate, block_map, nullptr, pool.get());
-    if (st) st->carry_ms += now_ms() - t0;
+                                        sizes.data(), num_to_merge, approximate, block_map, &cs, pool.get());
+    if (st) {
+        st->carry_ms += now_ms() - t0;
+        st->carry_load_ms += t_loaded - t0;
+        st->carry_evaluations += cs.evaluations;
+        st->carry_speculative += cs.speculative;
+        st->carry_reused += cs.reused;
+        st->carry_requeues += cs.requeues;
+    }
+    if (getenv("SBP_AMD_TRACE"))
+        fprintf(stderr, "[sbp] merge B=%d -> %d: propose %.1f ms, carry %.1f ms (load %.1f), pops %lld, evals %lld, "
+                "speculative %lld, reused %lld, requeues %lld\n", B, new_B, elapsed_ms(ctx, ctx->ev[2], ctx->ev[3]),
+                now_ms() - t0, t_loaded - t0, (long long)cs.pops, (long long)cs.evaluations,
+                (long long)cs.speculative, (long long)cs.reused, (long long)cs.requeues);
     double t1 = now_ms();
     int32_t *d_map = d_best_s;  // reuse
     SBP_HIP(hipMemcpyAsync(d_map, block_map.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice, s));

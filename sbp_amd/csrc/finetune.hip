@@ -231,7 +231,7 @@ __device__ __forceinline__ int cum_search(const int32_t *cum, int lo, int hi, lo
 }
 
 // Proposal of vertex v against the current state.  Returns s (== r for "no move").  Fills m.
-__device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w, unsigned long long &bytes) {
+__device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w) {
     const int lane = lane_id();
     m.v = v;
     m.o0 = a.out_off[v];
@@ -296,7 +296,6 @@ __device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w, unsigned
                 int i = cum_search(a.m.col_cum, cp0, cp1, target - dout_t);
                 if (i < cp1) s = a.m.col_cells[i].x;
             }
-            bytes += 8ull * 2 * 12;  // two binary searches over cumulative counts
         } else {
             long long base_sum = 0;
             for (int c0 = rp0; c0 < rp1 && s < 0; c0 += WAVE) {
@@ -316,7 +315,6 @@ __device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w, unsigned
                 if (hit) s = __shfl(cell.x, __ffsll((long long)hit) - 1, WAVE);
                 else base_sum = __shfl(incl, WAVE - 1, WAVE);
             }
-            bytes += 8ull * (unsigned long long)((rp1 - rp0) + (cp1 - cp0));
         }
         if (s < 0) {  // d_tot[t] disagrees with the cells of row/column t: report, propose no move
             if (lane == 0) atomicExch(a.err, 1);
@@ -324,8 +322,24 @@ __device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w, unsigned
         }
     }
     m.s = s;
-    bytes += 8ull * (unsigned long long)(ko + kia) + 24;
     return s;
+}
+
+// Algorithmic bytes of one proposal, the work-unit figure the roofline is quoted on: adjacency and
+// neighbour-block gathers 8 k(v), the multinomial over t 8 rc(t), the delta-MDL row/column scans
+// 12 (rc(r) + rc(s)) and the Delta cells 16 nb(v), plus 16 + 8 for the vertex and move records; the
+// Hastings correction adds 8 rc(s) + 8 nb(v).  rc(b) = cells in row b + cells in column b.  A proposal of
+// the current block evaluates nothing further.
+__device__ __forceinline__ unsigned long long proposal_bytes(const FtArgs &a, const MoveCtx &m, int nb, bool greedy) {
+    auto rc = [&](int b) -> unsigned long long {
+        return b < 0 ? 0ull : (unsigned long long)((a.m.row_end(b) - a.m.row_begin(b)) + (a.m.col_end(b) - a.m.col_begin(b)));
+    };
+    unsigned long long x = 8ull * (unsigned long long)m.k_all + 16 + 8ull * rc(m.t) + 8;
+    if (m.s != m.r) {
+        x += 12ull * (rc(m.r) + rc(m.s)) + 16ull * (unsigned long long)nb;
+        if (!greedy) x += 8ull * rc(m.s) + 8ull * (unsigned long long)nb;
+    }
+    return x;
 }
 
 // Hastings correction over the neighbour blocks (src/entropy.cpp:497-561) and the accept test
@@ -387,8 +401,7 @@ __device__ __noinline__ bool hastings_accept(const FtArgs &a, const MoveCtx &m, 
 // through one block degree d -> d', contributing M * (fastlog(d') - fastlog(d)), so the four bulk terms
 // are (row or column total - Delta and special cells) * (fastlog(d') - fastlog(d)).
 template <bool GREEDY>
-__device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept,
-                                unsigned long long &bytes) {
+__device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept) {
     const int lane = lane_id();
     const int v = m.v, r = m.r, s = m.s;
     for (int64_t e = m.o0 + lane; e < m.o1; e += WAVE) {
@@ -455,14 +468,11 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
     fx_acc(acc, (double)S_rcol * (fastlog(new_din(a, m, r)) - fastlog(a.d_in[r])));
     fx_acc(acc, (double)S_scol * (fastlog(new_din(a, m, s)) - fastlog(a.d_in[s])));
     const double dS = fx_to(acc);
-    // 2 neighbour-block keys + 4 lookups of ~log2(row) probes per Delta cell, 8 per lookup probe
-    bytes += 16ull * (unsigned long long)nb + 8ull * 4 * 10 * (unsigned long long)nb + 64;
     if (GREEDY) {
         accept = dS < 0.0;
         return dS;
     }
     accept = hastings_accept(a, m, t, w, dS, nb, mrr, mss);
-    bytes += 8ull * 6 * 10 * (unsigned long long)nb;
     return dS;
 }
 
@@ -487,21 +497,23 @@ __global__ void __launch_bounds__(FT_BLOCK) k_ft_propose(FtArgs a) {
         }
         MoveCtx m;
         U4 w;
-        const int s = propose_block(a, v, m, w, bytes);
+        const int s = propose_block(a, v, m, w);
         if (a.s_out && lane == 0) a.s_out[v] = s;
         if (s == m.r) {
             if (lane == 0) {
                 a.move_s[v] = -1;
                 if (a.dS_out) a.dS_out[v] = 0.0;
             }
+            bytes += proposal_bytes(a, m, 0, GREEDY);
             continue;
         }
         bool accept;
-        const double dS = evaluate_move<GREEDY>(a, m, t, w, accept, bytes);
+        const double dS = evaluate_move<GREEDY>(a, m, t, w, accept);
         if (lane == 0) {
             a.move_s[v] = accept ? s : -1;
             if (a.dS_out) a.dS_out[v] = dS;
         }
+        bytes += proposal_bytes(a, m, *t.n_used, GREEDY);
         table_clear_used(t);
     }
     if (lane == 0 && bytes) atomicAdd(a.bytes, bytes);
@@ -554,7 +566,7 @@ static FtArgs base_args(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
 FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int greedy, const int32_t *d_vertices,
                         int64_t count, int32_t *d_move_s, double *d_dS, int32_t *d_s_out) {
     hipStream_t st = ctx->stream;
-    FtLaunchInfo info{0, count, 0.f};
+    FtLaunchInfo info{0, 0, count, 0.f, 0.f, false};
     if (count == 0) return info;
     auto *counters = ctx->scratch[S_COUNTER].as<unsigned long long>(4);
     SBP_HIP(hipMemsetAsync(counters, 0, 4 * sizeof(unsigned long long), st));
@@ -574,7 +586,10 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     if (greedy) k_ft_propose<false, true><<<(int)blocks, FT_BLOCK, 0, st>>>(a);
     else k_ft_propose<false, false><<<(int)blocks, FT_BLOCK, 0, st>>>(a);
     SBP_HIP(hipGetLastError());
-    if (g->max_degree > SMALL_K) {
+    SBP_HIP(hipEventRecord(ctx->ev[6], st));
+    info.high_launched = g->max_degree > SMALL_K;
+    if (info.high_launched) {
+        a.bytes = counters + 3;
         // one wave per deferred vertex where the tables fit in kMaxTableBytes; a table holds the distinct
         // neighbour blocks of one vertex at load factor <= 1/2
         const int lg = std::max(10, log2_ceil(2 * std::min<int64_t>(g->max_degree, bm->B) + 2));
@@ -590,12 +605,14 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
         SBP_HIP(hipGetLastError());
     }
     SBP_HIP(hipEventRecord(ctx->ev[1], st));
-    unsigned long long hb[2] = {0, 0};
+    unsigned long long hb[4] = {0, 0, 0, 0};
     SBP_HIP(hipMemcpyAsync(hb, counters, sizeof(hb), hipMemcpyDeviceToHost, st));
     SBP_HIP(hipStreamSynchronize(st));
     SBP_CHECK(hb[1] == 0, SBP_E_STATE, "finetune proposal: block degrees inconsistent with the block matrix");
     info.bytes = hb[0];
-    info.kernel_ms = (float)elapsed_ms(ctx, ctx->ev[0], ctx->ev[1]);
+    info.high_bytes = hb[3];
+    info.kernel_ms = (float)elapsed_ms(ctx, ctx->ev[0], ctx->ev[6]);
+    if (info.high_launched) info.high_kernel_ms = (float)elapsed_ms(ctx, ctx->ev[6], ctx->ev[1]);
     return info;
 }
 
@@ -865,10 +882,14 @@ __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start)
         const Table &t = kall > SMALL_K ? big_t : small_t;
         MoveCtx m;
         U4 w;
-        const int s = propose_block(a, v, m, w, bytes);
-        if (s == m.r) continue;
+        const int s = propose_block(a, v, m, w);
+        if (s == m.r) {
+            bytes += proposal_bytes(a, m, 0, GREEDY);
+            continue;
+        }
         bool accept;
-        const double dS = evaluate_move<GREEDY>(a, m, t, w, accept, bytes);
+        const double dS = evaluate_move<GREEDY>(a, m, t, w, accept);
+        bytes += proposal_bytes(a, m, *t.n_used, GREEDY);
         if (!accept) {
             table_clear_used(t);
             continue;
@@ -1090,10 +1111,10 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
     if (stats) {
         stats->proposals += count;
         stats->moves += (int64_t)res.moves;
-        stats->ft_bytes += hb[0];
-        stats->ft_kernel_ms += kernel_ms;
+        stats->mh_bytes += hb[0];
+        stats->mh_kernel_ms += kernel_ms;
         stats->finetune_ms += kernel_ms;
-        stats->ft_launches += 1;
+        stats->mh_launches += 1;
     }
 }
 

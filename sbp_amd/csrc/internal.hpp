@@ -125,9 +125,12 @@ double log_posterior(sbp_ctx ctx, sbp_bm bm);
 double model_term(int64_t E, int32_t N, int32_t B);
 // finetune.hip
 struct FtLaunchInfo {
-    uint64_t bytes;
+    uint64_t bytes;       // algorithmic bytes of the LDS-table launch (vertices up to FT_SMALL_DEGREE)
+    uint64_t high_bytes;  // ... of the high-degree launch
     int64_t proposals;
-    float kernel_ms;
+    float kernel_ms;      // device time of the LDS-table launch
+    float high_kernel_ms; // device time of the high-degree launch (0 when not launched)
+    bool high_launched;
 };
 FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int greedy, const int32_t *d_vertices,
                         int64_t count, int32_t *d_move_s, double *d_dS, int32_t *d_accept);

@@ -44,7 +44,10 @@ class Stats(C.Structure):
                 ("finetune_ms", C.c_double), ("merge_ms", C.c_double), ("carry_ms", C.c_double),
                 ("build_ms", C.c_double), ("mdl_ms", C.c_double), ("total_ms", C.c_double),
                 ("ft_bytes", C.c_uint64), ("merge_bytes", C.c_uint64), ("ft_launches", C.c_int64),
-                ("ft_kernel_ms", C.c_double)]
+                ("ft_kernel_ms", C.c_double), ("carry_evaluations", C.c_int64), ("carry_speculative", C.c_int64),
+                ("carry_reused", C.c_int64), ("carry_requeues", C.c_int64), ("carry_load_ms", C.c_double),
+                ("ft_high_bytes", C.c_uint64), ("ft_high_launches", C.c_int64), ("ft_high_kernel_ms", C.c_double),
+                ("mh_bytes", C.c_uint64), ("mh_launches", C.c_int64), ("mh_kernel_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
