@@ -101,6 +101,26 @@ int sbp_graph_upload(sbp_ctx ctx, int32_t N, int64_t E, const int64_t *out_off, 
                      const int64_t *in_off, const int32_t *in_adj, sbp_graph *out);
 int sbp_graph_free(sbp_graph g);
 
+int sbp_graph_info(sbp_graph g, int32_t *num_vertices, int64_t *num_edges);
+/* Copies a graph's CSR to host (e.g. a sample's induced subgraph); any pointer may be NULL. */
+int sbp_graph_download(sbp_ctx ctx, sbp_graph g, int64_t *out_off, int32_t *out_adj, int64_t *in_off,
+                       int32_t *in_adj);
+
+/* ----------------------------------------------------------------- sampling */
+
+/* Replaces sample::sample (src/sample.cpp:352-365) with sample::from_vertices (:202-217): draws
+ * (long)(fraction * N) vertices and builds their induced subgraph on the device.  alg: 0 max_degree
+ * (:219-233), 1 random (:235-247), 2 degree_product (:28-49), 3 degree_weighted (new: weighted sampling
+ * without replacement, weight = degree).  The sample keeps its vertex mapping for sbp_extend.
+ * mapping_out: host int32[N] (vertex -> sample vertex or -1), may be NULL. */
+int sbp_sample(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, uint64_t seed, sbp_graph *sample_out,
+               int32_t *num_sampled, int32_t *mapping_out);
+/* Replaces sample::extend (src/sample.cpp:157-200) followed by Blockmodel(B, graph, 0.5, assignment)
+ * (src/main.cpp:139): extends sample_bm, a blockmodel of `sample`, to g and builds the full-graph
+ * blockmodel with the same B.  assignment_out: host int32[N], may be NULL. */
+int sbp_extend(sbp_ctx ctx, sbp_graph g, sbp_graph sample, sbp_bm sample_bm, uint64_t seed, sbp_bm *out,
+               int32_t *assignment_out);
+
 /* ----------------------------------------------------------------- blockmodel */
 
 /* Replaces Blockmodel(num_blocks, graph, rate, assignment) + initialize_edge_counts

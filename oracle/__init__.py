@@ -77,6 +77,8 @@ def lib():
             "orc_merge_blocks": (vp, [vp, G, P, u32, S]),
             "orc_sbp": (C.c_int, [G, P, vp, vp, S]),
             "orc_sample": (i64, [G, i32, dbl, u64, vp, vp]),
+            "orc_degree_weight_key": (C.c_float, [u32, i64]),
+            "orc_subgraph": (i64, [G, vp, vp, i32, vp, vp, vp, vp]),
             "orc_extend": (C.c_int, [G, vp, vp, i32, u64, vp]),
         }
         for name, (res, args) in sig.items():
@@ -219,6 +221,12 @@ class OBlockmodel:
                        C.byref(st))
         return ep.value, st
 
+    def finetune_assignment(self, prm, epoch):
+        ep = C.c_uint32(epoch)
+        st = Stats()
+        lib().orc_finetune_assignment(self.h, self.g.ref(), C.byref(prm), C.byref(ep), C.byref(st))
+        return ep.value, st
+
     def merge_propose(self, seed, epoch):
         B = self.B
         best_s = np.zeros(B, np.int32)
@@ -257,11 +265,33 @@ def sbp(graph, prm):
     return assign, nb.value, st
 
 
+SAMPLING = {"max_degree": 0, "random": 1, "degree_product": 2, "degree_weighted": 3}
+
+
 def sample(graph, alg, fraction, seed=0):
+    """orc_sample: (mapping[N] full -> sample id or -1, vertices[n] sample -> full id)."""
+    a = SAMPLING[alg] if isinstance(alg, str) else int(alg)
     mapping = np.zeros(graph.N, np.int32)
     verts = np.zeros(graph.N, np.int32)
-    n = lib().orc_sample(graph.ref(), alg, fraction, seed, _p(mapping), _p(verts))
+    n = lib().orc_sample(graph.ref(), a, fraction, seed, _p(mapping), _p(verts))
     return mapping, verts[:n]
+
+
+def degree_weight_key(w0, degree):
+    return lib().orc_degree_weight_key(int(w0), int(degree))
+
+
+def subgraph(graph, mapping, vertices):
+    """orc_subgraph (sample::from_vertices): the induced subgraph as an OGraph."""
+    n = len(vertices)
+    m = np.ascontiguousarray(mapping, np.int32)
+    vs = np.ascontiguousarray(vertices, np.int32)
+    out_off = np.zeros(n + 1, np.int64)
+    in_off = np.zeros(n + 1, np.int64)
+    out_adj = np.zeros(max(graph.E, 1), np.int32)
+    in_adj = np.zeros(max(graph.E, 1), np.int32)
+    E = lib().orc_subgraph(graph.ref(), _p(m), _p(vs), n, _p(out_off), _p(out_adj), _p(in_off), _p(in_adj))
+    return OGraph(out_off, out_adj[:E], in_off, in_adj[:E])
 
 
 def extend(graph, mapping, sample_assignment, sample_blocks, seed=0):

@@ -1472,30 +1472,129 @@ int orc_sbp(const orc_graph *g, const orc_params *p, int32_t *assignment_out, in
 /* Sampling (src/sample.cpp)                                                  */
 /* ------------------------------------------------------------------------- */
 
-/* alg 0 = max_degree (stable descending degree, sample.cpp:219-233), alg 1 = random (keyed shuffle,
- * sample.cpp:235-247).  Writes mapping[N] (full id -> sample id or -1) and vertices[n] (sample -> full). */
+/* Sampling orders.  Each returns the sampled vertices in sample-id order.
+ *   alg 0 = max_degree (sample.cpp:219-233): stable descending degree (utils::argsort), ties by vertex id;
+ *   alg 1 = random (sample.cpp:235-247): ascending Philox key (TAG_SAMPLE, epoch 0), ties by vertex id;
+ *   alg 2 = degree_product (sample.cpp:28-49): out-edges in CSR order, stable-sorted by descending
+ *           degree(src) * degree(dst) (Graph::sorted_edge_list, graph.cpp:302-316); endpoints are taken in
+ *           that order (source before destination) until the count reaches the target, checked after each
+ *           edge;
+ *   alg 3 = degree_weighted (new): weighted sampling without replacement with weights = degree, by
+ *           Efraimidis-Spirakis keys: key(v) = -logf(u_v) / degree(v) in float, u_v = (2 (w0 >> 9) + 1) 2^-24,
+ *           ascending key (degree 0: +inf), ties by vertex id.
+ * The target is (long)(fraction * N) (sample.cpp:228).  Writes mapping[N] (full id -> sample id or -1) and
+ * vertices[n] (sample id -> full id). */
+typedef struct { uint64_t key; int64_t idx; } key64_t;
+static int key64_cmp(const void *a, const void *b) {
+    const key64_t *x = (const key64_t *)a, *y = (const key64_t *)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return (x->idx > y->idx) - (x->idx < y->idx);
+}
+float orc_degree_weight_key(uint32_t w0, int64_t degree) {
+    if (degree <= 0) return INFINITY;
+    float u = (float)(((w0 >> 9) << 1) | 1u) * 0x1p-24f;
+    return -logf(u) / (float)degree;
+}
 int64_t orc_sample(const orc_graph *g, int32_t alg, double fraction, uint64_t seed, int32_t *mapping,
                    int32_t *vertices) {
     int32_t N = g->N;
+    int64_t target = (int64_t)(fraction * (double)N);
     int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    int64_t n = 0;
+    for (int32_t v = 0; v < N; ++v) mapping[v] = -1;
     if (alg == 0) {
         degv_t *d = (degv_t *)malloc(sizeof(degv_t) * (size_t)N);
         for (int32_t v = 0; v < N; ++v) { d[v].deg = vertex_degree(g, v); d[v].v = v; }
         qsort(d, (size_t)N, sizeof(degv_t), degv_cmp);
         for (int32_t i = 0; i < N; ++i) order[i] = d[i].v;
         free(d);
-    } else {
+        n = target;
+    } else if (alg == 1) {
         for (int32_t i = 0; i < N; ++i) order[i] = i;
         keyed_shuffle(order, N, seed, 0, ORC_TAG_SAMPLE);
+        n = target;
+    } else if (alg == 2) {
+        int64_t E = g->E;
+        int64_t *deg = (int64_t *)malloc(sizeof(int64_t) * (size_t)N);
+        for (int32_t v = 0; v < N; ++v) deg[v] = vertex_degree(g, v);
+        key64_t *k = (key64_t *)malloc(sizeof(key64_t) * (size_t)(E ? E : 1));
+        int32_t *src = (int32_t *)malloc(sizeof(int32_t) * (size_t)(E ? E : 1));
+        for (int32_t v = 0; v < N; ++v)
+            for (int64_t e = g->out_off[v]; e < g->out_off[v + 1]; ++e) src[e] = v;
+        for (int64_t e = 0; e < E; ++e) {
+            k[e].key = ~(uint64_t)(deg[src[e]] * deg[g->out_adj[e]]);  /* descending product */
+            k[e].idx = e;                                             /* stable */
+        }
+        qsort(k, (size_t)E, sizeof(key64_t), key64_cmp);
+        for (int64_t i = 0; i < E; ++i) {
+            int32_t ends[2] = { src[k[i].idx], g->out_adj[k[i].idx] };
+            for (int j = 0; j < 2; ++j)
+                if (mapping[ends[j]] == -1) { mapping[ends[j]] = (int32_t)n; order[n++] = ends[j]; }
+            if (n >= target) break;
+        }
+        free(k); free(src); free(deg);
+    } else {
+        key64_t *k = (key64_t *)malloc(sizeof(key64_t) * (size_t)N);
+        for (int32_t v = 0; v < N; ++v) {
+            uint32_t w[4];
+            orc_philox(seed, (uint32_t)v, 0, ORC_TAG_SAMPLE, 0, w);
+            float key = orc_degree_weight_key(w[0], vertex_degree(g, v));
+            uint32_t bits;
+            memcpy(&bits, &key, 4);
+            k[v].key = bits;
+            k[v].idx = v;
+        }
+        qsort(k, (size_t)N, sizeof(key64_t), key64_cmp);
+        for (int32_t i = 0; i < N; ++i) order[i] = (int32_t)k[i].idx;
+        free(k);
+        n = target;
     }
-    int64_t n = (int64_t)(fraction * (double)N);
-    for (int32_t v = 0; v < N; ++v) mapping[v] = -1;
-    for (int64_t i = 0; i < n; ++i) {
-        mapping[order[i]] = (int32_t)i;
-        vertices[i] = order[i];
-    }
+    if (alg != 2)
+        for (int64_t i = 0; i < n; ++i) mapping[order[i]] = (int32_t)i;
+    for (int64_t i = 0; i < n; ++i) vertices[i] = order[i];
     free(order);
     return n;
+}
+
+/* sample::from_vertices (src/sample.cpp:202-217): the induced subgraph in sample ids.  Edges are added
+ * for full vertices in ascending id, each in its out-list order (Graph::add_edge appends to the source's
+ * out-list and the destination's in-list), so a sample vertex's out-list keeps its full out-list order and
+ * its in-list is ordered by (full source id, position in the source's out-list).  Buffers hold n + 1
+ * offsets and up to E entries.  Returns the sample's edge count. */
+int64_t orc_subgraph(const orc_graph *g, const int32_t *mapping, const int32_t *vertices, int32_t n,
+                     int64_t *out_off, int32_t *out_adj, int64_t *in_off, int32_t *in_adj) {
+    int32_t N = g->N;
+    for (int32_t i = 0; i <= n; ++i) { out_off[i] = 0; in_off[i] = 0; }
+    for (int32_t i = 0; i < n; ++i) {
+        int32_t f = vertices[i];
+        for (int64_t e = g->out_off[f]; e < g->out_off[f + 1]; ++e) {
+            int32_t d = mapping[g->out_adj[e]];
+            if (d < 0) continue;
+            out_off[i + 1]++;
+            in_off[d + 1]++;
+        }
+    }
+    for (int32_t i = 0; i < n; ++i) { out_off[i + 1] += out_off[i]; in_off[i + 1] += in_off[i]; }
+    int64_t *ipos = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    for (int32_t i = 0; i <= n; ++i) ipos[i] = in_off[i];
+    for (int32_t i = 0; i < n; ++i) {
+        int64_t o = out_off[i];
+        int32_t f = vertices[i];
+        for (int64_t e = g->out_off[f]; e < g->out_off[f + 1]; ++e) {
+            int32_t d = mapping[g->out_adj[e]];
+            if (d >= 0) out_adj[o++] = d;
+        }
+    }
+    for (int32_t f = 0; f < N; ++f) {  /* in-lists in ascending full source id */
+        int32_t sf = mapping[f];
+        if (sf < 0) continue;
+        for (int64_t e = g->out_off[f]; e < g->out_off[f + 1]; ++e) {
+            int32_t d = mapping[g->out_adj[e]];
+            if (d >= 0) in_adj[ipos[d]++] = sf;
+        }
+    }
+    free(ipos);
+    return out_off[n];
 }
 
 /* sample::extend (src/sample.cpp:157-200): sequential plurality vote; ties to the smallest block. */

@@ -141,6 +141,9 @@ int      orc_sbp(const orc_graph *g, const orc_params *p, int32_t *assignment_ou
 /* Sampling */
 int64_t  orc_sample(const orc_graph *g, int32_t alg, double fraction, uint64_t seed, int32_t *mapping,
                     int32_t *vertices);
+float    orc_degree_weight_key(uint32_t w0, int64_t degree);
+int64_t  orc_subgraph(const orc_graph *g, const int32_t *mapping, const int32_t *vertices, int32_t n,
+                      int64_t *out_off, int32_t *out_adj, int64_t *in_off, int32_t *in_adj);
 int      orc_extend(const orc_graph *g, const int32_t *mapping, const int32_t *sample_assignment,
                     int32_t sample_blocks, uint64_t seed, int32_t *assignment_out);
 

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsbp_amd.so")
-SOURCES = ["csrc/build.hip", "csrc/mdl.hip", "csrc/finetune.hip", "csrc/merge.hip", "csrc/capi.hip",
+SOURCES = ["csrc/build.hip", "csrc/mdl.hip", "csrc/finetune.hip", "csrc/merge.hip", "csrc/graph.hip", "csrc/capi.hip",
            "host/sbp_api.cpp", "host/args.cpp"]
 CLI = os.path.join(HERE, "sbp_cli")
 CARRY_LIB = os.path.join(HERE, "libsbp_carry.so")  # host-only carry-out, for CPU-side checks

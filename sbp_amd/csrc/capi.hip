@@ -136,28 +136,12 @@ static bool early_stop3(int64_t iteration, double initial, const std::vector<dou
     return average < 1e-4 * initial;
 }
 
-// Graph::sort_vertices (src/graph.cpp:235-268): stable descending degree; top ceil(mh_percent*N) are high.
-static void split_by_degree(sbp_graph g, double mh_percent, std::vector<int32_t> &hdv, std::vector<int32_t> &ldv) {
-    const int32_t N = g->N;
-    std::vector<int64_t> deg(N);
-    for (int32_t v = 0; v < N; ++v) {
-        int64_t self = 0;
-        for (int64_t e = g->h_out_off[v]; e < g->h_out_off[v + 1]; ++e)
-            if (g->h_out_adj[e] == v) {
-                self = 1;
-                break;
-            }
-        deg[v] = (g->h_out_off[v + 1] - g->h_out_off[v]) + (g->h_in_off[v + 1] - g->h_in_off[v]) - self;
-    }
-    std::vector<int32_t> idx(N);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return deg[a] > deg[b]; });
-    hdv.clear();
-    ldv.clear();
-    for (int32_t i = 0; i < N; ++i) {
-        if ((double)i < mh_percent * (double)N) hdv.push_back(idx[i]);
-        else ldv.push_back(idx[i]);
-    }
+// Graph::sort_vertices (src/graph.cpp:235-268): stable descending degree; the first ceil(mh_percent * N)
+// vertices are the high-degree ones.
+static int64_t high_degree_count(sbp_graph g, double mh_percent) {
+    int64_t nh = 0;
+    while (nh < g->N && (double)nh < mh_percent * (double)g->N) ++nh;
+    return nh;
 }
 
 void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t epoch, const int32_t *d_order,
@@ -171,14 +155,13 @@ static void mcmc(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, bool 
     if (p.algorithm == 0 || p.algorithm == 2) {
         double initial = mdl_of(ctx, bm, g, st), last = initial;
         bm->overall_entropy = initial;
-        std::vector<int32_t> hdv, ldv;
-        int32_t *d_h = nullptr, *d_l = nullptr;
+        const int32_t *d_h = nullptr, *d_l = nullptr;
+        int64_t nh = 0, nl = 0;
         if (p.algorithm == 2) {
-            split_by_degree(g, p.mh_percent, hdv, ldv);
-            SBP_HIP(hipMalloc(&d_h, sizeof(int32_t) * (hdv.size() + 1)));
-            SBP_HIP(hipMalloc(&d_l, sizeof(int32_t) * (ldv.size() + 1)));
-            SBP_HIP(hipMemcpy(d_h, hdv.data(), sizeof(int32_t) * hdv.size(), hipMemcpyHostToDevice));
-            SBP_HIP(hipMemcpy(d_l, ldv.data(), sizeof(int32_t) * ldv.size(), hipMemcpyHostToDevice));
+            d_h = degree_order(ctx, g);
+            nh = high_degree_count(g, p.mh_percent);
+            nl = g->N - nh;
+            d_l = d_h + nh;
         }
         for (int64_t it = 0; it < max_it; ++it) {
             uint32_t ep = (*epoch)++;
@@ -186,7 +169,6 @@ static void mcmc(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, bool 
                 async_gibbs_sweep(ctx, bm, g, p, ep, st);
             } else {
                 // hybrid_mcmc (src/finetune.cpp:563-646)
-                const int64_t nh = (int64_t)hdv.size(), nl = (int64_t)ldv.size();
                 int32_t *h = ctx->scratch[S_ORDER].as<int32_t>(nh + nl + 2);
                 int32_t *l = h + nh + 1;
                 if (!p.ordered) {
@@ -211,8 +193,6 @@ static void mcmc(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, bool 
             last = e;
             if (early_stop4(it, golden_ratio_not_reached, initial, de)) break;
         }
-        if (d_h) (void)hipFree(d_h);
-        if (d_l) (void)hipFree(d_l);
         bm->overall_entropy = last;
     } else {
         // metropolis_hastings (src/finetune.cpp:773-820)
@@ -414,54 +394,95 @@ int sbp_graph_upload(sbp_ctx ctx, int32_t N, int64_t E, const int64_t *out_off, 
     SBP_API_BEGIN
     SBP_CHECK(out_off[0] == 0 && out_off[N] == E && in_off[0] == 0 && in_off[N] == E, SBP_E_ARG,
               "graph offsets do not match E");
-    auto g = std::make_unique<sbp_graph_s>();
+    for (int32_t v = 0; v < N; ++v) {
+        SBP_CHECK(out_off[v + 1] >= out_off[v] && in_off[v + 1] >= in_off[v], SBP_E_ARG, "offsets not monotone");
+        for (int64_t e = out_off[v]; e < out_off[v + 1]; ++e)
+            SBP_CHECK(out_adj[e] >= 0 && out_adj[e] < N, SBP_E_ARG, "out_adj id out of range");
+        for (int64_t e = in_off[v]; e < in_off[v + 1]; ++e)
+            SBP_CHECK(in_adj[e] >= 0 && in_adj[e] < N, SBP_E_ARG, "in_adj id out of range");
+    }
+    auto g = std::unique_ptr<sbp_graph_s, int (*)(sbp_graph)>(new sbp_graph_s(), sbp_graph_free);
     g->ctx = ctx;
     g->N = N;
     g->E = E;
-    g->h_out_off.assign(out_off, out_off + N + 1);
-    g->h_in_off.assign(in_off, in_off + N + 1);
-    g->h_out_adj.assign(out_adj, out_adj + E);
-    g->h_in_adj.assign(in_adj, in_adj + E);
-    std::vector<int32_t> src((size_t)E + N);
-    int64_t maxdeg = 0, nhigh = 0;
-    for (int32_t v = 0; v < N; ++v) {
-        SBP_CHECK(out_off[v + 1] >= out_off[v] && in_off[v + 1] >= in_off[v], SBP_E_ARG, "offsets not monotone");
-        for (int64_t e = out_off[v]; e < out_off[v + 1]; ++e) {
-            SBP_CHECK(out_adj[e] >= 0 && out_adj[e] < N, SBP_E_ARG, "out_adj id out of range");
-            src[e] = v;
-        }
-        for (int64_t e = in_off[v]; e < in_off[v + 1]; ++e)
-            SBP_CHECK(in_adj[e] >= 0 && in_adj[e] < N, SBP_E_ARG, "in_adj id out of range");
-        const int64_t kall = (out_off[v + 1] - out_off[v]) + (in_off[v + 1] - in_off[v]);
-        maxdeg = std::max<int64_t>(maxdeg, kall);
-        nhigh += kall > FT_SMALL_DEGREE;
-        src[(size_t)E + v] = v;
-    }
-    g->max_degree = maxdeg;
-    g->n_high_degree = nhigh;
     SBP_HIP(hipMalloc(&g->out_off, sizeof(int64_t) * (N + 1)));
     SBP_HIP(hipMalloc(&g->in_off, sizeof(int64_t) * (N + 1)));
     SBP_HIP(hipMalloc(&g->out_adj, sizeof(int32_t) * (E + 1)));
     SBP_HIP(hipMalloc(&g->in_adj, sizeof(int32_t) * (E + 1)));
-    SBP_HIP(hipMalloc(&g->edge_src, sizeof(int32_t) * (E + N)));
     SBP_HIP(hipMemcpy(g->out_off, out_off, sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
     SBP_HIP(hipMemcpy(g->in_off, in_off, sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
     if (E > 0) {
         SBP_HIP(hipMemcpy(g->out_adj, out_adj, sizeof(int32_t) * E, hipMemcpyHostToDevice));
         SBP_HIP(hipMemcpy(g->in_adj, in_adj, sizeof(int32_t) * E, hipMemcpyHostToDevice));
     }
-    SBP_HIP(hipMemcpy(g->edge_src, src.data(), sizeof(int32_t) * (E + N), hipMemcpyHostToDevice));
+    graph_finish(ctx, g.get());
     *out = g.release();
     SBP_API_END(ctx)
 }
 
 int sbp_graph_free(sbp_graph g) {
     if (!g) return SBP_OK;
-    void *p[] = {g->out_off, g->in_off, g->out_adj, g->in_adj, g->edge_src};
+    void *p[] = {g->out_off, g->in_off,   g->out_adj,    g->in_adj,         g->edge_src,
+                 g->degree,  g->deg_order, g->parent_map, g->sample_vertices};
     for (void *x : p)
         if (x) (void)hipFree(x);
     delete g;
     return SBP_OK;
+}
+
+int sbp_graph_info(sbp_graph g, int32_t *num_vertices, int64_t *num_edges) {
+    if (!g) return SBP_E_ARG;
+    if (num_vertices) *num_vertices = g->N;
+    if (num_edges) *num_edges = g->E;
+    return SBP_OK;
+}
+
+int sbp_graph_download(sbp_ctx ctx, sbp_graph g, int64_t *out_off, int32_t *out_adj, int64_t *in_off,
+                       int32_t *in_adj) {
+    if (!ctx || !g) return SBP_E_ARG;
+    SBP_API_BEGIN
+    SBP_HIP(hipStreamSynchronize(ctx->stream));
+    const size_t no = sizeof(int64_t) * ((size_t)g->N + 1), na = sizeof(int32_t) * (size_t)g->E;
+    if (out_off) SBP_HIP(hipMemcpy(out_off, g->out_off, no, hipMemcpyDeviceToHost));
+    if (in_off) SBP_HIP(hipMemcpy(in_off, g->in_off, no, hipMemcpyDeviceToHost));
+    if (out_adj && na) SBP_HIP(hipMemcpy(out_adj, g->out_adj, na, hipMemcpyDeviceToHost));
+    if (in_adj && na) SBP_HIP(hipMemcpy(in_adj, g->in_adj, na, hipMemcpyDeviceToHost));
+    SBP_API_END(ctx)
+}
+
+int sbp_sample(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, uint64_t seed, sbp_graph *sample_out,
+               int32_t *num_sampled, int32_t *mapping_out) {
+    if (!ctx || !g || !sample_out || !(fraction >= 0.0 && fraction <= 1.0)) return SBP_E_ARG;
+    *sample_out = nullptr;
+    SBP_API_BEGIN
+    sbp_graph s = sample_graph(ctx, g, alg, fraction, seed);
+    if (num_sampled) *num_sampled = s->N;
+    if (mapping_out) SBP_HIP(hipMemcpy(mapping_out, s->parent_map, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
+    *sample_out = s;
+    SBP_API_END(ctx)
+}
+
+int sbp_extend(sbp_ctx ctx, sbp_graph g, sbp_graph sample, sbp_bm sample_bm, uint64_t seed, sbp_bm *out,
+               int32_t *assignment_out) {
+    if (!ctx || !g || !sample || !sample_bm || !out) return SBP_E_ARG;
+    *out = nullptr;
+    SBP_API_BEGIN
+    sbp_bm bm = new_bm(ctx);
+    try {
+        bm_alloc(bm, sample_bm->B, g->N);
+        extend_assignment(ctx, g, sample, sample_bm, seed, bm->assign);
+        bm_rebuild(ctx, g, bm);
+        SBP_HIP(hipStreamSynchronize(ctx->stream));
+        bm->num_to_merge = (int64_t)((double)bm->B * 0.5);
+        if (assignment_out)
+            SBP_HIP(hipMemcpy(assignment_out, bm->assign, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
+    } catch (...) {
+        bm_release(bm);
+        delete bm;
+        throw;
+    }
+    *out = bm;
+    SBP_API_END(ctx)
 }
 
 int sbp_bm_build(sbp_ctx ctx, sbp_graph g, int32_t B, const int32_t *assignment, sbp_bm *out) {

@@ -1,0 +1,106 @@
+// Per-wavefront open-addressing histogram (block id -> out count, in count) in LDS or global memory.
+// The slots claimed for the current item are listed in `used`, so clearing costs O(distinct keys) rather
+// than O(capacity).  Layout of one table of capacity C: keys[C], outc[C], inc[C], used[C], n_used[1].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.hpp"
+
+namespace sbpdev {
+
+constexpr int EMPTY = -1;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+struct Table {
+    int *keys;
+    int *outc;
+    int *inc;
+    int *used;
+    int *n_used;
+    unsigned mask;
+    int shift;  // 32 - log2(cap)
+    int cap;
+};
+
+// ints of one table of capacity 2^cap_log2
+__host__ __device__ constexpr int64_t table_ints(int cap_log2) { return 4 * ((int64_t)1 << cap_log2) + 1; }
+
+__device__ __forceinline__ Table table_at(int *base, int cap_log2) {
+    Table t;
+    t.cap = 1 << cap_log2;
+    t.shift = 32 - cap_log2;
+    t.keys = base;
+    t.outc = t.keys + t.cap;
+    t.inc = t.outc + t.cap;
+    t.used = t.inc + t.cap;
+    t.n_used = t.used + t.cap;
+    t.mask = (unsigned)t.cap - 1;
+    return t;
+}
+
+__device__ __forceinline__ unsigned hash_slot(const Table &t, int key) {
+    return ((unsigned)key * 0x9E3779B1u) >> t.shift;
+}
+
+__device__ __forceinline__ void table_add(const Table &t, int key, int dout, int din) {
+    unsigned slot = hash_slot(t, key);
+    while (true) {
+        int prev = atomicCAS(&t.keys[slot], EMPTY, key);
+        if (prev == EMPTY) t.used[atomicAdd(t.n_used, 1)] = (int)slot;
+        if (prev == EMPTY || prev == key) {
+            if (dout) atomicAdd(&t.outc[slot], dout);
+            if (din) atomicAdd(&t.inc[slot], din);
+            return;
+        }
+        slot = (slot + 1) & t.mask;
+    }
+}
+
+__device__ __forceinline__ void table_get(const Table &t, int key, int &o, int &i) {
+    unsigned slot = hash_slot(t, key);
+    while (true) {
+        int k = t.keys[slot];
+        if (k == key) {
+            o = t.outc[slot];
+            i = t.inc[slot];
+            return;
+        }
+        if (k == EMPTY) {
+            o = 0;
+            i = 0;
+            return;
+        }
+        slot = (slot + 1) & t.mask;
+    }
+}
+
+__device__ __forceinline__ void table_clear_all(const Table &t) {
+    for (int i = lane_id(); i < t.cap; i += WAVE) {
+        t.keys[i] = EMPTY;
+        t.outc[i] = 0;
+        t.inc[i] = 0;
+    }
+    if (lane_id() == 0) *t.n_used = 0;
+    wave_sync();
+}
+
+__device__ __forceinline__ void table_clear_used(const Table &t) {
+    wave_sync();
+    const int n = *t.n_used;
+    for (int i = lane_id(); i < n; i += WAVE) {
+        int slot = t.used[i];
+        t.keys[slot] = EMPTY;
+        t.outc[slot] = 0;
+        t.inc[slot] = 0;
+    }
+    wave_sync();
+    if (lane_id() == 0) *t.n_used = 0;
+    wave_sync();
+}
+
+}  // namespace sbpdev

@@ -32,6 +32,7 @@
 #include <algorithm>
 
 #include "device_common.hpp"
+#include "device_table.hpp"
 #include "internal.hpp"
 
 namespace sbpimpl {
@@ -39,89 +40,10 @@ using namespace sbpdev;
 
 constexpr int FT_BLOCK = 256;
 constexpr int FT_WAVES = FT_BLOCK / WAVE;
-constexpr int LDS_CAP = 512;          // hash slots per wave
+constexpr int LDS_CAP_LOG2 = 9;
+constexpr int LDS_CAP = 1 << LDS_CAP_LOG2;  // hash slots per wave
 constexpr int SMALL_K = FT_SMALL_DEGREE;  // max |out| + |in| for the LDS table
 static_assert(2 * SMALL_K <= LDS_CAP, "LDS table load factor above 1/2");
-constexpr int EMPTY = -1;
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-}
-
-// Open-addressing neighbour-block histogram.  Slots claimed for the current vertex are listed in `used`,
-// so clearing costs O(distinct neighbour blocks) rather than O(capacity).
-struct Table {
-    int *keys;
-    int *outc;
-    int *inc;
-    int *used;
-    int *n_used;
-    unsigned mask;
-    int shift;  // 32 - log2(cap)
-    int cap;
-};
-
-__device__ __forceinline__ unsigned hash_slot(const Table &t, int key) {
-    return ((unsigned)key * 0x9E3779B1u) >> t.shift;
-}
-
-__device__ __forceinline__ void table_add(const Table &t, int key, int dout, int din) {
-    unsigned slot = hash_slot(t, key);
-    while (true) {
-        int prev = atomicCAS(&t.keys[slot], EMPTY, key);
-        if (prev == EMPTY) t.used[atomicAdd(t.n_used, 1)] = (int)slot;
-        if (prev == EMPTY || prev == key) {
-            if (dout) atomicAdd(&t.outc[slot], dout);
-            if (din) atomicAdd(&t.inc[slot], din);
-            return;
-        }
-        slot = (slot + 1) & t.mask;
-    }
-}
-
-__device__ __forceinline__ void table_get(const Table &t, int key, int &o, int &i) {
-    unsigned slot = hash_slot(t, key);
-    while (true) {
-        int k = t.keys[slot];
-        if (k == key) {
-            o = t.outc[slot];
-            i = t.inc[slot];
-            return;
-        }
-        if (k == EMPTY) {
-            o = 0;
-            i = 0;
-            return;
-        }
-        slot = (slot + 1) & t.mask;
-    }
-}
-
-__device__ __forceinline__ void table_clear_all(const Table &t) {
-    for (int i = lane_id(); i < t.cap; i += WAVE) {
-        t.keys[i] = EMPTY;
-        t.outc[i] = 0;
-        t.inc[i] = 0;
-    }
-    if (lane_id() == 0) *t.n_used = 0;
-    wave_sync();
-}
-
-__device__ __forceinline__ void table_clear_used(const Table &t) {
-    wave_sync();
-    const int n = *t.n_used;
-    for (int i = lane_id(); i < n; i += WAVE) {
-        int slot = t.used[i];
-        t.keys[slot] = EMPTY;
-        t.outc[slot] = 0;
-        t.inc[slot] = 0;
-    }
-    wave_sync();
-    if (lane_id() == 0) *t.n_used = 0;
-    wave_sync();
-}
-
 // Read-only view of the block matrix.  Compact CSR: row b = [ptr[b], ptr[b+1]), with per-row inclusive
 // cumulative counts (columns: excluding the diagonal cell) for multinomial selection by binary search.
 // Slack layout (Metropolis-Hastings): row b = [start[b], start[b] + len[b]), no cumulative counts.
@@ -202,22 +124,8 @@ __device__ __forceinline__ long long delta_row_s(const MoveCtx &m, int c, int oc
 }
 
 __device__ __forceinline__ Table make_table(const FtArgs &a, bool big, int *lds, int64_t gwave) {
-    Table t;
-    if (big) {
-        t.cap = 1 << a.gcap_log2;
-        t.keys = a.gtable + gwave * (4 * (int64_t)t.cap + 1);
-        t.shift = 32 - a.gcap_log2;
-    } else {
-        t.cap = LDS_CAP;
-        t.keys = lds;
-        t.shift = 32 - 9;
-    }
-    t.outc = t.keys + t.cap;
-    t.inc = t.outc + t.cap;
-    t.used = t.inc + t.cap;
-    t.n_used = t.used + t.cap;
-    t.mask = (unsigned)t.cap - 1;
-    return t;
+    if (big) return table_at(a.gtable + gwave * table_ints(a.gcap_log2), a.gcap_log2);
+    return table_at(lds, LDS_CAP_LOG2);
 }
 
 // First index in [lo, hi) whose inclusive cumulative count exceeds target.
@@ -478,11 +386,11 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
 
 template <bool BIG, bool GREEDY>
 __global__ void __launch_bounds__(FT_BLOCK) k_ft_propose(FtArgs a) {
-    __shared__ int lds[BIG ? 1 : FT_WAVES * (4 * LDS_CAP + 1)];
+    __shared__ int lds[BIG ? 1 : FT_WAVES * table_ints(LDS_CAP_LOG2)];
     const int lane = lane_id(), wid = threadIdx.x / WAVE;
     const int64_t gwave = (int64_t)blockIdx.x * FT_WAVES + wid;
     const int64_t nwaves = (int64_t)gridDim.x * FT_WAVES;
-    const Table t = make_table(a, BIG, lds + (BIG ? 0 : wid * (4 * LDS_CAP + 1)), gwave);
+    const Table t = make_table(a, BIG, lds + (BIG ? 0 : wid * table_ints(LDS_CAP_LOG2)), gwave);
     table_clear_all(t);
     const int64_t count = BIG ? (int64_t)*a.big_count : a.count;
     unsigned long long bytes = 0;
@@ -593,12 +501,12 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
         // one wave per deferred vertex where the tables fit in kMaxTableBytes; a table holds the distinct
         // neighbour blocks of one vertex at load factor <= 1/2
         const int lg = std::max(10, log2_ceil(2 * std::min<int64_t>(g->max_degree, bm->B) + 2));
-        const size_t table_ints = 4 * ((size_t)1 << lg) + 1;
+        const size_t tints = (size_t)table_ints(lg);
         constexpr size_t kMaxTableBytes = size_t(1) << 30;
         int64_t waves = std::min<int64_t>(g->n_high_degree, count);
-        waves = std::max<int64_t>(1, std::min<int64_t>(waves, kMaxTableBytes / (table_ints * sizeof(int))));
+        waves = std::max<int64_t>(1, std::min<int64_t>(waves, kMaxTableBytes / (tints * sizeof(int))));
         const int64_t big_blocks = (waves + FT_WAVES - 1) / FT_WAVES;
-        a.gtable = ctx->scratch[S_HASH].as<int>((size_t)big_blocks * FT_WAVES * table_ints);
+        a.gtable = ctx->scratch[S_HASH].as<int>((size_t)big_blocks * FT_WAVES * tints);
         a.gcap_log2 = lg;
         if (greedy) k_ft_propose<true, true><<<(int)big_blocks, FT_BLOCK, 0, st>>>(a);
         else k_ft_propose<true, false><<<(int)big_blocks, FT_BLOCK, 0, st>>>(a);
@@ -865,7 +773,7 @@ __device__ __forceinline__ bool lacks_room(const FtArgs &a, const MhState &h, in
 
 template <bool GREEDY>
 __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start) {
-    __shared__ int lds[4 * LDS_CAP + 1];
+    __shared__ int lds[table_ints(LDS_CAP_LOG2)];
     const int lane = lane_id();
     unsigned long long bytes = 0, moves = 0;
     Fx sum = fx_zero();
@@ -1060,7 +968,7 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         a.bytes = bytes;
         a.err = reinterpret_cast<int *>(bytes + 1);
         const int lg = std::max(10, log2_ceil(2 * std::min<int64_t>(g->max_degree, bm->B) + 2));
-        a.gtable = g->max_degree > SMALL_K ? ctx->scratch[S_HASH].as<int>(4 * ((size_t)1 << lg) + 1) : nullptr;
+        a.gtable = g->max_degree > SMALL_K ? ctx->scratch[S_HASH].as<int>((size_t)table_ints(lg)) : nullptr;
         a.gcap_log2 = lg;
         a.m.row_cum = nullptr;
         a.m.col_cum = nullptr;

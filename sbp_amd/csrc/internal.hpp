@@ -80,11 +80,16 @@ struct sbp_graph_s {
     int64_t E = 0;
     int64_t *out_off = nullptr, *in_off = nullptr;
     int32_t *out_adj = nullptr, *in_adj = nullptr;
-    int32_t *edge_src = nullptr;    // source vertex of each out-edge (CSR order)
+    int32_t *edge_src = nullptr;    // [E + N] source vertex of each out-edge (CSR order), then 0..N-1
+    int32_t *degree = nullptr;      // [N] |out| + |in| - self-loop flag (Graph::degrees)
+    int32_t *deg_order = nullptr;   // [N] stable descending degree order, computed on first use
     int64_t max_degree = 0;         // max over v of |out(v)| + |in(v)|
     int64_t n_high_degree = 0;      // vertices with |out(v)| + |in(v)| > FT_SMALL_DEGREE
-    std::vector<int64_t> h_out_off, h_in_off;  // host copies for launch planning
-    std::vector<int32_t> h_out_adj, h_in_adj;
+    // set when this graph is a sample of `parent` (sbp_sample)
+    sbp_graph parent = nullptr;
+    int32_t parent_N = 0;
+    int32_t *parent_map = nullptr;       // [parent_N] parent vertex -> sample vertex or -1
+    int32_t *sample_vertices = nullptr;  // [N] sample vertex -> parent vertex
 };
 
 // Vertices up to this total degree keep their neighbour-block histogram in LDS during finetune proposals;
@@ -136,6 +141,11 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
                         int64_t count, int32_t *d_move_s, double *d_dS, int32_t *d_accept);
 int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_t *refused_out);
 void shuffle_keys(sbp_ctx ctx, const int32_t *d_items, int64_t n, uint32_t epoch, int32_t *d_out);
+// graph.hip
+void graph_finish(sbp_ctx ctx, sbp_graph g);
+const int32_t *degree_order(sbp_ctx ctx, sbp_graph g);
+sbp_graph sample_graph(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, uint64_t seed);
+void extend_assignment(sbp_ctx ctx, sbp_graph g, sbp_graph sample, sbp_bm sample_bm, uint64_t seed, int32_t *d_assign);
 // merge.hip
 uint64_t merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *d_best_s, double *d_best_dS);
 void relabel_assignment(sbp_ctx ctx, sbp_bm bm, const int32_t *d_block_map, int32_t new_B, int32_t *d_out);

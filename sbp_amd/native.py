@@ -16,7 +16,8 @@ ALGORITHMS = {"async_gibbs": 0, "metropolis_hastings": 1, "hybrid_mcmc": 2}
 # every symbol the header declares
 EXPORTS = [
     "sbp_create", "sbp_destroy", "sbp_last_error", "sbp_set_seed", "sbp_synchronize",
-    "sbp_graph_upload", "sbp_graph_free",
+    "sbp_graph_upload", "sbp_graph_free", "sbp_graph_info", "sbp_graph_download",
+    "sbp_sample", "sbp_extend",
     "sbp_bm_build", "sbp_bm_copy", "sbp_bm_free", "sbp_bm_info", "sbp_bm_download", "sbp_bm_validate",
     "sbp_mdl",
     "sbp_ft_propose_all", "sbp_async_gibbs_sweep", "sbp_mh_sweep", "sbp_mcmc", "sbp_finetune_assignment",
@@ -80,6 +81,10 @@ def lib():
             "sbp_synchronize": [vp],
             "sbp_graph_upload": [vp, i32, i64, vp, vp, vp, vp, vp],
             "sbp_graph_free": [vp],
+            "sbp_graph_info": [vp, vp, vp],
+            "sbp_graph_download": [vp, vp, vp, vp, vp, vp],
+            "sbp_sample": [vp, vp, i32, dbl, u64, vp, vp, vp],
+            "sbp_extend": [vp, vp, vp, vp, u64, vp, vp],
             "sbp_bm_build": [vp, vp, i32, vp, vp],
             "sbp_bm_copy": [vp, vp, vp],
             "sbp_bm_free": [vp],
@@ -151,15 +156,45 @@ class Context:
         return out
 
 
+SAMPLING = {"max_degree": 0, "random": 1, "degree_product": 2, "degree_weighted": 3}
+
+
 class DeviceGraph:
-    def __init__(self, ctx, g):
+    def __init__(self, ctx, g=None, handle=None, parent=None):
         self.ctx = ctx
+        self.parent = parent  # a sample keeps the graph it was drawn from alive
+        if handle is not None:
+            self.h = handle
+            N, E = C.c_int32(), C.c_int64()
+            ctx.check(lib().sbp_graph_info(self.h, C.byref(N), C.byref(E)), "sbp_graph_info")
+            self.N, self.E = N.value, E.value
+            return
         self.N, self.E = g.N, g.E
         self.h = C.c_void_p()
         arrs = [np.ascontiguousarray(g.out_off, np.int64), np.ascontiguousarray(g.out_adj, np.int32),
                 np.ascontiguousarray(g.in_off, np.int64), np.ascontiguousarray(g.in_adj, np.int32)]
         ctx.check(lib().sbp_graph_upload(ctx.h, g.N, g.E, *[_p(a) for a in arrs], C.byref(self.h)),
                   "sbp_graph_upload")
+
+    def download(self):
+        """(out_off, out_adj, in_off, in_adj) of the device CSR."""
+        out_off = np.zeros(self.N + 1, np.int64)
+        in_off = np.zeros(self.N + 1, np.int64)
+        out_adj = np.zeros(max(self.E, 1), np.int32)
+        in_adj = np.zeros(max(self.E, 1), np.int32)
+        self.ctx.check(lib().sbp_graph_download(self.ctx.h, self.h, _p(out_off), _p(out_adj), _p(in_off),
+                                                _p(in_adj)), "sbp_graph_download")
+        return out_off, out_adj[:self.E], in_off, in_adj[:self.E]
+
+    def sample(self, alg, fraction, seed=0):
+        """sample::sample + from_vertices on the device: (sample DeviceGraph, mapping[N])."""
+        a = SAMPLING[alg] if isinstance(alg, str) else int(alg)
+        h = C.c_void_p()
+        n = C.c_int32()
+        mapping = np.zeros(self.N, np.int32)
+        self.ctx.check(lib().sbp_sample(self.ctx.h, self.h, a, float(fraction), C.c_uint64(seed), C.byref(h),
+                                        C.byref(n), _p(mapping)), "sbp_sample")
+        return DeviceGraph(self.ctx, handle=h, parent=self), mapping
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -197,6 +232,14 @@ class DeviceBlockmodel:
         h = C.c_void_p()
         self.ctx.check(lib().sbp_bm_copy(self.ctx.h, self.h, C.byref(h)), "sbp_bm_copy")
         return DeviceBlockmodel(self.g, h)
+
+    def extend(self, full_graph, seed=0):
+        """sample::extend of this blockmodel (of a sample graph) to full_graph: (blockmodel, assignment)."""
+        h = C.c_void_p()
+        a = np.zeros(full_graph.N, np.int32)
+        self.ctx.check(lib().sbp_extend(self.ctx.h, full_graph.h, self.g.h, self.h, C.c_uint64(seed), C.byref(h),
+                                        _p(a)), "sbp_extend")
+        return DeviceBlockmodel(full_graph, h), a
 
     def info(self):
         B, nnz = C.c_int32(), C.c_int64()
