@@ -48,6 +48,8 @@ class Graph {
     void assignment(std::vector<long> a) { truth_ = std::move(a); }
     /// Device copy on the context (uploaded on first use).
     sbp_graph device(sbp_ctx ctx) const;
+    /// Takes ownership of a device graph (e.g. a sample) and downloads its CSR.
+    static Graph adopt(sbp_ctx ctx, sbp_graph g, std::shared_ptr<struct sbp_graph_s> keep_alive = nullptr);
     const std::vector<int64_t> &out_off() const { return out_off_; }
     const std::vector<int32_t> &out_adj() const { return out_adj_; }
     const std::vector<int64_t> &in_off() const { return in_off_; }
@@ -59,6 +61,7 @@ class Graph {
     std::vector<int32_t> out_adj_, in_adj_;
     std::vector<long> truth_;
     mutable std::shared_ptr<struct sbp_graph_s> dev_;
+    std::shared_ptr<struct sbp_graph_s> parent_;  // a sample's device graph refers to its parent's
 };
 
 /// Device-resident blockmodel (include/blockmodel/blockmodel.hpp:46-219).  Copy-assignment shares the
@@ -148,6 +151,19 @@ double null_mdl_v1(const Graph &graph);
 /// src/entropy.cpp:563-565
 double normalize_mdl_v1(double mdl, const Graph &graph);
 }  // namespace entropy
+
+namespace sample {
+/// include/sample.hpp:27-30: the sampled subgraph and mapping[v] = sample id of v, or -1.
+struct Sample {
+    Graph graph;
+    std::vector<long> mapping;
+};
+/// src/sample.cpp:352-365: args.samplingalg in {max_degree, random, degree_product, degree_weighted},
+/// (long)(args.samplesize * N) vertices, induced subgraph (from_vertices) built on the device.
+Sample sample(const Graph &graph);
+/// src/sample.cpp:157-200: block of every vertex of `graph` from the sample's blockmodel.
+std::vector<long> extend(const Graph &graph, const Blockmodel &sample_blockmodel, const Sample &sample);
+}  // namespace sample
 
 namespace sbp_amd {
 /// Arguments used by the namespaced entry points above (the reference reads the global `args`).

@@ -25,7 +25,7 @@ class Params(C.Structure):
     _fields_ = [("algorithm", C.c_int32), ("batches", C.c_int32), ("greedy", C.c_int32),
                 ("approximate", C.c_int32), ("ordered", C.c_int32), ("asynciterations", C.c_int32),
                 ("mh_percent", C.c_double), ("seed", C.c_uint64), ("max_iterations", C.c_int32),
-                ("threads", C.c_int32)]
+                ("threads", C.c_int32), ("nonparametric", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -59,6 +59,13 @@ def lib():
             "orc_bm_nnz": (i64, [vp]),
             "orc_bm_export": (None, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
             "orc_bm_validate": (C.c_int, [vp, G]),
+            "orc_bm_set_model": (None, [vp, i32]),
+            "orc_lgamma": (dbl, [i64]),
+            "orc_lbinom": (dbl, [i64, i64]),
+            "orc_np_sparse_entropy": (dbl, [vp]),
+            "orc_np_partition_dl": (dbl, [vp]),
+            "orc_np_edges_dl": (dbl, [i64, i64]),
+            "orc_np_mdl": (dbl, [vp, G]),
             "orc_log_posterior": (dbl, [vp]),
             "orc_mdl": (dbl, [vp, G]),
             "orc_null_mdl_v1": (dbl, [i64]),
@@ -115,9 +122,11 @@ class OGraph:
 
 
 def params(algorithm="async_gibbs", batches=1, greedy=True, approximate=False, ordered=False,
-           asynciterations=2 ** 31 - 1, mh_percent=0.075, seed=0, max_iterations=100, threads=1):
+           asynciterations=2 ** 31 - 1, mh_percent=0.075, seed=0, max_iterations=100, threads=1,
+           nonparametric=False):
     return Params(ALG[algorithm] if isinstance(algorithm, str) else algorithm, batches, int(greedy),
-                  int(approximate), int(ordered), asynciterations, mh_percent, seed, max_iterations, threads)
+                  int(approximate), int(ordered), asynciterations, mh_percent, seed, max_iterations, threads,
+                  int(nonparametric))
 
 
 class OBlockmodel:
@@ -126,9 +135,24 @@ class OBlockmodel:
         self.g = graph
 
     @classmethod
-    def build(cls, graph, B, assignment):
+    def build(cls, graph, B, assignment, nonparametric=False):
         a = np.ascontiguousarray(assignment, dtype=np.int32)
-        return cls(lib().orc_bm_new(graph.ref(), B, _p(a)), graph)
+        bm = cls(lib().orc_bm_new(graph.ref(), B, _p(a)), graph)
+        if nonparametric:
+            bm.set_model(True)
+        return bm
+
+    def set_model(self, nonparametric):
+        """Entropy model used by mdl / delta-MDL: parametric (entropy::mdl) or nonparametric."""
+        lib().orc_bm_set_model(self.h, int(nonparametric))
+
+    def np_parts(self):
+        """(sparse_entropy, partition_dl, edges_dl) of the nonparametric description length."""
+        L = lib()
+        x = self.export()
+        nonempty = int((x["sizes"] > 0).sum())
+        return (L.orc_np_sparse_entropy(self.h), L.orc_np_partition_dl(self.h),
+                L.orc_np_edges_dl(nonempty, self.g.E))
 
     def __del__(self):
         if getattr(self, "h", None) is not None and _lib is not None:

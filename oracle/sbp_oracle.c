@@ -183,6 +183,7 @@ struct orc_bm {
     double overall_entropy;
     int64_t num_to_merge;
     int empty;
+    int32_t np;  /* entropy model: 0 parametric (entropy::mdl), 1 nonparametric (entropy::nonparametric) */
 };
 
 static void bm_add(orc_bm *bm, int32_t r, int32_t c, int64_t d) {
@@ -273,6 +274,8 @@ orc_bm *orc_bm_copy(const orc_bm *src) {
 
 int32_t orc_bm_num_blocks(const orc_bm *bm) { return bm->B; }
 
+void orc_bm_set_model(orc_bm *bm, int32_t nonparametric) { bm->np = nonparametric; }
+
 int64_t orc_bm_nnz(const orc_bm *bm) {
     int64_t n = 0;
     for (int32_t b = 0; b < bm->B; ++b) n += bm->row[b].len;
@@ -341,8 +344,70 @@ static double model_term(int64_t E, int32_t N, int32_t B) {
     return ((double)E * h) + ((double)N * log((double)B));
 }
 
+double orc_np_mdl(const orc_bm *bm, const orc_graph *g);
+
 double orc_mdl(const orc_bm *bm, const orc_graph *g) {
+    if (bm->np) return orc_np_mdl(bm, g);  /* entropy::mdl, src/entropy.cpp:597-599 */
     return model_term(g->E, g->N, bm->B) - orc_log_posterior(bm);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Nonparametric description length (entropy::nonparametric,                 */
+/* src/entropy.cpp:617-1225, include/entropy.hpp:101-194): directed graphs,   */
+/* not degree-corrected.  fastlgamma(x) is glibc lgamma(double(x)), +inf at   */
+/* 0 (include/fastlgamma.hpp:19-27, src/fastlgamma.cpp).  The cell and block  */
+/* terms of sparse_entropy and the Delta entries of a move are summed in      */
+/* 128-bit fixed point; every other sum keeps the reference's order.         */
+/* ------------------------------------------------------------------------- */
+double orc_lgamma(int64_t x) {
+    if (x <= 0) return INFINITY;
+    int sign;
+    return lgamma_r((double)x, &sign);
+}
+/* fastlbinom (include/entropy.hpp:145-149) */
+double orc_lbinom(int64_t n, int64_t k) {
+    if (n == 0 || k == 0 || k > n) return 0;
+    return ((orc_lgamma(n + 1) - orc_lgamma(k + 1)) - orc_lgamma(n - k + 1));
+}
+/* eterm_exact (include/entropy.hpp:103-112), directed */
+static double np_eterm(int64_t w) { return -orc_lgamma(w + 1); }
+/* vterm_exact (include/entropy.hpp:114-135), directed, not degree-corrected */
+static double np_vterm(int64_t d_out, int64_t d_in, int64_t w) {
+    if (w == 0) return 0.0;
+    return (double)(d_out + d_in) * fastlog(w);
+}
+/* get_edges_dl (src/entropy.cpp:733-738) */
+double orc_np_edges_dl(int64_t B, int64_t E) { return orc_lbinom(B * B + E - 1, E); }
+static int64_t np_nonempty(const orc_bm *bm) {
+    int64_t n = 0;
+    for (int32_t b = 0; b < bm->B; ++b) n += bm->size[b] > 0;
+    return n;
+}
+/* get_partition_dl (src/entropy.cpp:657-669) */
+double orc_np_partition_dl(const orc_bm *bm) {
+    int64_t N = bm->N;
+    double S = 0;
+    S += orc_lbinom(N - 1, np_nonempty(bm) - 1);
+    S += orc_lgamma(N + 1);
+    for (int32_t b = 0; b < bm->B; ++b) S -= orc_lgamma(bm->size[b] + 1);
+    S += fastlog(N);
+    return S;
+}
+/* sparse_entropy (src/entropy.cpp:623-655) */
+double orc_np_sparse_entropy(const orc_bm *bm) {
+    fx128 acc = 0;
+    for (int32_t r = 0; r < bm->B; ++r)
+        for (int32_t i = 0; i < bm->row[r].len; ++i) acc += fx_from(np_eterm(bm->row[r].val[i]));
+    for (int32_t b = 0; b < bm->B; ++b) acc += fx_from(np_vterm(bm->d_out[b], bm->d_in[b], bm->size[b]));
+    return fx_to(acc);
+}
+/* nonparametric::mdl (src/entropy.cpp:740-767) */
+double orc_np_mdl(const orc_bm *bm, const orc_graph *g) {
+    double S = orc_np_sparse_entropy(bm), S_dl = 0;
+    S_dl += orc_np_partition_dl(bm);
+    S_dl += 0.0;  /* get_deg_dl_dist: 0 without degree correction */
+    S_dl += orc_np_edges_dl(np_nonempty(bm), g->E);
+    return S + S_dl * 1.0;  /* BETA_DL = 1 (include/entropy.hpp:14) */
 }
 
 /* entropy::null_mdl_v1 (src/entropy.cpp:571-585) */
@@ -672,13 +737,15 @@ static int accept_move(double dS, double H, int greedy, const uint32_t w[4]) {
     return u <= a;
 }
 
+static double np_vertex_dS(const orc_bm *bm, const delta_t *d, const proposal_t *p);
+
 double orc_vertex_delta_mdl(const orc_bm *bm, const orc_graph *g, int32_t v, int32_t s) {
     uint32_t w[4] = { 0, 0, 0, 0 };
     proposal_t p = propose_vertex(bm, g, v, w);
     p.s = s;
     delta_t d;
     vertex_delta(bm, g, v, bm->assign[v], s, &d);
-    double dS = delta_mdl_factored(bm, &d, &p);
+    double dS = bm->np ? np_vertex_dS(bm, &d, &p) : delta_mdl_factored(bm, &d, &p);
     delta_free(&d);
     return dS;
 }
@@ -709,6 +776,77 @@ double orc_vertex_hastings(const orc_bm *bm, const orc_graph *g, int32_t v, int3
 }
 
 /* propose_gibbs_move_v3 / propose_move + eval (src/finetune.cpp:447-462,842-901), against a fixed bm. */
+/* entries_dS (src/entropy.cpp:771-798): eterm(M + change) - eterm(M) over the Delta entries. */
+static void np_entries(const orc_bm *bm, const delta_t *d, fx128 *acc) {
+    const imap *maps[4] = { &d->cur_row, &d->prop_row, &d->cur_col, &d->prop_col };
+    for (int which = 0; which < 4; ++which) {
+        const imap *mp = maps[which];
+        for (int32_t i = 0; i < mp->cap; ++i) {
+            if (!mp->used[i]) continue;
+            int32_t row, col;
+            if (which == 0) { row = d->r; col = mp->key[i]; }
+            else if (which == 1) { row = d->s; col = mp->key[i]; }
+            else if (which == 2) { row = mp->key[i]; col = d->r; }
+            else { row = mp->key[i]; col = d->s; }
+            int64_t value = bm_get(bm, row, col);
+            *acc += fx_from(np_eterm(value + mp->val[i]) - np_eterm(value));
+        }
+    }
+}
+/* get_delta_partition_dl + get_delta_deg_dl (0) + get_delta_edges_dl (src/entropy.cpp:865-1132) for
+ * moving `weight` vertices from r (size_r) to s (size_s). */
+static double np_delta_dl(int64_t N, int64_t E, int64_t nonempty, int64_t size_r, int64_t size_s, int64_t weight) {
+    double S_b = 0, S_a = 0;
+    S_b += -orc_lgamma(size_r + 1);
+    S_a += -orc_lgamma(size_r - weight + 1);
+    S_b += -orc_lgamma(size_s + 1);
+    S_a += -orc_lgamma(size_s + weight + 1);
+    int dB = 0;
+    if (size_r == weight) dB--;
+    if (size_s == 0) dB++;
+    if (dB != 0) {
+        S_b += orc_lbinom(N - 1, nonempty - 1);
+        S_a += orc_lbinom(N - 1, nonempty + dB - 1);
+    }
+    double dpart = 0;
+    dpart += S_a - S_b;
+    double eb = 0, ea = 0;
+    if (dB != 0) {
+        eb += orc_np_edges_dl(nonempty, E);
+        ea += orc_np_edges_dl(nonempty + dB, E);
+    }
+    double dedges = ea - eb;
+    double dS_dl = 0;
+    dS_dl += dpart;
+    dS_dl += 0.0;
+    dS_dl += dedges;
+    return dS_dl;
+}
+/* virtual_move_sparse (src/entropy.cpp:802-862) + the description-length deltas: nonparametric::delta_mdl
+ * (src/entropy.cpp:1135-1164) with weight 1, block_merge_delta_mdl (:1183-1206) with weight = size(r). */
+static double np_move_dS(const orc_bm *bm, const delta_t *d, int64_t k_out, int64_t k_in, int64_t weight) {
+    const int32_t r = d->r, s = d->s;
+    if (r == s) return 0.0;
+    int64_t E = 0;  /* the graph's edge count: every edge is one unit of some block's out-degree */
+    for (int32_t b = 0; b < bm->B; ++b) E += bm->d_out[b];
+    fx128 acc = 0;
+    np_entries(bm, d, &acc);
+    double dS = fx_to(acc);
+    dS += np_vterm(bm->d_out[r] - k_out, bm->d_in[r] - k_in, bm->size[r] - weight);
+    dS -= np_vterm(bm->d_out[r], bm->d_in[r], bm->size[r]);
+    dS += np_vterm(bm->d_out[s] + k_out, bm->d_in[s] + k_in, bm->size[s] + weight);
+    dS -= np_vterm(bm->d_out[s], bm->d_in[s], bm->size[s]);
+    double dS_dl = np_delta_dl(bm->N, E, np_nonempty(bm), bm->size[r], bm->size[s], weight);
+    return dS + 1.0 * dS_dl;
+}
+static double np_vertex_dS(const orc_bm *bm, const delta_t *d, const proposal_t *p) {
+    return np_move_dS(bm, d, p->k_out, p->k_in, 1);
+}
+static double np_merge_dS(const orc_bm *bm, const delta_t *d) {
+    if (d->r == d->s || bm->size[d->r] == 0 || bm->size[d->s] == 0) return 0.0;
+    return np_move_dS(bm, d, bm->d_out[d->r], bm->d_in[d->r], bm->size[d->r]);
+}
+
 static int evaluate_vertex(const orc_bm *bm, const orc_graph *g, uint64_t seed, uint32_t epoch, int32_t v,
                            int greedy, int32_t *s_out, double *dS_out, delta_t *keep, proposal_t *p_out) {
     uint32_t w[4];
@@ -721,8 +859,9 @@ static int evaluate_vertex(const orc_bm *bm, const orc_graph *g, uint64_t seed, 
     if (p.s == r) return 0;
     delta_t d;
     vertex_delta(bm, g, v, r, p.s, &d);
-    double H = greedy ? 1.0 : hastings_from(bm, g, v, &d, &p);
-    double dS = delta_mdl_factored(bm, &d, &p);
+    /* hastings_correction is 1 for greedy or nonparametric runs (src/entropy.cpp:499) */
+    double H = (greedy || bm->np) ? 1.0 : hastings_from(bm, g, v, &d, &p);
+    double dS = bm->np ? np_vertex_dS(bm, &d, &p) : delta_mdl_factored(bm, &d, &p);
     *dS_out = dS;
     int acc = accept_move(dS, H, greedy, w);
     if (keep && acc) *keep = d; else delta_free(&d);
@@ -844,6 +983,7 @@ static void gibbs_batch(orc_bm *bm, const orc_graph *g, const orc_params *p, uin
 /* One sweep of finetune::asynchronous_gibbs (src/finetune.cpp:50-86). */
 int orc_async_gibbs_sweep(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t epoch, int32_t *proposed_s,
                           orc_stats *st) {
+    bm->np = p->nonparametric;
     int32_t N = g->N;
     int32_t *order = NULL;
     if (!p->ordered) {
@@ -868,6 +1008,7 @@ int orc_async_gibbs_sweep(orc_bm *bm, const orc_graph *g, const orc_params *p, u
 /* One sweep of sequential Metropolis-Hastings (src/finetune.cpp:793-803,822-869). */
 int orc_mh_sweep(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t epoch, int32_t ordered,
                  double *sum_dS, orc_stats *st) {
+    bm->np = p->nonparametric;
     int32_t N = g->N;
     int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
     for (int32_t i = 0; i < N; ++i) order[i] = i;
@@ -944,6 +1085,7 @@ static void split_by_degree(const orc_graph *g, double mh_percent, int32_t *hdv,
 /* finetune::mcmc dispatch (src/finetune.cpp:754-771) and the sweeps' outer loops. */
 int orc_mcmc(orc_bm *bm, const orc_graph *g, const orc_params *p, int32_t golden_ratio_not_reached, uint32_t *epoch,
              orc_stats *st) {
+    bm->np = p->nonparametric;
     int32_t N = g->N;
     if (bm->B == 1) return 0;
     int algo = p->algorithm;
@@ -1025,6 +1167,7 @@ int orc_mcmc(orc_bm *bm, const orc_graph *g, const orc_params *p, int32_t golden
 
 /* finetune::finetune_assignment (src/finetune.cpp:903-932): ordered sequential MH over the full graph. */
 int orc_finetune_assignment(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t *epoch, orc_stats *st) {
+    bm->np = p->nonparametric;
     int max_it = p->max_iterations > 0 ? p->max_iterations : 100;
     double *de = (double *)malloc(sizeof(double) * (size_t)max_it);
     int64_t nde = 0;
@@ -1129,7 +1272,7 @@ static double merge_dS_factored(const orc_bm *bm, const delta_t *d, int64_t k_ou
 double orc_merge_delta_mdl(const orc_bm *bm, int32_t r, int32_t s) {
     delta_t d;
     merge_delta(bm, r, s, &d);
-    double dS = merge_dS_factored(bm, &d, bm->d_out[r], bm->d_in[r]);
+    double dS = bm->np ? np_merge_dS(bm, &d) : merge_dS_factored(bm, &d, bm->d_out[r], bm->d_in[r]);
     delta_free(&d);
     return dS;
 }
@@ -1267,6 +1410,7 @@ static orc_bm *finish_merge(orc_bm *bm, const orc_graph *g, const int32_t *block
     int32_t newB = B - (int32_t)merged;
     if (next > newB) newB = next;
     orc_bm *out = orc_bm_new(g, newB, assign);
+    out->np = bm->np;
     out->overall_entropy = bm->overall_entropy;
     free(used);
     free(assign);
@@ -1321,7 +1465,7 @@ orc_bm *orc_merge_apply(orc_bm *bm, const orc_graph *g, const int32_t *best_s, c
             int64_t k_out = bm->d_out[from], k_in = bm->d_in[from];
             delta_t d;
             merge_delta(bm, from, to, &d);
-            double actual = merge_dS_factored(bm, &d, k_out, k_in);
+            double actual = bm->np ? np_merge_dS(bm, &d) : merge_dS_factored(bm, &d, k_out, k_in);
             if (n > 0 && actual > heap[0].dS) {
                 m.dS = actual;
                 heap_push(heap, &n, m);
@@ -1370,6 +1514,7 @@ orc_bm *orc_merge_apply(orc_bm *bm, const orc_graph *g, const int32_t *best_s, c
 
 /* block_merge::merge_blocks (src/block_merge.cpp:178-228). Consumes bm. */
 orc_bm *orc_merge_blocks(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t epoch, orc_stats *st) {
+    bm->np = p->nonparametric;
     int32_t B = bm->B;
     int32_t *best_s = (int32_t *)malloc(sizeof(int32_t) * (size_t)B);
     double *best_dS = (double *)malloc(sizeof(double) * (size_t)B);

@@ -36,8 +36,23 @@ int main(int argc, char **argv) {
         if (args.filepath.empty()) throw std::invalid_argument("-f/--filepath is required");
         Graph graph = Graph::load(args);
         fprintf(stderr, "V: %ld E: %ld\n", graph.num_vertices(), graph.num_edges());
+        if (args.samplesize <= 0.0) {  // src/main.cpp:93-97
+            fprintf(stderr, "ERROR Sample size of %g is too low. Must be greater than 0.0\n", args.samplesize);
+            return -5;
+        }
         auto t0 = std::chrono::steady_clock::now();
-        Blockmodel bm = sbp::stochastic_block_partition(graph, args);
+        Blockmodel bm;
+        if (args.samplesize < 1.0) {  // src/main.cpp:98-146: partition a sample, extend, finetune the full graph
+            sbp_amd::args() = args;
+            sample::Sample s = sample::sample(graph);
+            fprintf(stderr, "sample V: %ld E: %ld\n", s.graph.num_vertices(), s.graph.num_edges());
+            Blockmodel sample_bm = sbp::stochastic_block_partition(s.graph, args);
+            std::vector<long> assignment = sample::extend(graph, sample_bm, s);
+            bm = Blockmodel(sample_bm.num_blocks(), graph, 0.5, assignment);
+            bm = finetune::finetune_assignment(bm, graph);
+        } else {
+            bm = sbp::stochastic_block_partition(graph, args);
+        }
         double runtime = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         const sbp_stats &st = sbp::last_stats();
         std::vector<long> a = bm.block_assignment();

@@ -204,6 +204,19 @@ sbp_graph Graph::device(sbp_ctx ctx) const {
     return dev_.get();
 }
 
+Graph Graph::adopt(sbp_ctx ctx, sbp_graph g, std::shared_ptr<struct sbp_graph_s> keep_alive) {
+    int32_t N = 0;
+    int64_t E = 0;
+    throw_on(sbp_graph_info(g, &N, &E), ctx, "sbp_graph_info");
+    std::vector<int64_t> oo(N + 1), io(N + 1);
+    std::vector<int32_t> oa(E), ia(E);
+    throw_on(sbp_graph_download(ctx, g, oo.data(), oa.data(), io.data(), ia.data()), ctx, "sbp_graph_download");
+    Graph out(N, std::move(oo), std::move(oa), std::move(io), std::move(ia));
+    out.dev_ = std::shared_ptr<sbp_graph_s>(g, [](sbp_graph x) { sbp_graph_free(x); });
+    out.parent_ = std::move(keep_alive);
+    return out;
+}
+
 // ------------------------------------------------------------------ Blockmodel
 
 static std::shared_ptr<sbp_bm_s> wrap(sbp_bm bm) {
@@ -449,6 +462,48 @@ Blockmodel &finetune_assignment(Blockmodel &bm, Graph &graph) {
 }
 
 }  // namespace finetune
+
+namespace sample {
+
+Sample sample(const Graph &graph) {
+    const Args &a = sbp_amd::args();
+    static const char *names[] = {"max_degree", "random", "degree_product", "degree_weighted"};
+    int32_t alg = -1;
+    for (int32_t i = 0; i < 4; ++i)
+        if (a.samplingalg == names[i]) alg = i;
+    if (alg < 0)
+        throw std::invalid_argument(a.samplingalg + " is not a valid sampling algorithm on the device path "
+                                    "(max_degree, random, degree_product, degree_weighted)");
+    sbp_ctx ctx = context(a);
+    sbp_graph parent = graph.device(ctx);
+    sbp_graph s = nullptr;
+    int32_t n = 0;
+    std::vector<int32_t> mapping(graph.num_vertices());
+    throw_on(sbp_sample(ctx, parent, alg, a.samplesize, a.seed, &s, &n, mapping.data()), ctx, "sbp_sample");
+    Sample out;
+    out.graph = Graph::adopt(ctx, s);
+    out.mapping.assign(mapping.begin(), mapping.end());
+    std::vector<long> truth(n, -1);  // from_vertices carries the true assignment over (sample.cpp:214)
+    const std::vector<long> &full_truth = graph.assignment();
+    if (!full_truth.empty())
+        for (long v = 0; v < graph.num_vertices(); ++v)
+            if (out.mapping[v] >= 0) truth[out.mapping[v]] = full_truth[v];
+    out.graph.assignment(std::move(truth));
+    return out;
+}
+
+std::vector<long> extend(const Graph &graph, const Blockmodel &sample_blockmodel, const Sample &sample) {
+    sbp_ctx ctx = sample_blockmodel.ctx();
+    std::vector<int32_t> a(graph.num_vertices());
+    sbp_bm full = nullptr;
+    throw_on(sbp_extend(ctx, graph.device(ctx), sample.graph.device(ctx), sample_blockmodel.handle(),
+                        sbp_amd::args().seed, &full, a.data()),
+             ctx, "sbp_extend");
+    sbp_bm_free(full);
+    return std::vector<long>(a.begin(), a.end());
+}
+
+}  // namespace sample
 
 namespace sbp {
 

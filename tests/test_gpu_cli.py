@@ -1,0 +1,56 @@
+"""The command-line driver (the reference's src/main.cpp flow over the C++ drop-in API) on the device,
+against the oracle on the same TSV graph: a plain run and a sampled run (--samplesize), bit-exact."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from sbp_amd.graph import dcsbm, load_tsv, save_tsv
+
+pytestmark = pytest.mark.gpu
+CLI = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sbp_amd", "sbp_cli")
+
+
+def run_cli(base, out, *flags):
+    cmd = [CLI, "-f", base, "--output_file", out, "--seed", "0", *flags]
+    print("[cli]", " ".join(cmd[1:]), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stderr
+    return json.load(open(out))
+
+
+@pytest.fixture(scope="module")
+def tsv(tmp_path_factory):
+    base = str(tmp_path_factory.mktemp("graph") / "dcsbm")
+    save_tsv(dcsbm(1500, 12, gamma=2.5, kmin=1, kmax=100, mean_degree=10, p_in=0.75, seed=5), base)
+    return base
+
+
+def test_cli_partition_matches_oracle(tsv, tmp_path):
+    res = run_cli(tsv, str(tmp_path / "r.json"), "-a", "async_gibbs")
+    g = load_tsv(tsv)
+    a, nb, st = O.sbp(O.OGraph.from_csr(g), O.params("async_gibbs", seed=0, threads=8))
+    assert res["num_blocks"] == nb
+    assert res["assignment"] == a.tolist()
+    assert res["mdl"] == st.mdl
+    assert res["nmi"] > 0.9
+
+
+def test_cli_sampled_run_matches_oracle(tsv, tmp_path):
+    res = run_cli(tsv, str(tmp_path / "s.json"), "-a", "hybrid_mcmc", "--samplesize", "0.5", "--samplingalg",
+                  "max_degree")
+    g = load_tsv(tsv)
+    og = O.OGraph.from_csr(g)
+    m, vs = O.sample(og, "max_degree", 0.5)
+    sg = O.subgraph(og, m, vs)
+    sa, snb, _ = O.sbp(sg, O.params("hybrid_mcmc", seed=0, threads=8))
+    a = O.extend(og, m, sa, snb, 0)
+    bm = O.OBlockmodel.build(og, snb, a)
+    # the full-graph finetune_assignment starts its own epoch count at 0 (a different graph's vertices)
+    bm.finetune_assignment(O.params("hybrid_mcmc", seed=0), 0)
+    assert res["num_blocks"] == snb
+    assert res["assignment"] == bm.export()["assignment"].tolist()
+    assert res["mdl"] == bm.mdl()
