@@ -48,6 +48,7 @@ typedef struct {
     uint64_t seed;            /* replay seed (the reference seeds from random_device; src/rng.cpp:34-46) */
     int32_t max_iterations;   /* MAX_NUM_ITERATIONS (include/finetune.hpp:34), default 100 */
     int32_t threads;          /* --threads: accepted for flag parity; the device ignores it */
+    int32_t nonparametric;    /* --nonparametric (args.hpp:126-127): entropy::nonparametric description length */
 } sbp_params;
 
 /* Counters and phase timers (the reference's timers::*, include/globals.hpp:19-67). */
@@ -141,9 +142,19 @@ int sbp_bm_validate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *ok);
 
 /* ----------------------------------------------------------------- entropy */
 
-/* Replaces entropy::mdl (src/entropy.cpp:597-604) and Blockmodel::log_posterior_probability
+/* Replaces entropy::mdl (src/entropy.cpp:597-604; nonparametric::mdl for a nonparametric blockmodel, whose
+ * log_posterior is then NaN) and Blockmodel::log_posterior_probability
  * (src/blockmodel/blockmodel.cpp:288-306). */
 int sbp_mdl(sbp_ctx ctx, sbp_bm bm, sbp_graph g, double *mdl, double *log_posterior);
+
+/* Sets the blockmodel's entropy model, the reference's global args.nonparametric: 0 = entropy::mdl and its
+ * deltas, 1 = entropy::nonparametric (src/entropy.cpp:617-1225; directed, not degree-corrected).  The
+ * calls that take sbp_params set it from p->nonparametric; blockmodels made by merges inherit it. */
+int sbp_bm_set_model(sbp_bm bm, int32_t nonparametric);
+/* Replaces entropy::nonparametric::mdl with its parts sparse_entropy, get_partition_dl and get_edges_dl
+ * (src/entropy.cpp:623-767); any output may be NULL. */
+int sbp_np_description_length(sbp_ctx ctx, sbp_bm bm, sbp_graph g, double *mdl, double *sparse_entropy,
+                              double *partition_dl, double *edges_dl);
 
 /* ----------------------------------------------------------------- finetune */
 

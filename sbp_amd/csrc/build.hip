@@ -151,6 +151,7 @@ void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm) {
     hipStream_t st = ctx->stream;
     const int32_t B = bm->B;
     const int64_t E = g->E;
+    bm->E = E;
     const int cbits = bits_for(B);
     const int end_bit = 2 * cbits;
     SBP_CHECK(end_bit <= 64, SBP_E_ARG, "block count too large for packed keys");
@@ -226,6 +227,8 @@ void bm_copy_into(sbp_ctx ctx, sbp_bm src, sbp_bm dst) {
     }
     dst->overall_entropy = src->overall_entropy;
     dst->num_to_merge = 0;
+    dst->np = src->np;
+    dst->E = src->E;
 }
 
 }  // namespace sbpimpl

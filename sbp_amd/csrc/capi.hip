@@ -57,11 +57,18 @@ static sbp_bm new_bm(sbp_ctx ctx) {
     return bm;
 }
 
+// entropy::mdl (src/entropy.cpp:597-604): nonparametric::mdl for nonparametric blockmodels
 static double mdl_of(sbp_ctx ctx, sbp_bm bm, sbp_graph g, sbp_stats *st) {
     double t0 = now_ms();
-    double lp = log_posterior(ctx, bm);
+    double m;
+    if (bm->np) {
+        m = np_mdl(ctx, g, bm, nullptr);
+    } else {
+        double lp = log_posterior(ctx, bm);
+        m = model_term(g->E, g->N, bm->B) - lp;
+    }
     if (st) st->mdl_ms += now_ms() - t0;
-    return model_term(g->E, g->N, bm->B) - lp;
+    return m;
 }
 
 // ------------------------------------------------------------------ finetune
@@ -275,6 +282,7 @@ static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, 
         const int32_t *rcp = reinterpret_cast<const int32_t *>(rc.data());
         const int32_t *ccp = reinterpret_cast<const int32_t *>(cc.data());
         model.load(B, rp.data(), rcp, rcp + 1, cp.data(), ccp, ccp + 1, 2, dout.data(), din.data(), pool.get());
+        if (bm->np) model.set_nonparametric(sizes.data(), bm->N);
     }
     const double t_loaded = now_ms();
     sbpcarry::CarryStats cs;
@@ -298,6 +306,7 @@ static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, 
     SBP_HIP(hipMemcpyAsync(d_map, block_map.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice, s));
     sbp_bm out = new_bm(ctx);
     bm_alloc(out, new_B, bm->N);
+    out->np = bm->np;
     relabel_assignment(ctx, bm, d_map, new_B, out->assign);
     bm_rebuild(ctx, g, out);
     SBP_HIP(hipStreamSynchronize(s));
@@ -367,6 +376,7 @@ int sbp_destroy(sbp_ctx ctx) {
         s.ptr = nullptr;
         s.bytes = 0;
     }
+    if (ctx->lgamma) (void)hipFree(ctx->lgamma);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SBP_OK;
@@ -422,7 +432,7 @@ int sbp_graph_upload(sbp_ctx ctx, int32_t N, int64_t E, const int64_t *out_off, 
 
 int sbp_graph_free(sbp_graph g) {
     if (!g) return SBP_OK;
-    void *p[] = {g->out_off, g->in_off,   g->out_adj,    g->in_adj,         g->edge_src,
+    void *p[] = {g->out_off, g->in_off,    g->out_adj,    g->in_adj,         g->edge_src,
                  g->degree,  g->deg_order, g->parent_map, g->sample_vertices};
     for (void *x : p)
         if (x) (void)hipFree(x);
@@ -576,9 +586,34 @@ int sbp_bm_validate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *ok) {
 int sbp_mdl(sbp_ctx ctx, sbp_bm bm, sbp_graph g, double *mdl, double *lp) {
     if (!ctx || !bm || !g) return SBP_E_ARG;
     SBP_API_BEGIN
-    double l = log_posterior(ctx, bm);
-    if (lp) *lp = l;
-    if (mdl) *mdl = model_term(g->E, g->N, bm->B) - l;
+    if (bm->np) {
+        const double m = np_mdl(ctx, g, bm, nullptr);
+        if (lp) *lp = NAN;
+        if (mdl) *mdl = m;
+    } else {
+        double l = log_posterior(ctx, bm);
+        if (lp) *lp = l;
+        if (mdl) *mdl = model_term(g->E, g->N, bm->B) - l;
+    }
+    SBP_API_END(ctx)
+}
+
+int sbp_bm_set_model(sbp_bm bm, int32_t nonparametric) {
+    if (!bm) return SBP_E_ARG;
+    bm->np = nonparametric ? 1 : 0;
+    return SBP_OK;
+}
+
+int sbp_np_description_length(sbp_ctx ctx, sbp_bm bm, sbp_graph g, double *mdl, double *sparse_entropy,
+                              double *partition_dl, double *edges_dl) {
+    if (!ctx || !bm || !g) return SBP_E_ARG;
+    SBP_API_BEGIN
+    double parts[3];
+    const double m = np_mdl(ctx, g, bm, parts);
+    if (mdl) *mdl = m;
+    if (sparse_entropy) *sparse_entropy = parts[0];
+    if (partition_dl) *partition_dl = parts[1];
+    if (edges_dl) *edges_dl = parts[2];
     SBP_API_END(ctx)
 }
 
@@ -607,6 +642,7 @@ int sbp_async_gibbs_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params 
                           sbp_stats *stats) {
     if (!ctx || !bm || !g || !p || p->batches < 1) return SBP_E_ARG;
     SBP_API_BEGIN
+    bm->np = p->nonparametric ? 1 : 0;
     async_gibbs_sweep(ctx, bm, g, *p, epoch, stats);
     SBP_API_END(ctx)
 }
@@ -615,6 +651,7 @@ int sbp_mh_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, uint3
                  double *sum_dS, sbp_stats *stats) {
     if (!ctx || !bm || !g || !p) return SBP_E_ARG;
     SBP_API_BEGIN
+    bm->np = p->nonparametric ? 1 : 0;
     int32_t *ident = iota_of(ctx, g);
     const int32_t *ord = ident;
     if (!ordered) {
@@ -633,6 +670,7 @@ int sbp_mcmc(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, int32_t g
              uint32_t *epoch, double *mdl_out, sbp_stats *stats) {
     if (!ctx || !bm || !g || !p || !epoch || p->batches < 1) return SBP_E_ARG;
     SBP_API_BEGIN
+    bm->np = p->nonparametric ? 1 : 0;
     mcmc(ctx, bm, g, *p, golden_ratio_not_reached != 0, epoch, stats);
     if (mdl_out) *mdl_out = bm->overall_entropy;
     SBP_API_END(ctx)
@@ -642,6 +680,7 @@ int sbp_finetune_assignment(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_param
                             double *mdl_out, sbp_stats *stats) {
     if (!ctx || !bm || !g || !p || !epoch) return SBP_E_ARG;
     SBP_API_BEGIN
+    bm->np = p->nonparametric ? 1 : 0;
     finetune_assignment(ctx, bm, g, *p, epoch, stats);
     if (mdl_out) *mdl_out = bm->overall_entropy;
     SBP_API_END(ctx)

@@ -183,6 +183,76 @@ inline double cell(int64_t value, int64_t din, int64_t dout) {
     return (double)value * (fastlog(value) - (fastlog(din) + fastlog(dout)));
 }
 
+// fastlgamma (include/fastlgamma.hpp:19-27): glibc lgamma(double(x)), +inf at 0; tabulated for small x with
+// the same function
+struct LgammaTable {
+    static constexpr int64_t SIZE = 1 << 20;
+    std::vector<double> v;
+    LgammaTable() : v(SIZE) {
+        for (int64_t i = 0; i < SIZE; ++i) v[i] = raw(i);
+    }
+    static double raw(int64_t x) {
+        if (x <= 0) return INFINITY;
+        int sign;
+        return lgamma_r((double)x, &sign);
+    }
+};
+inline const LgammaTable &lgamma_table() {
+    static const LgammaTable t;
+    return t;
+}
+inline double flgamma(int64_t x) { return x < LgammaTable::SIZE ? lgamma_table().v[x] : LgammaTable::raw(x); }
+inline double flbinom(int64_t n, int64_t k) {
+    if (n == 0 || k == 0 || k > n) return 0;
+    return ((flgamma(n + 1) - flgamma(k + 1)) - flgamma(n - k + 1));
+}
+inline double np_eterm(int64_t w) { return -flgamma(w + 1); }
+inline double np_entry(int64_t M, int64_t d) { return np_eterm(M + d) - np_eterm(M); }
+inline double np_vterm(int64_t d_out, int64_t d_in, int64_t w) {
+    if (w == 0) return 0.0;
+    return (double)(d_out + d_in) * fastlog(w);
+}
+inline double np_edges_dl(int64_t B, int64_t E) { return flbinom(B * B + E - 1, E); }
+
+// The block terms and description-length deltas of nonparametric::block_merge_delta_mdl
+// (src/entropy.cpp:1183-1206; virtual_move_sparse :836-859, get_delta_partition_dl :865-967,
+// get_delta_edges_dl :1098-1132) around the Delta entries' sum, for `nonempty` non-empty blocks -- the same
+// operations in the same order as the device kernels.
+inline double np_merge_finish(double entries, int64_t dout_r, int64_t din_r, int64_t size_r, int64_t dout_s,
+                              int64_t din_s, int64_t size_s, int64_t N, int64_t E, int64_t nonempty) {
+    const int64_t k_out = dout_r, k_in = din_r, weight = size_r;
+    double dS = entries;
+    dS += np_vterm(dout_r - k_out, din_r - k_in, size_r - weight);
+    dS -= np_vterm(dout_r, din_r, size_r);
+    dS += np_vterm(dout_s + k_out, din_s + k_in, size_s + weight);
+    dS -= np_vterm(dout_s, din_s, size_s);
+    double S_b = 0, S_a = 0;
+    S_b += -flgamma(size_r + 1);
+    S_a += -flgamma(size_r - weight + 1);
+    S_b += -flgamma(size_s + 1);
+    S_a += -flgamma(size_s + weight + 1);
+    int dB = 0;
+    if (size_r == weight) dB--;
+    if (size_s == 0) dB++;
+    if (dB != 0) {
+        S_b += flbinom(N - 1, nonempty - 1);
+        S_a += flbinom(N - 1, nonempty + dB - 1);
+    }
+    double dpart = 0;
+    dpart += S_a - S_b;
+    double eb = 0, ea = 0;
+    if (dB != 0) {
+        eb += np_edges_dl(nonempty, E);
+        ea += np_edges_dl(nonempty + dB, E);
+    }
+    const double dedges = ea - eb;
+    double dS_dl = 0;
+    dS_dl += dpart;
+    dS_dl += 0.0;
+    dS_dl += dedges;
+    return dS + 1.0 * dS_dl;
+}
+
 // Fork-join helper for the speculative evaluations: fn(i) for i in [0, n) on up to `threads` threads.
 class Pool {
   public:
@@ -262,6 +332,21 @@ struct Model {
     int32_t B = 0;
     std::vector<FlatMap> row, col;
     std::vector<int64_t> d_out, d_in;
+    // nonparametric entropy model: block sizes, vertices, edges and the non-empty block count
+    bool np = false;
+    std::vector<int64_t> size;
+    int64_t N = 0, E = 0, nonempty = 0;
+    void set_nonparametric(const int32_t *sizes, int64_t num_vertices) {
+        np = true;
+        size.assign(sizes, sizes + B);
+        N = num_vertices;
+        E = 0;
+        nonempty = 0;
+        for (int32_t b = 0; b < B; ++b) {
+            E += d_out[b];
+            nonempty += size[b] > 0;
+        }
+    }
 
     // row-major cells (col, count) per row and column-major cells (row, count) per column, as separate
     // index/count arrays (stride 1) or interleaved (index, count) pairs (stride 2)
@@ -333,6 +418,29 @@ inline double merge_dS(const Model &m, int32_t r, int32_t s) {
     return acc.value();
 }
 
+// The Delta entries' eterm differences of nonparametric::block_merge_delta_mdl: row r and column r empty,
+// (s, c) and (x, s) receive them, (s, s) receives (r, r) + (r, s) + (s, r); fixed-point sum.
+inline double np_merge_entries(const Model &m, int32_t r, int32_t s) {
+    Fx128 acc;
+    m.row[r].each([&](int32_t c, int64_t v) {
+        acc.add(np_entry(v, -v));
+        if (c != r && c != s) acc.add(np_entry(m.row[s].get(c), v));
+    });
+    m.col[r].each([&](int32_t x, int64_t v) {
+        if (x == r) return;
+        acc.add(np_entry(v, -v));
+        if (x != s) acc.add(np_entry(m.col[s].get(x), v));
+    });
+    const int64_t dss = m.get(r, r) + m.get(r, s) + m.get(s, r);
+    if (dss > 0) acc.add(np_entry(m.get(s, s), dss));
+    return acc.value();
+}
+inline double np_merge_dS(const Model &m, int32_t r, int32_t s, double entries) {
+    if (m.size[r] == 0 || m.size[s] == 0) return 0.0;
+    return np_merge_finish(entries, m.d_out[r], m.d_in[r], m.size[r], m.d_out[s], m.d_in[s], m.size[s], m.N, m.E,
+                           m.nonempty);
+}
+
 // Blockmodel::merge_block (src/blockmodel/blockmodel.cpp:337-363): row r and column r move into s.
 inline void merge_apply(Model &m, int32_t r, int32_t s) {
     std::vector<std::pair<int32_t, int64_t>> rr, rc;
@@ -360,6 +468,11 @@ inline void merge_apply(Model &m, int32_t r, int32_t s) {
     m.d_in[s] += m.d_in[r];
     m.d_out[r] = 0;
     m.d_in[r] = 0;
+    if (m.np) {
+        m.size[s] += m.size[r];
+        m.size[r] = 0;
+        m.nonempty--;
+    }
 }
 
 struct CarryStats {
@@ -432,7 +545,12 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
         std::vector<int32_t> c_to(B, -1);
         std::vector<int64_t> c_time(B, -1), stamp(B, 0);
         std::vector<double> c_val(B, 0.0);
-        if (seed_from_proposals)
+        // nonparametric: the cache holds the Delta entries' sum; the rest depends on the non-empty block
+        // count, which every merge changes, and is recomputed at use
+        const bool np = m->np;
+        auto evaluate = [&](int32_t r, int32_t t) { return np ? np_merge_entries(*m, r, t) : merge_dS(*m, r, t); };
+        auto finalize = [&](int32_t r, int32_t t, double v) { return np ? np_merge_dS(*m, r, t, v) : v; };
+        if (seed_from_proposals && !np)
             for (int32_t b = 0; b < B; ++b)
                 if (best_s[b] >= 0 && best_s[b] != b) {
                     c_to[b] = best_s[b];
@@ -476,7 +594,7 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
                         const Item &it = batch[i];
                         const int32_t t = root_ro(it.s);
                         if (t == it.r || valid(it.r, t)) return;
-                        c_val[it.r] = merge_dS(*m, it.r, t);
+                        c_val[it.r] = evaluate(it.r, t);
                         c_to[it.r] = t;
                         c_time[it.r] = now;
                         spec_count.fetch_add(1, std::memory_order_relaxed);
@@ -499,14 +617,15 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
             if (from == to) continue;
             double actual;
             if (valid(from, to)) {
-                actual = c_val[from];
+                actual = finalize(from, to, c_val[from]);
                 st.reused++;
             } else {
                 if (c_to[from] == to && c_time[from] == spec_time) batch_lost++;  // speculated, then invalidated
-                actual = merge_dS(*m, from, to);
+                const double v = evaluate(from, to);
                 c_to[from] = to;
                 c_time[from] = now;
-                c_val[from] = actual;
+                c_val[from] = v;
+                actual = finalize(from, to, v);
                 st.evaluations++;
             }
             const bool has_b = bi < batch.size();

@@ -34,6 +34,7 @@
 #include "device_common.hpp"
 #include "device_table.hpp"
 #include "internal.hpp"
+#include "np_entropy.hpp"
 
 namespace sbpimpl {
 using namespace sbpdev;
@@ -77,6 +78,7 @@ struct FtArgs {
     int *err;              // set when a multinomial runs past its cells (inconsistent degrees)
     int32_t *big_queue;    // high-degree vertices deferred by the LDS-table launch
     int32_t *big_count;
+    NpConst np;            // nonparametric description-length constants (nonparametric runs only)
 };
 
 __device__ __forceinline__ long long get_cell(const FtArgs &a, int row, int col) {
@@ -384,7 +386,70 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
     return dS;
 }
 
-template <bool BIG, bool GREEDY>
+// Nonparametric delta-MDL of the move (entropy::nonparametric::delta_mdl, src/entropy.cpp:1135-1164): the
+// Delta entries' eterm differences (unchanged cells do not change eterm, so no row or column is scanned),
+// the block terms of r and s and the description-length deltas.  No Hastings correction
+// (src/entropy.cpp:499); accept as finetune::accept (src/finetune.cpp:22-32).
+template <bool GREEDY>
+__device__ double evaluate_move_np(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept) {
+    const int lane = lane_id();
+    const int v = m.v, r = m.r, s = m.s;
+    for (int64_t e = m.o0 + lane; e < m.o1; e += WAVE) {
+        int u = a.out_adj[e];
+        if (u != v) table_add(t, a.assign[u], 1, 0);
+    }
+    for (int64_t e = m.i0 + lane; e < m.i1; e += WAVE) {
+        int u = a.in_adj[e];
+        if (u != v) table_add(t, a.assign[u], 0, 1);
+    }
+    wave_sync();
+    table_get(t, r, m.out_r, m.in_r);
+    table_get(t, s, m.out_s, m.in_s);
+    const long long mrr = get_cell(a, r, r), mrs = get_cell(a, r, s), msr = get_cell(a, s, r),
+                    mss = get_cell(a, s, s);
+    Fx acc = fx_zero();
+    const int nb = *t.n_used;
+    for (int i = lane; i < nb; i += WAVE) {
+        const int slot = t.used[i];
+        const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
+        if (oc > 0) {
+            const long long mrc = c == r ? mrr : (c == s ? mrs : get_cell(a, r, c));
+            const long long msc = c == r ? msr : (c == s ? mss : get_cell(a, s, c));
+            fx_acc(acc, np_entry(a.np, mrc, delta_row_r(m, c, oc)));
+            fx_acc(acc, np_entry(a.np, msc, delta_row_s(m, c, oc)));
+        }
+        if (ic > 0 && c != r && c != s) {
+            fx_acc(acc, np_entry(a.np, get_cell(a, c, r), -(long long)ic));
+            fx_acc(acc, np_entry(a.np, get_cell(a, c, s), (long long)ic));
+        }
+    }
+    if (lane == 0) {  // Delta entries not produced by an out-neighbour block
+        if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0)) fx_acc(acc, np_entry(a.np, mrr, -(long long)m.selfc - m.in_r));
+        if (m.out_s == 0 && m.in_r > 0) fx_acc(acc, np_entry(a.np, mrs, (long long)m.in_r));
+        if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) fx_acc(acc, np_entry(a.np, mss, (long long)m.selfc + m.in_s));
+        if (m.out_r == 0 && m.in_s > 0) fx_acc(acc, np_entry(a.np, msr, -(long long)m.in_s));
+    }
+    acc = wave_sum_fx(acc);
+    const double dS = np_finish(a.np, fx_to(acc), a.d_out[r], a.d_in[r], a.sizes[r], a.d_out[s], a.d_in[s],
+                                a.sizes[s], m.k_out, m.k_in, 1);
+    if (GREEDY) {
+        accept = dS < 0.0;
+    } else {
+        const double u = unit_double(w.z, w.w);
+        double p = exp(-3.0 * dS);
+        p = p >= 1.0 ? 1.0 : p;
+        accept = u <= p;
+    }
+    return dS;
+}
+
+template <bool GREEDY, bool NP>
+__device__ __forceinline__ double evaluate(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept) {
+    if constexpr (NP) return evaluate_move_np<GREEDY>(a, m, t, w, accept);
+    else return evaluate_move<GREEDY>(a, m, t, w, accept);
+}
+
+template <bool BIG, bool GREEDY, bool NP>
 __global__ void __launch_bounds__(FT_BLOCK) k_ft_propose(FtArgs a) {
     __shared__ int lds[BIG ? 1 : FT_WAVES * table_ints(LDS_CAP_LOG2)];
     const int lane = lane_id(), wid = threadIdx.x / WAVE;
@@ -416,7 +481,7 @@ __global__ void __launch_bounds__(FT_BLOCK) k_ft_propose(FtArgs a) {
             continue;
         }
         bool accept;
-        const double dS = evaluate_move<GREEDY>(a, m, t, w, accept);
+        const double dS = evaluate<GREEDY, NP>(a, m, t, w, accept);
         if (lane == 0) {
             a.move_s[v] = accept ? s : -1;
             if (a.dS_out) a.dS_out[v] = dS;
@@ -425,6 +490,17 @@ __global__ void __launch_bounds__(FT_BLOCK) k_ft_propose(FtArgs a) {
         table_clear_used(t);
     }
     if (lane == 0 && bytes) atomicAdd(a.bytes, bytes);
+}
+
+template <bool BIG>
+static void launch_ft(int blocks, hipStream_t st, const FtArgs &a, bool greedy, bool np) {
+    if (np) {
+        if (greedy) k_ft_propose<BIG, true, true><<<blocks, FT_BLOCK, 0, st>>>(a);
+        else k_ft_propose<BIG, false, true><<<blocks, FT_BLOCK, 0, st>>>(a);
+    } else {
+        if (greedy) k_ft_propose<BIG, true, false><<<blocks, FT_BLOCK, 0, st>>>(a);
+        else k_ft_propose<BIG, false, false><<<blocks, FT_BLOCK, 0, st>>>(a);
+    }
 }
 
 static int log2_ceil(int64_t x) {
@@ -468,6 +544,8 @@ static FtArgs base_args(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     a.err = nullptr;
     a.big_queue = nullptr;
     a.big_count = nullptr;
+    a.np = NpConst{};
+    if (bm->np) a.np = np_const(ctx, bm);
     return a;
 }
 
@@ -491,8 +569,7 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     int64_t blocks = (count + FT_WAVES - 1) / FT_WAVES;
     if (blocks > 4096) blocks = 4096;
     SBP_HIP(hipEventRecord(ctx->ev[0], st));
-    if (greedy) k_ft_propose<false, true><<<(int)blocks, FT_BLOCK, 0, st>>>(a);
-    else k_ft_propose<false, false><<<(int)blocks, FT_BLOCK, 0, st>>>(a);
+    launch_ft<false>((int)blocks, st, a, greedy != 0, bm->np != 0);
     SBP_HIP(hipGetLastError());
     SBP_HIP(hipEventRecord(ctx->ev[6], st));
     info.high_launched = g->max_degree > SMALL_K;
@@ -508,8 +585,7 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
         const int64_t big_blocks = (waves + FT_WAVES - 1) / FT_WAVES;
         a.gtable = ctx->scratch[S_HASH].as<int>((size_t)big_blocks * FT_WAVES * tints);
         a.gcap_log2 = lg;
-        if (greedy) k_ft_propose<true, true><<<(int)big_blocks, FT_BLOCK, 0, st>>>(a);
-        else k_ft_propose<true, false><<<(int)big_blocks, FT_BLOCK, 0, st>>>(a);
+        launch_ft<true>((int)big_blocks, st, a, greedy != 0, bm->np != 0);
         SBP_HIP(hipGetLastError());
     }
     SBP_HIP(hipEventRecord(ctx->ev[1], st));
@@ -771,7 +847,7 @@ __device__ __forceinline__ bool lacks_room(const FtArgs &a, const MhState &h, in
     return h.rlen[row] >= h.rcap[row] || h.clen[col] >= h.ccap[col];
 }
 
-template <bool GREEDY>
+template <bool GREEDY, bool NP>
 __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start) {
     __shared__ int lds[table_ints(LDS_CAP_LOG2)];
     const int lane = lane_id();
@@ -796,7 +872,7 @@ __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start)
             continue;
         }
         bool accept;
-        const double dS = evaluate_move<GREEDY>(a, m, t, w, accept);
+        const double dS = evaluate<GREEDY, NP>(a, m, t, w, accept);
         bytes += proposal_bytes(a, m, *t.n_used, GREEDY);
         if (!accept) {
             table_clear_used(t);
@@ -804,6 +880,8 @@ __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start)
         }
         const int r = m.r;
         const int nb = *t.n_used;
+        // occupying an empty block changes the non-empty block count the nonparametric constants are for
+        const bool occupies = NP && a.sizes[s] == 0;
         // ---- room check for every Delta cell that would create a new entry
         bool short_room = false;
         int new_row_s = 0, new_col_s = 0;
@@ -870,6 +948,11 @@ __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start)
         fx_add(sum, fx_from(dS));
         moves++;
         bytes += 12ull * (unsigned long long)kall;
+        if (occupies) {  // the host refreshes the constants and resumes with the next vertex
+            status = 2;
+            stop = (int32_t)(item + 1);
+            break;
+        }
     }
     if (lane == 0) {
         *h.status = status;
@@ -973,8 +1056,13 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         a.m.row_cum = nullptr;
         a.m.col_cum = nullptr;
         SBP_HIP(hipEventRecord(ctx->ev[4], st));
-        if (p.greedy) k_mh<true><<<1, WAVE, 0, st>>>(a, h, start);
-        else k_mh<false><<<1, WAVE, 0, st>>>(a, h, start);
+        if (bm->np) {
+            if (p.greedy) k_mh<true, true><<<1, WAVE, 0, st>>>(a, h, start);
+            else k_mh<false, true><<<1, WAVE, 0, st>>>(a, h, start);
+        } else {
+            if (p.greedy) k_mh<true, false><<<1, WAVE, 0, st>>>(a, h, start);
+            else k_mh<false, false><<<1, WAVE, 0, st>>>(a, h, start);
+        }
         SBP_HIP(hipGetLastError());
         SBP_HIP(hipEventRecord(ctx->ev[5], st));
         // compact the slack layout back into the blockmodel's CSR (row and column counts changed)
@@ -1000,9 +1088,11 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         bm_compute_cum(ctx, bm);
         SBP_HIP(hipStreamSynchronize(st));
         if (hs[1] == 0) break;
-        start = hs[0];  // resume at the vertex that ran out of room, with more slack
-        grow *= 2;
-        SBP_CHECK(grow <= (1 << 24), SBP_E_STATE, "Metropolis-Hastings: slack growth did not converge");
+        start = hs[0];
+        if (hs[1] == 1) {  // resume at the vertex that ran out of room, with more slack
+            grow *= 2;
+            SBP_CHECK(grow <= (1 << 24), SBP_E_STATE, "Metropolis-Hastings: slack growth did not converge");
+        }  // else (2): resume after a move into an empty block, with the constants of the new block count
         SBP_HIP(hipMemsetAsync(book, 0, 2 * sizeof(int32_t), st));
     }
     struct {

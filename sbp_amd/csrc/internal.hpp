@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/sbp_amd.h"
+#include "np_entropy.hpp"
 
 namespace sbpimpl {
 
@@ -59,7 +60,7 @@ struct Scratch {
 enum ScratchSlot {
     S_KEYS_A, S_KEYS_B, S_VALS_A, S_VALS_B, S_UNIQ, S_COUNTS, S_SORT_TMP, S_MISC, S_PARTIAL,
     S_MOVES, S_DS, S_ACCEPT, S_COUNTER, S_HASH, S_RISK, S_EVENTS_A, S_EVENTS_B, S_ORDER, S_BEST,
-    S_ASSIGN_TMP, S_FLAGS, S_NUM_SLOTS
+    S_ASSIGN_TMP, S_FLAGS, S_NP, S_NUM_SLOTS
 };
 
 }  // namespace sbpimpl
@@ -72,6 +73,8 @@ struct sbp_ctx_s {
     sbpimpl::Scratch scratch[sbpimpl::S_NUM_SLOTS];
     hipEvent_t ev[8] = {};
     void *pinned = nullptr;  // small pinned staging area for scalars
+    double *lgamma = nullptr;  // fastlgamma table for the nonparametric entropy, grown on demand
+    int64_t lgamma_n = 0;
 };
 
 struct sbp_graph_s {
@@ -114,6 +117,8 @@ struct sbp_bm_s {
     int32_t bcap = 0;            // capacity of the per-block arrays
     double overall_entropy = 1.7976931348623157e308;
     int64_t num_to_merge = 0;
+    int32_t np = 0;  // entropy model: 0 parametric (entropy::mdl), 1 nonparametric (entropy::nonparametric)
+    int64_t E = 0;   // edges of the graph the blockmodel was built on
 };
 
 namespace sbpimpl {
@@ -128,6 +133,12 @@ void bm_ensure_cells(sbp_bm bm, int64_t nnz);
 // mdl.hip
 double log_posterior(sbp_ctx ctx, sbp_bm bm);
 double model_term(int64_t E, int32_t N, int32_t B);
+double host_lgamma(int64_t x);
+double host_lbinom(int64_t n, int64_t k);
+double host_edges_dl(int64_t B, int64_t E);
+int64_t count_nonempty(sbp_ctx ctx, sbp_bm bm);
+sbpdev::NpConst np_const(sbp_ctx ctx, sbp_bm bm);
+double np_mdl(sbp_ctx ctx, sbp_graph g, sbp_bm bm, double *parts);
 // finetune.hip
 struct FtLaunchInfo {
     uint64_t bytes;       // algorithmic bytes of the LDS-table launch (vertices up to FT_SMALL_DEGREE)

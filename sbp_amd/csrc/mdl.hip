@@ -8,8 +8,12 @@
 // bit-identical to any other evaluation order.  Algorithmic bytes: 12 per cell + 8 per block.
 #include <math.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "np_entropy.hpp"
 
 namespace sbpimpl {
 using namespace sbpdev;
@@ -102,6 +106,139 @@ double model_term(int64_t E, int32_t N, int32_t B) {
     double x = pow((double)B, 2) / (double)E;
     double h = ((1 + x) * log(1 + x)) - (x * log(x));
     return ((double)E * h) + ((double)N * log((double)B));
+}
+
+// ------------------------------------------------------------------ nonparametric description length
+
+// fastlgamma / fastlbinom / get_edges_dl on the host with glibc lgamma (include/fastlgamma.hpp:19-27,
+// include/entropy.hpp:145-149, src/entropy.cpp:733-738)
+double host_lgamma(int64_t x) {
+    if (x <= 0) return INFINITY;
+    int sign;
+    return lgamma_r((double)x, &sign);
+}
+double host_lbinom(int64_t n, int64_t k) {
+    if (n == 0 || k == 0 || k > n) return 0;
+    return ((host_lgamma(n + 1) - host_lgamma(k + 1)) - host_lgamma(n - k + 1));
+}
+double host_edges_dl(int64_t B, int64_t E) { return host_lbinom(B * B + E - 1, E); }
+
+// Device table of fastlgamma(x) for x < max(E, N) + 2 (every cell value + 1 and block size + 1), kept on the
+// context and grown when a larger graph needs it.
+static const double *lgamma_table(sbp_ctx ctx, int64_t n, int64_t *n_out) {
+    if (!ctx->lgamma || ctx->lgamma_n < n) {
+        std::vector<double> h((size_t)n);
+        for (int64_t x = 0; x < n; ++x) h[(size_t)x] = host_lgamma(x);
+        SBP_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->lgamma) SBP_HIP(hipFree(ctx->lgamma));
+        ctx->lgamma = nullptr;
+        SBP_HIP(hipMalloc(&ctx->lgamma, sizeof(double) * (size_t)n));
+        SBP_HIP(hipMemcpy(ctx->lgamma, h.data(), sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+        ctx->lgamma_n = n;
+    }
+    *n_out = ctx->lgamma_n;
+    return ctx->lgamma;
+}
+
+__global__ void k_count_nonempty(int32_t B, const int32_t *__restrict__ sizes, int32_t *__restrict__ out) {
+    int32_t local = 0;
+    for (int32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x) local += sizes[b] > 0;
+    local = wave_sum(local);
+    if (lane_id() == 0 && local) atomicAdd(out, local);
+}
+
+int64_t count_nonempty(sbp_ctx ctx, sbp_bm bm) {
+    hipStream_t st = ctx->stream;
+    int32_t *d = ctx->scratch[S_NP].as<int32_t>(2);
+    SBP_HIP(hipMemsetAsync(d, 0, sizeof(int32_t), st));
+    int blocks = (int)std::min<int64_t>((bm->B + 255) / 256, 1024);
+    k_count_nonempty<<<std::max(blocks, 1), 256, 0, st>>>(bm->B, bm->sizes, d);
+    SBP_HIP(hipGetLastError());
+    int32_t h = 0;
+    SBP_HIP(hipMemcpyAsync(&h, d, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    SBP_HIP(hipStreamSynchronize(st));
+    return h;
+}
+
+// Constants of one launch: the lgamma table and the block-count terms around the current non-empty count.
+NpConst np_const(sbp_ctx ctx, sbp_bm bm) {
+    NpConst c;
+    const int64_t N = bm->N, E = bm->E;
+    c.lg = lgamma_table(ctx, std::max<int64_t>(E, N) + 2, &c.n);
+    const int64_t ne = count_nonempty(ctx, bm);
+    for (int i = 0; i < 3; ++i) {
+        c.lb[i] = host_lbinom(N - 1, ne - 2 + i);
+        c.edl[i] = host_edges_dl(ne - 1 + i, E);
+    }
+    return c;
+}
+
+// sparse_entropy (src/entropy.cpp:623-655): eterm over the cells + vterm over the blocks, fixed point
+__global__ void __launch_bounds__(MDL_BLOCK) k_np_sparse(int64_t nnz, int32_t B, const int2 *__restrict__ cells,
+                                                         const int32_t *__restrict__ d_out,
+                                                         const int32_t *__restrict__ d_in,
+                                                         const int32_t *__restrict__ sizes, NpConst c,
+                                                         Fx *__restrict__ partials) {
+    __shared__ Fx wave_part[MDL_BLOCK / WAVE];
+    const int lane = lane_id(), wid = threadIdx.x / WAVE;
+    Fx acc = fx_zero();
+    const int64_t stride = (int64_t)gridDim.x * MDL_BLOCK;
+    for (int64_t i = blockIdx.x * (int64_t)MDL_BLOCK + threadIdx.x; i < nnz + B; i += stride) {
+        if (i < nnz) {
+            fx_acc(acc, np_eterm(c, cells[i].y));
+        } else {
+            const int32_t b = (int32_t)(i - nnz);
+            fx_acc(acc, np_vterm(d_out[b], d_in[b], sizes[b]));
+        }
+    }
+    acc = wave_sum_fx(acc);
+    if (lane == 0) wave_part[wid] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Fx t = wave_part[0];
+        for (int w = 1; w < MDL_BLOCK / WAVE; ++w) fx_add(t, wave_part[w]);
+        partials[blockIdx.x] = t;
+    }
+}
+
+// nonparametric::mdl (src/entropy.cpp:740-767).  parts (optional): sparse entropy, partition dl, edges dl.
+double np_mdl(sbp_ctx ctx, sbp_graph g, sbp_bm bm, double *parts) {
+    hipStream_t st = ctx->stream;
+    const NpConst c = np_const(ctx, bm);
+    const int64_t total = bm->nnz + bm->B;
+    int64_t blocks = std::min<int64_t>((total + MDL_BLOCK - 1) / MDL_BLOCK, 4096);
+    blocks = std::max<int64_t>(blocks, 1);
+    Fx *partials = ctx->scratch[S_PARTIAL].as<Fx>(blocks + 2);
+    k_np_sparse<<<(int)blocks, MDL_BLOCK, 0, st>>>(bm->nnz, bm->B, bm->row_cells, bm->d_out, bm->d_in, bm->sizes, c,
+                                                     partials);
+    k_sum_partials<<<1, MDL_BLOCK, 0, st>>>((int)blocks, partials, partials + blocks);
+    SBP_HIP(hipGetLastError());
+    Fx h;
+    std::vector<int32_t> sizes(bm->B);
+    SBP_HIP(hipMemcpyAsync(&h, partials + blocks, sizeof(Fx), hipMemcpyDeviceToHost, st));
+    SBP_HIP(hipMemcpyAsync(sizes.data(), bm->sizes, sizeof(int32_t) * bm->B, hipMemcpyDeviceToHost, st));
+    SBP_HIP(hipStreamSynchronize(st));
+    const double S = (double)h.hi + (double)h.lo * 0x1p-64;
+    // get_partition_dl (src/entropy.cpp:657-669), in block order
+    int64_t ne = 0;
+    for (int32_t b = 0; b < bm->B; ++b) ne += sizes[b] > 0;
+    const int64_t N = g->N;
+    double part = 0;
+    part += host_lbinom(N - 1, ne - 1);
+    part += host_lgamma(N + 1);
+    for (int32_t b = 0; b < bm->B; ++b) part -= host_lgamma(sizes[b] + 1);
+    part += (double)logf((float)N);  // fastlog(N)
+    const double edges = host_edges_dl(ne, g->E);
+    double S_dl = 0;
+    S_dl += part;
+    S_dl += 0.0;  // get_deg_dl_dist: 0 without degree correction
+    S_dl += edges;
+    if (parts) {
+        parts[0] = S;
+        parts[1] = part;
+        parts[2] = edges;
+    }
+    return S + S_dl * 1.0;
 }
 
 __global__ void k_debug_logf(uint32_t first, uint32_t count, float *out) {

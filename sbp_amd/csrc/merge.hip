@@ -11,6 +11,7 @@
 
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "np_entropy.hpp"
 
 namespace sbpimpl {
 using namespace sbpdev;
@@ -29,6 +30,8 @@ struct MgArgs {
     double *best_dS;
     unsigned long long *bytes;
     int *err;
+    const int32_t *sizes;
+    NpConst np;  // nonparametric runs only
 };
 
 __device__ __forceinline__ long long mg_get(const MgArgs &a, int row, int col) {
@@ -111,6 +114,38 @@ __device__ double merge_delta_mdl(const MgArgs &a, int r, int s) {
     return fx_to(acc);
 }
 
+// Nonparametric delta-MDL of merging r into s (entropy::nonparametric::block_merge_delta_mdl,
+// src/entropy.cpp:1183-1206): eterm differences of the merge Delta's entries -- row r and column r empty,
+// (s, c) and (x, s) receive them, (s, s) receives (r, r) + (r, s) + (s, r) -- then the block terms and the
+// description-length deltas with weight = size(r).
+__device__ double merge_delta_mdl_np(const MgArgs &a, int r, int s) {
+    const int lane = lane_id();
+    if (a.sizes[r] == 0 || a.sizes[s] == 0) return 0.0;
+    const int rr0 = a.row_ptr[r], rr1 = a.row_ptr[r + 1], rs0 = a.row_ptr[s], rs1 = a.row_ptr[s + 1];
+    const int cr0 = a.col_ptr[r], cr1 = a.col_ptr[r + 1], cs0 = a.col_ptr[s], cs1 = a.col_ptr[s + 1];
+    Fx acc = fx_zero();
+    for (int i = rr0 + lane; i < rr1; i += WAVE) {
+        const int2 cell = a.row_cells[i];
+        const int c = cell.x;
+        fx_acc(acc, np_entry(a.np, cell.y, -(long long)cell.y));
+        if (c != r && c != s) fx_acc(acc, np_entry(a.np, cell_lookup(a.row_cells, rs0, rs1, c), cell.y));
+    }
+    for (int i = cr0 + lane; i < cr1; i += WAVE) {
+        const int2 cell = a.col_cells[i];
+        const int x = cell.x;
+        if (x == r) continue;
+        fx_acc(acc, np_entry(a.np, cell.y, -(long long)cell.y));
+        if (x != s) fx_acc(acc, np_entry(a.np, cell_lookup(a.col_cells, cs0, cs1, x), cell.y));
+    }
+    const long long mrr = mg_get(a, r, r), mrs = mg_get(a, r, s), msr = mg_get(a, s, r), mss = mg_get(a, s, s);
+    const long long dss = mrr + mrs + msr;
+    acc = wave_sum_fx(acc);
+    if (dss > 0) fx_acc(acc, np_entry(a.np, mss, dss));
+    return np_finish(a.np, fx_to(acc), a.d_out[r], a.d_in[r], a.sizes[r], a.d_out[s], a.d_in[s], a.sizes[s],
+                     a.d_out[r], a.d_in[r], a.sizes[r]);
+}
+
+template <bool NP>
 __global__ void __launch_bounds__(MG_BLOCK) k_merge_propose(MgArgs a) {
     const int lane = lane_id(), wid = threadIdx.x / WAVE;
     const int64_t nwaves = (int64_t)gridDim.x * MG_WAVES;
@@ -155,7 +190,7 @@ __global__ void __launch_bounds__(MG_BLOCK) k_merge_propose(MgArgs a) {
             for (int j = 0; j < npast; ++j) seen |= (past[j] == s);
             double dS = DBL_MAX;
             if (!seen) {
-                dS = merge_delta_mdl(a, r, s);
+                dS = NP ? merge_delta_mdl_np(a, r, s) : merge_delta_mdl(a, r, s);
                 past[npast++] = s;
                 long long rc_r = (rr1 - rr0) + (cr1 - cr0);
                 long long rc_s = (a.row_ptr[s + 1] - a.row_ptr[s]) + (a.col_ptr[s + 1] - a.col_ptr[s]);
@@ -193,10 +228,14 @@ uint64_t merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *d_best_s
     a.best_s = d_best_s;
     a.best_dS = d_best_dS;
     a.bytes = bytes;
+    a.sizes = bm->sizes;
+    a.np = NpConst{};
+    if (bm->np) a.np = np_const(ctx, bm);
     int64_t blocks = (bm->B + MG_WAVES - 1) / MG_WAVES;
     if (blocks > 8192) blocks = 8192;
     if (blocks < 1) blocks = 1;
-    k_merge_propose<<<(int)blocks, MG_BLOCK, 0, st>>>(a);
+    if (bm->np) k_merge_propose<true><<<(int)blocks, MG_BLOCK, 0, st>>>(a);
+    else k_merge_propose<false><<<(int)blocks, MG_BLOCK, 0, st>>>(a);
     SBP_HIP(hipGetLastError());
     unsigned long long hb[2] = {0, 0};
     SBP_HIP(hipMemcpyAsync(hb, bytes, sizeof(hb), hipMemcpyDeviceToHost, st));

@@ -14,7 +14,8 @@ extern "C" int sbp_host_carry_out(int32_t B, const int32_t *row_ptr, const int32
                                   const int32_t *d_out, const int32_t *d_in, const int32_t *sizes,
                                   const int32_t *best_s, const double *best_dS, int64_t num_to_merge,
                                   int32_t approximate, int32_t threads, int32_t seed_from_proposals,
-                                  int32_t verify_proposals, int32_t *block_map_out, int64_t *stats_out) {
+                                  int32_t verify_proposals, int32_t nonparametric, int64_t num_vertices,
+                                  int32_t *block_map_out, int64_t *stats_out) {
     using clk = std::chrono::steady_clock;
     auto us = [](clk::time_point a, clk::time_point b) {
         return (int64_t)std::chrono::duration_cast<std::chrono::microseconds>(b - a).count();
@@ -23,13 +24,19 @@ extern "C" int sbp_host_carry_out(int32_t B, const int32_t *row_ptr, const int32
     std::unique_ptr<sbpcarry::Pool> pool(nt > 1 ? new sbpcarry::Pool(nt) : nullptr);
     sbpcarry::Model m;
     auto t0 = clk::now();
-    if (!approximate)
+    if (!approximate) {
         m.load(B, row_ptr, row_cols, row_cnts, col_ptr, col_rows, col_cnts, 1, d_out, d_in, pool.get());
+        if (nonparametric) m.set_nonparametric(sizes, num_vertices);
+    }
     auto t1 = clk::now();
     int64_t mismatches = 0;
     if (verify_proposals && !approximate)
-        for (int32_t b = 0; b < B; ++b)
-            if (best_s[b] >= 0 && best_s[b] != b && sbpcarry::merge_dS(m, b, best_s[b]) != best_dS[b]) ++mismatches;
+        for (int32_t b = 0; b < B; ++b) {
+            if (best_s[b] < 0 || best_s[b] == b) continue;
+            const double v = nonparametric ? sbpcarry::np_merge_dS(m, b, best_s[b], sbpcarry::np_merge_entries(m, b, best_s[b]))
+                                           : sbpcarry::merge_dS(m, b, best_s[b]);
+            if (v != best_dS[b]) ++mismatches;
+        }
     auto t2 = clk::now();
     std::vector<int32_t> map;
     sbpcarry::CarryStats cs;

@@ -66,6 +66,10 @@ sbp_params to_params(const Args &a) {
     p.seed = a.seed;
     p.max_iterations = 100;
     p.threads = a.threads;
+    p.nonparametric = a.nonparametric ? 1 : 0;
+    if (a.nonparametric && (a.degreecorrected || a.undirected))
+        throw std::invalid_argument("the nonparametric description length is built for directed graphs without "
+                                    "degree correction");
     return p;
 }
 
@@ -237,6 +241,7 @@ Blockmodel::Blockmodel(long num_blocks, const Graph &graph, double rate, std::ve
     std::vector<int32_t> a(assignment.begin(), assignment.end());
     sbp_bm bm = nullptr;
     throw_on(sbp_bm_build(ctx_, graph.device(ctx_), (int32_t)num_blocks, a.data(), &bm), ctx_, "sbp_bm_build");
+    sbp_bm_set_model(bm, sbp_amd::args().nonparametric ? 1 : 0);
     bm_ = wrap(bm);
     num_blocks_to_merge_ = (long)((double)num_blocks * rate);  // blockmodel.hpp:70-76
     empty = false;
@@ -378,6 +383,7 @@ Blockmodel drive(sbp_ctx ctx, sbp_graph g, int32_t N, const sbp_params &p, sbp_s
     std::iota(ident.begin(), ident.end(), 0);
     sbp_bm first = nullptr;
     throw_on(sbp_bm_build(ctx, g, N, ident.data(), &first), ctx, "sbp_bm_build");
+    sbp_bm_set_model(first, p.nonparametric);
     Blockmodel blockmodel = Blockmodel::adopt(ctx, first, N, 1.7976931348623157e308, (long)((double)N * 0.5));
     BlockmodelTriplet triplet;
     uint32_t ep = 0;
@@ -407,6 +413,10 @@ double mdl(const Blockmodel &bm, const Graph &graph) {
 }
 
 double null_mdl_v1(const Graph &graph) {
+    if (sbp_amd::args().nonparametric) {  // src/entropy.cpp:572-577: the one-block model's description length
+        std::vector<long> zeros(graph.num_vertices(), 0);
+        return mdl(Blockmodel(1, graph, 0.5, zeros), graph);
+    }
     double E = (double)graph.num_edges();
     double log_posterior_p = E * log(1.0 / E);
     double x = 1.0 / E;

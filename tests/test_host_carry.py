@@ -17,7 +17,7 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def host_carry(bm, best_s, best_dS, num_to_merge, approximate, threads=0, seed=True, verify=False):
+def host_carry(bm, best_s, best_dS, num_to_merge, approximate, threads=0, seed=True, verify=False, np_model=False):
     L = C.CDLL(LIB)
     L.sbp_host_carry_out.restype = C.c_int32
     x = bm.export()
@@ -35,7 +35,8 @@ def host_carry(bm, best_s, best_dS, num_to_merge, approximate, threads=0, seed=T
     st = np.zeros(10, np.int64)
     nb = L.sbp_host_carry_out(B, _p(rp), _p(cols), _p(cnts), _p(cp), _p(crow), _p(ccnt), _p(dout), _p(din),
                               _p(x["sizes"]), _p(s), _p(d), C.c_int64(num_to_merge), int(approximate), int(threads),
-                              int(seed), int(verify), _p(bmap), _p(st))
+                              int(seed), int(verify), int(np_model), C.c_int64(len(x["assignment"])), _p(bmap),
+                              _p(st))
     return nb, bmap, st
 
 
@@ -84,3 +85,21 @@ def test_host_carry_speculation_is_exact(N, B):
         assert nb == ref[0] and np.array_equal(bmap, ref[1]), (threads, seed)
     out = bm.copy().merge_apply(best_s, best_dS, B // 2, False)
     assert np.array_equal(ref[1][assign], out.export()["assignment"])
+
+
+@pytest.mark.parametrize("N,B", [(2000, 2000), (3000, 300)])
+def test_host_carry_nonparametric_matches_oracle(N, B):
+    """Nonparametric re-evaluations (block_merge_delta_mdl with the changing non-empty block count)."""
+    g = dcsbm(N, 20, gamma=2.3, kmin=1, kmax=150, mean_degree=10, p_in=0.65, seed=N + 1)
+    og = O.OGraph.from_csr(g)
+    rng = np.random.default_rng(N)
+    assign = np.arange(N, dtype=np.int32) if B == N else np.concatenate(
+        [np.arange(B), rng.integers(0, B, N - B)]).astype(np.int32)
+    bm = O.OBlockmodel.build(og, B, assign, nonparametric=True)
+    best_s, best_dS = bm.merge_propose(seed=2, epoch=3)
+    for threads in (1, 8):
+        nb, bmap, st = host_carry(bm, best_s, best_dS, B // 2, False, threads=threads, verify=True, np_model=True)
+        assert st[9] == 0
+        out = bm.copy().merge_apply(best_s, best_dS, B // 2, False)
+        assert nb == out.B
+        assert np.array_equal(bmap[assign], out.export()["assignment"])
