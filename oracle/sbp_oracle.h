@@ -33,6 +33,18 @@
  *     the sample extension breaks plurality ties by the smallest block id;
  *     the vertex shuffle is the ascending order of Philox sort keys.
  *
+ * What pins each part (tests/test_oracle.py, tests/test_sampling_oracle.py,
+ * tests/test_nonparametric_oracle.py): Philox known-answer vectors; the
+ * reference's toy-graph known answers for the block matrix, degrees, MDL,
+ * null model, vertex / merge delta-MDL, Hastings, max_degree sampling and the
+ * nonparametric description length; brute-force description-length
+ * differences with glibc logf; restatements of the reference's sampling and
+ * extension loops.  The sweeps (async Gibbs, MH, hybrid, finetune_assignment),
+ * merge phases with the carry-out, and the driver have no reference output
+ * under any replay: their parity is unpinned against the reference and rests
+ * on invariants (rebuild equality after every sweep and merge, thread-count
+ * independence) and on recovering planted partitions.
+ *
  * Tags and epochs: every finetune sweep and every merge phase uses its own
  * epoch value, supplied by the driver (the driver numbers them 0, 1, 2, ...
  * in call order).
