@@ -7,6 +7,8 @@
 // integer state, identical to the reference's after any sequence of incremental updates.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -91,10 +93,12 @@ void bm_alloc(sbp_bm bm, int32_t B, int32_t N) {
 }
 
 void bm_release(sbp_bm bm) {
-    void *ptrs[] = {bm->assign, bm->sizes, bm->d_out, bm->d_in, bm->d_tot, bm->row_ptr, bm->col_ptr, bm->row_cells,
-                    bm->col_cells, bm->row_cum, bm->col_cum};
+    void *ptrs[] = {bm->assign,    bm->sizes,     bm->d_out,   bm->d_in,    bm->d_tot,    bm->row_ptr,
+                    bm->col_ptr,   bm->row_cells, bm->col_cells, bm->row_cum, bm->col_cum, bm->cell_hash};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    bm->cell_hash = nullptr;
+    bm->hash_valid = false;
     bm->assign = bm->sizes = bm->d_out = bm->d_in = bm->d_tot = bm->row_ptr = bm->col_ptr = nullptr;
     bm->row_cells = bm->col_cells = nullptr;
     bm->row_cum = bm->col_cum = nullptr;
@@ -147,8 +151,53 @@ void bm_compute_cum(sbp_ctx ctx, sbp_bm bm) {
     SBP_HIP(hipGetLastError());
 }
 
+// Cell hash: one wavefront per row inserts its cells; the table holds at least twice nnz slots.
+__global__ void k_hash_cells(int32_t B, const int32_t *__restrict__ ptr, const int2 *__restrict__ cells,
+                             sbpdev::CellSlot *__restrict__ table, int shift, unsigned mask) {
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / sbpdev::WAVE);
+    for (int64_t b = (int64_t)blockIdx.x * (blockDim.x / sbpdev::WAVE) + threadIdx.x / sbpdev::WAVE; b < B;
+         b += waves) {
+        for (int i = ptr[b] + sbpdev::lane_id(); i < ptr[b + 1]; i += sbpdev::WAVE) {
+            const int2 cell = cells[i];
+            const unsigned long long key = sbpdev::cell_key((int)b, cell.x);
+            for (unsigned h = sbpdev::cell_slot(key, shift);; h = (h + 1) & mask) {
+                if (atomicCAS(&table[h].key, sbpdev::CELL_EMPTY, key) == sbpdev::CELL_EMPTY) {
+                    table[h].val = cell.y;
+                    break;
+                }
+            }
+        }
+    }
+}
+
+void bm_ensure_hash(sbp_ctx ctx, sbp_bm bm) {
+    if (bm->hash_valid) return;
+    hipStream_t st = ctx->stream;
+    int lg = 10;
+    while ((int64_t(1) << lg) < 2 * bm->nnz + 2) ++lg;
+    if (!bm->cell_hash || bm->hash_log2 < lg) {
+        if (bm->cell_hash) {
+            SBP_HIP(hipStreamSynchronize(st));
+            SBP_HIP(hipFree(bm->cell_hash));
+        }
+        bm->cell_hash = nullptr;
+        SBP_HIP(hipMalloc(&bm->cell_hash, sizeof(sbpdev::CellSlot) << lg));
+        bm->hash_log2 = lg;
+    }
+    lg = bm->hash_log2;
+    SBP_HIP(hipMemsetAsync(bm->cell_hash, 0xff, sizeof(sbpdev::CellSlot) << lg, st));
+    if (bm->nnz > 0) {
+        int64_t blocks = std::min<int64_t>((bm->B + 3) / 4, 16384);
+        k_hash_cells<<<(int)std::max<int64_t>(blocks, 1), 256, 0, st>>>(bm->B, bm->row_ptr, bm->row_cells, bm->cell_hash,
+                                                                       64 - lg, (1u << lg) - 1);
+        SBP_HIP(hipGetLastError());
+    }
+    bm->hash_valid = true;
+}
+
 void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm) {
     hipStream_t st = ctx->stream;
+    bm->hash_valid = false;
     const int32_t B = bm->B;
     const int64_t E = g->E;
     bm->E = E;
@@ -227,6 +276,7 @@ void bm_copy_into(sbp_ctx ctx, sbp_bm src, sbp_bm dst) {
     }
     dst->overall_entropy = src->overall_entropy;
     dst->num_to_merge = 0;
+    dst->hash_valid = false;
     dst->np = src->np;
     dst->E = src->E;
 }

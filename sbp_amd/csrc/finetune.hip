@@ -52,6 +52,9 @@ struct BmView {
     const int32_t *rbeg, *rlen, *cbeg, *clen;  // rlen/clen null for compact CSR
     int2 *row_cells, *col_cells;
     const int32_t *row_cum, *col_cum;          // null for the slack layout
+    const CellSlot *hash;                      // cell hash of the compact CSR (null for the slack layout)
+    int hash_shift;
+    unsigned hash_mask;
     __device__ __forceinline__ int row_begin(int b) const { return rbeg[b]; }
     __device__ __forceinline__ int row_end(int b) const { return rlen ? rbeg[b] + rlen[b] : rbeg[b + 1]; }
     __device__ __forceinline__ int col_begin(int b) const { return cbeg[b]; }
@@ -82,6 +85,7 @@ struct FtArgs {
 };
 
 __device__ __forceinline__ long long get_cell(const FtArgs &a, int row, int col) {
+    if (a.m.hash) return cell_hash_get(a.m.hash, a.m.hash_shift, a.m.hash_mask, row, col);
     int r0 = a.m.row_begin(row), r1 = a.m.row_end(row);
     int c0 = a.m.col_begin(col), c1 = a.m.col_end(col);
     if (r1 - r0 <= c1 - c0) return cell_lookup(a.m.row_cells, r0, r1, col);
@@ -110,12 +114,6 @@ __device__ __forceinline__ long long new_dout(const FtArgs &a, const MoveCtx &m,
 }
 
 // old-minus-new entropy of one cell with current value M and change d
-// Kept out of line: its glibc-logf restatement is large and is used at a dozen call sites.
-__device__ __noinline__ void cell_term(const FtArgs &a, const MoveCtx &m, int row, int col, long long M, long long d,
-                                      Fx &acc) {
-    fx_acc(acc, cell_entropy(M, a.d_in[col], a.d_out[row]));
-    fx_sub_acc(acc, cell_entropy(M + d, new_din(a, m, col), new_dout(a, m, row)));
-}
 
 // Delta of row r / row s at column c from the histogram (0 when the cell is not in the Delta).
 __device__ __forceinline__ long long delta_row_r(const MoveCtx &m, int c, int oc) {
@@ -128,16 +126,6 @@ __device__ __forceinline__ long long delta_row_s(const MoveCtx &m, int c, int oc
 __device__ __forceinline__ Table make_table(const FtArgs &a, bool big, int *lds, int64_t gwave) {
     if (big) return table_at(a.gtable + gwave * table_ints(a.gcap_log2), a.gcap_log2);
     return table_at(lds, LDS_CAP_LOG2);
-}
-
-// First index in [lo, hi) whose inclusive cumulative count exceeds target.
-__device__ __forceinline__ int cum_search(const int32_t *cum, int lo, int hi, long long target) {
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if ((long long)cum[mid] <= target) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
 }
 
 // Proposal of vertex v against the current state.  Returns s (== r for "no move").  Fills m.
@@ -200,10 +188,10 @@ __device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w) {
         if (a.m.row_cum) {
             const long long dout_t = a.d_out[t];
             if (target < dout_t) {
-                int i = cum_search(a.m.row_cum, rp0, rp1, target);
+                int i = wave_cum_search(a.m.row_cum, rp0, rp1, target);
                 if (i < rp1) s = a.m.row_cells[i].x;
             } else {
-                int i = cum_search(a.m.col_cum, cp0, cp1, target - dout_t);
+                int i = wave_cum_search(a.m.col_cum, cp0, cp1, target - dout_t);
                 if (i < cp1) s = a.m.col_cells[i].x;
             }
         } else {
@@ -310,6 +298,16 @@ __device__ __noinline__ bool hastings_accept(const FtArgs &a, const MoveCtx &m, 
 // term by term + the other unchanged cells of rows r, s and columns r, s in bulk: each changes only
 // through one block degree d -> d', contributing M * (fastlog(d') - fastlog(d)), so the four bulk terms
 // are (row or column total - Delta and special cells) * (fastlog(d') - fastlog(d)).
+// Old-minus-new entropy of one cell: value M with log degrees (lin of its column, lout of its row) before the
+// move, M + d with (lin2, lout2) after -- common::cell_entropy twice, with the degree logs computed once per
+// proposal instead of per cell.
+__device__ __forceinline__ void cell_change(Fx &acc, long long M, long long d, double lin, double lout, double lin2,
+                                            double lout2) {
+    if (M) fx_acc(acc, (double)M * (fastlog(M) - (lin + lout)));
+    const long long M2 = M + d;
+    if (M2) fx_sub_acc(acc, (double)M2 * (fastlog(M2) - (lin2 + lout2)));
+}
+
 template <bool GREEDY>
 __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept) {
     const int lane = lane_id();
@@ -327,6 +325,11 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
     table_get(t, s, m.out_s, m.in_s);
     const long long mrr = get_cell(a, r, r), mrs = get_cell(a, r, s), msr = get_cell(a, s, r),
                     mss = get_cell(a, s, s);
+    // logs of the degrees of r and s before and after the move
+    const double lo_r = fastlog(a.d_out[r]), lo_r2 = fastlog(new_dout(a, m, r));
+    const double lo_s = fastlog(a.d_out[s]), lo_s2 = fastlog(new_dout(a, m, s));
+    const double li_r = fastlog(a.d_in[r]), li_r2 = fastlog(new_din(a, m, r));
+    const double li_s = fastlog(a.d_in[s]), li_s2 = fastlog(new_din(a, m, s));
     Fx acc = fx_zero();
     long long sub_rrow = 0, sub_srow = 0, sub_rcol = 0, sub_scol = 0;
     const int nb = *t.n_used;
@@ -335,34 +338,41 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
         const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
         const bool special = (c == r) || (c == s);
         if (oc > 0) {
-            long long mrc = c == r ? mrr : (c == s ? mrs : get_cell(a, r, c));
-            long long msc = c == r ? msr : (c == s ? mss : get_cell(a, s, c));
-            cell_term(a, m, r, c, mrc, delta_row_r(m, c, oc), acc);
-            cell_term(a, m, s, c, msc, delta_row_s(m, c, oc), acc);
+            const long long mrc = c == r ? mrr : (c == s ? mrs : get_cell(a, r, c));
+            const long long msc = c == r ? msr : (c == s ? mss : get_cell(a, s, c));
+            double lc, lc2;  // column c's in-degree log, before and after
+            if (c == r) lc = li_r, lc2 = li_r2;
+            else if (c == s) lc = li_s, lc2 = li_s2;
+            else lc = lc2 = fastlog(a.d_in[c]);
+            cell_change(acc, mrc, delta_row_r(m, c, oc), lc, lo_r, lc2, lo_r2);
+            cell_change(acc, msc, delta_row_s(m, c, oc), lc, lo_s, lc2, lo_s2);
             if (!special) {
                 sub_rrow += mrc;
                 sub_srow += msc;
             }
         }
         if (ic > 0 && !special) {
-            long long mcr = get_cell(a, c, r), mcs = get_cell(a, c, s);
-            cell_term(a, m, c, r, mcr, -(long long)ic, acc);
-            cell_term(a, m, c, s, mcs, (long long)ic, acc);
+            const long long mcr = get_cell(a, c, r), mcs = get_cell(a, c, s);
+            const double lr = fastlog(a.d_out[c]);  // row c's out-degree log (unchanged)
+            cell_change(acc, mcr, -(long long)ic, li_r, lr, li_r2, lr);
+            cell_change(acc, mcs, (long long)ic, li_s, lr, li_s2, lr);
             sub_rcol += mcr;
             sub_scol += mcs;
         }
     }
     if (lane == 0) {
         // Delta entries not produced by an out-neighbour block
-        if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0)) cell_term(a, m, r, r, mrr, -(long long)m.selfc - m.in_r, acc);
-        if (m.out_s == 0 && m.in_r > 0) cell_term(a, m, r, s, mrs, (long long)m.in_r, acc);
-        if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) cell_term(a, m, s, s, mss, (long long)m.selfc + m.in_s, acc);
-        if (m.out_r == 0 && m.in_s > 0) cell_term(a, m, s, r, msr, -(long long)m.in_s, acc);
+        if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0))
+            cell_change(acc, mrr, -(long long)m.selfc - m.in_r, li_r, lo_r, li_r2, lo_r2);
+        if (m.out_s == 0 && m.in_r > 0) cell_change(acc, mrs, (long long)m.in_r, li_s, lo_r, li_s2, lo_r2);
+        if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0))
+            cell_change(acc, mss, (long long)m.selfc + m.in_s, li_s, lo_s, li_s2, lo_s2);
+        if (m.out_r == 0 && m.in_s > 0) cell_change(acc, msr, -(long long)m.in_s, li_r, lo_s, li_r2, lo_s2);
         // unchanged cells at rows/columns {r, s} (a zero-net Delta entry counts here as well)
-        if (mrr && delta_row_r(m, r, m.out_r) == 0) cell_term(a, m, r, r, mrr, 0, acc);
-        if (mrs && delta_row_r(m, s, m.out_s) == 0) cell_term(a, m, r, s, mrs, 0, acc);
-        if (msr && delta_row_s(m, r, m.out_r) == 0) cell_term(a, m, s, r, msr, 0, acc);
-        if (mss && delta_row_s(m, s, m.out_s) == 0) cell_term(a, m, s, s, mss, 0, acc);
+        if (mrr && delta_row_r(m, r, m.out_r) == 0) cell_change(acc, mrr, 0, li_r, lo_r, li_r2, lo_r2);
+        if (mrs && delta_row_r(m, s, m.out_s) == 0) cell_change(acc, mrs, 0, li_s, lo_r, li_s2, lo_r2);
+        if (msr && delta_row_s(m, r, m.out_r) == 0) cell_change(acc, msr, 0, li_r, lo_s, li_r2, lo_s2);
+        if (mss && delta_row_s(m, s, m.out_s) == 0) cell_change(acc, mss, 0, li_s, lo_s, li_s2, lo_s2);
     }
     acc = wave_sum_fx(acc);
     sub_rrow = wave_sum(sub_rrow);
@@ -373,10 +383,10 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
     const long long S_srow = (long long)a.d_out[s] - mss - msr - sub_srow;
     const long long S_rcol = (long long)a.d_in[r] - mrr - msr - sub_rcol;
     const long long S_scol = (long long)a.d_in[s] - mss - mrs - sub_scol;
-    fx_acc(acc, (double)S_rrow * (fastlog(new_dout(a, m, r)) - fastlog(a.d_out[r])));
-    fx_acc(acc, (double)S_srow * (fastlog(new_dout(a, m, s)) - fastlog(a.d_out[s])));
-    fx_acc(acc, (double)S_rcol * (fastlog(new_din(a, m, r)) - fastlog(a.d_in[r])));
-    fx_acc(acc, (double)S_scol * (fastlog(new_din(a, m, s)) - fastlog(a.d_in[s])));
+    fx_acc(acc, (double)S_rrow * (lo_r2 - lo_r));
+    fx_acc(acc, (double)S_srow * (lo_s2 - lo_s));
+    fx_acc(acc, (double)S_rcol * (li_r2 - li_r));
+    fx_acc(acc, (double)S_scol * (li_s2 - li_s));
     const double dS = fx_to(acc);
     if (GREEDY) {
         accept = dS < 0.0;
@@ -533,6 +543,9 @@ static FtArgs base_args(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     a.m.col_cells = bm->col_cells;
     a.m.row_cum = bm->row_cum;
     a.m.col_cum = bm->col_cum;
+    a.m.hash = nullptr;
+    a.m.hash_shift = 0;
+    a.m.hash_mask = 0;
     a.items = nullptr;
     a.count = 0;
     a.move_s = nullptr;
@@ -557,6 +570,10 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     auto *counters = ctx->scratch[S_COUNTER].as<unsigned long long>(4);
     SBP_HIP(hipMemsetAsync(counters, 0, 4 * sizeof(unsigned long long), st));
     FtArgs a = base_args(ctx, bm, g, epoch, greedy);
+    bm_ensure_hash(ctx, bm);
+    a.m.hash = bm->cell_hash;
+    a.m.hash_shift = 64 - bm->hash_log2;
+    a.m.hash_mask = (1u << bm->hash_log2) - 1;
     a.bytes = counters;
     a.err = reinterpret_cast<int *>(counters + 1);
     a.big_count = reinterpret_cast<int32_t *>(counters + 2);
@@ -1085,6 +1102,7 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         k_compact_copy<<<grid_of((int64_t)B * 64), 64, 0, st>>>(B, h.rstart, h.rlen, rc, rp, bm->row_cells);
         k_compact_copy<<<grid_of((int64_t)B * 64), 64, 0, st>>>(B, h.cstart, h.clen, cc, cp, bm->col_cells);
         bm->nnz = nnz[0];
+        bm->hash_valid = false;
         bm_compute_cum(ctx, bm);
         SBP_HIP(hipStreamSynchronize(st));
         if (hs[1] == 0) break;

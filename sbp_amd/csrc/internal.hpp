@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/sbp_amd.h"
+#include "device_common.hpp"
 #include "np_entropy.hpp"
 
 namespace sbpimpl {
@@ -119,6 +120,10 @@ struct sbp_bm_s {
     int64_t num_to_merge = 0;
     int32_t np = 0;  // entropy model: 0 parametric (entropy::mdl), 1 nonparametric (entropy::nonparametric)
     int64_t E = 0;   // edges of the graph the blockmodel was built on
+    // cell hash for O(1) lookups (built on demand; any change to the cells invalidates it)
+    sbpdev::CellSlot *cell_hash = nullptr;
+    int hash_log2 = 0;
+    bool hash_valid = false;
 };
 
 namespace sbpimpl {
@@ -129,6 +134,7 @@ void bm_release(sbp_bm bm);
 void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm);  // from bm->assign (device), bm->B set
 void bm_copy_into(sbp_ctx ctx, sbp_bm src, sbp_bm dst);
 void bm_compute_cum(sbp_ctx ctx, sbp_bm bm);
+void bm_ensure_hash(sbp_ctx ctx, sbp_bm bm);
 void bm_ensure_cells(sbp_bm bm, int64_t nnz);
 // mdl.hip
 double log_posterior(sbp_ctx ctx, sbp_bm bm);

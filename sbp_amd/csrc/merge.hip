@@ -4,7 +4,7 @@
 // propose_merge_sparse (:251-285), merge-mode common::propose_new_block (src/common.cpp:138-198) and
 // entropy::block_merge_delta_mdl (src/entropy.cpp:129-182).  The merge Delta of
 // block_merge::blockmodel_delta (src/block_merge.cpp:14-38) moves all of row r and column r into s; its
-// cells are enumerated straight from the sorted row/column cells of r, with sorted lookups into row s /
+// cells are enumerated straight from the row/column cells of r, with hash lookups of the cells of row s /
 // column s, so no per-proposal map is built.  Repeated proposals of the same s are skipped as
 // past_proposals does.  Terms are summed in 128-bit fixed point.
 #include <float.h>
@@ -32,13 +32,13 @@ struct MgArgs {
     int *err;
     const int32_t *sizes;
     NpConst np;  // nonparametric runs only
+    const CellSlot *hash;
+    int hash_shift;
+    unsigned hash_mask;
 };
 
 __device__ __forceinline__ long long mg_get(const MgArgs &a, int row, int col) {
-    int r0 = a.row_ptr[row], r1 = a.row_ptr[row + 1];
-    int c0 = a.col_ptr[col], c1 = a.col_ptr[col + 1];
-    if (r1 - r0 <= c1 - c0) return cell_lookup(a.row_cells, r0, r1, col);
-    return cell_lookup(a.col_cells, c0, c1, row);
+    return cell_hash_get(a.hash, a.hash_shift, a.hash_mask, row, col);
 }
 
 // Wave-cooperative selection: first cell (in list order, skipping excluded ids) whose inclusive prefix of
@@ -66,8 +66,8 @@ __device__ __forceinline__ int wave_select(const int2 *cells, int begin, int end
 __device__ double merge_delta_mdl(const MgArgs &a, int r, int s) {
     const int lane = lane_id();
     const long long k_out = a.d_out[r], k_in = a.d_in[r];
-    const int rr0 = a.row_ptr[r], rr1 = a.row_ptr[r + 1], rs0 = a.row_ptr[s], rs1 = a.row_ptr[s + 1];
-    const int cr0 = a.col_ptr[r], cr1 = a.col_ptr[r + 1], cs0 = a.col_ptr[s], cs1 = a.col_ptr[s + 1];
+    const int rr0 = a.row_ptr[r], rr1 = a.row_ptr[r + 1];
+    const int cr0 = a.col_ptr[r], cr1 = a.col_ptr[r + 1];
     const long long din_s_new = (long long)a.d_in[s] + k_in, dout_s_new = (long long)a.d_out[s] + k_out;
     Fx acc = fx_zero();
     long long sub_row = 0, sub_col = 0;
@@ -76,7 +76,7 @@ __device__ double merge_delta_mdl(const MgArgs &a, int r, int s) {
         int c = cell.x;
         fx_acc(acc, cell_entropy(cell.y, a.d_in[c], a.d_out[r]));
         if (c != r && c != s) {
-            long long msc = cell_lookup(a.row_cells, rs0, rs1, c);
+            long long msc = mg_get(a, s, c);
             fx_acc(acc, cell_entropy(msc, a.d_in[c], a.d_out[s]));
             fx_sub_acc(acc, cell_entropy(msc + cell.y, a.d_in[c], dout_s_new));
             sub_row += msc;
@@ -88,7 +88,7 @@ __device__ double merge_delta_mdl(const MgArgs &a, int r, int s) {
         if (x == r) continue;
         fx_acc(acc, cell_entropy(cell.y, a.d_in[r], a.d_out[x]));
         if (x != s) {
-            long long mxs = cell_lookup(a.col_cells, cs0, cs1, x);
+            long long mxs = mg_get(a, x, s);
             fx_acc(acc, cell_entropy(mxs, a.d_in[s], a.d_out[x]));
             fx_sub_acc(acc, cell_entropy(mxs + cell.y, din_s_new, a.d_out[x]));
             sub_col += mxs;
@@ -121,21 +121,21 @@ __device__ double merge_delta_mdl(const MgArgs &a, int r, int s) {
 __device__ double merge_delta_mdl_np(const MgArgs &a, int r, int s) {
     const int lane = lane_id();
     if (a.sizes[r] == 0 || a.sizes[s] == 0) return 0.0;
-    const int rr0 = a.row_ptr[r], rr1 = a.row_ptr[r + 1], rs0 = a.row_ptr[s], rs1 = a.row_ptr[s + 1];
-    const int cr0 = a.col_ptr[r], cr1 = a.col_ptr[r + 1], cs0 = a.col_ptr[s], cs1 = a.col_ptr[s + 1];
+    const int rr0 = a.row_ptr[r], rr1 = a.row_ptr[r + 1];
+    const int cr0 = a.col_ptr[r], cr1 = a.col_ptr[r + 1];
     Fx acc = fx_zero();
     for (int i = rr0 + lane; i < rr1; i += WAVE) {
         const int2 cell = a.row_cells[i];
         const int c = cell.x;
         fx_acc(acc, np_entry(a.np, cell.y, -(long long)cell.y));
-        if (c != r && c != s) fx_acc(acc, np_entry(a.np, cell_lookup(a.row_cells, rs0, rs1, c), cell.y));
+        if (c != r && c != s) fx_acc(acc, np_entry(a.np, mg_get(a, s, c), cell.y));
     }
     for (int i = cr0 + lane; i < cr1; i += WAVE) {
         const int2 cell = a.col_cells[i];
         const int x = cell.x;
         if (x == r) continue;
         fx_acc(acc, np_entry(a.np, cell.y, -(long long)cell.y));
-        if (x != s) fx_acc(acc, np_entry(a.np, cell_lookup(a.col_cells, cs0, cs1, x), cell.y));
+        if (x != s) fx_acc(acc, np_entry(a.np, mg_get(a, x, s), cell.y));
     }
     const long long mrr = mg_get(a, r, r), mrs = mg_get(a, r, s), msr = mg_get(a, s, r), mss = mg_get(a, s, s);
     const long long dss = mrr + mrs + msr;
@@ -229,6 +229,10 @@ uint64_t merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *d_best_s
     a.best_dS = d_best_dS;
     a.bytes = bytes;
     a.sizes = bm->sizes;
+    bm_ensure_hash(ctx, bm);
+    a.hash = bm->cell_hash;
+    a.hash_shift = 64 - bm->hash_log2;
+    a.hash_mask = (1u << bm->hash_log2) - 1;
     a.np = NpConst{};
     if (bm->np) a.np = np_const(ctx, bm);
     int64_t blocks = (bm->B + MG_WAVES - 1) / MG_WAVES;
