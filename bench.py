@@ -86,9 +86,9 @@ def gpu_sweeps_at_truth(ctx, dg, g, algorithm, sweeps):
     return props / (time.perf_counter() - t0)
 
 
-KERNEL_NAMES = {  # rocprofv3 names of the finetune proposal launches (greedy / non-greedy)
-    "ft": ("k_ft_propose<false, true>", "k_ft_propose<false, false>"),
-    "ft_high": ("k_ft_propose<true, true>", "k_ft_propose<true, false>"),
+KERNEL_NAMES = {  # rocprofv3 names of the finetune proposal launches (greedy / non-greedy, parametric entropy)
+    "ft": ("k_ft_propose<false, true, false>", "k_ft_propose<false, false, false>"),
+    "ft_high": ("k_ft_propose<true, true, false>", "k_ft_propose<true, false, false>"),
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ft_propose_c2_traffic.txt")
 
