@@ -466,7 +466,7 @@ __global__ void __launch_bounds__(FT_BLOCK) k_ft_propose(FtArgs a) {
     const int64_t gwave = (int64_t)blockIdx.x * FT_WAVES + wid;
     const int64_t nwaves = (int64_t)gridDim.x * FT_WAVES;
     const Table t = make_table(a, BIG, lds + (BIG ? 0 : wid * table_ints(LDS_CAP_LOG2)), gwave);
-    table_clear_all(t);
+    if (!BIG) table_clear_all(t);  // the global tables arrive empty (prepare_tables) and are left empty
     const int64_t count = BIG ? (int64_t)*a.big_count : a.count;
     unsigned long long bytes = 0;
     for (int64_t item = gwave; item < count; item += nwaves) {
@@ -562,6 +562,36 @@ static FtArgs base_args(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     return a;
 }
 
+// Empties `tables` global neighbour-block tables of capacity 2^lg at the start of slot S_HASH, unless they are
+// already known to be empty: each use empties the slots it claims, so they stay empty between launches.
+__global__ void k_table_init(int *base, int lg, int64_t tables) {
+    const int64_t cap = (int64_t)1 << lg, ints = table_ints(lg);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tables * cap;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int *t = base + (i >> lg) * ints;
+        const int64_t k = i & (cap - 1);
+        t[k] = EMPTY;
+        t[cap + k] = 0;
+        t[2 * cap + k] = 0;
+        if (k == 0) t[4 * cap] = 0;
+    }
+}
+
+static int *prepare_tables(sbp_ctx ctx, int lg, int64_t tables) {
+    int *base = ctx->scratch[S_HASH].as<int>((size_t)tables * (size_t)table_ints(lg));
+    const uint64_t gen = ctx->scratch[S_HASH].generation;
+    if (gen != ctx->hash_clean_gen || lg != ctx->hash_clean_lg || tables > ctx->hash_clean_tables) {
+        const int64_t n = tables << lg;
+        const int blocks = (int)std::min<int64_t>((n + 255) / 256, 16384);
+        k_table_init<<<std::max(blocks, 1), 256, 0, ctx->stream>>>(base, lg, tables);
+        SBP_HIP(hipGetLastError());
+        ctx->hash_clean_gen = gen;
+        ctx->hash_clean_lg = lg;
+        ctx->hash_clean_tables = tables;
+    }
+    return base;
+}
+
 FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int greedy, const int32_t *d_vertices,
                         int64_t count, int32_t *d_move_s, double *d_dS, int32_t *d_s_out) {
     hipStream_t st = ctx->stream;
@@ -600,7 +630,7 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
         int64_t waves = std::min<int64_t>(g->n_high_degree, count);
         waves = std::max<int64_t>(1, std::min<int64_t>(waves, kMaxTableBytes / (tints * sizeof(int))));
         const int64_t big_blocks = (waves + FT_WAVES - 1) / FT_WAVES;
-        a.gtable = ctx->scratch[S_HASH].as<int>((size_t)big_blocks * FT_WAVES * tints);
+        a.gtable = prepare_tables(ctx, lg, big_blocks * FT_WAVES);
         a.gcap_log2 = lg;
         launch_ft<true>((int)big_blocks, st, a, greedy != 0, bm->np != 0);
         SBP_HIP(hipGetLastError());
@@ -873,8 +903,7 @@ __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start)
     int32_t status = 0, stop = (int32_t)a.count;
     const Table small_t = make_table(a, false, lds, 0);
     const Table big_t = make_table(a, true, lds, 0);
-    table_clear_all(small_t);
-    if (a.gtable) table_clear_all(big_t);
+    table_clear_all(small_t);  // the global table arrives empty (prepare_tables) and is left empty
     for (int64_t item = start; item < a.count; ++item) {
         const int v = a.items[item];
         const int r0 = a.assign[v];
@@ -1068,7 +1097,7 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         a.bytes = bytes;
         a.err = reinterpret_cast<int *>(bytes + 1);
         const int lg = std::max(10, log2_ceil(2 * std::min<int64_t>(g->max_degree, bm->B) + 2));
-        a.gtable = g->max_degree > SMALL_K ? ctx->scratch[S_HASH].as<int>((size_t)table_ints(lg)) : nullptr;
+        a.gtable = g->max_degree > SMALL_K ? prepare_tables(ctx, lg, 1) : nullptr;
         a.gcap_log2 = lg;
         a.m.row_cum = nullptr;
         a.m.col_cum = nullptr;

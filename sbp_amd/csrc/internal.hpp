@@ -35,8 +35,10 @@ struct Error : std::runtime_error {
 struct Scratch {
     void *ptr = nullptr;
     size_t bytes = 0;
+    uint64_t generation = 0;  // incremented whenever the buffer is replaced
     void *get(size_t need) {
         if (need > bytes) {
+            ++generation;
             // queued kernels may still read the old buffer
             if (ptr) {
                 SBP_HIP(hipDeviceSynchronize());
@@ -76,6 +78,11 @@ struct sbp_ctx_s {
     void *pinned = nullptr;  // small pinned staging area for scalars
     double *lgamma = nullptr;  // fastlgamma table for the nonparametric entropy, grown on demand
     int64_t lgamma_n = 0;
+    // the global-memory neighbour-block tables in scratch slot S_HASH that are known to be empty (every kernel
+    // that uses them leaves them empty): buffer, table geometry (log2 capacity) and count
+    uint64_t hash_clean_gen = 0;  // generation of slot S_HASH the state below refers to (0 = none)
+    int hash_clean_lg = -1;
+    int64_t hash_clean_tables = 0;
 };
 
 struct sbp_graph_s {
