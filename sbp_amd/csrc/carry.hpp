@@ -376,33 +376,70 @@ struct Model {
     int64_t get(int32_t r, int32_t c) const { return row[r].n <= col[c].n ? row[r].get(c) : col[c].get(r); }
 };
 
+// Copies the occupied slots of `f` into `out` (no branch per slot); returns their number.
+inline size_t gather(const FlatMap &f, std::vector<FlatMap::Slot> &out) {
+    if (out.size() < f.cap) out.resize(f.cap);
+    size_t k = 0;
+    FlatMap::Slot *o = out.data();
+    for (uint32_t i = 0; i < f.cap; ++i) {
+        o[k] = f.s[i];
+        k += f.s[i].key >= 0;
+    }
+    return k;
+}
+
 // entropy::block_merge_delta_mdl (src/entropy.cpp:129-182) for merging r into s on the current state:
 // Delta entries term by term, (s, s) when unchanged, the other unchanged cells of row s / column s in bulk
 // (M * (fastlog(d') - fastlog(d)) each) -- the same terms as the merge-proposal kernel.
 inline double merge_dS(const Model &m, int32_t r, int32_t s) {
+    const float *L = logf_table().v.data();
+    auto lg = [L](int64_t x) -> double {
+        if (x < LogfTable::SIZE) return x <= 0 ? 0.0 : (double)L[x];
+        return (double)logf((float)x);
+    };
     const int64_t k_out = m.d_out[r], k_in = m.d_in[r];
     const int64_t din_s_new = m.d_in[s] + k_in, dout_s_new = m.d_out[s] + k_out;
+    const double l_dout_r = lg(m.d_out[r]), l_dout_s = lg(m.d_out[s]), l_dout_sn = lg(dout_s_new);
+    const double l_din_r = lg(m.d_in[r]), l_din_s = lg(m.d_in[s]), l_din_sn = lg(din_s_new);
     Fx128 acc;
     int64_t sub_row = 0, sub_col = 0;
-    m.row[r].each([&](int32_t c, int64_t v) {
-        acc.add(cell(v, m.d_in[c], m.d_out[r]));
+    const FlatMap &Rs = m.row[s], &Cs = m.col[s];
+    thread_local std::vector<FlatMap::Slot> ent;
+    size_t n = gather(m.row[r], ent);
+    for (size_t i = 0; i < n; ++i) {
+        if (i + 8 < n) {
+            __builtin_prefetch(&m.d_in[ent[i + 8].key]);
+            __builtin_prefetch(&Rs.s[Rs.slot(ent[i + 8].key)]);
+        }
+        const int32_t c = ent[i].key;
+        const int64_t v = ent[i].val;
+        const double ld = lg(m.d_in[c]);
+        acc.add((double)v * (lg(v) - (ld + l_dout_r)));
         if (c != r && c != s) {
-            int64_t msc = m.row[s].get(c);
-            acc.add(cell(msc, m.d_in[c], m.d_out[s]));
-            acc.sub(cell(msc + v, m.d_in[c], dout_s_new));
+            const int64_t msc = Rs.get(c);
+            if (msc) acc.add((double)msc * (lg(msc) - (ld + l_dout_s)));
+            acc.sub((double)(msc + v) * (lg(msc + v) - (ld + l_dout_sn)));
             sub_row += msc;
         }
-    });
-    m.col[r].each([&](int32_t x, int64_t v) {
-        if (x == r) return;
-        acc.add(cell(v, m.d_in[r], m.d_out[x]));
+    }
+    n = gather(m.col[r], ent);
+    for (size_t i = 0; i < n; ++i) {
+        if (i + 8 < n) {
+            __builtin_prefetch(&m.d_out[ent[i + 8].key]);
+            __builtin_prefetch(&Cs.s[Cs.slot(ent[i + 8].key)]);
+        }
+        const int32_t x = ent[i].key;
+        if (x == r) continue;
+        const int64_t v = ent[i].val;
+        const double ld = lg(m.d_out[x]);
+        acc.add((double)v * (lg(v) - (l_din_r + ld)));
         if (x != s) {
-            int64_t mxs = m.col[s].get(x);
-            acc.add(cell(mxs, m.d_in[s], m.d_out[x]));
-            acc.sub(cell(mxs + v, din_s_new, m.d_out[x]));
+            const int64_t mxs = Cs.get(x);
+            if (mxs) acc.add((double)mxs * (lg(mxs) - (l_din_s + ld)));
+            acc.sub((double)(mxs + v) * (lg(mxs + v) - (l_din_sn + ld)));
             sub_col += mxs;
         }
-    });
+    }
     const int64_t mrr = m.get(r, r), mrs = m.get(r, s), msr = m.get(s, r), mss = m.get(s, s);
     const int64_t dss = mrr + mrs + msr;
     if (dss > 0) {
@@ -413,8 +450,8 @@ inline double merge_dS(const Model &m, int32_t r, int32_t s) {
         acc.sub(cell(mss, din_s_new, dout_s_new));
     }
     const int64_t S_row = m.d_out[s] - mss - msr - sub_row, S_col = m.d_in[s] - mss - mrs - sub_col;
-    acc.add((double)S_row * (fastlog(dout_s_new) - fastlog(m.d_out[s])));
-    acc.add((double)S_col * (fastlog(din_s_new) - fastlog(m.d_in[s])));
+    acc.add((double)S_row * (l_dout_sn - l_dout_s));
+    acc.add((double)S_col * (l_din_sn - l_din_s));
     return acc.value();
 }
 
@@ -441,29 +478,68 @@ inline double np_merge_dS(const Model &m, int32_t r, int32_t s, double entries) 
                            m.nonempty);
 }
 
+// Renames key `from` to `to` in the maps `maps[ent[i].key]`, i < n, which hold count ent[i].val under `from`,
+// prefetching each map and then its two slot lines a few entries ahead: the maps are scattered over a model
+// much larger than the caches.
+inline void rename_in(std::vector<FlatMap> &maps, const FlatMap::Slot *ent, size_t n, int32_t from, int32_t to) {
+    constexpr size_t D1 = 16, D2 = 6;
+    for (size_t i = 0; i < std::min(n, D1); ++i) __builtin_prefetch(&maps[ent[i].key]);
+    for (size_t i = 0; i < std::min(n, D2); ++i) {
+        const FlatMap &f = maps[ent[i].key];
+        __builtin_prefetch(&f.s[f.slot(from)]);
+        __builtin_prefetch(&f.s[f.slot(to)]);
+    }
+    for (size_t i = 0; i < n; ++i) {
+        if (i + D1 < n) __builtin_prefetch(&maps[ent[i + D1].key]);
+        if (i + D2 < n) {
+            const FlatMap &f = maps[ent[i + D2].key];
+            __builtin_prefetch(&f.s[f.slot(from)]);
+            __builtin_prefetch(&f.s[f.slot(to)]);
+        }
+        FlatMap &f = maps[ent[i].key];
+        f.erase(from);
+        f.add(to, ent[i].val);
+    }
+}
+
 // Blockmodel::merge_block (src/blockmodel/blockmodel.cpp:337-363): row r and column r move into s.
 inline void merge_apply(Model &m, int32_t r, int32_t s) {
-    std::vector<std::pair<int32_t, int64_t>> rr, rc;
-    rr.reserve(m.row[r].n);
-    rc.reserve(m.col[r].n);
-    m.row[r].each([&](int32_t c, int64_t v) { rr.emplace_back(c, v); });
-    m.col[r].each([&](int32_t x, int64_t v) { rc.emplace_back(x, v); });
-    for (auto &cv : rr) {  // (r, c) -> (s, c'), c' = s for c in {r, s}
-        int32_t c = cv.first, c2 = (c == r || c == s) ? s : c;
-        m.row[s].add(c2, cv.second);
-        m.col[c].erase(r);
-        m.col[c2].add(s, cv.second);
+    FlatMap &Rr = m.row[r], &Cr = m.col[r], &Rs = m.row[s], &Cs = m.col[s];
+    thread_local std::vector<FlatMap::Slot> ent;
+    // (r, c) -> (s, c'), c' = s for c in {r, s}.  Row r is only read here; column r loses key r (self cell).
+    ent.clear();
+    for (uint32_t i = 0; i < Rr.cap; ++i) {
+        const int32_t c = Rr.s[i].key;
+        if (c < 0) continue;
+        const int64_t v = Rr.s[i].val;
+        if (c == r || c == s) {
+            Rs.add(s, v);
+            m.col[c].erase(r);
+            Cs.add(s, v);
+        } else {
+            Rs.add(c, v);
+            ent.push_back(Rr.s[i]);
+        }
     }
-    for (auto &xv : rc) {  // (x, r) -> (x', s), x' = s for x = s
-        int32_t x = xv.first;
-        if (x == r) continue;
-        int32_t x2 = (x == s) ? s : x;
-        m.col[s].add(x2, xv.second);
-        m.row[x].erase(r);
-        m.row[x2].add(s, xv.second);
+    rename_in(m.col, ent.data(), ent.size(), r, s);
+    // (x, r) -> (x', s), x' = s for x = s
+    ent.clear();
+    for (uint32_t i = 0; i < Cr.cap; ++i) {
+        const int32_t x = Cr.s[i].key;
+        if (x < 0 || x == r) continue;
+        const int64_t v = Cr.s[i].val;
+        if (x == s) {
+            Cs.add(s, v);
+            Rs.erase(r);
+            Rs.add(s, v);
+        } else {
+            Cs.add(x, v);
+            ent.push_back(Cr.s[i]);
+        }
     }
-    m.row[r].clear();
-    m.col[r].clear();
+    rename_in(m.row, ent.data(), ent.size(), r, s);
+    Rr.clear();
+    Cr.clear();
     m.d_out[s] += m.d_out[r];
     m.d_in[s] += m.d_in[r];
     m.d_out[r] = 0;
@@ -558,13 +634,20 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
                     c_val[b] = best_dS[b];
                 }
         int64_t now = 0;
+        // newest stamp among the keys of f (no branch per slot)
+        auto newest = [&](const FlatMap &f) {
+            int64_t mx = 0;
+            for (uint32_t i = 0; i < f.cap; ++i) {
+                const int32_t k = f.s[i].key;
+                const int64_t t = stamp[k < 0 ? 0 : k];
+                mx = (k >= 0 && t > mx) ? t : mx;
+            }
+            return mx;
+        };
         auto valid = [&](int32_t r, int32_t t) {
             const int64_t T = c_time[r];
             if (c_to[r] != t || T < stamp[r] || T < stamp[t]) return false;
-            bool ok = true;
-            m->row[r].each([&](int32_t c, int64_t) { ok = ok && stamp[c] <= T; });
-            if (ok) m->col[r].each([&](int32_t x, int64_t) { ok = ok && stamp[x] <= T; });
-            return ok;
+            return newest(m->row[r]) <= T && newest(m->col[r]) <= T;
         };
         auto root_ro = [&](int32_t x) {
             while (parent[x] != x) x = parent[x];
