@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <functional>
 #include <memory>
@@ -253,77 +254,98 @@ inline double np_merge_finish(double entries, int64_t dout_r, int64_t din_r, int
     return dS + 1.0 * dS_dl;
 }
 
-// Fork-join helper for the speculative evaluations: fn(i) for i in [0, n) on up to `threads` threads.
+// Fork-join helper for the speculative evaluations: fn(i) for i in [0, n) on up to `threads` threads.  The
+// carry-out forks every few hundred microseconds, so idle workers spin on the generation counter for about
+// a millisecond before they sleep on the condition variable; a fork then costs a cache-line transfer, not
+// a futex wake-up per worker.  Every worker checks in once per generation, so a slow waker can never run a
+// later generation's indices with an earlier task.
 class Pool {
   public:
     explicit Pool(int threads) : nthreads_(std::max(1, threads)) {
         for (int t = 1; t < nthreads_; ++t) workers_.emplace_back([this] { work(); });
     }
     ~Pool() {
+        stop_.store(true);
+        gen_.fetch_add(1);
         {
             std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
-            ++gen_;
+            cv_.notify_all();
         }
-        cv_.notify_all();
         for (auto &w : workers_) w.join();
     }
+    int threads() const { return nthreads_; }
     template <typename F>
     void run(int64_t n, F fn) {
         if (n <= 0) return;
-        if (nthreads_ == 1 || n < 8) {
+        if (nthreads_ == 1 || n < 2) {
             for (int64_t i = 0; i < n; ++i) fn(i);
             return;
         }
         std::function<void(int64_t)> f = fn;
-        {
+        task_ = &f;
+        n_ = n;
+        next_.store(0, std::memory_order_relaxed);
+        pending_.store(nthreads_ - 1, std::memory_order_relaxed);
+        gen_.fetch_add(1);  // publishes the task (sequentially consistent, paired with sleepers_ below)
+        if (sleepers_.load() > 0) {
             std::lock_guard<std::mutex> lk(mu_);
-            task_ = &f;
-            n_ = n;
-            next_.store(0);
-            busy_ = nthreads_ - 1;
-            ++gen_;
+            cv_.notify_all();
         }
-        cv_.notify_all();
         drain(f);
-        std::unique_lock<std::mutex> lk(mu_);
-        done_cv_.wait(lk, [this] { return busy_ == 0; });
+        while (pending_.load(std::memory_order_acquire) != 0) relax();
         task_ = nullptr;
     }
 
   private:
+    static void relax() {
+#if defined(__x86_64__)
+        __builtin_ia32_pause();
+#else
+        std::this_thread::yield();
+#endif
+    }
     void drain(const std::function<void(int64_t)> &f) {
-        for (int64_t i; (i = next_.fetch_add(1)) < n_;) f(i);
+        for (int64_t i; (i = next_.fetch_add(1, std::memory_order_relaxed)) < n_;) f(i);
     }
     void work() {
         uint64_t seen = 0;
         while (true) {
-            const std::function<void(int64_t)> *f;
-            {
+            uint64_t g;
+            auto t0 = std::chrono::steady_clock::now();
+            for (uint32_t spins = 0; (g = gen_.load(std::memory_order_acquire)) == seen; ++spins) {
+                relax();
+                if ((spins & 255) != 255 || std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(1))
+                    continue;
                 std::unique_lock<std::mutex> lk(mu_);
-                cv_.wait(lk, [&] { return gen_ != seen; });
-                seen = gen_;
-                if (stop_) return;
-                f = task_;
+                sleepers_.fetch_add(1);
+                cv_.wait(lk, [&] { return gen_.load() != seen; });
+                sleepers_.fetch_sub(1);
+                t0 = std::chrono::steady_clock::now();
             }
-            drain(*f);
-            std::lock_guard<std::mutex> lk(mu_);
-            if (--busy_ == 0) done_cv_.notify_one();
+            seen = g;
+            if (stop_.load()) return;
+            drain(*task_);
+            pending_.fetch_sub(1, std::memory_order_acq_rel);
         }
     }
     int nthreads_;
     std::vector<std::thread> workers_;
     std::mutex mu_;
-    std::condition_variable cv_, done_cv_;
-    uint64_t gen_ = 0;
-    bool stop_ = false;
+    std::condition_variable cv_;
+    std::atomic<uint64_t> gen_{0};
+    std::atomic<int> sleepers_{0}, pending_{0};
+    std::atomic<bool> stop_{false};
     const std::function<void(int64_t)> *task_ = nullptr;
     int64_t n_ = 0;
     std::atomic<int64_t> next_{0};
-    int busy_ = 0;
 };
 
+// Worker threads of the carry-out: SBP_AMD_CARRY_THREADS when set (1..64), else the hardware threads up to 16.
 inline int carry_threads() {
+    if (const char *e = getenv("SBP_AMD_CARRY_THREADS")) {
+        const int t = atoi(e);
+        if (t >= 1 && t <= 64) return t;
+    }
     unsigned hw = std::thread::hardware_concurrency();
     return (int)std::min<unsigned>(16, hw ? hw : 1);
 }
@@ -569,9 +591,10 @@ struct CarryStats {
 //   * delta-MDL(r -> t) reads rows/columns r and t, the degrees of r, t and of N(r) = the blocks sharing a
 //     cell with r, and the cells of t keyed by N(r) + {r, t}.  A merge a -> b changes rows/columns a and b,
 //     the entries keyed a (which become b) of the rows/columns of N(a), and the degrees of a and b.  So
-//     it leaves delta-MDL(r -> t) unchanged unless a or b is r, t or (then or now) in N(r); a and b get
-//     stamped with the merge count, and an evaluation is still exact while no block of N(r) + {r, t} has
-//     a newer stamp;
+//     it leaves delta-MDL(r -> t) unchanged unless r or t is a or b, or r is in N(a) or N(b) (a in N(r)
+//     or b in N(r)).  After the merge every block of N(b), which now holds N(a), is marked with the merge
+//     count (its row and column were just written, so the scan is cache-hot), a and b are stamped with it,
+//     and an evaluation made at time T is still exact while stamp[r], stamp[t] and mark[r] are <= T;
 //   * the proposal values (computed on the same state with the same terms and fixed-point sum) are the
 //     first evaluations;
 //   * items are taken from the heap in batches and the ones without a valid evaluation are evaluated in
@@ -619,7 +642,7 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
         std::priority_queue<Item, std::vector<Item>, decltype(worse)> heap(worse, std::move(init));
         // evaluation cache per source block: target root, merge count when evaluated, value
         std::vector<int32_t> c_to(B, -1);
-        std::vector<int64_t> c_time(B, -1), stamp(B, 0);
+        std::vector<int64_t> c_time(B, -1), stamp(B, 0), mark(B + 1, 0);  // mark[B]: spare entry
         std::vector<double> c_val(B, 0.0);
         // nonparametric: the cache holds the Delta entries' sum; the rest depends on the non-empty block
         // count, which every merge changes, and is recomputed at use
@@ -634,20 +657,17 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
                     c_val[b] = best_dS[b];
                 }
         int64_t now = 0;
-        // newest stamp among the keys of f (no branch per slot)
-        auto newest = [&](const FlatMap &f) {
-            int64_t mx = 0;
-            for (uint32_t i = 0; i < f.cap; ++i) {
-                const int32_t k = f.s[i].key;
-                const int64_t t = stamp[k < 0 ? 0 : k];
-                mx = (k >= 0 && t > mx) ? t : mx;
-            }
-            return mx;
-        };
         auto valid = [&](int32_t r, int32_t t) {
             const int64_t T = c_time[r];
-            if (c_to[r] != t || T < stamp[r] || T < stamp[t]) return false;
-            return newest(m->row[r]) <= T && newest(m->col[r]) <= T;
+            return c_to[r] == t && T >= stamp[r] && T >= stamp[t] && T >= mark[r];
+        };
+        // marks the keys of f (no branch per slot: empty slots write the spare entry)
+        int64_t *mk = mark.data();
+        auto mark_keys = [&](const FlatMap &f, int64_t when) {
+            for (uint32_t i = 0; i < f.cap; ++i) {
+                const int32_t k = f.s[i].key;
+                mk[k < 0 ? B : k] = when;
+            }
         };
         auto root_ro = [&](int32_t x) {
             while (parent[x] != x) x = parent[x];
@@ -726,6 +746,8 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
             stamp[from] = stamp[to] = now;
             parent[from] = to;
             merge_apply(*m, from, to);
+            mark_keys(m->row[to], now);
+            mark_keys(m->col[to], now);
             merged++;
         }
     }
