@@ -14,6 +14,7 @@
 // the merges in stable ascending order of delta-MDL without re-evaluation, following block_map chains.
 #pragma once
 #include <math.h>
+#include <stdio.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -410,10 +411,28 @@ inline size_t gather(const FlatMap &f, std::vector<FlatMap::Slot> &out) {
     return k;
 }
 
+// Entries of row r + column r at or above which one evaluation, and renames at or above which one merge,
+// is split over the pool, and the entries per piece.  SBP_AMD_CARRY_SPLIT="eval_min,eval_piece,merge_min,
+// merge_piece" overrides them (all >= 1).
+struct SplitSizes {
+    size_t eval_min = 512, eval_piece = 128, merge_min = 128, merge_piece = 64;
+    static SplitSizes from_env() {
+        SplitSizes s;
+        if (const char *e = getenv("SBP_AMD_CARRY_SPLIT")) {
+            unsigned long a, b, c, d;
+            if (sscanf(e, "%lu,%lu,%lu,%lu", &a, &b, &c, &d) == 4 && a && b && c && d) s = SplitSizes{a, b, c, d};
+        }
+        return s;
+    }
+};
+
 // entropy::block_merge_delta_mdl (src/entropy.cpp:129-182) for merging r into s on the current state:
 // Delta entries term by term, (s, s) when unchanged, the other unchanged cells of row s / column s in bulk
-// (M * (fastlog(d') - fastlog(d)) each) -- the same terms as the merge-proposal kernel.
-inline double merge_dS(const Model &m, int32_t r, int32_t s) {
+// (M * (fastlog(d') - fastlog(d)) each) -- the same terms as the merge-proposal kernel.  With a pool, the
+// entries of a long row r / column r are split into pieces whose fixed-point partial sums add exactly, so
+// the result does not depend on the split.  Only the thread that owns `pool` may pass it.
+inline double merge_dS(const Model &m, int32_t r, int32_t s, Pool *pool = nullptr,
+                       const SplitSizes &split = SplitSizes()) {
     const float *L = logf_table().v.data();
     auto lg = [L](int64_t x) -> double {
         if (x < LogfTable::SIZE) return x <= 0 ? 0.0 : (double)L[x];
@@ -423,44 +442,76 @@ inline double merge_dS(const Model &m, int32_t r, int32_t s) {
     const int64_t din_s_new = m.d_in[s] + k_in, dout_s_new = m.d_out[s] + k_out;
     const double l_dout_r = lg(m.d_out[r]), l_dout_s = lg(m.d_out[s]), l_dout_sn = lg(dout_s_new);
     const double l_din_r = lg(m.d_in[r]), l_din_s = lg(m.d_in[s]), l_din_sn = lg(din_s_new);
+    const FlatMap &Rs = m.row[s], &Cs = m.col[s];
+    thread_local std::vector<FlatMap::Slot> row_buf, col_buf;
+    const size_t nr = gather(m.row[r], row_buf), nc = gather(m.col[r], col_buf);
+    const FlatMap::Slot *er = row_buf.data(), *ec = col_buf.data();  // the caller's buffers, also in workers
+    auto row_part = [&](size_t lo, size_t hi, Fx128 &acc, int64_t &sub) {
+        for (size_t i = lo; i < hi; ++i) {
+            if (i + 8 < hi) {
+                __builtin_prefetch(&m.d_in[er[i + 8].key]);
+                __builtin_prefetch(&Rs.s[Rs.slot(er[i + 8].key)]);
+            }
+            const int32_t c = er[i].key;
+            const int64_t v = er[i].val;
+            const double ld = lg(m.d_in[c]);
+            acc.add((double)v * (lg(v) - (ld + l_dout_r)));
+            if (c != r && c != s) {
+                const int64_t msc = Rs.get(c);
+                if (msc) acc.add((double)msc * (lg(msc) - (ld + l_dout_s)));
+                acc.sub((double)(msc + v) * (lg(msc + v) - (ld + l_dout_sn)));
+                sub += msc;
+            }
+        }
+    };
+    auto col_part = [&](size_t lo, size_t hi, Fx128 &acc, int64_t &sub) {
+        for (size_t i = lo; i < hi; ++i) {
+            if (i + 8 < hi) {
+                __builtin_prefetch(&m.d_out[ec[i + 8].key]);
+                __builtin_prefetch(&Cs.s[Cs.slot(ec[i + 8].key)]);
+            }
+            const int32_t x = ec[i].key;
+            if (x == r) continue;
+            const int64_t v = ec[i].val;
+            const double ld = lg(m.d_out[x]);
+            acc.add((double)v * (lg(v) - (l_din_r + ld)));
+            if (x != s) {
+                const int64_t mxs = Cs.get(x);
+                if (mxs) acc.add((double)mxs * (lg(mxs) - (l_din_s + ld)));
+                acc.sub((double)(mxs + v) * (lg(mxs + v) - (l_din_sn + ld)));
+                sub += mxs;
+            }
+        }
+    };
     Fx128 acc;
     int64_t sub_row = 0, sub_col = 0;
-    const FlatMap &Rs = m.row[s], &Cs = m.col[s];
-    thread_local std::vector<FlatMap::Slot> ent;
-    size_t n = gather(m.row[r], ent);
-    for (size_t i = 0; i < n; ++i) {
-        if (i + 8 < n) {
-            __builtin_prefetch(&m.d_in[ent[i + 8].key]);
-            __builtin_prefetch(&Rs.s[Rs.slot(ent[i + 8].key)]);
+    if (pool && pool->threads() > 1 && nr + nc >= split.eval_min) {
+        const size_t P = split.eval_piece;
+        struct alignas(64) Part {
+            Fx128 acc;
+            int64_t sub;
+        };
+        const size_t kr = (nr + P - 1) / P, kc = (nc + P - 1) / P;
+        thread_local std::vector<Part> part_buf;
+        if (part_buf.size() < kr + kc) part_buf.resize(kr + kc);
+        Part *parts = part_buf.data();
+        pool->run((int64_t)(kr + kc), [&](int64_t i) {
+            Part &p = parts[i];
+            p.acc = Fx128();
+            p.sub = 0;
+            if ((size_t)i < kr) row_part(i * P, std::min(nr, (i + 1) * P), p.acc, p.sub);
+            else {
+                const size_t j = i - kr;
+                col_part(j * P, std::min(nc, (j + 1) * P), p.acc, p.sub);
+            }
+        });
+        for (size_t i = 0; i < kr + kc; ++i) {
+            acc.v += parts[i].acc.v;
+            (i < kr ? sub_row : sub_col) += parts[i].sub;
         }
-        const int32_t c = ent[i].key;
-        const int64_t v = ent[i].val;
-        const double ld = lg(m.d_in[c]);
-        acc.add((double)v * (lg(v) - (ld + l_dout_r)));
-        if (c != r && c != s) {
-            const int64_t msc = Rs.get(c);
-            if (msc) acc.add((double)msc * (lg(msc) - (ld + l_dout_s)));
-            acc.sub((double)(msc + v) * (lg(msc + v) - (ld + l_dout_sn)));
-            sub_row += msc;
-        }
-    }
-    n = gather(m.col[r], ent);
-    for (size_t i = 0; i < n; ++i) {
-        if (i + 8 < n) {
-            __builtin_prefetch(&m.d_out[ent[i + 8].key]);
-            __builtin_prefetch(&Cs.s[Cs.slot(ent[i + 8].key)]);
-        }
-        const int32_t x = ent[i].key;
-        if (x == r) continue;
-        const int64_t v = ent[i].val;
-        const double ld = lg(m.d_out[x]);
-        acc.add((double)v * (lg(v) - (l_din_r + ld)));
-        if (x != s) {
-            const int64_t mxs = Cs.get(x);
-            if (mxs) acc.add((double)mxs * (lg(mxs) - (l_din_s + ld)));
-            acc.sub((double)(mxs + v) * (lg(mxs + v) - (l_din_sn + ld)));
-            sub_col += mxs;
-        }
+    } else {
+        row_part(0, nr, acc, sub_row);
+        col_part(0, nc, acc, sub_col);
     }
     const int64_t mrr = m.get(r, r), mrs = m.get(r, s), msr = m.get(s, r), mss = m.get(s, s);
     const int64_t dss = mrr + mrs + msr;
@@ -525,41 +576,72 @@ inline void rename_in(std::vector<FlatMap> &maps, const FlatMap::Slot *ent, size
 }
 
 // Blockmodel::merge_block (src/blockmodel/blockmodel.cpp:337-363): row r and column r move into s.
-inline void merge_apply(Model &m, int32_t r, int32_t s) {
+// (r, c) -> (s, c') and (x, r) -> (x', s), with c' = s for c in {r, s} and x' = s for x = s.  Row s and
+// column s take all of them on one thread; the renames r -> s in the columns of the other c and the rows of
+// the other x touch distinct maps, so with a pool a long list of them is split into pieces.  Only the
+// thread that owns `pool` may pass it.
+inline void merge_apply(Model &m, int32_t r, int32_t s, Pool *pool = nullptr,
+                        const SplitSizes &split = SplitSizes()) {
     FlatMap &Rr = m.row[r], &Cr = m.col[r], &Rs = m.row[s], &Cs = m.col[s];
-    thread_local std::vector<FlatMap::Slot> ent;
-    // (r, c) -> (s, c'), c' = s for c in {r, s}.  Row r is only read here; column r loses key r (self cell).
-    ent.clear();
-    for (uint32_t i = 0; i < Rr.cap; ++i) {
-        const int32_t c = Rr.s[i].key;
-        if (c < 0) continue;
-        const int64_t v = Rr.s[i].val;
-        if (c == r || c == s) {
-            Rs.add(s, v);
-            m.col[c].erase(r);
-            Cs.add(s, v);
-        } else {
-            Rs.add(c, v);
-            ent.push_back(Rr.s[i]);
-        }
+    thread_local std::vector<FlatMap::Slot> row_buf, col_buf, row_ren, col_ren;
+    const size_t nr = gather(Rr, row_buf), nc = gather(Cr, col_buf);
+    if (row_ren.size() < nr) row_ren.resize(nr);
+    if (col_ren.size() < nc) col_ren.resize(nc);
+    const FlatMap::Slot *er = row_buf.data(), *ec = col_buf.data();
+    FlatMap::Slot *fr = row_ren.data(), *fc = col_ren.data();  // the caller's buffers, also in workers
+    size_t kr = 0, kc = 0;
+    for (size_t i = 0; i < nr; ++i) {
+        fr[kr] = er[i];
+        kr += er[i].key != r && er[i].key != s;
     }
-    rename_in(m.col, ent.data(), ent.size(), r, s);
-    // (x, r) -> (x', s), x' = s for x = s
-    ent.clear();
-    for (uint32_t i = 0; i < Cr.cap; ++i) {
-        const int32_t x = Cr.s[i].key;
-        if (x < 0 || x == r) continue;
-        const int64_t v = Cr.s[i].val;
-        if (x == s) {
-            Cs.add(s, v);
-            Rs.erase(r);
-            Rs.add(s, v);
-        } else {
-            Cs.add(x, v);
-            ent.push_back(Cr.s[i]);
-        }
+    for (size_t i = 0; i < nc; ++i) {
+        fc[kc] = ec[i];
+        kc += ec[i].key != r && ec[i].key != s;
     }
-    rename_in(m.row, ent.data(), ent.size(), r, s);
+    auto own = [&] {
+        for (size_t i = 0; i < nr; ++i) {
+            const int32_t c = er[i].key;
+            const int64_t v = er[i].val;
+            if (c == r || c == s) {
+                Rs.add(s, v);
+                m.col[c].erase(r);
+                Cs.add(s, v);
+            } else {
+                Rs.add(c, v);
+            }
+        }
+        for (size_t i = 0; i < nc; ++i) {
+            const int32_t x = ec[i].key;
+            const int64_t v = ec[i].val;
+            if (x == r) continue;
+            if (x == s) {
+                Cs.add(s, v);
+                Rs.erase(r);
+                Rs.add(s, v);
+            } else {
+                Cs.add(x, v);
+            }
+        }
+    };
+    if (pool && pool->threads() > 1 && kr + kc >= split.merge_min) {
+        const size_t P = split.merge_piece;
+        const size_t pr = (kr + P - 1) / P, pc = (kc + P - 1) / P;
+        pool->run((int64_t)(1 + pr + pc), [&](int64_t i) {
+            if (i == 0) {
+                own();
+            } else if ((size_t)i <= pr) {
+                const size_t lo = (i - 1) * P;
+                rename_in(m.col, fr + lo, std::min(kr - lo, P), r, s);
+            } else {
+                const size_t lo = (i - 1 - pr) * P;
+                rename_in(m.row, fc + lo, std::min(kc - lo, P), r, s);
+            }
+        });
+    } else {
+        own();
+        rename_in(m.col, fr, kr, r, s);
+        rename_in(m.row, fc, kc, r, s);
+    }
     Rr.clear();
     Cr.clear();
     m.d_out[s] += m.d_out[r];
@@ -602,7 +684,8 @@ struct CarryStats {
 //     to the fraction of those evaluations that a merge inside the batch invalidates.
 inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const double *best_dS, const int32_t *sizes,
                          int64_t num_to_merge, bool approximate, std::vector<int32_t> &block_map,
-                         CarryStats *cs = nullptr, Pool *pool = nullptr, bool seed_from_proposals = true) {
+                         CarryStats *cs = nullptr, Pool *pool = nullptr, bool seed_from_proposals = true,
+                         const SplitSizes &split = SplitSizes::from_env()) {
     std::vector<int32_t> parent(B);
     for (int32_t b = 0; b < B; ++b) parent[b] = b;
     auto find = [&](int32_t x) {
@@ -647,7 +730,9 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
         // nonparametric: the cache holds the Delta entries' sum; the rest depends on the non-empty block
         // count, which every merge changes, and is recomputed at use
         const bool np = m->np;
-        auto evaluate = [&](int32_t r, int32_t t) { return np ? np_merge_entries(*m, r, t) : merge_dS(*m, r, t); };
+        auto evaluate = [&](int32_t r, int32_t t, Pool *p) {
+            return np ? np_merge_entries(*m, r, t) : merge_dS(*m, r, t, p, split);
+        };
         auto finalize = [&](int32_t r, int32_t t, double v) { return np ? np_merge_dS(*m, r, t, v) : v; };
         if (seed_from_proposals && !np)
             for (int32_t b = 0; b < B; ++b)
@@ -697,7 +782,7 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
                         const Item &it = batch[i];
                         const int32_t t = root_ro(it.s);
                         if (t == it.r || valid(it.r, t)) return;
-                        c_val[it.r] = evaluate(it.r, t);
+                        c_val[it.r] = evaluate(it.r, t, nullptr);
                         c_to[it.r] = t;
                         c_time[it.r] = now;
                         spec_count.fetch_add(1, std::memory_order_relaxed);
@@ -724,7 +809,7 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
                 st.reused++;
             } else {
                 if (c_to[from] == to && c_time[from] == spec_time) batch_lost++;  // speculated, then invalidated
-                const double v = evaluate(from, to);
+                const double v = evaluate(from, to, pool);
                 c_to[from] = to;
                 c_time[from] = now;
                 c_val[from] = v;
@@ -745,7 +830,7 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
             ++now;
             stamp[from] = stamp[to] = now;
             parent[from] = to;
-            merge_apply(*m, from, to);
+            merge_apply(*m, from, to, pool, split);
             mark_keys(m->row[to], now);
             mark_keys(m->col[to], now);
             merged++;

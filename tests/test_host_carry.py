@@ -103,3 +103,23 @@ def test_host_carry_nonparametric_matches_oracle(N, B):
         out = bm.copy().merge_apply(best_s, best_dS, B // 2, False)
         assert nb == out.B
         assert np.array_equal(bmap[assign], out.export()["assignment"])
+
+
+@pytest.mark.parametrize("split", ["1,1,1,1", "2,3,1,2", "64,16,32,8"])
+def test_host_carry_split_rows_are_exact(split, monkeypatch):
+    """One evaluation or one merge split over the pool (fixed-point partial sums; renames in distinct maps)
+    takes the same decisions as one thread, down to one-entry pieces."""
+    g = dcsbm(4000, 30, gamma=2.1, kmin=1, kmax=800, mean_degree=24, p_in=0.6, sigma=0.5, seed=9)
+    og = O.OGraph.from_csr(g)
+    B = 600
+    rng = np.random.default_rng(3)
+    assign = np.concatenate([np.arange(B), rng.integers(0, B, g.N - B)]).astype(np.int32)
+    bm = O.OBlockmodel.build(og, B, assign)
+    best_s, best_dS = bm.merge_propose(seed=4, epoch=1)
+    ref = host_carry(bm, best_s, best_dS, B // 2, False, threads=1)
+    monkeypatch.setenv("SBP_AMD_CARRY_SPLIT", split)
+    for seed in (False, True):
+        nb, bmap, st = host_carry(bm, best_s, best_dS, B // 2, False, threads=8, seed=seed)
+        assert nb == ref[0] and np.array_equal(bmap, ref[1]), seed
+    out = bm.copy().merge_apply(best_s, best_dS, B // 2, False)
+    assert np.array_equal(ref[1][assign], out.export()["assignment"])
