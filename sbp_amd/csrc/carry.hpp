@@ -132,11 +132,39 @@ struct FlatMap {
         for (uint32_t i = 0; i < cap; ++i)
             if (s[i].key != -1) f(s[i].key, (int64_t)s[i].val);
     }
-    void clear() {
-        free(s);
+    void clear() { free(detach()); }
+    // empties the map and hands over its slot array
+    Slot *detach() {
+        Slot *p = s;
         s = nullptr;
         cap = n = 0;
         shift = 32;
+        return p;
+    }
+    // room for `expected` keys without growing
+    void reserve(uint32_t expected) {
+        if (2 * expected + 2 <= cap) return;
+        Slot *old = s;
+        const uint32_t oc = cap;
+        s = nullptr;
+        init(expected);
+        for (uint32_t i = 0; i < oc; ++i)
+            if (old[i].key != -1) add(old[i].key, old[i].val);
+        free(old);
+    }
+};
+
+// Slot arrays of merged-away blocks, freed with the model instead of inside the merge loop.
+struct Retired {
+    std::vector<void *> p;
+    Retired() = default;
+    Retired(Retired &&) = default;
+    Retired &operator=(Retired &&o) {
+        std::swap(p, o.p);
+        return *this;
+    }
+    ~Retired() {
+        for (void *x : p) free(x);
     }
 };
 
@@ -262,7 +290,7 @@ inline double np_merge_finish(double entries, int64_t dout_r, int64_t din_r, int
 // later generation's indices with an earlier task.
 class Pool {
   public:
-    explicit Pool(int threads) : nthreads_(std::max(1, threads)) {
+    explicit Pool(int threads, int spin_us = spin_default()) : nthreads_(std::max(1, threads)), spin_us_(spin_us) {
         for (int t = 1; t < nthreads_; ++t) workers_.emplace_back([this] { work(); });
     }
     ~Pool() {
@@ -275,6 +303,14 @@ class Pool {
         for (auto &w : workers_) w.join();
     }
     int threads() const { return nthreads_; }
+    // SBP_AMD_CARRY_SPIN_US (0..100000) when set, else 1000
+    static int spin_default() {
+        if (const char *e = getenv("SBP_AMD_CARRY_SPIN_US")) {
+            const int v = atoi(e);
+            if (v >= 0 && v <= 100000) return v;
+        }
+        return 1000;
+    }
     template <typename F>
     void run(int64_t n, F fn) {
         if (n <= 0) return;
@@ -315,7 +351,7 @@ class Pool {
             auto t0 = std::chrono::steady_clock::now();
             for (uint32_t spins = 0; (g = gen_.load(std::memory_order_acquire)) == seen; ++spins) {
                 relax();
-                if ((spins & 255) != 255 || std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(1))
+                if ((spins & 255) != 255 || std::chrono::steady_clock::now() - t0 < std::chrono::microseconds(spin_us_))
                     continue;
                 std::unique_lock<std::mutex> lk(mu_);
                 sleepers_.fetch_add(1);
@@ -329,7 +365,7 @@ class Pool {
             pending_.fetch_sub(1, std::memory_order_acq_rel);
         }
     }
-    int nthreads_;
+    int nthreads_, spin_us_;
     std::vector<std::thread> workers_;
     std::mutex mu_;
     std::condition_variable cv_;
@@ -354,6 +390,7 @@ inline int carry_threads() {
 struct Model {
     int32_t B = 0;
     std::vector<FlatMap> row, col;
+    Retired retired;
     std::vector<int64_t> d_out, d_in;
     // nonparametric entropy model: block sizes, vertices, edges and the non-empty block count
     bool np = false;
@@ -598,8 +635,14 @@ inline void merge_apply(Model &m, int32_t r, int32_t s, Pool *pool = nullptr,
         fc[kc] = ec[i];
         kc += ec[i].key != r && ec[i].key != s;
     }
+    // start fetching the first neighbour maps to be renamed; row s / column s are updated meanwhile
+    for (size_t i = 0; i < std::min<size_t>(kr, 16); ++i) __builtin_prefetch(&m.col[fr[i].key]);
+    for (size_t i = 0; i < std::min<size_t>(kc, 16); ++i) __builtin_prefetch(&m.row[fc[i].key]);
     auto own = [&] {
+        Rs.reserve(Rs.n + (uint32_t)nr);
+        Cs.reserve(Cs.n + (uint32_t)nc);
         for (size_t i = 0; i < nr; ++i) {
+            if (i + 8 < nr) __builtin_prefetch(&Rs.s[Rs.slot(er[i + 8].key)]);
             const int32_t c = er[i].key;
             const int64_t v = er[i].val;
             if (c == r || c == s) {
@@ -611,6 +654,7 @@ inline void merge_apply(Model &m, int32_t r, int32_t s, Pool *pool = nullptr,
             }
         }
         for (size_t i = 0; i < nc; ++i) {
+            if (i + 8 < nc) __builtin_prefetch(&Cs.s[Cs.slot(ec[i + 8].key)]);
             const int32_t x = ec[i].key;
             const int64_t v = ec[i].val;
             if (x == r) continue;
@@ -642,8 +686,8 @@ inline void merge_apply(Model &m, int32_t r, int32_t s, Pool *pool = nullptr,
         rename_in(m.col, fr, kr, r, s);
         rename_in(m.row, fc, kc, r, s);
     }
-    Rr.clear();
-    Cr.clear();
+    m.retired.p.push_back(Rr.detach());
+    m.retired.p.push_back(Cr.detach());
     m.d_out[s] += m.d_out[r];
     m.d_in[s] += m.d_in[r];
     m.d_out[r] = 0;
@@ -798,6 +842,16 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
                 heap.pop();
             } else {
                 it = batch[bi++];
+                if (m) {  // the rows and columns of the next items are read when they are merged
+                    if (bi + 4 < batch.size()) {
+                        __builtin_prefetch(&m->row[batch[bi + 4].r]);
+                        __builtin_prefetch(&m->col[batch[bi + 4].r]);
+                    }
+                    if (bi + 2 < batch.size()) {
+                        __builtin_prefetch(m->row[batch[bi + 2].r].s);
+                        __builtin_prefetch(m->col[batch[bi + 2].r].s);
+                    }
+                }
             }
             st.pops++;
             const int32_t from = it.r;
