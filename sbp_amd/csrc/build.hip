@@ -75,17 +75,24 @@ static int grid_for(int64_t n, int block = 256) {
     return (int)g;
 }
 
+// Blockmodel buffers come from the context's pool (recycled across the blockmodels of a run).
+static DevicePool &pool_of(sbp_bm bm) {
+    if (!bm->pool) bm->pool = bm->ctx->pool;
+    return *bm->pool;
+}
+
 void bm_alloc(sbp_bm bm, int32_t B, int32_t N) {
+    DevicePool &pool = pool_of(bm);
     if (bm->N != N || bm->assign == nullptr) {
-        if (bm->assign) SBP_HIP(hipFree(bm->assign));
-        SBP_HIP(hipMalloc(&bm->assign, sizeof(int32_t) * (size_t)(N > 0 ? N : 1)));
+        pool.release(bm->assign);
+        bm->assign = static_cast<int32_t *>(pool.alloc(sizeof(int32_t) * (size_t)(N > 0 ? N : 1)));
         bm->N = N;
     }
     if (B + 1 > bm->bcap) {
         int32_t **arrs[] = {&bm->sizes, &bm->d_out, &bm->d_in, &bm->d_tot, &bm->row_ptr, &bm->col_ptr};
         for (auto a : arrs) {
-            if (*a) SBP_HIP(hipFree(*a));
-            SBP_HIP(hipMalloc(a, sizeof(int32_t) * (size_t)(B + 2)));
+            pool.release(*a);
+            *a = static_cast<int32_t *>(pool.alloc(sizeof(int32_t) * (size_t)(B + 2)));
         }
         bm->bcap = B + 1;
     }
@@ -95,8 +102,12 @@ void bm_alloc(sbp_bm bm, int32_t B, int32_t N) {
 void bm_release(sbp_bm bm) {
     void *ptrs[] = {bm->assign,    bm->sizes,     bm->d_out,   bm->d_in,    bm->d_tot,    bm->row_ptr,
                     bm->col_ptr,   bm->row_cells, bm->col_cells, bm->row_cum, bm->col_cum, bm->cell_hash};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
+    if (bm->pool) {
+        for (void *p : ptrs) bm->pool->release(p);
+    } else {
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+    }
     bm->cell_hash = nullptr;
     bm->hash_valid = false;
     bm->assign = bm->sizes = bm->d_out = bm->d_in = bm->d_tot = bm->row_ptr = bm->col_ptr = nullptr;
@@ -108,14 +119,14 @@ void bm_release(sbp_bm bm) {
 
 void bm_ensure_cells(sbp_bm bm, int64_t nnz) {
     if (nnz > bm->cap) {
+        DevicePool &pool = pool_of(bm);
         void *old[] = {bm->row_cells, bm->col_cells, bm->row_cum, bm->col_cum};
-        for (void *p : old)
-            if (p) SBP_HIP(hipFree(p));
+        for (void *p : old) pool.release(p);
         int64_t cap = nnz + nnz / 8 + 64;
-        SBP_HIP(hipMalloc(&bm->row_cells, sizeof(int2) * (size_t)cap));
-        SBP_HIP(hipMalloc(&bm->col_cells, sizeof(int2) * (size_t)cap));
-        SBP_HIP(hipMalloc(&bm->row_cum, sizeof(int32_t) * (size_t)cap));
-        SBP_HIP(hipMalloc(&bm->col_cum, sizeof(int32_t) * (size_t)cap));
+        bm->row_cells = static_cast<int2 *>(pool.alloc(sizeof(int2) * (size_t)cap));
+        bm->col_cells = static_cast<int2 *>(pool.alloc(sizeof(int2) * (size_t)cap));
+        bm->row_cum = static_cast<int32_t *>(pool.alloc(sizeof(int32_t) * (size_t)cap));
+        bm->col_cum = static_cast<int32_t *>(pool.alloc(sizeof(int32_t) * (size_t)cap));
         bm->cap = cap;
     }
 }
@@ -176,12 +187,9 @@ void bm_ensure_hash(sbp_ctx ctx, sbp_bm bm) {
     int lg = 10;
     while ((int64_t(1) << lg) < 2 * bm->nnz + 2) ++lg;
     if (!bm->cell_hash || bm->hash_log2 < lg) {
-        if (bm->cell_hash) {
-            SBP_HIP(hipStreamSynchronize(st));
-            SBP_HIP(hipFree(bm->cell_hash));
-        }
-        bm->cell_hash = nullptr;
-        SBP_HIP(hipMalloc(&bm->cell_hash, sizeof(sbpdev::CellSlot) << lg));
+        DevicePool &pool = pool_of(bm);
+        pool.release(bm->cell_hash);
+        bm->cell_hash = static_cast<sbpdev::CellSlot *>(pool.alloc(sizeof(sbpdev::CellSlot) << lg));
         bm->hash_log2 = lg;
     }
     lg = bm->hash_log2;
@@ -230,8 +238,7 @@ void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm) {
     SBP_HIP(hipcub::DeviceRadixSort::SortKeys(tmp_ptr, tmp_a, keys, sorted, (int)E, 0, end_bit, st));
     SBP_HIP(hipcub::DeviceRunLengthEncode::Encode(tmp_ptr, tmp_b, sorted, uniq, counts, num_runs, (int)E, st));
     int64_t nnz = 0;
-    SBP_HIP(hipMemcpyAsync(&nnz, num_runs, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    SBP_HIP(hipStreamSynchronize(st));
+    read_small(ctx, {{&nnz, num_runs, sizeof(int64_t)}});
     ensure_cells(bm, nnz);
     bm->nnz = nnz;
     k_fill_major<<<grid_for(nnz), 256, 0, st>>>(nnz, B, cbits, uniq, counts, bm->row_ptr, bm->row_cells, bm->d_out,

@@ -54,6 +54,7 @@ static sbp_params default_params() {
 static sbp_bm new_bm(sbp_ctx ctx) {
     sbp_bm bm = new sbp_bm_s();
     bm->ctx = ctx;
+    bm->pool = ctx->pool;
     return bm;
 }
 
@@ -357,6 +358,7 @@ int sbp_create(sbp_ctx *out, int device, uint64_t seed) {
         ctx->seed = seed;
         SBP_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         for (auto &e : ctx->ev) SBP_HIP(hipEventCreate(&e));
+        SBP_HIP(hipHostMalloc(&ctx->pinned, PINNED_BYTES, hipHostMallocDefault));
         *out = ctx;
         return SBP_OK;
     } catch (const sbpimpl::Error &e) {
@@ -377,6 +379,8 @@ int sbp_destroy(sbp_ctx ctx) {
         s.bytes = 0;
     }
     if (ctx->lgamma) (void)hipFree(ctx->lgamma);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    ctx->pool.reset();  // freed here unless blockmodels of this context still hold buffers
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SBP_OK;
@@ -504,7 +508,8 @@ int sbp_bm_build(sbp_ctx ctx, sbp_graph g, int32_t B, const int32_t *assignment,
     sbp_bm bm = new_bm(ctx);
     try {
         bm_alloc(bm, B, g->N);
-        SBP_HIP(hipMemcpy(bm->assign, assignment, sizeof(int32_t) * g->N, hipMemcpyHostToDevice));
+        // on the context's stream: the buffer may be recycled from a blockmodel that queued work still reads
+        SBP_HIP(hipMemcpyAsync(bm->assign, assignment, sizeof(int32_t) * g->N, hipMemcpyHostToDevice, ctx->stream));
         bm_rebuild(ctx, g, bm);
         SBP_HIP(hipStreamSynchronize(ctx->stream));
         bm->num_to_merge = (int64_t)((double)B * 0.5);

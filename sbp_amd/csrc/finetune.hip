@@ -637,8 +637,7 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     }
     SBP_HIP(hipEventRecord(ctx->ev[1], st));
     unsigned long long hb[4] = {0, 0, 0, 0};
-    SBP_HIP(hipMemcpyAsync(hb, counters, sizeof(hb), hipMemcpyDeviceToHost, st));
-    SBP_HIP(hipStreamSynchronize(st));
+    read_small(ctx, {{hb, counters, sizeof(hb)}});
     SBP_CHECK(hb[1] == 0, SBP_E_STATE, "finetune proposal: block degrees inconsistent with the block matrix");
     info.bytes = hb[0];
     info.high_bytes = hb[3];
@@ -750,8 +749,7 @@ int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_
     k_count_out<<<grid_of(N), 256, 0, st>>>(N, d_move_s, bm->assign, out_count);
     k_mark_risk<<<grid_of(B), 256, 0, st>>>(B, out_count, bm->sizes, risk, flags, counters + 2);
     int32_t any = 0;
-    SBP_HIP(hipMemcpyAsync(&any, flags, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    SBP_HIP(hipStreamSynchronize(st));
+    read_small(ctx, {{&any, flags, sizeof(int32_t)}});
     const int32_t *acc_final = nullptr;
     if (any) {
         auto *events = ctx->scratch[S_EVENTS_A].as<unsigned long long>(2 * (size_t)N + 2);
@@ -762,8 +760,7 @@ int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_
         SBP_HIP(hipMemsetAsync(seg, 0, sizeof(int32_t) * 2 * (size_t)B, st));
         k_make_events<<<grid_of(N), 256, 0, st>>>(N, d_move_s, bm->assign, risk, events, counters + 1, acc, acc + N);
         unsigned long long n_ev = 0;
-        SBP_HIP(hipMemcpyAsync(&n_ev, counters + 1, sizeof(n_ev), hipMemcpyDeviceToHost, st));
-        SBP_HIP(hipStreamSynchronize(st));
+        read_small(ctx, {{&n_ev, counters + 1, sizeof(n_ev)}});
         size_t tmp = 0;
         SBP_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, events, events_sorted, (int)n_ev, 0, 64, st));
         void *tp = ctx->scratch[S_SORT_TMP].get(tmp);
@@ -777,8 +774,7 @@ int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_
             k_resolve<<<grid_of(B), 256, 0, st>>>(B, risk, seg_start, seg_end, events_sorted, bm->assign, bm->sizes,
                                                    prev, next, flags + 1);
             int32_t changed = 0;
-            SBP_HIP(hipMemcpyAsync(&changed, flags + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-            SBP_HIP(hipStreamSynchronize(st));
+            read_small(ctx, {{&changed, flags + 1, sizeof(int32_t)}});
             std::swap(prev, next);
             if (!changed) break;
         }
@@ -787,9 +783,7 @@ int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_
     k_apply<<<grid_of(N), 256, 0, st>>>(N, d_move_s, acc_final, bm->assign, counters);
     SBP_HIP(hipGetLastError());
     unsigned long long c[2] = {0, 0};
-    SBP_HIP(hipMemcpyAsync(&c[0], counters, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    SBP_HIP(hipMemcpyAsync(&c[1], counters + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    SBP_HIP(hipStreamSynchronize(st));
+    read_small(ctx, {{&c[0], counters, sizeof(unsigned long long)}, {&c[1], counters + 2, sizeof(unsigned long long)}});
     if (refused_out) *refused_out = (int64_t)c[1] - (int64_t)c[0];
     bm_rebuild(ctx, g, bm);
     return (int64_t)c[0];
@@ -1078,9 +1072,7 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         SBP_HIP(hipcub::DeviceScan::ExclusiveSum(tp, tmp, h.rcap, h.rstart, B + 1, st));
         SBP_HIP(hipcub::DeviceScan::ExclusiveSum(tp, tmp, h.ccap, h.cstart, B + 1, st));
         int32_t tot[2] = {0, 0};
-        SBP_HIP(hipMemcpyAsync(&tot[0], h.rstart + B, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        SBP_HIP(hipMemcpyAsync(&tot[1], h.cstart + B, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        SBP_HIP(hipStreamSynchronize(st));
+        read_small(ctx, {{&tot[0], h.rstart + B, sizeof(int32_t)}, {&tot[1], h.cstart + B, sizeof(int32_t)}});
         int2 *rc = ctx->scratch[S_KEYS_A].as<int2>((size_t)tot[0] + 1);
         int2 *cc = ctx->scratch[S_KEYS_B].as<int2>((size_t)tot[1] + 1);
         k_slack_copy<<<grid_of((int64_t)B * 64), 64, 0, st>>>(B, bm->row_ptr, bm->row_cells, h.rstart, rc);
@@ -1121,10 +1113,8 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         SBP_HIP(hipcub::DeviceScan::ExclusiveSum(tp, tmp2, h.rlen, rp, B + 1, st));
         SBP_HIP(hipcub::DeviceScan::ExclusiveSum(tp, tmp2, h.clen, cp, B + 1, st));
         int32_t nnz[2] = {0, 0}, hs[2] = {0, 0};
-        SBP_HIP(hipMemcpyAsync(&nnz[0], rp + B, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        SBP_HIP(hipMemcpyAsync(&nnz[1], cp + B, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        SBP_HIP(hipMemcpyAsync(hs, book, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        SBP_HIP(hipStreamSynchronize(st));
+        read_small(ctx, {{&nnz[0], rp + B, sizeof(int32_t)}, {&nnz[1], cp + B, sizeof(int32_t)},
+                         {hs, book, 2 * sizeof(int32_t)}});
         kernel_ms += (float)elapsed_ms(ctx, ctx->ev[4], ctx->ev[5]);
         SBP_CHECK(nnz[0] == nnz[1], SBP_E_STATE, "Metropolis-Hastings: row and column cell counts diverged");
         bm_ensure_cells(bm, nnz[0]);  // more cells than the compact arrays hold: grow them
@@ -1148,9 +1138,7 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         Fx sum;
     } res;
     unsigned long long hb[2] = {0, 0};
-    SBP_HIP(hipMemcpyAsync(&res, book + 4, sizeof(res), hipMemcpyDeviceToHost, st));
-    SBP_HIP(hipMemcpyAsync(hb, bytes, sizeof(hb), hipMemcpyDeviceToHost, st));
-    SBP_HIP(hipStreamSynchronize(st));
+    read_small(ctx, {{&res, book + 4, sizeof(res)}, {hb, bytes, sizeof(hb)}});
     SBP_CHECK(hb[1] == 0, SBP_E_STATE, "Metropolis-Hastings: block degrees inconsistent with the block matrix");
     if (sum_dS) *sum_dS = (double)res.sum.hi + (double)res.sum.lo * 0x1p-64;
     if (stats) {

@@ -2,9 +2,15 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
+#include <algorithm>
+#include <initializer_list>
+#include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/sbp_amd.h"
@@ -60,6 +66,58 @@ struct Scratch {
     }
 };
 
+// Recycles the device buffers of blockmodels.  Blockmodels are created, copied and dropped in every merge
+// phase; hipMalloc and hipFree cost tens of microseconds each and hipFree waits for the whole device.  A
+// released buffer is kept and handed out again for a request of at most its size and more than half of it.
+// Reuse is ordered by the context's stream, on which every write to a blockmodel buffer is issued.  The
+// context and each of its blockmodels share the pool, so it outlives the context while blockmodels remain.
+struct DevicePool {
+    static constexpr size_t MAX_CACHED = size_t(16) << 30;
+    std::multimap<size_t, void *> cached;
+    std::unordered_map<void *, size_t> size_of;
+    size_t cached_bytes = 0;
+    void *alloc(size_t bytes) {
+        bytes = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+        auto it = cached.lower_bound(bytes);
+        if (it != cached.end() && it->first < 2 * bytes) {
+            void *p = it->second;
+            cached_bytes -= it->first;
+            cached.erase(it);
+            return p;
+        }
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            trim();
+            SBP_HIP(hipMalloc(&p, bytes));
+        }
+        size_of[p] = bytes;
+        return p;
+    }
+    void release(void *p) {
+        if (!p) return;
+        auto it = size_of.find(p);
+        if (it == size_of.end()) {
+            (void)hipFree(p);
+            return;
+        }
+        cached.emplace(it->second, p);
+        cached_bytes += it->second;
+        if (cached_bytes > MAX_CACHED) trim();
+    }
+    void trim() {
+        if (cached.empty()) return;
+        (void)hipDeviceSynchronize();
+        for (auto &kv : cached) {
+            size_of.erase(kv.second);
+            (void)hipFree(kv.second);
+        }
+        cached.clear();
+        cached_bytes = 0;
+    }
+    ~DevicePool() { trim(); }
+};
+
 enum ScratchSlot {
     S_KEYS_A, S_KEYS_B, S_VALS_A, S_VALS_B, S_UNIQ, S_COUNTS, S_SORT_TMP, S_MISC, S_PARTIAL,
     S_MOVES, S_DS, S_ACCEPT, S_COUNTER, S_HASH, S_RISK, S_EVENTS_A, S_EVENTS_B, S_ORDER, S_BEST,
@@ -74,8 +132,9 @@ struct sbp_ctx_s {
     uint64_t seed = 0;
     std::string err;
     sbpimpl::Scratch scratch[sbpimpl::S_NUM_SLOTS];
+    std::shared_ptr<sbpimpl::DevicePool> pool = std::make_shared<sbpimpl::DevicePool>();
     hipEvent_t ev[8] = {};
-    void *pinned = nullptr;  // small pinned staging area for scalars
+    void *pinned = nullptr;  // pinned staging area for small reads (sbpimpl::PINNED_BYTES)
     double *lgamma = nullptr;  // fastlgamma table for the nonparametric entropy, grown on demand
     int64_t lgamma_n = 0;
     // the global-memory neighbour-block tables in scratch slot S_HASH that are known to be empty (every kernel
@@ -84,6 +143,35 @@ struct sbp_ctx_s {
     int hash_clean_lg = -1;
     int64_t hash_clean_tables = 0;
 };
+
+namespace sbpimpl {
+
+constexpr size_t PINNED_BYTES = 4096;
+struct ReadPiece {
+    void *dst;
+    const void *src;
+    size_t bytes;
+};
+// Small device-to-host reads (counters, flags, totals) through the context's pinned staging area: the pieces
+// are copied on the context's stream, then one wait.  A pageable destination would make each copy a staged,
+// blocking transfer of its own.
+inline void read_small(sbp_ctx ctx, std::initializer_list<ReadPiece> pieces) {
+    char *stage = static_cast<char *>(ctx->pinned);
+    size_t off = 0;
+    for (const ReadPiece &p : pieces) {
+        SBP_CHECK(off + p.bytes <= PINNED_BYTES, SBP_E_STATE, "read_small: staging area exceeded");
+        SBP_HIP(hipMemcpyAsync(stage + off, p.src, p.bytes, hipMemcpyDeviceToHost, ctx->stream));
+        off += (p.bytes + 15) & ~size_t(15);
+    }
+    SBP_HIP(hipStreamSynchronize(ctx->stream));
+    off = 0;
+    for (const ReadPiece &p : pieces) {
+        memcpy(p.dst, stage + off, p.bytes);
+        off += (p.bytes + 15) & ~size_t(15);
+    }
+}
+
+}  // namespace sbpimpl
 
 struct sbp_graph_s {
     sbp_ctx ctx = nullptr;
@@ -109,6 +197,7 @@ constexpr int FT_SMALL_DEGREE = 256;
 
 struct sbp_bm_s {
     sbp_ctx ctx = nullptr;
+    std::shared_ptr<sbpimpl::DevicePool> pool;  // the context's, holding every buffer below
     int32_t B = 0, N = 0;
     int64_t nnz = 0, cap = 0;
     int32_t *assign = nullptr;   // [N]

@@ -93,9 +93,7 @@ double log_posterior(sbp_ctx ctx, sbp_bm bm) {
         Fx total;
         int32_t err;
     } h;
-    SBP_HIP(hipMemcpyAsync(&h.total, partials + blocks, sizeof(Fx), hipMemcpyDeviceToHost, st));
-    SBP_HIP(hipMemcpyAsync(&h.err, err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    SBP_HIP(hipStreamSynchronize(st));
+    read_small(ctx, {{&h.total, partials + blocks, sizeof(Fx)}, {&h.err, err, sizeof(int32_t)}});
     // The reference exits when a non-zero cell has a zero degree (blockmodel.cpp:297-302).
     SBP_CHECK(h.err == 0, SBP_E_STATE, "log_posterior: non-zero cell with zero block degree");
     return (double)h.total.hi + (double)h.total.lo * 0x1p-64;
@@ -155,8 +153,7 @@ int64_t count_nonempty(sbp_ctx ctx, sbp_bm bm) {
     k_count_nonempty<<<std::max(blocks, 1), 256, 0, st>>>(bm->B, bm->sizes, d);
     SBP_HIP(hipGetLastError());
     int32_t h = 0;
-    SBP_HIP(hipMemcpyAsync(&h, d, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    SBP_HIP(hipStreamSynchronize(st));
+    read_small(ctx, {{&h, d, sizeof(int32_t)}});
     return h;
 }
 

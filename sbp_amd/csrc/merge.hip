@@ -242,8 +242,7 @@ uint64_t merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *d_best_s
     else k_merge_propose<false><<<(int)blocks, MG_BLOCK, 0, st>>>(a);
     SBP_HIP(hipGetLastError());
     unsigned long long hb[2] = {0, 0};
-    SBP_HIP(hipMemcpyAsync(hb, bytes, sizeof(hb), hipMemcpyDeviceToHost, st));
-    SBP_HIP(hipStreamSynchronize(st));
+    read_small(ctx, {{hb, bytes, sizeof(hb)}});
     SBP_CHECK(hb[1] == 0, SBP_E_STATE, "merge proposal: block degrees inconsistent with the block matrix");
     return hb[0];
 }
