@@ -29,15 +29,14 @@ __global__ void k_make_keys(int64_t E, const int32_t *__restrict__ src, const in
 }
 
 // One thread per unique cell: write cells, row pointers at key boundaries, and degree sums.
+// Cells and row pointers from sorted unique (major, minor) keys and their counts.  The degrees are the row
+// totals, taken afterwards by k_cum (same-address atomics here would serialize whole waves when B is small).
 __global__ void k_fill_major(int64_t nnz, int32_t B, int cbits, const unsigned long long *__restrict__ uniq,
-                             const int32_t *__restrict__ counts, int32_t *__restrict__ ptr, int2 *__restrict__ cells,
-                             int32_t *__restrict__ deg_major, int32_t *__restrict__ deg_minor) {
+                             const int32_t *__restrict__ counts, int32_t *__restrict__ ptr, int2 *__restrict__ cells) {
     const unsigned long long mask = (1ull << cbits) - 1;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
         int32_t a = (int32_t)(uniq[i] >> cbits), b = (int32_t)(uniq[i] & mask), cnt = counts[i];
         cells[i] = make_int2(b, cnt);
-        if (deg_major) atomicAdd(&deg_major[a], cnt);
-        if (deg_minor) atomicAdd(&deg_minor[b], cnt);
         int32_t prev = i == 0 ? -1 : (int32_t)(uniq[i - 1] >> cbits);
         for (int32_t x = prev + 1; x <= a; ++x) ptr[x] = (int32_t)i;
         if (i == nnz - 1)
@@ -66,6 +65,21 @@ __global__ void k_dtot_sizes(int32_t B, const int32_t *__restrict__ row_ptr, con
 __global__ void k_sizes(int32_t N, const int32_t *__restrict__ assign, int32_t *__restrict__ sizes) {
     for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
         atomicAdd(&sizes[assign[v]], 1);
+}
+
+// Block sizes through a histogram in LDS per workgroup (B <= SIZES_LDS_MAX): few blocks means heavy contention
+// on the global counters.
+constexpr int32_t SIZES_LDS_MAX = 16384;
+__global__ void __launch_bounds__(256) k_sizes_lds(int32_t N, int32_t B, const int32_t *__restrict__ assign,
+                                                   int32_t *__restrict__ sizes) {
+    __shared__ int32_t hist[SIZES_LDS_MAX];
+    for (int32_t b = threadIdx.x; b < B; b += blockDim.x) hist[b] = 0;
+    __syncthreads();
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+        atomicAdd(&hist[assign[v]], 1);
+    __syncthreads();
+    for (int32_t b = threadIdx.x; b < B; b += blockDim.x)
+        if (hist[b]) atomicAdd(&sizes[b], hist[b]);
 }
 
 static int grid_for(int64_t n, int block = 256) {
@@ -134,32 +148,48 @@ static void ensure_cells(sbp_bm bm, int64_t nnz) { bm_ensure_cells(bm, nnz); }
 
 // Inclusive cumulative counts per row (and per column without the diagonal cell): the multinomial over
 // neighbors_weights(t) then selects by binary search instead of a scan.  One wavefront per row.
+// Inclusive cumulative counts per row (the diagonal cell counted as 0 when skip_diag) and, when deg is given,
+// each row's total including the diagonal (0 for an empty row).
 __global__ void k_cum(int32_t B, const int32_t *__restrict__ ptr, const int2 *__restrict__ cells, int skip_diag,
-                      int32_t *__restrict__ cum) {
+                      int32_t *__restrict__ cum, int32_t *__restrict__ deg) {
     const int lane = sbpdev::lane_id();
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x / sbpdev::WAVE);
     for (int64_t b = (int64_t)blockIdx.x * (blockDim.x / sbpdev::WAVE) + threadIdx.x / sbpdev::WAVE; b < B;
          b += waves) {
-        int base = 0;
+        int base = 0, diag = 0;
         for (int c0 = ptr[b]; c0 < ptr[b + 1]; c0 += sbpdev::WAVE) {
             int i = c0 + lane;
             int x = 0;
-            if (i < ptr[b + 1]) x = (skip_diag && cells[i].x == (int)b) ? 0 : cells[i].y;
+            if (i < ptr[b + 1]) {
+                x = cells[i].y;
+                if (skip_diag && cells[i].x == (int)b) {
+                    diag += x;
+                    x = 0;
+                }
+            }
             int incl = sbpdev::wave_inclusive_scan(x) + base;
             if (i < ptr[b + 1]) cum[i] = incl;
             base = __shfl(incl, sbpdev::WAVE - 1, sbpdev::WAVE);
         }
+        if (deg) {
+            diag = sbpdev::wave_sum(diag);
+            if (lane == 0) deg[b] = base + diag;
+        }
     }
+}
+
+static void launch_cum(sbp_ctx ctx, sbp_bm bm, int32_t *d_out, int32_t *d_in) {
+    int64_t blocks = (bm->B + 3) / 4;
+    if (blocks > 16384) blocks = 16384;
+    if (blocks < 1) blocks = 1;
+    k_cum<<<(int)blocks, 256, 0, ctx->stream>>>(bm->B, bm->row_ptr, bm->row_cells, 0, bm->row_cum, d_out);
+    k_cum<<<(int)blocks, 256, 0, ctx->stream>>>(bm->B, bm->col_ptr, bm->col_cells, 1, bm->col_cum, d_in);
+    SBP_HIP(hipGetLastError());
 }
 
 void bm_compute_cum(sbp_ctx ctx, sbp_bm bm) {
     if (bm->nnz == 0) return;
-    int64_t blocks = (bm->B + 3) / 4;
-    if (blocks > 16384) blocks = 16384;
-    if (blocks < 1) blocks = 1;
-    k_cum<<<(int)blocks, 256, 0, ctx->stream>>>(bm->B, bm->row_ptr, bm->row_cells, 0, bm->row_cum);
-    k_cum<<<(int)blocks, 256, 0, ctx->stream>>>(bm->B, bm->col_ptr, bm->col_cells, 1, bm->col_cum);
-    SBP_HIP(hipGetLastError());
+    launch_cum(ctx, bm, nullptr, nullptr);
 }
 
 // Cell hash: one wavefront per row inserts its cells; the table holds at least twice nnz slots.
@@ -212,11 +242,12 @@ void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm) {
     const int cbits = bits_for(B);
     const int end_bit = 2 * cbits;
     SBP_CHECK(end_bit <= 64, SBP_E_ARG, "block count too large for packed keys");
-    SBP_HIP(hipMemsetAsync(bm->d_out, 0, sizeof(int32_t) * B, st));
-    SBP_HIP(hipMemsetAsync(bm->d_in, 0, sizeof(int32_t) * B, st));
     SBP_HIP(hipMemsetAsync(bm->sizes, 0, sizeof(int32_t) * B, st));
-    k_sizes<<<grid_for(bm->N), 256, 0, st>>>(bm->N, bm->assign, bm->sizes);
+    if (B <= SIZES_LDS_MAX) k_sizes_lds<<<std::min(grid_for(bm->N), 512), 256, 0, st>>>(bm->N, B, bm->assign, bm->sizes);
+    else k_sizes<<<grid_for(bm->N), 256, 0, st>>>(bm->N, bm->assign, bm->sizes);
     if (E == 0) {
+        SBP_HIP(hipMemsetAsync(bm->d_out, 0, sizeof(int32_t) * B, st));
+        SBP_HIP(hipMemsetAsync(bm->d_in, 0, sizeof(int32_t) * B, st));
         SBP_HIP(hipMemsetAsync(bm->row_ptr, 0, sizeof(int32_t) * (B + 1), st));
         SBP_HIP(hipMemsetAsync(bm->col_ptr, 0, sizeof(int32_t) * (B + 1), st));
         SBP_HIP(hipMemsetAsync(bm->d_tot, 0, sizeof(int32_t) * B, st));
@@ -241,8 +272,7 @@ void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm) {
     read_small(ctx, {{&nnz, num_runs, sizeof(int64_t)}});
     ensure_cells(bm, nnz);
     bm->nnz = nnz;
-    k_fill_major<<<grid_for(nnz), 256, 0, st>>>(nnz, B, cbits, uniq, counts, bm->row_ptr, bm->row_cells, bm->d_out,
-                                                bm->d_in);
+    k_fill_major<<<grid_for(nnz), 256, 0, st>>>(nnz, B, cbits, uniq, counts, bm->row_ptr, bm->row_cells);
     SBP_HIP(hipGetLastError());
     // transpose: sort (col, row) keys carrying the counts
     auto *tkeys = keys;  // reuse
@@ -255,11 +285,10 @@ void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm) {
     tmp_ptr = ctx->scratch[S_SORT_TMP].get(tmp_c);
     SBP_HIP(hipcub::DeviceRadixSort::SortPairs(tmp_ptr, tmp_c, tkeys, tkeys_sorted, counts, tvals_sorted, (int)nnz, 0,
                                                end_bit, st));
-    k_fill_major<<<grid_for(nnz), 256, 0, st>>>(nnz, B, cbits, tkeys_sorted, tvals_sorted, bm->col_ptr, bm->col_cells,
-                                                nullptr, nullptr);
+    k_fill_major<<<grid_for(nnz), 256, 0, st>>>(nnz, B, cbits, tkeys_sorted, tvals_sorted, bm->col_ptr, bm->col_cells);
+    launch_cum(ctx, bm, bm->d_out, bm->d_in);  // cumulative counts, and the degrees as row / column totals
     k_dtot_sizes<<<grid_for(B), 256, 0, st>>>(B, bm->row_ptr, bm->row_cells, bm->d_out, bm->d_in, bm->d_tot);
     SBP_HIP(hipGetLastError());
-    bm_compute_cum(ctx, bm);
 }
 
 void bm_copy_into(sbp_ctx ctx, sbp_bm src, sbp_bm dst) {
