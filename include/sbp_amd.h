@@ -112,7 +112,9 @@ int sbp_graph_download(sbp_ctx ctx, sbp_graph g, int64_t *out_off, int32_t *out_
 /* Replaces sample::sample (src/sample.cpp:352-365) with sample::from_vertices (:202-217): draws
  * (long)(fraction * N) vertices and builds their induced subgraph on the device.  alg: 0 max_degree
  * (:219-233), 1 random (:235-247), 2 degree_product (:28-49), 3 degree_weighted (new: weighted sampling
- * without replacement, weight = degree).  The sample keeps its vertex mapping for sbp_extend.
+ * without replacement, weight = degree), 4 expansion_snowball (:94-155), 5 degree_product_snowball
+ * (:51-92); the two snowball samplers grow the sample sequentially on the host, the subgraph is built on
+ * the device.  The sample keeps its vertex mapping for sbp_extend.
  * mapping_out: host int32[N] (vertex -> sample vertex or -1), may be NULL. */
 int sbp_sample(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, uint64_t seed, sbp_graph *sample_out,
                int32_t *num_sampled, int32_t *mapping_out);

@@ -289,7 +289,8 @@ def sbp(graph, prm):
     return assign, nb.value, st
 
 
-SAMPLING = {"max_degree": 0, "random": 1, "degree_product": 2, "degree_weighted": 3}
+SAMPLING = {"max_degree": 0, "random": 1, "degree_product": 2, "degree_weighted": 3, "expansion_snowball": 4,
+            "degree_product_snowball": 5}
 
 
 def sample(graph, alg, fraction, seed=0):

@@ -1627,6 +1627,8 @@ int orc_sbp(const orc_graph *g, const orc_params *p, int32_t *assignment_out, in
  *   alg 3 = degree_weighted (new): weighted sampling without replacement with weights = degree, by
  *           Efraimidis-Spirakis keys: key(v) = -logf(u_v) / degree(v) in float, u_v = (2 (w0 >> 9) + 1) 2^-24,
  *           ascending key (degree 0: +inf), ties by vertex id.
+ *   alg 4 = expansion_snowball (sample.cpp:94-155) and alg 5 = degree_product_snowball (sample.cpp:51-92):
+ *           see orc_snowball below.
  * The target is (long)(fraction * N) (sample.cpp:228).  Writes mapping[N] (full id -> sample id or -1) and
  * vertices[n] (sample id -> full id). */
 typedef struct { uint64_t key; int64_t idx; } key64_t;
@@ -1640,10 +1642,161 @@ float orc_degree_weight_key(uint32_t w0, int64_t degree) {
     float u = (float)(((w0 >> 9) << 1) | 1u) * 0x1p-24f;
     return -logf(u) / (float)degree;
 }
+/* Snowball samplers, literally: common::random_integer(lo, hi) is draw j of
+ * Philox(seed; j, 0, TAG_SNOWBALL, 0), lo + (w0 (hi - lo + 1)) >> 32, j counting the calls.
+ * expansion_snowball (sample.cpp:94-155; es_add_vertex :94-110, es_update_contribution :112-124): the
+ * neighbour set is a flag array whose k-th set entry in ascending id stands for std::advance over the
+ * std::set; argmax (include/utils.hpp:241-253) scans all vertices for the first maximum, sampled ones
+ * included; a vertex chosen again is appended again and its mapping moves to the new slot.
+ * degree_product_snowball (sample.cpp:51-92): std::priority_queue<(src, dst, product)> ordered by product,
+ * restated with libstdc++'s heap operations (push_heap sift-up; pop_heap moves the last element to a hole
+ * that descends along the larger child, the right one unless the left is strictly larger, then sifts up),
+ * which fix the order in which equal products pop.  When a restart leaves the frontier empty the reference
+ * reads top() of an empty queue (undefined); here the loop continues.  Returns the vertices in sample
+ * order; mapping[v] = v's last slot or -1. */
+static int64_t snow_draw(uint64_t seed, uint32_t *next, int64_t lo, int64_t hi) {
+    uint32_t w[4];
+    orc_philox(seed, (*next)++, 0, ORC_TAG_SNOWBALL, 0, w);
+    return lo + (int64_t)(((uint64_t)w[0] * (uint64_t)(hi - lo + 1)) >> 32);
+}
+typedef struct { int64_t src, dst, prod; } snow_edge;
+static void snow_sift_up(snow_edge *h, int64_t hole, int64_t top, snow_edge v) {
+    int64_t parent = (hole - 1) / 2;
+    while (hole > top && h[parent].prod < v.prod) {
+        h[hole] = h[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    h[hole] = v;
+}
+static void snow_push(snow_edge *h, int64_t *len, snow_edge v) { snow_sift_up(h, (*len)++, 0, v); }
+static snow_edge snow_pop(snow_edge *h, int64_t *len) {
+    snow_edge top = h[0];
+    int64_t n = *len - 1;  /* elements left after the pop */
+    snow_edge v = h[n];
+    h[n] = h[0];
+    int64_t hole = 0, child = 0;
+    while (child < (n - 1) / 2) {
+        child = 2 * (child + 1);
+        if (h[child].prod < h[child - 1].prod) child--;
+        h[hole] = h[child];
+        hole = child;
+    }
+    if ((n & 1) == 0 && child == (n - 2) / 2) {
+        child = 2 * (child + 1);
+        h[hole] = h[child - 1];
+        hole = child - 1;
+    }
+    snow_sift_up(h, hole, 0, v);
+    *len = n;
+    return top;
+}
+static int64_t orc_snowball(const orc_graph *g, int32_t alg, int64_t target, uint64_t seed, int32_t *mapping,
+                            int32_t *order) {
+    int32_t N = g->N;
+    uint32_t next = 0;
+    int64_t n = 0;
+    for (int32_t v = 0; v < N; ++v) mapping[v] = -1;
+    if (alg == 4) {
+        int64_t *contrib = (int64_t *)calloc((size_t)N, sizeof(int64_t));
+        char *flag = (char *)calloc((size_t)N, 1);
+        int64_t sum = 0, nset = 0;
+#define ES_UPDATE(u)                                                                   \
+        do {                                                                           \
+            for (int64_t e_ = g->out_off[u]; e_ < g->out_off[(u) + 1]; ++e_) {         \
+                int32_t nb_ = g->out_adj[e_];                                          \
+                if (flag[nb_] || mapping[nb_] >= 0) continue;                          \
+                contrib[u]++, sum++;                                                   \
+            }                                                                          \
+            for (int64_t e_ = g->in_off[u]; e_ < g->in_off[(u) + 1]; ++e_) {           \
+                int32_t nb_ = g->in_adj[e_];                                           \
+                if (contrib[nb_] > 0) contrib[nb_]--, sum--;                           \
+            }                                                                          \
+        } while (0)
+#define ES_ADD(x)                                                                      \
+        do {                                                                           \
+            int32_t v_ = (x);                                                          \
+            order[n] = v_;                                                             \
+            mapping[v_] = (int32_t)n++;                                                \
+            for (int64_t e = g->out_off[v_]; e < g->out_off[v_ + 1]; ++e) {            \
+                int32_t nb = g->out_adj[e];                                            \
+                if (flag[nb] || mapping[nb] >= 0) continue;                            \
+                flag[nb] = 1, nset++;                                                  \
+                if (contrib[nb] > 0) continue;                                         \
+                ES_UPDATE(nb);                                                         \
+            }                                                                          \
+            if (flag[v_]) flag[v_] = 0, nset--;                                        \
+        } while (0)
+        ES_ADD((int32_t)snow_draw(seed, &next, 0, N - 1));
+        while (n < target) {
+            if (nset == 0) {
+                int32_t v;
+                do v = (int32_t)snow_draw(seed, &next, 0, N - 1); while (mapping[v] >= 0);
+                ES_ADD(v);
+            } else if (sum == 0) {
+                int64_t k = snow_draw(seed, &next, 0, nset - 1), seen = -1;
+                int32_t v = 0;
+                for (int32_t u = 0; u < N; ++u)
+                    if (flag[u] && ++seen == k) { v = u; break; }
+                ES_ADD(v);
+            } else {
+                int32_t best = 0;
+                for (int32_t u = 1; u < N; ++u)
+                    if (contrib[u] > contrib[best]) best = u;
+                ES_ADD(best);
+            }
+        }
+#undef ES_ADD
+#undef ES_UPDATE
+        free(contrib);
+        free(flag);
+    } else {
+        int64_t *deg = (int64_t *)malloc(sizeof(int64_t) * (size_t)N);
+        for (int32_t v = 0; v < N; ++v) deg[v] = vertex_degree(g, v);
+        snow_edge *h = (snow_edge *)malloc(sizeof(snow_edge) * (size_t)(2 * g->E + 1));
+        int64_t len = 0;
+#define DPS_ADD(x)                                                                     \
+        do {                                                                           \
+            int32_t v_ = (x);                                                          \
+            mapping[v_] = (int32_t)n;                                                  \
+            order[n++] = v_;                                                           \
+            for (int64_t e = g->out_off[v_]; e < g->out_off[v_ + 1]; ++e) {            \
+                int32_t nb = g->out_adj[e];                                            \
+                if (mapping[nb] != -1) continue;                                       \
+                snow_edge x_ = { v_, nb, deg[v_] * deg[nb] };                          \
+                snow_push(h, &len, x_);                                                \
+            }                                                                          \
+            for (int64_t e = g->in_off[v_]; e < g->in_off[v_ + 1]; ++e) {              \
+                int32_t nb = g->in_adj[e];                                             \
+                if (mapping[nb] != -1) continue;                                       \
+                snow_edge x_ = { nb, v_, deg[nb] * deg[v_] };                          \
+                snow_push(h, &len, x_);                                                \
+            }                                                                          \
+        } while (0)
+        DPS_ADD((int32_t)snow_draw(seed, &next, 0, N - 1));
+        while (n < target) {
+            if (len == 0) {
+                int32_t v;
+                do v = (int32_t)snow_draw(seed, &next, 0, N - 1); while (mapping[v] >= 0);
+                DPS_ADD(v);
+                if (len == 0) continue;
+            }
+            snow_edge top = snow_pop(h, &len);
+            if (mapping[top.src] == -1) DPS_ADD((int32_t)top.src);
+            if (mapping[top.dst] == -1) DPS_ADD((int32_t)top.dst);
+        }
+#undef DPS_ADD
+        free(h);
+        free(deg);
+    }
+    return n;
+}
+
 int64_t orc_sample(const orc_graph *g, int32_t alg, double fraction, uint64_t seed, int32_t *mapping,
                    int32_t *vertices) {
     int32_t N = g->N;
     int64_t target = (int64_t)(fraction * (double)N);
+    if (alg == 4 || alg == 5) return orc_snowball(g, alg, target, seed, mapping, vertices);
     int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
     int64_t n = 0;
     for (int32_t v = 0; v < N; ++v) mapping[v] = -1;

@@ -63,6 +63,7 @@ extern "C" {
 #define ORC_TAG_SHUFFLE 0x53480001u  /* vertex shuffle key: unit = vertex */
 #define ORC_TAG_EXTEND  0x45580001u  /* sample extension random block: unit = vertex */
 #define ORC_TAG_SAMPLE  0x53410001u  /* random sampling key: unit = vertex */
+#define ORC_TAG_SNOWBALL 0x53410002u  /* snowball samplers: unit = draw index */
 
 typedef struct {
     int32_t N;

@@ -25,6 +25,7 @@
 #include "device_common.hpp"
 #include "device_table.hpp"
 #include "internal.hpp"
+#include "snowball.hpp"
 
 namespace sbpimpl {
 using namespace sbpdev;
@@ -329,18 +330,34 @@ sbp_graph sample_graph(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, u
     const int64_t E = g->E;
     const int64_t target = (int64_t)(fraction * (double)N);
     SBP_CHECK(target >= 0 && target <= N, SBP_E_ARG, "sample fraction must be in [0, 1]");
-    SBP_CHECK(alg == 2 || target > 0, SBP_E_ARG, "sample would be empty");
+    SBP_CHECK(alg == 2 || alg >= 4 || target > 0, SBP_E_ARG, "sample would be empty");
     DevBuf<uint64_t> keys(N), sorted(N);
     DevBuf<int32_t> order(N);
     int64_t n = target;
-    if (alg == 0) {
+    std::vector<int32_t> host_order, host_mapping;  // the snowball samplers run on the host
+    if (alg == 4 || alg == 5) {
+        std::vector<int64_t> oo(N + 1), io(N + 1);
+        std::vector<int32_t> oa(E + 1), ia(E + 1), deg(N + 1);
+        SBP_HIP(hipStreamSynchronize(st));
+        SBP_HIP(hipMemcpy(oo.data(), g->out_off, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToHost));
+        SBP_HIP(hipMemcpy(io.data(), g->in_off, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToHost));
+        if (E > 0) {
+            SBP_HIP(hipMemcpy(oa.data(), g->out_adj, sizeof(int32_t) * E, hipMemcpyDeviceToHost));
+            SBP_HIP(hipMemcpy(ia.data(), g->in_adj, sizeof(int32_t) * E, hipMemcpyDeviceToHost));
+        }
+        SBP_HIP(hipMemcpy(deg.data(), g->degree, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+        const sbpsnow::Csr c{N, oo.data(), io.data(), oa.data(), ia.data()};
+        host_order = alg == 4 ? sbpsnow::expansion_snowball(c, target, seed, host_mapping)
+                              : sbpsnow::degree_product_snowball(c, deg.data(), target, seed, host_mapping);
+        n = (int64_t)host_order.size();
+    } else if (alg == 0) {
         k_degree_keys<<<grid_for(N), GB, 0, st>>>(N, g->degree, keys.p);
     } else if (alg == 1) {
         k_random_keys<<<grid_for(N), GB, 0, st>>>(N, seed, keys.p);
     } else if (alg == 3) {
         k_weighted_keys<<<grid_for(N), GB, 0, st>>>(N, seed, g->degree, keys.p);
     } else {
-        SBP_CHECK(alg == 2, SBP_E_ARG, "unknown sampling algorithm");
+        SBP_CHECK(alg == 2, SBP_E_ARG, "unknown sampling algorithm (0..5)");
         SBP_CHECK(2 * E < INT32_MAX, SBP_E_ARG, "degree_product sampling needs 2E < 2^31");
         n = 0;
         DevBuf<int32_t> first(N);
@@ -377,8 +394,10 @@ sbp_graph sample_graph(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, u
             }
         }
     }
-    if (alg != 2) sort_keys(st, keys.p, sorted.p, N);
-    k_low_words<<<grid_for(N), GB, 0, st>>>(N, sorted.p, order.p);
+    if (host_order.empty()) {
+        if (alg != 2) sort_keys(st, keys.p, sorted.p, N);
+        k_low_words<<<grid_for(N), GB, 0, st>>>(N, sorted.p, order.p);
+    }
 
     SBP_CHECK(n > 0, SBP_E_ARG, "sample would be empty");
     std::unique_ptr<sbp_graph_s, int (*)(sbp_graph)> s(new sbp_graph_s(), sbp_graph_free);
@@ -388,10 +407,15 @@ sbp_graph sample_graph(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, u
     s->parent_N = N;
     SBP_HIP(hipMalloc(&s->parent_map, sizeof(int32_t) * N));
     SBP_HIP(hipMalloc(&s->sample_vertices, sizeof(int32_t) * std::max<int64_t>(n, 1)));
-    k_fill<<<grid_for(N), GB, 0, st>>>(N, s->parent_map, -1);
-    if (n > 0) {
-        k_map_sample<<<grid_for(n), GB, 0, st>>>(n, order.p, s->parent_map);
-        SBP_HIP(hipMemcpyAsync(s->sample_vertices, order.p, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st));
+    if (!host_order.empty()) {  // as sampled: a vertex appended twice keeps an isolated earlier slot
+        SBP_HIP(hipMemcpyAsync(s->parent_map, host_mapping.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
+        SBP_HIP(hipMemcpyAsync(s->sample_vertices, host_order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+    } else {
+        k_fill<<<grid_for(N), GB, 0, st>>>(N, s->parent_map, -1);
+        if (n > 0) {
+            k_map_sample<<<grid_for(n), GB, 0, st>>>(n, order.p, s->parent_map);
+            SBP_HIP(hipMemcpyAsync(s->sample_vertices, order.p, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st));
+        }
     }
     // induced subgraph (from_vertices): the kept out-edges in full CSR order (source id ascending, list order),
     // stably sorted by sample source (out-lists) and by sample destination (in-lists)

@@ -1,10 +1,12 @@
-// Host-only entry to the merge carry-out (sbp_amd/csrc/carry.hpp), built without HIP so the CPU test
-// suite can check it against the oracle at full size.  Not part of the device C-ABI.
+// Host-only entries to the merge carry-out (sbp_amd/csrc/carry.hpp) and the snowball samplers
+// (sbp_amd/csrc/snowball.hpp), built without HIP so the CPU test suite can check them against the oracle at
+// full size.  Not part of the device C-ABI.
 #include <chrono>
 #include <memory>
 #include <vector>
 
 #include "../csrc/carry.hpp"
+#include "../csrc/snowball.hpp"
 
 // stats_out (10 entries): pops, sequential evaluations, speculative evaluations, reused evaluations,
 // requeues, merges, model load microseconds, carry-out loop microseconds, threads, proposal values that
@@ -50,4 +52,20 @@ extern "C" int sbp_host_carry_out(int32_t B, const int32_t *row_ptr, const int32
         for (int i = 0; i < 10; ++i) stats_out[i] = v[i];
     }
     return nb;
+}
+
+// alg 4 expansion_snowball, 5 degree_product_snowball over a host CSR; degree[N] = |out| + |in| - self-loop
+// flag.  Writes mapping[N] and vertices[<= N] (sample order); returns the number of sampled slots or -1.
+extern "C" int64_t sbp_host_snowball(int32_t alg, int32_t N, const int64_t *out_off, const int32_t *out_adj,
+                                     const int64_t *in_off, const int32_t *in_adj, const int32_t *degree,
+                                     double fraction, uint64_t seed, int32_t *mapping, int32_t *vertices) {
+    if (N <= 0 || (alg != 4 && alg != 5)) return -1;
+    const sbpsnow::Csr c{N, out_off, in_off, out_adj, in_adj};
+    const int64_t target = (int64_t)(fraction * (double)N);
+    std::vector<int32_t> map;
+    std::vector<int32_t> order = alg == 4 ? sbpsnow::expansion_snowball(c, target, seed, map)
+                                          : sbpsnow::degree_product_snowball(c, degree, target, seed, map);
+    for (int32_t v = 0; v < N; ++v) mapping[v] = map[v];
+    for (size_t i = 0; i < order.size(); ++i) vertices[i] = order[i];
+    return (int64_t)order.size();
 }

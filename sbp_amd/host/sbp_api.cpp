@@ -477,13 +477,14 @@ namespace sample {
 
 Sample sample(const Graph &graph) {
     const Args &a = sbp_amd::args();
-    static const char *names[] = {"max_degree", "random", "degree_product", "degree_weighted"};
+    static const char *names[] = {"max_degree",         "random", "degree_product", "degree_weighted",
+                                  "expansion_snowball", "degree_product_snowball"};
     int32_t alg = -1;
-    for (int32_t i = 0; i < 4; ++i)
+    for (int32_t i = 0; i < 6; ++i)
         if (a.samplingalg == names[i]) alg = i;
     if (alg < 0)
-        throw std::invalid_argument(a.samplingalg + " is not a valid sampling algorithm on the device path "
-                                    "(max_degree, random, degree_product, degree_weighted)");
+        throw std::invalid_argument(a.samplingalg + " is not a valid sampling algorithm (max_degree, random, "
+                                    "degree_product, degree_weighted, expansion_snowball, degree_product_snowball)");
     sbp_ctx ctx = context(a);
     sbp_graph parent = graph.device(ctx);
     sbp_graph s = nullptr;

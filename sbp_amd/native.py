@@ -159,7 +159,8 @@ class Context:
         return out
 
 
-SAMPLING = {"max_degree": 0, "random": 1, "degree_product": 2, "degree_weighted": 3}
+SAMPLING = {"max_degree": 0, "random": 1, "degree_product": 2, "degree_weighted": 3, "expansion_snowball": 4,
+            "degree_product_snowball": 5}
 
 
 class DeviceGraph:

@@ -11,7 +11,7 @@ from sbp_amd.graph import dcsbm
 
 pytestmark = pytest.mark.gpu
 
-ALGS = ["max_degree", "random", "degree_product", "degree_weighted"]
+ALGS = ["max_degree", "random", "degree_product", "degree_weighted", "expansion_snowball", "degree_product_snowball"]
 
 
 @pytest.fixture(scope="module")
@@ -45,7 +45,8 @@ def test_sample_and_subgraph_bit_exact(ctx, alg, fraction):
     same_graph(sg, O.subgraph(og, om, ovs))
 
 
-@pytest.mark.parametrize("alg", ["max_degree", "random", "degree_weighted"])
+@pytest.mark.parametrize("alg", ["max_degree", "random", "degree_weighted", "expansion_snowball",
+                                 "degree_product_snowball"])
 def test_extend_bit_exact(ctx, alg):
     print(f"[extend] alg={alg}", flush=True)
     g = graph(seed=22)
@@ -126,3 +127,23 @@ def test_sample_extend_500k_bit_exact(ctx):
         full, a = sbm.extend(dg, seed=3)
         assert np.array_equal(a, O.extend(og, om, sa, B, 3))
         assert (a >= 0).all() and (a < B).all()
+
+
+@pytest.mark.parametrize("alg", ["expansion_snowball", "degree_product_snowball"])
+def test_snowball_sampled_run_bit_exact(ctx, alg):
+    """A sampled run with a snowball sampler: sample (host frontier), subgraph (device), partition, extend."""
+    g = dcsbm(6000, 12, gamma=2.2, kmin=1, kmax=300, mean_degree=10, p_in=0.6, sigma=1.0, seed=4)
+    og = O.OGraph.from_csr(g)
+    dg = ctx.upload(g)
+    print(f"[snowball run] {alg}", flush=True)
+    sg, m = dg.sample(alg, 0.4, seed=6)
+    om, ovs = O.sample(og, alg, 0.4, 6)
+    assert np.array_equal(m, om)
+    osg = O.subgraph(og, om, ovs)
+    same_graph(sg, osg)
+    a_d, nb_d, _ = sg.partition(native.params("async_gibbs", seed=0))
+    a_o, nb_o, _ = O.sbp(osg, O.params("async_gibbs", seed=0, threads=8))
+    assert nb_d == nb_o and np.array_equal(a_d, a_o)
+    sbm = native.DeviceBlockmodel.build(sg, nb_d, a_d)
+    _, a = sbm.extend(dg, seed=0)
+    assert np.array_equal(a, O.extend(og, om, a_o, nb_o, 0))

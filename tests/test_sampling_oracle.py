@@ -170,3 +170,154 @@ def test_extend_matches_reference_loop(alg):
             a[v] = min(c for c, k in counts.items() if k == best)
     out = O.extend(og, m, sample_assign, B, 5)
     assert list(out) == a
+
+
+TAG_SNOWBALL = 0x53410002
+_HOST = None
+
+
+def _host_lib():
+    """The library's host-only build (std::priority_queue, segment and Fenwick trees)."""
+    global _HOST
+    if _HOST is None:
+        import os
+        path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sbp_amd", "libsbp_carry.so")
+        _HOST = ctypes.CDLL(path)
+        _HOST.sbp_host_snowball.restype = ctypes.c_int64
+        _HOST.sbp_host_snowball.argtypes = [ctypes.c_int32, ctypes.c_int32] + [ctypes.c_void_p] * 5 + [
+            ctypes.c_double, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]
+    return _HOST
+
+
+def _host_snowball(g, alg, fraction, seed):
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    deg = np.array(_degrees(g), np.int32)
+    oo, oa = np.ascontiguousarray(g.out_off, np.int64), np.ascontiguousarray(g.out_adj, np.int32)
+    io, ia = np.ascontiguousarray(g.in_off, np.int64), np.ascontiguousarray(g.in_adj, np.int32)
+    mapping = np.zeros(g.N, np.int32)
+    verts = np.zeros(g.N + 1, np.int32)
+    n = _host_lib().sbp_host_snowball(O.SAMPLING[alg], g.N, p(oo), p(oa), p(io), p(ia), p(deg), fraction, seed,
+                                      p(mapping), p(verts))
+    return mapping, verts[:n]
+
+
+def _components_graph(seed):
+    """Three small DCSBM graphs side by side plus isolated vertices: the frontier empties and restarts."""
+    from sbp_amd.graph import CSRGraph
+    parts, src, dst, off = [], [], [], 0
+    for i, n in enumerate((60, 90, 40)):
+        h = dcsbm(n, 3, gamma=2.3, kmin=1, kmax=12, mean_degree=4, p_in=0.8, seed=seed + i)
+        s = np.repeat(np.arange(h.N), np.diff(h.out_off))
+        src.append(s + off)
+        dst.append(np.asarray(h.out_adj) + off)
+        off += h.N
+    N = off + 7
+    return CSRGraph.from_edges(np.concatenate(src), np.concatenate(dst), N)
+
+
+def _ring_graph(N=120, k=3):
+    """Every vertex has the same degree: all degree products tie and only the heap order decides."""
+    from sbp_amd.graph import CSRGraph
+    src = np.repeat(np.arange(N), k)
+    dst = (src + np.tile(np.arange(1, k + 1), N)) % N
+    return CSRGraph.from_edges(src, dst, N)
+
+
+def _es_reference(g, fraction, seed):
+    """sample::expansion_snowball (src/sample.cpp:94-155) restated: std::set as a Python set read in ascending
+    order, utils::argmax as the first maximum over all vertices."""
+    outs, ins = _lists(g)
+    N, target = g.N, int(fraction * g.N)
+    nxt = [0]
+
+    def rint(lo, hi):
+        w = int(O.philox(seed, nxt[0], 0, TAG_SNOWBALL, 0)[0])
+        nxt[0] += 1
+        return lo + ((w * (hi - lo + 1)) >> 32)
+
+    contribution, flag, neighbors = [0] * N, [False] * N, set()
+    sampled, mapping, total = [], [-1] * N, [0]
+
+    def update(u):
+        for nb in outs[u]:
+            if flag[nb] or mapping[nb] >= 0:
+                continue
+            contribution[u] += 1
+            total[0] += 1
+        for nb in ins[u]:
+            if contribution[nb] > 0:
+                contribution[nb] -= 1
+                total[0] -= 1
+
+    def add(v):
+        sampled.append(v)
+        mapping[v] = len(sampled) - 1
+        for nb in outs[v]:
+            if flag[nb] or mapping[nb] >= 0:
+                continue
+            neighbors.add(nb)
+            flag[nb] = True
+            if contribution[nb] > 0:
+                continue
+            update(nb)
+        neighbors.discard(v)
+        flag[v] = False
+
+    add(rint(0, N - 1))
+    while len(sampled) < target:
+        if not neighbors:
+            v = rint(0, N - 1)
+            while mapping[v] >= 0:
+                v = rint(0, N - 1)
+            add(v)
+        elif total[0] == 0:
+            add(sorted(neighbors)[rint(0, len(neighbors) - 1)])
+        else:
+            add(max(range(N), key=lambda u: (contribution[u], -u)))
+    return np.array(mapping, np.int32), np.array(sampled, np.int32)
+
+
+@pytest.mark.parametrize("graph", ["dcsbm", "components"])
+@pytest.mark.parametrize("fraction", [0.05, 0.3, 0.9])
+def test_expansion_snowball_matches_reference_loop(graph, fraction):
+    g = _graph(seed=11) if graph == "dcsbm" else _components_graph(3)
+    for seed in (0, 5):
+        em, ev = _es_reference(g, fraction, seed)
+        m, vs = O.sample(O.OGraph.from_csr(g), "expansion_snowball", fraction, seed)
+        assert np.array_equal(vs, ev) and np.array_equal(m, em), seed
+
+
+@pytest.mark.parametrize("alg", ["expansion_snowball", "degree_product_snowball"])
+@pytest.mark.parametrize("graph", ["dcsbm", "components", "ring"])
+@pytest.mark.parametrize("fraction", [0.0, 0.1, 0.5, 1.0])
+def test_snowball_library_matches_oracle(alg, graph, fraction):
+    """The oracle's heap restatement against libstdc++'s std::priority_queue (ring: every product ties), and
+    the literal argmax / set scans against the library's segment and Fenwick trees."""
+    g = {"dcsbm": lambda: _graph(seed=12, N=600), "components": lambda: _components_graph(8),
+         "ring": _ring_graph}[graph]()
+    for seed in (1, 9):
+        m, vs = O.sample(O.OGraph.from_csr(g), alg, fraction, seed)
+        hm, hv = _host_snowball(g, alg, fraction, seed)
+        assert np.array_equal(vs, hv) and np.array_equal(m, hm), seed
+
+
+@pytest.mark.parametrize("graph", ["dcsbm", "components"])
+def test_degree_product_snowball_invariants(graph):
+    """Sizes, distinct vertices, and every vertex after a restart joins through an edge of the sample."""
+    g = _graph(seed=13, N=500) if graph == "dcsbm" else _components_graph(4)
+    outs, ins = _lists(g)
+    deg = _degrees(g)
+    for fraction in (0.1, 0.6):
+        m, vs = O.sample(O.OGraph.from_csr(g), "degree_product_snowball", fraction, 2)
+        target = int(fraction * g.N)
+        assert target <= len(vs) <= max(target, 1) + 1
+        assert len(set(vs.tolist())) == len(vs)
+        assert all(m[v] == i for i, v in enumerate(vs))
+        seen = {int(vs[0])}
+        for v in vs[1:]:
+            v = int(v)
+            linked = any(u in seen for u in outs[v]) or any(u in seen for u in ins[v])
+            frontier_empty = all(all(u in seen for u in outs[w]) and all(u in seen for u in ins[w]) for w in seen)
+            assert linked or frontier_empty, v
+            seen.add(v)
+        assert max(deg) >= 1
