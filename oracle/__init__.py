@@ -25,7 +25,7 @@ class Params(C.Structure):
     _fields_ = [("algorithm", C.c_int32), ("batches", C.c_int32), ("greedy", C.c_int32),
                 ("approximate", C.c_int32), ("ordered", C.c_int32), ("asynciterations", C.c_int32),
                 ("mh_percent", C.c_double), ("seed", C.c_uint64), ("max_iterations", C.c_int32),
-                ("threads", C.c_int32), ("nonparametric", C.c_int32)]
+                ("threads", C.c_int32), ("nonparametric", C.c_int32), ("degreecorrected", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -59,7 +59,10 @@ def lib():
             "orc_bm_nnz": (i64, [vp]),
             "orc_bm_export": (None, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
             "orc_bm_validate": (C.c_int, [vp, G]),
-            "orc_bm_set_model": (None, [vp, i32]),
+            "orc_bm_set_model": (None, [vp, vp, i32]),
+            "orc_spence": (C.c_double, [C.c_double]),
+            "orc_log_q": (C.c_double, [C.c_uint64, C.c_uint64]),
+            "orc_np_deg_dl": (C.c_double, [vp]),
             "orc_lgamma": (dbl, [i64]),
             "orc_lbinom": (dbl, [i64, i64]),
             "orc_np_sparse_entropy": (dbl, [vp]),
@@ -123,10 +126,10 @@ class OGraph:
 
 def params(algorithm="async_gibbs", batches=1, greedy=True, approximate=False, ordered=False,
            asynciterations=2 ** 31 - 1, mh_percent=0.075, seed=0, max_iterations=100, threads=1,
-           nonparametric=False):
+           nonparametric=False, degreecorrected=False):
     return Params(ALG[algorithm] if isinstance(algorithm, str) else algorithm, batches, int(greedy),
                   int(approximate), int(ordered), asynciterations, mh_percent, seed, max_iterations, threads,
-                  int(nonparametric))
+                  int(nonparametric), int(degreecorrected))
 
 
 class OBlockmodel:
@@ -135,16 +138,17 @@ class OBlockmodel:
         self.g = graph
 
     @classmethod
-    def build(cls, graph, B, assignment, nonparametric=False):
+    def build(cls, graph, B, assignment, nonparametric=False, degreecorrected=False):
         a = np.ascontiguousarray(assignment, dtype=np.int32)
         bm = cls(lib().orc_bm_new(graph.ref(), B, _p(a)), graph)
         if nonparametric:
-            bm.set_model(True)
+            bm.set_model(True, degreecorrected)
         return bm
 
-    def set_model(self, nonparametric):
-        """Entropy model used by mdl / delta-MDL: parametric (entropy::mdl) or nonparametric."""
-        lib().orc_bm_set_model(self.h, int(nonparametric))
+    def set_model(self, nonparametric, degreecorrected=False):
+        """Entropy model used by mdl / delta-MDL: parametric (entropy::mdl), nonparametric, or nonparametric
+        with degree correction (degree histograms built from the graph)."""
+        lib().orc_bm_set_model(self.h, self.g.ref(), (2 if degreecorrected else 1) if nonparametric else 0)
 
     def np_parts(self):
         """(sparse_entropy, partition_dl, edges_dl) of the nonparametric description length."""
@@ -153,6 +157,10 @@ class OBlockmodel:
         nonempty = int((x["sizes"] > 0).sum())
         return (L.orc_np_sparse_entropy(self.h), L.orc_np_partition_dl(self.h),
                 L.orc_np_edges_dl(nonempty, self.g.E))
+
+    def np_deg_dl(self):
+        """get_deg_dl_dist: the degree description length (0 without degree correction)."""
+        return lib().orc_np_deg_dl(self.h)
 
     def __del__(self):
         if getattr(self, "h", None) is not None and _lib is not None:

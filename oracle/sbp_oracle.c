@@ -184,6 +184,9 @@ struct orc_bm {
     int64_t num_to_merge;
     int empty;
     int32_t np;  /* entropy model: 0 parametric (entropy::mdl), 1 nonparametric (entropy::nonparametric) */
+    int32_t dc;  /* nonparametric with degree correction (args.degreecorrected) */
+    svec *hin, *hout;  /* dc: per block, vertex in-/out-degree -> number of its vertices (degree histograms) */
+    const orc_graph *g;  /* dc: the graph the histograms are keyed by */
 };
 
 static void bm_add(orc_bm *bm, int32_t r, int32_t c, int64_t d) {
@@ -218,6 +221,13 @@ void orc_bm_free(orc_bm *bm) {
         free(bm->col[b].idx); free(bm->col[b].val);
     }
     free(bm->row); free(bm->col);
+    if (bm->hin) {
+        for (int32_t b = 0; b < bm->B; ++b) {
+            free(bm->hin[b].idx); free(bm->hin[b].val);
+            free(bm->hout[b].idx); free(bm->hout[b].val);
+        }
+        free(bm->hin); free(bm->hout);
+    }
     free(bm->assign); free(bm->size); free(bm->d_out); free(bm->d_in); free(bm->d_tot);
     free(bm);
 }
@@ -268,13 +278,49 @@ orc_bm *orc_bm_copy(const orc_bm *src) {
         sv_copy(&bm->row[b], &src->row[b]);
         sv_copy(&bm->col[b], &src->col[b]);
     }
+    if (src->hin) {
+        bm->hin = (svec *)calloc((size_t)src->B, sizeof(svec));
+        bm->hout = (svec *)calloc((size_t)src->B, sizeof(svec));
+        for (int32_t b = 0; b < src->B; ++b) {
+            sv_copy(&bm->hin[b], &src->hin[b]);
+            sv_copy(&bm->hout[b], &src->hout[b]);
+        }
+    }
     bm->num_to_merge = 0;
     return bm;
 }
 
 int32_t orc_bm_num_blocks(const orc_bm *bm) { return bm->B; }
 
-void orc_bm_set_model(orc_bm *bm, int32_t nonparametric) { bm->np = nonparametric; }
+static int64_t vtx_out(const orc_graph *g, int32_t v) { return g->out_off[v + 1] - g->out_off[v]; }
+static int64_t vtx_in(const orc_graph *g, int32_t v) { return g->in_off[v + 1] - g->in_off[v]; }
+/* Degree histograms of initialize_edge_counts (src/blockmodel/blockmodel.cpp:234-235): keyed by the vertex's
+ * out_neighbors / in_neighbors sizes. */
+static void bm_fill_hist(orc_bm *bm, const orc_graph *g) {
+    if (bm->hin) {
+        for (int32_t b = 0; b < bm->B; ++b) {
+            free(bm->hin[b].idx); free(bm->hin[b].val);
+            free(bm->hout[b].idx); free(bm->hout[b].val);
+        }
+        free(bm->hin); free(bm->hout);
+    }
+    bm->hin = (svec *)calloc((size_t)bm->B, sizeof(svec));
+    bm->hout = (svec *)calloc((size_t)bm->B, sizeof(svec));
+    bm->g = g;
+    for (int32_t v = 0; v < g->N; ++v) {
+        sv_add(&bm->hout[bm->assign[v]], (int32_t)vtx_out(g, v), 1);
+        sv_add(&bm->hin[bm->assign[v]], (int32_t)vtx_in(g, v), 1);
+    }
+}
+/* model: 0 parametric, 1 nonparametric, 2 nonparametric degree-corrected (histograms built from g) */
+void orc_bm_set_model(orc_bm *bm, const orc_graph *g, int32_t model) {
+    bm->np = model >= 1;
+    bm->dc = model == 2;
+    if (bm->dc && (!bm->hin || bm->g != g)) bm_fill_hist(bm, g);
+}
+static void bm_model_from(orc_bm *bm, const orc_graph *g, const orc_params *p) {
+    orc_bm_set_model(bm, g, p->nonparametric ? (p->degreecorrected ? 2 : 1) : 0);
+}
 
 int64_t orc_bm_nnz(const orc_bm *bm) {
     int64_t n = 0;
@@ -371,10 +417,119 @@ double orc_lbinom(int64_t n, int64_t k) {
 }
 /* eterm_exact (include/entropy.hpp:103-112), directed */
 static double np_eterm(int64_t w) { return -orc_lgamma(w + 1); }
-/* vterm_exact (include/entropy.hpp:114-135), directed, not degree-corrected */
-static double np_vterm(int64_t d_out, int64_t d_in, int64_t w) {
+/* fastlgamma taking size_t (include/fastlgamma.hpp:19-27): a negative int or long argument converts to
+ * 2^64 + x, whose lgamma the reference then takes; 0 gives +inf. */
+static double flg_size(int64_t x) {
+    if (x < 0) { int sign; return lgamma_r((double)(uint64_t)x, &sign); }
+    return orc_lgamma(x);
+}
+/* vterm_exact (include/entropy.hpp:114-135), directed */
+static double np_vterm_m(int dc, int64_t d_out, int64_t d_in, int64_t w) {
+    if (dc) return flg_size(d_out + 1) + flg_size(d_in + 1);
     if (w == 0) return 0.0;
     return (double)(d_out + d_in) * fastlog(w);
+}
+static double np_vterm(int64_t d_out, int64_t d_in, int64_t w) { return np_vterm_m(0, d_out, d_in, w); }
+
+/* spence (src/spence.cpp:94-155): the Cephes dilogarithm, with the published rational coefficients. */
+static const double SPENCE_A[8] = {
+    4.65128586073990045278E-5, 7.31589045238094711071E-3, 1.33847639578309018650E-1, 8.79691311754530315341E-1,
+    2.71149851196553469920E0,  4.25697156008121755724E0,  3.29771340985225106936E0,  1.00000000000000000126E0,
+};
+static const double SPENCE_B[8] = {
+    6.90990488912553276999E-4, 2.54043763932544379113E-2, 2.82974860602568089943E-1, 1.41172597751831069617E0,
+    3.63800533345137075418E0,  5.03278880143316990390E0,  3.54771340985225096217E0,  9.99999999999999998740E-1,
+};
+static double spence_poly(double x, const double *c) {
+    double ans = c[0];
+    for (int i = 1; i < 8; ++i) ans = ans * x + c[i];
+    return ans;
+}
+double orc_spence(double x) {
+    if (x < 0.0) return NAN;
+    if (x == 1.0) return 0.0;
+    if (x == 0.0) return M_PI * M_PI / 6.0;
+    int flag = 0;
+    double w;
+    if (x > 2.0) { x = 1.0 / x; flag |= 2; }
+    if (x > 1.5) { w = (1.0 / x) - 1.0; flag |= 2; }
+    else if (x < 0.5) { w = -x; flag |= 1; }
+    else w = x - 1.0;
+    double y = -w * spence_poly(w, SPENCE_A) / spence_poly(w, SPENCE_B);
+    if (flag & 1) y = (M_PI * M_PI) / 6.0 - log(x) * log1p(-x) - y;
+    if (flag & 2) { double z = log(x); y = -0.5 * z * z - y; }
+    return y;
+}
+/* get_v (src/entropy.cpp:676-686): fixed point of v = u sqrt(spence(exp(-v))) */
+static double dc_get_v(double u, double epsilon) {
+    double v = u, delta = 1;
+    while (delta > epsilon) {
+        double n_v = u * sqrt(orc_spence(exp(-v)));
+        delta = fabs(n_v - v);
+        v = n_v;
+    }
+    return v;
+}
+/* log_q (src/entropy.cpp:688-717), size_t arguments: log of the number of partitions of n into at most k
+ * parts, approximated. */
+double orc_log_q(uint64_t n, uint64_t k) {
+    if (n <= 0 || k < 1) return 0;
+    if (k > n) k = n;
+    if ((double)k < pow((double)n, 1 / 4.)) return orc_lbinom((int64_t)(n - 1), (int64_t)(k - 1)) - flg_size((int64_t)(k + 1));
+    double u = (double)k / sqrt((double)n);
+    double v = dc_get_v(u, 1e-8);
+    double lf = log(v) - log1p(-exp(-v) * (1 + u * u / 2)) / 2 - log(2) * 3 / 2. - log(u) - log(M_PI);
+    double g = 2 * v / u - u * log1p(-exp(-v));
+    return lf - log((double)n) + sqrt((double)n) * g;
+}
+/* get_delta_deg_dl_dist_change (src/entropy.cpp:969-1060), directed: the degree description-length change of
+ * `block` when vweight vertices of in-/out-degree (vkin, vkout) leave (diff -1) or join (+1) it.  The int
+ * and size_t conversions of the reference are kept. */
+static double dc_change(const orc_bm *bm, int32_t block, int64_t vkin, int64_t vkout, int64_t vweight, int diff) {
+    const int64_t total_r = bm->size[block], ep_r = bm->d_out[block], em_r = bm->d_in[block];
+    const int tkin = (int)vkin, tkout = (int)vkout, n = (int)vweight;
+    const int nd_in = (int)sv_get(&bm->hin[block], (int32_t)vkin), nd_out = (int)sv_get(&bm->hout[block], (int32_t)vkout);
+#define DC_SK(delta) ((0.0 - flg_size((int64_t)(nd_in + (delta) + 1))) - flg_size((int64_t)(nd_out + (delta) + 1)))
+#define DC_SE(delta, kin, kout) ((0.0 + orc_log_q((uint64_t)(em_r + (kin)), (uint64_t)(total_r + (delta)))) + \
+                                 orc_log_q((uint64_t)(ep_r + (kout)), (uint64_t)(total_r + (delta))))
+#define DC_SR(delta) (2 * flg_size(total_r + (delta) + 1))
+    double S_b = 0, S_a = 0;
+    S_b += DC_SK(0);
+    S_a += DC_SK((int)(diff * vweight));
+    S_b += DC_SE(0, 0, 0);
+    S_a += DC_SE(diff * n, diff * tkin, diff * tkout);
+    S_b += DC_SR(0);
+    S_a += DC_SR(diff * n);
+#undef DC_SK
+#undef DC_SE
+#undef DC_SR
+    return S_a - S_b;
+}
+/* get_delta_deg_dl (src/entropy.cpp:1063-1096 vertex, :1166-1181 merge) */
+static double dc_delta_deg_dl(const orc_bm *bm, int32_t r, int32_t s, int64_t vkin, int64_t vkout, int64_t weight) {
+    if (r == s) return 0.0;
+    double dS = 0;
+    dS += dc_change(bm, r, vkin, vkout, weight, -1);
+    dS += dc_change(bm, s, vkin, vkout, weight, +1);
+    return dS;
+}
+/* get_deg_dl_dist (src/entropy.cpp:719-742), directed: block by block, in-histogram then out-histogram
+ * entries in ascending degree. */
+double orc_np_deg_dl(const orc_bm *bm) {
+    if (!bm->dc) return 0.0;
+    double S = 0;
+    for (int32_t b = 0; b < bm->B; ++b) {
+        S += orc_log_q((uint64_t)bm->d_out[b], (uint64_t)bm->size[b]);
+        S += orc_log_q((uint64_t)bm->d_in[b], (uint64_t)bm->size[b]);
+        uint64_t total = 0;
+        for (int32_t i = 0; i < bm->hin[b].len; ++i) S -= flg_size(bm->hin[b].val[i] + 1);
+        for (int32_t i = 0; i < bm->hout[b].len; ++i) {
+            S -= flg_size(bm->hout[b].val[i] + 1);
+            total += (uint64_t)bm->hout[b].val[i];
+        }
+        S += 2 * flg_size((int64_t)(total + 1));
+    }
+    return S;
 }
 /* get_edges_dl (src/entropy.cpp:733-738) */
 double orc_np_edges_dl(int64_t B, int64_t E) { return orc_lbinom(B * B + E - 1, E); }
@@ -398,14 +553,17 @@ double orc_np_sparse_entropy(const orc_bm *bm) {
     fx128 acc = 0;
     for (int32_t r = 0; r < bm->B; ++r)
         for (int32_t i = 0; i < bm->row[r].len; ++i) acc += fx_from(np_eterm(bm->row[r].val[i]));
-    for (int32_t b = 0; b < bm->B; ++b) acc += fx_from(np_vterm(bm->d_out[b], bm->d_in[b], bm->size[b]));
+    for (int32_t b = 0; b < bm->B; ++b) acc += fx_from(np_vterm_m(bm->dc, bm->d_out[b], bm->d_in[b], bm->size[b]));
+    if (bm->dc)  /* get_deg_entropy (src/entropy.cpp:619-623) for every vertex */
+        for (int32_t v = 0; v < bm->N; ++v)
+            acc += fx_from(-flg_size(vtx_in(bm->g, v) + 1) - flg_size(vtx_out(bm->g, v) + 1));
     return fx_to(acc);
 }
 /* nonparametric::mdl (src/entropy.cpp:740-767) */
 double orc_np_mdl(const orc_bm *bm, const orc_graph *g) {
     double S = orc_np_sparse_entropy(bm), S_dl = 0;
     S_dl += orc_np_partition_dl(bm);
-    S_dl += 0.0;  /* get_deg_dl_dist: 0 without degree correction */
+    S_dl += orc_np_deg_dl(bm);
     S_dl += orc_np_edges_dl(np_nonempty(bm), g->E);
     return S + S_dl * 1.0;  /* BETA_DL = 1 (include/entropy.hpp:14) */
 }
@@ -495,7 +653,7 @@ static int64_t delta_get(const delta_t *d, int32_t row, int32_t col) {
 /* ------------------------------------------------------------------------- */
 /* Finetune: vertex proposals (src/common.cpp:138-198, src/finetune.cpp)      */
 /* ------------------------------------------------------------------------- */
-typedef struct { int32_t s; int64_t k_out, k_in, k; } proposal_t;
+typedef struct { int32_t s; int64_t k_out, k_in, k; int64_t vkin, vkout; } proposal_t;  /* vk*: graph degrees */
 
 /* Vertex-mode propose_new_block with the canonical multinomial. */
 static proposal_t propose_vertex(const orc_bm *bm, const orc_graph *g, int32_t v, const uint32_t w[4]) {
@@ -506,6 +664,8 @@ static proposal_t propose_vertex(const orc_bm *bm, const orc_graph *g, int32_t v
     for (int64_t e = g->in_off[v]; e < g->in_off[v + 1]; ++e) self_in += (g->in_adj[e] == v);
     p.k_out = ko;
     p.k_in = ki_all - self_in;
+    p.vkout = ko;
+    p.vkin = ki_all;
     p.k = p.k_out + p.k_in;
     if (p.k == 0) {
         p.s = random_block(r, bm->B, w[1]);
@@ -795,7 +955,8 @@ static void np_entries(const orc_bm *bm, const delta_t *d, fx128 *acc) {
 }
 /* get_delta_partition_dl + get_delta_deg_dl (0) + get_delta_edges_dl (src/entropy.cpp:865-1132) for
  * moving `weight` vertices from r (size_r) to s (size_s). */
-static double np_delta_dl(int64_t N, int64_t E, int64_t nonempty, int64_t size_r, int64_t size_s, int64_t weight) {
+static double np_delta_dl(int64_t N, int64_t E, int64_t nonempty, int64_t size_r, int64_t size_s, int64_t weight,
+                          double ddeg) {
     double S_b = 0, S_a = 0;
     S_b += -orc_lgamma(size_r + 1);
     S_a += -orc_lgamma(size_r - weight + 1);
@@ -818,13 +979,14 @@ static double np_delta_dl(int64_t N, int64_t E, int64_t nonempty, int64_t size_r
     double dedges = ea - eb;
     double dS_dl = 0;
     dS_dl += dpart;
-    dS_dl += 0.0;
+    dS_dl += ddeg;  /* get_delta_deg_dl: 0 without degree correction */
     dS_dl += dedges;
     return dS_dl;
 }
 /* virtual_move_sparse (src/entropy.cpp:802-862) + the description-length deltas: nonparametric::delta_mdl
  * (src/entropy.cpp:1135-1164) with weight 1, block_merge_delta_mdl (:1183-1206) with weight = size(r). */
-static double np_move_dS(const orc_bm *bm, const delta_t *d, int64_t k_out, int64_t k_in, int64_t weight) {
+static double np_move_dS(const orc_bm *bm, const delta_t *d, int64_t k_out, int64_t k_in, int64_t weight,
+                         int64_t vkin, int64_t vkout) {
     const int32_t r = d->r, s = d->s;
     if (r == s) return 0.0;
     int64_t E = 0;  /* the graph's edge count: every edge is one unit of some block's out-degree */
@@ -832,19 +994,21 @@ static double np_move_dS(const orc_bm *bm, const delta_t *d, int64_t k_out, int6
     fx128 acc = 0;
     np_entries(bm, d, &acc);
     double dS = fx_to(acc);
-    dS += np_vterm(bm->d_out[r] - k_out, bm->d_in[r] - k_in, bm->size[r] - weight);
-    dS -= np_vterm(bm->d_out[r], bm->d_in[r], bm->size[r]);
-    dS += np_vterm(bm->d_out[s] + k_out, bm->d_in[s] + k_in, bm->size[s] + weight);
-    dS -= np_vterm(bm->d_out[s], bm->d_in[s], bm->size[s]);
-    double dS_dl = np_delta_dl(bm->N, E, np_nonempty(bm), bm->size[r], bm->size[s], weight);
+    dS += np_vterm_m(bm->dc, bm->d_out[r] - k_out, bm->d_in[r] - k_in, bm->size[r] - weight);
+    dS -= np_vterm_m(bm->dc, bm->d_out[r], bm->d_in[r], bm->size[r]);
+    dS += np_vterm_m(bm->dc, bm->d_out[s] + k_out, bm->d_in[s] + k_in, bm->size[s] + weight);
+    dS -= np_vterm_m(bm->dc, bm->d_out[s], bm->d_in[s], bm->size[s]);
+    const double ddeg = bm->dc ? dc_delta_deg_dl(bm, r, s, vkin, vkout, weight) : 0.0;
+    double dS_dl = np_delta_dl(bm->N, E, np_nonempty(bm), bm->size[r], bm->size[s], weight, ddeg);
     return dS + 1.0 * dS_dl;
 }
 static double np_vertex_dS(const orc_bm *bm, const delta_t *d, const proposal_t *p) {
-    return np_move_dS(bm, d, p->k_out, p->k_in, 1);
+    return np_move_dS(bm, d, p->k_out, p->k_in, 1, p->vkin, p->vkout);
 }
+/* the merge's degree terms take block r's degrees as the "vertex" degrees (src/entropy.cpp:1170-1173) */
 static double np_merge_dS(const orc_bm *bm, const delta_t *d) {
     if (d->r == d->s || bm->size[d->r] == 0 || bm->size[d->s] == 0) return 0.0;
-    return np_move_dS(bm, d, bm->d_out[d->r], bm->d_in[d->r], bm->size[d->r]);
+    return np_move_dS(bm, d, bm->d_out[d->r], bm->d_in[d->r], bm->size[d->r], bm->d_in[d->r], bm->d_out[d->r]);
 }
 
 static int evaluate_vertex(const orc_bm *bm, const orc_graph *g, uint64_t seed, uint32_t epoch, int32_t v,
@@ -905,6 +1069,12 @@ static int commit_v3(orc_bm *bm, const orc_graph *g, int32_t v, int32_t s) {
     bm->assign[v] = s;
     bm->size[r]--;
     bm->size[s]++;
+    if (bm->dc) {  /* histograms keyed by the vertex's graph degrees (blockmodel.cpp:489-492) */
+        sv_add(&bm->hout[r], (int32_t)vtx_out(g, v), -1);
+        sv_add(&bm->hin[r], (int32_t)vtx_in(g, v), -1);
+        sv_add(&bm->hout[s], (int32_t)vtx_out(g, v), 1);
+        sv_add(&bm->hin[s], (int32_t)vtx_in(g, v), 1);
+    }
     return 1;
 }
 
@@ -933,6 +1103,12 @@ static void commit_delta(orc_bm *bm, int32_t v, const delta_t *d, const proposal
     bm->d_tot[s] = bm->d_out[s] + bm->d_in[s] - bm_get(bm, s, s);
     bm->size[r]--;
     bm->size[s]++;
+    if (bm->dc) {  /* keyed by the proposal's edge counts, in-edges without self-loops (blockmodel.cpp:431-434) */
+        sv_add(&bm->hout[r], (int32_t)p->k_out, -1);
+        sv_add(&bm->hin[r], (int32_t)p->k_in, -1);
+        sv_add(&bm->hout[s], (int32_t)p->k_out, 1);
+        sv_add(&bm->hin[s], (int32_t)p->k_in, 1);
+    }
 }
 
 /* Keyed shuffle: ascending Philox keys, ties by vertex id. */
@@ -983,7 +1159,7 @@ static void gibbs_batch(orc_bm *bm, const orc_graph *g, const orc_params *p, uin
 /* One sweep of finetune::asynchronous_gibbs (src/finetune.cpp:50-86). */
 int orc_async_gibbs_sweep(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t epoch, int32_t *proposed_s,
                           orc_stats *st) {
-    bm->np = p->nonparametric;
+    bm_model_from(bm, g, p);
     int32_t N = g->N;
     int32_t *order = NULL;
     if (!p->ordered) {
@@ -1008,12 +1184,12 @@ int orc_async_gibbs_sweep(orc_bm *bm, const orc_graph *g, const orc_params *p, u
 /* One sweep of sequential Metropolis-Hastings (src/finetune.cpp:793-803,822-869). */
 int orc_mh_sweep(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t epoch, int32_t ordered,
                  double *sum_dS, orc_stats *st) {
-    bm->np = p->nonparametric;
+    bm_model_from(bm, g, p);
     int32_t N = g->N;
     int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
     for (int32_t i = 0; i < N; ++i) order[i] = i;
     if (!ordered) keyed_shuffle(order, N, p->seed, epoch, ORC_TAG_SHUFFLE);
-    fx128 total = 0;
+    double total = 0;  /* accepted delta-MDL, added in visit order (src/finetune.cpp:795-800) */
     for (int32_t index = 0; index < N; ++index) {
         int32_t v = order[index];
         int32_t r = bm->assign[v];
@@ -1027,12 +1203,12 @@ int orc_mh_sweep(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t e
         if (acc) {
             commit_delta(bm, v, &d, &prop);
             delta_free(&d);
-            total += fx_from(dS);
+            total += dS;
             if (st) st->moves++;
         }
     }
     free(order);
-    if (sum_dS) *sum_dS = fx_to(total);
+    if (sum_dS) *sum_dS = total;
     if (st) st->sweeps++;
     return 0;
 }
@@ -1085,7 +1261,7 @@ static void split_by_degree(const orc_graph *g, double mh_percent, int32_t *hdv,
 /* finetune::mcmc dispatch (src/finetune.cpp:754-771) and the sweeps' outer loops. */
 int orc_mcmc(orc_bm *bm, const orc_graph *g, const orc_params *p, int32_t golden_ratio_not_reached, uint32_t *epoch,
              orc_stats *st) {
-    bm->np = p->nonparametric;
+    bm_model_from(bm, g, p);
     int32_t N = g->N;
     if (bm->B == 1) return 0;
     int algo = p->algorithm;
@@ -1167,7 +1343,7 @@ int orc_mcmc(orc_bm *bm, const orc_graph *g, const orc_params *p, int32_t golden
 
 /* finetune::finetune_assignment (src/finetune.cpp:903-932): ordered sequential MH over the full graph. */
 int orc_finetune_assignment(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t *epoch, orc_stats *st) {
-    bm->np = p->nonparametric;
+    bm_model_from(bm, g, p);
     int max_it = p->max_iterations > 0 ? p->max_iterations : 100;
     double *de = (double *)malloc(sizeof(double) * (size_t)max_it);
     int64_t nde = 0;
@@ -1410,7 +1586,7 @@ static orc_bm *finish_merge(orc_bm *bm, const orc_graph *g, const int32_t *block
     int32_t newB = B - (int32_t)merged;
     if (next > newB) newB = next;
     orc_bm *out = orc_bm_new(g, newB, assign);
-    out->np = bm->np;
+    orc_bm_set_model(out, g, bm->dc ? 2 : bm->np);
     out->overall_entropy = bm->overall_entropy;
     free(used);
     free(assign);
@@ -1497,6 +1673,10 @@ orc_bm *orc_merge_apply(orc_bm *bm, const orc_graph *g, const int32_t *best_s, c
             bm->d_tot[to] = bm->d_out[to] + bm->d_in[to] - self_to;
             bm->size[to] += bm->size[from];
             bm->size[from] = 0;
+            if (bm->dc) {  /* Blockmodel::merge_block adds the histograms of `from` to `to` (blockmodel.cpp:357-362) */
+                for (int32_t i = 0; i < bm->hin[from].len; ++i) sv_add(&bm->hin[to], bm->hin[from].idx[i], bm->hin[from].val[i]);
+                for (int32_t i = 0; i < bm->hout[from].len; ++i) sv_add(&bm->hout[to], bm->hout[from].idx[i], bm->hout[from].val[i]);
+            }
             merged++;
         }
         for (int32_t b = 0; b < B; ++b) {
@@ -1514,7 +1694,7 @@ orc_bm *orc_merge_apply(orc_bm *bm, const orc_graph *g, const int32_t *best_s, c
 
 /* block_merge::merge_blocks (src/block_merge.cpp:178-228). Consumes bm. */
 orc_bm *orc_merge_blocks(orc_bm *bm, const orc_graph *g, const orc_params *p, uint32_t epoch, orc_stats *st) {
-    bm->np = p->nonparametric;
+    bm_model_from(bm, g, p);
     int32_t B = bm->B;
     int32_t *best_s = (int32_t *)malloc(sizeof(int32_t) * (size_t)B);
     double *best_dS = (double *)malloc(sizeof(double) * (size_t)B);

@@ -88,6 +88,7 @@ typedef struct {
     int32_t max_iterations;   /* MAX_NUM_ITERATIONS (100) */
     int32_t threads;          /* OpenMP threads for the proposal loops */
     int32_t nonparametric;    /* --nonparametric: entropy::nonparametric description length */
+    int32_t degreecorrected;  /* --degreecorrected: with --nonparametric, the degree-corrected terms */
 } orc_params;
 
 typedef struct {
@@ -100,13 +101,18 @@ typedef struct {
     double  mdl;
 } orc_stats;
 
-/* Nonparametric description length (entropy::nonparametric), directed, not degree-corrected */
+/* Nonparametric description length (entropy::nonparametric), directed */
 double   orc_lgamma(int64_t x);
 double   orc_lbinom(int64_t n, int64_t k);
 double   orc_np_sparse_entropy(const orc_bm *bm);
 double   orc_np_partition_dl(const orc_bm *bm);
 double   orc_np_edges_dl(int64_t B, int64_t E);
 double   orc_np_mdl(const orc_bm *bm, const orc_graph *g);
+/* Degree correction (args.degreecorrected): spence (src/spence.cpp), log_q (src/entropy.cpp:688-717),
+ * get_deg_dl_dist (src/entropy.cpp:719-742) */
+double   orc_spence(double x);
+double   orc_log_q(uint64_t n, uint64_t k);
+double   orc_np_deg_dl(const orc_bm *bm);
 
 /* RNG / math primitives */
 void     orc_philox(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t out[4]);
@@ -125,7 +131,8 @@ int64_t  orc_bm_nnz(const orc_bm *bm);
 /* Exports row-major CSR (ascending columns) and the per-block arrays. Any pointer may be NULL. */
 void     orc_bm_export(const orc_bm *bm, int32_t *row_ptr, int32_t *cols, int32_t *cnts,
                        int64_t *d_out, int64_t *d_in, int64_t *d_tot, int32_t *sizes, int32_t *assignment);
-void     orc_bm_set_model(orc_bm *bm, int32_t nonparametric);
+/* model: 0 parametric, 1 nonparametric, 2 nonparametric degree-corrected (degree histograms from g) */
+void     orc_bm_set_model(orc_bm *bm, const orc_graph *g, int32_t model);
 int      orc_bm_validate(const orc_bm *bm, const orc_graph *g);
 
 /* Entropy */
