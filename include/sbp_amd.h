@@ -49,6 +49,7 @@ typedef struct {
     int32_t max_iterations;   /* MAX_NUM_ITERATIONS (include/finetune.hpp:34), default 100 */
     int32_t threads;          /* --threads: accepted for flag parity; the device ignores it */
     int32_t nonparametric;    /* --nonparametric (args.hpp:126-127): entropy::nonparametric description length */
+    int32_t degreecorrected;  /* --degreecorrected (args.hpp:87-88): with nonparametric, the degree-corrected terms */
 } sbp_params;
 
 /* Counters and phase timers (the reference's timers::*, include/globals.hpp:19-67). */
@@ -149,14 +150,17 @@ int sbp_bm_validate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *ok);
  * (src/blockmodel/blockmodel.cpp:288-306). */
 int sbp_mdl(sbp_ctx ctx, sbp_bm bm, sbp_graph g, double *mdl, double *log_posterior);
 
-/* Sets the blockmodel's entropy model, the reference's global args.nonparametric: 0 = entropy::mdl and its
- * deltas, 1 = entropy::nonparametric (src/entropy.cpp:617-1225; directed, not degree-corrected).  The
- * calls that take sbp_params set it from p->nonparametric; blockmodels made by merges inherit it. */
-int sbp_bm_set_model(sbp_bm bm, int32_t nonparametric);
-/* Replaces entropy::nonparametric::mdl with its parts sparse_entropy, get_partition_dl and get_edges_dl
- * (src/entropy.cpp:623-767); any output may be NULL. */
+/* Sets the blockmodel's entropy model, the reference's global args.nonparametric / args.degreecorrected:
+ * 0 = entropy::mdl and its deltas, 1 = entropy::nonparametric (src/entropy.cpp:617-1225; directed), 2 =
+ * entropy::nonparametric with degree correction, whose vertex-degree histograms are then built from the
+ * current assignment and g (Blockmodel::initialize_edge_counts, src/blockmodel/blockmodel.cpp:234-235) and
+ * kept up to date by the moves.  The calls that take sbp_params set it from p->nonparametric and
+ * p->degreecorrected; blockmodels made by merges inherit it. */
+int sbp_bm_set_model(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t model);
+/* Replaces entropy::nonparametric::mdl with its parts sparse_entropy, get_partition_dl, get_edges_dl and
+ * get_deg_dl_dist (src/entropy.cpp:623-767); any output may be NULL. */
 int sbp_np_description_length(sbp_ctx ctx, sbp_bm bm, sbp_graph g, double *mdl, double *sparse_entropy,
-                              double *partition_dl, double *edges_dl);
+                              double *partition_dl, double *edges_dl, double *degree_dl);
 
 /* ----------------------------------------------------------------- finetune */
 

@@ -115,7 +115,8 @@ void bm_alloc(sbp_bm bm, int32_t B, int32_t N) {
 
 void bm_release(sbp_bm bm) {
     void *ptrs[] = {bm->assign,    bm->sizes,     bm->d_out,   bm->d_in,    bm->d_tot,    bm->row_ptr,
-                    bm->col_ptr,   bm->row_cells, bm->col_cells, bm->row_cum, bm->col_cum, bm->cell_hash};
+                    bm->col_ptr,   bm->row_cells, bm->col_cells, bm->row_cum, bm->col_cum, bm->cell_hash,
+                    bm->hist};
     if (bm->pool) {
         for (void *p : ptrs) bm->pool->release(p);
     } else {
@@ -124,6 +125,9 @@ void bm_release(sbp_bm bm) {
     }
     bm->cell_hash = nullptr;
     bm->hash_valid = false;
+    bm->hist = nullptr;
+    bm->hist_log2 = 0;
+    bm->hist_ready = false;
     bm->assign = bm->sizes = bm->d_out = bm->d_in = bm->d_tot = bm->row_ptr = bm->col_ptr = nullptr;
     bm->row_cells = bm->col_cells = nullptr;
     bm->row_cum = bm->col_cum = nullptr;
@@ -233,9 +237,81 @@ void bm_ensure_hash(sbp_ctx ctx, sbp_bm bm) {
     bm->hash_valid = true;
 }
 
-void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm) {
+__global__ void k_hist_clear(int64_t n, sbpdev::HistSlot *__restrict__ t) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        t[i] = sbpdev::HistSlot{sbpdev::HIST_EMPTY, 0, 0};
+}
+__global__ void k_hist_build(int32_t N, const int32_t *__restrict__ assign, const int64_t *__restrict__ out_off,
+                             const int64_t *__restrict__ in_off, sbpdev::HistSlot *__restrict__ t, int shift,
+                             uint32_t mask) {
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        const int b = assign[v];
+        sbpdev::hist_add(t, shift, mask, sbpdev::hist_key(b, 0, out_off[v + 1] - out_off[v]), 1);
+        sbpdev::hist_add(t, shift, mask, sbpdev::hist_key(b, 1, in_off[v + 1] - in_off[v]), 1);
+    }
+}
+
+// Table size: at least 8N slots.  A table holds at most 4N live entries (non-zero counts: one per vertex and
+// direction, plus bins left below zero) and one sweep opens at most 2N keys, so compacting when half of the
+// slots are claimed keeps it under three quarters full.
+static int hist_log2_for(int64_t N) {
+    int lg = 10;
+    while ((int64_t(1) << lg) < 8 * N + 16) ++lg;
+    return lg;
+}
+static sbpdev::HistSlot *hist_alloc(sbp_bm bm, int lg) {
+    return static_cast<sbpdev::HistSlot *>(pool_of(bm).alloc(sizeof(sbpdev::HistSlot) * ((size_t(1) << lg) + 1)));
+}
+
+__global__ void k_hist_compact(int64_t n, const sbpdev::HistSlot *__restrict__ from, sbpdev::HistSlot *__restrict__ to,
+                               int shift, uint32_t mask) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const sbpdev::HistSlot e = from[i];
+        if (e.key != sbpdev::HIST_EMPTY && e.val != 0) sbpdev::hist_add(to, shift, mask, e.key, e.val);
+    }
+}
+
+void bm_hist_after_moves(sbp_ctx ctx, sbp_bm bm, int32_t claimed, int32_t full) {
+    SBP_CHECK(full == 0, SBP_E_STATE, "degree histogram table full");
+    const int64_t n = int64_t(1) << bm->hist_log2;
+    if (claimed <= n / 2) return;
+    // entries at zero count are dropped: a lookup of an absent bin gives 0
+    hipStream_t st = ctx->stream;
+    sbpdev::HistSlot *fresh = hist_alloc(bm, bm->hist_log2);
+    k_hist_clear<<<grid_for(n + 1), 256, 0, st>>>(n + 1, fresh);
+    k_hist_compact<<<grid_for(n), 256, 0, st>>>(n, bm->hist, fresh, 64 - bm->hist_log2, (uint32_t)(n - 1));
+    SBP_HIP(hipGetLastError());
+    pool_of(bm).release(bm->hist);
+    bm->hist = fresh;
+}
+
+void bm_build_hist(sbp_ctx ctx, sbp_graph g, sbp_bm bm) {
+    hipStream_t st = ctx->stream;
+    const int lg = hist_log2_for(bm->N);
+    if (!bm->hist || bm->hist_log2 != lg) {
+        pool_of(bm).release(bm->hist);
+        bm->hist = hist_alloc(bm, lg);
+        bm->hist_log2 = lg;
+    }
+    const int64_t n = int64_t(1) << lg;
+    k_hist_clear<<<grid_for(n + 1), 256, 0, st>>>(n + 1, bm->hist);
+    k_hist_build<<<grid_for(bm->N), 256, 0, st>>>(bm->N, bm->assign, g->out_off, g->in_off, bm->hist, 64 - lg,
+                                                  (uint32_t)(n - 1));
+    SBP_HIP(hipGetLastError());
+    bm->hist_ready = true;
+}
+
+void bm_set_model(sbp_ctx ctx, sbp_graph g, sbp_bm bm, int np, int dc) {
+    bm->np = np ? 1 : 0;
+    bm->dc = (np && dc) ? 1 : 0;
+    if (bm->dc && !bm->hist_ready) bm_build_hist(ctx, g, bm);
+}
+
+void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm, bool keep_hist) {
     hipStream_t st = ctx->stream;
     bm->hash_valid = false;
+    if (bm->dc && !keep_hist) bm_build_hist(ctx, g, bm);
+    if (!bm->dc) bm->hist_ready = false;
     const int32_t B = bm->B;
     const int64_t E = g->E;
     bm->E = E;
@@ -314,6 +390,18 @@ void bm_copy_into(sbp_ctx ctx, sbp_bm src, sbp_bm dst) {
     dst->num_to_merge = 0;
     dst->hash_valid = false;
     dst->np = src->np;
+    dst->dc = src->dc;
+    dst->hist_ready = false;
+    if (src->dc && src->hist_ready) {
+        if (!dst->hist || dst->hist_log2 != src->hist_log2) {
+            pool_of(dst).release(dst->hist);
+            dst->hist = hist_alloc(dst, src->hist_log2);
+            dst->hist_log2 = src->hist_log2;
+        }
+        SBP_HIP(hipMemcpyAsync(dst->hist, src->hist, sizeof(sbpdev::HistSlot) * ((size_t(1) << src->hist_log2) + 1),
+                               hipMemcpyDeviceToDevice, st));
+        dst->hist_ready = true;
+    }
     dst->E = src->E;
 }
 

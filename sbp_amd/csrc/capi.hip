@@ -48,6 +48,8 @@ static sbp_params default_params() {
     p.seed = 0;
     p.max_iterations = 100;
     p.threads = 1;
+    p.nonparametric = 0;
+    p.degreecorrected = 0;
     return p;
 }
 
@@ -284,6 +286,19 @@ static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, 
         const int32_t *ccp = reinterpret_cast<const int32_t *>(cc.data());
         model.load(B, rp.data(), rcp, rcp + 1, cp.data(), ccp, ccp + 1, 2, dout.data(), din.data(), pool.get());
         if (bm->np) model.set_nonparametric(sizes.data(), bm->N);
+        if (bm->dc) {
+            SBP_CHECK(bm->hist_ready, SBP_E_STATE, "degree histograms not built");
+            const int64_t n = int64_t(1) << bm->hist_log2;
+            std::vector<sbpdev::HistSlot> h((size_t)n);
+            SBP_HIP(hipMemcpy(h.data(), bm->hist, sizeof(sbpdev::HistSlot) * n, hipMemcpyDeviceToHost));
+            std::vector<uint64_t> keys((size_t)n);
+            std::vector<int32_t> counts((size_t)n);
+            for (int64_t i = 0; i < n; ++i) {
+                keys[i] = h[i].key;
+                counts[i] = h[i].val;
+            }
+            model.set_degree_corrected(keys.data(), counts.data(), n);
+        }
     }
     const double t_loaded = now_ms();
     sbpcarry::CarryStats cs;
@@ -308,6 +323,7 @@ static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, 
     sbp_bm out = new_bm(ctx);
     bm_alloc(out, new_B, bm->N);
     out->np = bm->np;
+    out->dc = bm->dc;  // the rebuild below builds its degree histograms afresh (initialize_edge_counts)
     relabel_assignment(ctx, bm, d_map, new_B, out->assign);
     bm_rebuild(ctx, g, out);
     SBP_HIP(hipStreamSynchronize(s));
@@ -603,22 +619,25 @@ int sbp_mdl(sbp_ctx ctx, sbp_bm bm, sbp_graph g, double *mdl, double *lp) {
     SBP_API_END(ctx)
 }
 
-int sbp_bm_set_model(sbp_bm bm, int32_t nonparametric) {
-    if (!bm) return SBP_E_ARG;
-    bm->np = nonparametric ? 1 : 0;
-    return SBP_OK;
+int sbp_bm_set_model(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t model) {
+    if (!ctx || !bm || !g || model < 0 || model > 2 || g->N != bm->N) return SBP_E_ARG;
+    SBP_API_BEGIN
+    bm_set_model(ctx, g, bm, model >= 1, model == 2);
+    SBP_HIP(hipStreamSynchronize(ctx->stream));
+    SBP_API_END(ctx)
 }
 
 int sbp_np_description_length(sbp_ctx ctx, sbp_bm bm, sbp_graph g, double *mdl, double *sparse_entropy,
-                              double *partition_dl, double *edges_dl) {
+                              double *partition_dl, double *edges_dl, double *degree_dl) {
     if (!ctx || !bm || !g) return SBP_E_ARG;
     SBP_API_BEGIN
-    double parts[3];
+    double parts[4];
     const double m = np_mdl(ctx, g, bm, parts);
     if (mdl) *mdl = m;
     if (sparse_entropy) *sparse_entropy = parts[0];
     if (partition_dl) *partition_dl = parts[1];
     if (edges_dl) *edges_dl = parts[2];
+    if (degree_dl) *degree_dl = parts[3];
     SBP_API_END(ctx)
 }
 
@@ -647,7 +666,7 @@ int sbp_async_gibbs_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params 
                           sbp_stats *stats) {
     if (!ctx || !bm || !g || !p || p->batches < 1) return SBP_E_ARG;
     SBP_API_BEGIN
-    bm->np = p->nonparametric ? 1 : 0;
+    bm_set_model(ctx, g, bm, p->nonparametric, p->nonparametric && p->degreecorrected);
     async_gibbs_sweep(ctx, bm, g, *p, epoch, stats);
     SBP_API_END(ctx)
 }
@@ -656,7 +675,7 @@ int sbp_mh_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, uint3
                  double *sum_dS, sbp_stats *stats) {
     if (!ctx || !bm || !g || !p) return SBP_E_ARG;
     SBP_API_BEGIN
-    bm->np = p->nonparametric ? 1 : 0;
+    bm_set_model(ctx, g, bm, p->nonparametric, p->nonparametric && p->degreecorrected);
     int32_t *ident = iota_of(ctx, g);
     const int32_t *ord = ident;
     if (!ordered) {
@@ -675,7 +694,7 @@ int sbp_mcmc(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, int32_t g
              uint32_t *epoch, double *mdl_out, sbp_stats *stats) {
     if (!ctx || !bm || !g || !p || !epoch || p->batches < 1) return SBP_E_ARG;
     SBP_API_BEGIN
-    bm->np = p->nonparametric ? 1 : 0;
+    bm_set_model(ctx, g, bm, p->nonparametric, p->nonparametric && p->degreecorrected);
     mcmc(ctx, bm, g, *p, golden_ratio_not_reached != 0, epoch, stats);
     if (mdl_out) *mdl_out = bm->overall_entropy;
     SBP_API_END(ctx)
@@ -685,7 +704,7 @@ int sbp_finetune_assignment(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_param
                             double *mdl_out, sbp_stats *stats) {
     if (!ctx || !bm || !g || !p || !epoch) return SBP_E_ARG;
     SBP_API_BEGIN
-    bm->np = p->nonparametric ? 1 : 0;
+    bm_set_model(ctx, g, bm, p->nonparametric, p->nonparametric && p->degreecorrected);
     finetune_assignment(ctx, bm, g, *p, epoch, stats);
     if (mdl_out) *mdl_out = bm->overall_entropy;
     SBP_API_END(ctx)

@@ -244,18 +244,109 @@ inline double np_vterm(int64_t d_out, int64_t d_in, int64_t w) {
 }
 inline double np_edges_dl(int64_t B, int64_t E) { return flbinom(B * B + E - 1, E); }
 
+// Degree correction (args.degreecorrected), with the host library's math as the reference uses it.
+// fastlgamma of a size_t argument: a negative int or long converts to 2^64 + x; 0 gives +inf.
+inline double flg_size(int64_t x) {
+    if (x < 0) {
+        int sign;
+        return lgamma_r((double)(uint64_t)x, &sign);
+    }
+    return flgamma(x);
+}
+inline double np_vterm_dc(int64_t d_out, int64_t d_in) { return flg_size(d_out + 1) + flg_size(d_in + 1); }
+// spence (src/spence.cpp:94-155): the Cephes dilogarithm with its published rational coefficients
+inline double dc_spence(double x) {
+    static const double A[8] = {4.65128586073990045278E-5, 7.31589045238094711071E-3, 1.33847639578309018650E-1,
+                                8.79691311754530315341E-1, 2.71149851196553469920E0,  4.25697156008121755724E0,
+                                3.29771340985225106936E0,  1.00000000000000000126E0};
+    static const double Bc[8] = {6.90990488912553276999E-4, 2.54043763932544379113E-2, 2.82974860602568089943E-1,
+                                 1.41172597751831069617E0,  3.63800533345137075418E0,  5.03278880143316990390E0,
+                                 3.54771340985225096217E0,  9.99999999999999998740E-1};
+    if (x < 0.0) return NAN;
+    if (x == 1.0) return 0.0;
+    if (x == 0.0) return M_PI * M_PI / 6.0;
+    int flag = 0;
+    double w;
+    if (x > 2.0) {
+        x = 1.0 / x;
+        flag |= 2;
+    }
+    if (x > 1.5) {
+        w = (1.0 / x) - 1.0;
+        flag |= 2;
+    } else if (x < 0.5) {
+        w = -x;
+        flag |= 1;
+    } else {
+        w = x - 1.0;
+    }
+    double pa = A[0], pb = Bc[0];
+    for (int i = 1; i < 8; ++i) {
+        pa = pa * w + A[i];
+        pb = pb * w + Bc[i];
+    }
+    double y = -w * pa / pb;
+    if (flag & 1) y = (M_PI * M_PI) / 6.0 - log(x) * log1p(-x) - y;
+    if (flag & 2) {
+        const double z = log(x);
+        y = -0.5 * z * z - y;
+    }
+    return y;
+}
+// log_q (src/entropy.cpp:676-717) with size_t arguments
+inline double dc_log_q(uint64_t n, uint64_t k) {
+    if (n <= 0 || k < 1) return 0;
+    if (k > n) k = n;
+    if ((double)k < pow((double)n, 1 / 4.)) return flbinom((int64_t)(n - 1), (int64_t)(k - 1)) - flg_size((int64_t)(k + 1));
+    const double u = (double)k / sqrt((double)n);
+    double v = u, delta = 1;
+    while (delta > 1e-8) {
+        const double n_v = u * sqrt(dc_spence(exp(-v)));
+        delta = fabs(n_v - v);
+        v = n_v;
+    }
+    const double lf = log(v) - log1p(-exp(-v) * (1 + u * u / 2)) / 2 - log(2) * 3 / 2. - log(u) - log(M_PI);
+    const double g = 2 * v / u - u * log1p(-exp(-v));
+    return lf - log((double)n) + sqrt((double)n) * g;
+}
+// get_delta_deg_dl_dist_change (src/entropy.cpp:969-1060), directed; nd_in / nd_out are block's histogram
+// counts at degrees vkin / vkout.
+inline double dc_change(int64_t size, int64_t dout, int64_t din, int nd_in, int nd_out, int64_t vkin, int64_t vkout,
+                        int64_t vweight, int diff) {
+    const int64_t total_r = size, ep_r = dout, em_r = din;
+    const int tkin = (int)vkin, tkout = (int)vkout, n = (int)vweight;
+    auto sk = [&](int delta) {
+        return (0.0 - flg_size((int64_t)(nd_in + delta + 1))) - flg_size((int64_t)(nd_out + delta + 1));
+    };
+    auto se = [&](int64_t delta, int64_t kin, int64_t kout) {
+        return (0.0 + dc_log_q((uint64_t)(em_r + kin), (uint64_t)(total_r + delta))) +
+               dc_log_q((uint64_t)(ep_r + kout), (uint64_t)(total_r + delta));
+    };
+    auto sr = [&](int64_t delta) { return 2 * flg_size(total_r + delta + 1); };
+    double S_b = 0, S_a = 0;
+    S_b += sk(0);
+    S_a += sk((int)(diff * vweight));
+    S_b += se(0, 0, 0);
+    S_a += se(diff * n, diff * tkin, diff * tkout);
+    S_b += sr(0);
+    S_a += sr(diff * n);
+    return S_a - S_b;
+}
+
 // The block terms and description-length deltas of nonparametric::block_merge_delta_mdl
 // (src/entropy.cpp:1183-1206; virtual_move_sparse :836-859, get_delta_partition_dl :865-967,
 // get_delta_edges_dl :1098-1132) around the Delta entries' sum, for `nonempty` non-empty blocks -- the same
 // operations in the same order as the device kernels.
 inline double np_merge_finish(double entries, int64_t dout_r, int64_t din_r, int64_t size_r, int64_t dout_s,
-                              int64_t din_s, int64_t size_s, int64_t N, int64_t E, int64_t nonempty) {
+                              int64_t din_s, int64_t size_s, int64_t N, int64_t E, int64_t nonempty, bool dc = false,
+                              double ddeg = 0.0) {
     const int64_t k_out = dout_r, k_in = din_r, weight = size_r;
+    auto vt = [dc](int64_t o, int64_t i, int64_t w) { return dc ? np_vterm_dc(o, i) : np_vterm(o, i, w); };
     double dS = entries;
-    dS += np_vterm(dout_r - k_out, din_r - k_in, size_r - weight);
-    dS -= np_vterm(dout_r, din_r, size_r);
-    dS += np_vterm(dout_s + k_out, din_s + k_in, size_s + weight);
-    dS -= np_vterm(dout_s, din_s, size_s);
+    dS += vt(dout_r - k_out, din_r - k_in, size_r - weight);
+    dS -= vt(dout_r, din_r, size_r);
+    dS += vt(dout_s + k_out, din_s + k_in, size_s + weight);
+    dS -= vt(dout_s, din_s, size_s);
     double S_b = 0, S_a = 0;
     S_b += -flgamma(size_r + 1);
     S_a += -flgamma(size_r - weight + 1);
@@ -278,7 +369,7 @@ inline double np_merge_finish(double entries, int64_t dout_r, int64_t din_r, int
     const double dedges = ea - eb;
     double dS_dl = 0;
     dS_dl += dpart;
-    dS_dl += 0.0;
+    dS_dl += ddeg;  // get_delta_deg_dl: 0 without degree correction
     dS_dl += dedges;
     return dS + 1.0 * dS_dl;
 }
@@ -396,6 +487,22 @@ struct Model {
     bool np = false;
     std::vector<int64_t> size;
     int64_t N = 0, E = 0, nonempty = 0;
+    // degree-corrected: per block, vertex in-/out-degree -> count (histograms), from (key, count) slots with
+    // key = block << 33 | direction << 32 | degree (direction 1 = in; empty slots have key ~0)
+    bool dc = false;
+    std::vector<FlatMap> hin, hout;
+    void set_degree_corrected(const uint64_t *keys, const int32_t *counts, int64_t n) {
+        dc = true;
+        hin.clear();
+        hout.clear();
+        hin.resize(B);
+        hout.resize(B);
+        for (int64_t i = 0; i < n; ++i) {
+            if (keys[i] == ~0ull || counts[i] == 0) continue;
+            const int32_t b = (int32_t)(keys[i] >> 33), deg = (int32_t)(uint32_t)keys[i];
+            ((keys[i] >> 32) & 1 ? hin : hout)[b].add(deg, counts[i]);
+        }
+    }
     void set_nonparametric(const int32_t *sizes, int64_t num_vertices) {
         np = true;
         size.assign(sizes, sizes + B);
@@ -582,10 +689,20 @@ inline double np_merge_entries(const Model &m, int32_t r, int32_t s) {
     if (dss > 0) acc.add(np_entry(m.get(s, s), dss));
     return acc.value();
 }
+// With degree correction the merge's degree terms take block r's degrees as the vertex degrees
+// (src/entropy.cpp:1166-1181), looked up in the histograms of r and s.
 inline double np_merge_dS(const Model &m, int32_t r, int32_t s, double entries) {
     if (m.size[r] == 0 || m.size[s] == 0) return 0.0;
+    double ddeg = 0.0;
+    if (m.dc && r != s) {
+        const int64_t vkin = m.d_in[r], vkout = m.d_out[r], w = m.size[r];
+        ddeg += dc_change(m.size[r], m.d_out[r], m.d_in[r], (int)m.hin[r].get((int32_t)vkin),
+                          (int)m.hout[r].get((int32_t)vkout), vkin, vkout, w, -1);
+        ddeg += dc_change(m.size[s], m.d_out[s], m.d_in[s], (int)m.hin[s].get((int32_t)vkin),
+                          (int)m.hout[s].get((int32_t)vkout), vkin, vkout, w, +1);
+    }
     return np_merge_finish(entries, m.d_out[r], m.d_in[r], m.size[r], m.d_out[s], m.d_in[s], m.size[s], m.N, m.E,
-                           m.nonempty);
+                           m.nonempty, m.dc, ddeg);
 }
 
 // Renames key `from` to `to` in the maps `maps[ent[i].key]`, i < n, which hold count ent[i].val under `from`,
@@ -696,6 +813,10 @@ inline void merge_apply(Model &m, int32_t r, int32_t s, Pool *pool = nullptr,
         m.size[s] += m.size[r];
         m.size[r] = 0;
         m.nonempty--;
+    }
+    if (m.dc) {  // Blockmodel::merge_block adds r's histograms to s's (src/blockmodel/blockmodel.cpp:357-362)
+        m.hin[r].each([&](int32_t deg, int64_t c) { m.hin[s].add(deg, c); });
+        m.hout[r].each([&](int32_t deg, int64_t c) { m.hout[s].add(deg, c); });
     }
 }
 

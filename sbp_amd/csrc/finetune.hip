@@ -82,6 +82,7 @@ struct FtArgs {
     int32_t *big_queue;    // high-degree vertices deferred by the LDS-table launch
     int32_t *big_count;
     NpConst np;            // nonparametric description-length constants (nonparametric runs only)
+    HistSlot *hist;        // degree-corrected runs: the histograms, updated in place by Metropolis-Hastings
 };
 
 __device__ __forceinline__ long long get_cell(const FtArgs &a, int row, int col) {
@@ -440,8 +441,8 @@ __device__ double evaluate_move_np(const FtArgs &a, MoveCtx &m, const Table &t, 
         if (m.out_r == 0 && m.in_s > 0) fx_acc(acc, np_entry(a.np, msr, -(long long)m.in_s));
     }
     acc = wave_sum_fx(acc);
-    const double dS = np_finish(a.np, fx_to(acc), a.d_out[r], a.d_in[r], a.sizes[r], a.d_out[s], a.d_in[s],
-                                a.sizes[s], m.k_out, m.k_in, 1);
+    const double dS = np_finish(a.np, fx_to(acc), r, s, a.d_out[r], a.d_in[r], a.sizes[r], a.d_out[s], a.d_in[s],
+                                a.sizes[s], m.k_out, m.k_in, 1, m.i1 - m.i0, m.o1 - m.o0);
     if (GREEDY) {
         accept = dS < 0.0;
     } else {
@@ -558,7 +559,12 @@ static FtArgs base_args(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     a.big_queue = nullptr;
     a.big_count = nullptr;
     a.np = NpConst{};
-    if (bm->np) a.np = np_const(ctx, bm);
+    a.hist = nullptr;
+    if (bm->np) {
+        bm_set_model(ctx, g, bm, bm->np, bm->dc);
+        a.np = np_const(ctx, bm);
+        if (bm->dc) a.hist = bm->hist;
+    }
     return a;
 }
 
@@ -716,6 +722,23 @@ __global__ void k_resolve(int32_t B, const int32_t *__restrict__ risk, const int
     }
 }
 
+// Blockmodel::move_vertex(const VertexMove_v3&) (src/blockmodel/blockmodel.cpp:489-492): the committed moves'
+// histogram entries, keyed by the vertex's out- and in-degree.  Runs before k_apply (reads the old blocks).
+__global__ void k_hist_moves(int32_t N, const int32_t *__restrict__ move_s, const int32_t *__restrict__ acc,
+                             const int32_t *__restrict__ assign, const int64_t *__restrict__ out_off,
+                             const int64_t *__restrict__ in_off, HistSlot *__restrict__ t, int shift, uint32_t mask) {
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        const int s = move_s[v];
+        if (s < 0 || (acc != nullptr && !acc[v])) continue;
+        const int r = assign[v];
+        const long long ko = out_off[v + 1] - out_off[v], ki = in_off[v + 1] - in_off[v];
+        hist_add(t, shift, mask, hist_key(r, 0, ko), -1);
+        hist_add(t, shift, mask, hist_key(r, 1, ki), -1);
+        hist_add(t, shift, mask, hist_key(s, 0, ko), 1);
+        hist_add(t, shift, mask, hist_key(s, 1, ki), 1);
+    }
+}
+
 __global__ void k_apply(int32_t N, const int32_t *__restrict__ move_s, const int32_t *__restrict__ acc,
                         int32_t *__restrict__ assign, unsigned long long *__restrict__ moved) {
     unsigned long long local = 0;
@@ -780,12 +803,21 @@ int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_
         }
         acc_final = prev;
     }
+    if (bm->dc)
+        k_hist_moves<<<grid_of(N), 256, 0, st>>>(N, d_move_s, acc_final, bm->assign, g->out_off, g->in_off, bm->hist,
+                                                 64 - bm->hist_log2, (uint32_t)((1ull << bm->hist_log2) - 1));
     k_apply<<<grid_of(N), 256, 0, st>>>(N, d_move_s, acc_final, bm->assign, counters);
     SBP_HIP(hipGetLastError());
     unsigned long long c[2] = {0, 0};
-    read_small(ctx, {{&c[0], counters, sizeof(unsigned long long)}, {&c[1], counters + 2, sizeof(unsigned long long)}});
+    HistSlot book{0, 0, 0};
+    if (bm->dc)
+        read_small(ctx, {{&c[0], counters, sizeof(unsigned long long)}, {&c[1], counters + 2, sizeof(unsigned long long)},
+                         {&book, bm->hist + (size_t(1) << bm->hist_log2), sizeof(HistSlot)}});
+    else
+        read_small(ctx, {{&c[0], counters, sizeof(unsigned long long)}, {&c[1], counters + 2, sizeof(unsigned long long)}});
+    if (bm->dc) bm_hist_after_moves(ctx, bm, book.val, book.pad);
     if (refused_out) *refused_out = (int64_t)c[1] - (int64_t)c[0];
-    bm_rebuild(ctx, g, bm);
+    bm_rebuild(ctx, g, bm, true);
     return (int64_t)c[0];
 }
 
@@ -827,7 +859,7 @@ struct MhState {
     int32_t *stop_at;   // first order index not processed (== count when done)
     int32_t *status;    // 0 done, 1 out of slack
     unsigned long long *moves;
-    Fx *sum;            // fixed-point sum of accepted delta-MDL
+    double *sum;        // running sum of accepted delta-MDL in visit order, carried across resumed launches
 };
 
 // Inserts/updates/erases one cell in a sorted slack row: cells[beg, beg+len), capacity cap.
@@ -893,7 +925,7 @@ __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start)
     __shared__ int lds[table_ints(LDS_CAP_LOG2)];
     const int lane = lane_id();
     unsigned long long bytes = 0, moves = 0;
-    Fx sum = fx_zero();
+    double sum = *h.sum;
     int32_t status = 0, stop = (int32_t)a.count;
     const Table small_t = make_table(a, false, lds, 0);
     const Table big_t = make_table(a, true, lds, 0);
@@ -978,6 +1010,12 @@ __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start)
             a.sizes[r] -= 1;
             a.sizes[s] += 1;
             a.assign[v] = s;
+            if (NP && a.hist) {  // keyed by the proposal's edge counts (src/blockmodel/blockmodel.cpp:431-434)
+                hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(r, 0, m.k_out), -1);
+                hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(r, 1, m.k_in), -1);
+                hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(s, 0, m.k_out), 1);
+                hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(s, 1, m.k_in), 1);
+            }
         }
         wave_sync();
         if (lane == 0) {
@@ -985,7 +1023,7 @@ __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start)
             a.d_tot[s] = a.d_out[s] + a.d_in[s] - (int)get_cell(a, s, s);
         }
         wave_sync();
-        fx_add(sum, fx_from(dS));
+        sum += dS;
         moves++;
         bytes += 12ull * (unsigned long long)kall;
         if (occupies) {  // the host refreshes the constants and resumes with the next vertex
@@ -998,9 +1036,7 @@ __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start)
         *h.status = status;
         *h.stop_at = stop;
         atomicAdd(h.moves, moves);
-        Fx old = *h.sum;
-        fx_add(old, sum);
-        *h.sum = old;
+        *h.sum = sum;
         if (bytes) atomicAdd(a.bytes, bytes);
     }
 }
@@ -1050,7 +1086,7 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
     h.stop_at = book;
     h.status = book + 1;
     h.moves = reinterpret_cast<unsigned long long *>(book + 4);
-    h.sum = reinterpret_cast<Fx *>(book + 8);
+    h.sum = reinterpret_cast<double *>(book + 8);
     auto *bytes = ctx->scratch[S_COUNTER].as<unsigned long long>(4);
     SBP_HIP(hipMemsetAsync(book, 0, 64 * sizeof(int32_t), st));
     SBP_HIP(hipMemsetAsync(bytes, 0, 4 * sizeof(unsigned long long), st));
@@ -1135,12 +1171,18 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
     struct {
         unsigned long long moves;
         unsigned long long pad;
-        Fx sum;
+        double sum;
     } res;
     unsigned long long hb[2] = {0, 0};
-    read_small(ctx, {{&res, book + 4, sizeof(res)}, {hb, bytes, sizeof(hb)}});
+    HistSlot hbook{0, 0, 0};
+    if (bm->dc)
+        read_small(ctx, {{&res, book + 4, sizeof(res)}, {hb, bytes, sizeof(hb)},
+                         {&hbook, bm->hist + (size_t(1) << bm->hist_log2), sizeof(HistSlot)}});
+    else
+        read_small(ctx, {{&res, book + 4, sizeof(res)}, {hb, bytes, sizeof(hb)}});
+    if (bm->dc) bm_hist_after_moves(ctx, bm, hbook.val, hbook.pad);
     SBP_CHECK(hb[1] == 0, SBP_E_STATE, "Metropolis-Hastings: block degrees inconsistent with the block matrix");
-    if (sum_dS) *sum_dS = (double)res.sum.hi + (double)res.sum.lo * 0x1p-64;
+    if (sum_dS) *sum_dS = res.sum;
     if (stats) {
         stats->proposals += count;
         stats->moves += (int64_t)res.moves;

@@ -215,6 +215,10 @@ struct sbp_bm_s {
     double overall_entropy = 1.7976931348623157e308;
     int64_t num_to_merge = 0;
     int32_t np = 0;  // entropy model: 0 parametric (entropy::mdl), 1 nonparametric (entropy::nonparametric)
+    int32_t dc = 0;  // nonparametric with degree correction: the degree histograms below are kept
+    sbpdev::HistSlot *hist = nullptr;  // dc: (block, direction, vertex degree) -> vertex count
+    int hist_log2 = 0;
+    bool hist_ready = false;  // dc: built from the current assignment (and kept up to date by moves)
     int64_t E = 0;   // edges of the graph the blockmodel was built on
     // cell hash for O(1) lookups (built on demand; any change to the cells invalidates it)
     sbpdev::CellSlot *cell_hash = nullptr;
@@ -227,9 +231,18 @@ namespace sbpimpl {
 // build.hip
 void bm_alloc(sbp_bm bm, int32_t B, int32_t N);
 void bm_release(sbp_bm bm);
-void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm);  // from bm->assign (device), bm->B set
+// From bm->assign (device), bm->B set.  The degree histograms of a degree-corrected blockmodel are rebuilt too,
+// unless keep_hist (a finetune commit, which updates them move by move as the reference does).
+void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm, bool keep_hist = false);
 void bm_copy_into(sbp_ctx ctx, sbp_bm src, sbp_bm dst);
 void bm_compute_cum(sbp_ctx ctx, sbp_bm bm);
+// Degree histograms (initialize_edge_counts, src/blockmodel/blockmodel.cpp:234-235) from the assignment and g.
+void bm_build_hist(sbp_ctx ctx, sbp_graph g, sbp_bm bm);
+// After a batch of moves: fails on a full table, compacts it when half of its slots are claimed (claimed and
+// full are the table's bookkeeping slot, read with the batch's other counters).
+void bm_hist_after_moves(sbp_ctx ctx, sbp_bm bm, int32_t claimed, int32_t full);
+// Sets the entropy model (args.nonparametric / args.degreecorrected), building the histograms when needed.
+void bm_set_model(sbp_ctx ctx, sbp_graph g, sbp_bm bm, int np, int dc);
 void bm_ensure_hash(sbp_ctx ctx, sbp_bm bm);
 void bm_ensure_cells(sbp_bm bm, int64_t nnz);
 // mdl.hip

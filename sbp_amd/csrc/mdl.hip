@@ -159,7 +159,16 @@ int64_t count_nonempty(sbp_ctx ctx, sbp_bm bm) {
 
 // Constants of one launch: the lgamma table and the block-count terms around the current non-empty count.
 NpConst np_const(sbp_ctx ctx, sbp_bm bm) {
-    NpConst c;
+    NpConst c{};
+    c.dc = bm->dc;
+    if (bm->dc) {
+        SBP_CHECK(bm->hist_ready, SBP_E_STATE, "degree histograms not built");
+        c.hist = bm->hist;
+        c.hist_shift = 64 - bm->hist_log2;
+        c.hist_mask = (uint32_t)((1ull << bm->hist_log2) - 1);
+    }
+    c.ln2 = log(2.0);
+    c.lnpi = log(M_PI);
     const int64_t N = bm->N, E = bm->E;
     c.lg = lgamma_table(ctx, std::max<int64_t>(E, N) + 2, &c.n);
     const int64_t ne = count_nonempty(ctx, bm);
@@ -171,21 +180,28 @@ NpConst np_const(sbp_ctx ctx, sbp_bm bm) {
 }
 
 // sparse_entropy (src/entropy.cpp:623-655): eterm over the cells + vterm over the blocks, fixed point
+// and, degree-corrected, get_deg_entropy (src/entropy.cpp:619-623) over the N vertices
 __global__ void __launch_bounds__(MDL_BLOCK) k_np_sparse(int64_t nnz, int32_t B, const int2 *__restrict__ cells,
                                                          const int32_t *__restrict__ d_out,
                                                          const int32_t *__restrict__ d_in,
-                                                         const int32_t *__restrict__ sizes, NpConst c,
+                                                         const int32_t *__restrict__ sizes, int32_t N,
+                                                         const int64_t *__restrict__ out_off,
+                                                         const int64_t *__restrict__ in_off, NpConst c,
                                                          Fx *__restrict__ partials) {
     __shared__ Fx wave_part[MDL_BLOCK / WAVE];
     const int lane = lane_id(), wid = threadIdx.x / WAVE;
     Fx acc = fx_zero();
     const int64_t stride = (int64_t)gridDim.x * MDL_BLOCK;
-    for (int64_t i = blockIdx.x * (int64_t)MDL_BLOCK + threadIdx.x; i < nnz + B; i += stride) {
+    const int64_t total = nnz + B + (c.dc ? N : 0);
+    for (int64_t i = blockIdx.x * (int64_t)MDL_BLOCK + threadIdx.x; i < total; i += stride) {
         if (i < nnz) {
             fx_acc(acc, np_eterm(c, cells[i].y));
-        } else {
+        } else if (i < nnz + B) {
             const int32_t b = (int32_t)(i - nnz);
-            fx_acc(acc, np_vterm(d_out[b], d_in[b], sizes[b]));
+            fx_acc(acc, np_vterm(c, d_out[b], d_in[b], sizes[b]));
+        } else {
+            const int32_t v = (int32_t)(i - nnz - B);
+            fx_acc(acc, -np_flg(c, in_off[v + 1] - in_off[v] + 1) - np_flg(c, out_off[v + 1] - out_off[v] + 1));
         }
     }
     acc = wave_sum_fx(acc);
@@ -198,16 +214,65 @@ __global__ void __launch_bounds__(MDL_BLOCK) k_np_sparse(int64_t nnz, int32_t B,
     }
 }
 
-// nonparametric::mdl (src/entropy.cpp:740-767).  parts (optional): sparse entropy, partition dl, edges dl.
+// get_deg_dl_dist (src/entropy.cpp:719-742), directed: per block log_q(d_out, size) + log_q(d_in, size) +
+// 2 lgamma(size + 1) (the out-histogram of a block counts each of its vertices once), and -lgamma(count + 1)
+// per histogram entry.  Summed in double (a count that has gone to -1 gives -inf, as in the reference).
+__global__ void __launch_bounds__(MDL_BLOCK) k_np_deg_dl(int32_t B, const int32_t *__restrict__ d_out,
+                                                         const int32_t *__restrict__ d_in,
+                                                         const int32_t *__restrict__ sizes, int64_t nslots,
+                                                         NpConst c, double *__restrict__ partials) {
+    __shared__ double wave_part[MDL_BLOCK / WAVE];
+    const int lane = lane_id(), wid = threadIdx.x / WAVE;
+    double acc = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * MDL_BLOCK;
+    for (int64_t i = blockIdx.x * (int64_t)MDL_BLOCK + threadIdx.x; i < B + nslots; i += stride) {
+        if (i < B) {
+            const int32_t b = (int32_t)i;
+            acc += dc_log_q(c, (unsigned long long)d_out[b], (unsigned long long)sizes[b]);
+            acc += dc_log_q(c, (unsigned long long)d_in[b], (unsigned long long)sizes[b]);
+            acc += 2 * np_flg(c, (long long)sizes[b] + 1);
+        } else {
+            const HistSlot e = c.hist[i - B];
+            if (e.key != HIST_EMPTY && e.val != 0) acc -= np_flg(c, (long long)e.val + 1);
+        }
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) wave_part[wid] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < MDL_BLOCK / WAVE; ++w) t += wave_part[w];
+        partials[blockIdx.x] = t;
+    }
+}
+
+static double np_deg_dl(sbp_ctx ctx, sbp_bm bm, const NpConst &c) {
+    hipStream_t st = ctx->stream;
+    const int64_t nslots = int64_t(1) << bm->hist_log2, total = bm->B + nslots;
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((total + MDL_BLOCK - 1) / MDL_BLOCK, 1024));
+    double *partials = reinterpret_cast<double *>(ctx->scratch[S_NP].as<double>((size_t)blocks + 4));
+    k_np_deg_dl<<<(int)blocks, MDL_BLOCK, 0, st>>>(bm->B, bm->d_out, bm->d_in, bm->sizes, nslots, c, partials);
+    SBP_HIP(hipGetLastError());
+    std::vector<double> h((size_t)blocks);
+    SBP_HIP(hipMemcpyAsync(h.data(), partials, sizeof(double) * blocks, hipMemcpyDeviceToHost, st));
+    SBP_HIP(hipStreamSynchronize(st));
+    double S = 0.0;
+    for (double x : h) S += x;
+    return S;
+}
+
+// nonparametric::mdl (src/entropy.cpp:740-767).  parts (optional): sparse entropy, partition dl, edges dl,
+// degree dl.
 double np_mdl(sbp_ctx ctx, sbp_graph g, sbp_bm bm, double *parts) {
     hipStream_t st = ctx->stream;
+    bm_set_model(ctx, g, bm, bm->np, bm->dc);
     const NpConst c = np_const(ctx, bm);
-    const int64_t total = bm->nnz + bm->B;
+    const int64_t total = bm->nnz + bm->B + (bm->dc ? (int64_t)g->N : 0);
     int64_t blocks = std::min<int64_t>((total + MDL_BLOCK - 1) / MDL_BLOCK, 4096);
     blocks = std::max<int64_t>(blocks, 1);
     Fx *partials = ctx->scratch[S_PARTIAL].as<Fx>(blocks + 2);
-    k_np_sparse<<<(int)blocks, MDL_BLOCK, 0, st>>>(bm->nnz, bm->B, bm->row_cells, bm->d_out, bm->d_in, bm->sizes, c,
-                                                     partials);
+    k_np_sparse<<<(int)blocks, MDL_BLOCK, 0, st>>>(bm->nnz, bm->B, bm->row_cells, bm->d_out, bm->d_in, bm->sizes,
+                                                     g->N, g->out_off, g->in_off, c, partials);
     k_sum_partials<<<1, MDL_BLOCK, 0, st>>>((int)blocks, partials, partials + blocks);
     SBP_HIP(hipGetLastError());
     Fx h;
@@ -226,14 +291,16 @@ double np_mdl(sbp_ctx ctx, sbp_graph g, sbp_bm bm, double *parts) {
     for (int32_t b = 0; b < bm->B; ++b) part -= host_lgamma(sizes[b] + 1);
     part += (double)logf((float)N);  // fastlog(N)
     const double edges = host_edges_dl(ne, g->E);
+    const double deg = bm->dc ? np_deg_dl(ctx, bm, c) : 0.0;  // get_deg_dl_dist: 0 without degree correction
     double S_dl = 0;
     S_dl += part;
-    S_dl += 0.0;  // get_deg_dl_dist: 0 without degree correction
+    S_dl += deg;
     S_dl += edges;
     if (parts) {
         parts[0] = S;
         parts[1] = part;
         parts[2] = edges;
+        parts[3] = deg;
     }
     return S + S_dl * 1.0;
 }

@@ -141,8 +141,8 @@ __device__ double merge_delta_mdl_np(const MgArgs &a, int r, int s) {
     const long long dss = mrr + mrs + msr;
     acc = wave_sum_fx(acc);
     if (dss > 0) fx_acc(acc, np_entry(a.np, mss, dss));
-    return np_finish(a.np, fx_to(acc), a.d_out[r], a.d_in[r], a.sizes[r], a.d_out[s], a.d_in[s], a.sizes[s],
-                     a.d_out[r], a.d_in[r], a.sizes[r]);
+    return np_finish(a.np, fx_to(acc), r, s, a.d_out[r], a.d_in[r], a.sizes[r], a.d_out[s], a.d_in[s], a.sizes[s],
+                     a.d_out[r], a.d_in[r], a.sizes[r], a.d_in[r], a.d_out[r]);
 }
 
 template <bool NP>

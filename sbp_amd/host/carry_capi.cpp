@@ -8,6 +8,15 @@
 #include "../csrc/carry.hpp"
 #include "../csrc/snowball.hpp"
 
+extern "C" int sbp_host_carry_out_dc(int32_t B, const int32_t *row_ptr, const int32_t *row_cols,
+                                     const int32_t *row_cnts, const int32_t *col_ptr, const int32_t *col_rows,
+                                     const int32_t *col_cnts, const int32_t *d_out, const int32_t *d_in,
+                                     const int32_t *sizes, const int32_t *best_s, const double *best_dS,
+                                     int64_t num_to_merge, int32_t approximate, int32_t threads,
+                                     int32_t seed_from_proposals, int32_t verify_proposals, int32_t nonparametric,
+                                     int64_t num_vertices, const uint64_t *hist_keys, const int32_t *hist_counts,
+                                     int64_t hist_n, int32_t *block_map_out, int64_t *stats_out);
+
 // stats_out (10 entries): pops, sequential evaluations, speculative evaluations, reused evaluations,
 // requeues, merges, model load microseconds, carry-out loop microseconds, threads, proposal values that
 // differ from a host re-evaluation (checked only when verify_proposals != 0).
@@ -18,6 +27,22 @@ extern "C" int sbp_host_carry_out(int32_t B, const int32_t *row_ptr, const int32
                                   int32_t approximate, int32_t threads, int32_t seed_from_proposals,
                                   int32_t verify_proposals, int32_t nonparametric, int64_t num_vertices,
                                   int32_t *block_map_out, int64_t *stats_out) {
+    return sbp_host_carry_out_dc(B, row_ptr, row_cols, row_cnts, col_ptr, col_rows, col_cnts, d_out, d_in, sizes,
+                                 best_s, best_dS, num_to_merge, approximate, threads, seed_from_proposals,
+                                 verify_proposals, nonparametric, num_vertices, nullptr, nullptr, 0, block_map_out,
+                                 stats_out);
+}
+
+// As sbp_host_carry_out, degree-corrected when hist_keys is given: the vertex-degree histograms as (key,
+// count) slots, key = block << 33 | direction << 32 | degree (direction 1 = in).
+extern "C" int sbp_host_carry_out_dc(int32_t B, const int32_t *row_ptr, const int32_t *row_cols,
+                                     const int32_t *row_cnts, const int32_t *col_ptr, const int32_t *col_rows,
+                                     const int32_t *col_cnts, const int32_t *d_out, const int32_t *d_in,
+                                     const int32_t *sizes, const int32_t *best_s, const double *best_dS,
+                                     int64_t num_to_merge, int32_t approximate, int32_t threads,
+                                     int32_t seed_from_proposals, int32_t verify_proposals, int32_t nonparametric,
+                                     int64_t num_vertices, const uint64_t *hist_keys, const int32_t *hist_counts,
+                                     int64_t hist_n, int32_t *block_map_out, int64_t *stats_out) {
     using clk = std::chrono::steady_clock;
     auto us = [](clk::time_point a, clk::time_point b) {
         return (int64_t)std::chrono::duration_cast<std::chrono::microseconds>(b - a).count();
@@ -29,6 +54,7 @@ extern "C" int sbp_host_carry_out(int32_t B, const int32_t *row_ptr, const int32
     if (!approximate) {
         m.load(B, row_ptr, row_cols, row_cnts, col_ptr, col_rows, col_cnts, 1, d_out, d_in, pool.get());
         if (nonparametric) m.set_nonparametric(sizes, num_vertices);
+        if (nonparametric && hist_keys) m.set_degree_corrected(hist_keys, hist_counts, hist_n);
     }
     auto t1 = clk::now();
     int64_t mismatches = 0;

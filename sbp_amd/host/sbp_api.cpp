@@ -67,9 +67,9 @@ sbp_params to_params(const Args &a) {
     p.max_iterations = 100;
     p.threads = a.threads;
     p.nonparametric = a.nonparametric ? 1 : 0;
-    if (a.nonparametric && (a.degreecorrected || a.undirected))
-        throw std::invalid_argument("the nonparametric description length is built for directed graphs without "
-                                    "degree correction");
+    p.degreecorrected = a.degreecorrected ? 1 : 0;
+    if (a.nonparametric && a.undirected)
+        throw std::invalid_argument("the nonparametric description length is built for directed graphs");
     return p;
 }
 
@@ -241,7 +241,9 @@ Blockmodel::Blockmodel(long num_blocks, const Graph &graph, double rate, std::ve
     std::vector<int32_t> a(assignment.begin(), assignment.end());
     sbp_bm bm = nullptr;
     throw_on(sbp_bm_build(ctx_, graph.device(ctx_), (int32_t)num_blocks, a.data(), &bm), ctx_, "sbp_bm_build");
-    sbp_bm_set_model(bm, sbp_amd::args().nonparametric ? 1 : 0);
+    const Args &args = sbp_amd::args();
+    throw_on(sbp_bm_set_model(ctx_, bm, graph.device(ctx_), args.nonparametric ? (args.degreecorrected ? 2 : 1) : 0),
+             ctx_, "sbp_bm_set_model");
     bm_ = wrap(bm);
     num_blocks_to_merge_ = (long)((double)num_blocks * rate);  // blockmodel.hpp:70-76
     empty = false;
@@ -383,7 +385,8 @@ Blockmodel drive(sbp_ctx ctx, sbp_graph g, int32_t N, const sbp_params &p, sbp_s
     std::iota(ident.begin(), ident.end(), 0);
     sbp_bm first = nullptr;
     throw_on(sbp_bm_build(ctx, g, N, ident.data(), &first), ctx, "sbp_bm_build");
-    sbp_bm_set_model(first, p.nonparametric);
+    throw_on(sbp_bm_set_model(ctx, first, g, p.nonparametric ? (p.degreecorrected ? 2 : 1) : 0), ctx,
+             "sbp_bm_set_model");
     Blockmodel blockmodel = Blockmodel::adopt(ctx, first, N, 1.7976931348623157e308, (long)((double)N * 0.5));
     BlockmodelTriplet triplet;
     uint32_t ep = 0;

@@ -36,7 +36,7 @@ class Params(C.Structure):
     _fields_ = [("algorithm", C.c_int32), ("batches", C.c_int32), ("greedy", C.c_int32),
                 ("approximate", C.c_int32), ("ordered", C.c_int32), ("asynciterations", C.c_int32),
                 ("mh_percent", C.c_double), ("seed", C.c_uint64), ("max_iterations", C.c_int32),
-                ("threads", C.c_int32), ("nonparametric", C.c_int32)]
+                ("threads", C.c_int32), ("nonparametric", C.c_int32), ("degreecorrected", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -56,11 +56,11 @@ class Stats(C.Structure):
 
 def params(algorithm="hybrid_mcmc", batches=1, greedy=True, approximate=False, ordered=False,
            asynciterations=2 ** 31 - 1, mh_percent=0.075, seed=0, max_iterations=100, threads=1,
-           nonparametric=False):
+           nonparametric=False, degreecorrected=False):
     """Defaults follow include/args.hpp (algorithm hybrid_mcmc, greedy on, one batch, mh_percent 0.075)."""
     alg = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
     return Params(alg, batches, int(greedy), int(approximate), int(ordered), asynciterations, mh_percent, seed,
-                  max_iterations, threads, int(nonparametric))
+                  max_iterations, threads, int(nonparametric), int(degreecorrected))
 
 
 _lib = None
@@ -93,8 +93,8 @@ def lib():
             "sbp_bm_download": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
             "sbp_bm_validate": [vp, vp, vp, vp],
             "sbp_mdl": [vp, vp, vp, vp, vp],
-            "sbp_bm_set_model": [vp, i32],
-            "sbp_np_description_length": [vp, vp, vp, vp, vp, vp, vp],
+            "sbp_bm_set_model": [vp, vp, vp, i32],
+            "sbp_np_description_length": [vp, vp, vp, vp, vp, vp, vp, vp],
             "sbp_ft_propose_all": [vp, vp, vp, u32, i32, vp, vp, vp],
             "sbp_async_gibbs_sweep": [vp, vp, vp, P, u32, S],
             "sbp_mh_sweep": [vp, vp, vp, P, u32, i32, vp, S],
@@ -221,21 +221,26 @@ class DeviceBlockmodel:
         self.h = handle
 
     @classmethod
-    def build(cls, graph, B, assignment, nonparametric=False):
+    def build(cls, graph, B, assignment, nonparametric=False, degreecorrected=False):
         a = np.ascontiguousarray(assignment, np.int32)
         h = C.c_void_p()
         graph.ctx.check(lib().sbp_bm_build(graph.ctx.h, graph.h, int(B), _p(a), C.byref(h)), "sbp_bm_build")
         bm = cls(graph, h)
         if nonparametric:
-            bm.set_model(True)
+            bm.set_model(True, degreecorrected)
         return bm
 
-    def set_model(self, nonparametric):
-        self.ctx.check(lib().sbp_bm_set_model(self.h, int(nonparametric)), "sbp_bm_set_model")
+    def set_model(self, nonparametric, degreecorrected=False):
+        model = (2 if degreecorrected else 1) if nonparametric else 0
+        self.ctx.check(lib().sbp_bm_set_model(self.ctx.h, self.h, self.g.h, model), "sbp_bm_set_model")
 
     def np_parts(self):
         """(mdl, sparse_entropy, partition_dl, edges_dl) of the nonparametric description length."""
-        out = [C.c_double() for _ in range(4)]
+        return self.np_parts_dc()[:4]
+
+    def np_parts_dc(self):
+        """(mdl, sparse_entropy, partition_dl, edges_dl, degree_dl) of the nonparametric description length."""
+        out = [C.c_double() for _ in range(5)]
         self.ctx.check(lib().sbp_np_description_length(self.ctx.h, self.h, self.g.h, *[C.byref(x) for x in out]),
                        "sbp_np_description_length")
         return tuple(x.value for x in out)

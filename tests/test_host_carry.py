@@ -17,9 +17,20 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def host_carry(bm, best_s, best_dS, num_to_merge, approximate, threads=0, seed=True, verify=False, np_model=False):
+def hist_slots(g, assignment):
+    """Degree histograms as (key, count) slots: key = block << 33 | direction << 32 | degree (1 = in)."""
+    out_deg, in_deg = np.diff(g.out_off), np.diff(g.in_off)
+    keys = np.concatenate([(assignment.astype(np.uint64) << np.uint64(33)) | out_deg.astype(np.uint64),
+                           (assignment.astype(np.uint64) << np.uint64(33)) | np.uint64(1 << 32) | in_deg.astype(np.uint64)])
+    u, c = np.unique(keys, return_counts=True)
+    return u.astype(np.uint64), c.astype(np.int32)
+
+
+def host_carry(bm, best_s, best_dS, num_to_merge, approximate, threads=0, seed=True, verify=False, np_model=False,
+               hist=None):
     L = C.CDLL(LIB)
     L.sbp_host_carry_out.restype = C.c_int32
+    L.sbp_host_carry_out_dc.restype = C.c_int32
     x = bm.export()
     B = x["B"]
     rp, cols, cnts = x["row_ptr"], x["cols"], x["cnts"]
@@ -33,10 +44,18 @@ def host_carry(bm, best_s, best_dS, num_to_merge, approximate, threads=0, seed=T
     d = np.ascontiguousarray(best_dS, np.float64)
     bmap = np.zeros(B, np.int32)
     st = np.zeros(10, np.int64)
-    nb = L.sbp_host_carry_out(B, _p(rp), _p(cols), _p(cnts), _p(cp), _p(crow), _p(ccnt), _p(dout), _p(din),
-                              _p(x["sizes"]), _p(s), _p(d), C.c_int64(num_to_merge), int(approximate), int(threads),
-                              int(seed), int(verify), int(np_model), C.c_int64(len(x["assignment"])), _p(bmap),
-                              _p(st))
+    if hist is None:
+        nb = L.sbp_host_carry_out(B, _p(rp), _p(cols), _p(cnts), _p(cp), _p(crow), _p(ccnt), _p(dout), _p(din),
+                                  _p(x["sizes"]), _p(s), _p(d), C.c_int64(num_to_merge), int(approximate),
+                                  int(threads), int(seed), int(verify), int(np_model),
+                                  C.c_int64(len(x["assignment"])), _p(bmap), _p(st))
+    else:
+        hk, hc = hist
+        nb = L.sbp_host_carry_out_dc(B, _p(rp), _p(cols), _p(cnts), _p(cp), _p(crow), _p(ccnt), _p(dout), _p(din),
+                                     _p(x["sizes"]), _p(s), _p(d), C.c_int64(num_to_merge), int(approximate),
+                                     int(threads), int(seed), int(verify), int(np_model),
+                                     C.c_int64(len(x["assignment"])), _p(hk), _p(hc), C.c_int64(len(hk)), _p(bmap),
+                                     _p(st))
     return nb, bmap, st
 
 
@@ -123,3 +142,22 @@ def test_host_carry_split_rows_are_exact(split, monkeypatch):
         assert nb == ref[0] and np.array_equal(bmap, ref[1]), seed
     out = bm.copy().merge_apply(best_s, best_dS, B // 2, False)
     assert np.array_equal(ref[1][assign], out.export()["assignment"])
+
+
+@pytest.mark.parametrize("N,B", [(2000, 2000), (3000, 300)])
+def test_host_carry_degree_corrected_matches_oracle(N, B):
+    """Degree-corrected re-evaluations: the histograms of the merged blocks add up as merges happen."""
+    g = dcsbm(N, 20, gamma=2.3, kmin=1, kmax=150, mean_degree=10, p_in=0.65, seed=N + 5)
+    og = O.OGraph.from_csr(g)
+    rng = np.random.default_rng(N + 1)
+    assign = np.arange(N, dtype=np.int32) if B == N else np.concatenate(
+        [np.arange(B), rng.integers(0, B, N - B)]).astype(np.int32)
+    bm = O.OBlockmodel.build(og, B, assign, nonparametric=True, degreecorrected=True)
+    best_s, best_dS = bm.merge_propose(seed=2, epoch=3)
+    for threads in (1, 8):
+        nb, bmap, st = host_carry(bm, best_s, best_dS, B // 2, False, threads=threads, verify=True, np_model=True,
+                                  hist=hist_slots(g, assign))
+        assert st[9] == 0
+        out = bm.copy().merge_apply(best_s, best_dS, B // 2, False)
+        assert nb == out.B
+        assert np.array_equal(bmap[assign], out.export()["assignment"])
