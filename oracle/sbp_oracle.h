@@ -22,7 +22,12 @@
  *     accumulated in 128-bit fixed point: each term is scaled by 2^64 and
  *     truncated toward zero; the total is converted back as
  *     (double)hi + (double)lo * 2^-64.  Integer addition is associative, so
- *     the sum does not depend on the order of the terms.
+ *     the sum does not depend on the order of the terms.  The MH sweep's
+ *     sum of accepted delta-MDL is a double sum in visit order, as in the
+ *     reference (with degree correction a term can be infinite).
+ *   - The snowball samplers' random_integer(lo, hi) calls are
+ *     lo + (w0 (hi - lo + 1)) >> 32 over Philox(seed; j, 0, TAG_SNOWBALL, 0),
+ *     j counting the calls.
  *   - logf is glibc's (the reference's fastlog / nat_log are logf).
  *   - delta-MDL sums the Delta cells and the cells at rows/columns {r, s} term by term, and the remaining
  *     unchanged cells of rows r, s / columns r, s in bulk: such a cell changes only through one block
@@ -34,12 +39,17 @@
  *     the vertex shuffle is the ascending order of Philox sort keys.
  *
  * What pins each part (tests/test_oracle.py, tests/test_sampling_oracle.py,
- * tests/test_nonparametric_oracle.py): Philox known-answer vectors; the
- * reference's toy-graph known answers for the block matrix, degrees, MDL,
- * null model, vertex / merge delta-MDL, Hastings, max_degree sampling and the
- * nonparametric description length; brute-force description-length
- * differences with glibc logf; restatements of the reference's sampling and
- * extension loops.  The sweeps (async Gibbs, MH, hybrid, finetune_assignment),
+ * tests/test_nonparametric_oracle.py, tests/test_degree_corrected_oracle.py):
+ * Philox known-answer vectors; the reference's toy-graph known answers for the
+ * block matrix, degrees, MDL, null model, vertex / merge delta-MDL, Hastings,
+ * max_degree sampling, the nonparametric description length and the
+ * degree-corrected sparse entropy; brute-force description-length differences
+ * with glibc logf; restatements of the reference's sampling and extension
+ * loops; libstdc++'s heap order (degree_product_snowball) against
+ * std::priority_queue; spence against the Cephes dilogarithm and log_q
+ * against exact partition counts.  The reference's DEGREE_DL and
+ * degree-corrected MDL constants are not reproducible from its code and are
+ * not used.  The sweeps (async Gibbs, MH, hybrid, finetune_assignment),
  * merge phases with the carry-out, and the driver have no reference output
  * under any replay: their parity is unpinned against the reference and rests
  * on invariants (rebuild equality after every sweep and merge, thread-count
