@@ -11,9 +11,19 @@ carries accepted moves, merge proposals, MDL, NMI against the planted partition,
 finetune proposal kernel (algorithmic bytes per launch / average launch time, measured with HIP events on
 the kernel's stream) and a CPU baseline: the oracle's OpenMP asynchronous-Gibbs sweeps on the same graph
 at the planted partition, timed for a bounded sample on the host cores.
+
+Two more legs measure many graphs on the device (sbp_amd/workloads.py names the mixed-size workloads):
+- "batched": graphs 0..G-1 of the batched workload (N log-uniform in [200, 50000]), resident in device
+  memory, partitioned one after another on one context (the plain GPU path) and as one
+  sbp_partition_batch call over worker slots; graphs/s of both, and whether every result is identical.
+- "served": the first R requests of the served workload (N log-uniform in [1000, --served-max-n]) arriving
+  as a Poisson process, replayed against a request server with one worker (the plain sequential path) and
+  with several; p50/p99 latency from arrival to completion and graphs/s of both.
 """
 import argparse
+import concurrent.futures as cf
 import json
+import multiprocessing as mp
 import os
 import sys
 import time
@@ -135,6 +145,88 @@ def roofline(tot, greedy=True):
             "launches": {k: {kk: vv for kk, vv in v.items() if kk != "total_ms"} for k, v in legs.items()}}
 
 
+def batched_leg(ctx, graphs, algorithm, slots):
+    """The batched workload's graphs one after another on ctx (plain path) and as one sbp_partition_batch."""
+    from sbp_amd import native
+    prms = [native.params(algorithm, seed=i) for i in range(len(graphs))]
+    dgs = [ctx.upload(g) for g in graphs]
+    ctx.synchronize()
+    ctx.partition_batch(dgs[:slots], prms[:slots], slots=slots)  # creates and warms the worker contexts
+    t0 = time.perf_counter()
+    plain = [dg.partition(p) for dg, p in zip(dgs, prms)]
+    t_plain = time.perf_counter() - t0
+    print(f"[bench] batched: plain path {len(graphs)} graphs in {t_plain:.2f}s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    outs, nb, st, fin = ctx.partition_batch(dgs, prms, slots=slots)
+    t_batch = time.perf_counter() - t0
+    print(f"[bench] batched: {slots} slots in {t_batch:.2f}s", file=sys.stderr, flush=True)
+    same = all(np.array_equal(outs[i], plain[i][0]) and int(nb[i]) == plain[i][1] and st[i].mdl == plain[i][2].mdl
+               for i in range(len(graphs)))
+    return {
+        "workload": f"graphs 0..{len(graphs) - 1} of the batched mixed-size workload: directed DCSBM, N log-uniform "
+                    "in [200, 50000], B = round(N^0.4), mean degree U[8, 32] (power law 2.5), p_in U[0.5, 0.8], "
+                    f"graph i seed i; {algorithm}",
+        "graphs": len(graphs), "vertices": int(sum(g.N for g in graphs)), "edges": int(sum(g.E for g in graphs)),
+        "slots": slots,
+        "graphs_per_s": len(graphs) / t_batch, "plain_graphs_per_s": len(graphs) / t_plain,
+        "speedup_vs_plain": t_plain / t_batch,
+        "proposals_per_s": sum(x.proposals for x in st) / t_batch,
+        "identical_to_plain": bool(same),
+        "nmi_mean": float(np.mean([nmi(g.truth, o) for g, o in zip(graphs, outs)])),
+        "plain_nmi_mean": float(np.mean([nmi(g.truth, p[0]) for g, p in zip(graphs, plain)])),
+    }
+
+
+def serve_trace(graphs, arrivals, prms, slots):
+    """Replays the arrival trace against a request server with `slots` workers; per-request results and
+    latencies (ms) and the wall time from the first arrival to the last completion."""
+    from sbp_amd import native
+    srv = native.Server(0, slots=slots, hbm_budget_bytes=200 * 10 ** 9)
+    t0 = time.perf_counter()
+    tickets = []
+    for g, at, p in zip(graphs, arrivals, prms):
+        dt = at - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        tickets.append(srv.submit(g, p))
+    res = [srv.wait(t) for t in tickets]
+    wall = time.perf_counter() - t0
+    info = srv.info()
+    srv.close()
+    return res, wall, info
+
+
+def served_leg(graphs, arrivals, algorithm, slots, rate, max_n):
+    from sbp_amd import native
+    prms = [native.params(algorithm, seed=j) for j in range(len(graphs))]
+    out = {}
+    runs = {}
+    for name, k in (("plain", 1), ("served", slots)):
+        res, wall, info = serve_trace(graphs, arrivals, prms, k)
+        lat = np.array([r[3] for r in res])
+        runs[name] = res
+        out[name] = {"slots": k, "graphs_per_s": len(graphs) / wall, "p50_ms": float(np.percentile(lat, 50)),
+                     "p99_ms": float(np.percentile(lat, 99)), "mean_queue_ms": float(np.mean([r[4] for r in res])),
+                     "max_running": info["max_running"], "peak_reserved_bytes": info["peak_in_use"]}
+        print(f"[bench] served: {k} slot(s) p50 {out[name]['p50_ms']:.0f} ms p99 {out[name]['p99_ms']:.0f} ms "
+              f"{out[name]['graphs_per_s']:.2f} graphs/s", file=sys.stderr, flush=True)
+    same = all(np.array_equal(a[0], b[0]) and a[1] == b[1] and a[2].mdl == b[2].mdl
+               for a, b in zip(runs["plain"], runs["served"]))
+    return {
+        "workload": f"requests 0..{len(graphs) - 1} of the served workload: directed DCSBM, N log-uniform in "
+                    f"[1000, {max_n}], B = round(N^0.4), mean degree 16 (power law 2.1), p_in 0.55, request j seed "
+                    f"j; Poisson arrivals at {rate} req/s (arrival seed 0); {algorithm}; 200 GB admission budget",
+        "requests": len(graphs), "vertices": int(sum(g.N for g in graphs)), "edges": int(sum(g.E for g in graphs)),
+        "rate_per_s": rate,
+        "graphs_per_s": out["served"]["graphs_per_s"], "p50_ms": out["served"]["p50_ms"],
+        "p99_ms": out["served"]["p99_ms"],
+        "speedup_vs_plain": out["served"]["graphs_per_s"] / out["plain"]["graphs_per_s"],
+        "identical_to_plain": bool(same),
+        "nmi_mean": float(np.mean([nmi(g.truth, r[0]) for g, r in zip(graphs, runs["served"])])),
+        "plain": out["plain"], "served": out["served"],
+    }
+
+
 def calibrate_copy(mib):
     """A plain device-to-device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
     import torch
@@ -155,6 +247,13 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched-graphs", type=int, default=64, help="graphs in the batched leg (0: skip it)")
+    ap.add_argument("--batched-slots", type=int, default=8)
+    ap.add_argument("--served-requests", type=int, default=24, help="requests in the served leg (0: skip it)")
+    ap.add_argument("--served-rate", type=float, default=8.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--served-max-n", type=int, default=200_000, help="upper bound of the requests' N")
+    ap.add_argument("--served-slots", type=int, default=8)
+    ap.add_argument("--many-algorithm", default="async_gibbs", help="algorithm of the batched and served legs")
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
                     help="only copy MIB MiB device-to-device 3 times (counter calibration) and exit")
     args = ap.parse_args()
@@ -165,6 +264,14 @@ def main():
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the path is single-device, "
               "see the sharding note in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    from sbp_amd import workloads
+    # the mixed-size graphs are generated in worker processes (forked before the device is touched) while
+    # the single-graph workload is generated and warmed up
+    pool = cf.ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 1), mp_context=mp.get_context("fork"))
+    fut_b = [pool.submit(workloads.make, workloads.batched_spec(i)) for i in range(args.batched_graphs)]
+    fut_s = [pool.submit(workloads.make, workloads.served_spec(j, n_max=args.served_max_n))
+             for j in range(args.served_requests)]
 
     from sbp_amd import native
     from sbp_amd.graph import dcsbm
@@ -183,6 +290,10 @@ def main():
         a, nb, st = dg.partition(prm)
         print(f"[bench] warmup {i}: B={nb} {st.total_ms / 1e3:.2f}s", file=sys.stderr, flush=True)
 
+    # the timed steps and the CPU baseline run with the generator processes finished
+    many_b = [f.result() for f in fut_b]
+    many_s = [f.result() for f in fut_s]
+    pool.shutdown()
     ctx.synchronize()
     t_start = time.perf_counter()
     tot = {k: 0.0 for k in native.Stats().as_dict()}
@@ -209,6 +320,14 @@ def main():
                "sample": f"{sweeps} oracle {w['algorithm']} sweeps ({sweeps * g.N} proposals) over the same "
                          f"graph at its planted partition, {dt:.1f}s on {threads} OpenMP threads",
                "gpu_same_sample": gpu_truth_rate}
+
+    batched = served = None
+    if args.batched_graphs:
+        batched = batched_leg(ctx, many_b, args.many_algorithm, args.batched_slots)
+    if args.served_requests:
+        arrivals = workloads.poisson_arrivals(args.served_rate, args.served_requests, seed=0)
+        served = served_leg(many_s, arrivals, args.many_algorithm, args.served_slots, args.served_rate,
+                            args.served_max_n)
 
     line = {
         "metric": "MCMC moves/s, batched graphs/s & p99 latency vs plain GPU SBP at equal NMI",
@@ -240,6 +359,8 @@ def main():
                             "carry_requeues")},
         "roofline": roofline(tot, greedy=True),
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(line))
 

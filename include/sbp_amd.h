@@ -204,6 +204,55 @@ int sbp_merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int64_
 int sbp_partition(sbp_ctx ctx, sbp_graph g, const sbp_params *p, int32_t *assignment_out,
                   int32_t *num_blocks_out, sbp_stats *stats);
 
+/* ----------------------------------------------------------------- many graphs on one device
+ *
+ * No counterpart in the reference, which partitions one graph per process (src/main.cpp:60-177).  Each graph
+ * runs the driver above (sbp_partition) on a worker context of its own (stream, buffer pool, pinned staging;
+ * merge carry-out on one host thread), one host thread per worker; workers overlap on the device.  A graph's
+ * result (assignment, block count, description length) is the one sbp_partition gives it alone. */
+
+/* Device bytes a context holds (buffer pool, scratch slots, lgamma table; graphs not counted), now and the
+ * largest seen at the end of an sbp_partition call. */
+int sbp_ctx_memory(sbp_ctx ctx, uint64_t *current_bytes, uint64_t *peak_bytes);
+/* Frees the buffers the context's pool keeps cached for reuse and its scratch slots. */
+int sbp_ctx_trim(sbp_ctx ctx);
+/* Host threads of the context's merge carry-out (0: SBP_AMD_CARRY_THREADS, else up to 16). */
+int sbp_set_carry_threads(sbp_ctx ctx, int32_t threads);
+/* Device bytes of a graph with N vertices and E edges, and an upper bound of the device bytes one partition
+ * run of it needs (graph, blockmodels, scratch), given the largest |out(v)| + |in(v)|, the number of vertices
+ * where that exceeds 256 and the algorithm (sbp_params.algorithm): the server admits requests by the latter. */
+uint64_t sbp_graph_bytes(int32_t N, int64_t E);
+uint64_t sbp_footprint_bytes(int32_t N, int64_t E, int64_t max_degree, int64_t n_high_degree, int32_t algorithm);
+
+/* Partitions n device graphs (uploaded on any context of ctx's device) with per-graph parameters params[n]
+ * (each its own seed) over `slots` workers (0: SBP_AMD_SLOTS, else min(12, host threads)), largest graphs
+ * first.  The workers are contexts owned by ctx, kept for later calls.  Optional outputs per graph:
+ * assignments[i] (host int32[N_i]), num_blocks[i], stats[i], finish_ms[i] (milliseconds from the start of the
+ * call to the graph's completion).  Returns the first failing graph's status, its index in the message. */
+int sbp_partition_batch(sbp_ctx ctx, const sbp_graph *graphs, int32_t n, const sbp_params *params, int32_t slots,
+                        int32_t *const *assignments, int32_t *num_blocks, sbp_stats *stats, double *finish_ms);
+
+/* Request server: `slots` workers (0 as above) take submitted partition requests in arrival order, each as soon
+ * as a worker is free and its footprint (sbp_footprint_bytes) fits in the budget left by the requests in
+ * flight; each worker also keeps up to min(4 GiB, budget / (4 slots)) cached between requests, which is set
+ * aside from the budget.  A request uploads its graph on its worker's stream, partitions it and frees it. */
+typedef struct sbp_server_s *sbp_server;
+int sbp_server_create(int device, int32_t slots, uint64_t hbm_budget_bytes, sbp_server *out);
+const char *sbp_server_last_error(sbp_server srv);
+/* Queues a request.  The host CSR arrays are read when the request starts and must stay valid until
+ * sbp_server_wait returns for it.  SBP_E_OOM when the footprint exceeds the budget. */
+int sbp_server_submit(sbp_server srv, int32_t N, int64_t E, const int64_t *out_off, const int32_t *out_adj,
+                      const int64_t *in_off, const int32_t *in_adj, const sbp_params *p, int64_t *ticket);
+/* Blocks until the request is done and returns its status and results: latency_ms from submission to
+ * completion, queue_ms from submission to start.  A ticket is waited for once. */
+int sbp_server_wait(sbp_server srv, int64_t ticket, int32_t *assignment_out, int32_t *num_blocks_out,
+                    sbp_stats *stats, double *latency_ms, double *queue_ms);
+int sbp_server_info(sbp_server srv, int32_t *running, int32_t *queued, int32_t *max_running, uint64_t *budget_bytes,
+                    uint64_t *in_use_bytes, uint64_t *peak_in_use_bytes);
+/* Lets running requests finish, ends queued ones with SBP_E_STATE, joins the workers.  Call it after every
+ * sbp_server_wait has returned. */
+int sbp_server_destroy(sbp_server srv);
+
 /* ----------------------------------------------------------------- diagnostics */
 
 /* Device logf over float bit patterns [first_bits, first_bits + count) (checks the restated glibc logf). */

@@ -281,7 +281,7 @@ static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, 
             SBP_HIP(hipMemcpy(rc.data(), bm->row_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
             SBP_HIP(hipMemcpy(cc.data(), bm->col_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
         }
-        pool.reset(new sbpcarry::Pool(sbpcarry::carry_threads()));
+        pool.reset(new sbpcarry::Pool(ctx->carry_threads > 0 ? ctx->carry_threads : sbpcarry::carry_threads()));
         const int32_t *rcp = reinterpret_cast<const int32_t *>(rc.data());
         const int32_t *ccp = reinterpret_cast<const int32_t *>(cc.data());
         model.load(B, rp.data(), rcp, rcp + 1, cp.data(), ccp, ccp + 1, 2, dout.data(), din.data(), pool.get());
@@ -385,6 +385,8 @@ int sbp_create(sbp_ctx *out, int device, uint64_t seed) {
 
 int sbp_destroy(sbp_ctx ctx) {
     if (!ctx) return SBP_OK;
+    for (sbp_ctx w : ctx->slots) sbp_destroy(w);
+    ctx->slots.clear();
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &e : ctx->ev)
@@ -749,8 +751,43 @@ int sbp_partition(sbp_ctx ctx, sbp_graph g, const sbp_params *p, int32_t *assign
         throw;
     }
     ctx->seed = saved;
+    ctx->peak_bytes = std::max<uint64_t>(ctx->peak_bytes, ctx_device_bytes(ctx));
+    if (getenv("SBP_AMD_TRACE")) {
+        fprintf(stderr, "[sbp] device bytes: pool %zu (cached %zu), lgamma %lld, scratch", ctx->pool->total_bytes,
+                ctx->pool->cached_bytes, (long long)ctx->lgamma_n * 8);
+        for (int i = 0; i < S_NUM_SLOTS; ++i) fprintf(stderr, " %d:%zu", i, ctx->scratch[i].bytes);
+        fprintf(stderr, "\n");
+    }
     if (stats) stats->total_ms += now_ms() - t0;
     SBP_API_END(ctx)
+}
+
+int sbp_ctx_memory(sbp_ctx ctx, uint64_t *current_bytes, uint64_t *peak_bytes) {
+    if (!ctx) return SBP_E_ARG;
+    const uint64_t now = ctx_device_bytes(ctx);
+    ctx->peak_bytes = std::max(ctx->peak_bytes, now);
+    if (current_bytes) *current_bytes = now;
+    if (peak_bytes) *peak_bytes = ctx->peak_bytes;
+    return SBP_OK;
+}
+
+int sbp_ctx_trim(sbp_ctx ctx) {
+    if (!ctx) return SBP_E_ARG;
+    SBP_API_BEGIN
+    SBP_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->pool) ctx->pool->trim();
+    for (auto &sc : ctx->scratch) {  // the next use allocates afresh under a new generation
+        if (sc.ptr) SBP_HIP(hipFree(sc.ptr));
+        sc.ptr = nullptr;
+        sc.bytes = 0;
+    }
+    SBP_API_END(ctx)
+}
+
+int sbp_set_carry_threads(sbp_ctx ctx, int32_t threads) {
+    if (!ctx || threads < 0 || threads > 64) return SBP_E_ARG;
+    ctx->carry_threads = threads;
+    return SBP_OK;
 }
 
 int sbp_debug_logf(sbp_ctx ctx, uint32_t first_bits, uint32_t count, float *out) {

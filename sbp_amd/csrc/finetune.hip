@@ -856,6 +856,7 @@ void shuffle_keys(sbp_ctx ctx, const int32_t *d_items, int64_t n, uint32_t epoch
 
 struct MhState {
     int32_t *rstart, *rlen, *rcap, *cstart, *clen, *ccap;  // [B]
+    int32_t *rboost, *cboost;  // [B] multiplier of a row's / column's extra capacity at the next layout
     int32_t *stop_at;   // first order index not processed (== count when done)
     int32_t *status;    // 0 done, 1 out of slack
     unsigned long long *moves;
@@ -913,11 +914,26 @@ __device__ __forceinline__ void apply_cell(const FtArgs &a, const MhState &h, in
     slack_add(a.m.col_cells, h.cstart[col], &h.clen[col], row, d);
 }
 
-// Would adding cell (row, col) need a new slot that row or column lacks?
+// Asks for room for `more` new cells in row / column b at the next layout: its extra capacity (max(8, len / 2)
+// times the boost) at least doubles and covers the request.  Lanes that ask for the same block store values
+// that each double it, so any of them makes progress.
+__device__ __forceinline__ void want_room(int32_t *boost, const int32_t *len, int b, int more) {
+    const int l = len[b];
+    const int extra = l / 2 > 8 ? l / 2 : 8;
+    long long nb = 2ll * boost[b];
+    const long long need = ((long long)more + extra - 1) / extra;
+    if (need > nb) nb = need;
+    boost[b] = (int32_t)(nb < (1 << 24) ? nb : (1 << 24));
+}
+
+// Would adding cell (row, col) need a new slot that row or column lacks?  A short row / column asks for room.
 __device__ __forceinline__ bool lacks_room(const FtArgs &a, const MhState &h, int row, int col, long long d) {
     if (d <= 0) return false;
     if (get_cell(a, row, col) != 0) return false;
-    return h.rlen[row] >= h.rcap[row] || h.clen[col] >= h.ccap[col];
+    const bool short_r = h.rlen[row] >= h.rcap[row], short_c = h.clen[col] >= h.ccap[col];
+    if (short_r) want_room(h.rboost, h.rlen, row, 1);
+    if (short_c) want_room(h.cboost, h.clen, col, 1);
+    return short_r || short_c;
 }
 
 template <bool GREEDY, bool NP>
@@ -976,7 +992,10 @@ __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start)
         // row s and column s may receive several new cells: check their totals
         new_row_s = wave_sum(new_row_s) + 2;
         new_col_s = wave_sum(new_col_s) + 2;
-        short_room |= (h.rlen[s] + new_row_s > h.rcap[s]) || (h.clen[s] + new_col_s > h.ccap[s]);
+        const bool short_rs = h.rlen[s] + new_row_s > h.rcap[s], short_cs = h.clen[s] + new_col_s > h.ccap[s];
+        if (lane == 0 && short_rs) want_room(h.rboost, h.rlen, s, new_row_s);
+        if (lane == 0 && short_cs) want_room(h.cboost, h.clen, s, new_col_s);
+        short_room |= short_rs || short_cs;
         if (ballot(short_room)) {
             table_clear_used(t);
             status = 1;
@@ -1041,15 +1060,22 @@ __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start)
     }
 }
 
-// Slack layout: capacity = len + max(8, len / 2) (times `grow`), for rows and columns.
+// Slack layout: capacity = len + max(8, len / 2) times the row's (column's) boost, at most B + 2: a row never
+// holds more than B cells, and the room check for a destination row asks for its new cells plus 2.
 __global__ void k_slack_caps(int32_t B, const int32_t *__restrict__ ptr, int32_t *__restrict__ len,
-                             int32_t *__restrict__ cap, int grow) {
+                             int32_t *__restrict__ cap, const int32_t *__restrict__ boost) {
     for (int32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x) {
-        int l = ptr[b + 1] - ptr[b];
+        const int l = ptr[b + 1] - ptr[b];
         len[b] = l;
-        int extra = l / 2 > 8 ? l / 2 : 8;
-        cap[b] = l + extra * grow;
+        const long long extra = l / 2 > 8 ? l / 2 : 8;
+        const long long c = (long long)l + extra * boost[b];
+        cap[b] = (int32_t)(c < (long long)B + 2 ? c : (long long)B + 2);
     }
+}
+
+__global__ void k_fill_i32(int64_t n, int32_t *__restrict__ p, int32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 __global__ void k_slack_copy(int32_t B, const int32_t *__restrict__ ptr, const int2 *__restrict__ src,
@@ -1073,8 +1099,8 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         return;
     }
     const int32_t B = bm->B;
-    // per-block arrays: rstart rlen rcap cstart clen ccap + bookkeeping
-    int32_t *arr = ctx->scratch[S_EVENTS_A].as<int32_t>(6 * ((size_t)B + 1) + 64);
+    // per-block arrays: rstart rlen rcap cstart clen ccap + bookkeeping + row and column boosts
+    int32_t *arr = ctx->scratch[S_EVENTS_A].as<int32_t>(8 * ((size_t)B + 1) + 64);
     MhState h;
     h.rstart = arr;
     h.rlen = arr + (B + 1);
@@ -1083,6 +1109,8 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
     h.clen = arr + 4 * (B + 1);
     h.ccap = arr + 5 * (B + 1);
     int32_t *book = arr + 6 * (B + 1);
+    h.rboost = book + 64;
+    h.cboost = h.rboost + (B + 1);
     h.stop_at = book;
     h.status = book + 1;
     h.moves = reinterpret_cast<unsigned long long *>(book + 4);
@@ -1090,15 +1118,13 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
     auto *bytes = ctx->scratch[S_COUNTER].as<unsigned long long>(4);
     SBP_HIP(hipMemsetAsync(book, 0, 64 * sizeof(int32_t), st));
     SBP_HIP(hipMemsetAsync(bytes, 0, 4 * sizeof(unsigned long long), st));
-    int grow = 1;
-    int32_t start = 0;
-    double t0 = 0;
+    k_fill_i32<<<grid_of(2 * ((int64_t)B + 1)), 256, 0, st>>>(2 * ((int64_t)B + 1), h.rboost, 1);
+    int32_t start = 0, retries = 0;
     float kernel_ms = 0.f;
-    (void)t0;
     while (true) {
         // lay out rows and columns with slack
-        k_slack_caps<<<grid_of(B), 256, 0, st>>>(B, bm->row_ptr, h.rlen, h.rcap, grow);
-        k_slack_caps<<<grid_of(B), 256, 0, st>>>(B, bm->col_ptr, h.clen, h.ccap, grow);
+        k_slack_caps<<<grid_of(B), 256, 0, st>>>(B, bm->row_ptr, h.rlen, h.rcap, h.rboost);
+        k_slack_caps<<<grid_of(B), 256, 0, st>>>(B, bm->col_ptr, h.clen, h.ccap, h.cboost);
         size_t tmp = 0;
         SBP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, h.rcap, h.rstart, B + 1, st));
         void *tp = ctx->scratch[S_SORT_TMP].get(tmp);
@@ -1161,11 +1187,12 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         bm_compute_cum(ctx, bm);
         SBP_HIP(hipStreamSynchronize(st));
         if (hs[1] == 0) break;
+        // (1): resume at the vertex that ran out of room; the rows and columns it lacked room in asked for
+        // more (their capacities at least double, up to B + 2, where no move lacks room)
+        // (2): resume after a move into an empty block, with the constants of the new block count
+        retries = (hs[1] == 1 && hs[0] == start) ? retries + 1 : 0;
+        SBP_CHECK(retries <= 64, SBP_E_STATE, "Metropolis-Hastings: slack growth did not converge");
         start = hs[0];
-        if (hs[1] == 1) {  // resume at the vertex that ran out of room, with more slack
-            grow *= 2;
-            SBP_CHECK(grow <= (1 << 24), SBP_E_STATE, "Metropolis-Hastings: slack growth did not converge");
-        }  // else (2): resume after a move into an empty block, with the constants of the new block count
         SBP_HIP(hipMemsetAsync(book, 0, 2 * sizeof(int32_t), st));
     }
     struct {

@@ -76,6 +76,7 @@ struct DevicePool {
     std::multimap<size_t, void *> cached;
     std::unordered_map<void *, size_t> size_of;
     size_t cached_bytes = 0;
+    size_t total_bytes = 0;  // every buffer the pool owns, handed out or cached
     void *alloc(size_t bytes) {
         bytes = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
         auto it = cached.lower_bound(bytes);
@@ -92,6 +93,7 @@ struct DevicePool {
             SBP_HIP(hipMalloc(&p, bytes));
         }
         size_of[p] = bytes;
+        total_bytes += bytes;
         return p;
     }
     void release(void *p) {
@@ -112,6 +114,7 @@ struct DevicePool {
             size_of.erase(kv.second);
             (void)hipFree(kv.second);
         }
+        total_bytes -= cached_bytes;
         cached.clear();
         cached_bytes = 0;
     }
@@ -142,9 +145,20 @@ struct sbp_ctx_s {
     uint64_t hash_clean_gen = 0;  // generation of slot S_HASH the state below refers to (0 = none)
     int hash_clean_lg = -1;
     int64_t hash_clean_tables = 0;
+    int carry_threads = 0;    // host threads of the merge carry-out (0: SBP_AMD_CARRY_THREADS or up to 16)
+    uint64_t peak_bytes = 0;  // largest device footprint seen at the end of a partition (sbp_ctx_memory)
+    std::vector<sbp_ctx> slots;  // worker contexts of sbp_partition_batch, created on first use
 };
 
 namespace sbpimpl {
+
+// Device bytes a context holds: its buffer pool (blockmodels, handed out or cached), scratch slots and the
+// lgamma table.  Graphs are allocated apart and not counted.
+inline uint64_t ctx_device_bytes(sbp_ctx ctx) {
+    uint64_t b = ctx->pool ? ctx->pool->total_bytes : 0;
+    for (const Scratch &s : ctx->scratch) b += s.bytes;
+    return b + sizeof(double) * (uint64_t)ctx->lgamma_n;
+}
 
 constexpr size_t PINNED_BYTES = 4096;
 struct ReadPiece {

@@ -23,6 +23,10 @@ EXPORTS = [
     "sbp_ft_propose_all", "sbp_async_gibbs_sweep", "sbp_mh_sweep", "sbp_mcmc", "sbp_finetune_assignment",
     "sbp_merge_propose", "sbp_merge_blocks",
     "sbp_partition",
+    "sbp_ctx_memory", "sbp_ctx_trim", "sbp_set_carry_threads", "sbp_graph_bytes", "sbp_footprint_bytes",
+    "sbp_partition_batch",
+    "sbp_server_create", "sbp_server_last_error", "sbp_server_submit", "sbp_server_wait", "sbp_server_info",
+    "sbp_server_destroy",
     "sbp_debug_logf",
 ]
 
@@ -104,13 +108,27 @@ def lib():
             "sbp_merge_blocks": [vp, vp, vp, u32, i64, i32, vp, S],
             "sbp_partition": [vp, vp, P, vp, vp, S],
             "sbp_debug_logf": [vp, u32, u32, vp],
+            "sbp_ctx_memory": [vp, vp, vp],
+            "sbp_ctx_trim": [vp],
+            "sbp_set_carry_threads": [vp, i32],
+            "sbp_partition_batch": [vp, vp, i32, vp, i32, vp, vp, vp, vp],
+            "sbp_server_create": [C.c_int, i32, u64, vp],
+            "sbp_server_submit": [vp, i32, i64, vp, vp, vp, vp, P, vp],
+            "sbp_server_wait": [vp, i64, vp, vp, S, vp, vp],
+            "sbp_server_info": [vp, vp, vp, vp, vp, vp, vp],
+            "sbp_server_destroy": [vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
             f.restype = C.c_int
             f.argtypes = args
-        L.sbp_last_error.restype = C.c_char_p
-        L.sbp_last_error.argtypes = [vp]
+        for name in ("sbp_last_error", "sbp_server_last_error"):
+            getattr(L, name).restype = C.c_char_p
+            getattr(L, name).argtypes = [vp]
+        L.sbp_graph_bytes.restype = u64
+        L.sbp_graph_bytes.argtypes = [i32, i64]
+        L.sbp_footprint_bytes.restype = u64
+        L.sbp_footprint_bytes.argtypes = [i32, i64, i64, i64, i32]
         _lib = L
     return _lib
 
@@ -157,6 +175,101 @@ class Context:
         out = np.zeros(count, np.float32)
         self.check(lib().sbp_debug_logf(self.h, first_bits, count, _p(out)), "sbp_debug_logf")
         return out
+
+    def memory(self):
+        """(current, peak) device bytes of the context (sbp_ctx_memory)."""
+        cur, peak = C.c_uint64(), C.c_uint64()
+        self.check(lib().sbp_ctx_memory(self.h, C.byref(cur), C.byref(peak)), "sbp_ctx_memory")
+        return cur.value, peak.value
+
+    def trim(self):
+        self.check(lib().sbp_ctx_trim(self.h), "sbp_ctx_trim")
+
+    def set_carry_threads(self, threads):
+        self.check(lib().sbp_set_carry_threads(self.h, threads), "sbp_set_carry_threads")
+
+    def partition_batch(self, graphs, prms, slots=0):
+        """sbp_partition_batch over DeviceGraphs with one Params each.  Returns (assignments, num_blocks,
+        stats, finish_ms) per graph, in input order."""
+        n = len(graphs)
+        assert len(prms) == n
+        hs = (C.c_void_p * n)(*[g.h for g in graphs])
+        ps = (Params * n)(*prms)
+        outs = [np.zeros(g.N, np.int32) for g in graphs]
+        ptrs = (C.c_void_p * n)(*[o.ctypes.data for o in outs])
+        nb = np.zeros(n, np.int32)
+        st = (Stats * n)()
+        fin = np.zeros(n, np.float64)
+        self.check(lib().sbp_partition_batch(self.h, hs, n, ps, slots, ptrs, _p(nb), st, _p(fin)),
+                   "sbp_partition_batch")
+        return outs, nb, list(st), fin
+
+
+def graph_bytes(N, E):
+    return int(lib().sbp_graph_bytes(N, E))
+
+
+def footprint_bytes(g, algorithm="async_gibbs"):
+    """sbp_footprint_bytes of a CSRGraph for `algorithm` (name or sbp_params.algorithm value)."""
+    k = np.diff(g.out_off) + np.diff(g.in_off)
+    alg = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
+    return int(lib().sbp_footprint_bytes(g.N, g.E, int(k.max(initial=0)), int((k > 256).sum()), alg))
+
+
+class Server:
+    """Request server (sbp_server_*): workers take partition requests in arrival order as a worker frees up
+    and the request's footprint fits in the HBM budget."""
+
+    def __init__(self, device=0, slots=0, hbm_budget_bytes=200 * 10 ** 9):
+        self.h = C.c_void_p()
+        rc = lib().sbp_server_create(device, slots, hbm_budget_bytes, C.byref(self.h))
+        if rc != 0:
+            raise SbpError(f"sbp_server_create failed with status {rc}")
+        self._held = {}  # ticket -> the host arrays the request reads
+
+    def check(self, rc, what):
+        if rc != 0:
+            msg = lib().sbp_server_last_error(self.h)
+            raise SbpError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def submit(self, g, prm):
+        arrs = (np.ascontiguousarray(g.out_off, np.int64), np.ascontiguousarray(g.out_adj, np.int32),
+                np.ascontiguousarray(g.in_off, np.int64), np.ascontiguousarray(g.in_adj, np.int32))
+        t = C.c_int64()
+        self.check(lib().sbp_server_submit(self.h, g.N, g.E, _p(arrs[0]), _p(arrs[1]), _p(arrs[2]), _p(arrs[3]),
+                                           C.byref(prm), C.byref(t)), "sbp_server_submit")
+        self._held[t.value] = (g.N, arrs)
+        return t.value
+
+    def wait(self, ticket):
+        """Returns (assignment, num_blocks, stats, latency_ms, queue_ms)."""
+        N, _ = self._held[ticket]
+        out = np.zeros(N, np.int32)
+        nb, st, lat, q = C.c_int32(), Stats(), C.c_double(), C.c_double()
+        rc = lib().sbp_server_wait(self.h, ticket, _p(out), C.byref(nb), C.byref(st), C.byref(lat), C.byref(q))
+        del self._held[ticket]
+        self.check(rc, f"sbp_server_wait({ticket})")
+        return out, nb.value, st, lat.value, q.value
+
+    def info(self):
+        r, q, m = C.c_int32(), C.c_int32(), C.c_int32()
+        b, u, pk = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.check(lib().sbp_server_info(self.h, C.byref(r), C.byref(q), C.byref(m), C.byref(b), C.byref(u),
+                                         C.byref(pk)), "sbp_server_info")
+        return dict(running=r.value, queued=q.value, max_running=m.value, budget=b.value, in_use=u.value,
+                    peak_in_use=pk.value)
+
+    def close(self):
+        if self.h:
+            lib().sbp_server_destroy(self.h)
+            self.h = C.c_void_p()
+            self._held.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 SAMPLING = {"max_degree": 0, "random": 1, "degree_product": 2, "degree_weighted": 3, "expansion_snowball": 4,

@@ -1,0 +1,71 @@
+"""Host-side parts of the batched and served modes (no GPU): the mixed-size workload draws and the footprint
+bound the server admits requests by."""
+import math
+
+import numpy as np
+
+from sbp_amd import native, workloads
+
+
+def test_batched_specs_follow_the_stated_distribution():
+    specs = [workloads.batched_spec(i) for i in range(2000)]
+    assert specs[7] == workloads.batched_spec(7)  # graph i is a function of i
+    N = np.array([s["N"] for s in specs])
+    assert N.min() >= 200 and N.max() <= 50_000
+    # log-uniform: log N is uniform on [log 200, log 50000]
+    u = (np.log(N) - math.log(200)) / (math.log(50_000) - math.log(200))
+    assert abs(u.mean() - 0.5) < 0.03 and abs(np.quantile(u, 0.25) - 0.25) < 0.04
+    for s in specs[:50]:
+        assert s["B"] == max(1, round(s["N"] ** 0.4))
+        assert 8 <= s["mean_degree"] <= 32 and 0.5 <= s["p_in"] <= 0.8 and s["gamma"] == 2.5
+    assert [s["seed"] for s in specs[:5]] == [0, 1, 2, 3, 4]
+
+
+def test_served_specs_and_arrivals():
+    specs = [workloads.served_spec(j) for j in range(500)]
+    N = np.array([s["N"] for s in specs])
+    assert N.min() >= 1000 and N.max() <= 2_000_000
+    capped = [workloads.served_spec(j, n_max=200_000)["N"] for j in range(500)]
+    assert max(capped) <= 200_000
+    assert all(s["mean_degree"] == 16.0 and s["p_in"] == 0.55 and s["gamma"] == 2.1 for s in specs)
+    t = workloads.poisson_arrivals(10.0, 4000, seed=0)
+    assert t[0] == 0.0 and np.all(np.diff(t) >= 0)
+    assert abs(np.diff(t).mean() - 0.1) < 0.01
+    assert np.array_equal(t, workloads.poisson_arrivals(10.0, 4000, seed=0))
+
+
+def test_generated_graph_matches_its_spec():
+    s = workloads.batched_spec(3, n_max=3000)
+    g = workloads.make(s)
+    assert g.N == s["N"] and len(np.unique(g.truth)) == s["B"]
+    deg = np.diff(g.out_off)
+    assert deg.min() >= 1 and deg.max() <= s["kmax"]
+
+
+def test_footprint_bound_is_monotone_and_covers_the_graph():
+    L = native.lib()  # host-only functions: no device needed
+    prev = 0
+    for N, E in [(200, 2000), (1000, 16000), (50_000, 800_000), (2_000_000, 32_000_000)]:
+        g, f = native.graph_bytes(N, E), int(L.sbp_footprint_bytes(N, E, 100, 0, 0))
+        assert 0 < g < f and f > prev
+        assert g >= 16 * (N + 1) + 8 * E
+        prev = f
+    base = int(L.sbp_footprint_bytes(1000, 20000, 100, 0, 0))
+    assert int(L.sbp_footprint_bytes(1000, 40000, 100, 0, 0)) > base
+    # high-degree vertices add one neighbour-block table each; Metropolis-Hastings adds its slack layout
+    assert int(L.sbp_footprint_bytes(1000, 20000, 900, 10, 0)) > base
+    assert int(L.sbp_footprint_bytes(1000, 20000, 900, 20, 0)) > int(L.sbp_footprint_bytes(1000, 20000, 900, 10, 0))
+    assert int(L.sbp_footprint_bytes(1000, 20000, 100, 0, 1)) > base
+    assert int(L.sbp_footprint_bytes(1000, 20000, 100, 0, 2)) == int(L.sbp_footprint_bytes(1000, 20000, 100, 0, 1))
+    # the tables are capped at 1 GiB: the largest served request stays a small part of a 200 GB budget
+    big = int(L.sbp_footprint_bytes(2_000_000, 32_000_000, 2000, 100_000, 0))
+    assert big < 20e9
+    assert big - int(L.sbp_footprint_bytes(2_000_000, 32_000_000, 2000, 1, 0)) <= (1 << 30) * 5 // 4 + (1 << 20)
+
+
+def test_footprint_of_a_graph():
+    from sbp_amd.graph import dcsbm
+    g = dcsbm(3000, 20, gamma=2.1, kmin=1, kmax=1000, mean_degree=12, seed=1)
+    k = np.diff(g.out_off) + np.diff(g.in_off)
+    assert native.footprint_bytes(g, "async_gibbs") == int(native.lib().sbp_footprint_bytes(
+        g.N, g.E, int(k.max()), int((k > 256).sum()), 0))
