@@ -151,7 +151,9 @@ def batched_leg(ctx, graphs, algorithm, slots):
     prms = [native.params(algorithm, seed=i) for i in range(len(graphs))]
     dgs = [ctx.upload(g) for g in graphs]
     ctx.synchronize()
-    ctx.partition_batch(dgs[:slots], prms[:slots], slots=slots)  # creates and warms the worker contexts
+    # creates the worker contexts and grows their buffers on the largest graphs
+    big = sorted(range(len(graphs)), key=lambda i: -graphs[i].E)[:slots]
+    ctx.partition_batch([dgs[i] for i in big], [prms[i] for i in big], slots=slots)
     t0 = time.perf_counter()
     plain = [dg.partition(p) for dg, p in zip(dgs, prms)]
     t_plain = time.perf_counter() - t0
@@ -248,11 +250,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-graphs", type=int, default=64, help="graphs in the batched leg (0: skip it)")
-    ap.add_argument("--batched-slots", type=int, default=8)
-    ap.add_argument("--served-requests", type=int, default=24, help="requests in the served leg (0: skip it)")
-    ap.add_argument("--served-rate", type=float, default=8.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--batched-slots", type=int, default=16)
+    ap.add_argument("--served-requests", type=int, default=32, help="requests in the served leg (0: skip it)")
+    ap.add_argument("--served-rate", type=float, default=32.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--served-max-n", type=int, default=200_000, help="upper bound of the requests' N")
-    ap.add_argument("--served-slots", type=int, default=8)
+    ap.add_argument("--served-slots", type=int, default=16)
     ap.add_argument("--many-algorithm", default="async_gibbs", help="algorithm of the batched and served legs")
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
                     help="only copy MIB MiB device-to-device 3 times (counter calibration) and exit")

@@ -269,7 +269,7 @@ static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, 
     std::vector<int32_t> block_map, sizes(B);
     SBP_HIP(hipMemcpy(sizes.data(), bm->sizes, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
     sbpcarry::Model model;
-    std::unique_ptr<sbpcarry::Pool> pool;
+    sbpcarry::Pool *pool = nullptr;
     if (!approximate) {  // the advanced carry-out re-evaluates merges against the changing blockmodel
         std::vector<int32_t> rp(B + 1), cp(B + 1), dout(B), din(B);
         std::vector<int2> rc(bm->nnz), cc(bm->nnz);
@@ -281,10 +281,13 @@ static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, 
             SBP_HIP(hipMemcpy(rc.data(), bm->row_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
             SBP_HIP(hipMemcpy(cc.data(), bm->col_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
         }
-        pool.reset(new sbpcarry::Pool(ctx->carry_threads > 0 ? ctx->carry_threads : sbpcarry::carry_threads()));
+        const int threads = ctx->carry_threads > 0 ? ctx->carry_threads : sbpcarry::carry_threads();
+        if (!ctx->carry_pool || ctx->carry_pool->threads() != threads)
+            ctx->carry_pool = std::make_shared<sbpcarry::Pool>(threads);
+        pool = ctx->carry_pool.get();
         const int32_t *rcp = reinterpret_cast<const int32_t *>(rc.data());
         const int32_t *ccp = reinterpret_cast<const int32_t *>(cc.data());
-        model.load(B, rp.data(), rcp, rcp + 1, cp.data(), ccp, ccp + 1, 2, dout.data(), din.data(), pool.get());
+        model.load(B, rp.data(), rcp, rcp + 1, cp.data(), ccp, ccp + 1, 2, dout.data(), din.data(), pool);
         if (bm->np) model.set_nonparametric(sizes.data(), bm->N);
         if (bm->dc) {
             SBP_CHECK(bm->hist_ready, SBP_E_STATE, "degree histograms not built");
@@ -303,7 +306,7 @@ static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, 
     const double t_loaded = now_ms();
     sbpcarry::CarryStats cs;
     int32_t new_B = sbpcarry::carry_out(approximate ? nullptr : &model, B, best_s.data(), best_dS.data(),
-                                        sizes.data(), num_to_merge, approximate, block_map, &cs, pool.get());
+                                        sizes.data(), num_to_merge, approximate, block_map, &cs, pool);
     if (st) {
         st->carry_ms += now_ms() - t0;
         st->carry_load_ms += t_loaded - t0;

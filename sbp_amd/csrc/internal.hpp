@@ -17,6 +17,10 @@
 #include "device_common.hpp"
 #include "np_entropy.hpp"
 
+namespace sbpcarry {
+class Pool;
+}
+
 namespace sbpimpl {
 
 struct Error : std::runtime_error {
@@ -146,6 +150,7 @@ struct sbp_ctx_s {
     int hash_clean_lg = -1;
     int64_t hash_clean_tables = 0;
     int carry_threads = 0;    // host threads of the merge carry-out (0: SBP_AMD_CARRY_THREADS or up to 16)
+    std::shared_ptr<sbpcarry::Pool> carry_pool;  // the carry-out's threads, kept between merge phases
     uint64_t peak_bytes = 0;  // largest device footprint seen at the end of a partition (sbp_ctx_memory)
     std::vector<sbp_ctx> slots;  // worker contexts of sbp_partition_batch, created on first use
 };
