@@ -154,13 +154,16 @@ def batched_leg(ctx, graphs, algorithm, slots):
     # creates the worker contexts and grows their buffers on the largest graphs
     big = sorted(range(len(graphs)), key=lambda i: -graphs[i].E)[:slots]
     ctx.partition_batch([dgs[i] for i in big], [prms[i] for i in big], slots=slots)
+    ctx.debug_logf(0, 1)  # a one-thread kernel that marks the start of each phase in a kernel trace
     t0 = time.perf_counter()
     plain = [dg.partition(p) for dg, p in zip(dgs, prms)]
     t_plain = time.perf_counter() - t0
+    ctx.debug_logf(0, 1)
     print(f"[bench] batched: plain path {len(graphs)} graphs in {t_plain:.2f}s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     outs, nb, st, fin = ctx.partition_batch(dgs, prms, slots=slots)
     t_batch = time.perf_counter() - t0
+    ctx.debug_logf(0, 1)
     print(f"[bench] batched: {slots} slots in {t_batch:.2f}s", file=sys.stderr, flush=True)
     same = all(np.array_equal(outs[i], plain[i][0]) and int(nb[i]) == plain[i][1] and st[i].mdl == plain[i][2].mdl
                for i in range(len(graphs)))
@@ -256,6 +259,8 @@ def main():
     ap.add_argument("--served-max-n", type=int, default=200_000, help="upper bound of the requests' N")
     ap.add_argument("--served-slots", type=int, default=16)
     ap.add_argument("--many-algorithm", default="async_gibbs", help="algorithm of the batched and served legs")
+    ap.add_argument("--only", choices=["batched", "served"],
+                    help="run only that leg and print its JSON object (for profiling)")
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
                     help="only copy MIB MiB device-to-device 3 times (counter calibration) and exit")
     args = ap.parse_args()
@@ -277,6 +282,20 @@ def main():
 
     from sbp_amd import native
     from sbp_amd.graph import dcsbm
+
+    if args.only:
+        many_b = [f.result() for f in fut_b]
+        many_s = [f.result() for f in fut_s]
+        pool.shutdown()
+        if args.only == "batched":
+            ctx = native.Context(0, seed=0)
+            out = batched_leg(ctx, many_b, args.many_algorithm, args.batched_slots)
+        else:
+            arrivals = workloads.poisson_arrivals(args.served_rate, args.served_requests, seed=0)
+            out = served_leg(many_s, arrivals, args.many_algorithm, args.served_slots, args.served_rate,
+                             args.served_max_n)
+        print(json.dumps({args.only: out}))
+        return
 
     w = WORKLOADS[args.workload]
     t0 = time.perf_counter()
