@@ -152,6 +152,28 @@ double null_mdl_v1(const Graph &graph);
 double normalize_mdl_v1(double mdl, const Graph &graph);
 }  // namespace entropy
 
+namespace Hungarian {
+/// The contingency-table type of the reference's evaluation (extern/hungarian): rows x columns of counts.
+typedef std::vector<std::vector<int>> Matrix;
+}  // namespace Hungarian
+
+namespace evaluate {
+/// include/evaluate.hpp:16-20
+struct Eval {
+    double f1_score;
+    double nmi;
+    double true_mdl;
+};
+/// src/evaluate.cpp:9-59 (on the host; prints Accuracy, Rand index, Recall, Precision and F1 Score)
+double calculate_f1_score(long num_vertices, Hungarian::Matrix &contingency_table);
+/// src/evaluate.cpp:61-96
+double calculate_nmi(long num_vertices, Hungarian::Matrix &contingency_table);
+/// src/evaluate.cpp:134-210: the table matched by maximum weight, rows = the blockmodel's communities
+Hungarian::Matrix hungarian(const Graph &graph, Blockmodel &blockmodel);
+/// src/evaluate.cpp:98-112: F1, NMI and the description length of the true partition
+Eval evaluate_blockmodel(const Graph &graph, Blockmodel &blockmodel);
+}  // namespace evaluate
+
 namespace sample {
 /// include/sample.hpp:27-30: the sampled subgraph and mapping[v] = sample id of v, or -1.
 struct Sample {

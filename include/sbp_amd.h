@@ -253,6 +253,25 @@ int sbp_server_info(sbp_server srv, int32_t *running, int32_t *queued, int32_t *
  * sbp_server_wait has returned. */
 int sbp_server_destroy(sbp_server srv);
 
+/* ----------------------------------------------------------------- evaluation (host only) */
+
+/* Scores of a partition against a true one (evaluate::Eval and the values calculate_f1_score prints,
+ * src/evaluate.cpp:9-96).  The reference's names are kept: recall = same-pair agreement over the pairs
+ * within the table's rows (the algorithm's communities), precision over the pairs within its columns. */
+typedef struct {
+    double f1_score, nmi, accuracy, rand_index, precision, recall;
+    int32_t found_blocks, true_blocks;  /* table rows, columns */
+} sbp_evaluation;
+/* Replaces calculate_f1_score + calculate_nmi (src/evaluate.cpp:9-96) on a contingency table (row-major
+ * int32[nrows * ncols], rows = the algorithm's communities); accuracy = diagonal / num_vertices. */
+int sbp_evaluate_table(int64_t num_vertices, int32_t nrows, int32_t ncols, const int32_t *table, sbp_evaluation *out);
+/* Replaces evaluate::hungarian + the two above (src/evaluate.cpp:134-210): builds the table from the true
+ * assignment (-1 = unknown, left out) and the algorithm's (block ids >= 0), matches communities by maximum
+ * weight (not above 4096 communities: then accuracy is NaN) and scores it.  Optional outputs: the matched
+ * table, row-major int32[found_blocks * true_blocks] (rows = the algorithm's communities), and its shape. */
+int sbp_evaluate(int32_t N, const int32_t *true_assignment, const int32_t *assignment, sbp_evaluation *out,
+                 int32_t *table_out, int32_t *table_rows, int32_t *table_cols);
+
 /* ----------------------------------------------------------------- diagnostics */
 
 /* Device logf over float bit patterns [first_bits, first_bits + count) (checks the restated glibc logf). */

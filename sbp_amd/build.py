@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsbp_amd.so")
 SOURCES = ["csrc/build.hip", "csrc/mdl.hip", "csrc/finetune.hip", "csrc/merge.hip", "csrc/graph.hip", "csrc/capi.hip",
-           "host/sbp_api.cpp", "host/args.cpp", "host/batch.cpp"]
+           "host/sbp_api.cpp", "host/args.cpp", "host/batch.cpp", "host/evaluate.cpp"]
 CLI = os.path.join(HERE, "sbp_cli")
 CARRY_LIB = os.path.join(HERE, "libsbp_carry.so")  # host-only carry-out, for CPU-side checks
 

@@ -1,5 +1,6 @@
 // Command-line driver with the reference's flags (src/main.cpp:64-164, single-rank path): load the graph,
-// partition it on the device, write a JSON result and, when a true partition is given, report NMI.
+// partition it on the device, write a JSON result and, when a true partition is given, evaluate it
+// (evaluate::evaluate_blockmodel: F1, NMI and the other pair-counting scores).
 #include <math.h>
 #include <stdio.h>
 
@@ -10,25 +11,6 @@
 #include <vector>
 
 #include "../../include/sbp/sbp.hpp"
-
-// evaluate::calculate_nmi (src/evaluate.cpp:97-132): MI / sqrt(H(truth) H(found)).
-static double nmi(const std::vector<long> &a, const std::vector<long> &b) {
-    std::map<long, double> pa, pb;
-    std::map<std::pair<long, long>, double> pab;
-    double n = 0;
-    for (size_t i = 0; i < a.size(); ++i) {
-        if (a[i] < 0) continue;
-        pa[a[i]] += 1;
-        pb[b[i]] += 1;
-        pab[{a[i], b[i]}] += 1;
-        n += 1;
-    }
-    double mi = 0, ha = 0, hb = 0;
-    for (auto &x : pab) mi += x.second / n * log((x.second / n) / ((pa[x.first.first] / n) * (pb[x.first.second] / n)));
-    for (auto &x : pa) ha -= x.second / n * log(x.second / n);
-    for (auto &x : pb) hb -= x.second / n * log(x.second / n);
-    return (ha > 0 && hb > 0) ? mi / sqrt(ha * hb) : 1.0;
-}
 
 int main(int argc, char **argv) {
     try {
@@ -56,15 +38,20 @@ int main(int argc, char **argv) {
         double runtime = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         const sbp_stats &st = sbp::last_stats();
         std::vector<long> a = bm.block_assignment();
-        double score = graph.assignment().empty() ? -1.0 : nmi(graph.assignment(), a);
+        double score = -1.0, f1 = -1.0;
+        if (!graph.assignment().empty()) {  // evaluate_partition (src/main.cpp:41-53)
+            evaluate::Eval e = evaluate::evaluate_blockmodel(graph, bm);
+            score = e.nmi;
+            f1 = e.f1_score;
+        }
         std::string out = args.output_file.empty() ? args.json + ".json" : args.output_file;
         FILE *f = fopen(out.c_str(), "w");
         if (!f) throw std::runtime_error("cannot write " + out);
-        fprintf(f, "{\"num_blocks\": %ld, \"mdl\": %.17g, \"normalized_mdl_v1\": %.17g, \"nmi\": %.6f, "
+        fprintf(f, "{\"num_blocks\": %ld, \"mdl\": %.17g, \"normalized_mdl_v1\": %.17g, \"nmi\": %.6f, \"f1_score\": %.6f, "
                    "\"runtime\": %.6f, \"mcmc_proposals\": %ld, \"mcmc_moves\": %ld, \"sweeps\": %ld, "
                    "\"merge_proposals\": %ld, \"assignment\": [",
                 bm.num_blocks(), bm.getOverall_entropy(), entropy::normalize_mdl_v1(bm.getOverall_entropy(), graph),
-                score, runtime, (long)st.proposals, (long)st.moves, (long)st.sweeps, (long)st.merge_proposals);
+                score, f1, runtime, (long)st.proposals, (long)st.moves, (long)st.sweeps, (long)st.merge_proposals);
         for (size_t i = 0; i < a.size(); ++i) fprintf(f, "%s%ld", i ? ", " : "", a[i]);
         fprintf(f, "]}\n");
         fclose(f);

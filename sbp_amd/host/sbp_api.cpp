@@ -12,6 +12,7 @@
 #include <numeric>
 #include <sstream>
 #include <stdexcept>
+#include <unordered_set>
 
 #include "../../include/sbp/sbp.hpp"
 
@@ -430,6 +431,66 @@ double null_mdl_v1(const Graph &graph) {
 double normalize_mdl_v1(double mdl, const Graph &graph) { return mdl / null_mdl_v1(graph); }
 
 }  // namespace entropy
+
+namespace evaluate {
+
+static sbp_evaluation score_table(long num_vertices, const Hungarian::Matrix &t) {
+    if (t.empty() || t[0].empty()) throw std::invalid_argument("evaluate: empty contingency table");
+    const int32_t nr = (int32_t)t.size(), nc = (int32_t)t[0].size();
+    std::vector<int32_t> flat((size_t)nr * nc);
+    for (int32_t r = 0; r < nr; ++r)
+        for (int32_t c = 0; c < nc; ++c) flat[(size_t)r * nc + c] = t[r][c];
+    sbp_evaluation e{};
+    if (sbp_evaluate_table(num_vertices, nr, nc, flat.data(), &e) != SBP_OK)
+        throw std::invalid_argument("evaluate: invalid contingency table");
+    return e;
+}
+
+double calculate_f1_score(long num_vertices, Hungarian::Matrix &contingency_table) {
+    const sbp_evaluation e = score_table(num_vertices, contingency_table);
+    printf("Accuracy: %g\nRand index: %g\nRecall: %g\nPrecision: %g\nF1 Score: %g\n", e.accuracy, e.rand_index,
+           e.recall, e.precision, e.f1_score);
+    return e.f1_score;
+}
+
+double calculate_nmi(long num_vertices, Hungarian::Matrix &contingency_table) {
+    return score_table(num_vertices, contingency_table).nmi;
+}
+
+Hungarian::Matrix hungarian(const Graph &graph, Blockmodel &blockmodel) {
+    const std::vector<long> &truth = graph.assignment();
+    const long N = graph.num_vertices();
+    if ((long)truth.size() != N) throw std::invalid_argument("evaluate: the graph has no true assignment");
+    std::vector<long> found = blockmodel.block_assignment();
+    std::vector<int32_t> t(N), a(N);
+    for (long i = 0; i < N; ++i) t[i] = (int32_t)truth[i], a[i] = (int32_t)found[i];
+    std::unordered_set<long> labels;
+    for (long x : truth)
+        if (x >= 0) labels.insert(x);
+    long found_blocks = 0;
+    for (long x : found) found_blocks = std::max(found_blocks, x + 1);
+    std::vector<int32_t> table((size_t)std::max<long>(1, found_blocks * (long)labels.size()));
+    int32_t rows = 0, cols = 0;
+    sbp_evaluation e{};
+    if (sbp_evaluate((int32_t)N, t.data(), a.data(), &e, table.data(), &rows, &cols) != SBP_OK)
+        throw std::invalid_argument("evaluate: invalid assignments");
+    Hungarian::Matrix m(rows, std::vector<int>(cols));
+    for (int32_t r = 0; r < rows; ++r)
+        for (int32_t c = 0; c < cols; ++c) m[r][c] = table[(size_t)r * cols + c];
+    return m;
+}
+
+Eval evaluate_blockmodel(const Graph &graph, Blockmodel &blockmodel) {
+    Hungarian::Matrix table = hungarian(graph, blockmodel);
+    const double f1 = calculate_f1_score(graph.num_vertices(), table);
+    const double nmi = calculate_nmi(graph.num_vertices(), table);
+    std::vector<long> truth(graph.assignment());
+    const long true_blocks = 1 + *std::max_element(truth.begin(), truth.end());
+    Blockmodel true_blockmodel(true_blocks, graph, 0.5, truth);
+    return Eval{f1, nmi, entropy::mdl(true_blockmodel, graph)};
+}
+
+}  // namespace evaluate
 
 namespace block_merge {
 

@@ -27,6 +27,7 @@ EXPORTS = [
     "sbp_partition_batch",
     "sbp_server_create", "sbp_server_last_error", "sbp_server_submit", "sbp_server_wait", "sbp_server_info",
     "sbp_server_destroy",
+    "sbp_evaluate_table", "sbp_evaluate",
     "sbp_debug_logf",
 ]
 
@@ -53,6 +54,16 @@ class Stats(C.Structure):
                 ("carry_reused", C.c_int64), ("carry_requeues", C.c_int64), ("carry_load_ms", C.c_double),
                 ("ft_high_bytes", C.c_uint64), ("ft_high_launches", C.c_int64), ("ft_high_kernel_ms", C.c_double),
                 ("mh_bytes", C.c_uint64), ("mh_launches", C.c_int64), ("mh_kernel_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Evaluation(C.Structure):
+    """sbp_evaluation: scores of a partition against a true one (src/evaluate.cpp:9-96)."""
+    _fields_ = [("f1_score", C.c_double), ("nmi", C.c_double), ("accuracy", C.c_double),
+                ("rand_index", C.c_double), ("precision", C.c_double), ("recall", C.c_double),
+                ("found_blocks", C.c_int32), ("true_blocks", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -117,6 +128,8 @@ def lib():
             "sbp_server_wait": [vp, i64, vp, vp, S, vp, vp],
             "sbp_server_info": [vp, vp, vp, vp, vp, vp, vp],
             "sbp_server_destroy": [vp],
+            "sbp_evaluate_table": [i64, i32, i32, vp, C.POINTER(Evaluation)],
+            "sbp_evaluate": [i32, vp, vp, C.POINTER(Evaluation), vp, vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -203,6 +216,29 @@ class Context:
         self.check(lib().sbp_partition_batch(self.h, hs, n, ps, slots, ptrs, _p(nb), st, _p(fin)),
                    "sbp_partition_batch")
         return outs, nb, list(st), fin
+
+
+def evaluate_table(num_vertices, table):
+    """sbp_evaluate_table on a contingency table (rows = the algorithm's communities)."""
+    t = np.ascontiguousarray(table, np.int32)
+    e = Evaluation()
+    rc = lib().sbp_evaluate_table(num_vertices, t.shape[0], t.shape[1], _p(t), C.byref(e))
+    if rc != 0:
+        raise SbpError(f"sbp_evaluate_table failed ({rc})")
+    return e
+
+
+def evaluate(true_assignment, assignment):
+    """sbp_evaluate: (Evaluation, matched contingency table with rows = the algorithm's communities)."""
+    t = np.ascontiguousarray(true_assignment, np.int32)
+    a = np.ascontiguousarray(assignment, np.int32)
+    labels = len(np.unique(t[t >= 0]))
+    table = np.zeros(max(1, (int(a.max(initial=0)) + 1) * labels), np.int32)
+    e, r, c = Evaluation(), C.c_int32(), C.c_int32()
+    rc = lib().sbp_evaluate(len(t), _p(t), _p(a), C.byref(e), _p(table), C.byref(r), C.byref(c))
+    if rc != 0:
+        raise SbpError(f"sbp_evaluate failed ({rc})")
+    return e, table[: r.value * c.value].reshape(r.value, c.value)
 
 
 def graph_bytes(N, E):
