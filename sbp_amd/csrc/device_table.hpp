@@ -27,10 +27,13 @@ struct Table {
     int cap;
 };
 
-// ints of one table of capacity 2^cap_log2
-__host__ __device__ constexpr int64_t table_ints(int cap_log2) { return 4 * ((int64_t)1 << cap_log2) + 1; }
+// ints of one table of capacity 2^cap_log2 whose used-slot list holds used_cap entries (default: every slot)
+__host__ __device__ constexpr int64_t table_ints(int cap_log2, int64_t used_cap = -1) {
+    return 3 * ((int64_t)1 << cap_log2) + (used_cap < 0 ? ((int64_t)1 << cap_log2) : used_cap) + 1;
+}
 
-__device__ __forceinline__ Table table_at(int *base, int cap_log2) {
+// A table takes at most used_cap distinct keys (default: its capacity).
+__device__ __forceinline__ Table table_at(int *base, int cap_log2, int used_cap = -1) {
     Table t;
     t.cap = 1 << cap_log2;
     t.shift = 32 - cap_log2;
@@ -38,7 +41,7 @@ __device__ __forceinline__ Table table_at(int *base, int cap_log2) {
     t.outc = t.keys + t.cap;
     t.inc = t.outc + t.cap;
     t.used = t.inc + t.cap;
-    t.n_used = t.used + t.cap;
+    t.n_used = t.used + (used_cap < 0 ? t.cap : used_cap);
     t.mask = (unsigned)t.cap - 1;
     return t;
 }

@@ -42,6 +42,10 @@ using namespace sbpdev;
 constexpr int FT_BLOCK = 256;
 constexpr int FT_WAVES = FT_BLOCK / WAVE;
 constexpr int LDS_CAP_LOG2 = 9;
+// The LDS table serves vertices with |out| + |in| <= SMALL_K, whose distinct neighbour blocks are at most
+// SMALL_K: its used-slot list needs no more entries, which keeps five 4-wave blocks within a CU's 160 KB.
+constexpr int LDS_USED = FT_SMALL_DEGREE;
+constexpr int64_t LDS_TABLE_INTS = table_ints(LDS_CAP_LOG2, LDS_USED);
 constexpr int LDS_CAP = 1 << LDS_CAP_LOG2;  // hash slots per wave
 constexpr int SMALL_K = FT_SMALL_DEGREE;  // max |out| + |in| for the LDS table
 static_assert(2 * SMALL_K <= LDS_CAP, "LDS table load factor above 1/2");
@@ -126,7 +130,7 @@ __device__ __forceinline__ long long delta_row_s(const MoveCtx &m, int c, int oc
 
 __device__ __forceinline__ Table make_table(const FtArgs &a, bool big, int *lds, int64_t gwave) {
     if (big) return table_at(a.gtable + gwave * table_ints(a.gcap_log2), a.gcap_log2);
-    return table_at(lds, LDS_CAP_LOG2);
+    return table_at(lds, LDS_CAP_LOG2, LDS_USED);
 }
 
 // Proposal of vertex v against the current state.  Returns s (== r for "no move").  Fills m.
@@ -461,12 +465,12 @@ __device__ __forceinline__ double evaluate(const FtArgs &a, MoveCtx &m, const Ta
 }
 
 template <bool BIG, bool GREEDY, bool NP>
-__global__ void __launch_bounds__(FT_BLOCK) k_ft_propose(FtArgs a) {
-    __shared__ int lds[BIG ? 1 : FT_WAVES * table_ints(LDS_CAP_LOG2)];
+__global__ void __launch_bounds__(FT_BLOCK) __attribute__((amdgpu_waves_per_eu(BIG ? 3 : 4))) k_ft_propose(FtArgs a) {
+    __shared__ int lds[BIG ? 1 : FT_WAVES * LDS_TABLE_INTS];
     const int lane = lane_id(), wid = threadIdx.x / WAVE;
     const int64_t gwave = (int64_t)blockIdx.x * FT_WAVES + wid;
     const int64_t nwaves = (int64_t)gridDim.x * FT_WAVES;
-    const Table t = make_table(a, BIG, lds + (BIG ? 0 : wid * table_ints(LDS_CAP_LOG2)), gwave);
+    const Table t = make_table(a, BIG, lds + (BIG ? 0 : wid * LDS_TABLE_INTS), gwave);
     if (!BIG) table_clear_all(t);  // the global tables arrive empty (prepare_tables) and are left empty
     const int64_t count = BIG ? (int64_t)*a.big_count : a.count;
     unsigned long long bytes = 0;
@@ -938,7 +942,7 @@ __device__ __forceinline__ bool lacks_room(const FtArgs &a, const MhState &h, in
 
 template <bool GREEDY, bool NP>
 __global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start) {
-    __shared__ int lds[table_ints(LDS_CAP_LOG2)];
+    __shared__ int lds[LDS_TABLE_INTS];
     const int lane = lane_id();
     unsigned long long bytes = 0, moves = 0;
     double sum = *h.sum;

@@ -146,7 +146,7 @@ __device__ double merge_delta_mdl_np(const MgArgs &a, int r, int s) {
 }
 
 template <bool NP>
-__global__ void __launch_bounds__(MG_BLOCK) k_merge_propose(MgArgs a) {
+__global__ void __launch_bounds__(MG_BLOCK) __attribute__((amdgpu_waves_per_eu(NP ? 1 : 4))) k_merge_propose(MgArgs a) {
     const int lane = lane_id(), wid = threadIdx.x / WAVE;
     const int64_t nwaves = (int64_t)gridDim.x * MG_WAVES;
     unsigned long long bytes = 0;
