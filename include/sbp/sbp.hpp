@@ -185,6 +185,11 @@ struct Sample {
 Sample sample(const Graph &graph);
 /// src/sample.cpp:157-200: block of every vertex of `graph` from the sample's blockmodel.
 std::vector<long> extend(const Graph &graph, const Blockmodel &sample_blockmodel, const Sample &sample);
+/// src/sample.cpp:12-26: the subgraph of the vertices of degree > 1 (--detach).
+Sample detach(const Graph &graph);
+/// src/sample.cpp:367-405: the full graph's blockmodel from the detached graph's (random draws keyed by
+/// args.seed).
+Blockmodel reattach(const Graph &graph, const Blockmodel &sample_blockmodel, const Sample &sample);
 }  // namespace sample
 
 namespace sbp_amd {

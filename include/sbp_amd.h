@@ -119,6 +119,18 @@ int sbp_graph_download(sbp_ctx ctx, sbp_graph g, int64_t *out_off, int32_t *out_
  * mapping_out: host int32[N] (vertex -> sample vertex or -1), may be NULL. */
 int sbp_sample(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, uint64_t seed, sbp_graph *sample_out,
                int32_t *num_sampled, int32_t *mapping_out);
+/* Replaces sample::detach (src/sample.cpp:12-26): the induced subgraph of the vertices of degree > 1
+ * (Graph::degrees, |out| + |in| - self-loop flag) in id order.  mapping_out: host int32[N] (vertex ->
+ * detached vertex or -1), may be NULL. */
+int sbp_detach(sbp_ctx ctx, sbp_graph g, sbp_graph *detached_out, int32_t *num_kept, int32_t *mapping_out);
+/* Replaces sample::reattach (src/sample.cpp:367-405) on the host (sequential by definition): the assignment
+ * of every vertex of the full graph (host CSR) from the detached graph's assignment.  Vertices left out draw
+ * random_integer(0, detached_blocks - 1) from the replay stream Philox(seed; j, 0, 0x52410001, 0), j counting
+ * the draws, and take the blocks of their neighbours as the reference's loop does.  The caller builds the
+ * blockmodel with sbp_bm_build(ctx, g, detached_blocks, assignment_out, ...). */
+int sbp_reattach(int32_t N, const int64_t *out_off, const int32_t *out_adj, const int64_t *in_off,
+                 const int32_t *in_adj, const int32_t *mapping, const int32_t *detached_assignment,
+                 int32_t detached_blocks, uint64_t seed, int32_t *assignment_out);
 /* Replaces sample::extend (src/sample.cpp:157-200) followed by Blockmodel(B, graph, 0.5, assignment)
  * (src/main.cpp:139): extends sample_bm, a blockmodel of `sample`, to g and builds the full-graph
  * blockmodel with the same B.  assignment_out: host int32[N], may be NULL. */

@@ -90,6 +90,8 @@ def lib():
             "orc_degree_weight_key": (C.c_float, [u32, i64]),
             "orc_subgraph": (i64, [G, vp, vp, i32, vp, vp, vp, vp]),
             "orc_extend": (C.c_int, [G, vp, vp, i32, u64, vp]),
+            "orc_detach": (i64, [G, vp, vp]),
+            "orc_reattach": (C.c_int, [G, vp, vp, i32, u64, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -325,6 +327,23 @@ def subgraph(graph, mapping, vertices):
     in_adj = np.zeros(max(graph.E, 1), np.int32)
     E = lib().orc_subgraph(graph.ref(), _p(m), _p(vs), n, _p(out_off), _p(out_adj), _p(in_off), _p(in_adj))
     return OGraph(out_off, out_adj[:E], in_off, in_adj[:E])
+
+
+def detach(graph):
+    """orc_detach (sample::detach): (mapping[N] -> detached id or -1, vertices[n])."""
+    mapping = np.zeros(graph.N, np.int32)
+    verts = np.zeros(graph.N, np.int32)
+    n = lib().orc_detach(graph.ref(), _p(mapping), _p(verts))
+    return mapping, verts[:n]
+
+
+def reattach(graph, mapping, detached_assignment, detached_blocks, seed=0):
+    """orc_reattach (sample::reattach): the full graph's assignment."""
+    out = np.zeros(graph.N, np.int32)
+    m = np.ascontiguousarray(mapping, np.int32)
+    da = np.ascontiguousarray(detached_assignment, np.int32)
+    lib().orc_reattach(graph.ref(), _p(m), _p(da), detached_blocks, seed, _p(out))
+    return out
 
 
 def extend(graph, mapping, sample_assignment, sample_blocks, seed=0):

@@ -2076,6 +2076,58 @@ int64_t orc_subgraph(const orc_graph *g, const int32_t *mapping, const int32_t *
 }
 
 /* sample::extend (src/sample.cpp:157-200): sequential plurality vote; ties to the smallest block. */
+/* sample::detach (src/sample.cpp:12-26): the vertices of degree > 1 (Graph::degrees, src/graph.cpp:24-31)
+ * in id order.  Returns their count; mapping[v] = index or -1. */
+int64_t orc_detach(const orc_graph *g, int32_t *mapping, int32_t *vertices) {
+    int64_t n = 0;
+    for (int32_t v = 0; v < g->N; ++v) {
+        mapping[v] = -1;
+        if (vertex_degree(g, v) > 1) {
+            mapping[v] = (int32_t)n;
+            vertices[n++] = v;
+        }
+    }
+    return n;
+}
+
+/* sample::reattach (src/sample.cpp:367-405).  Each vertex still unassigned at its turn (id order) draws
+ * random_integer(0, B - 1), here draw j of Philox(seed; j, 0, TAG_REATTACH, 0); then every out-neighbour,
+ * then every in-neighbour, in list order: an assigned neighbour's block is copied to the vertex, an
+ * unassigned one takes the drawn block together with the vertex and ends that list; a vertex left without a
+ * block takes the drawn one. */
+int orc_reattach(const orc_graph *g, const int32_t *mapping, const int32_t *detached_assignment,
+                 int32_t detached_blocks, uint64_t seed, int32_t *assignment) {
+    const int32_t N = g->N;
+    uint32_t next = 0;
+    for (int32_t v = 0; v < N; ++v) assignment[v] = mapping[v] >= 0 ? detached_assignment[mapping[v]] : -1;
+    for (int32_t v = 0; v < N; ++v) {
+        if (assignment[v] != -1) continue;
+        uint32_t w[4];
+        orc_philox(seed, next++, 0, ORC_TAG_REATTACH, 0, w);
+        const int32_t random_community = (int32_t)(((uint64_t)w[0] * (uint64_t)detached_blocks) >> 32);
+        for (int64_t e = g->out_off[v]; e < g->out_off[v + 1]; ++e) {
+            int32_t u = g->out_adj[e];
+            if (assignment[u] == -1) {
+                assignment[v] = random_community;
+                assignment[u] = random_community;
+                break;
+            }
+            assignment[v] = assignment[u];
+        }
+        for (int64_t e = g->in_off[v]; e < g->in_off[v + 1]; ++e) {
+            int32_t u = g->in_adj[e];
+            if (assignment[u] == -1) {
+                assignment[v] = random_community;
+                assignment[u] = random_community;
+                break;
+            }
+            assignment[v] = assignment[u];
+        }
+        if (assignment[v] < 0) assignment[v] = random_community;
+    }
+    return 0;
+}
+
 int orc_extend(const orc_graph *g, const int32_t *mapping, const int32_t *sample_assignment, int32_t sample_blocks,
                uint64_t seed, int32_t *assignment) {
     int32_t N = g->N;

@@ -27,7 +27,7 @@
  *     reference (with degree correction a term can be infinite).
  *   - The snowball samplers' random_integer(lo, hi) calls are
  *     lo + (w0 (hi - lo + 1)) >> 32 over Philox(seed; j, 0, TAG_SNOWBALL, 0),
- *     j counting the calls.
+ *     j counting the calls; reattach's the same over TAG_REATTACH.
  *   - logf is glibc's (the reference's fastlog / nat_log are logf).
  *   - delta-MDL sums the Delta cells and the cells at rows/columns {r, s} term by term, and the remaining
  *     unchanged cells of rows r, s / columns r, s in bulk: such a cell changes only through one block
@@ -74,6 +74,7 @@ extern "C" {
 #define ORC_TAG_EXTEND  0x45580001u  /* sample extension random block: unit = vertex */
 #define ORC_TAG_SAMPLE  0x53410001u  /* random sampling key: unit = vertex */
 #define ORC_TAG_SNOWBALL 0x53410002u  /* snowball samplers: unit = draw index */
+#define ORC_TAG_REATTACH 0x52410001u  /* reattach: unit = draw index */
 
 typedef struct {
     int32_t N;
@@ -186,6 +187,9 @@ int64_t  orc_subgraph(const orc_graph *g, const int32_t *mapping, const int32_t 
                       int64_t *out_off, int32_t *out_adj, int64_t *in_off, int32_t *in_adj);
 int      orc_extend(const orc_graph *g, const int32_t *mapping, const int32_t *sample_assignment,
                     int32_t sample_blocks, uint64_t seed, int32_t *assignment_out);
+int64_t  orc_detach(const orc_graph *g, int32_t *mapping, int32_t *vertices);
+int      orc_reattach(const orc_graph *g, const int32_t *mapping, const int32_t *detached_assignment,
+                      int32_t detached_blocks, uint64_t seed, int32_t *assignment_out);
 
 #ifdef __cplusplus
 }

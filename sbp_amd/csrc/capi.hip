@@ -18,6 +18,7 @@
 #include "internal.hpp"
 #include "carry.hpp"
 #include "host_blockmodel.hpp"
+#include "snowball.hpp"
 #include "../../include/sbp/sbp.hpp"
 
 using namespace sbpimpl;
@@ -487,7 +488,7 @@ int sbp_graph_download(sbp_ctx ctx, sbp_graph g, int64_t *out_off, int32_t *out_
 
 int sbp_sample(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, uint64_t seed, sbp_graph *sample_out,
                int32_t *num_sampled, int32_t *mapping_out) {
-    if (!ctx || !g || !sample_out || !(fraction >= 0.0 && fraction <= 1.0)) return SBP_E_ARG;
+    if (!ctx || !g || !sample_out || !(fraction >= 0.0 && fraction <= 1.0) || alg < 0 || alg > 5) return SBP_E_ARG;
     *sample_out = nullptr;
     SBP_API_BEGIN
     sbp_graph s = sample_graph(ctx, g, alg, fraction, seed);
@@ -495,6 +496,32 @@ int sbp_sample(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, uint64_t 
     if (mapping_out) SBP_HIP(hipMemcpy(mapping_out, s->parent_map, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
     *sample_out = s;
     SBP_API_END(ctx)
+}
+
+int sbp_detach(sbp_ctx ctx, sbp_graph g, sbp_graph *detached_out, int32_t *num_kept, int32_t *mapping_out) {
+    if (!ctx || !g || !detached_out) return SBP_E_ARG;
+    *detached_out = nullptr;
+    SBP_API_BEGIN
+    sbp_graph s = sample_graph(ctx, g, SAMPLE_DETACH, 1.0, 0);
+    if (num_kept) *num_kept = s->N;
+    if (mapping_out) SBP_HIP(hipMemcpy(mapping_out, s->parent_map, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
+    *detached_out = s;
+    SBP_API_END(ctx)
+}
+
+int sbp_reattach(int32_t N, const int64_t *out_off, const int32_t *out_adj, const int64_t *in_off,
+                 const int32_t *in_adj, const int32_t *mapping, const int32_t *detached_assignment,
+                 int32_t detached_blocks, uint64_t seed, int32_t *assignment_out) {
+    if (N <= 0 || !out_off || !in_off || !mapping || !detached_assignment || detached_blocks < 1 || !assignment_out)
+        return SBP_E_ARG;
+    if ((out_off[N] > 0 && !out_adj) || (in_off[N] > 0 && !in_adj)) return SBP_E_ARG;
+    try {
+        const sbpsnow::Csr c{N, out_off, in_off, out_adj, in_adj};
+        sbpsnow::reattach(c, mapping, detached_assignment, detached_blocks, seed, assignment_out);
+    } catch (const std::bad_alloc &) {
+        return SBP_E_OOM;
+    }
+    return SBP_OK;
 }
 
 int sbp_extend(sbp_ctx ctx, sbp_graph g, sbp_graph sample, sbp_bm sample_bm, uint64_t seed, sbp_bm *out,

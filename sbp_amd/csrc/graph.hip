@@ -331,11 +331,24 @@ sbp_graph sample_graph(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, u
     const int64_t target = (int64_t)(fraction * (double)N);
     SBP_CHECK(target >= 0 && target <= N, SBP_E_ARG, "sample fraction must be in [0, 1]");
     SBP_CHECK(alg == 2 || alg >= 4 || target > 0, SBP_E_ARG, "sample would be empty");
+    SBP_CHECK(alg >= 0 && alg <= SAMPLE_DETACH, SBP_E_ARG, "unknown sampling algorithm");
     DevBuf<uint64_t> keys(N), sorted(N);
     DevBuf<int32_t> order(N);
     int64_t n = target;
     std::vector<int32_t> host_order, host_mapping;  // the snowball samplers run on the host
-    if (alg == 4 || alg == 5) {
+    if (alg == SAMPLE_DETACH) {  // sample::detach (src/sample.cpp:12-26): the vertices of degree > 1, in id order
+        std::vector<int32_t> deg(N + 1);
+        SBP_HIP(hipStreamSynchronize(st));
+        SBP_HIP(hipMemcpy(deg.data(), g->degree, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+        host_mapping.assign(N, -1);
+        for (int32_t v = 0; v < N; ++v)
+            if (deg[v] > 1) {
+                host_mapping[v] = (int32_t)host_order.size();
+                host_order.push_back(v);
+            }
+        n = (int64_t)host_order.size();
+        SBP_CHECK(n > 0, SBP_E_ARG, "detach: no vertex has degree > 1");
+    } else if (alg == 4 || alg == 5) {
         std::vector<int64_t> oo(N + 1), io(N + 1);
         std::vector<int32_t> oa(E + 1), ia(E + 1), deg(N + 1);
         SBP_HIP(hipStreamSynchronize(st));

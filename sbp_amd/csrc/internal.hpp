@@ -289,6 +289,8 @@ void shuffle_keys(sbp_ctx ctx, const int32_t *d_items, int64_t n, uint32_t epoch
 // graph.hip
 void graph_finish(sbp_ctx ctx, sbp_graph g);
 const int32_t *degree_order(sbp_ctx ctx, sbp_graph g);
+// alg: sbp_sample's 0..5, or SAMPLE_DETACH (sample::detach; fraction and seed unused)
+constexpr int32_t SAMPLE_DETACH = 6;
 sbp_graph sample_graph(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, uint64_t seed);
 void extend_assignment(sbp_ctx ctx, sbp_graph g, sbp_graph sample, sbp_bm sample_bm, uint64_t seed, int32_t *d_assign);
 // merge.hip

@@ -222,4 +222,45 @@ inline std::vector<int32_t> degree_product_snowball(const Csr &g, const int32_t 
     return sampled;
 }
 
+// sample::reattach (src/sample.cpp:367-405): the assignment of every vertex of the full graph from the
+// partition of its detached graph.  Kept vertices take their block; the others, in id order, draw
+// random_integer(0, B - 1) (draw j of the stream with tag TAG_REATTACH, one per unassigned vertex) and then
+// scan their out- and then in-neighbours: each assigned neighbour's block overwrites theirs, and the first
+// unassigned neighbour gets the drawn block together with the vertex and ends that list's scan.  A vertex
+// with no neighbour keeps the drawn block.
+constexpr uint32_t TAG_REATTACH = 0x52410001u;
+
+inline void reattach(const Csr &g, const int32_t *mapping, const int32_t *sample_assignment, int32_t B, uint64_t seed,
+                     int32_t *assignment) {
+    const int32_t N = g.N;
+    for (int32_t v = 0; v < N; ++v) assignment[v] = mapping[v] >= 0 ? sample_assignment[mapping[v]] : -1;
+    uint32_t next = 0;
+    auto draw = [&]() {
+        uint32_t w[4];
+        philox(seed, next++, 0, TAG_REATTACH, 0, w);
+        return (int32_t)(((uint64_t)w[0] * (uint64_t)B) >> 32);
+    };
+    for (int32_t v = 0; v < N; ++v) {
+        if (assignment[v] != -1) continue;
+        const int32_t random_block = draw();
+        for (int64_t e = g.out_off[v]; e < g.out_off[v + 1]; ++e) {
+            const int32_t u = g.out_adj[e];
+            if (assignment[u] == -1) {
+                assignment[v] = assignment[u] = random_block;
+                break;
+            }
+            assignment[v] = assignment[u];
+        }
+        for (int64_t e = g.in_off[v]; e < g.in_off[v + 1]; ++e) {
+            const int32_t u = g.in_adj[e];
+            if (assignment[u] == -1) {
+                assignment[v] = assignment[u] = random_block;
+                break;
+            }
+            assignment[v] = assignment[u];
+        }
+        if (assignment[v] < 0) assignment[v] = random_block;
+    }
+}
+
 }  // namespace sbpsnow

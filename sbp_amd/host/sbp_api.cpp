@@ -578,6 +578,37 @@ std::vector<long> extend(const Graph &graph, const Blockmodel &sample_blockmodel
     return std::vector<long>(a.begin(), a.end());
 }
 
+Sample detach(const Graph &graph) {
+    sbp_ctx ctx = context(sbp_amd::args());
+    sbp_graph d = nullptr;
+    int32_t n = 0;
+    std::vector<int32_t> mapping(graph.num_vertices());
+    throw_on(sbp_detach(ctx, graph.device(ctx), &d, &n, mapping.data()), ctx, "sbp_detach");
+    Sample out;
+    out.graph = Graph::adopt(ctx, d);
+    out.mapping.assign(mapping.begin(), mapping.end());
+    std::vector<long> truth(n, -1);
+    const std::vector<long> &full_truth = graph.assignment();
+    if (!full_truth.empty())
+        for (long v = 0; v < graph.num_vertices(); ++v)
+            if (out.mapping[v] >= 0) truth[out.mapping[v]] = full_truth[v];
+    out.graph.assignment(std::move(truth));
+    return out;
+}
+
+Blockmodel reattach(const Graph &graph, const Blockmodel &sample_blockmodel, const Sample &sample) {
+    const long N = graph.num_vertices();
+    std::vector<int32_t> mapping(sample.mapping.begin(), sample.mapping.end());
+    std::vector<long> sa = sample_blockmodel.block_assignment();
+    std::vector<int32_t> detached(sa.begin(), sa.end()), assignment(N);
+    if (sbp_reattach((int32_t)N, graph.out_off().data(), graph.out_adj().data(), graph.in_off().data(),
+                     graph.in_adj().data(), mapping.data(), detached.data(), (int32_t)sample_blockmodel.num_blocks(),
+                     sbp_amd::args().seed, assignment.data()) != SBP_OK)
+        throw std::invalid_argument("sample::reattach: invalid arguments");
+    std::vector<long> a(assignment.begin(), assignment.end());
+    return Blockmodel(sample_blockmodel.num_blocks(), graph, 0.5, a);
+}
+
 }  // namespace sample
 
 namespace sbp {

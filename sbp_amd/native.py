@@ -17,7 +17,7 @@ ALGORITHMS = {"async_gibbs": 0, "metropolis_hastings": 1, "hybrid_mcmc": 2}
 EXPORTS = [
     "sbp_create", "sbp_destroy", "sbp_last_error", "sbp_set_seed", "sbp_synchronize",
     "sbp_graph_upload", "sbp_graph_free", "sbp_graph_info", "sbp_graph_download",
-    "sbp_sample", "sbp_extend",
+    "sbp_sample", "sbp_extend", "sbp_detach", "sbp_reattach",
     "sbp_bm_build", "sbp_bm_copy", "sbp_bm_free", "sbp_bm_info", "sbp_bm_download", "sbp_bm_validate",
     "sbp_mdl", "sbp_bm_set_model", "sbp_np_description_length",
     "sbp_ft_propose_all", "sbp_async_gibbs_sweep", "sbp_mh_sweep", "sbp_mcmc", "sbp_finetune_assignment",
@@ -101,6 +101,8 @@ def lib():
             "sbp_graph_download": [vp, vp, vp, vp, vp, vp],
             "sbp_sample": [vp, vp, i32, dbl, u64, vp, vp, vp],
             "sbp_extend": [vp, vp, vp, vp, u64, vp, vp],
+            "sbp_detach": [vp, vp, vp, vp, vp],
+            "sbp_reattach": [i32, vp, vp, vp, vp, vp, vp, i32, u64, vp],
             "sbp_bm_build": [vp, vp, i32, vp, vp],
             "sbp_bm_copy": [vp, vp, vp],
             "sbp_bm_free": [vp],
@@ -241,6 +243,20 @@ def evaluate(true_assignment, assignment):
     return e, table[: r.value * c.value].reshape(r.value, c.value)
 
 
+def reattach(g, mapping, detached_assignment, detached_blocks, seed=0):
+    """sbp_reattach (host) on a CSRGraph: the full graph's assignment."""
+    oo, oa = np.ascontiguousarray(g.out_off, np.int64), np.ascontiguousarray(g.out_adj, np.int32)
+    io, ia = np.ascontiguousarray(g.in_off, np.int64), np.ascontiguousarray(g.in_adj, np.int32)
+    m = np.ascontiguousarray(mapping, np.int32)
+    da = np.ascontiguousarray(detached_assignment, np.int32)
+    out = np.zeros(g.N, np.int32)
+    rc = lib().sbp_reattach(g.N, _p(oo), _p(oa), _p(io), _p(ia), _p(m), _p(da), detached_blocks,
+                            C.c_uint64(seed), _p(out))
+    if rc != 0:
+        raise SbpError(f"sbp_reattach failed ({rc})")
+    return out
+
+
 def graph_bytes(N, E):
     return int(lib().sbp_graph_bytes(N, E))
 
@@ -347,6 +363,14 @@ class DeviceGraph:
         mapping = np.zeros(self.N, np.int32)
         self.ctx.check(lib().sbp_sample(self.ctx.h, self.h, a, float(fraction), C.c_uint64(seed), C.byref(h),
                                         C.byref(n), _p(mapping)), "sbp_sample")
+        return DeviceGraph(self.ctx, handle=h, parent=self), mapping
+
+    def detach(self):
+        """sample::detach on the device: (detached DeviceGraph, mapping[N])."""
+        h = C.c_void_p()
+        n = C.c_int32()
+        mapping = np.zeros(self.N, np.int32)
+        self.ctx.check(lib().sbp_detach(self.ctx.h, self.h, C.byref(h), C.byref(n), _p(mapping)), "sbp_detach")
         return DeviceGraph(self.ctx, handle=h, parent=self), mapping
 
     def __del__(self):

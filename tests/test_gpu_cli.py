@@ -19,7 +19,11 @@ def run_cli(base, out, *flags):
     print("[cli]", " ".join(cmd[1:]), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=180)
     assert r.returncode == 0, r.stderr
-    return json.load(open(out))
+    text = open(out).read()
+    try:
+        return json.loads(text)
+    except json.JSONDecodeError:
+        raise AssertionError(f"not JSON: {text[:300]}")
 
 
 @pytest.fixture(scope="module")
@@ -54,3 +58,19 @@ def test_cli_sampled_run_matches_oracle(tsv, tmp_path):
     assert res["num_blocks"] == snb
     assert res["assignment"] == bm.export()["assignment"].tolist()
     assert res["mdl"] == bm.mdl()
+
+
+def test_cli_detached_run_matches_oracle(tmp_path):
+    """--detach: partition the vertices of degree > 1, reattach the rest (src/main.cpp:86-92,153-156)."""
+    from test_detach_oracle import leafy_graph
+    base = str(tmp_path / "leafy")
+    save_tsv(leafy_graph(7, N=2000, mean_degree=10), base)
+    res = run_cli(base, str(tmp_path / "d.json"), "-a", "hybrid_mcmc", "--detach")
+    g = load_tsv(base)
+    og = O.OGraph.from_csr(g)
+    m, vs = O.detach(og)
+    sa, snb, _ = O.sbp(O.subgraph(og, m, vs), O.params("hybrid_mcmc", seed=0, threads=8))
+    a = O.reattach(og, m, sa, snb, seed=0)
+    assert res["num_blocks"] == snb
+    assert res["assignment"] == a.tolist()
+    assert res["mdl"] == O.OBlockmodel.build(og, snb, a).mdl()

@@ -147,3 +147,25 @@ def test_snowball_sampled_run_bit_exact(ctx, alg):
     sbm = native.DeviceBlockmodel.build(sg, nb_d, a_d)
     _, a = sbm.extend(dg, seed=0)
     assert np.array_equal(a, O.extend(og, om, a_o, nb_o, 0))
+
+
+def test_detached_run_bit_exact(ctx):
+    """--detach (src/main.cpp:86-92,153-156): the vertices of degree > 1 are partitioned, the others
+    reattached; the detached graph, its partition and the reattached assignment are bit-exact."""
+    from test_detach_oracle import leafy_graph
+    g = leafy_graph(5, N=3000, mean_degree=10)
+    og = O.OGraph.from_csr(g)
+    dg = ctx.upload(g)
+    dd, m = dg.detach()
+    om, ovs = O.detach(og)
+    assert np.array_equal(m, om)
+    osg = O.subgraph(og, om, ovs)
+    same_graph(dd, osg)
+    print("[detached run] partition of the detached graph", flush=True)
+    a_d, nb_d, _ = dd.partition(native.params("hybrid_mcmc", seed=3))
+    a_o, nb_o, _ = O.sbp(osg, O.params("hybrid_mcmc", seed=3, threads=8))
+    assert nb_d == nb_o and np.array_equal(a_d, a_o)
+    full = native.reattach(g, m, a_d, nb_d, seed=3)
+    assert np.array_equal(full, O.reattach(og, om, a_o, nb_o, seed=3))
+    bm = native.DeviceBlockmodel.build(dg, nb_d, full)
+    assert bm.validate()
