@@ -282,7 +282,9 @@ static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, 
             SBP_HIP(hipMemcpy(rc.data(), bm->row_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
             SBP_HIP(hipMemcpy(cc.data(), bm->col_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
         }
-        const int threads = ctx->carry_threads > 0 ? ctx->carry_threads : sbpcarry::carry_threads();
+        int threads = ctx->carry_threads > 0 ? ctx->carry_threads : sbpcarry::carry_threads();
+        // the parametric carry-out applies merges on a thread of its own beside the pool: leave it a core
+        if (!bm->np && B >= sbpcarry::PIPE_MIN_BLOCKS && threads > 2 && sbpcarry::pipe_enabled()) --threads;
         if (!ctx->carry_pool || ctx->carry_pool->threads() != threads)
             ctx->carry_pool = std::make_shared<sbpcarry::Pool>(threads);
         pool = ctx->carry_pool.get();

@@ -734,8 +734,10 @@ inline void rename_in(std::vector<FlatMap> &maps, const FlatMap::Slot *ent, size
 // column s take all of them on one thread; the renames r -> s in the columns of the other c and the rows of
 // the other x touch distinct maps, so with a pool a long list of them is split into pieces.  Only the
 // thread that owns `pool` may pass it.
-inline void merge_apply(Model &m, int32_t r, int32_t s, Pool *pool = nullptr,
-                        const SplitSizes &split = SplitSizes()) {
+// The block-matrix part of merging r into s: row and column s take r's cells, the rows and columns of r's
+// neighbours rename key r to s, and r's maps are retired.  It writes the maps of r, s and of the keys of row r
+// and column r, and nothing else.
+inline void merge_cells(Model &m, int32_t r, int32_t s, Pool *pool = nullptr, const SplitSizes &split = SplitSizes()) {
     FlatMap &Rr = m.row[r], &Cr = m.col[r], &Rs = m.row[s], &Cs = m.col[s];
     thread_local std::vector<FlatMap::Slot> row_buf, col_buf, row_ren, col_ren;
     const size_t nr = gather(Rr, row_buf), nc = gather(Cr, col_buf);
@@ -805,6 +807,10 @@ inline void merge_apply(Model &m, int32_t r, int32_t s, Pool *pool = nullptr,
     }
     m.retired.p.push_back(Rr.detach());
     m.retired.p.push_back(Cr.detach());
+}
+
+// The block degrees, sizes and degree histograms of merging r into s.
+inline void merge_totals(Model &m, int32_t r, int32_t s) {
     m.d_out[s] += m.d_out[r];
     m.d_in[s] += m.d_in[r];
     m.d_out[r] = 0;
@@ -818,6 +824,83 @@ inline void merge_apply(Model &m, int32_t r, int32_t s, Pool *pool = nullptr,
         m.hin[r].each([&](int32_t deg, int64_t c) { m.hin[s].add(deg, c); });
         m.hout[r].each([&](int32_t deg, int64_t c) { m.hout[s].add(deg, c); });
     }
+}
+
+inline void merge_apply(Model &m, int32_t r, int32_t s, Pool *pool = nullptr,
+                        const SplitSizes &split = SplitSizes()) {
+    merge_cells(m, r, s, pool, split);
+    merge_totals(m, r, s);
+}
+
+// Applies the block-matrix part of merges (merge_cells) on a thread of its own, in the order submitted, while
+// the caller goes on deciding.  Ticket t is the t-th merge submitted; wait(t) returns once it is applied,
+// after which its writes are visible to the caller.
+class Applier {
+  public:
+    explicit Applier(Model &m) : m_(m), ring_(RING) { th_ = std::thread([this] { run(); }); }
+    ~Applier() {
+        stop_.store(true, std::memory_order_release);
+        th_.join();
+    }
+    int64_t next_ticket() const { return submitted_ + 1; }
+    int64_t submit(int32_t r, int32_t s) {
+        const int64_t t = submitted_ + 1;
+        while (t - applied_.load(std::memory_order_acquire) > RING) relax();
+        ring_[(size_t)((t - 1) & (RING - 1))] = {r, s};
+        submitted_ = t;
+        published_.store(t, std::memory_order_release);
+        return t;
+    }
+    void wait(int64_t ticket) const {
+        while (applied_.load(std::memory_order_acquire) < ticket) relax();
+    }
+    void drain() const { wait(submitted_); }
+
+  private:
+    static constexpr int64_t RING = int64_t(1) << 14;
+    static void relax() {
+#if defined(__x86_64__)
+        __builtin_ia32_pause();
+#else
+        std::this_thread::yield();
+#endif
+    }
+    void run() {
+        int64_t done = 0;
+        uint32_t idle = 0;
+        while (true) {
+            const int64_t p = published_.load(std::memory_order_acquire);
+            if (p == done) {
+                if (stop_.load(std::memory_order_acquire) && published_.load(std::memory_order_acquire) == done) return;
+                // idle while the caller evaluates in parallel on the pool: give the core to its workers
+                if (++idle < 4096) relax();
+                else std::this_thread::yield();
+                continue;
+            }
+            idle = 0;
+            for (; done < p; ++done) {
+                const std::pair<int32_t, int32_t> job = ring_[(size_t)(done & (RING - 1))];
+                merge_cells(m_, job.first, job.second);
+                applied_.store(done + 1, std::memory_order_release);
+            }
+        }
+    }
+    Model &m_;
+    std::vector<std::pair<int32_t, int32_t>> ring_;
+    int64_t submitted_ = 0;  // caller only
+    std::atomic<int64_t> published_{0}, applied_{0};
+    std::atomic<bool> stop_{false};
+    std::thread th_;
+};
+
+// Merges are applied on the Applier thread (SBP_AMD_CARRY_PIPE=0 turns this off) for the parametric model
+// with a thread pool and at least this many blocks.  Fewer blocks share more neighbours, and the loop waits
+// for the previous merges too often: on the 200,000-vertex workload the carry-out took 25% less time at
+// B = 200,000, 14% less at 50,000 and 11% more at 25,000.
+constexpr int32_t PIPE_MIN_BLOCKS = 32768;
+inline bool pipe_enabled() {
+    const char *e = getenv("SBP_AMD_CARRY_PIPE");
+    return !(e && e[0] == '0');
 }
 
 struct CarryStats {
@@ -923,6 +1006,18 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
             while (parent[x] != x) x = parent[x];
             return x;
         };
+        // merges applied on a thread of their own: busy[x] is the ticket of the last queued merge that writes
+        // the row or column of x (its r, s and the keys of row r and column r); the maps of x are read after
+        // waiting for it
+        std::unique_ptr<Applier> applier;
+        std::vector<int64_t> busy;
+        if (m && !m->np && !m->dc && pool && pool->threads() > 1 && B >= PIPE_MIN_BLOCKS && pipe_enabled()) {
+            applier.reset(new Applier(*m));
+            busy.assign(B, 0);
+        }
+        auto ready = [&](int32_t x) {
+            if (applier) applier->wait(busy[x]);
+        };
         std::vector<Item> batch;
         std::atomic<int64_t> spec_count{0};
         size_t bi = 0;
@@ -943,6 +1038,7 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
                     heap.pop();
                 }
                 if (pool && batch.size() >= 16) {
+                    if (applier) applier->drain();
                     pool->run((int64_t)batch.size(), [&](int64_t i) {
                         const Item &it = batch[i];
                         const int32_t t = root_ro(it.s);
@@ -963,7 +1059,7 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
                 heap.pop();
             } else {
                 it = batch[bi++];
-                if (m) {  // the rows and columns of the next items are read when they are merged
+                if (m && !applier) {  // the rows and columns of the next items are read when they are merged
                     if (bi + 4 < batch.size()) {
                         __builtin_prefetch(&m->row[batch[bi + 4].r]);
                         __builtin_prefetch(&m->col[batch[bi + 4].r]);
@@ -984,6 +1080,8 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
                 st.reused++;
             } else {
                 if (c_to[from] == to && c_time[from] == spec_time) batch_lost++;  // speculated, then invalidated
+                ready(from);
+                ready(to);
                 const double v = evaluate(from, to, pool);
                 c_to[from] = to;
                 c_time[from] = now;
@@ -1005,9 +1103,29 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
             ++now;
             stamp[from] = stamp[to] = now;
             parent[from] = to;
-            merge_apply(*m, from, to, pool, split);
-            mark_keys(m->row[to], now);
-            mark_keys(m->col[to], now);
+            if (applier) {
+                // the merged row and column `to` hold the keys of rows and columns from and to (key from becoming
+                // to, which needs no mark: from is merged and to is stamped)
+                ready(from);
+                ready(to);
+                const int64_t ticket = applier->next_ticket();
+                for (const FlatMap *f : {&m->row[from], &m->col[from]})
+                    for (uint32_t i = 0; i < f->cap; ++i) {
+                        const int32_t k = f->s[i].key;
+                        if (k < 0) continue;
+                        mk[k] = now;
+                        busy[k] = ticket;
+                    }
+                mark_keys(m->row[to], now);
+                mark_keys(m->col[to], now);
+                busy[from] = busy[to] = ticket;
+                merge_totals(*m, from, to);
+                applier->submit(from, to);
+            } else {
+                merge_apply(*m, from, to, pool, split);
+                mark_keys(m->row[to], now);
+                mark_keys(m->col[to], now);
+            }
             merged++;
         }
     }

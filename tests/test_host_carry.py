@@ -161,3 +161,22 @@ def test_host_carry_degree_corrected_matches_oracle(N, B):
         out = bm.copy().merge_apply(best_s, best_dS, B // 2, False)
         assert nb == out.B
         assert np.array_equal(bmap[assign], out.export()["assignment"])
+
+
+def test_host_carry_pipelined_merges_are_exact(monkeypatch):
+    """Merges applied on their own thread while the loop goes on deciding (parametric, several threads,
+    >= 32768 blocks) give the block map of the loop that applies each merge itself."""
+    N = 40000
+    g = dcsbm(N, 40, gamma=2.1, kmin=1, kmax=600, mean_degree=16, p_in=0.6, sigma=0.5, seed=9)
+    og = O.OGraph.from_csr(g)
+    bm = O.OBlockmodel.build(og, N, np.arange(N, dtype=np.int32))
+    best_s, best_dS = bm.merge_propose(seed=4, epoch=1)
+    maps = {}
+    for pipe in ("0", "1"):
+        monkeypatch.setenv("SBP_AMD_CARRY_PIPE", pipe)
+        nb, bmap, st = host_carry(bm, best_s, best_dS, N // 2, False, threads=4)
+        maps[pipe] = (nb, bmap, tuple(st[:6]))
+    assert maps["0"][0] == maps["1"][0] and np.array_equal(maps["0"][1], maps["1"][1])
+    assert maps["0"][2] == maps["1"][2]  # the same pops, evaluations and requeues
+    out = bm.copy().merge_apply(best_s, best_dS, N // 2, False)
+    assert np.array_equal(maps["1"][1], out.export()["assignment"])
