@@ -194,7 +194,11 @@ def serve_trace(graphs, arrivals, prms, slots):
         if dt > 0:
             time.sleep(dt)
         tickets.append(srv.submit(g, p))
-    res = [srv.wait(t) for t in tickets]
+    res = []
+    for j, t in enumerate(tickets):
+        res.append(srv.wait(t))
+        print(f"[bench] served ({slots} slots): request {j} (N={graphs[j].N}) in {res[-1][3]:.0f} ms",
+              file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     info = srv.info()
     srv.close()
@@ -285,7 +289,10 @@ def main():
 
     if args.only:
         many_b = [f.result() for f in fut_b]
-        many_s = [f.result() for f in fut_s]
+        many_s = []
+        for j, f in enumerate(fut_s):
+            many_s.append(f.result())
+            print(f"[bench] generated request {j}: N={many_s[-1].N} E={many_s[-1].E}", file=sys.stderr, flush=True)
         pool.shutdown()
         if args.only == "batched":
             ctx = native.Context(0, seed=0)
