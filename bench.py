@@ -257,11 +257,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-graphs", type=int, default=64, help="graphs in the batched leg (0: skip it)")
-    ap.add_argument("--batched-slots", type=int, default=16)
+    ap.add_argument("--batched-slots", type=int, default=None, help="default: the autotuned count, else 16")
     ap.add_argument("--served-requests", type=int, default=32, help="requests in the served leg (0: skip it)")
     ap.add_argument("--served-rate", type=float, default=32.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--served-max-n", type=int, default=200_000, help="upper bound of the requests' N")
-    ap.add_argument("--served-slots", type=int, default=16)
+    ap.add_argument("--served-slots", type=int, default=None, help="default: the autotuned count, else 16")
     ap.add_argument("--many-algorithm", default="async_gibbs", help="algorithm of the batched and served legs")
     ap.add_argument("--only", choices=["batched", "served"],
                     help="run only that leg and print its JSON object (for profiling)")
@@ -277,6 +277,11 @@ def main():
         sys.exit(2)
 
     from sbp_amd import workloads
+    from sbp_amd.autotune import tuned_slots
+    if args.batched_slots is None:
+        args.batched_slots = tuned_slots(16)
+    if args.served_slots is None:
+        args.served_slots = tuned_slots(16)
     # the mixed-size graphs are generated in worker processes (forked before the device is touched) while
     # the single-graph workload is generated and warmed up
     pool = cf.ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 1), mp_context=mp.get_context("fork"))

@@ -203,9 +203,13 @@ class Context:
     def set_carry_threads(self, threads):
         self.check(lib().sbp_set_carry_threads(self.h, threads), "sbp_set_carry_threads")
 
-    def partition_batch(self, graphs, prms, slots=0):
-        """sbp_partition_batch over DeviceGraphs with one Params each.  Returns (assignments, num_blocks,
-        stats, finish_ms) per graph, in input order."""
+    def partition_batch(self, graphs, prms, slots=None):
+        """sbp_partition_batch over DeviceGraphs with one Params each (slots None: the autotuned count for
+        this host, else the library's default).  Returns (assignments, num_blocks, stats, finish_ms) per
+        graph, in input order."""
+        if slots is None:
+            from .autotune import tuned_slots
+            slots = tuned_slots(0)
         n = len(graphs)
         assert len(prms) == n
         hs = (C.c_void_p * n)(*[g.h for g in graphs])

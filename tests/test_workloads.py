@@ -69,3 +69,15 @@ def test_footprint_of_a_graph():
     k = np.diff(g.out_off) + np.diff(g.in_off)
     assert native.footprint_bytes(g, "async_gibbs") == int(native.lib().sbp_footprint_bytes(
         g.N, g.E, int(k.max()), int((k > 256).sum()), 0))
+
+
+def test_autotuned_slots_lookup(tmp_path, monkeypatch):
+    import json
+    from sbp_amd import autotune
+    path = tmp_path / "autotune.json"
+    monkeypatch.setattr(autotune, "TUNED", str(path))
+    assert autotune.tuned_slots(16) == 16  # nothing measured
+    path.write_text(json.dumps({autotune.host_key(): {"slots": 12}, "9999cpu": {"slots": 3}}))
+    assert autotune.tuned_slots(16) == 12
+    path.write_text("not json")
+    assert autotune.tuned_slots(7) == 7
