@@ -19,6 +19,11 @@ Two more legs measure many graphs on the device (sbp_amd/workloads.py names the 
 - "served": the first R requests of the served workload (N log-uniform in [1000, --served-max-n]) arriving
   as a Poisson process, replayed against a request server with one worker (the plain sequential path) and
   with several; p50/p99 latency from arrival to completion and graphs/s of both.
+
+`--only sampled` runs the large single-graph workload instead (5,000,000 vertices, about 80M edges, 5,000
+lognormal(1.0) blocks, p_in 0.4): per sampler (max_degree, degree_weighted) a 50% sample is partitioned,
+extended to the full graph and finetuned there; wall time per phase, proposals/s, MDL, NMI and the roofline
+of the full-graph finetune launches.
 """
 import argparse
 import concurrent.futures as cf
@@ -61,7 +66,15 @@ def nmi(a, b):
     mi = (pab[nz] * np.log(pab[nz] / np.outer(pa, pb)[nz])).sum()
     ha = -(pa[pa > 0] * np.log(pa[pa > 0])).sum()
     hb = -(pb[pb > 0] * np.log(pb[pb > 0])).sum()
-    return float(mi / np.sqrt(ha * hb)) if ha > 0 and hb > 0 else 1.0
+    if ha > 0 and hb > 0:
+        return float(mi / np.sqrt(ha * hb))
+    return None  # 0/0 when either side is one block: the reference's value is not finite
+
+
+def mean_nmi(values):
+    """Mean of the finite NMI values (None where a partition is one block)."""
+    v = [x for x in values if x is not None]
+    return float(np.mean(v)) if v else None
 
 
 def cpu_baseline(g, algorithm, seconds, threads):
@@ -103,11 +116,11 @@ KERNEL_NAMES = {  # rocprofv3 names of the finetune proposal launches (greedy / 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ft_propose_c2_traffic.txt")
 
 
-def committed_traffic(kernel):
+def committed_traffic(kernel, path=TRAFFIC_FILE):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE,
     scaled by a copy-kernel calibration), or None when there is none for this kernel."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             lines = f.read().splitlines()
     except OSError:
         return None
@@ -120,9 +133,10 @@ def committed_traffic(kernel):
         return None
 
 
-def roofline(tot, greedy=True):
+def roofline(tot, greedy=True, traffic_file=TRAFFIC_FILE):
     """Roofline of the dominant finetune proposal launch: algorithmic bytes per launch over its average
-    device time (HIP events around that launch alone, on the library's stream)."""
+    device time (HIP events around that launch alone, on the library's stream).  traffic_file names the
+    committed rocprofv3 summary of that launch on the same workload (None: no measured traffic)."""
     legs = {}
     for key, prefix in (("ft", "ft"), ("ft_high", "ft_high")):
         n = tot[f"{prefix}_launches"]
@@ -136,12 +150,12 @@ def roofline(tot, greedy=True):
     for leg in legs.values():
         leg["achieved_GBps"] = leg["algorithmic_bytes_per_launch"] / (leg["avg_launch_ms"] * 1e-3) / 1e9
     dom = max(legs.values(), key=lambda l: l["total_ms"])
-    traffic = committed_traffic(dom["kernel"])
+    traffic = committed_traffic(dom["kernel"], traffic_file) if traffic_file else None
     return {"bound": "hbm", "achieved": dom["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": dom["achieved_GBps"] / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": dom["kernel"], "avg_launch_ms": dom["avg_launch_ms"],
             "algorithmic_bytes_per_launch": dom["algorithmic_bytes_per_launch"],
-            "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic is not None else None,
+            "traffic_source": os.path.relpath(traffic_file, ROOT) if traffic is not None else None,
             "launches": {k: {kk: vv for kk, vv in v.items() if kk != "total_ms"} for k, v in legs.items()}}
 
 
@@ -177,8 +191,8 @@ def batched_leg(ctx, graphs, algorithm, slots):
         "speedup_vs_plain": t_plain / t_batch,
         "proposals_per_s": sum(x.proposals for x in st) / t_batch,
         "identical_to_plain": bool(same),
-        "nmi_mean": float(np.mean([nmi(g.truth, o) for g, o in zip(graphs, outs)])),
-        "plain_nmi_mean": float(np.mean([nmi(g.truth, p[0]) for g, p in zip(graphs, plain)])),
+        "nmi_mean": mean_nmi(nmi(g.truth, o) for g, o in zip(graphs, outs)),
+        "plain_nmi_mean": mean_nmi(nmi(g.truth, p[0]) for g, p in zip(graphs, plain)),
     }
 
 
@@ -231,9 +245,81 @@ def served_leg(graphs, arrivals, algorithm, slots, rate, max_n):
         "p99_ms": out["served"]["p99_ms"],
         "speedup_vs_plain": out["served"]["graphs_per_s"] / out["plain"]["graphs_per_s"],
         "identical_to_plain": bool(same),
-        "nmi_mean": float(np.mean([nmi(g.truth, r[0]) for g, r in zip(graphs, runs["served"])])),
+        "nmi_mean": mean_nmi(nmi(g.truth, r[0]) for g, r in zip(graphs, runs["served"])),
         "plain": out["plain"], "served": out["served"],
     }
+
+
+SAMPLED = dict(N=5_000_000, B=5000, gamma=2.1, kmin=1, kmax=20000, mean_degree=16.0, p_in=0.4, sigma=1.0)
+SAMPLED_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ft_propose_c3_traffic.txt")
+
+
+def sampled_leg(algs, algorithm, n):
+    """The large single-graph workload: a directed DCSBM with 5,000,000 vertices and 5,000 lognormal(1.0)
+    blocks, out-degree power law 2.1 on [1, 20000] with mean 16 (about 80M edges), p_in 0.4, seed 0.  Per
+    sampler: keep 50% of the vertices (sample::sample + from_vertices), partition the sample, extend its
+    partition to the full graph (sample::extend) and finetune it there (finetune_assignment).  The graph is
+    resident on the device before the clock starts."""
+    from sbp_amd import native
+    from sbp_amd.graph import dcsbm
+    os.environ.setdefault("SBP_AMD_TRACE", "1")  # one line per merge phase, so a long partition shows progress
+    w = dict(SAMPLED, N=n, B=max(1, round(SAMPLED["B"] * n / SAMPLED["N"])))
+    t0 = time.perf_counter()
+    g = dcsbm(w["N"], w["B"], gamma=w["gamma"], kmin=w["kmin"], kmax=w["kmax"], mean_degree=w["mean_degree"],
+              p_in=w["p_in"], sigma=w["sigma"], seed=0)
+    print(f"[bench] sampled: graph N={g.N} E={g.E} generated in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    ctx = native.Context(0, seed=0)
+    warm = dcsbm(2000, 12, gamma=2.5, kmin=1, kmax=100, mean_degree=10, p_in=0.7, seed=1)
+    ctx.upload(warm).partition(native.params(algorithm, seed=0))  # loads the code objects
+    dg = ctx.upload(g)
+    ctx.synchronize()
+    prm = native.params(algorithm, seed=0)
+    runs = {}
+    for alg in algs:
+        ctx.set_seed(0)
+        t = [time.perf_counter()]
+        sg, mapping = dg.sample(alg, 0.5, seed=0)
+        ctx.synchronize()
+        t.append(time.perf_counter())
+        print(f"[bench] sampled/{alg}: sample N={sg.N} E={sg.E} in {t[1] - t[0]:.2f}s", file=sys.stderr, flush=True)
+        a_s, nb, st_p = sg.partition(prm)
+        t.append(time.perf_counter())
+        print(f"[bench] sampled/{alg}: sample partitioned, B={nb} in {t[2] - t[1]:.1f}s", file=sys.stderr,
+              flush=True)
+        sbm = native.DeviceBlockmodel.build(sg, nb, a_s)
+        full, _ = sbm.extend(dg, seed=0)
+        ctx.synchronize()
+        t.append(time.perf_counter())
+        ctx.set_seed(0)
+        _, mdl, st_f = full.finetune_assignment(prm, 0)
+        a = full.export()["assignment"]
+        t.append(time.perf_counter())
+        print(f"[bench] sampled/{alg}: extended and finetuned in {t[3] - t[2]:.1f}s + {t[4] - t[3]:.1f}s, "
+              f"mdl {mdl:.2f}", file=sys.stderr, flush=True)
+        del sbm, full, sg
+        wall = t[4] - t[0]
+        ft = st_f.as_dict()
+        sampled_truth = g.truth[mapping >= 0][np.argsort(mapping[mapping >= 0])]
+        runs[alg] = {
+            "sample_vertices": int((mapping >= 0).sum()), "num_blocks": nb, "mdl": mdl,
+            "nmi": nmi(g.truth, a), "nmi_sample": nmi(sampled_truth, a_s),
+            "wall_s": wall, "phase_s": {"sample": t[1] - t[0], "partition_sample": t[2] - t[1],
+                                        "extend": t[3] - t[2], "finetune_full": t[4] - t[3]},
+            "proposals_per_s": (st_p.proposals + st_f.proposals) / wall,
+            "accepted_moves_per_s": (st_p.moves + st_f.moves) / wall,
+            "partition_sample": {k: st_p.as_dict()[k] for k in ("proposals", "moves", "sweeps", "merge_phases",
+                                                                "finetune_ms", "merge_ms", "carry_ms", "build_ms",
+                                                                "mdl_ms")},
+            "finetune_full": {"proposals": ft["proposals"], "moves": ft["moves"], "sweeps": ft["sweeps"],
+                              "finetune_ms": ft["finetune_ms"],
+                              "proposals_per_s": ft["proposals"] / max(t[4] - t[3], 1e-9),
+                              "roofline": roofline(ft, greedy=True, traffic_file=SAMPLED_TRAFFIC_FILE)},
+        }
+    return {"workload": f"directed DCSBM N={g.N} B={w['B']} lognormal(1.0) sizes, power-law out-degree gamma=2.1 "
+                        f"on [1,20000] mean 16, p_in=0.4, seed 0; 50% sample, {algorithm} partition of the sample, "
+                        "extend, finetune_assignment on the full graph",
+            "N": g.N, "E": g.E, "algorithm": algorithm, "runs": runs}
 
 
 def calibrate_copy(mib):
@@ -263,8 +349,13 @@ def main():
     ap.add_argument("--served-max-n", type=int, default=200_000, help="upper bound of the requests' N")
     ap.add_argument("--served-slots", type=int, default=None, help="default: the autotuned count, else 16")
     ap.add_argument("--many-algorithm", default="async_gibbs", help="algorithm of the batched and served legs")
-    ap.add_argument("--only", choices=["batched", "served"],
-                    help="run only that leg and print its JSON object (for profiling)")
+    ap.add_argument("--only", choices=["batched", "served", "sampled"],
+                    help="run only that leg and print its JSON object (sampled: the 5,000,000-vertex sampled "
+                         "workload, which the default run leaves out for its length)")
+    ap.add_argument("--sampled-algs", default="max_degree,degree_weighted", help="samplers of the sampled leg")
+    ap.add_argument("--sampled-algorithm", default="async_gibbs", help="finetune algorithm of the sampled leg")
+    ap.add_argument("--sampled-n", type=int, default=SAMPLED["N"],
+                    help="vertices of the sampled leg's graph (blocks scale with it)")
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
                     help="only copy MIB MiB device-to-device 3 times (counter calibration) and exit")
     args = ap.parse_args()
@@ -275,6 +366,11 @@ def main():
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the path is single-device, "
               "see the sharding note in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.only == "sampled":
+        print(json.dumps({"sampled": sampled_leg(args.sampled_algs.split(","), args.sampled_algorithm,
+                                                 args.sampled_n)}))
+        return
 
     from sbp_amd import workloads
     from sbp_amd.autotune import tuned_slots

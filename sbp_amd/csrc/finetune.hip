@@ -41,6 +41,8 @@ using namespace sbpdev;
 
 constexpr int FT_BLOCK = 256;
 constexpr int FT_WAVES = FT_BLOCK / WAVE;
+// Wavefronts of the Metropolis-Hastings workgroup: vertices evaluated per window of the sequential walk.
+constexpr int MH_WAVES = 8;
 constexpr int LDS_CAP_LOG2 = 9;
 // The LDS table serves vertices with |out| + |in| <= SMALL_K, whose distinct neighbour blocks are at most
 // SMALL_K: its used-slot list needs no more entries, which keeps five 4-wave blocks within a CU's 160 KB.
@@ -940,127 +942,170 @@ __device__ __forceinline__ bool lacks_room(const FtArgs &a, const MhState &h, in
     return short_r || short_c;
 }
 
+// Sequential Metropolis-Hastings over a.items[start, count) (src/finetune.cpp:773-869), evaluated a window of
+// MH_WAVES vertices at a time: each wavefront of the one workgroup proposes and evaluates the next vertex of
+// the window against the current blockmodel (propose_block and evaluate only read it).  The first accepted
+// move of the window is committed by its wavefront, and the next window starts at the vertex after it.  Every
+// evaluation up to and including that move read exactly the state a one-vertex-at-a-time walk reads (no move
+// was applied between them), and the draws are keyed by vertex, so decisions, commits and the visit-order sum
+// of accepted delta-MDL are those of the walk.  Evaluations after the committed move are discarded.
 template <bool GREEDY, bool NP>
-__global__ void __launch_bounds__(WAVE) k_mh(FtArgs a, MhState h, int32_t start) {
-    __shared__ int lds[LDS_TABLE_INTS];
+__global__ void __launch_bounds__(MH_WAVES * WAVE) k_mh(FtArgs a, MhState h, int32_t start) {
+    __shared__ int lds[MH_WAVES * LDS_TABLE_INTS];
+    __shared__ int s_accept[MH_WAVES];
+    __shared__ long long s_next;      // first item of the next window
+    __shared__ int s_status, s_stop;  // as *h.status / *h.stop_at
+    __shared__ double s_sum;
+    __shared__ unsigned long long s_moves;
     const int lane = lane_id();
-    unsigned long long bytes = 0, moves = 0;
-    double sum = *h.sum;
-    int32_t status = 0, stop = (int32_t)a.count;
-    const Table small_t = make_table(a, false, lds, 0);
-    const Table big_t = make_table(a, true, lds, 0);
-    table_clear_all(small_t);  // the global table arrives empty (prepare_tables) and is left empty
-    for (int64_t item = start; item < a.count; ++item) {
-        const int v = a.items[item];
-        const int r0 = a.assign[v];
-        if (a.sizes[r0] == 1) continue;  // propose_move skips singleton blocks (src/finetune.cpp:845-847)
-        const long long kall = (a.out_off[v + 1] - a.out_off[v]) + (a.in_off[v + 1] - a.in_off[v]);
-        const Table &t = kall > SMALL_K ? big_t : small_t;
+    const int wid = (int)(threadIdx.x / WAVE);
+    unsigned long long bytes = 0;
+    const Table small_t = make_table(a, false, lds + wid * LDS_TABLE_INTS, 0);
+    const Table big_t = make_table(a, true, lds, wid);
+    table_clear_all(small_t);  // the global tables arrive empty (prepare_tables) and are left empty
+    if (threadIdx.x == 0) {
+        s_next = start;
+        s_status = 0;
+        s_stop = (int32_t)a.count;
+        s_sum = *h.sum;
+        s_moves = 0;
+    }
+    __syncthreads();
+    while (true) {
+        const long long base = s_next;
+        if (base >= a.count) break;
+        const long long item = base + wid;
+        bool accept = false;
+        double dS = 0.0;
+        unsigned long long pb = 0;
+        long long kall = 0;
         MoveCtx m;
         U4 w;
-        const int s = propose_block(a, v, m, w);
-        if (s == m.r) {
-            bytes += proposal_bytes(a, m, 0, GREEDY);
-            continue;
-        }
-        bool accept;
-        const double dS = evaluate<GREEDY, NP>(a, m, t, w, accept);
-        bytes += proposal_bytes(a, m, *t.n_used, GREEDY);
-        if (!accept) {
-            table_clear_used(t);
-            continue;
-        }
-        const int r = m.r;
-        const int nb = *t.n_used;
-        // occupying an empty block changes the non-empty block count the nonparametric constants are for
-        const bool occupies = NP && a.sizes[s] == 0;
-        // ---- room check for every Delta cell that would create a new entry
-        bool short_room = false;
-        int new_row_s = 0, new_col_s = 0;
-        for (int i = lane; i < nb; i += WAVE) {
-            const int slot = t.used[i];
-            const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
-            if (oc > 0) {
-                short_room |= lacks_room(a, h, s, c, delta_row_s(m, c, oc));
-                if (delta_row_s(m, c, oc) > 0 && get_cell(a, s, c) == 0) new_row_s++;
-            }
-            if (ic > 0 && c != r && c != s) {
-                short_room |= lacks_room(a, h, c, s, ic);
-                if (get_cell(a, c, s) == 0) new_col_s++;
+        Table tsel = small_t;
+        if (item < a.count) {
+            const int v = a.items[item];
+            if (a.sizes[a.assign[v]] != 1) {  // propose_move skips singleton blocks (src/finetune.cpp:845-847)
+                kall = (a.out_off[v + 1] - a.out_off[v]) + (a.in_off[v + 1] - a.in_off[v]);
+                tsel = kall > SMALL_K ? big_t : small_t;
+                const int s = propose_block(a, v, m, w);
+                if (s == m.r) {
+                    pb = proposal_bytes(a, m, 0, GREEDY);
+                } else {
+                    dS = evaluate<GREEDY, NP>(a, m, tsel, w, accept);
+                    pb = proposal_bytes(a, m, *tsel.n_used, GREEDY);
+                    if (!accept) table_clear_used(tsel);
+                }
             }
         }
-        if (lane == 0) {
-            if (m.out_s == 0 && m.in_r > 0) short_room |= lacks_room(a, h, r, s, m.in_r);
-            if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) short_room |= lacks_room(a, h, s, s, m.selfc + m.in_s);
-        }
-        // row s and column s may receive several new cells: check their totals
-        new_row_s = wave_sum(new_row_s) + 2;
-        new_col_s = wave_sum(new_col_s) + 2;
-        const bool short_rs = h.rlen[s] + new_row_s > h.rcap[s], short_cs = h.clen[s] + new_col_s > h.ccap[s];
-        if (lane == 0 && short_rs) want_room(h.rboost, h.rlen, s, new_row_s);
-        if (lane == 0 && short_cs) want_room(h.cboost, h.clen, s, new_col_s);
-        short_room |= short_rs || short_cs;
-        if (ballot(short_room)) {
-            table_clear_used(t);
-            status = 1;
-            stop = (int32_t)item;
-            break;
-        }
-        // ---- apply the Delta in place (update_edge_counts(delta), dict_transpose_matrix.cpp:430-446)
-        for (int i = 0; i < nb; ++i) {  // one cell at a time: each update is wave-cooperative
-            const int slot = t.used[i];
-            const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
-            if (oc > 0) {
-                long long dr = delta_row_r(m, c, oc), ds = delta_row_s(m, c, oc);
-                if (dr) apply_cell(a, h, r, c, (int)dr);
-                if (ds) apply_cell(a, h, s, c, (int)ds);
+        if (lane == 0) s_accept[wid] = accept ? 1 : 0;
+        __syncthreads();
+        int first = MH_WAVES;
+        for (int i = 0; i < MH_WAVES; ++i)
+            if (s_accept[i]) {
+                first = i;
+                break;
             }
-            if (ic > 0 && c != r && c != s) {
-                apply_cell(a, h, c, r, -ic);
-                apply_cell(a, h, c, s, ic);
+        if (wid <= first) bytes += pb;  // later evaluations are repeated in the next window
+        if (wid > first && accept) table_clear_used(tsel);
+        if (wid == first) {
+            const Table &t = tsel;
+            const int s = m.s;
+            const int r = m.r;
+            const int nb = *t.n_used;
+            // occupying an empty block changes the non-empty block count the nonparametric constants are for
+            const bool occupies = NP && a.sizes[s] == 0;
+            // ---- room check for every Delta cell that would create a new entry
+            bool short_room = false;
+            int new_row_s = 0, new_col_s = 0;
+            for (int i = lane; i < nb; i += WAVE) {
+                const int slot = t.used[i];
+                const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
+                if (oc > 0) {
+                    short_room |= lacks_room(a, h, s, c, delta_row_s(m, c, oc));
+                    if (delta_row_s(m, c, oc) > 0 && get_cell(a, s, c) == 0) new_row_s++;
+                }
+                if (ic > 0 && c != r && c != s) {
+                    short_room |= lacks_room(a, h, c, s, ic);
+                    if (get_cell(a, c, s) == 0) new_col_s++;
+                }
+            }
+            if (lane == 0) {
+                if (m.out_s == 0 && m.in_r > 0) short_room |= lacks_room(a, h, r, s, m.in_r);
+                if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0))
+                    short_room |= lacks_room(a, h, s, s, m.selfc + m.in_s);
+            }
+            // row s and column s may receive several new cells: check their totals
+            new_row_s = wave_sum(new_row_s) + 2;
+            new_col_s = wave_sum(new_col_s) + 2;
+            const bool short_rs = h.rlen[s] + new_row_s > h.rcap[s], short_cs = h.clen[s] + new_col_s > h.ccap[s];
+            if (lane == 0 && short_rs) want_room(h.rboost, h.rlen, s, new_row_s);
+            if (lane == 0 && short_cs) want_room(h.cboost, h.clen, s, new_col_s);
+            short_room |= short_rs || short_cs;
+            if (ballot(short_room)) {
+                table_clear_used(t);
+                if (lane == 0) {
+                    s_status = 1;
+                    s_stop = (int32_t)item;
+                }
+            } else {
+                // ---- apply the Delta in place (update_edge_counts(delta), dict_transpose_matrix.cpp:430-446)
+                for (int i = 0; i < nb; ++i) {  // one cell at a time: each update is wave-cooperative
+                    const int slot = t.used[i];
+                    const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
+                    if (oc > 0) {
+                        long long dr = delta_row_r(m, c, oc), ds = delta_row_s(m, c, oc);
+                        if (dr) apply_cell(a, h, r, c, (int)dr);
+                        if (ds) apply_cell(a, h, s, c, (int)ds);
+                    }
+                    if (ic > 0 && c != r && c != s) {
+                        apply_cell(a, h, c, r, -ic);
+                        apply_cell(a, h, c, s, ic);
+                    }
+                }
+                if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0)) apply_cell(a, h, r, r, -m.selfc - m.in_r);
+                if (m.out_s == 0 && m.in_r > 0) apply_cell(a, h, r, s, m.in_r);
+                if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) apply_cell(a, h, s, s, m.selfc + m.in_s);
+                if (m.out_r == 0 && m.in_s > 0) apply_cell(a, h, s, r, -m.in_s);
+                table_clear_used(t);
+                if (lane == 0) {
+                    a.d_out[r] -= (int)m.k_out;
+                    a.d_out[s] += (int)m.k_out;
+                    a.d_in[r] -= (int)m.shift_in;
+                    a.d_in[s] += (int)m.shift_in;
+                    a.sizes[r] -= 1;
+                    a.sizes[s] += 1;
+                    a.assign[m.v] = s;
+                    if (NP && a.hist) {  // keyed by the proposal's edge counts (src/blockmodel/blockmodel.cpp:431-434)
+                        hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(r, 0, m.k_out), -1);
+                        hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(r, 1, m.k_in), -1);
+                        hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(s, 0, m.k_out), 1);
+                        hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(s, 1, m.k_in), 1);
+                    }
+                }
+                wave_sync();
+                if (lane == 0) {
+                    a.d_tot[r] = a.d_out[r] + a.d_in[r] - (int)get_cell(a, r, r);
+                    a.d_tot[s] = a.d_out[s] + a.d_in[s] - (int)get_cell(a, s, s);
+                    s_sum += dS;
+                    s_moves += 1;
+                    if (occupies) {  // the host refreshes the constants and resumes with the next vertex
+                        s_status = 2;
+                        s_stop = (int32_t)(item + 1);
+                    }
+                }
+                bytes += 12ull * (unsigned long long)kall;
             }
         }
-        if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0)) apply_cell(a, h, r, r, -m.selfc - m.in_r);
-        if (m.out_s == 0 && m.in_r > 0) apply_cell(a, h, r, s, m.in_r);
-        if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) apply_cell(a, h, s, s, m.selfc + m.in_s);
-        if (m.out_r == 0 && m.in_s > 0) apply_cell(a, h, s, r, -m.in_s);
-        table_clear_used(t);
-        if (lane == 0) {
-            a.d_out[r] -= (int)m.k_out;
-            a.d_out[s] += (int)m.k_out;
-            a.d_in[r] -= (int)m.shift_in;
-            a.d_in[s] += (int)m.shift_in;
-            a.sizes[r] -= 1;
-            a.sizes[s] += 1;
-            a.assign[v] = s;
-            if (NP && a.hist) {  // keyed by the proposal's edge counts (src/blockmodel/blockmodel.cpp:431-434)
-                hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(r, 0, m.k_out), -1);
-                hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(r, 1, m.k_in), -1);
-                hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(s, 0, m.k_out), 1);
-                hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(s, 1, m.k_in), 1);
-            }
-        }
-        wave_sync();
-        if (lane == 0) {
-            a.d_tot[r] = a.d_out[r] + a.d_in[r] - (int)get_cell(a, r, r);
-            a.d_tot[s] = a.d_out[s] + a.d_in[s] - (int)get_cell(a, s, s);
-        }
-        wave_sync();
-        sum += dS;
-        moves++;
-        bytes += 12ull * (unsigned long long)kall;
-        if (occupies) {  // the host refreshes the constants and resumes with the next vertex
-            status = 2;
-            stop = (int32_t)(item + 1);
-            break;
-        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_next = s_status ? (long long)a.count : base + (first < MH_WAVES ? first + 1 : MH_WAVES);
+        __syncthreads();
     }
-    if (lane == 0) {
-        *h.status = status;
-        *h.stop_at = stop;
-        atomicAdd(h.moves, moves);
-        *h.sum = sum;
-        if (bytes) atomicAdd(a.bytes, bytes);
+    if (lane == 0 && bytes) atomicAdd(a.bytes, bytes);
+    if (threadIdx.x == 0) {
+        *h.status = s_status;
+        *h.stop_at = s_stop;
+        atomicAdd(h.moves, s_moves);
+        *h.sum = s_sum;
     }
 }
 
@@ -1155,17 +1200,17 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         a.bytes = bytes;
         a.err = reinterpret_cast<int *>(bytes + 1);
         const int lg = std::max(10, log2_ceil(2 * std::min<int64_t>(g->max_degree, bm->B) + 2));
-        a.gtable = g->max_degree > SMALL_K ? prepare_tables(ctx, lg, 1) : nullptr;
+        a.gtable = g->max_degree > SMALL_K ? prepare_tables(ctx, lg, MH_WAVES) : nullptr;
         a.gcap_log2 = lg;
         a.m.row_cum = nullptr;
         a.m.col_cum = nullptr;
         SBP_HIP(hipEventRecord(ctx->ev[4], st));
         if (bm->np) {
-            if (p.greedy) k_mh<true, true><<<1, WAVE, 0, st>>>(a, h, start);
-            else k_mh<false, true><<<1, WAVE, 0, st>>>(a, h, start);
+            if (p.greedy) k_mh<true, true><<<1, MH_WAVES * WAVE, 0, st>>>(a, h, start);
+            else k_mh<false, true><<<1, MH_WAVES * WAVE, 0, st>>>(a, h, start);
         } else {
-            if (p.greedy) k_mh<true, false><<<1, WAVE, 0, st>>>(a, h, start);
-            else k_mh<false, false><<<1, WAVE, 0, st>>>(a, h, start);
+            if (p.greedy) k_mh<true, false><<<1, MH_WAVES * WAVE, 0, st>>>(a, h, start);
+            else k_mh<false, false><<<1, MH_WAVES * WAVE, 0, st>>>(a, h, start);
         }
         SBP_HIP(hipGetLastError());
         SBP_HIP(hipEventRecord(ctx->ev[5], st));

@@ -146,30 +146,40 @@ def dcsbm(N, B, *, gamma=2.5, kmin=1, kmax=100, mean_degree=10.0, p_in=0.75, sig
         other = other + (other >= own)
         tb = np.where(stay | (B == 1), own, other)
         u = rng.random(len(src))
-        s, e = block_start[tb], block_start[tb + 1]
-        # binary search within each block's CDF slice
-        lo, hi = s.copy(), e - 1
-        while True:
-            active = lo < hi
-            if not active.any():
-                break
-            mid = (lo + hi) // 2
-            go_right = theta_cdf[mid] < u
-            lo = np.where(active & go_right, mid + 1, lo)
-            hi = np.where(active & ~go_right, mid, hi)
-        return lo
+        # within the target block: the first vertex whose CDF value is >= u (the CDF is non-decreasing),
+        # clipped to the block's last vertex; one search per block over the edges grouped by block
+        order = np.argsort(tb.astype(np.int16) if B < 32768 else tb, kind="stable")
+        cnt = np.bincount(tb, minlength=B)
+        ends = np.cumsum(cnt)
+        out = np.empty(len(src), np.int64)
+        for b in np.flatnonzero(cnt):
+            sel = order[ends[b] - cnt[b]:ends[b]]
+            s, e = block_start[b], block_start[b + 1]
+            out[sel] = s + np.minimum(np.searchsorted(theta_cdf[s:e], u[sel], side="left"), e - s - 1)
+        return out
 
     src = np.repeat(np.arange(N, dtype=np.int64), deg)
+    src_off = np.concatenate([[0], np.cumsum(deg)])
     slot = np.arange(len(src), dtype=np.int64)
     dst = draw_targets(src)
+    # Each pass redraws every edge that repeats an earlier (lower-index) edge of the same source.  A
+    # duplicate can only involve edges of one source, so after the first pass only the edges of sources
+    # that had a redraw are checked again; the redrawn set, and so the draws, are those of a check over
+    # all edges.
+    cand = slot
     for _ in range(64):
-        key = src * N + dst
+        key = src[cand] * N + dst[cand]
         _, first = np.unique(key, return_index=True)
         dup = np.ones(len(key), bool)
         dup[first] = False
         if not dup.any():
             break
-        dst[dup] = draw_targets(src[dup])
+        redraw = cand[dup]
+        dst[redraw] = draw_targets(src[redraw])
+        touched = np.unique(src[redraw])
+        lens = deg[touched]
+        starts = src_off[touched]
+        cand = np.repeat(starts - np.concatenate([[0], np.cumsum(lens)[:-1]]), lens) + np.arange(lens.sum())
     else:
         key = src * N + dst
         _, first = np.unique(key, return_index=True)
