@@ -112,6 +112,7 @@ def gpu_sweeps_at_truth(ctx, dg, g, algorithm, sweeps):
 KERNEL_NAMES = {  # rocprofv3 names of the finetune proposal launches (greedy / non-greedy, parametric entropy)
     "ft": ("k_ft_propose<false, true, false>", "k_ft_propose<false, false, false>"),
     "ft_high": ("k_ft_propose<true, true, false>", "k_ft_propose<true, false, false>"),
+    "mh": ("k_mh<true, false>", "k_mh<false, false>"),  # the sequential walk (windowed)
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ft_propose_c2_traffic.txt")
 
@@ -138,7 +139,7 @@ def roofline(tot, greedy=True, traffic_file=TRAFFIC_FILE):
     device time (HIP events around that launch alone, on the library's stream).  traffic_file names the
     committed rocprofv3 summary of that launch on the same workload (None: no measured traffic)."""
     legs = {}
-    for key, prefix in (("ft", "ft"), ("ft_high", "ft_high")):
+    for key, prefix in (("ft", "ft"), ("ft_high", "ft_high"), ("mh", "mh")):
         n = tot[f"{prefix}_launches"]
         if n:
             legs[key] = dict(kernel=KERNEL_NAMES[key][0 if greedy else 1], launches=n,
@@ -275,6 +276,7 @@ def sampled_leg(algs, algorithm, n):
     dg = ctx.upload(g)
     ctx.synchronize()
     prm = native.params(algorithm, seed=0)
+    planted_mdl = native.DeviceBlockmodel.build(dg, w["B"], g.truth).mdl()[0]
     runs = {}
     for alg in algs:
         ctx.set_seed(0)
@@ -308,9 +310,10 @@ def sampled_leg(algs, algorithm, n):
                                         "extend": t[3] - t[2], "finetune_full": t[4] - t[3]},
             "proposals_per_s": (st_p.proposals + st_f.proposals) / wall,
             "accepted_moves_per_s": (st_p.moves + st_f.moves) / wall,
-            "partition_sample": {k: st_p.as_dict()[k] for k in ("proposals", "moves", "sweeps", "merge_phases",
-                                                                "finetune_ms", "merge_ms", "carry_ms", "build_ms",
-                                                                "mdl_ms")},
+            "partition_sample": dict({k: st_p.as_dict()[k] for k in ("proposals", "moves", "sweeps", "merge_phases",
+                                                                     "finetune_ms", "merge_ms", "carry_ms", "build_ms",
+                                                                     "mdl_ms")},
+                                     roofline=roofline(st_p.as_dict(), greedy=True, traffic_file=None)),
             "finetune_full": {"proposals": ft["proposals"], "moves": ft["moves"], "sweeps": ft["sweeps"],
                               "finetune_ms": ft["finetune_ms"],
                               "proposals_per_s": ft["proposals"] / max(t[4] - t[3], 1e-9),
@@ -319,7 +322,7 @@ def sampled_leg(algs, algorithm, n):
     return {"workload": f"directed DCSBM N={g.N} B={w['B']} lognormal(1.0) sizes, power-law out-degree gamma=2.1 "
                         f"on [1,20000] mean 16, p_in=0.4, seed 0; 50% sample, {algorithm} partition of the sample, "
                         "extend, finetune_assignment on the full graph",
-            "N": g.N, "E": g.E, "algorithm": algorithm, "runs": runs}
+            "N": g.N, "E": g.E, "algorithm": algorithm, "planted_mdl": planted_mdl, "runs": runs}
 
 
 def calibrate_copy(mib):

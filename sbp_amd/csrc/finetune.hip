@@ -42,7 +42,7 @@ using namespace sbpdev;
 constexpr int FT_BLOCK = 256;
 constexpr int FT_WAVES = FT_BLOCK / WAVE;
 // Wavefronts of the Metropolis-Hastings workgroup: vertices evaluated per window of the sequential walk.
-constexpr int MH_WAVES = 8;
+constexpr int MH_WAVES = 16;
 constexpr int LDS_CAP_LOG2 = 9;
 // The LDS table serves vertices with |out| + |in| <= SMALL_K, whose distinct neighbour blocks are at most
 // SMALL_K: its used-slot list needs no more entries, which keeps five 4-wave blocks within a CU's 160 KB.
