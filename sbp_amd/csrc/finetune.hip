@@ -43,6 +43,7 @@ constexpr int FT_BLOCK = 256;
 constexpr int FT_WAVES = FT_BLOCK / WAVE;
 // Wavefronts of the Metropolis-Hastings workgroup: vertices evaluated per window of the sequential walk.
 constexpr int MH_WAVES = 16;
+static_assert(MH_WAVES > 4, "the Metropolis-Hastings commit gives four wavefronts a row or column of r and s each");
 constexpr int LDS_CAP_LOG2 = 9;
 // The LDS table serves vertices with |out| + |in| <= SMALL_K, whose distinct neighbour blocks are at most
 // SMALL_K: its used-slot list needs no more entries, which keeps five 4-wave blocks within a CU's 160 KB.
@@ -942,6 +943,69 @@ __device__ __forceinline__ bool lacks_room(const FtArgs &a, const MhState &h, in
     return short_r || short_c;
 }
 
+// One wavefront's share of applying an accepted move's Delta (finetune::blockmodel_delta,
+// src/finetune.cpp:201-235) to the slack rows and columns: wavefront 0 writes row r, 1 row s, 2 column r, 3
+// column s, and the others the rows and columns of the remaining neighbour blocks c (entries i = w - 4 mod
+// MH_WAVES - 4).  A list is written by one wavefront only, with its updates in the order of a one-wavefront
+// commit (table entries in used-slot order, then the cells (r, r), (r, s), (s, s), (s, r)), so each list
+// passes through the same states and the room checked for that order suffices.
+__device__ void mh_apply_share(const FtArgs &a, const MhState &h, const MoveCtx &m, const Table &t, int nb, int wid) {
+    const int r = m.r, s = m.s;
+    int2 *rows = a.m.row_cells, *cols = a.m.col_cells;
+    if (wid == 0 || wid == 1) {
+        const int x = wid == 0 ? r : s;
+        for (int i = 0; i < nb; ++i) {
+            const int slot = t.used[i];
+            const int c = t.keys[slot], oc = t.outc[slot];
+            if (oc > 0) {
+                const long long d = x == r ? delta_row_r(m, c, oc) : delta_row_s(m, c, oc);
+                if (d) slack_add(rows, h.rstart[x], &h.rlen[x], c, (int)d);
+            }
+        }
+        if (x == r) {
+            if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0)) slack_add(rows, h.rstart[r], &h.rlen[r], r, -m.selfc - m.in_r);
+            if (m.out_s == 0 && m.in_r > 0) slack_add(rows, h.rstart[r], &h.rlen[r], s, m.in_r);
+        } else {
+            if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) slack_add(rows, h.rstart[s], &h.rlen[s], s, m.selfc + m.in_s);
+            if (m.out_r == 0 && m.in_s > 0) slack_add(rows, h.rstart[s], &h.rlen[s], r, -m.in_s);
+        }
+    } else if (wid == 2 || wid == 3) {
+        const int x = wid == 2 ? r : s;
+        for (int i = 0; i < nb; ++i) {
+            const int slot = t.used[i];
+            const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
+            if (oc > 0 && c == x) {
+                const long long dr = delta_row_r(m, c, oc), ds = delta_row_s(m, c, oc);
+                if (dr) slack_add(cols, h.cstart[x], &h.clen[x], r, (int)dr);
+                if (ds) slack_add(cols, h.cstart[x], &h.clen[x], s, (int)ds);
+            }
+            if (ic > 0 && c != r && c != s) slack_add(cols, h.cstart[x], &h.clen[x], c, x == r ? -ic : ic);
+        }
+        if (x == r) {
+            if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0)) slack_add(cols, h.cstart[r], &h.clen[r], r, -m.selfc - m.in_r);
+            if (m.out_r == 0 && m.in_s > 0) slack_add(cols, h.cstart[r], &h.clen[r], s, -m.in_s);
+        } else {
+            if (m.out_s == 0 && m.in_r > 0) slack_add(cols, h.cstart[s], &h.clen[s], r, m.in_r);
+            if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) slack_add(cols, h.cstart[s], &h.clen[s], s, m.selfc + m.in_s);
+        }
+    } else {
+        for (int i = wid - 4; i < nb; i += MH_WAVES - 4) {
+            const int slot = t.used[i];
+            const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
+            if (c == r || c == s) continue;
+            if (oc > 0) {
+                const long long dr = delta_row_r(m, c, oc), ds = delta_row_s(m, c, oc);
+                if (dr) slack_add(cols, h.cstart[c], &h.clen[c], r, (int)dr);
+                if (ds) slack_add(cols, h.cstart[c], &h.clen[c], s, (int)ds);
+            }
+            if (ic > 0) {
+                slack_add(rows, h.rstart[c], &h.rlen[c], r, -ic);
+                slack_add(rows, h.rstart[c], &h.rlen[c], s, ic);
+            }
+        }
+    }
+}
+
 // Sequential Metropolis-Hastings over a.items[start, count) (src/finetune.cpp:773-869), evaluated a window of
 // MH_WAVES vertices at a time: each wavefront of the one workgroup proposes and evaluates the next vertex of
 // the window against the current blockmodel (propose_block and evaluate only read it).  The first accepted
@@ -957,6 +1021,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) k_mh(FtArgs a, MhState h, int
     __shared__ int s_status, s_stop;  // as *h.status / *h.stop_at
     __shared__ double s_sum;
     __shared__ unsigned long long s_moves;
+    __shared__ int s_mv[10];  // the committed move: r, s, entries, selfc, in_r, in_s, out_r, out_s, big table, room
     const int lane = lane_id();
     const int wid = (int)(threadIdx.x / WAVE);
     unsigned long long bytes = 0;
@@ -1007,93 +1072,107 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) k_mh(FtArgs a, MhState h, int
             }
         if (wid <= first) bytes += pb;  // later evaluations are repeated in the next window
         if (wid > first && accept) table_clear_used(tsel);
-        if (wid == first) {
-            const Table &t = tsel;
-            const int s = m.s;
-            const int r = m.r;
-            const int nb = *t.n_used;
-            // occupying an empty block changes the non-empty block count the nonparametric constants are for
-            const bool occupies = NP && a.sizes[s] == 0;
-            // ---- room check for every Delta cell that would create a new entry
-            bool short_room = false;
-            int new_row_s = 0, new_col_s = 0;
-            for (int i = lane; i < nb; i += WAVE) {
-                const int slot = t.used[i];
-                const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
-                if (oc > 0) {
-                    short_room |= lacks_room(a, h, s, c, delta_row_s(m, c, oc));
-                    if (delta_row_s(m, c, oc) > 0 && get_cell(a, s, c) == 0) new_row_s++;
-                }
-                if (ic > 0 && c != r && c != s) {
-                    short_room |= lacks_room(a, h, c, s, ic);
-                    if (get_cell(a, c, s) == 0) new_col_s++;
-                }
-            }
-            if (lane == 0) {
-                if (m.out_s == 0 && m.in_r > 0) short_room |= lacks_room(a, h, r, s, m.in_r);
-                if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0))
-                    short_room |= lacks_room(a, h, s, s, m.selfc + m.in_s);
-            }
-            // row s and column s may receive several new cells: check their totals
-            new_row_s = wave_sum(new_row_s) + 2;
-            new_col_s = wave_sum(new_col_s) + 2;
-            const bool short_rs = h.rlen[s] + new_row_s > h.rcap[s], short_cs = h.clen[s] + new_col_s > h.ccap[s];
-            if (lane == 0 && short_rs) want_room(h.rboost, h.rlen, s, new_row_s);
-            if (lane == 0 && short_cs) want_room(h.cboost, h.clen, s, new_col_s);
-            short_room |= short_rs || short_cs;
-            if (ballot(short_room)) {
-                table_clear_used(t);
-                if (lane == 0) {
-                    s_status = 1;
-                    s_stop = (int32_t)item;
-                }
-            } else {
-                // ---- apply the Delta in place (update_edge_counts(delta), dict_transpose_matrix.cpp:430-446)
-                for (int i = 0; i < nb; ++i) {  // one cell at a time: each update is wave-cooperative
+        if (first < MH_WAVES) {
+            bool occupies = false;
+            if (wid == first) {
+                const Table &t = tsel;
+                const int s = m.s;
+                const int r = m.r;
+                const int nb = *t.n_used;
+                // occupying an empty block changes the non-empty block count the nonparametric constants are for
+                occupies = NP && a.sizes[s] == 0;
+                // ---- room check for every Delta cell that would create a new entry
+                bool short_room = false;
+                int new_row_s = 0, new_col_s = 0;
+                for (int i = lane; i < nb; i += WAVE) {
                     const int slot = t.used[i];
                     const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
                     if (oc > 0) {
-                        long long dr = delta_row_r(m, c, oc), ds = delta_row_s(m, c, oc);
-                        if (dr) apply_cell(a, h, r, c, (int)dr);
-                        if (ds) apply_cell(a, h, s, c, (int)ds);
+                        short_room |= lacks_room(a, h, s, c, delta_row_s(m, c, oc));
+                        if (delta_row_s(m, c, oc) > 0 && get_cell(a, s, c) == 0) new_row_s++;
                     }
                     if (ic > 0 && c != r && c != s) {
-                        apply_cell(a, h, c, r, -ic);
-                        apply_cell(a, h, c, s, ic);
+                        short_room |= lacks_room(a, h, c, s, ic);
+                        if (get_cell(a, c, s) == 0) new_col_s++;
                     }
                 }
-                if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0)) apply_cell(a, h, r, r, -m.selfc - m.in_r);
-                if (m.out_s == 0 && m.in_r > 0) apply_cell(a, h, r, s, m.in_r);
-                if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) apply_cell(a, h, s, s, m.selfc + m.in_s);
-                if (m.out_r == 0 && m.in_s > 0) apply_cell(a, h, s, r, -m.in_s);
-                table_clear_used(t);
                 if (lane == 0) {
-                    a.d_out[r] -= (int)m.k_out;
-                    a.d_out[s] += (int)m.k_out;
-                    a.d_in[r] -= (int)m.shift_in;
-                    a.d_in[s] += (int)m.shift_in;
-                    a.sizes[r] -= 1;
-                    a.sizes[s] += 1;
-                    a.assign[m.v] = s;
-                    if (NP && a.hist) {  // keyed by the proposal's edge counts (src/blockmodel/blockmodel.cpp:431-434)
-                        hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(r, 0, m.k_out), -1);
-                        hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(r, 1, m.k_in), -1);
-                        hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(s, 0, m.k_out), 1);
-                        hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(s, 1, m.k_in), 1);
-                    }
+                    if (m.out_s == 0 && m.in_r > 0) short_room |= lacks_room(a, h, r, s, m.in_r);
+                    if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0))
+                        short_room |= lacks_room(a, h, s, s, m.selfc + m.in_s);
                 }
-                wave_sync();
+                // row s and column s may receive several new cells: check their totals
+                new_row_s = wave_sum(new_row_s) + 2;
+                new_col_s = wave_sum(new_col_s) + 2;
+                const bool short_rs = h.rlen[s] + new_row_s > h.rcap[s], short_cs = h.clen[s] + new_col_s > h.ccap[s];
+                if (lane == 0 && short_rs) want_room(h.rboost, h.rlen, s, new_row_s);
+                if (lane == 0 && short_cs) want_room(h.cboost, h.clen, s, new_col_s);
+                short_room |= short_rs || short_cs;
+                const bool lacking = ballot(short_room) != 0;
+                if (lacking) table_clear_used(t);
                 if (lane == 0) {
-                    a.d_tot[r] = a.d_out[r] + a.d_in[r] - (int)get_cell(a, r, r);
-                    a.d_tot[s] = a.d_out[s] + a.d_in[s] - (int)get_cell(a, s, s);
-                    s_sum += dS;
-                    s_moves += 1;
-                    if (occupies) {  // the host refreshes the constants and resumes with the next vertex
-                        s_status = 2;
-                        s_stop = (int32_t)(item + 1);
+                    s_mv[0] = r;
+                    s_mv[1] = s;
+                    s_mv[2] = nb;
+                    s_mv[3] = m.selfc;
+                    s_mv[4] = m.in_r;
+                    s_mv[5] = m.in_s;
+                    s_mv[6] = m.out_r;
+                    s_mv[7] = m.out_s;
+                    s_mv[8] = kall > SMALL_K ? 1 : 0;
+                    s_mv[9] = lacking ? 0 : 1;
+                    if (lacking) {
+                        s_status = 1;
+                        s_stop = (int32_t)item;
                     }
                 }
-                bytes += 12ull * (unsigned long long)kall;
+            }
+            __syncthreads();
+            if (s_mv[9]) {
+                // ---- apply the Delta in place (update_edge_counts(delta), dict_transpose_matrix.cpp:430-446), each
+                // wavefront its share of the lists
+                MoveCtx mm;
+                mm.r = s_mv[0];
+                mm.s = s_mv[1];
+                mm.selfc = s_mv[3];
+                mm.in_r = s_mv[4];
+                mm.in_s = s_mv[5];
+                mm.out_r = s_mv[6];
+                mm.out_s = s_mv[7];
+                const Table t = make_table(a, s_mv[8] != 0, lds + first * LDS_TABLE_INTS, first);
+                mh_apply_share(a, h, mm, t, s_mv[2], wid);
+                __syncthreads();
+                if (wid == first) {
+                    const int r = m.r, s = m.s;
+                    table_clear_used(tsel);
+                    if (lane == 0) {
+                        a.d_out[r] -= (int)m.k_out;
+                        a.d_out[s] += (int)m.k_out;
+                        a.d_in[r] -= (int)m.shift_in;
+                        a.d_in[s] += (int)m.shift_in;
+                        a.sizes[r] -= 1;
+                        a.sizes[s] += 1;
+                        a.assign[m.v] = s;
+                        if (NP && a.hist) {  // keyed by the proposal's edge counts (src/blockmodel/blockmodel.cpp:431-434)
+                            hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(r, 0, m.k_out), -1);
+                            hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(r, 1, m.k_in), -1);
+                            hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(s, 0, m.k_out), 1);
+                            hist_add(a.hist, a.np.hist_shift, a.np.hist_mask, hist_key(s, 1, m.k_in), 1);
+                        }
+                    }
+                    wave_sync();
+                    if (lane == 0) {
+                        a.d_tot[r] = a.d_out[r] + a.d_in[r] - (int)get_cell(a, r, r);
+                        a.d_tot[s] = a.d_out[s] + a.d_in[s] - (int)get_cell(a, s, s);
+                        s_sum += dS;
+                        s_moves += 1;
+                        if (occupies) {  // the host refreshes the constants and resumes with the next vertex
+                            s_status = 2;
+                            s_stop = (int32_t)(item + 1);
+                        }
+                    }
+                    bytes += 12ull * (unsigned long long)kall;
+                }
             }
         }
         __syncthreads();
