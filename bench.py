@@ -160,9 +160,32 @@ def roofline(tot, greedy=True, traffic_file=TRAFFIC_FILE):
             "launches": {k: {kk: vv for kk, vv in v.items() if kk != "total_ms"} for k, v in legs.items()}}
 
 
-def batched_leg(ctx, graphs, algorithm, slots):
-    """The batched workload's graphs one after another on ctx (plain path) and as one sbp_partition_batch."""
+def batched_leg(ctx, graphs, algorithm, slots, plain_max=None):
+    """The batched workload's graphs one after another on ctx (plain path) and as one sbp_partition_batch.
+    With plain_max below the graph count, the comparison with the plain path runs on graphs 0..plain_max-1,
+    and one more sbp_partition_batch over all the graphs gives the throughput of the whole workload."""
     from sbp_amd import native
+    if plain_max is not None and plain_max < len(graphs):
+        full = graphs
+        prms_full = [native.params(algorithm, seed=i) for i in range(len(full))]
+        out = batched_leg(ctx, full[:plain_max], algorithm, slots)
+        dgs = [ctx.upload(g) for g in full]
+        ctx.synchronize()
+        os.environ["SBP_AMD_BATCH_PROGRESS"] = "1"
+        t0 = time.perf_counter()
+        outs, nb, st, fin = ctx.partition_batch(dgs, prms_full, slots=slots)
+        t_all = time.perf_counter() - t0
+        del os.environ["SBP_AMD_BATCH_PROGRESS"]
+        print(f"[bench] batched: all {len(full)} graphs on {slots} slots in {t_all:.2f}s", file=sys.stderr, flush=True)
+        out["whole_workload"] = {
+            "graphs": len(full), "vertices": int(sum(g.N for g in full)), "edges": int(sum(g.E for g in full)),
+            "slots": slots, "seconds": t_all, "graphs_per_s": len(full) / t_all,
+            "proposals_per_s": sum(x.proposals for x in st) / t_all,
+            "p50_finish_s": float(np.percentile(fin, 50)) / 1e3, "last_finish_s": float(fin.max()) / 1e3,
+            "same_as_prefix_run": bool(all(np.array_equal(outs[i], o) for i, o in enumerate(out.pop("_outs")))),
+            "nmi_mean": mean_nmi(nmi(g.truth, o) for g, o in zip(full, outs)),
+        }
+        return out
     prms = [native.params(algorithm, seed=i) for i in range(len(graphs))]
     dgs = [ctx.upload(g) for g in graphs]
     ctx.synchronize()
@@ -194,6 +217,7 @@ def batched_leg(ctx, graphs, algorithm, slots):
         "identical_to_plain": bool(same),
         "nmi_mean": mean_nmi(nmi(g.truth, o) for g, o in zip(graphs, outs)),
         "plain_nmi_mean": mean_nmi(nmi(g.truth, p[0]) for g, p in zip(graphs, plain)),
+        "_outs": outs,
     }
 
 
@@ -347,6 +371,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-graphs", type=int, default=64, help="graphs in the batched leg (0: skip it)")
     ap.add_argument("--batched-slots", type=int, default=None, help="default: the autotuned count, else 16")
+    ap.add_argument("--batched-plain-max", type=int, default=None,
+                    help="compare with the plain path on the first K graphs only, and time one batch over all "
+                         "of them (e.g. --batched-graphs 4096 --batched-plain-max 512)")
     ap.add_argument("--served-requests", type=int, default=32, help="requests in the served leg (0: skip it)")
     ap.add_argument("--served-rate", type=float, default=32.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--served-max-n", type=int, default=200_000, help="upper bound of the requests' N")
@@ -392,7 +419,11 @@ def main():
     from sbp_amd.graph import dcsbm
 
     if args.only:
-        many_b = [f.result() for f in fut_b]
+        many_b = []
+        for i, f in enumerate(fut_b):
+            many_b.append(f.result())
+            if (i + 1) % 256 == 0:
+                print(f"[bench] generated {i + 1} batched graphs", file=sys.stderr, flush=True)
         many_s = []
         for j, f in enumerate(fut_s):
             many_s.append(f.result())
@@ -400,7 +431,8 @@ def main():
         pool.shutdown()
         if args.only == "batched":
             ctx = native.Context(0, seed=0)
-            out = batched_leg(ctx, many_b, args.many_algorithm, args.batched_slots)
+            out = batched_leg(ctx, many_b, args.many_algorithm, args.batched_slots, args.batched_plain_max)
+            out.pop("_outs", None)
         else:
             arrivals = workloads.poisson_arrivals(args.served_rate, args.served_requests, seed=0)
             out = served_leg(many_s, arrivals, args.many_algorithm, args.served_slots, args.served_rate,
@@ -455,7 +487,8 @@ def main():
 
     batched = served = None
     if args.batched_graphs:
-        batched = batched_leg(ctx, many_b, args.many_algorithm, args.batched_slots)
+        batched = batched_leg(ctx, many_b, args.many_algorithm, args.batched_slots, args.batched_plain_max)
+        batched.pop("_outs", None)
     if args.served_requests:
         arrivals = workloads.poisson_arrivals(args.served_rate, args.served_requests, seed=0)
         served = served_leg(many_s, arrivals, args.many_algorithm, args.served_slots, args.served_rate,

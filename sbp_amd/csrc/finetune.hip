@@ -316,18 +316,31 @@ __device__ __forceinline__ void cell_change(Fx &acc, long long M, long long d, d
     if (M2) fx_sub_acc(acc, (double)M2 * (fastlog(M2) - (lin2 + lout2)));
 }
 
+// Counts the blocks of v's neighbours adj[e0, e1) (self-loops excluded) into the table: each lane takes its
+// edges e0 + lane + 64 j four at a time and issues their adjacency and assignment loads before the table
+// updates, so the loads of a high-degree vertex overlap instead of costing two round trips per 64 edges.
+// The lanes' updates race as before; the table's counts do not depend on their order.
+__device__ __forceinline__ void add_neighbour_blocks(const FtArgs &a, const Table &t, const int32_t *adj, int64_t e0,
+                                                     int64_t e1, int v, int dout, int din) {
+    constexpr int U = 4;
+    for (int64_t e = e0 + lane_id(); e < e1; e += U * WAVE) {
+        int u[U], b[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) u[j] = e + j * WAVE < e1 ? adj[e + j * WAVE] : v;
+#pragma unroll
+        for (int j = 0; j < U; ++j) b[j] = u[j] != v ? a.assign[u[j]] : -1;
+#pragma unroll
+        for (int j = 0; j < U; ++j)
+            if (b[j] >= 0) table_add(t, b[j], dout, din);
+    }
+}
+
 template <bool GREEDY>
 __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept) {
     const int lane = lane_id();
     const int v = m.v, r = m.r, s = m.s;
-    for (int64_t e = m.o0 + lane; e < m.o1; e += WAVE) {
-        int u = a.out_adj[e];
-        if (u != v) table_add(t, a.assign[u], 1, 0);
-    }
-    for (int64_t e = m.i0 + lane; e < m.i1; e += WAVE) {
-        int u = a.in_adj[e];
-        if (u != v) table_add(t, a.assign[u], 0, 1);
-    }
+    add_neighbour_blocks(a, t, a.out_adj, m.o0, m.o1, v, 1, 0);
+    add_neighbour_blocks(a, t, a.in_adj, m.i0, m.i1, v, 0, 1);
     wave_sync();
     table_get(t, r, m.out_r, m.in_r);
     table_get(t, s, m.out_s, m.in_s);
@@ -412,14 +425,8 @@ template <bool GREEDY>
 __device__ double evaluate_move_np(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept) {
     const int lane = lane_id();
     const int v = m.v, r = m.r, s = m.s;
-    for (int64_t e = m.o0 + lane; e < m.o1; e += WAVE) {
-        int u = a.out_adj[e];
-        if (u != v) table_add(t, a.assign[u], 1, 0);
-    }
-    for (int64_t e = m.i0 + lane; e < m.i1; e += WAVE) {
-        int u = a.in_adj[e];
-        if (u != v) table_add(t, a.assign[u], 0, 1);
-    }
+    add_neighbour_blocks(a, t, a.out_adj, m.o0, m.o1, v, 1, 0);
+    add_neighbour_blocks(a, t, a.in_adj, m.i0, m.i1, v, 0, 1);
     wave_sync();
     table_get(t, r, m.out_r, m.in_r);
     table_get(t, s, m.out_s, m.in_s);

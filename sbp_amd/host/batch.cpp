@@ -120,7 +120,10 @@ int sbp_partition_batch(sbp_ctx ctx, const sbp_graph *graphs, int32_t n, const s
             if (graphs[a]->E != graphs[b]->E) return graphs[a]->E > graphs[b]->E;
             return graphs[a]->N > graphs[b]->N;
         });
-        std::atomic<int32_t> next{0};
+        std::atomic<int32_t> next{0}, done{0};
+        // SBP_AMD_BATCH_PROGRESS set: one line on stderr each time another 1/32 of the graphs has finished
+        const bool progress = getenv("SBP_AMD_BATCH_PROGRESS") != nullptr;
+        const int32_t step = std::max<int32_t>(1, n / 32);
         std::mutex mu;
         int first_rc = SBP_OK;
         int32_t first_bad = -1;
@@ -140,6 +143,9 @@ int sbp_partition_batch(sbp_ctx ctx, const sbp_graph *graphs, int32_t n, const s
                                              stats ? &stats[i] : &local);
                 if (num_blocks) num_blocks[i] = nb;
                 if (finish_ms) finish_ms[i] = now_ms() - t0;
+                const int32_t d = done.fetch_add(1) + 1;
+                if (progress && (d % step == 0 || d == n))
+                    fprintf(stderr, "[sbp] batch: %d of %d graphs done at %.1f s\n", d, n, (now_ms() - t0) / 1e3);
                 if (rc != SBP_OK) {
                     std::lock_guard<std::mutex> lk(mu);
                     if (first_rc == SBP_OK || i < first_bad) first_rc = rc, first_bad = i, first_err = w->err;
