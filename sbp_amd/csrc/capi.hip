@@ -284,7 +284,9 @@ static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, 
         }
         int threads = ctx->carry_threads > 0 ? ctx->carry_threads : sbpcarry::carry_threads();
         // the parametric carry-out applies merges on a thread of its own beside the pool: leave it a core
-        if (!bm->np && B >= sbpcarry::PIPE_MIN_BLOCKS && threads > 2 && sbpcarry::pipe_enabled()) --threads;
+        if (!bm->np && B >= sbpcarry::PIPE_MIN_BLOCKS && bm->nnz < sbpcarry::PIPE_MAX_ROW_CELLS * (int64_t)B &&
+            threads > 2 && sbpcarry::pipe_enabled())
+            --threads;
         if (!ctx->carry_pool || ctx->carry_pool->threads() != threads)
             ctx->carry_pool = std::make_shared<sbpcarry::Pool>(threads);
         pool = ctx->carry_pool.get();

@@ -894,10 +894,15 @@ class Applier {
 };
 
 // Merges are applied on the Applier thread (SBP_AMD_CARRY_PIPE=0 turns this off) for the parametric model
-// with a thread pool and at least this many blocks.  Fewer blocks share more neighbours, and the loop waits
-// for the previous merges too often: on the 200,000-vertex workload the carry-out took 25% less time at
-// B = 200,000, 14% less at 50,000 and 11% more at 25,000.
+// with a thread pool, at least this many blocks and rows of fewer than PIPE_MAX_ROW_CELLS cells on average.
+// Fewer blocks share more neighbours, and longer rows make each merge's renames long: the one applier thread
+// falls behind and the loop waits for it, where the in-loop path splits the renames over the pool.  On the
+// 200,000-vertex workload the carry-out took 25% less time at B = 200,000, 14% less at 50,000 (about 75 cells
+// per row) and 11% more at 25,000 (about 150).  On the 2,500,000-vertex sample of the 5,000,000-vertex
+// workload it took 23%, 25% and 17% less at B = 2,500,000, 1,250,000 and 625,000 (up to about 95 cells per
+// row), and 17% to 69% more from 312,500 (about 190) down to 39,063.
 constexpr int32_t PIPE_MIN_BLOCKS = 32768;
+constexpr int64_t PIPE_MAX_ROW_CELLS = 128;
 inline bool pipe_enabled() {
     const char *e = getenv("SBP_AMD_CARRY_PIPE");
     return !(e && e[0] == '0');
@@ -1012,8 +1017,12 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
         std::unique_ptr<Applier> applier;
         std::vector<int64_t> busy;
         if (m && !m->np && !m->dc && pool && pool->threads() > 1 && B >= PIPE_MIN_BLOCKS && pipe_enabled()) {
-            applier.reset(new Applier(*m));
-            busy.assign(B, 0);
+            int64_t cells = 0;
+            for (int32_t b = 0; b < B; ++b) cells += m->row[b].n;
+            if (cells < PIPE_MAX_ROW_CELLS * (int64_t)B) {
+                applier.reset(new Applier(*m));
+                busy.assign(B, 0);
+            }
         }
         auto ready = [&](int32_t x) {
             if (applier) applier->wait(busy[x]);
