@@ -49,7 +49,9 @@ double orc_unit_double(uint32_t a, uint32_t b) {
 }
 
 static int32_t random_block(int32_t current, int32_t B, uint32_t w) {
-    /* common::propose_random_block (src/common.cpp:200-209): candidates = U[0, B-2] */
+    /* common::propose_random_block (src/common.cpp:200-209): candidates = U[0, B-2]; with B < 2 there is no
+     * other block (U[0, -1] is undefined in the reference): the current block, no move */
+    if (B < 2) return current;
     int32_t proposed = (int32_t)orc_uniform_below(w, (uint32_t)(B - 1));
     if (proposed >= current) proposed++;
     return proposed;

@@ -232,6 +232,10 @@ static void finetune_assignment(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_p
     const int max_it = p.max_iterations > 0 ? p.max_iterations : 100;
     std::vector<double> de;
     bm->overall_entropy = mdl_of(ctx, bm, g, st);
+    // One block: no vertex has another block to move to.  The reference walks the sweeps anyway (every proposal
+    // is the current block, and an isolated vertex draws from the empty range U[0, B-2]); the result is the same
+    // blockmodel, so the walk is skipped, as metropolis_hastings does (src/finetune.cpp:775-777).
+    if (bm->B < 2) return;
     int32_t *ident = iota_of(ctx, g);
     for (int64_t it = 0; it < max_it; ++it) {
         uint32_t ep = (*epoch)++;
@@ -712,6 +716,11 @@ int sbp_mh_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, uint3
     if (!ctx || !bm || !g || !p) return SBP_E_ARG;
     SBP_API_BEGIN
     bm_set_model(ctx, g, bm, p->nonparametric, p->nonparametric && p->degreecorrected);
+    if (bm->B < 2) {  // one block: nothing can move (src/finetune.cpp:775-777)
+        if (sum_dS) *sum_dS = 0.0;
+        if (stats) stats->sweeps++;
+        return SBP_OK;
+    }
     int32_t *ident = iota_of(ctx, g);
     const int32_t *ord = ident;
     if (!ordered) {
@@ -749,6 +758,7 @@ int sbp_finetune_assignment(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_param
 int sbp_merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *best_s, double *best_dS) {
     if (!ctx || !bm) return SBP_E_ARG;
     SBP_API_BEGIN
+    SBP_CHECK(bm->B >= 2, SBP_E_ARG, "merge proposals need at least two blocks");
     const int32_t B = bm->B;
     int32_t *d_s = ctx->scratch[S_BEST].as<int32_t>(3 * (size_t)B + 4);
     double *d_dS = ctx->scratch[S_DS].as<double>((size_t)B + 2);

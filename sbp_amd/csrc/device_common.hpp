@@ -114,8 +114,10 @@ __device__ __forceinline__ double unit_double(uint32_t a, uint32_t b) {
     return (double)u * 0x1p-53;
 }
 
-// common::propose_random_block (src/common.cpp:200-209) with candidates = U[0, B-2].
+// common::propose_random_block (src/common.cpp:200-209) with candidates = U[0, B-2].  With fewer than two blocks
+// there is no other block (the reference's U[0, -1] is undefined there): propose the current one, no move.
 __device__ __forceinline__ int random_block(int current, int B, uint32_t w) {
+    if (B < 2) return current;
     int p = (int)uniform_below(w, (uint32_t)(B - 1));
     return p >= current ? p + 1 : p;
 }

@@ -309,18 +309,28 @@ void graph_finish(sbp_ctx ctx, sbp_graph g) {
     g->n_high_degree = (int64_t)h[1];
 }
 
-// Graph::sort_vertices order: stable descending degree (computed once per graph).
+// Graph::sort_vertices order: stable descending degree (computed once per graph).  Worker contexts of a batch may
+// share one graph from their own streams: the order is computed under the graph's lock and published only after
+// its stream has finished writing it, so no caller reads a partly written order.
 const int32_t *degree_order(sbp_ctx ctx, sbp_graph g) {
+    std::lock_guard<std::mutex> lk(g->deg_mu);
     if (g->deg_order) return g->deg_order;
     hipStream_t st = ctx->stream;
     const int32_t N = g->N;
     DevBuf<uint64_t> keys(N), sorted(N);
     k_degree_keys<<<grid_for(N), GB, 0, st>>>(N, g->degree, keys.p);
     sort_keys(st, keys.p, sorted.p, N);
-    SBP_HIP(hipMalloc(&g->deg_order, sizeof(int32_t) * N));
-    k_low_words<<<grid_for(N), GB, 0, st>>>(N, sorted.p, g->deg_order);
-    SBP_HIP(hipStreamSynchronize(st));
-    return g->deg_order;
+    int32_t *order = nullptr;
+    SBP_HIP(hipMalloc(&order, sizeof(int32_t) * N));
+    k_low_words<<<grid_for(N), GB, 0, st>>>(N, sorted.p, order);
+    SBP_HIP(hipGetLastError());
+    const hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        (void)hipFree(order);
+        SBP_HIP(e);
+    }
+    g->deg_order = order;
+    return order;
 }
 
 // sample::sample + from_vertices.  Returns the sample as a new device graph that keeps its mapping.

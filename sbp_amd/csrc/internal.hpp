@@ -8,6 +8,7 @@
 #include <initializer_list>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -200,7 +201,9 @@ struct sbp_graph_s {
     int32_t *out_adj = nullptr, *in_adj = nullptr;
     int32_t *edge_src = nullptr;    // [E + N] source vertex of each out-edge (CSR order), then 0..N-1
     int32_t *degree = nullptr;      // [N] |out| + |in| - self-loop flag (Graph::degrees)
-    int32_t *deg_order = nullptr;   // [N] stable descending degree order, computed on first use
+    int32_t *deg_order = nullptr;   // [N] stable descending degree order, computed on first use; published
+                                    // under deg_mu once complete (workers on other streams may share the graph)
+    std::mutex deg_mu;
     int64_t max_degree = 0;         // max over v of |out(v)| + |in(v)|
     int64_t n_high_degree = 0;      // vertices with |out(v)| + |in(v)| > FT_SMALL_DEGREE
     // set when this graph is a sample of `parent` (sbp_sample)

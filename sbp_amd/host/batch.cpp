@@ -189,6 +189,7 @@ struct Request {
     sbp_params p{};
     uint64_t footprint = 0;
     ReqState state = QUEUED;
+    bool claimed = false;  // a sbp_server_wait call owns the result
     double t_submit = 0, t_start = 0, t_done = 0;
     std::vector<int32_t> assignment;
     int32_t num_blocks = 0;
@@ -340,8 +341,15 @@ int sbp_server_wait(sbp_server srv, int64_t ticket, int32_t *assignment_out, int
         srv->err = "sbp_server_wait: unknown ticket";
         return SBP_E_ARG;
     }
-    srv->cv_done.wait(lk, [&] { return srv->reqs[ticket].state == DONE; });
-    Request &r = srv->reqs[ticket];
+    if (it->second.claimed) {  // a ticket is waited for once: a second waiter would find it erased
+        srv->err = "sbp_server_wait: ticket " + std::to_string(ticket) + " is already being waited for";
+        return SBP_E_ARG;
+    }
+    it->second.claimed = true;
+    // std::unordered_map keeps references to its elements valid across inserts of other keys, and only this
+    // waiter erases the entry
+    Request &r = it->second;
+    srv->cv_done.wait(lk, [&] { return r.state == DONE; });
     const int rc = r.rc;
     if (rc == SBP_OK) {
         if (assignment_out) memcpy(assignment_out, r.assignment.data(), sizeof(int32_t) * r.N);

@@ -32,6 +32,9 @@ EXPORTS = [
 ]
 
 
+SBP_E_ARG = -3
+
+
 class SbpError(RuntimeError):
     pass
 
@@ -299,11 +302,14 @@ class Server:
 
     def wait(self, ticket):
         """Returns (assignment, num_blocks, stats, latency_ms, queue_ms)."""
-        N, _ = self._held[ticket]
-        out = np.zeros(N, np.int32)
+        held = self._held.get(ticket)
+        if held is None:
+            raise SbpError(f"sbp_server_wait({ticket}): unknown ticket")
+        out = np.zeros(held[0], np.int32)
         nb, st, lat, q = C.c_int32(), Stats(), C.c_double(), C.c_double()
         rc = lib().sbp_server_wait(self.h, ticket, _p(out), C.byref(nb), C.byref(st), C.byref(lat), C.byref(q))
-        del self._held[ticket]
+        if rc != SBP_E_ARG:  # SBP_E_ARG: another call is waiting for this ticket and still needs its arrays
+            self._held.pop(ticket, None)
         self.check(rc, f"sbp_server_wait({ticket})")
         return out, nb.value, st, lat.value, q.value
 

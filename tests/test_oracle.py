@@ -347,6 +347,24 @@ def test_sweeps_keep_blockmodel_consistent(alg):
     assert abs(bm.mdl() - O.OBlockmodel.build(og, 20, bm.export()["assignment"]).mdl()) == 0.0
 
 
+def test_one_block_with_isolated_vertex_does_not_move():
+    """At B = 1 no vertex has another block: an isolated vertex's random-block draw (U[0, B-2], empty) proposes
+    the current block, and every other proposal is block 0; MH sweeps and finetune_assignment change nothing."""
+    g = dcsbm(300, 4, seed=3)
+    g = CSRGraph(g.N + 1, np.append(g.out_off, g.out_off[-1]).astype(np.int64), g.out_adj.copy(),
+                 np.append(g.in_off, g.in_off[-1]).astype(np.int64), g.in_adj.copy())
+    og = O.OGraph.from_csr(g)
+    assign = np.zeros(g.N, np.int32)
+    bm = O.OBlockmodel.build(og, 1, assign)
+    m0 = bm.mdl()
+    for ordered in (False, True):
+        tot, st = bm.mh_sweep(O.params("metropolis_hastings", seed=2), 0, ordered=ordered)
+        assert tot == 0.0 and st.moves == 0
+    ep, st = bm.finetune_assignment(O.params("metropolis_hastings", seed=2), 0)
+    assert st.moves == 0 and bm.mdl() == m0
+    assert np.array_equal(bm.export()["assignment"], assign) and bm.validate()
+
+
 def test_merge_apply_counts_and_labels():
     g = dcsbm(400, 8, gamma=2.5, kmin=1, kmax=40, mean_degree=8, p_in=0.7, seed=12)
     og = O.OGraph.from_csr(g)
