@@ -49,11 +49,14 @@
  * std::priority_queue; spence against the Cephes dilogarithm and log_q
  * against exact partition counts.  The reference's DEGREE_DL and
  * degree-corrected MDL constants are not reproducible from its code and are
- * not used.  The sweeps (async Gibbs, MH, hybrid, finetune_assignment),
- * merge phases with the carry-out, and the driver have no reference output
- * under any replay: their parity is unpinned against the reference and rests
- * on invariants (rebuild equality after every sweep and merge, thread-count
- * independence) and on recovering planted partitions.
+ * not used.  The sweeps (async Gibbs, MH, hybrid, finetune_assignment), the
+ * merge proposals, both carry-outs with label compaction, and the driver with
+ * its golden-section triplet have no reference output under any replay; they
+ * are pinned against a literal Python restatement of the reference's loops
+ * (tests/reference_loops.py, tests/test_oracle_loops.py), exact on tiny
+ * graphs, with deliberate mistakes in each loop that the comparison rejects,
+ * and on invariants (rebuild equality after every sweep and merge,
+ * thread-count independence, recovering planted partitions).
  *
  * Tags and epochs: every finetune sweep and every merge phase uses its own
  * epoch value, supplied by the driver (the driver numbers them 0, 1, 2, ...
