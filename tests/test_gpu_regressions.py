@@ -71,7 +71,10 @@ def test_server_second_wait_refused():
     got = {}
 
     def first():
-        got["first"] = srv.wait(t)
+        try:
+            got["first"] = srv.wait(t)
+        except native.SbpError as e:
+            got["first_error"] = str(e)
 
     th = threading.Thread(target=first)
     th.start()
@@ -83,7 +86,7 @@ def test_server_second_wait_refused():
     th.join(timeout=120)
     assert not th.is_alive()
     if "second" in got:  # this thread claimed the ticket first; the thread's call was refused
-        assert "first" not in got
+        assert "first" not in got and "first_error" in got
         res = got["second"]
     else:
         assert "already being waited for" in got["second_error"] or "unknown ticket" in got["second_error"]
