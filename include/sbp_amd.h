@@ -162,6 +162,13 @@ int sbp_bm_validate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *ok);
  * (src/blockmodel/blockmodel.cpp:288-306). */
 int sbp_mdl(sbp_ctx ctx, sbp_bm bm, sbp_graph g, double *mdl, double *log_posterior);
 
+/* sbp_mdl for a parametric blockmodel without allocating device memory.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The reduction's work area is a fixed per-device array of 20,500 bytes (the partial sums and error
+ * flags of at most 1,024 workgroups), set up by sbp_create; every blockmodel fits, since the reduction grid is capped at 1,024 workgroups.  Calls that overlap on
+ * one device race on that array.  SBP_E_ARG for a nonparametric blockmodel. */
+int sbp_mdl_noalloc(sbp_ctx ctx, sbp_bm bm, sbp_graph g, double *mdl, double *log_posterior);
+
 /* Sets the blockmodel's entropy model, the reference's global args.nonparametric / args.degreecorrected:
  * 0 = entropy::mdl and its deltas, 1 = entropy::nonparametric (src/entropy.cpp:617-1225; directed), 2 =
  * entropy::nonparametric with degree correction, whose vertex-degree histograms are then built from the

@@ -381,6 +381,7 @@ int sbp_create(sbp_ctx *out, int device, uint64_t seed) {
         SBP_HIP(hipGetDeviceCount(&n));
         SBP_CHECK(device >= 0 && device < n, SBP_E_ARG, "no such HIP device");
         SBP_HIP(hipSetDevice(device));
+        noalloc_setup(device);
         ctx = new sbp_ctx_s();
         ctx->device = device;
         ctx->seed = seed;
@@ -656,6 +657,16 @@ int sbp_mdl(sbp_ctx ctx, sbp_bm bm, sbp_graph g, double *mdl, double *lp) {
         if (lp) *lp = l;
         if (mdl) *mdl = model_term(g->E, g->N, bm->B) - l;
     }
+    SBP_API_END(ctx)
+}
+
+int sbp_mdl_noalloc(sbp_ctx ctx, sbp_bm bm, sbp_graph g, double *mdl, double *lp) {
+    if (!ctx || !bm || !g) return SBP_E_ARG;
+    SBP_API_BEGIN
+    SBP_CHECK(!bm->np, SBP_E_ARG, "sbp_mdl_noalloc: the parametric description length only (entropy::mdl)");
+    const double l = log_posterior_noalloc(ctx, bm);
+    if (lp) *lp = l;
+    if (mdl) *mdl = model_term(g->E, g->N, bm->B) - l;
     SBP_API_END(ctx)
 }
 

@@ -269,6 +269,10 @@ void bm_ensure_hash(sbp_ctx ctx, sbp_bm bm);
 void bm_ensure_cells(sbp_bm bm, int64_t nnz);
 // mdl.hip
 double log_posterior(sbp_ctx ctx, sbp_bm bm);
+// The same reduction with its work area in a fixed per-device array (sbp_mdl_noalloc): allocates nothing.
+double log_posterior_noalloc(sbp_ctx ctx, sbp_bm bm);
+// Looks up the fixed array's address on the current device (once per device); sbp_create calls it.
+void noalloc_setup(int device);
 double model_term(int64_t E, int32_t N, int32_t B);
 double host_lgamma(int64_t x);
 double host_lbinom(int64_t n, int64_t k);

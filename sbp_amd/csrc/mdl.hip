@@ -9,6 +9,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "device_common.hpp"
@@ -20,14 +21,18 @@ using namespace sbpdev;
 
 constexpr int MDL_BLOCK = 256;
 
+// One partial sum and one error flag per workgroup; nothing needs to be cleared before the launch.
 __global__ void __launch_bounds__(MDL_BLOCK) k_log_posterior(int64_t nnz, int32_t B, const int32_t *__restrict__ row_ptr,
                                                              const int2 *__restrict__ cells,
                                                              const int32_t *__restrict__ d_out,
                                                              const int32_t *__restrict__ d_in, Fx *__restrict__ partials,
-                                                             int32_t *__restrict__ err) {
+                                                             int32_t *__restrict__ errs) {
     __shared__ Fx wave_part[MDL_BLOCK / WAVE];
+    __shared__ int32_t block_err;
     const int lane = lane_id(), wid = threadIdx.x / WAVE;
     const int64_t waves = (int64_t)gridDim.x * (MDL_BLOCK / WAVE);
+    if (threadIdx.x == 0) block_err = 0;
+    __syncthreads();
     Fx acc = fx_zero();
     for (int64_t chunk = blockIdx.x * (int64_t)(MDL_BLOCK / WAVE) + wid; chunk * WAVE < nnz; chunk += waves) {
         int64_t first = chunk * WAVE;
@@ -45,7 +50,7 @@ __global__ void __launch_bounds__(MDL_BLOCK) k_log_posterior(int64_t nnz, int32_
             int2 cell = cells[i];
             double value = (double)cell.y;
             double dout = (double)d_out[r], din = (double)d_in[cell.x];
-            if (dout == 0.0 || din == 0.0) atomicExch(err, 1);
+            if (dout == 0.0 || din == 0.0) block_err = 1;
             double ratio = value / (dout * din);
             double term = value * (double)logf_glibc((float)ratio);
             fx_acc(acc, term);
@@ -58,45 +63,114 @@ __global__ void __launch_bounds__(MDL_BLOCK) k_log_posterior(int64_t nnz, int32_
         Fx t = wave_part[0];
         for (int w = 1; w < MDL_BLOCK / WAVE; ++w) fx_add(t, wave_part[w]);
         partials[blockIdx.x] = t;
+        errs[blockIdx.x] = block_err;
     }
 }
 
-__global__ void __launch_bounds__(MDL_BLOCK) k_sum_partials(int n, const Fx *__restrict__ partials, Fx *__restrict__ out) {
+// Sums the n partials into *out and ORs the n error flags into *err_out (errs and err_out may be null).
+__global__ void __launch_bounds__(MDL_BLOCK) k_sum_partials(int n, const Fx *__restrict__ partials,
+                                                            const int32_t *__restrict__ errs, Fx *__restrict__ out,
+                                                            int32_t *__restrict__ err_out) {
     __shared__ Fx wave_part[MDL_BLOCK / WAVE];
+    __shared__ int32_t wave_err[MDL_BLOCK / WAVE];
     Fx acc = fx_zero();
-    for (int i = threadIdx.x; i < n; i += MDL_BLOCK) fx_add(acc, partials[i]);
+    int32_t e = 0;
+    for (int i = threadIdx.x; i < n; i += MDL_BLOCK) {
+        fx_add(acc, partials[i]);
+        if (errs) e |= errs[i];
+    }
     acc = wave_sum_fx(acc);
+    e = wave_sum(e);
     const int lane = lane_id(), wid = threadIdx.x / WAVE;
-    if (lane == 0) wave_part[wid] = acc;
+    if (lane == 0) {
+        wave_part[wid] = acc;
+        wave_err[wid] = e;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         Fx t = wave_part[0];
-        for (int w = 1; w < MDL_BLOCK / WAVE; ++w) fx_add(t, wave_part[w]);
+        int32_t x = wave_err[0];
+        for (int w = 1; w < MDL_BLOCK / WAVE; ++w) {
+            fx_add(t, wave_part[w]);
+            x |= wave_err[w];
+        }
         *out = t;
+        if (err_out) *err_out = x;
     }
 }
 
-double log_posterior(sbp_ctx ctx, sbp_bm bm) {
+// Workgroups of the log-posterior reduction for nnz cells, at most max_blocks (the kernel strides over the rest).
+static int64_t lp_blocks(int64_t nnz, int64_t max_blocks) {
+    const int64_t chunks = (nnz + WAVE - 1) / WAVE;
+    const int64_t blocks = (chunks + (MDL_BLOCK / WAVE) - 1) / (MDL_BLOCK / WAVE);
+    return std::max<int64_t>(1, std::min(blocks, max_blocks));
+}
+
+// Launches the reduction with its partials in `work` (blocks + 1 Fx, then blocks + 1 int32 flags) and reads the
+// total back through the context's pinned staging area.
+static double lp_run(sbp_ctx ctx, sbp_bm bm, int64_t blocks, void *work) {
     hipStream_t st = ctx->stream;
-    if (bm->nnz == 0) return 0.0;
-    int64_t chunks = (bm->nnz + WAVE - 1) / WAVE;
-    int64_t blocks = (chunks + (MDL_BLOCK / WAVE) - 1) / (MDL_BLOCK / WAVE);
-    if (blocks > 4096) blocks = 4096;
-    Fx *partials = ctx->scratch[S_PARTIAL].as<Fx>(blocks + 2);
-    int32_t *err = reinterpret_cast<int32_t *>(partials + blocks + 1);
-    SBP_HIP(hipMemsetAsync(err, 0, sizeof(int32_t), st));
+    Fx *partials = static_cast<Fx *>(work);
+    int32_t *errs = reinterpret_cast<int32_t *>(partials + blocks + 1);
     k_log_posterior<<<(int)blocks, MDL_BLOCK, 0, st>>>(bm->nnz, bm->B, bm->row_ptr, bm->row_cells, bm->d_out,
-                                                         bm->d_in, partials, err);
-    k_sum_partials<<<1, MDL_BLOCK, 0, st>>>((int)blocks, partials, partials + blocks);
+                                                         bm->d_in, partials, errs);
+    k_sum_partials<<<1, MDL_BLOCK, 0, st>>>((int)blocks, partials, errs, partials + blocks, errs + blocks);
     SBP_HIP(hipGetLastError());
     struct {
         Fx total;
         int32_t err;
     } h;
-    read_small(ctx, {{&h.total, partials + blocks, sizeof(Fx)}, {&h.err, err, sizeof(int32_t)}});
+    read_small(ctx, {{&h.total, partials + blocks, sizeof(Fx)}, {&h.err, errs + blocks, sizeof(int32_t)}});
     // The reference exits when a non-zero cell has a zero degree (blockmodel.cpp:297-302).
     SBP_CHECK(h.err == 0, SBP_E_STATE, "log_posterior: non-zero cell with zero block degree");
     return (double)h.total.hi + (double)h.total.lo * 0x1p-64;
+}
+
+static size_t lp_work_bytes(int64_t blocks) { return sizeof(Fx) * (blocks + 1) + sizeof(int32_t) * (blocks + 1); }
+
+double log_posterior(sbp_ctx ctx, sbp_bm bm) {
+    if (bm->nnz == 0) return 0.0;
+    const int64_t blocks = lp_blocks(bm->nnz, 4096);
+    return lp_run(ctx, bm, blocks, ctx->scratch[S_PARTIAL].get(lp_work_bytes(blocks)));
+}
+
+// ------------------------------------------------------------------ allocation-free MDL (sbp_mdl_noalloc)
+//
+// The reduction's work area lives in a fixed device array instead of the context's growable scratch, so the call
+// allocates nothing.  The grid is capped at NOALLOC_BLOCKS workgroups (1,024 x 4 wavefronts, enough to cover the
+// 256 CUs; the kernel strides over any number of cells), so every blockmodel fits.  The array is one per device
+// and holds the last call's partials; calls that overlap on one device race on it.
+constexpr int64_t NOALLOC_BLOCKS = 1024;
+constexpr size_t NOALLOC_BYTES = sizeof(Fx) * (NOALLOC_BLOCKS + 1) + sizeof(int32_t) * (NOALLOC_BLOCKS + 1);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex noalloc_mu;
+static void *noalloc_ptr[NOALLOC_MAX_DEVICES];  // device address of `scratch` on each device, set at set-up
+
+// Looks up the array's device address on `device` (current), once per device; called by sbp_create.  In host code
+// the name `scratch` is only a host-side handle for the symbol, so kernels get this address instead.
+void noalloc_setup(int device) {
+    SBP_CHECK(device >= 0 && device < NOALLOC_MAX_DEVICES, SBP_E_ARG, "device index beyond the supported range");
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (noalloc_ptr[device]) return;
+    void *ptr = nullptr;
+    SBP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    noalloc_ptr[device] = ptr;
+}
+
+double log_posterior_noalloc(sbp_ctx ctx, sbp_bm bm) {
+    void *work = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(noalloc_mu);
+        work = (ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES) ? noalloc_ptr[ctx->device] : nullptr;
+    }
+    SBP_CHECK(work != nullptr, SBP_E_STATE, "sbp_mdl_noalloc: the device's work area was not set up (sbp_create)");
+    if (bm->nnz == 0) return 0.0;
+    const int64_t blocks = lp_blocks(bm->nnz, NOALLOC_BLOCKS);
+    SBP_CHECK(lp_work_bytes(blocks) <= NOALLOC_BYTES, SBP_E_ARG,
+              "sbp_mdl_noalloc: the reduction needs more than the fixed work area");
+    return lp_run(ctx, bm, blocks, work);
 }
 
 // E*h(x) + N*ln(B), x = B^2/E, h(x) = (1+x)ln(1+x) - x ln x (src/entropy.cpp:600-603), host glibc.
@@ -273,7 +347,7 @@ double np_mdl(sbp_ctx ctx, sbp_graph g, sbp_bm bm, double *parts) {
     Fx *partials = ctx->scratch[S_PARTIAL].as<Fx>(blocks + 2);
     k_np_sparse<<<(int)blocks, MDL_BLOCK, 0, st>>>(bm->nnz, bm->B, bm->row_cells, bm->d_out, bm->d_in, bm->sizes,
                                                      g->N, g->out_off, g->in_off, c, partials);
-    k_sum_partials<<<1, MDL_BLOCK, 0, st>>>((int)blocks, partials, partials + blocks);
+    k_sum_partials<<<1, MDL_BLOCK, 0, st>>>((int)blocks, partials, nullptr, partials + blocks, nullptr);
     SBP_HIP(hipGetLastError());
     Fx h;
     std::vector<int32_t> sizes(bm->B);

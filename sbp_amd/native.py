@@ -19,7 +19,7 @@ EXPORTS = [
     "sbp_graph_upload", "sbp_graph_free", "sbp_graph_info", "sbp_graph_download",
     "sbp_sample", "sbp_extend", "sbp_detach", "sbp_reattach",
     "sbp_bm_build", "sbp_bm_copy", "sbp_bm_free", "sbp_bm_info", "sbp_bm_download", "sbp_bm_validate",
-    "sbp_mdl", "sbp_bm_set_model", "sbp_np_description_length",
+    "sbp_mdl", "sbp_mdl_noalloc", "sbp_bm_set_model", "sbp_np_description_length",
     "sbp_ft_propose_all", "sbp_async_gibbs_sweep", "sbp_mh_sweep", "sbp_mcmc", "sbp_finetune_assignment",
     "sbp_merge_propose", "sbp_merge_blocks",
     "sbp_partition",
@@ -113,6 +113,7 @@ def lib():
             "sbp_bm_download": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
             "sbp_bm_validate": [vp, vp, vp, vp],
             "sbp_mdl": [vp, vp, vp, vp, vp],
+            "sbp_mdl_noalloc": [vp, vp, vp, vp, vp],
             "sbp_bm_set_model": [vp, vp, vp, i32],
             "sbp_np_description_length": [vp, vp, vp, vp, vp, vp, vp, vp],
             "sbp_ft_propose_all": [vp, vp, vp, u32, i32, vp, vp, vp],
@@ -473,6 +474,12 @@ class DeviceBlockmodel:
         ok = C.c_int32()
         self.ctx.check(lib().sbp_bm_validate(self.ctx.h, self.h, self.g.h, C.byref(ok)), "sbp_bm_validate")
         return bool(ok.value)
+
+    def mdl_noalloc(self):
+        """(mdl, log_posterior) through sbp_mdl_noalloc: the same values, no device allocation."""
+        m, lp = C.c_double(), C.c_double()
+        self.ctx.check(lib().sbp_mdl_noalloc(self.ctx.h, self.h, self.g.h, C.byref(m), C.byref(lp)), "sbp_mdl_noalloc")
+        return m.value, lp.value
 
     def mdl(self):
         m, lp = C.c_double(), C.c_double()
