@@ -911,14 +911,37 @@ void shuffle_keys(sbp_ctx ctx, const int32_t *d_items, int64_t n, uint32_t epoch
 
 // ------------------------------------------------------------------ Metropolis-Hastings
 
+// State of the walk between launches.  The host queues several rounds of k_mh -> k_mh_scan -> k_mh_advance
+// without reading anything back; each kernel acts only in the status it expects and is a no-op otherwise.
+enum MhStatus : int32_t {
+    MH_RUN = 0,     // k_mh walks on from stop_at
+    MH_SLACK = 1,   // a row or column lacks room at stop_at: the host re-lays the matrix out
+    MH_OCCUPY = 2,  // a move occupied an empty block (nonparametric): the host refreshes the constants
+    MH_SCAN = 3,    // k_mh handed over after windows without an accepted move: the wide scan evaluates ahead
+    MH_DONE = 4,    // the walk reached the end of the order
+};
+
 struct MhState {
     int32_t *rstart, *rlen, *rcap, *cstart, *clen, *ccap;  // [B]
     int32_t *rboost, *cboost;  // [B] multiplier of a row's / column's extra capacity at the next layout
     int32_t *stop_at;   // first order index not processed (== count when done)
-    int32_t *status;    // 0 done, 1 out of slack
+    int32_t *status;    // MhStatus
     unsigned long long *moves;
     double *sum;        // running sum of accepted delta-MDL in visit order, carried across resumed launches
 };
+
+// The wide scan's work area.
+struct MhScan {
+    unsigned long long *first;       // lowest order index with an accepted proposal in the window (~0 when none)
+    int32_t *width;                  // window length of the next scan (adapted by k_mh_advance)
+    int32_t *n_queued;               // high-degree items of the window queued by the first scan launch
+    int32_t *queue;                  // [MH_SCAN_MAX] their offsets in the window
+    unsigned long long *item_bytes;  // [MH_SCAN_MAX] algorithmic bytes of each evaluated item of the window
+};
+// Windows without an accepted move after which k_mh hands the walk to the wide scan, and the scan's window
+// lengths (first, least, most).
+constexpr int MH_HANDOFF_WINDOWS = 2;
+constexpr int32_t MH_SCAN_FIRST = 4096, MH_SCAN_MIN = 256, MH_SCAN_MAX = 1 << 16;
 
 // Inserts/updates/erases one cell in a sorted slack row: cells[beg, beg+len), capacity cap.
 __device__ void slack_add(int2 *cells, int beg, int32_t *len_p, int key, int delta) {
@@ -1063,8 +1086,13 @@ __device__ void mh_apply_share(const FtArgs &a, const MhState &h, const MoveCtx 
 // evaluation up to and including that move read exactly the state a one-vertex-at-a-time walk reads (no move
 // was applied between them), and the draws are keyed by vertex, so decisions, commits and the visit-order sum
 // of accepted delta-MDL are those of the walk.  Evaluations after the committed move are discarded.
+//
+// The walk starts at *h.stop_at when *h.status is MH_RUN (else the launch does nothing).  With `handoff`, after
+// MH_HANDOFF_WINDOWS windows in a row without an accepted move it stops with MH_SCAN at the next window, for the
+// wide scan to evaluate far ahead against the same state.
 template <bool GREEDY, bool NP>
-__global__ void __launch_bounds__(MH_WAVES * WAVE) k_mh(FtArgs a, MhState h, int32_t start) {
+__global__ void __launch_bounds__(MH_WAVES * WAVE) k_mh(FtArgs a, MhState h, int handoff) {
+    if (*h.status != MH_RUN) return;
     __shared__ int lds[MH_WAVES * LDS_TABLE_INTS];
     __shared__ int s_accept[MH_WAVES];
     __shared__ long long s_next;      // first item of the next window
@@ -1079,13 +1107,14 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) k_mh(FtArgs a, MhState h, int
     const Table big_t = make_table(a, true, lds, wid);
     table_clear_all(small_t);  // the global tables arrive empty (prepare_tables) and are left empty
     if (threadIdx.x == 0) {
-        s_next = start;
-        s_status = 0;
+        s_next = *h.stop_at;
+        s_status = MH_RUN;
         s_stop = (int32_t)a.count;
         s_sum = *h.sum;
         s_moves = 0;
     }
     __syncthreads();
+    int empty_windows = 0;  // the same in every thread: `first` below is
     while (true) {
         const long long base = s_next;
         if (base >= a.count) break;
@@ -1172,7 +1201,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) k_mh(FtArgs a, MhState h, int
                     s_mv[8] = kall > SMALL_K ? 1 : 0;
                     s_mv[9] = lacking ? 0 : 1;
                     if (lacking) {
-                        s_status = 1;
+                        s_status = MH_SLACK;
                         s_stop = (int32_t)item;
                     }
                 }
@@ -1217,7 +1246,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) k_mh(FtArgs a, MhState h, int
                         s_sum += dS;
                         s_moves += 1;
                         if (occupies) {  // the host refreshes the constants and resumes with the next vertex
-                            s_status = 2;
+                            s_status = MH_OCCUPY;
                             s_stop = (int32_t)(item + 1);
                         }
                     }
@@ -1226,15 +1255,122 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) k_mh(FtArgs a, MhState h, int
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) s_next = s_status ? (long long)a.count : base + (first < MH_WAVES ? first + 1 : MH_WAVES);
+        empty_windows = first < MH_WAVES ? 0 : empty_windows + 1;
+        if (threadIdx.x == 0) {
+            long long nxt = base + (first < MH_WAVES ? first + 1 : MH_WAVES);
+            if (s_status != MH_RUN) {
+                nxt = a.count;
+            } else if (handoff && empty_windows >= MH_HANDOFF_WINDOWS && nxt < a.count) {
+                s_status = MH_SCAN;
+                s_stop = (int32_t)nxt;
+                nxt = a.count;
+            }
+            s_next = nxt;
+        }
         __syncthreads();
     }
     if (lane == 0 && bytes) atomicAdd(a.bytes, bytes);
     if (threadIdx.x == 0) {
-        *h.status = s_status;
+        *h.status = s_status == MH_RUN ? MH_DONE : s_status;
         *h.stop_at = s_stop;
         atomicAdd(h.moves, s_moves);
         *h.sum = s_sum;
+    }
+}
+
+// Wide scan of the walk (MH_SCAN): every wavefront of the grid proposes and evaluates the items of
+// [stop_at, stop_at + width) in turn against the current blockmodel, which nothing changes meanwhile, with the
+// device functions k_mh uses, so its accept decisions are the walk's.  The lowest accepted item goes to
+// *first; a wavefront stops once its items pass it (they would be evaluated again after that move).  Every
+// item below the final *first is evaluated, so all of them are rejected (or skipped) proposals of the walk.
+//
+// Like the finetune proposal launch, the scan is two launches: the first (BIG false, neighbour-block tables in
+// LDS) evaluates the items of total degree <= SMALL_K and queues the others; the second (BIG true) gives each
+// queued item a wavefront with a global-memory table.
+__device__ __forceinline__ unsigned long long uniform_first(const unsigned long long *first) {
+    // the same value in every lane; a stale one only delays an exit, since *first only decreases
+    const unsigned long long seen = __atomic_load_n(first, __ATOMIC_RELAXED);
+    return ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(seen >> 32)) << 32) |
+           (unsigned)__builtin_amdgcn_readfirstlane((unsigned)seen);
+}
+
+template <bool GREEDY, bool NP, bool BIG>
+__global__ void __launch_bounds__(FT_BLOCK) k_mh_scan(FtArgs a, MhState h, MhScan sc) {
+    if (*h.status != MH_SCAN) return;
+    __shared__ int lds[BIG ? 1 : FT_WAVES * LDS_TABLE_INTS];
+    const int lane = lane_id(), wid = threadIdx.x / WAVE;
+    const int64_t gwave = (int64_t)blockIdx.x * FT_WAVES + wid;
+    const int64_t nwaves = (int64_t)gridDim.x * FT_WAVES;
+    const Table t = make_table(a, BIG, lds + (BIG ? 0 : wid * LDS_TABLE_INTS), gwave);
+    if (!BIG) table_clear_all(t);  // the global tables arrive empty (prepare_tables) and are left empty
+    const int64_t start = *h.stop_at;
+    const int64_t end = start + *sc.width < a.count ? start + *sc.width : a.count;
+    const int64_t n = BIG ? (int64_t)*sc.n_queued : end - start;
+    for (int64_t j = gwave; j < n; j += nwaves) {
+        const int64_t item = start + (BIG ? (int64_t)sc.queue[j] : j);
+        if ((unsigned long long)item > uniform_first(sc.first)) {
+            if (BIG) continue;  // the queue is in no particular order
+            break;              // this wavefront's later items lie further still
+        }
+        const int v = a.items[item];
+        bool accept = false;
+        unsigned long long pb = 0;
+        if (a.sizes[a.assign[v]] != 1) {  // propose_move skips singleton blocks (src/finetune.cpp:845-847)
+            if (!BIG) {
+                const long long kall = (a.out_off[v + 1] - a.out_off[v]) + (a.in_off[v + 1] - a.in_off[v]);
+                if (kall > SMALL_K) {  // deferred to the global-memory-table launch
+                    if (lane == 0) sc.queue[atomicAdd(sc.n_queued, 1)] = (int32_t)j;
+                    continue;
+                }
+            }
+            MoveCtx m;
+            U4 w;
+            const int s = propose_block(a, v, m, w);
+            if (s == m.r) {
+                pb = proposal_bytes(a, m, 0, GREEDY);
+            } else {
+                evaluate<GREEDY, NP>(a, m, t, w, accept);
+                pb = proposal_bytes(a, m, *t.n_used, GREEDY);
+                table_clear_used(t);
+            }
+        }
+        if (lane == 0) {
+            sc.item_bytes[item - start] = pb;
+            if (accept) atomicMin(sc.first, (unsigned long long)item);
+        }
+    }
+}
+
+// After a scan: the walk skips the rejected items before the first accepted one (adding their bytes) and resumes
+// there, or past the window when nothing was accepted.  The window doubles after a scan without an accepted
+// move and halves when the first one came in its first quarter.  Resets *first for the next scan.
+__global__ void __launch_bounds__(256) k_mh_advance(MhState h, MhScan sc, int64_t count,
+                                                    unsigned long long *bytes_total) {
+    if (*h.status != MH_SCAN) return;
+    __shared__ unsigned long long part[256 / WAVE];
+    const int64_t start = *h.stop_at;
+    const int32_t width = *sc.width;
+    const unsigned long long f = *sc.first;
+    int64_t stop = start + width < count ? start + width : count;
+    const bool found = f < (unsigned long long)stop;
+    if (found) stop = (int64_t)f;
+    unsigned long long b = 0;
+    for (int64_t i = threadIdx.x; i < stop - start; i += blockDim.x) b += sc.item_bytes[i];
+    b = wave_sum(b);
+    if (lane_id() == 0) part[threadIdx.x / WAVE] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < 256 / WAVE; ++w) t += part[w];
+        if (t) atomicAdd(bytes_total, t);
+        int32_t w2 = width;
+        if (!found) w2 = width < MH_SCAN_MAX / 2 ? 2 * width : MH_SCAN_MAX;
+        else if ((int64_t)f - start < width / 4) w2 = width / 2 > MH_SCAN_MIN ? width / 2 : MH_SCAN_MIN;
+        *sc.width = w2;
+        *sc.first = ~0ull;
+        *sc.n_queued = 0;
+        *h.stop_at = (int32_t)stop;
+        *h.status = stop >= count ? MH_DONE : MH_RUN;
     }
 }
 
@@ -1269,6 +1405,33 @@ __global__ void k_compact_copy(int32_t B, const int32_t *__restrict__ start, con
         for (int i = threadIdx.x; i < len[b]; i += blockDim.x) dst[ptr[b] + i] = src[start[b] + i];
 }
 
+constexpr int MH_SCAN_BLOCKS = 1024;  // workgroups of the first scan launch: 4,096 wavefronts over the 256 CUs
+
+template <bool GREEDY, bool NP>
+static void launch_mh_t(const FtArgs &a, const MhState &h, const MhScan &sc, bool scan, int big_blocks,
+                        hipStream_t st) {
+    k_mh<GREEDY, NP><<<1, MH_WAVES * WAVE, 0, st>>>(a, h, scan ? 1 : 0);
+    if (scan) {
+        k_mh_scan<GREEDY, NP, false><<<MH_SCAN_BLOCKS, FT_BLOCK, 0, st>>>(a, h, sc);
+        if (big_blocks > 0) k_mh_scan<GREEDY, NP, true><<<big_blocks, FT_BLOCK, 0, st>>>(a, h, sc);
+        k_mh_advance<<<1, 256, 0, st>>>(h, sc, a.count, a.bytes);
+    }
+    SBP_HIP(hipGetLastError());
+}
+
+// One round of the walk: k_mh, then (with `scan`) the wide scan and the advance.  big_blocks: workgroups of the
+// scan's high-degree launch, one global neighbour-block table per wavefront (0: the graph has no such vertex).
+static void launch_mh(const FtArgs &a, const MhState &h, const MhScan &sc, bool np, bool greedy, bool scan,
+                      int big_blocks, hipStream_t st) {
+    if (np) {
+        if (greedy) launch_mh_t<true, true>(a, h, sc, scan, big_blocks, st);
+        else launch_mh_t<false, true>(a, h, sc, scan, big_blocks, st);
+    } else {
+        if (greedy) launch_mh_t<true, false>(a, h, sc, scan, big_blocks, st);
+        else launch_mh_t<false, false>(a, h, sc, scan, big_blocks, st);
+    }
+}
+
 void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t epoch, const int32_t *d_order,
             int64_t count, double *sum_dS, sbp_stats *stats) {
     hipStream_t st = ctx->stream;
@@ -1294,9 +1457,21 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
     h.moves = reinterpret_cast<unsigned long long *>(book + 4);
     h.sum = reinterpret_cast<double *>(book + 8);
     auto *bytes = ctx->scratch[S_COUNTER].as<unsigned long long>(4);
+    MhScan sc;
+    sc.first = reinterpret_cast<unsigned long long *>(book + 12);
+    sc.width = book + 14;
+    sc.item_bytes = ctx->scratch[S_DS].as<unsigned long long>(MH_SCAN_MAX + MH_SCAN_MAX / 2);
+    sc.queue = reinterpret_cast<int32_t *>(sc.item_bytes + MH_SCAN_MAX);
+    sc.n_queued = book + 15;
+    // book: stop_at 0, status MH_RUN, no accepted scan item, the first scan's width
     SBP_HIP(hipMemsetAsync(book, 0, 64 * sizeof(int32_t), st));
     SBP_HIP(hipMemsetAsync(bytes, 0, 4 * sizeof(unsigned long long), st));
+    k_fill_i32<<<1, 64, 0, st>>>(2, book + 12, -1);
+    k_fill_i32<<<1, 64, 0, st>>>(1, book + 14, MH_SCAN_FIRST);
     k_fill_i32<<<grid_of(2 * ((int64_t)B + 1)), 256, 0, st>>>(2 * ((int64_t)B + 1), h.rboost, 1);
+    // The wide scan pays off once several windows pass without an accepted move; a walk shorter than a few
+    // windows stays in k_mh.
+    const bool scan = count > 4 * MH_WAVES;
     int32_t start = 0, retries = 0;
     float kernel_ms = 0.f;
     while (true) {
@@ -1329,19 +1504,31 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         a.bytes = bytes;
         a.err = reinterpret_cast<int *>(bytes + 1);
         const int lg = std::max(10, log2_ceil(2 * std::min<int64_t>(g->max_degree, bm->B) + 2));
-        a.gtable = g->max_degree > SMALL_K ? prepare_tables(ctx, lg, MH_WAVES) : nullptr;
+        // The scan's high-degree launch takes at most the tables the finetune proposal launch uses for this graph
+        // (one per vertex of total degree > SMALL_K, within kMaxTableBytes), and at least one workgroup's.
+        constexpr size_t kMaxTableBytes = size_t(1) << 30;
+        int64_t big_waves = 0;
+        if (g->max_degree > SMALL_K) {
+            big_waves = std::min<int64_t>(g->n_high_degree, 4096);
+            big_waves = std::min<int64_t>(big_waves, (int64_t)(kMaxTableBytes / (table_ints(lg) * sizeof(int))));
+            big_waves = std::max<int64_t>(FT_WAVES, big_waves / FT_WAVES * FT_WAVES);
+        }
+        a.gtable = g->max_degree > SMALL_K ? prepare_tables(ctx, lg, std::max<int64_t>(MH_WAVES, big_waves)) : nullptr;
         a.gcap_log2 = lg;
         a.m.row_cum = nullptr;
         a.m.col_cum = nullptr;
         SBP_HIP(hipEventRecord(ctx->ev[4], st));
-        if (bm->np) {
-            if (p.greedy) k_mh<true, true><<<1, MH_WAVES * WAVE, 0, st>>>(a, h, start);
-            else k_mh<false, true><<<1, MH_WAVES * WAVE, 0, st>>>(a, h, start);
-        } else {
-            if (p.greedy) k_mh<true, false><<<1, MH_WAVES * WAVE, 0, st>>>(a, h, start);
-            else k_mh<false, false><<<1, MH_WAVES * WAVE, 0, st>>>(a, h, start);
+        // Rounds of walk -> scan -> advance, queued without reading back; each kernel is a no-op outside its
+        // status.  The host reads the status after each batch of rounds.
+        int32_t hs[2] = {MH_RUN, 0};
+        while (true) {
+            for (int round = 0; round < 4; ++round) {
+                launch_mh(a, h, sc, bm->np != 0, p.greedy != 0, scan, (int)(big_waves / FT_WAVES), st);
+                if (!scan) break;
+            }
+            read_small(ctx, {{hs, book, 2 * sizeof(int32_t)}});
+            if (hs[1] != MH_RUN && hs[1] != MH_SCAN) break;
         }
-        SBP_HIP(hipGetLastError());
         SBP_HIP(hipEventRecord(ctx->ev[5], st));
         // compact the slack layout back into the blockmodel's CSR (row and column counts changed)
         size_t tmp2 = 0;
@@ -1352,9 +1539,8 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         tp = ctx->scratch[S_SORT_TMP].get(tmp2);
         SBP_HIP(hipcub::DeviceScan::ExclusiveSum(tp, tmp2, h.rlen, rp, B + 1, st));
         SBP_HIP(hipcub::DeviceScan::ExclusiveSum(tp, tmp2, h.clen, cp, B + 1, st));
-        int32_t nnz[2] = {0, 0}, hs[2] = {0, 0};
-        read_small(ctx, {{&nnz[0], rp + B, sizeof(int32_t)}, {&nnz[1], cp + B, sizeof(int32_t)},
-                         {hs, book, 2 * sizeof(int32_t)}});
+        int32_t nnz[2] = {0, 0};
+        read_small(ctx, {{&nnz[0], rp + B, sizeof(int32_t)}, {&nnz[1], cp + B, sizeof(int32_t)}});
         kernel_ms += (float)elapsed_ms(ctx, ctx->ev[4], ctx->ev[5]);
         SBP_CHECK(nnz[0] == nnz[1], SBP_E_STATE, "Metropolis-Hastings: row and column cell counts diverged");
         bm_ensure_cells(bm, nnz[0]);  // more cells than the compact arrays hold: grow them
@@ -1364,14 +1550,15 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         bm->hash_valid = false;
         bm_compute_cum(ctx, bm);
         SBP_HIP(hipStreamSynchronize(st));
-        if (hs[1] == 0) break;
-        // (1): resume at the vertex that ran out of room; the rows and columns it lacked room in asked for
+        if (hs[1] == MH_DONE) break;
+        // MH_SLACK: resume at the vertex that ran out of room; the rows and columns it lacked room in asked for
         // more (their capacities at least double, up to B + 2, where no move lacks room)
-        // (2): resume after a move into an empty block, with the constants of the new block count
-        retries = (hs[1] == 1 && hs[0] == start) ? retries + 1 : 0;
+        // MH_OCCUPY: resume after a move into an empty block, with the constants of the new block count
+        SBP_CHECK(hs[1] == MH_SLACK || hs[1] == MH_OCCUPY, SBP_E_STATE, "Metropolis-Hastings: unexpected walk status");
+        retries = (hs[1] == MH_SLACK && hs[0] == start) ? retries + 1 : 0;
         SBP_CHECK(retries <= 64, SBP_E_STATE, "Metropolis-Hastings: slack growth did not converge");
         start = hs[0];
-        SBP_HIP(hipMemsetAsync(book, 0, 2 * sizeof(int32_t), st));
+        k_fill_i32<<<1, 64, 0, st>>>(1, h.status, MH_RUN);  // stop_at stays: the walk resumes there
     }
     struct {
         unsigned long long moves;
