@@ -1,29 +1,28 @@
-"""Benchmark: end-to-end stochastic block partitioning on one MI355X.
+"""Benchmark: SBP's hot path on one MI355X.
 
-Workload (default): a directed DCSBM with N=200,000 vertices and 500 planted blocks of lognormal size
-(sigma 0.5); out-degrees follow a power law with exponent 2.1 clipped to [1, 2000] with mean 20 (about
-4M edges); 60% of edges are intra-block.  One step = one full agglomerative SBP run
-(sbp::stochastic_block_partition with asynchronous-Gibbs finetune, block merges at rate 0.5 and the
-golden-section search), with the graph already resident in device memory.
+Headline (`value`): MCMC moves/s on the 5,000,000-vertex workload -- a directed DCSBM with 5,000 lognormal(1.0)
+blocks, power-law out-degree (exponent 2.1, clipped to [1, 20000], mean 16; about 80M edges), 40% intra-block
+edges, seed 0, resident in device memory.  One step is one asynchronous-Gibbs sweep over all 5,000,000 vertices
+(finetune::asynchronous_gibbs' iteration body: a proposal per vertex against the frozen blockmodel, the commit in
+vertex-id order, the rebuild, and the MDL), starting from the planted 5,000-block partition and continuing from
+sweep to sweep.  value = accepted vertex moves per second of wall time (the reference's timers::MCMC_moves
+counts accepted moves); the line also gives proposals/s, the roofline of the dominant proposal kernel
+(algorithmic bytes per launch over its HIP-event time, and the rocprofv3-measured HBM bytes over the same time)
+and a CPU baseline: the oracle's OpenMP asynchronous-Gibbs sweeps on the same graph at the same partition.
 
-value = MCMC vertex-move proposals evaluated per second of whole-run wall time.  The JSON line also
-carries accepted moves, merge proposals, MDL, NMI against the planted partition, the roofline of the
-finetune proposal kernel (algorithmic bytes per launch / average launch time, measured with HIP events on
-the kernel's stream) and a CPU baseline: the oracle's OpenMP asynchronous-Gibbs sweeps on the same graph
-at the planted partition, timed for a bounded sample on the host cores.
-
-Two more legs measure many graphs on the device (sbp_amd/workloads.py names the mixed-size workloads):
-- "batched": graphs 0..G-1 of the batched workload (N log-uniform in [200, 50000]), resident in device
-  memory, partitioned one after another on one context (the plain GPU path) and as one
-  sbp_partition_batch call over worker slots; graphs/s of both, and whether every result is identical.
-- "served": the first R requests of the served workload (N log-uniform in [1000, --served-max-n]) arriving
-  as a Poisson process, replayed against a request server with one worker (the plain sequential path) and
-  with several; p50/p99 latency from arrival to completion and graphs/s of both.
-
-`--only sampled` runs the large single-graph workload instead (5,000,000 vertices, about 80M edges, 5,000
-lognormal(1.0) blocks, p_in 0.4): per sampler (max_degree, degree_weighted) a 50% sample is partitioned,
-extended to the full graph and finetuned there; wall time per phase, proposals/s, MDL, NMI and the roofline
-of the full-graph finetune launches.
+Further legs, each once per run:
+- "sampled": the same graph's whole sampled flow (degree-weighted 50% sample, partition of the sample, extension
+  to the full graph, finetune_assignment) with wall time per phase.
+- "c2": the 200,000-vertex workload (500 lognormal(0.5) blocks, power law 2.1 on [1, 2000], mean 20, p_in 0.6):
+  full agglomerative partitions with asynchronous-Gibbs finetune, against the oracle's committed partition.
+- "c1": the 1,000-vertex workload (16 equal blocks, power law 2.5 on [1, 100], mean 10, p_in 0.75): full
+  Metropolis-Hastings partitions, and the oracle's full run of the same on one host core.
+- "batched": graphs of the batched mixed-size workload (N log-uniform in [200, 50000]) resident on the device:
+  all of them in one sbp_partition_batch call, and the first few one after another on one context (the plain
+  GPU path) and as a batch, with graphs/s of both and whether every result is identical.
+- "served": requests of the served workload (N log-uniform in [1000, 2000000]) arriving as a Poisson process,
+  replayed against a request server with one worker (the plain sequential path) and with several; p50/p99
+  latency from arrival to completion and graphs/s of both.
 """
 import argparse
 import concurrent.futures as cf
@@ -40,16 +39,27 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-WORKLOADS = {
-    "c2": dict(desc="directed DCSBM N=200000 B=500 lognormal(0.5) sizes, power-law out-degree gamma=2.1 "
-                    "on [1,2000] mean 20, p_in=0.6, seed 0; async_gibbs finetune + block_merge",
-               N=200000, B=500, gamma=2.1, kmin=1, kmax=2000, mean_degree=20.0, p_in=0.6, sigma=0.5,
-               algorithm="async_gibbs"),
-    "c1": dict(desc="directed DCSBM N=1000 B=16 equal sizes, power-law out-degree gamma=2.5 on [1,100] "
-                    "mean 10, p_in=0.75, seed 0; metropolis_hastings finetune + block_merge",
-               N=1000, B=16, gamma=2.5, kmin=1, kmax=100, mean_degree=10.0, p_in=0.75, sigma=None,
-               algorithm="metropolis_hastings"),
+GRAPHS = {
+    "c3": dict(desc="directed DCSBM N=5000000 B=5000 lognormal(1.0) sizes, power-law out-degree gamma=2.1 on "
+                    "[1,20000] mean 16, p_in=0.4, seed 0",
+               N=5_000_000, B=5000, gamma=2.1, kmin=1, kmax=20000, mean_degree=16.0, p_in=0.4, sigma=1.0),
+    "c2": dict(desc="directed DCSBM N=200000 B=500 lognormal(0.5) sizes, power-law out-degree gamma=2.1 on "
+                    "[1,2000] mean 20, p_in=0.6, seed 0",
+               N=200000, B=500, gamma=2.1, kmin=1, kmax=2000, mean_degree=20.0, p_in=0.6, sigma=0.5),
+    "c1": dict(desc="directed DCSBM N=1000 B=16 equal sizes, power-law out-degree gamma=2.5 on [1,100] mean 10, "
+                    "p_in=0.75, seed 0",
+               N=1000, B=16, gamma=2.5, kmin=1, kmax=100, mean_degree=10.0, p_in=0.75, sigma=None),
 }
+
+
+def make_graph(key, n=None):
+    from sbp_amd.graph import dcsbm
+    w = dict(GRAPHS[key])
+    if n is not None and n != w["N"]:  # a scaled-down copy (blocks scale with N), for quick checks
+        w["B"] = max(1, round(w["B"] * n / w["N"]))
+        w["N"] = n
+    return dcsbm(w["N"], w["B"], gamma=w["gamma"], kmin=w["kmin"], kmax=w["kmax"], mean_degree=w["mean_degree"],
+                 p_in=w["p_in"], sigma=w["sigma"], seed=0), w
 
 
 def nmi(a, b):
@@ -77,36 +87,23 @@ def mean_nmi(values):
     return float(np.mean(v)) if v else None
 
 
-def cpu_baseline(g, algorithm, seconds, threads):
+def cpu_baseline(g, B, seconds, threads):
+    """The oracle's OpenMP asynchronous-Gibbs sweeps over g from its planted partition, as many as fit in about
+    `seconds` (at least one): (proposals/s, accepted moves/s, sweeps, seconds)."""
     import oracle as O
     og = O.OGraph.from_csr(g)
-    ob = O.OBlockmodel.build(og, int(g.truth.max()) + 1, g.truth)
-    prm = O.params(algorithm if algorithm != "metropolis_hastings" else "metropolis_hastings", seed=0,
-                   threads=threads)
-    props, t0, ep = 0, time.perf_counter(), 0
+    ob = O.OBlockmodel.build(og, B, g.truth)
+    prm = O.params("async_gibbs", seed=0, threads=threads)
+    props = moves = 0
+    t0, ep = time.perf_counter(), 0
     while time.perf_counter() - t0 < seconds or ep == 0:
-        if algorithm == "metropolis_hastings":
-            _, st = ob.mh_sweep(prm, ep)
-        else:
-            _, st = ob.async_gibbs_sweep(prm, ep)
+        _, st = ob.async_gibbs_sweep(prm, ep)
+        ob.mdl()  # each iteration of asynchronous_gibbs ends with the MDL (src/finetune.cpp:88)
         props += st.proposals
+        moves += st.moves
         ep += 1
     dt = time.perf_counter() - t0
-    return props / dt, ep, dt
-
-
-def gpu_sweeps_at_truth(ctx, dg, g, algorithm, sweeps):
-    from sbp_amd import native
-    db = native.DeviceBlockmodel.build(dg, int(g.truth.max()) + 1, g.truth)
-    prm = native.params(algorithm, seed=0)
-    props, t0 = 0, time.perf_counter()
-    for ep in range(sweeps):
-        if algorithm == "metropolis_hastings":
-            _, st = db.mh_sweep(prm, ep)
-        else:
-            st = db.async_gibbs_sweep(prm, ep)
-        props += st.proposals
-    return props / (time.perf_counter() - t0)
+    return props / dt, moves / dt, ep, dt
 
 
 KERNEL_NAMES = {  # rocprofv3 names of the finetune proposal launches (greedy / non-greedy, parametric entropy)
@@ -114,7 +111,8 @@ KERNEL_NAMES = {  # rocprofv3 names of the finetune proposal launches (greedy / 
     "ft_high": ("k_ft_propose<true, true, false>", "k_ft_propose<true, false, false>"),
     "mh": ("k_mh<true, false>", "k_mh<false, false>"),  # the sequential walk (windowed)
 }
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ft_propose_c2_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ft_propose_c3_traffic.txt")
+C2_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ft_propose_c2_traffic.txt")
 
 
 def committed_traffic(kernel, path=TRAFFIC_FILE):
@@ -152,8 +150,12 @@ def roofline(tot, greedy=True, traffic_file=TRAFFIC_FILE):
         leg["achieved_GBps"] = leg["algorithmic_bytes_per_launch"] / (leg["avg_launch_ms"] * 1e-3) / 1e9
     dom = max(legs.values(), key=lambda l: l["total_ms"])
     traffic = committed_traffic(dom["kernel"], traffic_file) if traffic_file else None
+    # HBM bytes the counters measured per launch of the same kernel on the same workload, over this run's average
+    # launch time: the fraction of the HBM peak the kernel actually moves
+    hbm_gbs = traffic / (dom["avg_launch_ms"] * 1e-3) / 1e9 if traffic is not None else None
     return {"bound": "hbm", "achieved": dom["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": dom["achieved_GBps"] / HBM_PEAK_GBS, "traffic": traffic,
+            "hbm_achieved": hbm_gbs, "hbm_frac": hbm_gbs / HBM_PEAK_GBS if hbm_gbs is not None else None,
             "kernel": dom["kernel"], "avg_launch_ms": dom["avg_launch_ms"],
             "algorithmic_bytes_per_launch": dom["algorithmic_bytes_per_launch"],
             "traffic_source": os.path.relpath(traffic_file, ROOT) if traffic is not None else None,
@@ -275,78 +277,148 @@ def served_leg(graphs, arrivals, algorithm, slots, rate, max_n):
     }
 
 
-SAMPLED = dict(N=5_000_000, B=5000, gamma=2.1, kmin=1, kmax=20000, mean_degree=16.0, p_in=0.4, sigma=1.0)
-SAMPLED_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ft_propose_c3_traffic.txt")
-
-
-def sampled_leg(algs, algorithm, n):
-    """The large single-graph workload: a directed DCSBM with 5,000,000 vertices and 5,000 lognormal(1.0)
-    blocks, out-degree power law 2.1 on [1, 20000] with mean 16 (about 80M edges), p_in 0.4, seed 0.  Per
-    sampler: keep 50% of the vertices (sample::sample + from_vertices), partition the sample, extend its
-    partition to the full graph (sample::extend) and finetune it there (finetune_assignment).  The graph is
-    resident on the device before the clock starts."""
+def headline_leg(ctx, dg, g, w, steps, warmup):
+    """Asynchronous-Gibbs sweeps over the whole graph from its planted partition: `warmup` untimed, then `steps`
+    timed, each with the MDL that closes an iteration of finetune::asynchronous_gibbs (src/finetune.cpp:50-88)."""
     from sbp_amd import native
-    from sbp_amd.graph import dcsbm
-    os.environ.setdefault("SBP_AMD_TRACE", "1")  # one line per merge phase, so a long partition shows progress
-    w = dict(SAMPLED, N=n, B=max(1, round(SAMPLED["B"] * n / SAMPLED["N"])))
-    t0 = time.perf_counter()
-    g = dcsbm(w["N"], w["B"], gamma=w["gamma"], kmin=w["kmin"], kmax=w["kmax"], mean_degree=w["mean_degree"],
-              p_in=w["p_in"], sigma=w["sigma"], seed=0)
-    print(f"[bench] sampled: graph N={g.N} E={g.E} generated in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
-          flush=True)
-    ctx = native.Context(0, seed=0)
-    warm = dcsbm(2000, 12, gamma=2.5, kmin=1, kmax=100, mean_degree=10, p_in=0.7, seed=1)
-    ctx.upload(warm).partition(native.params(algorithm, seed=0))  # loads the code objects
-    dg = ctx.upload(g)
+    db = native.DeviceBlockmodel.build(dg, w["B"], g.truth)
+    prm = native.params("async_gibbs", seed=0)
+    mdl0 = db.mdl()[0]
+    ep = 0
+    for _ in range(warmup):
+        db.async_gibbs_sweep(prm, ep)
+        db.mdl()
+        ep += 1
     ctx.synchronize()
+    tot = {k: 0.0 for k in native.Stats().as_dict()}
+    t0 = time.perf_counter()
+    for i in range(steps):
+        st = db.async_gibbs_sweep(prm, ep)
+        mdl = db.mdl()[0]
+        ep += 1
+        d = st.as_dict()
+        for k in tot:
+            tot[k] += d[k]
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    print(f"[bench] headline: {steps} sweeps in {wall:.2f}s, {tot['moves'] / wall:.0f} moves/s, "
+          f"{tot['proposals'] / wall:.0f} proposals/s, mdl {mdl0:.1f} -> {mdl:.1f}", file=sys.stderr, flush=True)
+    return tot, wall, mdl0, mdl, db.B
+
+
+def sampled_leg(ctx, dg, g, w, sampler, algorithm="async_gibbs"):
+    """The large graph's sampled flow: keep 50% of the vertices (sample::sample + from_vertices), partition the
+    sample, extend its partition to the full graph (sample::extend) and finetune it there
+    (finetune_assignment).  The graph is resident on the device before the clock starts."""
+    from sbp_amd import native
+    os.environ.setdefault("SBP_AMD_TRACE", "1")  # one line per merge phase, so a long partition shows progress
     prm = native.params(algorithm, seed=0)
-    planted_mdl = native.DeviceBlockmodel.build(dg, w["B"], g.truth).mdl()[0]
-    runs = {}
-    for alg in algs:
-        ctx.set_seed(0)
-        t = [time.perf_counter()]
-        sg, mapping = dg.sample(alg, 0.5, seed=0)
-        ctx.synchronize()
-        t.append(time.perf_counter())
-        print(f"[bench] sampled/{alg}: sample N={sg.N} E={sg.E} in {t[1] - t[0]:.2f}s", file=sys.stderr, flush=True)
-        a_s, nb, st_p = sg.partition(prm)
-        t.append(time.perf_counter())
-        print(f"[bench] sampled/{alg}: sample partitioned, B={nb} in {t[2] - t[1]:.1f}s", file=sys.stderr,
-              flush=True)
-        sbm = native.DeviceBlockmodel.build(sg, nb, a_s)
-        full, _ = sbm.extend(dg, seed=0)
-        ctx.synchronize()
-        t.append(time.perf_counter())
-        ctx.set_seed(0)
-        _, mdl, st_f = full.finetune_assignment(prm, 0)
-        a = full.export()["assignment"]
-        t.append(time.perf_counter())
-        print(f"[bench] sampled/{alg}: extended and finetuned in {t[3] - t[2]:.1f}s + {t[4] - t[3]:.1f}s, "
-              f"mdl {mdl:.2f}", file=sys.stderr, flush=True)
-        del sbm, full, sg
-        wall = t[4] - t[0]
-        ft = st_f.as_dict()
-        sampled_truth = g.truth[mapping >= 0][np.argsort(mapping[mapping >= 0])]
-        runs[alg] = {
-            "sample_vertices": int((mapping >= 0).sum()), "num_blocks": nb, "mdl": mdl,
-            "nmi": nmi(g.truth, a), "nmi_sample": nmi(sampled_truth, a_s),
-            "wall_s": wall, "phase_s": {"sample": t[1] - t[0], "partition_sample": t[2] - t[1],
-                                        "extend": t[3] - t[2], "finetune_full": t[4] - t[3]},
-            "proposals_per_s": (st_p.proposals + st_f.proposals) / wall,
-            "accepted_moves_per_s": (st_p.moves + st_f.moves) / wall,
-            "partition_sample": dict({k: st_p.as_dict()[k] for k in ("proposals", "moves", "sweeps", "merge_phases",
-                                                                     "finetune_ms", "merge_ms", "carry_ms", "build_ms",
-                                                                     "mdl_ms")},
-                                     roofline=roofline(st_p.as_dict(), greedy=True, traffic_file=None)),
-            "finetune_full": {"proposals": ft["proposals"], "moves": ft["moves"], "sweeps": ft["sweeps"],
-                              "finetune_ms": ft["finetune_ms"],
-                              "proposals_per_s": ft["proposals"] / max(t[4] - t[3], 1e-9),
-                              "roofline": roofline(ft, greedy=True, traffic_file=SAMPLED_TRAFFIC_FILE)},
-        }
-    return {"workload": f"directed DCSBM N={g.N} B={w['B']} lognormal(1.0) sizes, power-law out-degree gamma=2.1 "
-                        f"on [1,20000] mean 16, p_in=0.4, seed 0; 50% sample, {algorithm} partition of the sample, "
-                        "extend, finetune_assignment on the full graph",
-            "N": g.N, "E": g.E, "algorithm": algorithm, "planted_mdl": planted_mdl, "runs": runs}
+    ctx.set_seed(0)
+    t = [time.perf_counter()]
+    sg, mapping = dg.sample(sampler, 0.5, seed=0)
+    ctx.synchronize()
+    t.append(time.perf_counter())
+    print(f"[bench] sampled/{sampler}: sample N={sg.N} E={sg.E} in {t[1] - t[0]:.2f}s", file=sys.stderr, flush=True)
+    a_s, nb, st_p = sg.partition(prm)
+    t.append(time.perf_counter())
+    print(f"[bench] sampled/{sampler}: sample partitioned, B={nb} in {t[2] - t[1]:.1f}s", file=sys.stderr, flush=True)
+    sbm = native.DeviceBlockmodel.build(sg, nb, a_s)
+    full, _ = sbm.extend(dg, seed=0)
+    ctx.synchronize()
+    t.append(time.perf_counter())
+    ctx.set_seed(0)
+    _, mdl, st_f = full.finetune_assignment(prm, 0)
+    a = full.export()["assignment"]
+    t.append(time.perf_counter())
+    print(f"[bench] sampled/{sampler}: extended and finetuned in {t[3] - t[2]:.1f}s + {t[4] - t[3]:.1f}s, "
+          f"mdl {mdl:.2f}", file=sys.stderr, flush=True)
+    del sbm, full, sg
+    wall = t[4] - t[0]
+    ft = st_f.as_dict()
+    p = st_p.as_dict()
+    sampled_truth = g.truth[mapping >= 0][np.argsort(mapping[mapping >= 0])]
+    return {
+        "workload": f"{w['desc']}; {sampler} 50% sample, {algorithm} partition of the sample, extend, "
+                    "finetune_assignment on the full graph",
+        "sampler": sampler, "sample_vertices": int((mapping >= 0).sum()), "num_blocks": nb, "mdl": mdl,
+        "nmi": nmi(g.truth, a), "nmi_sample": nmi(sampled_truth, a_s),
+        "wall_s": wall, "phase_s": {"sample": t[1] - t[0], "partition_sample": t[2] - t[1],
+                                    "extend": t[3] - t[2], "finetune_full": t[4] - t[3]},
+        "proposals_per_s": (st_p.proposals + st_f.proposals) / wall,
+        "accepted_moves_per_s": (st_p.moves + st_f.moves) / wall,
+        "partition_sample": dict({k: p[k] for k in ("proposals", "moves", "sweeps", "merge_phases", "finetune_ms",
+                                                    "merge_ms", "carry_ms", "carry_load_ms", "build_ms", "mdl_ms")},
+                                 roofline=roofline(p, greedy=True, traffic_file=None)),
+        "finetune_full": {"proposals": ft["proposals"], "moves": ft["moves"], "sweeps": ft["sweeps"],
+                          "finetune_ms": ft["finetune_ms"]},
+    }
+
+
+def c2_leg(runs):
+    """Full agglomerative partitions of the 200,000-vertex graph (asynchronous Gibbs), compared with the oracle's
+    partition committed in tests/golden/partition_200k.npz."""
+    from sbp_amd import native
+    g, w = make_graph("c2")
+    ctx = native.Context(0, seed=0)
+    dg = ctx.upload(g)
+    prm = native.params("async_gibbs", seed=0)
+    a, nb, st = dg.partition(prm)  # warm-up: loads the code objects, grows the buffer pool
+    ctx.synchronize()
+    tot = {k: 0.0 for k in native.Stats().as_dict()}
+    t0 = time.perf_counter()
+    for _ in range(runs):
+        a, nb, st = dg.partition(prm)
+        d = st.as_dict()
+        for k in tot:
+            tot[k] += d[k]
+    wall = time.perf_counter() - t0
+    ctx.close()
+    want = np.load(os.path.join(ROOT, "tests", "golden", "partition_200k.npz"))
+    same = bool(nb == int(want["num_blocks"]) and np.array_equal(a, want["assignment"].astype(np.int32)) and
+                st.mdl == float(want["mdl"]))
+    print(f"[bench] c2: {runs} runs in {wall:.2f}s, B={nb}, identical to the oracle: {same}", file=sys.stderr,
+          flush=True)
+    return {
+        "workload": w["desc"] + "; full SBP: async_gibbs finetune + block_merge, golden-section search",
+        "runs": runs, "seconds_per_run": wall / runs, "graphs_per_s": runs / wall,
+        "accepted_moves_per_s": tot["moves"] / wall, "proposals_per_s": tot["proposals"] / wall,
+        "merge_proposals_per_s": tot["merge_proposals"] / wall,
+        "num_blocks": nb, "mdl": st.mdl, "nmi": nmi(g.truth, a), "identical_to_oracle": same,
+        "phase_ms_per_run": {k: tot[k] / runs for k in ("finetune_ms", "merge_ms", "carry_ms", "carry_load_ms",
+                                                        "build_ms", "mdl_ms", "ft_kernel_ms", "ft_high_kernel_ms")},
+        "roofline": roofline(tot, greedy=True, traffic_file=C2_TRAFFIC_FILE),
+    }
+
+
+def c1_leg(runs):
+    """Full Metropolis-Hastings partitions of the 1,000-vertex graph on the device, and the oracle's full run of the
+    same (sequential by definition) on one host core."""
+    import oracle as O
+    from sbp_amd import native
+    g, w = make_graph("c1")
+    ctx = native.Context(0, seed=0)
+    dg = ctx.upload(g)
+    prm = native.params("metropolis_hastings", seed=0)
+    dg.partition(prm)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    props = 0
+    for _ in range(runs):
+        a, nb, st = dg.partition(prm)
+        props += st.proposals
+    wall = time.perf_counter() - t0
+    ctx.close()
+    og = O.OGraph.from_csr(g)
+    t0 = time.perf_counter()
+    a_o, nb_o, st_o = O.sbp(og, O.params("metropolis_hastings", seed=0, threads=1))
+    cpu = time.perf_counter() - t0
+    same = bool(nb == nb_o and np.array_equal(a, a_o) and st.mdl == st_o.mdl)
+    print(f"[bench] c1: {wall / runs * 1e3:.0f} ms per run on the device, {cpu * 1e3:.0f} ms for the oracle on one "
+          f"core, identical: {same}", file=sys.stderr, flush=True)
+    return {"workload": w["desc"] + "; full SBP with metropolis_hastings finetune",
+            "runs": runs, "ms_per_run": wall / runs * 1e3, "proposals_per_s": props / wall,
+            "cpu_ms_per_run": cpu * 1e3, "cpu_proposals_per_s": st_o.proposals / cpu, "cpu_cores": 1,
+            "num_blocks": nb, "mdl": st.mdl, "nmi": nmi(g.truth, a), "identical_to_oracle": same}
 
 
 def calibrate_copy(mib):
@@ -361,31 +433,31 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_bytes": n, "copies": 3}))
 
 
+LEGS = ("sampled", "c2", "c1", "batched", "served")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=20, help="timed asynchronous-Gibbs sweeps of the headline")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=None, help="vertices of the large graph (default 5,000,000)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batched-graphs", type=int, default=64, help="graphs in the batched leg (0: skip it)")
+    ap.add_argument("--legs", default=",".join(LEGS), help="legs to run besides the headline (comma list, or none)")
+    ap.add_argument("--only", choices=["headline", *LEGS], help="run only that part and print its JSON object")
+    ap.add_argument("--sampler", default="degree_weighted", help="sampler of the sampled leg")
+    ap.add_argument("--c2-runs", type=int, default=2)
+    ap.add_argument("--c1-runs", type=int, default=3)
+    ap.add_argument("--batched-graphs", type=int, default=4096, help="graphs in the batched leg")
+    ap.add_argument("--batched-plain-max", type=int, default=128,
+                    help="graphs compared with the plain path (the batch over all graphs is timed on its own)")
     ap.add_argument("--batched-slots", type=int, default=None, help="default: the autotuned count, else 16")
-    ap.add_argument("--batched-plain-max", type=int, default=None,
-                    help="compare with the plain path on the first K graphs only, and time one batch over all "
-                         "of them (e.g. --batched-graphs 4096 --batched-plain-max 512)")
-    ap.add_argument("--served-requests", type=int, default=32, help="requests in the served leg (0: skip it)")
-    ap.add_argument("--served-rate", type=float, default=32.0, help="Poisson arrival rate, requests/s")
-    ap.add_argument("--served-max-n", type=int, default=200_000, help="upper bound of the requests' N")
+    ap.add_argument("--served-requests", type=int, default=16, help="requests in the served leg")
+    ap.add_argument("--served-rate", type=float, default=2.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--served-max-n", type=int, default=2_000_000, help="upper bound of the requests' N")
     ap.add_argument("--served-slots", type=int, default=None, help="default: 16")
     ap.add_argument("--many-algorithm", default="async_gibbs", help="algorithm of the batched and served legs")
-    ap.add_argument("--only", choices=["batched", "served", "sampled"],
-                    help="run only that leg and print its JSON object (sampled: the 5,000,000-vertex sampled "
-                         "workload, which the default run leaves out for its length)")
-    ap.add_argument("--sampled-algs", default="max_degree,degree_weighted", help="samplers of the sampled leg")
-    ap.add_argument("--sampled-algorithm", default="async_gibbs", help="finetune algorithm of the sampled leg")
-    ap.add_argument("--sampled-n", type=int, default=SAMPLED["N"],
-                    help="vertices of the sampled leg's graph (blocks scale with it)")
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
                     help="only copy MIB MiB device-to-device 3 times (counter calibration) and exit")
     args = ap.parse_args()
@@ -393,14 +465,12 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the path is single-device, "
-              "see the sharding note in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the path is single-device; the "
+              "design notes say how it would shard)", file=sys.stderr)
         sys.exit(2)
-
-    if args.only == "sampled":
-        print(json.dumps({"sampled": sampled_leg(args.sampled_algs.split(","), args.sampled_algorithm,
-                                                 args.sampled_n)}))
-        return
+    legs = set() if args.legs == "none" else set(x for x in args.legs.split(",") if x)
+    if args.only:
+        legs = {args.only} - {"headline"}
 
     from sbp_amd import workloads
     from sbp_amd.autotune import tuned_slots
@@ -408,127 +478,89 @@ def main():
         args.batched_slots = tuned_slots(16)
     if args.served_slots is None:
         # the autotuner measures the batched workload; for Poisson-arriving requests 16 workers measured best
-        # (7.53 graphs/s against 6.76 with 12 at 32 requests/s)
         args.served_slots = 16
-    # the mixed-size graphs are generated in worker processes (forked before the device is touched) while
-    # the single-graph workload is generated and warmed up
+    # the mixed-size graphs are generated in worker processes (forked before the device is touched) while the
+    # large graph is generated and the headline runs
     pool = cf.ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 1), mp_context=mp.get_context("fork"))
-    fut_b = [pool.submit(workloads.make, workloads.batched_spec(i)) for i in range(args.batched_graphs)]
-    fut_s = [pool.submit(workloads.make, workloads.served_spec(j, n_max=args.served_max_n))
-             for j in range(args.served_requests)]
+    nb_graphs = args.batched_graphs if "batched" in legs else 0
+    ns_graphs = args.served_requests if "served" in legs else 0
+    fut_b = [pool.submit(workloads.make, workloads.batched_spec(i)) for i in range(nb_graphs)]
+    fut_s = [pool.submit(workloads.make, workloads.served_spec(j, n_max=args.served_max_n)) for j in range(ns_graphs)]
 
     from sbp_amd import native
-    from sbp_amd.graph import dcsbm
 
-    if args.only:
+    out = {}
+    line = None
+    if args.only in (None, "headline", "sampled"):
+        t0 = time.perf_counter()
+        g, w = make_graph("c3", args.n)
+        print(f"[bench] graph N={g.N} E={g.E} generated in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+              flush=True)
+        ctx = native.Context(0, seed=0)
+        dg = ctx.upload(g)
+        ctx.synchronize()
+        if args.only in (None, "headline"):
+            tot, wall, mdl0, mdl, B = headline_leg(ctx, dg, g, w, args.steps, args.warmup)
+            cpu = None
+            if not args.no_cpu:
+                threads = min(os.cpu_count() or 1, 16)
+                rate, mrate, sweeps, dt = cpu_baseline(g, w["B"], args.cpu_seconds, threads)
+                cpu = {"value": mrate, "unit": "MCMC moves/s (accepted)", "cores": threads, "kind": "port",
+                       "proposals_per_s": rate,
+                       "sample": f"{sweeps} oracle asynchronous-Gibbs sweeps ({sweeps * g.N} proposals) with the MDL "
+                                 f"after each, over the same graph from its planted partition, {dt:.1f}s on "
+                                 f"{threads} OpenMP threads"}
+                print(f"[bench] cpu baseline: {mrate:.0f} moves/s, {rate:.0f} proposals/s on {threads} threads",
+                      file=sys.stderr, flush=True)
+            line = {
+                "metric": "MCMC moves/s, batched graphs/s & p99 latency vs plain GPU SBP at equal NMI",
+                "value": tot["moves"] / wall,
+                "unit": "MCMC moves/s (accepted vertex moves per second of wall time)",
+                "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "int32 counts / fp64 delta-MDL", "data": "synthetic",
+                "config": {"workload": w["desc"] + "; one step = one asynchronous-Gibbs sweep over every vertex "
+                                                   "(proposals, commit, rebuild, MDL) from the planted partition",
+                           "N": g.N, "E": g.E, "planted_blocks": w["B"], "algorithm": "async_gibbs", "batches": 1},
+                "proposals_per_s": tot["proposals"] / wall,
+                "moves_per_sweep": tot["moves"] / args.steps,
+                "mdl_start": mdl0, "mdl_end": mdl, "num_blocks": B,
+                "phase_ms_per_step": {k: tot[k] / args.steps for k in
+                                      ("finetune_ms", "build_ms", "mdl_ms", "ft_kernel_ms", "ft_high_kernel_ms")},
+                "roofline": roofline(tot, greedy=True, traffic_file=TRAFFIC_FILE),
+                "cpu_baseline": cpu,
+            }
+        if "sampled" in legs:
+            out["sampled"] = sampled_leg(ctx, dg, g, w, args.sampler)
+        del dg
+        ctx.close()
+        del g
+    if "c2" in legs:
+        out["c2"] = c2_leg(args.c2_runs)
+    if "c1" in legs:
+        out["c1"] = c1_leg(args.c1_runs)
+    if "batched" in legs:
         many_b = []
         for i, f in enumerate(fut_b):
             many_b.append(f.result())
-            if (i + 1) % 256 == 0:
+            if (i + 1) % 512 == 0:
                 print(f"[bench] generated {i + 1} batched graphs", file=sys.stderr, flush=True)
-        many_s = []
-        for j, f in enumerate(fut_s):
-            many_s.append(f.result())
-            print(f"[bench] generated request {j}: N={many_s[-1].N} E={many_s[-1].E}", file=sys.stderr, flush=True)
-        pool.shutdown()
-        if args.only == "batched":
-            ctx = native.Context(0, seed=0)
-            out = batched_leg(ctx, many_b, args.many_algorithm, args.batched_slots, args.batched_plain_max)
-            out.pop("_outs", None)
-        else:
-            arrivals = workloads.poisson_arrivals(args.served_rate, args.served_requests, seed=0)
-            out = served_leg(many_s, arrivals, args.many_algorithm, args.served_slots, args.served_rate,
-                             args.served_max_n)
-        print(json.dumps({args.only: out}))
-        return
-
-    w = WORKLOADS[args.workload]
-    t0 = time.perf_counter()
-    g = dcsbm(w["N"], w["B"], gamma=w["gamma"], kmin=w["kmin"], kmax=w["kmax"], mean_degree=w["mean_degree"],
-              p_in=w["p_in"], sigma=w["sigma"], seed=0)
-    gen_s = time.perf_counter() - t0
-    print(f"[bench] graph N={g.N} E={g.E} generated in {gen_s:.1f}s", file=sys.stderr, flush=True)
-    ctx = native.Context(0, seed=0)
-    dg = ctx.upload(g)
-    prm = native.params(w["algorithm"], seed=0)
-
-    for i in range(args.warmup):
-        a, nb, st = dg.partition(prm)
-        print(f"[bench] warmup {i}: B={nb} {st.total_ms / 1e3:.2f}s", file=sys.stderr, flush=True)
-
-    # the timed steps and the CPU baseline run with the generator processes finished
-    many_b = [f.result() for f in fut_b]
-    many_s = [f.result() for f in fut_s]
+        ctx = native.Context(0, seed=0)
+        plain_max = min(args.batched_plain_max, len(many_b))
+        out["batched"] = batched_leg(ctx, many_b, args.many_algorithm, args.batched_slots, plain_max)
+        out["batched"].pop("_outs", None)
+        ctx.close()
+        del many_b
+    if "served" in legs:
+        many_s = [f.result() for f in fut_s]
+        arrivals = workloads.poisson_arrivals(args.served_rate, len(many_s), seed=0)
+        out["served"] = served_leg(many_s, arrivals, args.many_algorithm, args.served_slots, args.served_rate,
+                                   args.served_max_n)
     pool.shutdown()
-    ctx.synchronize()
-    t_start = time.perf_counter()
-    tot = {k: 0.0 for k in native.Stats().as_dict()}
-    results = []
-    for i in range(args.steps):
-        a, nb, st = dg.partition(prm)
-        d = st.as_dict()
-        for k in tot:
-            tot[k] += d[k]
-        results.append((a, nb, st.mdl))
-        print(f"[bench] step {i}: B={nb} mdl={st.mdl:.6f} {st.total_ms / 1e3:.2f}s", file=sys.stderr, flush=True)
-    ctx.synchronize()
-    wall = time.perf_counter() - t_start
-
-    a, nb, mdl = results[-1]
-    value = tot["proposals"] / wall
-
-    gpu_truth_rate = gpu_sweeps_at_truth(ctx, dg, g, w["algorithm"], 10)
-    cpu = None
-    if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        rate, sweeps, dt = cpu_baseline(g, w["algorithm"], args.cpu_seconds, threads)
-        cpu = {"value": rate, "unit": "MCMC proposals/s", "cores": threads, "kind": "port",
-               "sample": f"{sweeps} oracle {w['algorithm']} sweeps ({sweeps * g.N} proposals) over the same "
-                         f"graph at its planted partition, {dt:.1f}s on {threads} OpenMP threads",
-               "gpu_same_sample": gpu_truth_rate}
-
-    batched = served = None
-    if args.batched_graphs:
-        batched = batched_leg(ctx, many_b, args.many_algorithm, args.batched_slots, args.batched_plain_max)
-        batched.pop("_outs", None)
-    if args.served_requests:
-        arrivals = workloads.poisson_arrivals(args.served_rate, args.served_requests, seed=0)
-        served = served_leg(many_s, arrivals, args.many_algorithm, args.served_slots, args.served_rate,
-                            args.served_max_n)
-
-    line = {
-        "metric": "MCMC moves/s, batched graphs/s & p99 latency vs plain GPU SBP at equal NMI",
-        "value": value,
-        "unit": "MCMC proposals/s (whole-run wall time)",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32 counts / fp64 delta-MDL",
-        "data": "synthetic",
-        "config": {"workload": w["desc"], "N": g.N, "E": g.E, "planted_blocks": w["B"],
-                   "algorithm": w["algorithm"], "batches": 1, "merge_rate": 0.5},
-        "accepted_moves_per_s": tot["moves"] / wall,
-        "merge_proposals_per_s": tot["merge_proposals"] / wall,
-        "graphs_per_s": args.steps / wall,
-        "num_blocks": nb,
-        "mdl": mdl,
-        "nmi": nmi(g.truth, a),
-        "phase_ms_per_step": {k: tot[k] / args.steps for k in
-                              ("finetune_ms", "merge_ms", "carry_ms", "build_ms", "mdl_ms", "ft_kernel_ms",
-                               "ft_high_kernel_ms", "mh_kernel_ms")},
-        "sweeps_per_step": tot["sweeps"] / args.steps,
-        "carry_per_step": {k: tot[k] / args.steps for k in
-                           ("carry_load_ms", "carry_evaluations", "carry_speculative", "carry_reused",
-                            "carry_requeues")},
-        "roofline": roofline(tot, greedy=True),
-        "cpu_baseline": cpu,
-        "batched": batched,
-        "served": served,
-    }
+    if args.only and args.only != "headline":
+        print(json.dumps({args.only: out[args.only]}))
+        return
+    line.update(out)
     print(json.dumps(line))
 
 
