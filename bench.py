@@ -422,14 +422,19 @@ def c1_leg(runs):
 
 
 def calibrate_copy(mib):
-    """A plain device-to-device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
-    import torch
+    """A plain device-to-device copy of known size (the HIP runtime's copy kernel), for scaling the FETCH_SIZE /
+    WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
     n = mib * (1 << 20)
-    src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
-    dst = torch.empty_like(src)
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(dst), C.c_size_t(n)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(n)) == 0
     for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
+        assert hip.hipMemcpy(dst, src, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
     print(json.dumps({"copy_bytes": n, "copies": 3}))
 
 
