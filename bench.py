@@ -108,7 +108,7 @@ def cpu_baseline(g, B, seconds, threads):
 
 KERNEL_NAMES = {  # rocprofv3 names of the finetune proposal launches (greedy / non-greedy, parametric entropy)
     "ft": ("k_ft_propose<false, true, false>", "k_ft_propose<false, false, false>"),
-    "ft_high": ("k_ft_propose<true, true, false>", "k_ft_propose<true, false, false>"),
+    "ft_high": ("k_ft_hub<true, false>", "k_ft_hub<false, false>"),  # one workgroup per high-degree vertex
     "mh": ("k_mh<true, false>", "k_mh<false, false>"),  # the sequential walk (windowed)
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ft_propose_c3_traffic.txt")

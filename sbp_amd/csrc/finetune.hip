@@ -170,6 +170,45 @@ __device__ __forceinline__ Table make_table(const FtArgs &a, bool big, int *lds,
     return table_at(lds, LDS_CAP_LOG2, LDS_USED);
 }
 
+// The threads that evaluate one proposal together: one wavefront (WaveGroup), or a whole workgroup
+// (BlockGroup, the high-degree launch).  Loops over edges and neighbour blocks stride by size(); sum() returns
+// the group's total to every thread.  The fixed-point sums make the result independent of the decomposition.
+struct WaveGroup {
+    __device__ __forceinline__ int rank() const { return lane_id(); }
+    __device__ __forceinline__ int size() const { return WAVE; }
+    __device__ __forceinline__ void sync() const { wave_sync(); }
+    __device__ __forceinline__ Fx sum(Fx x) const { return wave_sum_fx(x); }
+    __device__ __forceinline__ long long sum(long long x) const { return wave_sum(x); }
+};
+
+struct BlockGroup {
+    Fx *fx;         // LDS, one per wavefront of the workgroup
+    long long *ll;  // LDS, one per wavefront of the workgroup
+    __device__ __forceinline__ int rank() const { return (int)threadIdx.x; }
+    __device__ __forceinline__ int size() const { return (int)blockDim.x; }
+    __device__ __forceinline__ void sync() const { __syncthreads(); }
+    __device__ Fx sum(Fx x) const {
+        x = wave_sum_fx(x);
+        const int nw = (int)(blockDim.x / WAVE);
+        if (lane_id() == 0) fx[threadIdx.x / WAVE] = x;
+        __syncthreads();
+        Fx t = fx[0];
+        for (int w = 1; w < nw; ++w) fx_add(t, fx[w]);
+        __syncthreads();
+        return t;
+    }
+    __device__ long long sum(long long x) const {
+        x = wave_sum(x);
+        const int nw = (int)(blockDim.x / WAVE);
+        if (lane_id() == 0) ll[threadIdx.x / WAVE] = x;
+        __syncthreads();
+        long long t = 0;
+        for (int w = 0; w < nw; ++w) t += ll[w];
+        __syncthreads();
+        return t;
+    }
+};
+
 // Proposal of vertex v against the current state.  Returns s (== r for "no move").  Fills m.
 __device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w) {
     const int lane = lane_id();
@@ -284,14 +323,14 @@ __device__ __forceinline__ unsigned long long proposal_bytes(const FtArgs &a, co
 
 // Hastings correction over the neighbour blocks (src/entropy.cpp:497-561) and the accept test
 // (src/finetune.cpp:22-32); out of line so the greedy path keeps its registers.
+template <class G>
 __device__ __noinline__ bool hastings_accept(const FtArgs &a, const MoveCtx &m, const Table &t, const U4 &w,
-                                             double dS, int nb, long long mrr, long long mss) {
-    const int lane = lane_id();
+                                             double dS, int nb, long long mrr, long long mss, const G &grp) {
     const int r = m.r, s = m.s;
     Fx fwd = fx_zero(), bwd = fx_zero();
     const long long Bn = a.B;
     const bool self_only_r = m.selfc > 0 && m.out_r == 0 && m.in_r == 0;
-    for (int i = lane; i < nb + 1; i += WAVE) {
+    for (int i = grp.rank(); i < nb + 1; i += grp.size()) {
         int x, cntx, oc, ic;
         if (i < nb) {
             const int slot = t.used[i];
@@ -324,8 +363,8 @@ __device__ __noinline__ bool hastings_accept(const FtArgs &a, const MoveCtx &m, 
         fx_acc(fwd, counts * pw / bd);
         fx_acc(bwd, counts * bw / pd);
     }
-    fwd = wave_sum_fx(fwd);
-    bwd = wave_sum_fx(bwd);
+    fwd = grp.sum(fwd);
+    bwd = grp.sum(bwd);
     double H = (m.k_out + m.k_in == 0) ? 1.0 : fx_to(bwd) / fx_to(fwd);
     double u = unit_double(w.z, w.w);
     double p = exp(-3.0 * dS) * H;
@@ -354,13 +393,15 @@ __device__ __forceinline__ void cell_change(Fx &acc, long long M, long long d, d
 // edges e0 + lane + 64 j four at a time and issues their adjacency and assignment loads before the table
 // updates, so the loads of a high-degree vertex overlap instead of costing two round trips per 64 edges.
 // The lanes' updates race as before; the table's counts do not depend on their order.
+template <class G>
 __device__ __forceinline__ void add_neighbour_blocks(const FtArgs &a, const Table &t, const int32_t *adj, int64_t e0,
-                                                     int64_t e1, int v, int dout, int din) {
+                                                     int64_t e1, int v, int dout, int din, const G &grp) {
     constexpr int U = 4;
-    for (int64_t e = e0 + lane_id(); e < e1; e += U * WAVE) {
+    const int n = grp.size();
+    for (int64_t e = e0 + grp.rank(); e < e1; e += U * n) {
         int u[U], b[U];
 #pragma unroll
-        for (int j = 0; j < U; ++j) u[j] = e + j * WAVE < e1 ? adj[e + j * WAVE] : v;
+        for (int j = 0; j < U; ++j) u[j] = e + j * n < e1 ? adj[e + j * n] : v;
 #pragma unroll
         for (int j = 0; j < U; ++j) b[j] = u[j] != v ? a.assign[u[j]] : -1;
 #pragma unroll
@@ -369,13 +410,12 @@ __device__ __forceinline__ void add_neighbour_blocks(const FtArgs &a, const Tabl
     }
 }
 
-template <bool GREEDY>
-__device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept) {
-    const int lane = lane_id();
+template <bool GREEDY, class G>
+__device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept, const G &grp) {
     const int v = m.v, r = m.r, s = m.s;
-    add_neighbour_blocks(a, t, a.out_adj, m.o0, m.o1, v, 1, 0);
-    add_neighbour_blocks(a, t, a.in_adj, m.i0, m.i1, v, 0, 1);
-    wave_sync();
+    add_neighbour_blocks(a, t, a.out_adj, m.o0, m.o1, v, 1, 0, grp);
+    add_neighbour_blocks(a, t, a.in_adj, m.i0, m.i1, v, 0, 1, grp);
+    grp.sync();
     table_get(t, r, m.out_r, m.in_r);
     table_get(t, s, m.out_s, m.in_s);
     long long mrr, mrs, msr, mss;
@@ -389,7 +429,7 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
     Fx acc = fx_zero();
     long long sub_rrow = 0, sub_srow = 0, sub_rcol = 0, sub_scol = 0;
     const int nb = *t.n_used;
-    for (int i = lane; i < nb; i += WAVE) {
+    for (int i = grp.rank(); i < nb; i += grp.size()) {
         const int slot = t.used[i];
         const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
         const bool special = (c == r) || (c == s);
@@ -419,7 +459,7 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
             sub_scol += mcs;
         }
     }
-    if (lane == 0) {
+    if (grp.rank() == 0) {
         // Delta entries not produced by an out-neighbour block
         if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0))
             cell_change(acc, mrr, -(long long)m.selfc - m.in_r, li_r, lo_r, li_r2, lo_r2);
@@ -433,11 +473,11 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
         if (msr && delta_row_s(m, r, m.out_r) == 0) cell_change(acc, msr, 0, li_r, lo_s, li_r2, lo_s2);
         if (mss && delta_row_s(m, s, m.out_s) == 0) cell_change(acc, mss, 0, li_s, lo_s, li_s2, lo_s2);
     }
-    acc = wave_sum_fx(acc);
-    sub_rrow = wave_sum(sub_rrow);
-    sub_srow = wave_sum(sub_srow);
-    sub_rcol = wave_sum(sub_rcol);
-    sub_scol = wave_sum(sub_scol);
+    acc = grp.sum(acc);
+    sub_rrow = grp.sum(sub_rrow);
+    sub_srow = grp.sum(sub_srow);
+    sub_rcol = grp.sum(sub_rcol);
+    sub_scol = grp.sum(sub_scol);
     const long long S_rrow = (long long)a.d_out[r] - mrr - mrs - sub_rrow;
     const long long S_srow = (long long)a.d_out[s] - mss - msr - sub_srow;
     const long long S_rcol = (long long)a.d_in[r] - mrr - msr - sub_rcol;
@@ -451,7 +491,7 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
         accept = dS < 0.0;
         return dS;
     }
-    accept = hastings_accept(a, m, t, w, dS, nb, mrr, mss);
+    accept = hastings_accept(a, m, t, w, dS, nb, mrr, mss, grp);
     return dS;
 }
 
@@ -459,13 +499,13 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
 // Delta entries' eterm differences (unchanged cells do not change eterm, so no row or column is scanned),
 // the block terms of r and s and the description-length deltas.  No Hastings correction
 // (src/entropy.cpp:499); accept as finetune::accept (src/finetune.cpp:22-32).
-template <bool GREEDY>
-__device__ double evaluate_move_np(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept) {
-    const int lane = lane_id();
+template <bool GREEDY, class G>
+__device__ double evaluate_move_np(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept,
+                                   const G &grp) {
     const int v = m.v, r = m.r, s = m.s;
-    add_neighbour_blocks(a, t, a.out_adj, m.o0, m.o1, v, 1, 0);
-    add_neighbour_blocks(a, t, a.in_adj, m.i0, m.i1, v, 0, 1);
-    wave_sync();
+    add_neighbour_blocks(a, t, a.out_adj, m.o0, m.o1, v, 1, 0, grp);
+    add_neighbour_blocks(a, t, a.in_adj, m.i0, m.i1, v, 0, 1, grp);
+    grp.sync();
     table_get(t, r, m.out_r, m.in_r);
     table_get(t, s, m.out_s, m.in_s);
     long long mrr, mrs, msr, mss;
@@ -473,7 +513,7 @@ __device__ double evaluate_move_np(const FtArgs &a, MoveCtx &m, const Table &t, 
     get_cell_pair(a, s, r, s, s, msr, mss);
     Fx acc = fx_zero();
     const int nb = *t.n_used;
-    for (int i = lane; i < nb; i += WAVE) {
+    for (int i = grp.rank(); i < nb; i += grp.size()) {
         const int slot = t.used[i];
         const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
         if (oc > 0) {
@@ -491,13 +531,13 @@ __device__ double evaluate_move_np(const FtArgs &a, MoveCtx &m, const Table &t, 
             fx_acc(acc, np_entry(a.np, mcs, (long long)ic));
         }
     }
-    if (lane == 0) {  // Delta entries not produced by an out-neighbour block
+    if (grp.rank() == 0) {  // Delta entries not produced by an out-neighbour block
         if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0)) fx_acc(acc, np_entry(a.np, mrr, -(long long)m.selfc - m.in_r));
         if (m.out_s == 0 && m.in_r > 0) fx_acc(acc, np_entry(a.np, mrs, (long long)m.in_r));
         if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) fx_acc(acc, np_entry(a.np, mss, (long long)m.selfc + m.in_s));
         if (m.out_r == 0 && m.in_s > 0) fx_acc(acc, np_entry(a.np, msr, -(long long)m.in_s));
     }
-    acc = wave_sum_fx(acc);
+    acc = grp.sum(acc);
     const double dS = np_finish(a.np, fx_to(acc), r, s, a.d_out[r], a.d_in[r], a.sizes[r], a.d_out[s], a.d_in[s],
                                 a.sizes[s], m.k_out, m.k_in, 1, m.i1 - m.i0, m.o1 - m.o0);
     if (GREEDY) {
@@ -511,10 +551,11 @@ __device__ double evaluate_move_np(const FtArgs &a, MoveCtx &m, const Table &t, 
     return dS;
 }
 
-template <bool GREEDY, bool NP>
-__device__ __forceinline__ double evaluate(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept) {
-    if constexpr (NP) return evaluate_move_np<GREEDY>(a, m, t, w, accept);
-    else return evaluate_move<GREEDY>(a, m, t, w, accept);
+template <bool GREEDY, bool NP, class G = WaveGroup>
+__device__ __forceinline__ double evaluate(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept,
+                                           const G &grp = G{}) {
+    if constexpr (NP) return evaluate_move_np<GREEDY>(a, m, t, w, accept, grp);
+    else return evaluate_move<GREEDY>(a, m, t, w, accept, grp);
 }
 
 template <bool BIG, bool GREEDY, bool NP>
@@ -558,6 +599,99 @@ __global__ void __launch_bounds__(FT_BLOCK) __attribute__((amdgpu_waves_per_eu(B
         table_clear_used(t);
     }
     if (lane == 0 && bytes) atomicAdd(a.bytes, bytes);
+}
+
+// Vertices of total degree > SMALL_K (queued by the first launch), one workgroup each: HUB_THREADS threads
+// split the vertex's edges and its distinct neighbour blocks, with the neighbour-block table in LDS.  A vertex
+// that may have more distinct neighbour blocks than half the table's slots (min(k, B) > HUB_CAP / 2) uses its
+// workgroup's table in global memory instead.  Wavefront 0 draws the proposal; the delta-MDL, the Hastings
+// correction and the accept test are those of the one-wavefront evaluation, summed in fixed point over the
+// workgroup (src/finetune.cpp:447-462,871-901).
+constexpr int HUB_THREADS = 512;
+constexpr int HUB_CAP_LOG2 = 12;
+constexpr int HUB_CAP = 1 << HUB_CAP_LOG2;
+constexpr int64_t HUB_TABLE_INTS = table_ints(HUB_CAP_LOG2, HUB_CAP / 2);  // 57 KB: two workgroups per CU
+
+template <bool GREEDY, bool NP>
+__global__ void __launch_bounds__(HUB_THREADS) k_ft_hub(FtArgs a) {
+    __shared__ int lds[HUB_TABLE_INTS];
+    __shared__ Fx red_fx[HUB_THREADS / WAVE];
+    __shared__ long long red_ll[HUB_THREADS / WAVE];
+    __shared__ MoveCtx s_m;
+    __shared__ U4 s_w;
+    __shared__ int s_s;
+    BlockGroup grp;
+    grp.fx = red_fx;
+    grp.ll = red_ll;
+    const Table lt = table_at(lds, HUB_CAP_LOG2, HUB_CAP / 2);
+    const Table gt = table_at(a.gtable + (int64_t)blockIdx.x * table_ints(a.gcap_log2), a.gcap_log2);
+    for (int i = threadIdx.x; i < HUB_CAP; i += HUB_THREADS) {
+        lt.keys[i] = EMPTY;
+        lt.outc[i] = 0;
+        lt.inc[i] = 0;
+    }
+    if (threadIdx.x == 0) *lt.n_used = 0;
+    __syncthreads();
+    const int64_t count = *a.big_count;
+    unsigned long long bytes = 0;
+    for (int64_t item = blockIdx.x; item < count; item += gridDim.x) {
+        const int v = a.big_queue[item];
+        if (threadIdx.x < WAVE) {
+            MoveCtx m;
+            U4 w;
+            const int s = propose_block(a, v, m, w);
+            if (threadIdx.x == 0) {
+                s_m = m;
+                s_w = w;
+                s_s = s;
+            }
+        }
+        __syncthreads();
+        MoveCtx m = s_m;
+        const U4 w = s_w;
+        const int s = s_s;
+        if (threadIdx.x == 0 && a.s_out) a.s_out[v] = s;
+        if (s == m.r) {
+            if (threadIdx.x == 0) {
+                a.move_s[v] = -1;
+                if (a.dS_out) a.dS_out[v] = 0.0;
+                bytes += proposal_bytes(a, m, 0, GREEDY);
+            }
+            __syncthreads();  // s_m is written again for the next vertex
+            continue;
+        }
+        const bool in_lds = 2 * (m.k_all < (long long)a.B ? m.k_all : (long long)a.B) <= HUB_CAP;
+        const Table &t = in_lds ? lt : gt;
+        bool accept;
+        const double dS = evaluate<GREEDY, NP>(a, m, t, w, accept, grp);
+        const int nb = *t.n_used;
+        if (threadIdx.x == 0) {
+            a.move_s[v] = accept ? s : -1;
+            if (a.dS_out) a.dS_out[v] = dS;
+            bytes += proposal_bytes(a, m, nb, GREEDY);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nb; i += HUB_THREADS) {
+            const int slot = t.used[i];
+            t.keys[slot] = EMPTY;
+            t.outc[slot] = 0;
+            t.inc[slot] = 0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) *t.n_used = 0;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && bytes) atomicAdd(a.bytes, bytes);
+}
+
+static void launch_hub(int blocks, hipStream_t st, const FtArgs &a, bool greedy, bool np) {
+    if (np) {
+        if (greedy) k_ft_hub<true, true><<<blocks, HUB_THREADS, 0, st>>>(a);
+        else k_ft_hub<false, true><<<blocks, HUB_THREADS, 0, st>>>(a);
+    } else {
+        if (greedy) k_ft_hub<true, false><<<blocks, HUB_THREADS, 0, st>>>(a);
+        else k_ft_hub<false, false><<<blocks, HUB_THREADS, 0, st>>>(a);
+    }
 }
 
 template <bool BIG>
@@ -685,17 +819,21 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     info.high_launched = g->max_degree > SMALL_K;
     if (info.high_launched) {
         a.bytes = counters + 3;
-        // one wave per deferred vertex where the tables fit in kMaxTableBytes; a table holds the distinct
-        // neighbour blocks of one vertex at load factor <= 1/2
-        const int lg = std::max(10, log2_ceil(2 * std::min<int64_t>(g->max_degree, bm->B) + 2));
+        // one workgroup per deferred vertex, up to 1,024 at once; global tables (one per workgroup, each holding
+        // the distinct neighbour blocks of one vertex at load factor <= 1/2, within kMaxTableBytes) only when a
+        // vertex can have more distinct neighbour blocks than the LDS table takes
+        const int64_t kmax = std::min<int64_t>(g->max_degree, bm->B);
+        const int lg = std::max(10, log2_ceil(2 * kmax + 2));
         const size_t tints = (size_t)table_ints(lg);
         constexpr size_t kMaxTableBytes = size_t(1) << 30;
-        int64_t waves = std::min<int64_t>(g->n_high_degree, count);
-        waves = std::max<int64_t>(1, std::min<int64_t>(waves, kMaxTableBytes / (tints * sizeof(int))));
-        const int64_t big_blocks = (waves + FT_WAVES - 1) / FT_WAVES;
-        a.gtable = prepare_tables(ctx, lg, big_blocks * FT_WAVES);
+        int64_t blocks_h = std::min<int64_t>(std::min<int64_t>(g->n_high_degree, count), 1024);
+        a.gtable = nullptr;
+        if (2 * kmax > HUB_CAP) {
+            blocks_h = std::min<int64_t>(blocks_h, (int64_t)(kMaxTableBytes / (tints * sizeof(int))));
+            a.gtable = prepare_tables(ctx, lg, std::max<int64_t>(blocks_h, 1));
+        }
         a.gcap_log2 = lg;
-        launch_ft<true>((int)big_blocks, st, a, greedy != 0, bm->np != 0);
+        launch_hub((int)std::max<int64_t>(blocks_h, 1), st, a, greedy != 0, bm->np != 0);
         SBP_HIP(hipGetLastError());
     }
     SBP_HIP(hipEventRecord(ctx->ev[1], st));

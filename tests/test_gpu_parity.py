@@ -117,6 +117,45 @@ def test_propose_all_bit_exact(ctx, greedy, B):
     assert mismatches == 0
 
 
+def hub_graph(seed=5, N=8000, hubs=(0, 1, 2), degree=3000):
+    """A DCSBM plus a few hub vertices with `degree` out-edges and `degree` in-edges to distinct vertices."""
+    g = dcsbm(N, 8, gamma=2.5, kmin=1, kmax=60, mean_degree=6, p_in=0.7, seed=seed)
+    src = np.repeat(np.arange(g.N), np.diff(g.out_off)).astype(np.int64)
+    dst = g.out_adj.astype(np.int64)
+    rng = np.random.default_rng(seed)
+    s_extra, d_extra = [], []
+    for h in hubs:
+        outs = rng.choice(np.setdiff1d(np.arange(N), [h]), degree, replace=False)
+        ins = rng.choice(np.setdiff1d(np.arange(N), [h]), degree, replace=False)
+        s_extra += [np.full(degree, h), ins]
+        d_extra += [outs, np.full(degree, h)]
+    src = np.concatenate([src] + s_extra)
+    dst = np.concatenate([dst] + d_extra)
+    key = np.unique(src * N + dst)  # unweighted: duplicates dropped
+    return CSRGraph.from_edges(key // N, key % N, N)
+
+
+@pytest.mark.parametrize("greedy", [True, False])
+@pytest.mark.parametrize("B", [1000, 3000])
+def test_propose_all_hubs_bit_exact(ctx, greedy, B):
+    """Vertices of 6,000 edges: at B = 3,000 they can have more distinct neighbour blocks than the high-degree
+    launch's LDS table holds, so they use its global-memory table; at B = 1,000 the LDS table."""
+    g = hub_graph()
+    rng = np.random.default_rng(B)
+    assign = rng.integers(0, B, g.N).astype(np.int32)
+    assign[:B] = np.arange(B)
+    og, ob, dg, db = both(ctx, g, B, assign)
+    for epoch in range(4):
+        ctx.set_seed(77)
+        s, dS, acc = db.propose_all(epoch=epoch, greedy=greedy)
+        for v in list(range(3)) + list(rng.choice(g.N, 300, replace=False)):
+            os_, odS, oacc = ob.ft_propose(77, epoch, int(v), greedy)
+            assert s[v] == os_, (epoch, v)
+            if s[v] != assign[v]:
+                assert dS[v] == odS, (epoch, v, dS[v], odS)
+            assert bool(acc[v]) == bool(oacc), (epoch, v)
+
+
 def test_async_gibbs_sweeps_match_with_refusals(ctx):
     g = small_graph(seed=21, N=800, mean=6)
     B = 400  # blocks of about two vertices: many commits would empty a block
