@@ -35,6 +35,15 @@ class Args {
     int threads = 1;                        // --threads (:151-152); the device path ignores it
     bool no_transpose = false;              // --no_transpose (:153-154); one CSR+CSC layout on the device
     bool undirected = false;                // --undirected (:157-158)
+    // named in the reference's JSON/CSV records only (:21-50; the graph-tool generator's naming):
+    std::string blocksizevar = "low";       // -b/--blocksizevar
+    std::string csv = "./eval/test";        // -c/--csv: results CSV prefix (evaluate::write_results)
+    std::string distribute = "none";        // --distribute (MPI data distribution; single rank here)
+    int numvertices = 1000;                 // -n/--numvertices
+    std::string overlap = "low";            // -o/--overlap
+    int subgraphs = 0;                      // --subgraphs
+    std::string subgraphpartition = "round_robin";  // --subgraphpartition
+    std::string type = "static";            // -t/--type
     uint64_t seed = 0;                      // --seed: replay seed (new; the reference seeds from random_device)
     int device = 0;                         // --device: HIP device ordinal (new)
 

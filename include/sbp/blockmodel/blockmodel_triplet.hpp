@@ -1,0 +1,4 @@
+// The reference's header name blockmodel/blockmodel_triplet.hpp, for callers that keep their includes: everything is declared
+// in sbp.hpp.
+#pragma once
+#include "../sbp.hpp"

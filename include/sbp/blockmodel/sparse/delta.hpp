@@ -1,0 +1,4 @@
+// The reference's header name blockmodel/sparse/delta.hpp, for callers that keep their includes: everything is
+// declared in sbp.hpp.
+#pragma once
+#include "../../sbp.hpp"

@@ -1,0 +1,3 @@
+// The reference's header name sample.hpp, for callers that keep their includes: everything is declared in sbp.hpp.
+#pragma once
+#include "sbp.hpp"

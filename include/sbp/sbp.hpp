@@ -16,10 +16,60 @@
 
 #include <memory>
 #include <string>
+#include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "../sbp_amd.h"
 #include "args.hpp"
+
+// ------------------------------------------------------------------ the reference's shared types and globals
+
+/// include/typedefs.hpp: MapVector (a robin_map there; ids in any order), EdgeWeights (:16-19), Vertex
+/// (:114-118), VertexMove_v3 (:140-147).
+template <typename T>
+using MapVector = std::unordered_map<long, T>;
+struct EdgeWeights {
+    std::vector<long> indices;
+    std::vector<long> values;
+};
+struct Vertex {
+    long id;
+    long out_degree;
+    long in_degree;
+};
+struct VertexMove_v3 {
+    double delta_entropy;
+    bool did_move;
+    Vertex vertex;
+    long proposed_block;
+    EdgeWeights out_edges;
+    EdgeWeights in_edges;
+};
+
+/// include/globals.hpp:17: the arguments every entry point below reads (set them, or pass them to
+/// sbp::stochastic_block_partition, which stores them here).
+extern Args args;
+/// include/mpi_data.hpp: the single rank of this build (no MPI).
+struct MPI_t {
+    int rank = 0;
+    int num_processes = 1;
+};
+extern MPI_t mpi;
+
+/// include/globals.hpp:19-67: phase timers (seconds) and counters, filled by the partition driver, the sampled
+/// flow's callers and finetune::finetune_assignment.
+namespace timers {
+extern double BlockMerge_time, BlockMerge_loop_time, BLOCKMODEL_BUILD_TIME, MCMC_time, MCMC_parallel_time, total_time;
+extern double finetune_time, sample_time, sample_extend_time, sample_finetune_time;
+extern long MCMC_iterations, total_num_islands;
+extern unsigned long MCMC_moves;
+}  // namespace timers
+
+/// src/rng.cpp:34-46: creates the device context for args.device and keys the replay stream with args.seed.
+namespace rng {
+void init_generators();
+}
 
 namespace sbp_amd {
 /// Process-wide device context (one HIP stream), created on first use from the Args' device and seed.
@@ -36,6 +86,10 @@ class Graph {
     /// Graph::load (src/graph.cpp:33-72): <filepath>.tsv (1-indexed "from to" lines) or .mtx, plus
     /// <filepath>_truePartition.tsv when present.
     static Graph load(const Args &args);
+    /// Graph::load() (src/main.cpp:82): Graph::load(args) with the global args.
+    static Graph load();
+    /// Graph::num_islands (src/graph.cpp:293-300): vertices of degree 0.
+    long num_islands() const;
     /// Builds from an edge list in file order (0-indexed).
     static Graph from_edges(long num_vertices, const std::vector<std::pair<long, long>> &edges);
     long num_vertices() const { return N_; }
@@ -64,6 +118,76 @@ class Graph {
     std::shared_ptr<struct sbp_graph_s> parent_;  // a sample's device graph refers to its parent's
 };
 
+/// Changed cells of a move (include/blockmodel/sparse/delta.hpp:22-149): rows current_block / proposed_block, then
+/// columns (routing priority row == current, row == proposed, col == current, col == proposed); zero-net entries
+/// stay.  A Delta made by finetune::blockmodel_delta or block_merge::blockmodel_delta also records its move, which
+/// entropy::delta_mdl / block_merge_delta_mdl / hastings_correction evaluate on the device.
+class Delta {
+  public:
+    Delta() = default;
+    Delta(long current_block, long proposed_block, long buckets = 10);
+    void add(long row, long col, long value);
+    void sub(long row, long col, long value) { add(row, col, -value); }
+    long get(long row, long col) const;
+    std::vector<std::tuple<long, long, long>> entries() const;
+    long current_block() const { return current_; }
+    long proposed_block() const { return proposed_; }
+    long self_edge_weight() const { return self_; }
+    void self_edge_weight(long w) { self_ = w; }
+    long vertex = -1;    // the moved vertex (finetune::blockmodel_delta), or -1
+    bool merge = false;  // a block merge of current_block into proposed_block (block_merge::blockmodel_delta)
+
+  private:
+    long current_ = -1, proposed_ = -1, self_ = 0;
+    MapVector<long> cur_row_, prop_row_, cur_col_, prop_col_;
+};
+
+namespace utils {
+/// include/utils.hpp:25-30
+typedef struct proposal_and_edge_counts_t {
+    long proposal;
+    long num_out_neighbor_edges;
+    long num_in_neighbor_edges;
+    long num_neighbor_edges;
+} ProposalAndEdgeCounts;
+/// src/utils.cpp:258-292: the run's JSON record appended to <args.json>/<args.output_file>, with the
+/// reference's keys.
+void write_json(const std::vector<long> &block_assignment, double description_length, unsigned long MCMC_moves,
+                unsigned long MCMC_iterations, double runtime);
+}  // namespace utils
+
+/// A host snapshot of a blockmodel's block matrix (the read side of ISparseMatrix,
+/// include/blockmodel/sparse/csparse_matrix.hpp:43-120), taken when Blockmodel::blockmatrix() is called.  It does
+/// not write back: change the blockmodel through its methods (the mutators throw std::logic_error).
+class ISparseMatrix {
+  public:
+    ISparseMatrix(long num_blocks, std::vector<int32_t> row_ptr, std::vector<int32_t> cols, std::vector<int32_t> cnts);
+    long get(long row, long col) const;
+    std::vector<long> getrow(long row) const;
+    std::vector<long> getcol(long col) const;
+    MapVector<long> getrow_sparse(long row) const;
+    MapVector<long> getcol_sparse(long col) const;
+    EdgeWeights outgoing_edges(long block) const;
+    EdgeWeights incoming_edges(long block) const;
+    /// row(block) + col(block), the diagonal once (dict_transpose_matrix.cpp:176-188)
+    MapVector<long> neighbors_weights(long block) const;
+    std::vector<std::tuple<long, long, long>> entries() const;
+    long distinct_edges(long block) const;
+    long edges() const;
+    long trace() const;
+    std::vector<long> shape() const { return {B_, B_}; }
+    void add(long, long, long) { read_only(); }
+    void sub(long, long, long) { read_only(); }
+    void clearrow(long) { read_only(); }
+    void clearcol(long) { read_only(); }
+
+  private:
+    [[noreturn]] static void read_only();
+    long B_;
+    std::vector<int32_t> row_ptr_, cols_, cnts_;      // CSR, ascending columns
+    std::vector<int32_t> col_ptr_, rows_, col_cnts_;  // CSC, ascending rows
+};
+
 /// Device-resident blockmodel (include/blockmodel/blockmodel.hpp:46-219).  Copy-assignment shares the
 /// device state, as the reference's copy-assignment shares its block matrix; copy() deep-copies.
 class Blockmodel {
@@ -90,6 +214,28 @@ class Blockmodel {
     bool validate(const Graph &graph) const;
     /// log_posterior_probability (src/blockmodel/blockmodel.cpp:288-306)
     double log_posterior_probability(const Graph &graph) const;
+    /// initialize_edge_counts (src/blockmodel/blockmodel.cpp:193-276): recount from the assignment.
+    void initialize_edge_counts(const Graph &graph);
+    /// move_vertex(const VertexMove_v3&) (src/blockmodel/blockmodel.cpp:437-494): false (no change) when the
+    /// vertex's block has one vertex.
+    bool move_vertex(const VertexMove_v3 &move);
+    /// move_vertex(Vertex, const Delta&, ProposalAndEdgeCounts&) (src/blockmodel/blockmodel.cpp:408-435): the
+    /// blockmodel of the new assignment (not for degree-corrected blockmodels, whose histograms this form keys by
+    /// the proposal's edge counts).
+    void move_vertex(Vertex vertex, const Delta &delta, utils::ProposalAndEdgeCounts &proposal);
+    /// merge_block (src/blockmodel/blockmodel.cpp:337-363): every vertex of merge_from moves to merge_to.
+    void merge_block(long merge_from, long merge_to, const Delta &delta, utils::ProposalAndEdgeCounts proposal);
+    /// carry_out_best_merges (src/blockmodel/blockmodel.cpp:55-122), the approximate carry-out, of
+    /// getNum_blocks_to_merge() merges, with label compaction; the block matrix is recounted as merge_blocks'
+    /// initialize_edge_counts would.
+    void carry_out_best_merges(const std::vector<double> &delta_entropy_for_each_block,
+                               const std::vector<long> &best_merge_for_each_block);
+    /// blockmatrix() (include/blockmodel/blockmodel.hpp:150): a snapshot of the block matrix (see ISparseMatrix).
+    std::shared_ptr<ISparseMatrix> blockmatrix() const;
+    /// Graph the blockmodel was built on (needed to recount after a move); set by the constructors and the
+    /// driver.  The graph must outlive the blockmodel's use of it.
+    const Graph *graph() const { return graph_; }
+    void set_graph(const Graph *g) { graph_ = g; }
     sbp_bm handle() const { return bm_.get(); }
     sbp_ctx ctx() const { return ctx_; }
     /// Takes ownership of a device blockmodel handle.
@@ -100,6 +246,7 @@ class Blockmodel {
   private:
     std::shared_ptr<struct sbp_bm_s> bm_;
     sbp_ctx ctx_ = nullptr;
+    const Graph *graph_ = nullptr;
     long N_ = 0;
     double overall_entropy_ = 1.7976931348623157e308;
     long num_blocks_to_merge_ = 0;
@@ -127,11 +274,19 @@ Blockmodel stochastic_block_partition(Graph &graph, Args &args, bool divide_and_
 bool done_blockmodeling(Blockmodel &blockmodel, BlockmodelTriplet &blockmodel_triplet, long min_num_blocks = 0);
 /// Statistics of the last stochastic_block_partition call on this thread (timers::* equivalents).
 const sbp_stats &last_stats();
+/// Block count of the last stochastic_block_partition call's result (the final row of the results CSV).
+long last_num_blocks();
 }  // namespace sbp
 
 namespace block_merge {
 /// src/block_merge.cpp:178-228.  Consumes one epoch of the replay counter.
 Blockmodel &merge_blocks(Blockmodel &blockmodel, const Graph &graph, long num_edges);
+/// src/block_merge.cpp:14-38: the Delta of merging current_block into proposed_block.
+Delta blockmodel_delta(long current_block, long proposed_block, const Blockmodel &blockmodel);
+/// src/block_merge.cpp:40-117: the heap carry-out with re-evaluation, of getNum_blocks_to_merge() merges, label
+/// compaction and recount.
+void carry_out_best_merges_advanced(Blockmodel &blockmodel, const std::vector<double> &delta_entropy_for_each_block,
+                                    const std::vector<long> &best_merge_for_each_block, const Graph &graph);
 }  // namespace block_merge
 
 namespace finetune {
@@ -141,11 +296,25 @@ Blockmodel &asynchronous_gibbs(Blockmodel &blockmodel, const Graph &graph, bool 
 Blockmodel &metropolis_hastings(Blockmodel &blockmodel, const Graph &graph, bool golden_ratio_not_reached);
 Blockmodel &hybrid_mcmc(Blockmodel &blockmodel, const Graph &graph, bool golden_ratio_not_reached);
 Blockmodel &finetune_assignment(Blockmodel &blockmodel, Graph &graph);
+/// src/finetune.cpp:419-430: a vertex's neighbours (weights 1); ignore_self drops the vertex itself.
+EdgeWeights edge_weights(const Graph &graph, long vertex, bool in_edges, bool ignore_self);
+/// src/finetune.cpp:201-235: the Delta of moving `vertex` from current_block to proposed_block.
+Delta blockmodel_delta(long vertex, long current_block, long proposed_block, EdgeWeights &out_edges,
+                       EdgeWeights &in_edges, const Blockmodel &blockmodel);
 }  // namespace finetune
 
 namespace entropy {
 /// src/entropy.cpp:597-604
 double mdl(const Blockmodel &blockmodel, const Graph &graph);
+/// src/entropy.cpp:298-403: the change in description length of the vertex move a Delta from
+/// finetune::blockmodel_delta describes (evaluated on the device; std::invalid_argument for other Deltas).
+double delta_mdl(const Blockmodel &blockmodel, const Delta &delta, const utils::ProposalAndEdgeCounts &proposal);
+/// src/entropy.cpp:129-182: the same for a block merge (a Delta from block_merge::blockmodel_delta).
+double block_merge_delta_mdl(long current_block, utils::ProposalAndEdgeCounts proposal, const Blockmodel &blockmodel,
+                             const Delta &delta);
+/// src/entropy.cpp:497-561: the Hastings correction of that vertex move (1 when greedy or nonparametric).
+double hastings_correction(long vertex, const Graph &graph, const Blockmodel &blockmodel, const Delta &delta,
+                           long current_block, const utils::ProposalAndEdgeCounts &proposal);
 /// src/entropy.cpp:571-585
 double null_mdl_v1(const Graph &graph);
 /// src/entropy.cpp:563-565
@@ -172,6 +341,9 @@ double calculate_nmi(long num_vertices, Hungarian::Matrix &contingency_table);
 Hungarian::Matrix hungarian(const Graph &graph, Blockmodel &blockmodel);
 /// src/evaluate.cpp:98-112: F1, NMI and the description length of the true partition
 Eval evaluate_blockmodel(const Graph &graph, Blockmodel &blockmodel);
+/// src/evaluate.cpp:205-326: appends the run's row to <args.csv><args.numvertices>/<args.type>.csv (header first
+/// when the file is new), with the reference's columns; the run is one profile row (iteration -1, totals).
+void write_results(const Graph &graph, const Eval &eval, double runtime);
 }  // namespace evaluate
 
 namespace sample {

@@ -147,6 +147,15 @@ int sbp_bm_build(sbp_ctx ctx, sbp_graph g, int32_t B, const int32_t *assignment,
 int sbp_bm_copy(sbp_ctx ctx, sbp_bm src, sbp_bm *out);
 int sbp_bm_free(sbp_bm bm);
 int sbp_bm_info(sbp_bm bm, int32_t *num_blocks, int64_t *nnz);
+/* Replaces Blockmodel::initialize_edge_counts (src/blockmodel/blockmodel.cpp:193-276): recounts the block matrix,
+ * degrees, sizes (and degree histograms) from the blockmodel's assignment. */
+int sbp_bm_rebuild(sbp_ctx ctx, sbp_bm bm, sbp_graph g);
+/* Replaces Blockmodel::move_vertex(const VertexMove_v3&) (src/blockmodel/blockmodel.cpp:437-494): moves `vertex` to
+ * `block` unless its block has one vertex (then *moved = 0 and nothing changes). */
+int sbp_bm_move_vertex(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t vertex, int32_t block, int32_t *moved);
+/* Replaces Blockmodel::merge_block (src/blockmodel/blockmodel.cpp:337-363): every vertex of block `from` moves to
+ * `to`; the block count stays (block `from` is left empty). */
+int sbp_bm_merge_block(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t from, int32_t to);
 /* Copies device state to host; any pointer may be NULL.  row_ptr has B+1 entries; cols/cnts nnz entries
  * (row-major, ascending columns).  Degrees are Blockmodel::degrees_out/_in/degrees. */
 int sbp_bm_download(sbp_ctx ctx, sbp_bm bm, int32_t *assignment, int32_t *sizes, int32_t *d_out, int32_t *d_in,
@@ -188,6 +197,13 @@ int sbp_np_description_length(sbp_ctx ctx, sbp_bm bm, sbp_graph g, double *mdl, 
  * s_out = proposed block, dS_out = delta-MDL (0 when s == current block), accept_out = 0/1. */
 int sbp_ft_propose_all(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int32_t greedy, int32_t *s_out,
                        double *dS_out, int32_t *accept_out);
+/* Replaces entropy::delta_mdl and entropy::hastings_correction of given vertex moves (src/entropy.cpp:298-403,
+ * 497-561) with the Delta of finetune::blockmodel_delta (src/finetune.cpp:201-235): for i < n, the change in
+ * description length and the Hastings correction of moving vertices[i] to block targets[i] (0 and 1 when the
+ * target is the vertex's block), against the current blockmodel; the proposal kernels' evaluation.
+ * hastings_out may be NULL (it is 1 for a nonparametric blockmodel, src/entropy.cpp:499). */
+int sbp_ft_evaluate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int64_t n, const int32_t *vertices, const int32_t *targets,
+                    double *dS_out, double *hastings_out);
 /* One sweep of finetune::asynchronous_gibbs (src/finetune.cpp:50-86): proposals against the frozen
  * blockmodel per batch, commit in vertex-id order with refusal of moves that would empty a block
  * (src/blockmodel/blockmodel.cpp:437-494). */
@@ -210,6 +226,17 @@ int sbp_finetune_assignment(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_param
 /* Replaces the proposal loop of block_merge::merge_blocks (src/block_merge.cpp:189-217): 10 proposals
  * per block, best (lowest delta-MDL) per block.  Host outputs of B entries. */
 int sbp_merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *best_s, double *best_dS);
+/* Replaces entropy::block_merge_delta_mdl (src/entropy.cpp:129-182) with the Delta of block_merge::blockmodel_delta
+ * (src/block_merge.cpp:14-38): for i < n, the change in description length of merging blocks[i] into targets[i]
+ * (distinct blocks). */
+int sbp_merge_evaluate(sbp_ctx ctx, sbp_bm bm, int64_t n, const int32_t *blocks, const int32_t *targets,
+                       double *dS_out);
+/* Replaces the carry-out half of block_merge::merge_blocks (src/block_merge.cpp:218-227): carry_out_best_merges_advanced
+ * (:40-117), or Blockmodel::carry_out_best_merges (src/blockmodel/blockmodel.cpp:55-122) when approximate, of given
+ * proposals best_s / best_dS (B entries), then label compaction and the rebuild: a new blockmodel with
+ * B - num_to_merge blocks in *out. */
+int sbp_merge_apply(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const int32_t *best_s, const double *best_dS,
+                    int64_t num_to_merge, int32_t approximate, sbp_bm *out);
 /* Replaces block_merge::merge_blocks (src/block_merge.cpp:178-228): proposals, carry-out
  * (carry_out_best_merges_advanced, or Blockmodel::carry_out_best_merges when approximate), label
  * compaction and rebuild.  Writes a new blockmodel with B - num_to_merge blocks to *out. */

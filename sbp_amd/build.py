@@ -15,8 +15,10 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsbp_amd.so")
 SOURCES = ["csrc/build.hip", "csrc/mdl.hip", "csrc/finetune.hip", "csrc/merge.hip", "csrc/graph.hip", "csrc/capi.hip",
-           "host/sbp_api.cpp", "host/args.cpp", "host/batch.cpp", "host/evaluate.cpp"]
+           "host/sbp_api.cpp", "host/reference_api.cpp", "host/args.cpp", "host/batch.cpp", "host/evaluate.cpp"]
+INCLUDE_SBP = os.path.join(os.path.dirname(HERE), "include", "sbp")  # the reference's header names
 CLI = os.path.join(HERE, "sbp_cli")
+API_CHECK = os.path.join(os.path.dirname(HERE), "tests", "dropin", "api_check")
 CARRY_LIB = os.path.join(HERE, "libsbp_carry.so")  # host-only carry-out, for CPU-side checks
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -47,12 +49,19 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
-    cmd = [HIPCC, *FLAGS, "-o", CLI + ".tmp", os.path.join(HERE, "host", "main.cpp"), "-L" + HERE, "-lsbp_amd",
-           "-Wl,-rpath,$ORIGIN"]
+    cmd = [HIPCC, *FLAGS, "-I" + INCLUDE_SBP, "-o", CLI + ".tmp", os.path.join(HERE, "host", "main.cpp"), "-L" + HERE,
+           "-lsbp_amd", "-Wl,-rpath,$ORIGIN"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"CLI link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(CLI + ".tmp", CLI)
+    # the drop-in API check program (tests/dropin/api_check.cpp), built against the reference's header names
+    cmd = [HIPCC, *FLAGS, "-I" + INCLUDE_SBP, "-o", API_CHECK + ".tmp", API_CHECK + ".cpp", "-L" + HERE, "-lsbp_amd",
+           "-Wl,-rpath,$ORIGIN/../../sbp_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"api_check link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(API_CHECK + ".tmp", API_CHECK)
     cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-ffp-contract=off", "-o", CARRY_LIB + ".tmp",
            os.path.join(HERE, "host", "carry_capi.cpp")]
     r = subprocess.run(cmd, capture_output=True, text=True)

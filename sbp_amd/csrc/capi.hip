@@ -250,6 +250,9 @@ static void finetune_assignment(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_p
 
 // ------------------------------------------------------------------ block merge
 
+static sbp_bm merge_apply(sbp_ctx ctx, sbp_bm bm, sbp_graph g, std::vector<int32_t> &best_s,
+                          std::vector<double> &best_dS, int64_t num_to_merge, bool approximate, sbp_stats *st);
+
 static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int64_t num_to_merge,
                            bool approximate, sbp_stats *st) {
     hipStream_t s = ctx->stream;
@@ -270,6 +273,16 @@ static sbp_bm merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, 
         st->merge_proposals += 10 * (int64_t)B;
         st->merge_phases++;
     }
+    return merge_apply(ctx, bm, g, best_s, best_dS, num_to_merge, approximate, st);
+}
+
+// The carry-out of given proposals (best_s / best_dS per block), label compaction and rebuild: a new blockmodel
+// with B - num_to_merge blocks.
+static sbp_bm merge_apply(sbp_ctx ctx, sbp_bm bm, sbp_graph g, std::vector<int32_t> &best_s,
+                          std::vector<double> &best_dS, int64_t num_to_merge, bool approximate, sbp_stats *st) {
+    hipStream_t s = ctx->stream;
+    const int32_t B = bm->B;
+    int32_t *d_best_s = ctx->scratch[S_BEST].as<int32_t>(3 * (size_t)B + 4);
     double t0 = now_ms();
     std::vector<int32_t> block_map, sizes(B);
     SBP_HIP(hipMemcpy(sizes.data(), bm->sizes, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
@@ -785,6 +798,98 @@ int sbp_merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int64_
     *out = nullptr;
     SBP_API_BEGIN
     *out = merge_blocks(ctx, bm, g, epoch, num_to_merge, approximate != 0, stats);
+    SBP_API_END(ctx)
+}
+
+int sbp_merge_apply(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const int32_t *best_s, const double *best_dS,
+                    int64_t num_to_merge, int32_t approximate, sbp_bm *out) {
+    if (!ctx || !bm || !g || !best_s || !best_dS || !out || num_to_merge < 0 || num_to_merge >= bm->B) return SBP_E_ARG;
+    *out = nullptr;
+    SBP_API_BEGIN
+    for (int32_t b = 0; b < bm->B; ++b)
+        SBP_CHECK(best_s[b] >= 0 && best_s[b] < bm->B, SBP_E_ARG, "sbp_merge_apply: proposed block out of range");
+    std::vector<int32_t> s(best_s, best_s + bm->B);
+    std::vector<double> d(best_dS, best_dS + bm->B);
+    *out = merge_apply(ctx, bm, g, s, d, num_to_merge, approximate != 0, nullptr);
+    SBP_API_END(ctx)
+}
+
+int sbp_ft_evaluate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int64_t n, const int32_t *vertices, const int32_t *targets,
+                    double *dS_out, double *hastings_out) {
+    if (!ctx || !bm || !g || n < 0 || (n > 0 && (!vertices || !targets))) return SBP_E_ARG;
+    SBP_API_BEGIN
+    if (n == 0) return SBP_OK;
+    for (int64_t i = 0; i < n; ++i)
+        SBP_CHECK(vertices[i] >= 0 && vertices[i] < g->N && targets[i] >= 0 && targets[i] < bm->B, SBP_E_ARG,
+                  "sbp_ft_evaluate: vertex or block out of range");
+    int32_t *d_io = ctx->scratch[S_MISC].as<int32_t>(2 * (size_t)n);
+    double *d_out = ctx->scratch[S_PARTIAL].as<double>(2 * (size_t)n);
+    SBP_HIP(hipMemcpyAsync(d_io, vertices, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    SBP_HIP(hipMemcpyAsync(d_io + n, targets, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    ft_evaluate(ctx, bm, g, n, d_io, d_io + n, d_out, d_out + n);
+    if (dS_out) SBP_HIP(hipMemcpyAsync(dS_out, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (hastings_out)
+        SBP_HIP(hipMemcpyAsync(hastings_out, d_out + n, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    SBP_HIP(hipStreamSynchronize(ctx->stream));
+    SBP_API_END(ctx)
+}
+
+int sbp_merge_evaluate(sbp_ctx ctx, sbp_bm bm, int64_t n, const int32_t *blocks, const int32_t *targets,
+                       double *dS_out) {
+    if (!ctx || !bm || n < 0 || (n > 0 && (!blocks || !targets || !dS_out))) return SBP_E_ARG;
+    SBP_API_BEGIN
+    if (n == 0) return SBP_OK;
+    for (int64_t i = 0; i < n; ++i)
+        SBP_CHECK(blocks[i] >= 0 && blocks[i] < bm->B && targets[i] >= 0 && targets[i] < bm->B && blocks[i] != targets[i],
+                  SBP_E_ARG, "sbp_merge_evaluate: blocks out of range or equal");
+    int32_t *d_io = ctx->scratch[S_MISC].as<int32_t>(2 * (size_t)n);
+    double *d_out = ctx->scratch[S_PARTIAL].as<double>((size_t)n);
+    SBP_HIP(hipMemcpyAsync(d_io, blocks, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    SBP_HIP(hipMemcpyAsync(d_io + n, targets, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    merge_evaluate(ctx, bm, n, d_io, d_io + n, d_out);
+    SBP_HIP(hipMemcpyAsync(dS_out, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    SBP_HIP(hipStreamSynchronize(ctx->stream));
+    SBP_API_END(ctx)
+}
+
+int sbp_bm_rebuild(sbp_ctx ctx, sbp_bm bm, sbp_graph g) {
+    if (!ctx || !bm || !g || g->N != bm->N) return SBP_E_ARG;
+    SBP_API_BEGIN
+    bm_rebuild(ctx, g, bm);
+    SBP_HIP(hipStreamSynchronize(ctx->stream));
+    SBP_API_END(ctx)
+}
+
+int sbp_bm_move_vertex(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t vertex, int32_t block, int32_t *moved) {
+    if (!ctx || !bm || !g || g->N != bm->N || vertex < 0 || vertex >= bm->N || block < 0 || block >= bm->B)
+        return SBP_E_ARG;
+    SBP_API_BEGIN
+    int32_t r = 0, size_r = 0;
+    SBP_HIP(hipStreamSynchronize(ctx->stream));
+    SBP_HIP(hipMemcpy(&r, bm->assign + vertex, sizeof(int32_t), hipMemcpyDeviceToHost));
+    SBP_HIP(hipMemcpy(&size_r, bm->sizes + r, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (size_r == 1) {  // refused: the move would empty its block (src/blockmodel/blockmodel.cpp:439-443)
+        if (moved) *moved = 0;
+        return SBP_OK;
+    }
+    SBP_HIP(hipMemcpy(bm->assign + vertex, &block, sizeof(int32_t), hipMemcpyHostToDevice));
+    bm_rebuild(ctx, g, bm);  // the counts and degree histograms of the new assignment, as the moves leave them
+    SBP_HIP(hipStreamSynchronize(ctx->stream));
+    if (moved) *moved = 1;
+    SBP_API_END(ctx)
+}
+
+int sbp_bm_merge_block(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t from, int32_t to) {
+    if (!ctx || !bm || !g || g->N != bm->N || from < 0 || from >= bm->B || to < 0 || to >= bm->B) return SBP_E_ARG;
+    SBP_API_BEGIN
+    std::vector<int32_t> map(bm->B);
+    std::iota(map.begin(), map.end(), 0);
+    map[from] = to;
+    int32_t *d_map = ctx->scratch[S_BEST].as<int32_t>((size_t)bm->B + 4);
+    SBP_HIP(hipMemcpyAsync(d_map, map.data(), sizeof(int32_t) * bm->B, hipMemcpyHostToDevice, ctx->stream));
+    relabel_assignment(ctx, bm, d_map, bm->B, bm->assign);
+    bm_rebuild(ctx, g, bm);
+    SBP_HIP(hipStreamSynchronize(ctx->stream));
     SBP_API_END(ctx)
 }
 

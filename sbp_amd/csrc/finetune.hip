@@ -324,8 +324,8 @@ __device__ __forceinline__ unsigned long long proposal_bytes(const FtArgs &a, co
 // Hastings correction over the neighbour blocks (src/entropy.cpp:497-561) and the accept test
 // (src/finetune.cpp:22-32); out of line so the greedy path keeps its registers.
 template <class G>
-__device__ __noinline__ bool hastings_accept(const FtArgs &a, const MoveCtx &m, const Table &t, const U4 &w,
-                                             double dS, int nb, long long mrr, long long mss, const G &grp) {
+__device__ __noinline__ double hastings_value(const FtArgs &a, const MoveCtx &m, const Table &t, int nb, long long mrr,
+                                              long long mss, const G &grp) {
     const int r = m.r, s = m.s;
     Fx fwd = fx_zero(), bwd = fx_zero();
     const long long Bn = a.B;
@@ -365,7 +365,13 @@ __device__ __noinline__ bool hastings_accept(const FtArgs &a, const MoveCtx &m, 
     }
     fwd = grp.sum(fwd);
     bwd = grp.sum(bwd);
-    double H = (m.k_out + m.k_in == 0) ? 1.0 : fx_to(bwd) / fx_to(fwd);
+    return (m.k_out + m.k_in == 0) ? 1.0 : fx_to(bwd) / fx_to(fwd);
+}
+
+template <class G>
+__device__ __forceinline__ bool hastings_accept(const FtArgs &a, const MoveCtx &m, const Table &t, const U4 &w,
+                                                double dS, int nb, long long mrr, long long mss, const G &grp) {
+    const double H = hastings_value(a, m, t, nb, mrr, mss, grp);
     double u = unit_double(w.z, w.w);
     double p = exp(-3.0 * dS) * H;
     p = p >= 1.0 ? 1.0 : p;
@@ -612,8 +618,17 @@ constexpr int HUB_CAP_LOG2 = 12;
 constexpr int HUB_CAP = 1 << HUB_CAP_LOG2;
 constexpr int64_t HUB_TABLE_INTS = table_ints(HUB_CAP_LOG2, HUB_CAP / 2);  // 57 KB: two workgroups per CU
 
-template <bool GREEDY, bool NP>
-__global__ void __launch_bounds__(HUB_THREADS) k_ft_hub(FtArgs a) {
+// Given moves (vertex, target block) to evaluate instead of proposals to draw: entropy::delta_mdl and
+// hastings_correction of each (src/entropy.cpp:298-403,497-561), as the proposal kernels compute them.
+struct EvalIO {
+    const int32_t *vertices, *targets;
+    int64_t n;
+    double *dS;  // [n]
+    double *H;   // [n] (1 for the nonparametric description length, src/entropy.cpp:499)
+};
+
+template <bool GREEDY, bool NP, bool EVAL = false>
+__global__ void __launch_bounds__(HUB_THREADS) k_ft_hub(FtArgs a, EvalIO io = EvalIO{}) {
     __shared__ int lds[HUB_TABLE_INTS];
     __shared__ Fx red_fx[HUB_THREADS / WAVE];
     __shared__ long long red_ll[HUB_THREADS / WAVE];
@@ -632,14 +647,15 @@ __global__ void __launch_bounds__(HUB_THREADS) k_ft_hub(FtArgs a) {
     }
     if (threadIdx.x == 0) *lt.n_used = 0;
     __syncthreads();
-    const int64_t count = *a.big_count;
+    const int64_t count = EVAL ? io.n : *a.big_count;
     unsigned long long bytes = 0;
     for (int64_t item = blockIdx.x; item < count; item += gridDim.x) {
-        const int v = a.big_queue[item];
+        const int v = EVAL ? io.vertices[item] : a.big_queue[item];
         if (threadIdx.x < WAVE) {
             MoveCtx m;
             U4 w;
-            const int s = propose_block(a, v, m, w);
+            int s = propose_block(a, v, m, w);
+            if (EVAL) s = m.s = io.targets[item];
             if (threadIdx.x == 0) {
                 s_m = m;
                 s_w = w;
@@ -650,12 +666,17 @@ __global__ void __launch_bounds__(HUB_THREADS) k_ft_hub(FtArgs a) {
         MoveCtx m = s_m;
         const U4 w = s_w;
         const int s = s_s;
-        if (threadIdx.x == 0 && a.s_out) a.s_out[v] = s;
+        if (!EVAL && threadIdx.x == 0 && a.s_out) a.s_out[v] = s;
         if (s == m.r) {
             if (threadIdx.x == 0) {
-                a.move_s[v] = -1;
-                if (a.dS_out) a.dS_out[v] = 0.0;
-                bytes += proposal_bytes(a, m, 0, GREEDY);
+                if (EVAL) {
+                    io.dS[item] = 0.0;
+                    io.H[item] = 1.0;
+                } else {
+                    a.move_s[v] = -1;
+                    if (a.dS_out) a.dS_out[v] = 0.0;
+                    bytes += proposal_bytes(a, m, 0, GREEDY);
+                }
             }
             __syncthreads();  // s_m is written again for the next vertex
             continue;
@@ -663,9 +684,20 @@ __global__ void __launch_bounds__(HUB_THREADS) k_ft_hub(FtArgs a) {
         const bool in_lds = 2 * (m.k_all < (long long)a.B ? m.k_all : (long long)a.B) <= HUB_CAP;
         const Table &t = in_lds ? lt : gt;
         bool accept;
-        const double dS = evaluate<GREEDY, NP>(a, m, t, w, accept, grp);
+        const double dS = evaluate<EVAL ? true : GREEDY, NP>(a, m, t, w, accept, grp);
         const int nb = *t.n_used;
-        if (threadIdx.x == 0) {
+        if (EVAL) {
+            double H = 1.0;
+            if (!NP) {
+                long long mrr, mss;
+                get_cell_pair(a, m.r, m.r, m.s, m.s, mrr, mss);
+                H = hastings_value(a, m, t, nb, mrr, mss, grp);
+            }
+            if (threadIdx.x == 0) {
+                io.dS[item] = dS;
+                io.H[item] = H;
+            }
+        } else if (threadIdx.x == 0) {
             a.move_s[v] = accept ? s : -1;
             if (a.dS_out) a.dS_out[v] = dS;
             bytes += proposal_bytes(a, m, nb, GREEDY);
@@ -845,6 +877,37 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     info.kernel_ms = (float)elapsed_ms(ctx, ctx->ev[0], ctx->ev[6]);
     if (info.high_launched) info.high_kernel_ms = (float)elapsed_ms(ctx, ctx->ev[6], ctx->ev[1]);
     return info;
+}
+
+void ft_evaluate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int64_t n, const int32_t *d_vertices, const int32_t *d_targets,
+                 double *d_dS, double *d_H) {
+    if (n == 0) return;
+    hipStream_t st = ctx->stream;
+    auto *counters = ctx->scratch[S_COUNTER].as<unsigned long long>(4);
+    SBP_HIP(hipMemsetAsync(counters, 0, 4 * sizeof(unsigned long long), st));
+    FtArgs a = base_args(ctx, bm, g, 0, 1);
+    bm_ensure_hash(ctx, bm);
+    a.m.hash = bm->cell_hash;
+    a.m.hash_shift = 64 - bm->hash_log2;
+    a.m.hash_mask = (1u << bm->hash_log2) - 1;
+    a.bytes = counters;
+    a.err = reinterpret_cast<int *>(counters + 1);
+    const int64_t kmax = std::min<int64_t>(g->max_degree, bm->B);
+    const int lg = std::max(10, log2_ceil(2 * kmax + 2));
+    int64_t blocks = std::min<int64_t>(n, 1024);
+    a.gtable = nullptr;
+    if (2 * kmax > HUB_CAP) {
+        blocks = std::min<int64_t>(blocks, (int64_t)((size_t(1) << 30) / ((size_t)table_ints(lg) * sizeof(int))));
+        a.gtable = prepare_tables(ctx, lg, std::max<int64_t>(blocks, 1));
+    }
+    a.gcap_log2 = lg;
+    const EvalIO io{d_vertices, d_targets, n, d_dS, d_H};
+    if (bm->np) k_ft_hub<true, true, true><<<(int)blocks, HUB_THREADS, 0, st>>>(a, io);
+    else k_ft_hub<true, false, true><<<(int)blocks, HUB_THREADS, 0, st>>>(a, io);
+    SBP_HIP(hipGetLastError());
+    unsigned long long hb[2] = {0, 0};
+    read_small(ctx, {{hb, counters, sizeof(hb)}});
+    SBP_CHECK(hb[1] == 0, SBP_E_STATE, "delta-MDL evaluation: block degrees inconsistent with the block matrix");
 }
 
 // ------------------------------------------------------------------ async-Gibbs commit

@@ -292,6 +292,9 @@ struct FtLaunchInfo {
 FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int greedy, const int32_t *d_vertices,
                         int64_t count, int32_t *d_move_s, double *d_dS, int32_t *d_accept);
 int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_t *refused_out);
+// delta-MDL and Hastings correction of moving d_vertices[i] to d_targets[i] (device arrays of n), against bm.
+void ft_evaluate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int64_t n, const int32_t *d_vertices, const int32_t *d_targets,
+                 double *d_dS, double *d_H);
 void shuffle_keys(sbp_ctx ctx, const int32_t *d_items, int64_t n, uint32_t epoch, int32_t *d_out);
 // graph.hip
 void graph_finish(sbp_ctx ctx, sbp_graph g);
@@ -302,6 +305,8 @@ sbp_graph sample_graph(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, u
 void extend_assignment(sbp_ctx ctx, sbp_graph g, sbp_graph sample, sbp_bm sample_bm, uint64_t seed, int32_t *d_assign);
 // merge.hip
 uint64_t merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *d_best_s, double *d_best_dS);
+// delta-MDL of merging d_blocks[i] into d_targets[i] (device arrays of n), against bm.
+void merge_evaluate(sbp_ctx ctx, sbp_bm bm, int64_t n, const int32_t *d_blocks, const int32_t *d_targets, double *d_dS);
 void relabel_assignment(sbp_ctx ctx, sbp_bm bm, const int32_t *d_block_map, int32_t new_B, int32_t *d_out);
 void debug_logf(sbp_ctx ctx, uint32_t first, uint32_t count, float *d_out);
 

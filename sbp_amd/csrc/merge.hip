@@ -9,6 +9,8 @@
 // past_proposals does.  Terms are summed in 128-bit fixed point.
 #include <float.h>
 
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "internal.hpp"
 #include "np_entropy.hpp"
@@ -210,10 +212,7 @@ __global__ void __launch_bounds__(MG_BLOCK) __attribute__((amdgpu_waves_per_eu(N
     if (lane == 0 && bytes) atomicAdd(a.bytes, bytes);
 }
 
-uint64_t merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *d_best_s, double *d_best_dS) {
-    hipStream_t st = ctx->stream;
-    auto *bytes = ctx->scratch[S_COUNTER].as<unsigned long long>(4);
-    SBP_HIP(hipMemsetAsync(bytes, 0, 4 * sizeof(unsigned long long), st));
+static MgArgs mg_args(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, unsigned long long *bytes) {
     MgArgs a;
     a.err = reinterpret_cast<int *>(bytes + 1);
     a.B = bm->B;
@@ -225,8 +224,8 @@ uint64_t merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *d_best_s
     a.col_ptr = bm->col_ptr;
     a.row_cells = bm->row_cells;
     a.col_cells = bm->col_cells;
-    a.best_s = d_best_s;
-    a.best_dS = d_best_dS;
+    a.best_s = nullptr;
+    a.best_dS = nullptr;
     a.bytes = bytes;
     a.sizes = bm->sizes;
     bm_ensure_hash(ctx, bm);
@@ -235,6 +234,42 @@ uint64_t merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *d_best_s
     a.hash_mask = (1u << bm->hash_log2) - 1;
     a.np = NpConst{};
     if (bm->np) a.np = np_const(ctx, bm);
+    return a;
+}
+
+// entropy::block_merge_delta_mdl of given merges (block r into block s), one wavefront each, with the device
+// function the proposals use.
+template <bool NP>
+__global__ void __launch_bounds__(MG_BLOCK) k_merge_eval(MgArgs a, const int32_t *__restrict__ blocks,
+                                                         const int32_t *__restrict__ targets, int64_t n,
+                                                         double *__restrict__ dS_out) {
+    const int64_t gwave = (int64_t)blockIdx.x * MG_WAVES + threadIdx.x / WAVE;
+    const int64_t nwaves = (int64_t)gridDim.x * MG_WAVES;
+    for (int64_t i = gwave; i < n; i += nwaves) {
+        const double dS = NP ? merge_delta_mdl_np(a, blocks[i], targets[i]) : merge_delta_mdl(a, blocks[i], targets[i]);
+        if (lane_id() == 0) dS_out[i] = dS;
+    }
+}
+
+void merge_evaluate(sbp_ctx ctx, sbp_bm bm, int64_t n, const int32_t *d_blocks, const int32_t *d_targets,
+                    double *d_dS) {
+    if (n == 0) return;
+    auto *bytes = ctx->scratch[S_COUNTER].as<unsigned long long>(4);
+    SBP_HIP(hipMemsetAsync(bytes, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    MgArgs a = mg_args(ctx, bm, 0, bytes);
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + MG_WAVES - 1) / MG_WAVES, 8192));
+    if (bm->np) k_merge_eval<true><<<blocks, MG_BLOCK, 0, ctx->stream>>>(a, d_blocks, d_targets, n, d_dS);
+    else k_merge_eval<false><<<blocks, MG_BLOCK, 0, ctx->stream>>>(a, d_blocks, d_targets, n, d_dS);
+    SBP_HIP(hipGetLastError());
+}
+
+uint64_t merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *d_best_s, double *d_best_dS) {
+    hipStream_t st = ctx->stream;
+    auto *bytes = ctx->scratch[S_COUNTER].as<unsigned long long>(4);
+    SBP_HIP(hipMemsetAsync(bytes, 0, 4 * sizeof(unsigned long long), st));
+    MgArgs a = mg_args(ctx, bm, epoch, bytes);
+    a.best_s = d_best_s;
+    a.best_dS = d_best_dS;
     int64_t blocks = (bm->B + MG_WAVES - 1) / MG_WAVES;
     if (blocks > 8192) blocks = 8192;
     if (blocks < 1) blocks = 1;

@@ -1,5 +1,9 @@
 // Flag parser for the reference's command line (include/args.hpp:61-212 lists the flags and defaults).
+#include <unistd.h>
+
+#include <cstdio>
 #include <cstdlib>
+#include <ctime>
 #include <stdexcept>
 #include <string>
 
@@ -41,14 +45,25 @@ Args::Args(int argc, char **argv) {
         else if (a == "--undirected") undirected = true;
         else if (a == "--seed") seed = std::strtoull(need(i).c_str(), nullptr, 10);
         else if (a == "--device") device = std::stoi(need(i));
-        // flags of modes outside this path (MPI, divisive, reporting) take a value and are accepted
-        else if (a == "-c" || a == "--csv" || a == "-d" || a == "--directory" || a == "-o" || a == "--overlap" ||
-                 a == "-b" || a == "--blocksizevar" || a == "-t" || a == "--type" || a == "-n" ||
-                 a == "--numvertices" || a == "--delimiter" || a == "--subgraphs" || a == "--subgraphpartition" ||
-                 a == "--splitinit")
+        // named in the JSON and CSV records
+        else if (a == "-c" || a == "--csv") csv = need(i);
+        else if (a == "-o" || a == "--overlap") overlap = need(i);
+        else if (a == "-b" || a == "--blocksizevar") blocksizevar = need(i);
+        else if (a == "-t" || a == "--type") type = need(i);
+        else if (a == "-n" || a == "--numvertices") numvertices = std::stoi(need(i));
+        else if (a == "--subgraphs") subgraphs = std::stoi(need(i));
+        else if (a == "--subgraphpartition") subgraphpartition = need(i);
+        else if (a == "--distribute") distribute = need(i);
+        // flags of modes outside this path (MPI, divisive) are accepted
+        else if (a == "-d" || a == "--directory" || a == "--delimiter" || a == "--splitinit")
             (void)need(i);
-        else if (a == "--distribute" || a == "--nonblocking" || a == "--split" || a == "--mix") {
+        else if (a == "--nonblocking" || a == "--split" || a == "--mix") {
         } else throw std::invalid_argument("unknown flag " + a);
+    }
+    if (output_file.empty()) {  // args.hpp:187-191
+        char b[64];
+        snprintf(b, sizeof(b), "%ld_%ld.json", (long)getpid(), (long)time(nullptr));
+        output_file = b;
     }
     if (batches < 1) throw std::invalid_argument("--batches must be >= 1");
     if (threads <= 0) threads = 1;  // threads <= 0 creates no RNG generators in the reference (src/rng.cpp:37-38)

@@ -1,86 +1,102 @@
-// Command-line driver with the reference's flags (src/main.cpp:64-164, single-rank path): load the graph,
-// partition it on the device, write a JSON result and, when a true partition is given, evaluate it
-// (evaluate::evaluate_blockmodel: F1, NMI and the other pair-counting scores).
-#include <math.h>
-#include <stdio.h>
-
-#include <algorithm>
+// Command-line driver: the reference's single-rank flow (src/main.cpp:55-164) written against the drop-in API
+// with the reference's own header names and signatures -- the global `args`, rng::init_generators(),
+// Graph::load(), sbp::stochastic_block_partition, sample::sample / extend / detach / reattach,
+// Blockmodel(num_blocks, graph, 0.5, assignment), finetune::finetune_assignment, utils::write_json and
+// evaluate::write_results.  MPI is not part of this build: the run is rank 0 of 1.
 #include <chrono>
 #include <cmath>
-#include <map>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
-#include "../../include/sbp/sbp.hpp"
+#include "args.hpp"
+#include "blockmodel/blockmodel.hpp"
+#include "entropy.hpp"
+#include "evaluate.hpp"
+#include "finetune.hpp"
+#include "globals.hpp"
+#include "graph.hpp"
+#include "rng.hpp"
+#include "sample.hpp"
+#include "sbp.hpp"
+#include "utils.hpp"
 
-int main(int argc, char **argv) {
+namespace {
+
+struct Partition {
+    Graph graph;
+    Blockmodel blockmodel;
+};
+
+double seconds_since(std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+}
+
+// The JSON record, and with --evaluate the scores and the CSV row.
+void report(Graph &graph, Blockmodel &blockmodel, double runtime) {
+    if (mpi.rank != 0) return;
+    utils::write_json(blockmodel.block_assignment(), blockmodel.getOverall_entropy(), timers::MCMC_moves,
+                      timers::MCMC_iterations, runtime);
+    if (!args.evaluate) return;
+    evaluate::Eval result = evaluate::evaluate_blockmodel(graph, blockmodel);
+    printf("Final F1 score = %g\nCommunity detection runtime = %gs\n", result.f1_score, runtime);
+    if (!std::isfinite(result.nmi)) result.nmi = 0.0;
+    evaluate::write_results(graph, result, runtime);
+}
+
+void partition_graph(Partition &p) {
+    timers::total_num_islands = p.graph.num_islands();
+    p.blockmodel = sbp::stochastic_block_partition(p.graph, args);
+}
+
+}  // namespace
+
+int main(int argc, char *argv[]) {
     try {
-        Args args(argc, argv);
+        args = Args(argc, argv);
         if (args.filepath.empty()) throw std::invalid_argument("-f/--filepath is required");
-        Graph graph = Graph::load(args);
+        rng::init_generators();
+        Graph graph = Graph::load();
         fprintf(stderr, "V: %ld E: %ld\n", graph.num_vertices(), graph.num_edges());
-        sbp_amd::args() = args;
         sample::Sample detached;
-        Graph *work = &graph;
-        if (args.detach) {  // src/main.cpp:86-92: partition the vertices of degree > 1, reattach the rest
+        Partition partition;
+        const auto start = std::chrono::steady_clock::now();
+        if (args.detach) {  // partition the vertices of degree > 1, reattach the others at the end
             detached = sample::detach(graph);
-            work = &detached.graph;
-            fprintf(stderr, "detached num vertices: %ld E: %ld\n", work->num_vertices(), work->num_edges());
+            partition.graph = detached.graph;
+            fprintf(stderr, "detached num vertices: %ld E: %ld\n", partition.graph.num_vertices(),
+                    partition.graph.num_edges());
+        } else {
+            partition.graph = graph;
         }
-        if (args.samplesize <= 0.0) {  // src/main.cpp:93-97
+        if (args.samplesize <= 0.0) {
             fprintf(stderr, "ERROR Sample size of %g is too low. Must be greater than 0.0\n", args.samplesize);
             return -5;
         }
-        auto t0 = std::chrono::steady_clock::now();
-        Blockmodel bm;
-        if (args.samplesize < 1.0) {  // src/main.cpp:98-146: partition a sample, extend, finetune the full graph
-            sample::Sample s = sample::sample(*work);
-            fprintf(stderr, "sample V: %ld E: %ld\n", s.graph.num_vertices(), s.graph.num_edges());
-            Blockmodel sample_bm = sbp::stochastic_block_partition(s.graph, args);
-            std::vector<long> assignment = sample::extend(*work, sample_bm, s);
-            bm = Blockmodel(sample_bm.num_blocks(), *work, 0.5, assignment);
-            bm = finetune::finetune_assignment(bm, *work);
+        if (args.samplesize < 1.0) {  // partition a sample, extend its partition, finetune on the whole graph
+            const auto t_sample = std::chrono::steady_clock::now();
+            sample::Sample s = sample::sample(partition.graph);
+            Partition sample_partition;
+            sample_partition.graph = s.graph;
+            timers::sample_time = seconds_since(t_sample);
+            partition_graph(sample_partition);
+            const auto t_extend = std::chrono::steady_clock::now();
+            std::vector<long> assignment = sample::extend(partition.graph, sample_partition.blockmodel, s);
+            const auto t_finetune = std::chrono::steady_clock::now();
+            partition.blockmodel = Blockmodel(sample_partition.blockmodel.num_blocks(), partition.graph, 0.5,
+                                              assignment);
+            partition.blockmodel = finetune::finetune_assignment(partition.blockmodel, partition.graph);
+            timers::sample_extend_time = std::chrono::duration<double>(t_finetune - t_extend).count();
+            timers::sample_finetune_time = seconds_since(t_finetune);
         } else {
-            bm = sbp::stochastic_block_partition(*work, args);
+            partition_graph(partition);
         }
-        if (args.detach) {  // src/main.cpp:153-156; the written description length is the full graph's
-            bm = sample::reattach(graph, bm, detached);
-            bm.setOverall_entropy(entropy::mdl(bm, graph));
-        }
-        double runtime = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        const sbp_stats &st = sbp::last_stats();
-        std::vector<long> a = bm.block_assignment();
-        // evaluate_partition (src/main.cpp:41-53) with --evaluate, and whenever a true partition with more than
-        // one community was loaded (Graph::load fills a missing one with zeros)
-        const std::vector<long> &truth = graph.assignment();
-        const bool has_truth = !truth.empty() && *std::max_element(truth.begin(), truth.end()) > 0;
-        double score = -1.0, f1 = -1.0;
-        if (!truth.empty() && (args.evaluate || has_truth)) {
-            evaluate::Eval e = evaluate::evaluate_blockmodel(graph, bm);
-            score = e.nmi;
-            f1 = e.f1_score;
-        }
-        std::string out = args.output_file.empty() ? args.json + ".json" : args.output_file;
-        FILE *f = fopen(out.c_str(), "w");
-        if (!f) throw std::runtime_error("cannot write " + out);
-        auto num = [](double x, const char *fmt) {  // JSON has no NaN or infinity
-            char b[64];
-            if (!std::isfinite(x)) return std::string("null");
-            snprintf(b, sizeof(b), fmt, x);
-            return std::string(b);
-        };
-        fprintf(f, "{\"num_blocks\": %ld, \"mdl\": %s, \"normalized_mdl_v1\": %s, \"nmi\": %s, \"f1_score\": %s, "
-                   "\"runtime\": %.6f, \"mcmc_proposals\": %ld, \"mcmc_moves\": %ld, \"sweeps\": %ld, "
-                   "\"merge_proposals\": %ld, \"assignment\": [",
-                bm.num_blocks(), num(bm.getOverall_entropy(), "%.17g").c_str(),
-                num(entropy::normalize_mdl_v1(bm.getOverall_entropy(), graph), "%.17g").c_str(),
-                num(score, "%.6f").c_str(), num(f1, "%.6f").c_str(), runtime, (long)st.proposals, (long)st.moves, (long)st.sweeps, (long)st.merge_proposals);
-        for (size_t i = 0; i < a.size(); ++i) fprintf(f, "%s%ld", i ? ", " : "", a[i]);
-        fprintf(f, "]}\n");
-        fclose(f);
-        printf("blocks=%ld mdl=%.6f nmi=%.4f runtime=%.3fs\n", bm.num_blocks(), bm.getOverall_entropy(), score,
-               runtime);
+        if (args.detach) partition.blockmodel = sample::reattach(graph, partition.blockmodel, detached);
+        const double runtime = seconds_since(start);
+        report(graph, partition.blockmodel, runtime);
+        printf("blocks=%ld mdl=%.6f runtime=%.3fs\n", partition.blockmodel.num_blocks(),
+               partition.blockmodel.getOverall_entropy(), runtime);
         return 0;
     } catch (const std::exception &e) {
         fprintf(stderr, "ERROR %s\n", e.what());

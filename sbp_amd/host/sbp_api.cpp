@@ -42,10 +42,7 @@ sbp_ctx context(const Args &a) {
     return g_ctx;
 }
 
-Args &args() {
-    static Args a;
-    return a;
-}
+Args &args() { return ::args; }
 
 uint32_t &epoch() {
     static uint32_t e = 0;
@@ -246,6 +243,7 @@ Blockmodel::Blockmodel(long num_blocks, const Graph &graph, double rate, std::ve
     throw_on(sbp_bm_set_model(ctx_, bm, graph.device(ctx_), args.nonparametric ? (args.degreecorrected ? 2 : 1) : 0),
              ctx_, "sbp_bm_set_model");
     bm_ = wrap(bm);
+    graph_ = &graph;
     num_blocks_to_merge_ = (long)((double)num_blocks * rate);  // blockmodel.hpp:70-76
     empty = false;
 }
@@ -266,7 +264,9 @@ Blockmodel Blockmodel::copy() const {
     if (empty) return Blockmodel();
     sbp_bm out = nullptr;
     throw_on(sbp_bm_copy(ctx_, bm_.get(), &out), ctx_, "sbp_bm_copy");
-    return adopt(ctx_, out, N_, overall_entropy_, 0);  // copy() resets num_blocks_to_merge (blockmodel.cpp:152)
+    Blockmodel b = adopt(ctx_, out, N_, overall_entropy_, 0);  // copy() resets num_blocks_to_merge (blockmodel.cpp:152)
+    b.graph_ = graph_;
+    return b;
 }
 
 long Blockmodel::num_blocks() const {
@@ -498,6 +498,7 @@ Blockmodel &merge_blocks(Blockmodel &blockmodel, const Graph &graph, long num_ed
     (void)num_edges;
     sbp_ctx ctx = blockmodel.ctx();
     blockmodel = merge_step(ctx, graph.device(ctx), blockmodel, to_params(sbp_amd::args()), sbp_amd::epoch(), &g_stats);
+    blockmodel.set_graph(&graph);
     return blockmodel;
 }
 
@@ -528,10 +529,13 @@ Blockmodel &mcmc(int iteration, const Graph &graph, Blockmodel &bm, BlockmodelTr
 Blockmodel &finetune_assignment(Blockmodel &bm, Graph &graph) {
     sbp_params p = to_params(sbp_amd::args());
     double mdl = 0;
+    auto t0 = std::chrono::steady_clock::now();
     throw_on(sbp_finetune_assignment(bm.ctx(), bm.handle(), graph.device(bm.ctx()), &p, &sbp_amd::epoch(), &mdl,
                                      &g_stats),
              bm.ctx(), "sbp_finetune_assignment");
+    timers::finetune_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     bm.setOverall_entropy(mdl);
+    bm.set_graph(&graph);
     return bm;
 }
 
@@ -622,15 +626,29 @@ bool done_blockmodeling(Blockmodel &blockmodel, BlockmodelTriplet &triplet, long
 
 const sbp_stats &last_stats() { return g_stats; }
 
+static long g_last_num_blocks = 0;
+long last_num_blocks() { return g_last_num_blocks; }
+
 Blockmodel stochastic_block_partition(Graph &graph, Args &a, bool divide_and_conquer) {
     (void)divide_and_conquer;  // divide-and-conquer is an MPI mode, not part of this path
-    sbp_amd::args() = a;
+    if (&a != &::args) sbp_amd::args() = a;
     sbp_amd::epoch() = 0;
     memset(&g_stats, 0, sizeof(g_stats));
     auto t0 = std::chrono::steady_clock::now();
     sbp_ctx ctx = context(a);
     Blockmodel result = drive(ctx, graph.device(ctx), (int32_t)graph.num_vertices(), to_params(a), &g_stats);
     g_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    result.set_graph(&graph);
+    g_last_num_blocks = result.num_blocks();
+    // the reference's phase timers and counters (include/globals.hpp:19-67), in seconds
+    timers::MCMC_moves += (unsigned long)g_stats.moves;
+    timers::MCMC_iterations += g_stats.sweeps;
+    timers::MCMC_time += g_stats.finetune_ms / 1e3;
+    timers::MCMC_parallel_time += g_stats.ft_kernel_ms / 1e3;
+    timers::BlockMerge_time += (g_stats.merge_ms + g_stats.carry_ms) / 1e3;
+    timers::BlockMerge_loop_time += g_stats.merge_ms / 1e3;
+    timers::BLOCKMODEL_BUILD_TIME += g_stats.build_ms / 1e3;
+    timers::total_time += g_stats.total_ms / 1e3;
     return result;
 }
 

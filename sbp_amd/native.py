@@ -19,6 +19,8 @@ EXPORTS = [
     "sbp_graph_upload", "sbp_graph_free", "sbp_graph_info", "sbp_graph_download",
     "sbp_sample", "sbp_extend", "sbp_detach", "sbp_reattach",
     "sbp_bm_build", "sbp_bm_copy", "sbp_bm_free", "sbp_bm_info", "sbp_bm_download", "sbp_bm_validate",
+    "sbp_bm_rebuild", "sbp_bm_move_vertex", "sbp_bm_merge_block", "sbp_ft_evaluate", "sbp_merge_evaluate",
+    "sbp_merge_apply",
     "sbp_mdl", "sbp_mdl_noalloc", "sbp_bm_set_model", "sbp_np_description_length",
     "sbp_ft_propose_all", "sbp_async_gibbs_sweep", "sbp_mh_sweep", "sbp_mcmc", "sbp_finetune_assignment",
     "sbp_merge_propose", "sbp_merge_blocks",
@@ -112,6 +114,12 @@ def lib():
             "sbp_bm_info": [vp, vp, vp],
             "sbp_bm_download": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
             "sbp_bm_validate": [vp, vp, vp, vp],
+            "sbp_bm_rebuild": [vp, vp, vp],
+            "sbp_bm_move_vertex": [vp, vp, vp, i32, i32, vp],
+            "sbp_bm_merge_block": [vp, vp, vp, i32, i32],
+            "sbp_ft_evaluate": [vp, vp, vp, i64, vp, vp, vp, vp],
+            "sbp_merge_evaluate": [vp, vp, i64, vp, vp, vp],
+            "sbp_merge_apply": [vp, vp, vp, vp, vp, i64, i32, vp],
             "sbp_mdl": [vp, vp, vp, vp, vp],
             "sbp_mdl_noalloc": [vp, vp, vp, vp, vp],
             "sbp_bm_set_model": [vp, vp, vp, i32],
@@ -523,6 +531,44 @@ class DeviceBlockmodel:
         self.ctx.check(lib().sbp_finetune_assignment(self.ctx.h, self.h, self.g.h, C.byref(prm), C.byref(ep),
                                                      C.byref(mdl), C.byref(st)), "sbp_finetune_assignment")
         return ep.value, mdl.value, st
+
+    def rebuild(self):
+        self.ctx.check(lib().sbp_bm_rebuild(self.ctx.h, self.h, self.g.h), "sbp_bm_rebuild")
+
+    def move_vertex(self, vertex, block):
+        """Blockmodel::move_vertex(const VertexMove_v3&): True when moved, False when refused."""
+        moved = C.c_int32()
+        self.ctx.check(lib().sbp_bm_move_vertex(self.ctx.h, self.h, self.g.h, int(vertex), int(block),
+                                                C.byref(moved)), "sbp_bm_move_vertex")
+        return bool(moved.value)
+
+    def merge_block(self, frm, to):
+        self.ctx.check(lib().sbp_bm_merge_block(self.ctx.h, self.h, self.g.h, int(frm), int(to)), "sbp_bm_merge_block")
+
+    def evaluate_moves(self, vertices, targets):
+        """(delta-MDL, Hastings correction) of moving vertices[i] to targets[i], each against the current state."""
+        v = np.ascontiguousarray(vertices, np.int32)
+        t = np.ascontiguousarray(targets, np.int32)
+        dS, H = np.zeros(len(v)), np.zeros(len(v))
+        self.ctx.check(lib().sbp_ft_evaluate(self.ctx.h, self.h, self.g.h, len(v), _p(v), _p(t), _p(dS), _p(H)),
+                       "sbp_ft_evaluate")
+        return dS, H
+
+    def evaluate_merges(self, blocks, targets):
+        b = np.ascontiguousarray(blocks, np.int32)
+        t = np.ascontiguousarray(targets, np.int32)
+        dS = np.zeros(len(b))
+        self.ctx.check(lib().sbp_merge_evaluate(self.ctx.h, self.h, len(b), _p(b), _p(t), _p(dS)),
+                       "sbp_merge_evaluate")
+        return dS
+
+    def merge_apply(self, best_s, best_dS, num_to_merge, approximate=False):
+        s = np.ascontiguousarray(best_s, np.int32)
+        d = np.ascontiguousarray(best_dS, np.float64)
+        h = C.c_void_p()
+        self.ctx.check(lib().sbp_merge_apply(self.ctx.h, self.h, self.g.h, _p(s), _p(d), int(num_to_merge),
+                                             int(approximate), C.byref(h)), "sbp_merge_apply")
+        return DeviceBlockmodel(self.g, h)
 
     def merge_propose(self, epoch):
         B = self.B
