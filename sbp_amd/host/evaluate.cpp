@@ -2,9 +2,8 @@
 // matching of communities (evaluate::hungarian, src/evaluate.cpp:134-210), pair-counting scores
 // (calculate_f1_score, :9-59) and normalised mutual information (calculate_nmi, :61-96).
 //
-// The reference matches communities with the RoboJackets "hungarian" library (extern/hungarian, a git
-// submodule not present in the reference tree). The matching here is the Hungarian method with shortest
-// augmenting paths (Kuhn 1955; Munkres 1957; Jonker and Volgenant 1987) on the table padded to a square. Only
+// The reference matches communities with the RoboJackets "hungarian" library (extern/hungarian/hungarian.cpp,
+// vendored in its tree). The matching here is this project's own Hungarian method with shortest augmenting paths (Kuhn 1955; Munkres 1957; Jonker and Volgenant 1987) on the table padded to a square. Only
 // the accuracy (the matched diagonal) depends on the matching; every maximum-weight matching gives the same
 // value. F1, Rand index, precision, recall and NMI do not depend on the order of rows or columns.
 #include <math.h>
