@@ -106,60 +106,91 @@ def cpu_baseline(g, B, seconds, threads):
     return props / dt, moves / dt, ep, dt
 
 
-KERNEL_NAMES = {  # rocprofv3 names of the finetune proposal launches (greedy / non-greedy, parametric entropy)
-    "ft": ("k_ft_propose<false, true, false>", "k_ft_propose<false, false, false>"),
-    "ft_high": ("k_ft_hub<true, false>", "k_ft_hub<false, false>"),  # one workgroup per high-degree vertex
-    "mh": ("k_mh<true, false>", "k_mh<false, false>"),  # the sequential walk (windowed)
-}
+CLASS_RANGES = ("k <= 8", "8 < k <= 32", "32 < k <= 128", "128 < k <= 256", "256 < k <= 1024", "k > 1024")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ft_propose_c3_traffic.txt")
 C2_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ft_propose_c2_traffic.txt")
 
 
-def committed_traffic(kernel, path=TRAFFIC_FILE):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE,
-    scaled by a copy-kernel calibration), or None when there is none for this kernel."""
+def class_kernels(shape, greedy=True, B=None):
+    """rocprofv3 names of the finetune proposal launch of each degree class (k = |out(v)| + |in(v)|) under the
+    launch shape (lanes, waves per SIMD) of Context.finetune_shape(): lane-group launches for classes 0-3,
+    one workgroup per vertex for classes 4-5 (the last indexes its table by block id when B <= 6144)."""
+    lanes, occ = shape
+    g = "true" if greedy else "false"
+    names = [f"k_ft_lanes<{lanes[b]}, {g}, false, {occ[b] if greedy else 4}>" for b in range(4)]
+    names.append(f"k_ft_hub<{g}, false, false, {lanes[4]}, 11, false>")
+    dense = "*" if B is None else ("true" if B <= 6144 else "false")  # a whole partition run passes through both
+    names.append(f"k_ft_hub<{g}, false, false, 512, 12, {dense}>")
+    return names
+
+
+def add_stats(tot, d):
+    """Accumulates one Stats.as_dict() into tot (per-class fields are lists)."""
+    for k, v in d.items():
+        if isinstance(v, list):
+            tot[k] = [a + b for a, b in zip(tot.get(k, [0] * len(v)), v)]
+        else:
+            tot[k] = tot.get(k, 0.0) + v
+    return tot
+
+
+def committed_traffic(path):
+    """HBM bytes per launch of each degree class's proposal launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE scaled by a copy-kernel calibration): {class: (kernel, bytes)}, or {} when absent."""
+    out = {}
     try:
         with open(path) as f:
-            lines = f.read().splitlines()
-    except OSError:
-        return None
-    fields = dict(l.split(":", 1) for l in lines if ":" in l and not l.startswith("#"))
-    if fields.get("kernel", "").strip() != kernel:
-        return None
-    try:
-        return float(fields["traffic_bytes_per_launch"].split()[0])
-    except (KeyError, ValueError, IndexError):
-        return None
+            for line in f:
+                if line.startswith("class "):
+                    parts = dict(x.split("=", 1) for x in line.split()[2:] if "=" in x)
+                    out[int(line.split()[1])] = (parts.get("kernel", "").replace("_", " "),
+                                                 float(parts["traffic_bytes_per_launch"]))
+    except (OSError, KeyError, ValueError):
+        return {}
+    return out
 
 
-def roofline(tot, greedy=True, traffic_file=TRAFFIC_FILE):
-    """Roofline of the dominant finetune proposal launch: algorithmic bytes per launch over its average
-    device time (HIP events around that launch alone, on the library's stream).  traffic_file names the
-    committed rocprofv3 summary of that launch on the same workload (None: no measured traffic)."""
+def roofline(tot, shape, greedy=True, traffic_file=TRAFFIC_FILE, B=None):
+    """Roofline of the finetune proposal launches, one entry per degree class: algorithmic bytes per launch over the
+    launch's average device time (HIP events around each class's launch on the library's stream).  The headline
+    figures are those of the dominant class (largest device time); traffic_file names the committed rocprofv3
+    summary of the same launches on the same workload (None: no measured traffic)."""
+    names = class_kernels(shape, greedy, B)
+    traffic = committed_traffic(traffic_file) if traffic_file else {}
     legs = {}
-    for key, prefix in (("ft", "ft"), ("ft_high", "ft_high"), ("mh", "mh")):
-        n = tot[f"{prefix}_launches"]
-        if n:
-            legs[key] = dict(kernel=KERNEL_NAMES[key][0 if greedy else 1], launches=n,
-                             avg_launch_ms=tot[f"{prefix}_kernel_ms"] / n,
-                             algorithmic_bytes_per_launch=tot[f"{prefix}_bytes"] / n,
-                             total_ms=tot[f"{prefix}_kernel_ms"])
+    for b in range(6):
+        n = tot["ft_class_launches"][b]
+        if not n:
+            continue
+        ms = tot["ft_class_ms"][b] / n
+        byts = tot["ft_class_bytes"][b] / n
+        leg = dict(degree_class=CLASS_RANGES[b], kernel=names[b], launches=n, avg_launch_ms=ms,
+                   algorithmic_bytes_per_launch=byts, achieved_GBps=byts / (ms * 1e-3) / 1e9,
+                   total_ms=tot["ft_class_ms"][b])
+        t = traffic.get(b)
+        if t is not None and t[0] == names[b]:
+            leg["traffic_bytes_per_launch"] = t[1]
+            leg["hbm_achieved_GBps"] = t[1] / (ms * 1e-3) / 1e9
+        legs[b] = leg
     if not legs:
         return None
-    for leg in legs.values():
-        leg["achieved_GBps"] = leg["algorithmic_bytes_per_launch"] / (leg["avg_launch_ms"] * 1e-3) / 1e9
     dom = max(legs.values(), key=lambda l: l["total_ms"])
-    traffic = committed_traffic(dom["kernel"], traffic_file) if traffic_file else None
-    # HBM bytes the counters measured per launch of the same kernel on the same workload, over this run's average
-    # launch time: the fraction of the HBM peak the kernel actually moves
-    hbm_gbs = traffic / (dom["avg_launch_ms"] * 1e-3) / 1e9 if traffic is not None else None
-    return {"bound": "hbm", "achieved": dom["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": dom["achieved_GBps"] / HBM_PEAK_GBS, "traffic": traffic,
-            "hbm_achieved": hbm_gbs, "hbm_frac": hbm_gbs / HBM_PEAK_GBS if hbm_gbs is not None else None,
-            "kernel": dom["kernel"], "avg_launch_ms": dom["avg_launch_ms"],
-            "algorithmic_bytes_per_launch": dom["algorithmic_bytes_per_launch"],
-            "traffic_source": os.path.relpath(traffic_file, ROOT) if traffic is not None else None,
-            "launches": {k: {kk: vv for kk, vv in v.items() if kk != "total_ms"} for k, v in legs.items()}}
+    all_ms = sum(l["total_ms"] for l in legs.values())
+    all_bytes = sum(tot["ft_class_bytes"])
+    out = {"bound": "hbm", "achieved": dom["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": dom["achieved_GBps"] / HBM_PEAK_GBS, "traffic": dom.get("traffic_bytes_per_launch"),
+           "hbm_achieved": dom.get("hbm_achieved_GBps"),
+           "hbm_frac": dom["hbm_achieved_GBps"] / HBM_PEAK_GBS if "hbm_achieved_GBps" in dom else None,
+           "kernel": dom["kernel"], "degree_class": dom["degree_class"], "avg_launch_ms": dom["avg_launch_ms"],
+           "algorithmic_bytes_per_launch": dom["algorithmic_bytes_per_launch"],
+           "traffic_source": os.path.relpath(traffic_file, ROOT) if dom.get("traffic_bytes_per_launch") else None,
+           "all_proposal_launches": {"ms": all_ms, "algorithmic_GBps": all_bytes / (all_ms * 1e-3) / 1e9}}
+    if all(("traffic_bytes_per_launch" in l) for l in legs.values()):
+        hb = sum(l["traffic_bytes_per_launch"] * l["launches"] for l in legs.values())
+        out["all_proposal_launches"]["hbm_GBps"] = hb / (all_ms * 1e-3) / 1e9
+        out["all_proposal_launches"]["hbm_frac"] = hb / (all_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+    out["classes"] = {str(b): {k: v for k, v in l.items() if k != "total_ms"} for b, l in legs.items()}
+    return out
 
 
 def batched_leg(ctx, graphs, algorithm, slots, plain_max=None):
@@ -290,15 +321,13 @@ def headline_leg(ctx, dg, g, w, steps, warmup):
         db.mdl()
         ep += 1
     ctx.synchronize()
-    tot = {k: 0.0 for k in native.Stats().as_dict()}
+    tot = {}
     t0 = time.perf_counter()
     for i in range(steps):
         st = db.async_gibbs_sweep(prm, ep)
         mdl = db.mdl()[0]
         ep += 1
-        d = st.as_dict()
-        for k in tot:
-            tot[k] += d[k]
+        add_stats(tot, st.as_dict())
     ctx.synchronize()
     wall = time.perf_counter() - t0
     print(f"[bench] headline: {steps} sweeps in {wall:.2f}s, {tot['moves'] / wall:.0f} moves/s, "
@@ -348,7 +377,7 @@ def sampled_leg(ctx, dg, g, w, sampler, algorithm="async_gibbs"):
         "accepted_moves_per_s": (st_p.moves + st_f.moves) / wall,
         "partition_sample": dict({k: p[k] for k in ("proposals", "moves", "sweeps", "merge_phases", "finetune_ms",
                                                     "merge_ms", "carry_ms", "carry_load_ms", "build_ms", "mdl_ms")},
-                                 roofline=roofline(p, greedy=True, traffic_file=None)),
+                                 roofline=roofline(p, ctx.finetune_shape(), greedy=True, traffic_file=None)),
         "finetune_full": {"proposals": ft["proposals"], "moves": ft["moves"], "sweeps": ft["sweeps"],
                           "finetune_ms": ft["finetune_ms"]},
     }
@@ -364,14 +393,13 @@ def c2_leg(runs):
     prm = native.params("async_gibbs", seed=0)
     a, nb, st = dg.partition(prm)  # warm-up: loads the code objects, grows the buffer pool
     ctx.synchronize()
-    tot = {k: 0.0 for k in native.Stats().as_dict()}
+    tot = {}
     t0 = time.perf_counter()
     for _ in range(runs):
         a, nb, st = dg.partition(prm)
-        d = st.as_dict()
-        for k in tot:
-            tot[k] += d[k]
+        add_stats(tot, st.as_dict())
     wall = time.perf_counter() - t0
+    ctx_shape = ctx.finetune_shape()
     ctx.close()
     want = np.load(os.path.join(ROOT, "tests", "golden", "partition_200k.npz"))
     same = bool(nb == int(want["num_blocks"]) and np.array_equal(a, want["assignment"].astype(np.int32)) and
@@ -386,7 +414,7 @@ def c2_leg(runs):
         "num_blocks": nb, "mdl": st.mdl, "nmi": nmi(g.truth, a), "identical_to_oracle": same,
         "phase_ms_per_run": {k: tot[k] / runs for k in ("finetune_ms", "merge_ms", "carry_ms", "carry_load_ms",
                                                         "build_ms", "mdl_ms", "ft_kernel_ms", "ft_high_kernel_ms")},
-        "roofline": roofline(tot, greedy=True, traffic_file=C2_TRAFFIC_FILE),
+        "roofline": roofline(tot, ctx_shape, greedy=True, traffic_file=C2_TRAFFIC_FILE),
     }
 
 
@@ -532,7 +560,8 @@ def main():
                 "mdl_start": mdl0, "mdl_end": mdl, "num_blocks": B,
                 "phase_ms_per_step": {k: tot[k] / args.steps for k in
                                       ("finetune_ms", "build_ms", "mdl_ms", "ft_kernel_ms", "ft_high_kernel_ms")},
-                "roofline": roofline(tot, greedy=True, traffic_file=TRAFFIC_FILE),
+                "roofline": roofline(tot, ctx.finetune_shape(), greedy=True, traffic_file=TRAFFIC_FILE, B=B),
+                "finetune_shape": {"lanes": ctx.finetune_shape()[0], "waves_per_simd": ctx.finetune_shape()[1]},
                 "cpu_baseline": cpu,
             }
         if "sampled" in legs:

@@ -67,22 +67,28 @@ typedef struct {
     double build_ms;          /* blockmodel (re)builds */
     double mdl_ms;            /* MDL reductions */
     double total_ms;          /* wall time of the call */
-    uint64_t ft_bytes;        /* algorithmic bytes of the finetune proposal launch for vertices of total
-                               * degree <= 256 (neighbour-block histogram in LDS) */
+    uint64_t ft_bytes;        /* algorithmic bytes of the finetune proposal launches on lane groups (vertices of
+                               * total degree <= 256: degree classes 0-3) */
     uint64_t merge_bytes;     /* algorithmic bytes of the merge proposal kernels */
-    int64_t ft_launches;      /* launches of that finetune proposal kernel */
-    double ft_kernel_ms;      /* device time of that finetune proposal kernel */
+    int64_t ft_launches;      /* proposal batches with lane-group launches */
+    double ft_kernel_ms;      /* device time of those lane-group launches */
     int64_t carry_evaluations;   /* merge carry-out: delta-MDL re-evaluations on the sequential path */
     int64_t carry_speculative;   /* merge carry-out: re-evaluations run ahead in parallel */
     int64_t carry_reused;        /* merge carry-out: pops answered by a still-exact earlier evaluation */
     int64_t carry_requeues;      /* merge carry-out: merges pushed back because they got worse */
     double carry_load_ms;        /* merge carry-out: device-to-host copy and hash-map build */
-    uint64_t ft_high_bytes;      /* bytes, launches, device time of the high-degree finetune proposal launch */
+    uint64_t ft_high_bytes;      /* bytes, batches, device time of the workgroup-per-vertex finetune proposal
+                                  * launches (total degree > 256: degree classes 4-5) */
     int64_t ft_high_launches;
     double ft_high_kernel_ms;
     uint64_t mh_bytes;           /* bytes, launches, device time of the Metropolis-Hastings kernel */
     int64_t mh_launches;
     double mh_kernel_ms;
+    /* finetune proposal launches per degree class of k = |out(v)| + |in(v)| (k <= 8, 32, 128, 256, 1,024, more):
+     * algorithmic bytes, launches, device time */
+    uint64_t ft_class_bytes[6];
+    int64_t ft_class_launches[6];
+    double ft_class_ms[6];
 } sbp_stats;
 
 /* ----------------------------------------------------------------- context */
@@ -264,6 +270,16 @@ int sbp_ctx_memory(sbp_ctx ctx, uint64_t *current_bytes, uint64_t *peak_bytes);
 int sbp_ctx_trim(sbp_ctx ctx);
 /* Host threads of the context's merge carry-out (0: SBP_AMD_CARRY_THREADS, else up to 16). */
 int sbp_set_carry_threads(sbp_ctx ctx, int32_t threads);
+/* Launch shape of the finetune proposal launches, per degree class of k = |out(v)| + |in(v)| (b = 0..4 for
+ * k <= 8, 32, 128, 256 and 1,024; the rest take 512-thread workgroups).  lanes[b]: lanes (threads) per vertex,
+ * allowed 8, 16, 32 or 64 for classes 0 and 1, 32 or 64 for class 2, 64 for class 3, 128 or 256 for class 4.
+ * waves_per_simd[b] (b = 0..3): the wavefronts per SIMD that the greedy parametric lane-group launch's register
+ * budget is sized for, 3 or 4.  SBP_E_ARG (nothing changed) otherwise.  With ctx NULL, sets the default of the
+ * contexts created afterwards (lanes 8, 8, 32, 64, 256 at 4 wavefronts per SIMD).  Results do not depend on it.
+ * No counterpart in the reference: a tuning knob of the launch shape, which the autotuner
+ * (sbp_amd/autotune.py) sets. */
+int sbp_set_finetune_shape(sbp_ctx ctx, const int32_t lanes[5], const int32_t waves_per_simd[4]);
+int sbp_get_finetune_shape(sbp_ctx ctx, int32_t lanes[5], int32_t waves_per_simd[4]);
 /* Device bytes of a graph with N vertices and E edges, and an upper bound of the device bytes one partition
  * run of it needs (graph, blockmodels, scratch), given the largest |out(v)| + |in(v)|, the number of vertices
  * where that exceeds 256 and the algorithm (sbp_params.algorithm): the server admits requests by the latter. */

@@ -103,6 +103,12 @@ static void gibbs_batch(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int greedy, uint32_
             st->ft_high_launches += 1;
             st->ft_high_kernel_ms += info.high_kernel_ms;
         }
+        for (int b = 0; b < FT_NB; ++b)
+            if (info.class_launched[b]) {
+                st->ft_class_bytes[b] += info.class_bytes[b];
+                st->ft_class_launches[b] += 1;
+                st->ft_class_ms[b] += info.class_ms[b];
+            }
         st->finetune_ms += (t1 - t0) + (t2 - t1);
     }
 }
@@ -385,6 +391,11 @@ static int32_t *iota_of(sbp_ctx ctx, sbp_graph g) {
 
 extern "C" {
 
+// finetune launch shape that new contexts start with (sbp_set_finetune_shape(NULL, ...))
+static std::mutex g_shape_mu;
+static int32_t g_default_lanes[5] = {8, 8, 32, 64, 256};
+static int32_t g_default_occ[4] = {4, 4, 4, 4};
+
 int sbp_create(sbp_ctx *out, int device, uint64_t seed) {
     if (!out) return SBP_E_ARG;
     *out = nullptr;
@@ -398,8 +409,14 @@ int sbp_create(sbp_ctx *out, int device, uint64_t seed) {
         ctx = new sbp_ctx_s();
         ctx->device = device;
         ctx->seed = seed;
+        {
+            std::lock_guard<std::mutex> lk(g_shape_mu);
+            std::copy(g_default_lanes, g_default_lanes + 5, ctx->ft_lanes);
+            std::copy(g_default_occ, g_default_occ + 4, ctx->ft_occ);
+        }
         SBP_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         for (auto &e : ctx->ev) SBP_HIP(hipEventCreate(&e));
+        for (auto &e : ctx->ft_ev) SBP_HIP(hipEventCreate(&e));
         SBP_HIP(hipHostMalloc(&ctx->pinned, PINNED_BYTES, hipHostMallocDefault));
         *out = ctx;
         return SBP_OK;
@@ -416,6 +433,8 @@ int sbp_destroy(sbp_ctx ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : ctx->ft_ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &s : ctx->scratch) {
         if (s.ptr) (void)hipFree(s.ptr);
@@ -480,8 +499,8 @@ int sbp_graph_upload(sbp_ctx ctx, int32_t N, int64_t E, const int64_t *out_off, 
 
 int sbp_graph_free(sbp_graph g) {
     if (!g) return SBP_OK;
-    void *p[] = {g->out_off, g->in_off,    g->out_adj,    g->in_adj,         g->edge_src,
-                 g->degree,  g->deg_order, g->parent_map, g->sample_vertices};
+    void *p[] = {g->out_off, g->in_off,    g->out_adj,    g->in_adj,          g->edge_src, g->degree,
+                 g->deg_order, g->parent_map, g->sample_vertices, g->ft_blist, g->ft_bmeta};
     for (void *x : p)
         if (x) (void)hipFree(x);
     delete g;
@@ -947,6 +966,30 @@ int sbp_ctx_trim(sbp_ctx ctx) {
 int sbp_set_carry_threads(sbp_ctx ctx, int32_t threads) {
     if (!ctx || threads < 0 || threads > 64) return SBP_E_ARG;
     ctx->carry_threads = threads;
+    return SBP_OK;
+}
+
+int sbp_set_finetune_shape(sbp_ctx ctx, const int32_t lanes[5], const int32_t waves_per_simd[4]) {
+    if (!lanes || !waves_per_simd) return SBP_E_ARG;
+    if (ctx) return set_finetune_shape(ctx, lanes, waves_per_simd) ? SBP_OK : SBP_E_ARG;
+    sbp_ctx_s probe;
+    if (!set_finetune_shape(&probe, lanes, waves_per_simd)) return SBP_E_ARG;
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    std::copy(lanes, lanes + 5, g_default_lanes);
+    std::copy(waves_per_simd, waves_per_simd + 4, g_default_occ);
+    return SBP_OK;
+}
+
+int sbp_get_finetune_shape(sbp_ctx ctx, int32_t lanes[5], int32_t waves_per_simd[4]) {
+    if (!lanes || !waves_per_simd) return SBP_E_ARG;
+    if (ctx) {
+        std::copy(ctx->ft_lanes, ctx->ft_lanes + 5, lanes);
+        std::copy(ctx->ft_occ, ctx->ft_occ + 4, waves_per_simd);
+    } else {
+        std::lock_guard<std::mutex> lk(g_shape_mu);
+        std::copy(g_default_lanes, g_default_lanes + 5, lanes);
+        std::copy(g_default_occ, g_default_occ + 4, waves_per_simd);
+    }
     return SBP_OK;
 }
 

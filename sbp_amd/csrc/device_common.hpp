@@ -245,27 +245,6 @@ __device__ __forceinline__ int cell_hash_get(const CellSlot *table, int shift, u
     }
 }
 
-// First index in [lo, hi) whose inclusive cumulative count exceeds target (hi when none), by a 64-ary search:
-// each round the wave loads 64 evenly spaced counts and keeps the interval the first larger one closes, so
-// a list of up to 4096 cells takes two dependent loads instead of twelve.
-__device__ __forceinline__ int wave_cum_search(const int32_t *cum, int lo, int hi, long long target) {
-    const int lane = lane_id();
-    while (hi - lo > WAVE) {
-        const int len = hi - lo;
-        const int p = lo + (int)(((long long)(lane + 1) * len) / WAVE) - 1;
-        const unsigned long long hit = ballot((long long)cum[p] > target);
-        if (!hit) return hi;
-        const int j = __ffsll((long long)hit) - 1;
-        const int new_hi = __shfl(p, j, WAVE) + 1;
-        const int new_lo = j == 0 ? lo : __shfl(p, j - 1, WAVE) + 1;
-        lo = new_lo;
-        hi = new_hi;
-    }
-    const int i = lo + lane;
-    const unsigned long long hit = ballot(i < hi && (long long)cum[i] > target);
-    return hit ? lo + __ffsll((long long)hit) - 1 : hi;
-}
-
 // Cells are stored as int2 {index, count}, sorted by index within a row/column.
 __device__ __forceinline__ int cell_lookup(const int2 *cells, int begin, int end, int key) {
     int lo = begin, hi = end;

@@ -82,6 +82,55 @@ __device__ __forceinline__ void table_get(const Table &t, int key, int &o, int &
     }
 }
 
+__device__ __forceinline__ int table_size(const Table &t) { return *t.n_used; }
+// The i-th distinct key of the current item with its counts.
+__device__ __forceinline__ void table_entry(const Table &t, int i, int &key, int &o, int &in) {
+    const int slot = t.used[i];
+    key = t.keys[slot];
+    o = t.outc[slot];
+    in = t.inc[slot];
+}
+
+// Histogram indexed by the key itself (keys in [0, capacity)): cnt[key] packs the out count (low 32 bits) and the
+// in count (high 32 bits), so one 64-bit atomic adds both and its old value tells the first insertion.  No
+// probing; the keys touched by the current item are listed in used.  Layout: cnt[capacity] (8-byte aligned),
+// used[used_cap], n_used[1].
+struct DenseTable {
+    unsigned long long *cnt;
+    int *used;
+    int *n_used;
+};
+
+__host__ __device__ constexpr int64_t dense_table_ints(int64_t capacity, int64_t used_cap) {
+    return 2 * capacity + used_cap + 1;
+}
+
+__device__ __forceinline__ DenseTable dense_table_at(int *base, int capacity) {
+    DenseTable t;
+    t.cnt = reinterpret_cast<unsigned long long *>(base);
+    t.used = base + 2 * capacity;
+    t.n_used = t.used + capacity;
+    return t;
+}
+
+__device__ __forceinline__ void table_add(const DenseTable &t, int key, int dout, int din) {
+    const unsigned long long add = (unsigned long long)(unsigned)dout | ((unsigned long long)(unsigned)din << 32);
+    if (atomicAdd(&t.cnt[key], add) == 0ull) t.used[atomicAdd(t.n_used, 1)] = key;
+}
+
+__device__ __forceinline__ void table_get(const DenseTable &t, int key, int &o, int &i) {
+    const unsigned long long v = t.cnt[key];
+    o = (int)(unsigned)(v & 0xffffffffull);
+    i = (int)(unsigned)(v >> 32);
+}
+
+__device__ __forceinline__ int table_size(const DenseTable &t) { return *t.n_used; }
+
+__device__ __forceinline__ void table_entry(const DenseTable &t, int i, int &key, int &o, int &in) {
+    key = t.used[i];
+    table_get(t, key, o, in);
+}
+
 __device__ __forceinline__ void table_clear_all(const Table &t) {
     for (int i = lane_id(); i < t.cap; i += WAVE) {
         t.keys[i] = EMPTY;

@@ -30,6 +30,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "device_common.hpp"
 #include "device_table.hpp"
@@ -52,6 +53,22 @@ constexpr int64_t LDS_TABLE_INTS = table_ints(LDS_CAP_LOG2, LDS_USED);
 constexpr int LDS_CAP = 1 << LDS_CAP_LOG2;  // hash slots per wave
 constexpr int SMALL_K = FT_SMALL_DEGREE;  // max |out| + |in| for the LDS table
 static_assert(2 * SMALL_K <= LDS_CAP, "LDS table load factor above 1/2");
+// Degree classes of the proposal launches, by k = |out(v)| + |in(v)|: classes 0..FT_NB-3 hold k up to
+// ft_bucket_max(b) (and above the previous bound) and run on lane groups; classes FT_NB-2 (k <= 1,024) and FT_NB-1
+// (the rest) run one workgroup per vertex.  Each lane-group table has 2 * ft_bucket_max(b)
+// slots (load factor <= 1/2) and lists at most ft_bucket_max(b) used slots.
+__host__ __device__ constexpr int ft_bucket_max(int b) {
+    return b == 0 ? 8 : b == 1 ? 32 : b == 2 ? 128 : b == 3 ? SMALL_K : 1024;
+}
+__host__ __device__ constexpr int ft_bucket_log2(int b) { return b == 0 ? 4 : b == 1 ? 6 : b == 2 ? 8 : 9; }
+static_assert(FT_NB == 6 && ft_bucket_max(FT_NB - 3) == FT_SMALL_DEGREE, "degree classes");
+// LDS ints per wavefront for the lane-group tables: (WAVE / L) tables of class b must fit, which allows
+// L >= 4 for class 0, L >= 8 for class 1, L >= 32 for class 2 and L = 64 for class 3.
+constexpr int LDS_WAVE_INTS = 1808;
+__host__ __device__ constexpr bool ft_lanes_fit(int b, int L) {
+    return (WAVE / L) * table_ints(ft_bucket_log2(b), ft_bucket_max(b)) <= LDS_WAVE_INTS;
+}
+static_assert(ft_lanes_fit(0, 4) && ft_lanes_fit(1, 8) && ft_lanes_fit(2, 32) && ft_lanes_fit(3, 64), "LDS budget");
 // Read-only view of the block matrix.  Compact CSR: row b = [ptr[b], ptr[b+1]), with per-row inclusive
 // cumulative counts (columns: excluding the diagonal cell) for multinomial selection by binary search.
 // Slack layout (Metropolis-Hastings): row b = [start[b], start[b] + len[b]), no cumulative counts.
@@ -86,8 +103,8 @@ struct FtArgs {
     int *gtable;           // high-degree tables: per wave 4 * 2^gcap_log2 + 1 ints
     int gcap_log2;
     int *err;              // set when a multinomial runs past its cells (inconsistent degrees)
-    int32_t *big_queue;    // high-degree vertices deferred by the LDS-table launch
-    int32_t *big_count;
+    const int32_t *blist;  // the items grouped by degree class (class b: bmeta[b] items from blist + bmeta[FT_NB + b])
+    const int32_t *bmeta;
     NpConst np;            // nonparametric description-length constants (nonparametric runs only)
     HistSlot *hist;        // degree-corrected runs: the histograms, updated in place by Metropolis-Hastings
 };
@@ -170,16 +187,104 @@ __device__ __forceinline__ Table make_table(const FtArgs &a, bool big, int *lds,
     return table_at(lds, LDS_CAP_LOG2, LDS_USED);
 }
 
-// The threads that evaluate one proposal together: one wavefront (WaveGroup), or a whole workgroup
-// (BlockGroup, the high-degree launch).  Loops over edges and neighbour blocks stride by size(); sum() returns
-// the group's total to every thread.  The fixed-point sums make the result independent of the decomposition.
-struct WaveGroup {
-    __device__ __forceinline__ int rank() const { return lane_id(); }
-    __device__ __forceinline__ int size() const { return WAVE; }
+// The threads that evaluate one proposal together: L aligned lanes of a wavefront (LaneGroup<L>; a whole
+// wavefront is LaneGroup<WAVE>), or a whole workgroup (BlockGroup, the high-degree launch).  Loops over edges
+// and neighbour blocks stride by size(); sum() returns the group's total to every thread.  The fixed-point sums
+// make the result independent of the decomposition.  A lane group's lanes take the same branches at every
+// collective (their conditions are uniform over the group), so shuffles read only lanes that are active.
+template <int L>
+struct LaneGroup {
+    static_assert(L >= 2 && L <= WAVE && (L & (L - 1)) == 0, "a lane group is a power of two within a wavefront");
+    __device__ __forceinline__ int rank() const { return lane_id() & (L - 1); }
+    __device__ __forceinline__ int size() const { return L; }
+    __device__ __forceinline__ int first_lane() const { return lane_id() & ~(L - 1); }
     __device__ __forceinline__ void sync() const { wave_sync(); }
-    __device__ __forceinline__ Fx sum(Fx x) const { return wave_sum_fx(x); }
-    __device__ __forceinline__ long long sum(long long x) const { return wave_sum(x); }
+    __device__ __forceinline__ Fx sum(Fx v) const {
+#pragma unroll
+        for (int off = L / 2; off >= 1; off >>= 1) {
+            Fx o;
+            o.lo = __shfl_xor(v.lo, off, WAVE);
+            o.hi = __shfl_xor(v.hi, off, WAVE);
+            fx_add(v, o);
+        }
+        return v;
+    }
+    template <typename T>
+    __device__ __forceinline__ T sum(T x) const {
+#pragma unroll
+        for (int off = L / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off, WAVE);
+        return x;
+    }
+    // bit i set when p holds on the group's lane of rank i
+    __device__ __forceinline__ unsigned long long ballot(bool p) const {
+        const unsigned long long b = __ballot(p);
+        if constexpr (L == WAVE) return b;
+        else return (b >> first_lane()) & ((1ull << L) - 1ull);
+    }
+    // the value held by the group's lane of rank r
+    template <typename T>
+    __device__ __forceinline__ T bcast(T v, int r) const {
+        return __shfl(v, first_lane() + r, WAVE);
+    }
+    template <typename T>
+    __device__ __forceinline__ T inclusive_scan(T v) const {
+        const int k = rank();
+#pragma unroll
+        for (int off = 1; off < L; off <<= 1) {
+            const T o = __shfl(v, lane_id() - (k >= off ? off : 0), WAVE);
+            if (k >= off) v += o;
+        }
+        return v;
+    }
 };
+using WaveGroup = LaneGroup<WAVE>;
+
+// First index in [lo, hi) whose inclusive cumulative count exceeds target (hi when none), by an L-ary search:
+// each round the group loads L evenly spaced counts and keeps the interval the first larger one closes.
+template <class G>
+__device__ __forceinline__ int group_cum_search(const int32_t *cum, int lo, int hi, long long target, const G &g) {
+    const int L = g.size(), k = g.rank();
+    while (hi - lo > L) {
+        const int len = hi - lo;
+        const int p = lo + (int)(((long long)(k + 1) * len) / L) - 1;
+        const unsigned long long hit = g.ballot((long long)cum[p] > target);
+        if (!hit) return hi;
+        const int j = __ffsll((long long)hit) - 1;
+        const int new_hi = g.bcast(p, j) + 1;
+        const int prev = g.bcast(p, j > 0 ? j - 1 : 0) + 1;
+        lo = j == 0 ? lo : prev;
+        hi = new_hi;
+    }
+    const int i = lo + k;
+    const unsigned long long hit = g.ballot(i < hi && (long long)cum[i] > target);
+    return hit ? lo + __ffsll((long long)hit) - 1 : hi;
+}
+
+// Empties a lane group's table: every slot (table_clear_all) or the slots its last item claimed.
+template <class G>
+__device__ __forceinline__ void group_table_clear_all(const Table &t, const G &g) {
+    for (int i = g.rank(); i < t.cap; i += g.size()) {
+        t.keys[i] = EMPTY;
+        t.outc[i] = 0;
+        t.inc[i] = 0;
+    }
+    if (g.rank() == 0) *t.n_used = 0;
+    g.sync();
+}
+template <class G>
+__device__ __forceinline__ void group_table_clear_used(const Table &t, const G &g) {
+    g.sync();
+    const int n = *t.n_used;
+    for (int i = g.rank(); i < n; i += g.size()) {
+        const int slot = t.used[i];
+        t.keys[slot] = EMPTY;
+        t.outc[slot] = 0;
+        t.inc[slot] = 0;
+    }
+    g.sync();
+    if (g.rank() == 0) *t.n_used = 0;
+    g.sync();
+}
 
 struct BlockGroup {
     Fx *fx;         // LDS, one per wavefront of the workgroup
@@ -209,9 +314,11 @@ struct BlockGroup {
     }
 };
 
-// Proposal of vertex v against the current state.  Returns s (== r for "no move").  Fills m.
-__device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w) {
-    const int lane = lane_id();
+// Proposal of vertex v against the current state by the lane group g (uniform v).  Returns s (== r for "no
+// move").  Fills m.
+template <class G = WaveGroup>
+__device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w, const G &g = G{}) {
+    const int k_rank = g.rank(), L = g.size();
     m.v = v;
     m.o0 = a.out_off[v];
     m.o1 = a.out_off[v + 1];
@@ -220,10 +327,10 @@ __device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w) {
     const long long ko = m.o1 - m.o0, kia = m.i1 - m.i0;
     m.r = a.assign[v];
     int self_out = 0, self_in = 0;
-    for (int64_t e = m.o0 + lane; e < m.o1; e += WAVE) self_out += (a.out_adj[e] == v);
-    for (int64_t e = m.i0 + lane; e < m.i1; e += WAVE) self_in += (a.in_adj[e] == v);
-    self_out = wave_sum(self_out);
-    self_in = wave_sum(self_in);
+    for (int64_t e = m.o0 + k_rank; e < m.o1; e += L) self_out += (a.out_adj[e] == v);
+    for (int64_t e = m.i0 + k_rank; e < m.i1; e += L) self_in += (a.in_adj[e] == v);
+    self_out = g.sum(self_out);
+    self_in = g.sum(self_in);
     m.selfc = self_out;
     m.k_out = ko;
     m.k_in = kia - self_in;
@@ -245,16 +352,16 @@ __device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w) {
         } else {
             long long want = (long long)idx - ko;
             u = -1;
-            for (int64_t base = m.i0; base < m.i1 && u < 0; base += WAVE) {
-                int64_t e = base + lane;
+            for (int64_t base = m.i0; base < m.i1 && u < 0; base += L) {
+                int64_t e = base + k_rank;
                 int x = e < m.i1 ? a.in_adj[e] : v;
                 bool ok = (e < m.i1) && (x != v);
-                unsigned long long msk = ballot(ok);
+                unsigned long long msk = g.ballot(ok);
                 int cnt = __popcll(msk);
                 if (want < cnt) {
-                    int before = __popcll(msk & ((1ull << lane) - 1ull));
-                    unsigned long long hit = ballot(ok && before == want);
-                    u = __shfl(x, __ffsll((long long)hit) - 1, WAVE);
+                    int before = __popcll(msk & ((1ull << k_rank) - 1ull));
+                    unsigned long long hit = g.ballot(ok && before == want);
+                    u = g.bcast(x, __ffsll((long long)hit) - 1);
                 } else {
                     want -= cnt;
                 }
@@ -269,34 +376,34 @@ __device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w) {
         if (a.m.row_cum) {
             const long long dout_t = a.d_out[t];
             if (target < dout_t) {
-                int i = wave_cum_search(a.m.row_cum, rp0, rp1, target);
+                int i = group_cum_search(a.m.row_cum, rp0, rp1, target, g);
                 if (i < rp1) s = a.m.row_cells[i].x;
             } else {
-                int i = wave_cum_search(a.m.col_cum, cp0, cp1, target - dout_t);
+                int i = group_cum_search(a.m.col_cum, cp0, cp1, target - dout_t, g);
                 if (i < cp1) s = a.m.col_cells[i].x;
             }
         } else {
             long long base_sum = 0;
-            for (int c0 = rp0; c0 < rp1 && s < 0; c0 += WAVE) {
-                int ci = c0 + lane;
+            for (int c0 = rp0; c0 < rp1 && s < 0; c0 += L) {
+                int ci = c0 + k_rank;
                 int2 cell = ci < rp1 ? a.m.row_cells[ci] : make_int2(0, 0);
-                long long incl = wave_inclusive_scan((long long)cell.y) + base_sum;
-                unsigned long long hit = ballot(ci < rp1 && target < incl);
-                if (hit) s = __shfl(cell.x, __ffsll((long long)hit) - 1, WAVE);
-                else base_sum = __shfl(incl, WAVE - 1, WAVE);
+                long long incl = g.inclusive_scan((long long)cell.y) + base_sum;
+                unsigned long long hit = g.ballot(ci < rp1 && target < incl);
+                if (hit) s = g.bcast(cell.x, __ffsll((long long)hit) - 1);
+                else base_sum = g.bcast(incl, L - 1);
             }
-            for (int c0 = cp0; c0 < cp1 && s < 0; c0 += WAVE) {
-                int ci = c0 + lane;
+            for (int c0 = cp0; c0 < cp1 && s < 0; c0 += L) {
+                int ci = c0 + k_rank;
                 int2 cell = ci < cp1 ? a.m.col_cells[ci] : make_int2(0, 0);
                 bool ok = ci < cp1 && cell.x != t;
-                long long incl = wave_inclusive_scan(ok ? (long long)cell.y : 0ll) + base_sum;
-                unsigned long long hit = ballot(ok && target < incl);
-                if (hit) s = __shfl(cell.x, __ffsll((long long)hit) - 1, WAVE);
-                else base_sum = __shfl(incl, WAVE - 1, WAVE);
+                long long incl = g.inclusive_scan(ok ? (long long)cell.y : 0ll) + base_sum;
+                unsigned long long hit = g.ballot(ok && target < incl);
+                if (hit) s = g.bcast(cell.x, __ffsll((long long)hit) - 1);
+                else base_sum = g.bcast(incl, L - 1);
             }
         }
         if (s < 0) {  // d_tot[t] disagrees with the cells of row/column t: report, propose no move
-            if (lane == 0) atomicExch(a.err, 1);
+            if (k_rank == 0) atomicExch(a.err, 1);
             s = m.r;
         }
     }
@@ -323,8 +430,8 @@ __device__ __forceinline__ unsigned long long proposal_bytes(const FtArgs &a, co
 
 // Hastings correction over the neighbour blocks (src/entropy.cpp:497-561) and the accept test
 // (src/finetune.cpp:22-32); out of line so the greedy path keeps its registers.
-template <class G>
-__device__ __noinline__ double hastings_value(const FtArgs &a, const MoveCtx &m, const Table &t, int nb, long long mrr,
+template <class G, class T>
+__device__ __noinline__ double hastings_value(const FtArgs &a, const MoveCtx &m, const T &t, int nb, long long mrr,
                                               long long mss, const G &grp) {
     const int r = m.r, s = m.s;
     Fx fwd = fx_zero(), bwd = fx_zero();
@@ -333,10 +440,7 @@ __device__ __noinline__ double hastings_value(const FtArgs &a, const MoveCtx &m,
     for (int i = grp.rank(); i < nb + 1; i += grp.size()) {
         int x, cntx, oc, ic;
         if (i < nb) {
-            const int slot = t.used[i];
-            x = t.keys[slot];
-            oc = t.outc[slot];
-            ic = t.inc[slot];
+            table_entry(t, i, x, oc, ic);
             cntx = oc + ic + (x == r ? m.selfc : 0);
         } else {
             if (!self_only_r) continue;  // the self-loop is the only link to block r
@@ -368,8 +472,8 @@ __device__ __noinline__ double hastings_value(const FtArgs &a, const MoveCtx &m,
     return (m.k_out + m.k_in == 0) ? 1.0 : fx_to(bwd) / fx_to(fwd);
 }
 
-template <class G>
-__device__ __forceinline__ bool hastings_accept(const FtArgs &a, const MoveCtx &m, const Table &t, const U4 &w,
+template <class G, class T>
+__device__ __forceinline__ bool hastings_accept(const FtArgs &a, const MoveCtx &m, const T &t, const U4 &w,
                                                 double dS, int nb, long long mrr, long long mss, const G &grp) {
     const double H = hastings_value(a, m, t, nb, mrr, mss, grp);
     double u = unit_double(w.z, w.w);
@@ -399,8 +503,8 @@ __device__ __forceinline__ void cell_change(Fx &acc, long long M, long long d, d
 // edges e0 + lane + 64 j four at a time and issues their adjacency and assignment loads before the table
 // updates, so the loads of a high-degree vertex overlap instead of costing two round trips per 64 edges.
 // The lanes' updates race as before; the table's counts do not depend on their order.
-template <class G>
-__device__ __forceinline__ void add_neighbour_blocks(const FtArgs &a, const Table &t, const int32_t *adj, int64_t e0,
+template <class G, class T>
+__device__ __forceinline__ void add_neighbour_blocks(const FtArgs &a, const T &t, const int32_t *adj, int64_t e0,
                                                      int64_t e1, int v, int dout, int din, const G &grp) {
     constexpr int U = 4;
     const int n = grp.size();
@@ -416,8 +520,8 @@ __device__ __forceinline__ void add_neighbour_blocks(const FtArgs &a, const Tabl
     }
 }
 
-template <bool GREEDY, class G>
-__device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept, const G &grp) {
+template <bool GREEDY, class G, class T>
+__device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const T &t, const U4 &w, bool &accept, const G &grp) {
     const int v = m.v, r = m.r, s = m.s;
     add_neighbour_blocks(a, t, a.out_adj, m.o0, m.o1, v, 1, 0, grp);
     add_neighbour_blocks(a, t, a.in_adj, m.i0, m.i1, v, 0, 1, grp);
@@ -434,10 +538,10 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
     const double li_s = fastlog(a.d_in[s]), li_s2 = fastlog(new_din(a, m, s));
     Fx acc = fx_zero();
     long long sub_rrow = 0, sub_srow = 0, sub_rcol = 0, sub_scol = 0;
-    const int nb = *t.n_used;
+    const int nb = table_size(t);
     for (int i = grp.rank(); i < nb; i += grp.size()) {
-        const int slot = t.used[i];
-        const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
+        int c, oc, ic;
+        table_entry(t, i, c, oc, ic);
         const bool special = (c == r) || (c == s);
         if (oc > 0) {
             long long mrc, msc;
@@ -505,8 +609,8 @@ __device__ double evaluate_move(const FtArgs &a, MoveCtx &m, const Table &t, con
 // Delta entries' eterm differences (unchanged cells do not change eterm, so no row or column is scanned),
 // the block terms of r and s and the description-length deltas.  No Hastings correction
 // (src/entropy.cpp:499); accept as finetune::accept (src/finetune.cpp:22-32).
-template <bool GREEDY, class G>
-__device__ double evaluate_move_np(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept,
+template <bool GREEDY, class G, class T>
+__device__ double evaluate_move_np(const FtArgs &a, MoveCtx &m, const T &t, const U4 &w, bool &accept,
                                    const G &grp) {
     const int v = m.v, r = m.r, s = m.s;
     add_neighbour_blocks(a, t, a.out_adj, m.o0, m.o1, v, 1, 0, grp);
@@ -518,10 +622,10 @@ __device__ double evaluate_move_np(const FtArgs &a, MoveCtx &m, const Table &t, 
     get_cell_pair(a, r, r, r, s, mrr, mrs);
     get_cell_pair(a, s, r, s, s, msr, mss);
     Fx acc = fx_zero();
-    const int nb = *t.n_used;
+    const int nb = table_size(t);
     for (int i = grp.rank(); i < nb; i += grp.size()) {
-        const int slot = t.used[i];
-        const int c = t.keys[slot], oc = t.outc[slot], ic = t.inc[slot];
+        int c, oc, ic;
+        table_entry(t, i, c, oc, ic);
         if (oc > 0) {
             long long mrc, msc;
             if (c == r) mrc = mrr, msc = msr;
@@ -557,38 +661,41 @@ __device__ double evaluate_move_np(const FtArgs &a, MoveCtx &m, const Table &t, 
     return dS;
 }
 
-template <bool GREEDY, bool NP, class G = WaveGroup>
-__device__ __forceinline__ double evaluate(const FtArgs &a, MoveCtx &m, const Table &t, const U4 &w, bool &accept,
+template <bool GREEDY, bool NP, class G = WaveGroup, class T = Table>
+__device__ __forceinline__ double evaluate(const FtArgs &a, MoveCtx &m, const T &t, const U4 &w, bool &accept,
                                            const G &grp = G{}) {
     if constexpr (NP) return evaluate_move_np<GREEDY>(a, m, t, w, accept, grp);
     else return evaluate_move<GREEDY>(a, m, t, w, accept, grp);
 }
 
-template <bool BIG, bool GREEDY, bool NP>
-__global__ void __launch_bounds__(FT_BLOCK) __attribute__((amdgpu_waves_per_eu(BIG ? 3 : 4))) k_ft_propose(FtArgs a) {
-    __shared__ int lds[BIG ? 1 : FT_WAVES * LDS_TABLE_INTS];
-    const int lane = lane_id(), wid = threadIdx.x / WAVE;
-    const int64_t gwave = (int64_t)blockIdx.x * FT_WAVES + wid;
-    const int64_t nwaves = (int64_t)gridDim.x * FT_WAVES;
-    const Table t = make_table(a, BIG, lds + (BIG ? 0 : wid * LDS_TABLE_INTS), gwave);
-    if (!BIG) table_clear_all(t);  // the global tables arrive empty (prepare_tables) and are left empty
-    const int64_t count = BIG ? (int64_t)*a.big_count : a.count;
+// Proposals of one degree class by lane groups: L lanes per vertex, WAVE / L vertices per wavefront, each group
+// with its own neighbour-block table in LDS sized for the class's largest degree (distinct neighbour blocks <=
+// |out| + |in|).  Small vertices keep few lanes busy, so packing several into a wavefront keeps more of their
+// dependent load chains (offsets -> neighbour -> its block -> the multinomial -> the cell lookups) in flight.
+// The result of each vertex does not depend on L: its sums are fixed point.
+// OCC: wavefronts per SIMD the register budget is sized for (4: 128 VGPRs; 3: 168, which the greedy parametric
+// evaluation needs to run without spills).
+template <int L, bool GREEDY, bool NP, int OCC>
+__global__ void __launch_bounds__(FT_BLOCK) __attribute__((amdgpu_waves_per_eu(OCC))) k_ft_lanes(FtArgs a, int bucket) {
+    __shared__ int lds[FT_WAVES * LDS_WAVE_INTS];
+    constexpr int GPW = WAVE / L;  // lane groups per wavefront
+    const LaneGroup<L> grp;
+    const int wid = threadIdx.x / WAVE, gi = lane_id() / L;
+    const int dmax = ft_bucket_max(bucket), lg = ft_bucket_log2(bucket);
+    const Table t = table_at(lds + wid * LDS_WAVE_INTS + gi * (int)table_ints(lg, dmax), lg, dmax);
+    group_table_clear_all(t, grp);
+    const int64_t count = a.bmeta[bucket];
+    const int32_t *items = a.blist + a.bmeta[FT_NB + bucket];
+    const int64_t stride = (int64_t)gridDim.x * FT_WAVES * GPW;
     unsigned long long bytes = 0;
-    for (int64_t item = gwave; item < count; item += nwaves) {
-        const int v = BIG ? a.big_queue[item] : a.items[item];
-        if (!BIG) {
-            const long long kall = (a.out_off[v + 1] - a.out_off[v]) + (a.in_off[v + 1] - a.in_off[v]);
-            if (kall > SMALL_K) {  // deferred to the global-memory-table launch
-                if (lane == 0) a.big_queue[atomicAdd(a.big_count, 1)] = v;
-                continue;
-            }
-        }
+    for (int64_t item = ((int64_t)blockIdx.x * FT_WAVES + wid) * GPW + gi; item < count; item += stride) {
+        const int v = items[item];
         MoveCtx m;
         U4 w;
-        const int s = propose_block(a, v, m, w);
-        if (a.s_out && lane == 0) a.s_out[v] = s;
+        const int s = propose_block(a, v, m, w, grp);
+        if (a.s_out && grp.rank() == 0) a.s_out[v] = s;
         if (s == m.r) {
-            if (lane == 0) {
+            if (grp.rank() == 0) {
                 a.move_s[v] = -1;
                 if (a.dS_out) a.dS_out[v] = 0.0;
             }
@@ -596,27 +703,32 @@ __global__ void __launch_bounds__(FT_BLOCK) __attribute__((amdgpu_waves_per_eu(B
             continue;
         }
         bool accept;
-        const double dS = evaluate<GREEDY, NP>(a, m, t, w, accept);
-        if (lane == 0) {
+        const double dS = evaluate<GREEDY, NP>(a, m, t, w, accept, grp);
+        if (grp.rank() == 0) {
             a.move_s[v] = accept ? s : -1;
             if (a.dS_out) a.dS_out[v] = dS;
         }
         bytes += proposal_bytes(a, m, *t.n_used, GREEDY);
-        table_clear_used(t);
+        group_table_clear_used(t, grp);
     }
-    if (lane == 0 && bytes) atomicAdd(a.bytes, bytes);
+    bytes = wave_sum(grp.rank() == 0 ? bytes : 0ull);
+    if (lane_id() == 0 && bytes) atomicAdd(a.bytes, bytes);
 }
 
-// Vertices of total degree > SMALL_K (queued by the first launch), one workgroup each: HUB_THREADS threads
-// split the vertex's edges and its distinct neighbour blocks, with the neighbour-block table in LDS.  A vertex
-// that may have more distinct neighbour blocks than half the table's slots (min(k, B) > HUB_CAP / 2) uses its
-// workgroup's table in global memory instead.  Wavefront 0 draws the proposal; the delta-MDL, the Hastings
-// correction and the accept test are those of the one-wavefront evaluation, summed in fixed point over the
-// workgroup (src/finetune.cpp:447-462,871-901).
+// Vertices of total degree > SMALL_K, one workgroup each: THREADS threads split the vertex's edges and its distinct
+// neighbour blocks, with the neighbour-block table in LDS (2^CAPLG slots).  A vertex that may have more distinct
+// neighbour blocks than half the table's slots (min(k, B) > 2^(CAPLG-1)) uses its workgroup's table in global
+// memory instead.  Wavefront 0 draws the proposal; the delta-MDL, the Hastings correction and the accept test are
+// those of the one-wavefront evaluation, summed in fixed point over the workgroup (src/finetune.cpp:447-462,
+// 871-901).  Degree class FT_NB-2 (k <= 1,024) runs HUB_MID_THREADS-thread workgroups with 2,048-slot tables
+// (28 KB), the last class 512-thread workgroups with 4,096-slot tables (57 KB: two workgroups per CU).
 constexpr int HUB_THREADS = 512;
 constexpr int HUB_CAP_LOG2 = 12;
 constexpr int HUB_CAP = 1 << HUB_CAP_LOG2;
-constexpr int64_t HUB_TABLE_INTS = table_ints(HUB_CAP_LOG2, HUB_CAP / 2);  // 57 KB: two workgroups per CU
+constexpr int HUB_MID_CAP_LOG2 = 11;
+// Block counts up to which the last class indexes its table by block id in LDS (74 KB: two workgroups per CU).
+constexpr int HUB_DENSE_MAX = 6144;
+static_assert((1 << (HUB_MID_CAP_LOG2 - 1)) >= ft_bucket_max(FT_NB - 2), "mid-hub tables hold every block");
 
 // Given moves (vertex, target block) to evaluate instead of proposals to draw: entropy::delta_mdl and
 // hastings_correction of each (src/entropy.cpp:298-403,497-561), as the proposal kernels compute them.
@@ -627,30 +739,84 @@ struct EvalIO {
     double *H;   // [n] (1 for the nonparametric description length, src/entropy.cpp:499)
 };
 
-template <bool GREEDY, bool NP, bool EVAL = false>
-__global__ void __launch_bounds__(HUB_THREADS) k_ft_hub(FtArgs a, EvalIO io = EvalIO{}) {
-    __shared__ int lds[HUB_TABLE_INTS];
-    __shared__ Fx red_fx[HUB_THREADS / WAVE];
-    __shared__ long long red_ll[HUB_THREADS / WAVE];
+// One high-degree item's evaluation after its proposal: delta-MDL (and, for EVAL, the Hastings correction) with
+// table t, the outputs, and the table emptied again.
+template <bool GREEDY, bool NP, bool EVAL, int THREADS, class T>
+__device__ __forceinline__ void hub_finish(const FtArgs &a, MoveCtx &m, const T &t, const U4 &w, int s, int v,
+                                           int64_t item, const EvalIO &io, const BlockGroup &grp,
+                                           unsigned long long &bytes) {
+    bool accept;
+    const double dS = evaluate<EVAL ? true : GREEDY, NP>(a, m, t, w, accept, grp);
+    const int nb = table_size(t);
+    if (EVAL) {
+        double H = 1.0;
+        if (!NP) {
+            long long mrr, mss;
+            get_cell_pair(a, m.r, m.r, m.s, m.s, mrr, mss);
+            H = hastings_value(a, m, t, nb, mrr, mss, grp);
+        }
+        if (threadIdx.x == 0) {
+            io.dS[item] = dS;
+            io.H[item] = H;
+        }
+    } else if (threadIdx.x == 0) {
+        a.move_s[v] = accept ? s : -1;
+        if (a.dS_out) a.dS_out[v] = dS;
+        bytes += proposal_bytes(a, m, nb, GREEDY);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += THREADS) {
+        if constexpr (std::is_same<T, DenseTable>::value) {
+            t.cnt[t.used[i]] = 0ull;
+        } else {
+            const int slot = t.used[i];
+            t.keys[slot] = EMPTY;
+            t.outc[slot] = 0;
+            t.inc[slot] = 0;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *t.n_used = 0;
+    __syncthreads();
+}
+
+// DENSE: the neighbour-block table is indexed by block id (B <= HUB_DENSE_MAX), in LDS.
+template <bool GREEDY, bool NP, bool EVAL = false, int THREADS = HUB_THREADS, int CAPLG = HUB_CAP_LOG2,
+          bool DENSE = false>
+__global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+k_ft_hub(FtArgs a, int bucket, EvalIO io = EvalIO{}) {
+    constexpr int CAP = 1 << CAPLG;
+    constexpr int64_t LDS_INTS = DENSE ? dense_table_ints(HUB_DENSE_MAX, HUB_DENSE_MAX) : table_ints(CAPLG, CAP / 2);
+    __shared__ unsigned long long lds64[(LDS_INTS + 1) / 2];
+    int *lds = reinterpret_cast<int *>(lds64);
+    __shared__ Fx red_fx[THREADS / WAVE];
+    __shared__ long long red_ll[THREADS / WAVE];
     __shared__ MoveCtx s_m;
     __shared__ U4 s_w;
     __shared__ int s_s;
     BlockGroup grp;
     grp.fx = red_fx;
     grp.ll = red_ll;
-    const Table lt = table_at(lds, HUB_CAP_LOG2, HUB_CAP / 2);
-    const Table gt = table_at(a.gtable + (int64_t)blockIdx.x * table_ints(a.gcap_log2), a.gcap_log2);
-    for (int i = threadIdx.x; i < HUB_CAP; i += HUB_THREADS) {
-        lt.keys[i] = EMPTY;
-        lt.outc[i] = 0;
-        lt.inc[i] = 0;
+    const Table lt = table_at(lds, CAPLG, CAP / 2);
+    const DenseTable dt = dense_table_at(lds, HUB_DENSE_MAX);
+    if constexpr (DENSE) {
+        for (int i = threadIdx.x; i < a.B; i += THREADS) dt.cnt[i] = 0ull;
+        if (threadIdx.x == 0) *dt.n_used = 0;
+    } else {
+        for (int i = threadIdx.x; i < CAP; i += THREADS) {
+            lt.keys[i] = EMPTY;
+            lt.outc[i] = 0;
+            lt.inc[i] = 0;
+        }
+        if (threadIdx.x == 0) *lt.n_used = 0;
     }
-    if (threadIdx.x == 0) *lt.n_used = 0;
     __syncthreads();
-    const int64_t count = EVAL ? io.n : *a.big_count;
+    const Table gt = table_at(a.gtable + (int64_t)blockIdx.x * table_ints(a.gcap_log2), a.gcap_log2);
+    const int64_t count = EVAL ? io.n : a.bmeta[bucket];
+    const int32_t *hubs = EVAL ? io.vertices : a.blist + a.bmeta[FT_NB + bucket];
     unsigned long long bytes = 0;
     for (int64_t item = blockIdx.x; item < count; item += gridDim.x) {
-        const int v = EVAL ? io.vertices[item] : a.big_queue[item];
+        const int v = hubs[item];
         if (threadIdx.x < WAVE) {
             MoveCtx m;
             U4 w;
@@ -681,59 +847,71 @@ __global__ void __launch_bounds__(HUB_THREADS) k_ft_hub(FtArgs a, EvalIO io = Ev
             __syncthreads();  // s_m is written again for the next vertex
             continue;
         }
-        const bool in_lds = 2 * (m.k_all < (long long)a.B ? m.k_all : (long long)a.B) <= HUB_CAP;
-        const Table &t = in_lds ? lt : gt;
-        bool accept;
-        const double dS = evaluate<EVAL ? true : GREEDY, NP>(a, m, t, w, accept, grp);
-        const int nb = *t.n_used;
-        if (EVAL) {
-            double H = 1.0;
-            if (!NP) {
-                long long mrr, mss;
-                get_cell_pair(a, m.r, m.r, m.s, m.s, mrr, mss);
-                H = hastings_value(a, m, t, nb, mrr, mss, grp);
-            }
-            if (threadIdx.x == 0) {
-                io.dS[item] = dS;
-                io.H[item] = H;
-            }
-        } else if (threadIdx.x == 0) {
-            a.move_s[v] = accept ? s : -1;
-            if (a.dS_out) a.dS_out[v] = dS;
-            bytes += proposal_bytes(a, m, nb, GREEDY);
+        if constexpr (DENSE) {
+            hub_finish<GREEDY, NP, EVAL, THREADS>(a, m, dt, w, s, v, item, io, grp, bytes);
+        } else {
+            const bool in_lds = 2 * (m.k_all < (long long)a.B ? m.k_all : (long long)a.B) <= CAP;
+            hub_finish<GREEDY, NP, EVAL, THREADS>(a, m, in_lds ? lt : gt, w, s, v, item, io, grp, bytes);
         }
-        __syncthreads();
-        for (int i = threadIdx.x; i < nb; i += HUB_THREADS) {
-            const int slot = t.used[i];
-            t.keys[slot] = EMPTY;
-            t.outc[slot] = 0;
-            t.inc[slot] = 0;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) *t.n_used = 0;
-        __syncthreads();
     }
     if (threadIdx.x == 0 && bytes) atomicAdd(a.bytes, bytes);
 }
 
-static void launch_hub(int blocks, hipStream_t st, const FtArgs &a, bool greedy, bool np) {
+template <int THREADS, int CAPLG, bool DENSE = false>
+static void launch_hub_t(int blocks, hipStream_t st, const FtArgs &a, int bucket, bool greedy, bool np) {
     if (np) {
-        if (greedy) k_ft_hub<true, true><<<blocks, HUB_THREADS, 0, st>>>(a);
-        else k_ft_hub<false, true><<<blocks, HUB_THREADS, 0, st>>>(a);
+        if (greedy) k_ft_hub<true, true, false, THREADS, CAPLG, DENSE><<<blocks, THREADS, 0, st>>>(a, bucket);
+        else k_ft_hub<false, true, false, THREADS, CAPLG, DENSE><<<blocks, THREADS, 0, st>>>(a, bucket);
     } else {
-        if (greedy) k_ft_hub<true, false><<<blocks, HUB_THREADS, 0, st>>>(a);
-        else k_ft_hub<false, false><<<blocks, HUB_THREADS, 0, st>>>(a);
+        if (greedy) k_ft_hub<true, false, false, THREADS, CAPLG, DENSE><<<blocks, THREADS, 0, st>>>(a, bucket);
+        else k_ft_hub<false, false, false, THREADS, CAPLG, DENSE><<<blocks, THREADS, 0, st>>>(a, bucket);
     }
 }
 
-template <bool BIG>
-static void launch_ft(int blocks, hipStream_t st, const FtArgs &a, bool greedy, bool np) {
-    if (np) {
-        if (greedy) k_ft_propose<BIG, true, true><<<blocks, FT_BLOCK, 0, st>>>(a);
-        else k_ft_propose<BIG, false, true><<<blocks, FT_BLOCK, 0, st>>>(a);
+// The workgroup class FT_NB-2 (k <= 1,024) with `threads` (128 or 256) per vertex; the last class with 512 and,
+// when the blocks fit (B <= HUB_DENSE_MAX), the table indexed by block id.
+static void launch_hub(int blocks, hipStream_t st, const FtArgs &a, int bucket, int threads, bool greedy, bool np) {
+    if (bucket == FT_NB - 1) {
+        if (a.B <= HUB_DENSE_MAX) launch_hub_t<HUB_THREADS, HUB_CAP_LOG2, true>(blocks, st, a, bucket, greedy, np);
+        else launch_hub_t<HUB_THREADS, HUB_CAP_LOG2>(blocks, st, a, bucket, greedy, np);
+    } else if (threads == 128) {
+        launch_hub_t<128, HUB_MID_CAP_LOG2>(blocks, st, a, bucket, greedy, np);
     } else {
-        if (greedy) k_ft_propose<BIG, true, false><<<blocks, FT_BLOCK, 0, st>>>(a);
-        else k_ft_propose<BIG, false, false><<<blocks, FT_BLOCK, 0, st>>>(a);
+        launch_hub_t<256, HUB_MID_CAP_LOG2>(blocks, st, a, bucket, greedy, np);
+    }
+}
+
+template <int L>
+static void launch_lanes_t(int blocks, hipStream_t st, const FtArgs &a, int bucket, bool greedy, bool np, int occ) {
+    if (np) {
+        if (greedy) k_ft_lanes<L, true, true, 4><<<blocks, FT_BLOCK, 0, st>>>(a, bucket);
+        else k_ft_lanes<L, false, true, 4><<<blocks, FT_BLOCK, 0, st>>>(a, bucket);
+    } else if (greedy) {
+        if (occ == 3) k_ft_lanes<L, true, false, 3><<<blocks, FT_BLOCK, 0, st>>>(a, bucket);
+        else k_ft_lanes<L, true, false, 4><<<blocks, FT_BLOCK, 0, st>>>(a, bucket);
+    } else {
+        k_ft_lanes<L, false, false, 4><<<blocks, FT_BLOCK, 0, st>>>(a, bucket);
+    }
+}
+
+// Lanes per vertex that a degree class may use: a power of two from 8 to 64 whose tables fit the LDS budget; and
+// the wavefronts per SIMD of the greedy parametric launch (3 or 4).
+static bool shape_allowed(int bucket, int L, int occ) {
+    return (L == 8 || L == 16 || L == 32 || L == 64) && bucket >= 0 && bucket < FT_NB - 2 && ft_lanes_fit(bucket, L) &&
+           (occ == 3 || occ == 4);
+}
+
+// One lane-group launch over degree class `bucket` (items counted on the device), sized for up to `upper` items.
+static void launch_lanes(int64_t upper, hipStream_t st, const FtArgs &a, int bucket, int L, int occ, bool greedy,
+                         bool np) {
+    const int64_t per_block = (int64_t)FT_WAVES * (WAVE / L);
+    int64_t blocks = (upper + per_block - 1) / per_block;
+    blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, 4096));
+    switch (L) {
+        case 8: launch_lanes_t<8>((int)blocks, st, a, bucket, greedy, np, occ); break;
+        case 16: launch_lanes_t<16>((int)blocks, st, a, bucket, greedy, np, occ); break;
+        case 32: launch_lanes_t<32>((int)blocks, st, a, bucket, greedy, np, occ); break;
+        default: launch_lanes_t<64>((int)blocks, st, a, bucket, greedy, np, occ); break;
     }
 }
 
@@ -779,8 +957,8 @@ static FtArgs base_args(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     a.gtable = nullptr;
     a.gcap_log2 = 0;
     a.err = nullptr;
-    a.big_queue = nullptr;
-    a.big_count = nullptr;
+    a.blist = nullptr;
+    a.bmeta = nullptr;
     a.np = NpConst{};
     a.hist = nullptr;
     if (bm->np) {
@@ -821,62 +999,223 @@ static int *prepare_tables(sbp_ctx ctx, int lg, int64_t tables) {
     return base;
 }
 
+// ------------------------------------------------------------------ degree classes
+
+__device__ __forceinline__ int ft_class_of(long long k) {
+    return k <= ft_bucket_max(0)   ? 0
+           : k <= ft_bucket_max(1) ? 1
+           : k <= ft_bucket_max(2) ? 2
+           : k <= ft_bucket_max(3) ? 3
+           : k <= ft_bucket_max(4) ? 4
+                                   : 5;
+}
+
+__global__ void k_vertex_class(int32_t N, const int64_t *__restrict__ out_off, const int64_t *__restrict__ in_off,
+                               uint8_t *__restrict__ cls) {
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+        cls[v] = (uint8_t)ft_class_of((out_off[v + 1] - out_off[v]) + (in_off[v + 1] - in_off[v]));
+}
+
+// meta: [0, FT_NB) counts, [FT_NB, 2 FT_NB) offsets, [2 FT_NB, 3 FT_NB) scatter cursors
+__global__ void k_class_zero(int32_t *__restrict__ meta) {
+    if (threadIdx.x < 3 * FT_NB) meta[threadIdx.x] = 0;
+}
+
+// Counts the items of each class with one atomic per wavefront and class.
+__global__ void k_class_count(int64_t n, const int32_t *__restrict__ items, const int64_t *__restrict__ out_off,
+                              const int64_t *__restrict__ in_off, int32_t *__restrict__ meta) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int v = items[i];
+        const int c = ft_class_of((out_off[v + 1] - out_off[v]) + (in_off[v + 1] - in_off[v]));
+#pragma unroll
+        for (int b = 0; b < FT_NB; ++b) {
+            const unsigned long long msk = __ballot(c == b);
+            if (msk && lane_id() == __ffsll((long long)msk) - 1) atomicAdd(&meta[b], __popcll(msk));
+        }
+    }
+}
+
+// Writes each item to its class's range of list (order within a class is immaterial: every proposal reads only
+// the frozen state).
+__global__ void k_class_scatter(int64_t n, const int32_t *__restrict__ items, const int64_t *__restrict__ out_off,
+                                const int64_t *__restrict__ in_off, int32_t *__restrict__ meta,
+                                int32_t *__restrict__ list) {
+    int off[FT_NB];
+    int run = 0;
+#pragma unroll
+    for (int b = 0; b < FT_NB; ++b) {
+        off[b] = run;
+        run += meta[b];
+    }
+    if (blockIdx.x == 0 && threadIdx.x < FT_NB) meta[FT_NB + threadIdx.x] = off[threadIdx.x];
+    const int lane = lane_id();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int v = items[i];
+        const int c = ft_class_of((out_off[v + 1] - out_off[v]) + (in_off[v + 1] - in_off[v]));
+#pragma unroll
+        for (int b = 0; b < FT_NB; ++b) {
+            const unsigned long long msk = __ballot(c == b);
+            if (!msk) continue;
+            const int leader = __ffsll((long long)msk) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&meta[2 * FT_NB + b], __popcll(msk));
+            base = __shfl(base, leader, WAVE);
+            if (c == b) list[off[b] + base + __popcll(msk & ((1ull << lane) - 1ull))] = v;
+        }
+    }
+}
+
+// Every vertex of g grouped by degree class, ascending id within a class; built once per graph (degrees do not
+// change) and published under deg_mu once complete, since workers on other streams may share the graph.
+static void graph_classes(sbp_ctx ctx, sbp_graph g) {
+    std::lock_guard<std::mutex> lk(g->deg_mu);
+    if (g->ft_blist) return;
+    hipStream_t st = ctx->stream;
+    const int32_t N = g->N;
+    uint8_t *d_cls = reinterpret_cast<uint8_t *>(ctx->scratch[S_BUCKETS].as<int32_t>((size_t)N / 4 + 16));
+    std::vector<uint8_t> cls((size_t)N);
+    if (N > 0) {
+        k_vertex_class<<<(int)std::min<int64_t>((N + 255) / 256, 16384), 256, 0, st>>>(N, g->out_off, g->in_off, d_cls);
+        SBP_HIP(hipGetLastError());
+        SBP_HIP(hipMemcpyAsync(cls.data(), d_cls, (size_t)N, hipMemcpyDeviceToHost, st));
+        SBP_HIP(hipStreamSynchronize(st));
+    }
+    int64_t cnt[FT_NB] = {};
+    for (uint8_t c : cls) cnt[c]++;
+    std::vector<int32_t> meta(2 * FT_NB);
+    int64_t run = 0;
+    for (int b = 0; b < FT_NB; ++b) {
+        meta[b] = (int32_t)cnt[b];
+        meta[FT_NB + b] = (int32_t)run;
+        run += cnt[b];
+    }
+    std::vector<int32_t> list((size_t)std::max<int32_t>(N, 1));
+    int64_t pos[FT_NB];
+    for (int b = 0; b < FT_NB; ++b) pos[b] = meta[FT_NB + b];
+    for (int32_t v = 0; v < N; ++v) list[(size_t)pos[cls[v]]++] = v;
+    int32_t *d_list = nullptr, *d_meta = nullptr;
+    SBP_HIP(hipMalloc(&d_list, sizeof(int32_t) * list.size()));
+    if (hipMalloc(&d_meta, sizeof(int32_t) * meta.size()) != hipSuccess) {
+        (void)hipFree(d_list);
+        SBP_HIP(hipErrorOutOfMemory);
+    }
+    hipError_t e = hipMemcpy(d_list, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_meta, meta.data(), sizeof(int32_t) * meta.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d_list);
+        (void)hipFree(d_meta);
+        SBP_HIP(e);
+    }
+    for (int b = 0; b < FT_NB; ++b) g->ft_bcount[b] = cnt[b];
+    g->ft_bmeta = d_meta;
+    g->ft_blist = d_list;
+}
+
 FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int greedy, const int32_t *d_vertices,
                         int64_t count, int32_t *d_move_s, double *d_dS, int32_t *d_s_out) {
     hipStream_t st = ctx->stream;
-    FtLaunchInfo info{0, 0, count, 0.f, 0.f, false};
+    FtLaunchInfo info{};
+    info.proposals = count;
     if (count == 0) return info;
-    auto *counters = ctx->scratch[S_COUNTER].as<unsigned long long>(4);
-    SBP_HIP(hipMemsetAsync(counters, 0, 4 * sizeof(unsigned long long), st));
+    // [0, FT_NB): algorithmic bytes per degree class; [FT_NB]: the error flag
+    auto *counters = ctx->scratch[S_COUNTER].as<unsigned long long>(8);
+    SBP_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(unsigned long long), st));
     FtArgs a = base_args(ctx, bm, g, epoch, greedy);
     bm_ensure_hash(ctx, bm);
     a.m.hash = bm->cell_hash;
     a.m.hash_shift = 64 - bm->hash_log2;
     a.m.hash_mask = (1u << bm->hash_log2) - 1;
-    a.bytes = counters;
-    a.err = reinterpret_cast<int *>(counters + 1);
-    a.big_count = reinterpret_cast<int32_t *>(counters + 2);
-    a.big_queue = ctx->scratch[S_EVENTS_B].as<int32_t>((size_t)g->N + 1);
+    a.err = reinterpret_cast<int *>(counters + FT_NB);
     a.items = d_vertices;
     a.count = count;
     a.move_s = d_move_s;
     a.dS_out = d_dS;
     a.s_out = d_s_out;
-    int64_t blocks = (count + FT_WAVES - 1) / FT_WAVES;
-    if (blocks > 4096) blocks = 4096;
-    SBP_HIP(hipEventRecord(ctx->ev[0], st));
-    launch_ft<false>((int)blocks, st, a, greedy != 0, bm->np != 0);
+    // the items grouped by degree class: every vertex (the common sweep) from the graph's grouping, any other list
+    // grouped here on the device; upper[b] bounds class b's size
+    int64_t upper[FT_NB];
+    if (d_vertices == g->edge_src + g->E && count == g->N) {
+        graph_classes(ctx, g);
+        a.blist = g->ft_blist;
+        a.bmeta = g->ft_bmeta;
+        for (int b = 0; b < FT_NB; ++b) upper[b] = g->ft_bcount[b];
+    } else {
+        int32_t *meta = ctx->scratch[S_BUCKETS].as<int32_t>((size_t)count + 4 * FT_NB);
+        int32_t *list = meta + 4 * FT_NB;
+        const int grid = (int)std::min<int64_t>((count + 255) / 256, 4096);
+        k_class_zero<<<1, WAVE, 0, st>>>(meta);
+        k_class_count<<<grid, 256, 0, st>>>(count, d_vertices, g->out_off, g->in_off, meta);
+        k_class_scatter<<<grid, 256, 0, st>>>(count, d_vertices, g->out_off, g->in_off, meta, list);
+        SBP_HIP(hipGetLastError());
+        a.blist = list;
+        a.bmeta = meta;
+        for (int b = 0; b < FT_NB; ++b) upper[b] = b < FT_NB - 2 ? count : std::min<int64_t>(count, g->n_high_degree);
+    }
+    SBP_HIP(hipEventRecord(ctx->ft_ev[0], st));
+    for (int b = 0; b < FT_NB - 2; ++b) {
+        info.class_launched[b] = upper[b] > 0;
+        if (info.class_launched[b]) {
+            a.bytes = counters + b;
+            launch_lanes(upper[b], st, a, b, ctx->ft_lanes[b], ctx->ft_occ[b], greedy != 0, bm->np != 0);
+        }
+        SBP_HIP(hipEventRecord(ctx->ft_ev[b + 1], st));
+    }
     SBP_HIP(hipGetLastError());
-    SBP_HIP(hipEventRecord(ctx->ev[6], st));
-    info.high_launched = g->max_degree > SMALL_K;
-    if (info.high_launched) {
-        a.bytes = counters + 3;
-        // one workgroup per deferred vertex, up to 1,024 at once; global tables (one per workgroup, each holding
-        // the distinct neighbour blocks of one vertex at load factor <= 1/2, within kMaxTableBytes) only when a
-        // vertex can have more distinct neighbour blocks than the LDS table takes
+    // k <= 1,024: one workgroup per vertex, LDS tables only (2 min(k, B) <= 2,048 slots)
+    info.class_launched[FT_NB - 2] = upper[FT_NB - 2] > 0;
+    if (info.class_launched[FT_NB - 2]) {
+        a.bytes = counters + FT_NB - 2;
+        launch_hub((int)std::min<int64_t>(upper[FT_NB - 2], 2048), st, a, FT_NB - 2, ctx->ft_lanes[FT_NB - 2],
+                   greedy != 0, bm->np != 0);
+    }
+    SBP_HIP(hipEventRecord(ctx->ft_ev[FT_NB - 1], st));
+    info.class_launched[FT_NB - 1] = upper[FT_NB - 1] > 0;
+    if (info.class_launched[FT_NB - 1]) {
+        a.bytes = counters + FT_NB - 1;
+        // one workgroup per vertex, up to 1,024 at once; global tables (one per workgroup, each holding the
+        // distinct neighbour blocks of one vertex at load factor <= 1/2, within kMaxTableBytes) only when a vertex
+        // can have more distinct neighbour blocks than the LDS table takes
         const int64_t kmax = std::min<int64_t>(g->max_degree, bm->B);
         const int lg = std::max(10, log2_ceil(2 * kmax + 2));
         const size_t tints = (size_t)table_ints(lg);
         constexpr size_t kMaxTableBytes = size_t(1) << 30;
-        int64_t blocks_h = std::min<int64_t>(std::min<int64_t>(g->n_high_degree, count), 1024);
+        int64_t blocks_h = std::min<int64_t>(upper[FT_NB - 1], 1024);
         a.gtable = nullptr;
-        if (2 * kmax > HUB_CAP) {
+        if (2 * kmax > HUB_CAP && bm->B > HUB_DENSE_MAX) {
             blocks_h = std::min<int64_t>(blocks_h, (int64_t)(kMaxTableBytes / (tints * sizeof(int))));
             a.gtable = prepare_tables(ctx, lg, std::max<int64_t>(blocks_h, 1));
         }
         a.gcap_log2 = lg;
-        launch_hub((int)std::max<int64_t>(blocks_h, 1), st, a, greedy != 0, bm->np != 0);
-        SBP_HIP(hipGetLastError());
+        launch_hub((int)std::max<int64_t>(blocks_h, 1), st, a, FT_NB - 1, HUB_THREADS, greedy != 0, bm->np != 0);
     }
-    SBP_HIP(hipEventRecord(ctx->ev[1], st));
-    unsigned long long hb[4] = {0, 0, 0, 0};
+    SBP_HIP(hipGetLastError());
+    SBP_HIP(hipEventRecord(ctx->ft_ev[FT_NB], st));
+    unsigned long long hb[8] = {};
     read_small(ctx, {{hb, counters, sizeof(hb)}});
-    SBP_CHECK(hb[1] == 0, SBP_E_STATE, "finetune proposal: block degrees inconsistent with the block matrix");
-    info.bytes = hb[0];
-    info.high_bytes = hb[3];
-    info.kernel_ms = (float)elapsed_ms(ctx, ctx->ev[0], ctx->ev[6]);
-    if (info.high_launched) info.high_kernel_ms = (float)elapsed_ms(ctx, ctx->ev[6], ctx->ev[1]);
+    SBP_CHECK(hb[FT_NB] == 0, SBP_E_STATE, "finetune proposal: block degrees inconsistent with the block matrix");
+    for (int b = 0; b < FT_NB; ++b) {
+        info.class_bytes[b] = hb[b];
+        info.class_ms[b] = info.class_launched[b] ? (float)elapsed_ms(ctx, ctx->ft_ev[b], ctx->ft_ev[b + 1]) : 0.f;
+        if (b < FT_NB - 2) {
+            info.bytes += hb[b];
+            info.kernel_ms += info.class_ms[b];
+        } else {
+            info.high_bytes += hb[b];
+            info.high_kernel_ms += info.class_ms[b];
+        }
+    }
+    info.high_launched = info.class_launched[FT_NB - 2] || info.class_launched[FT_NB - 1];
     return info;
+}
+
+// Launch shape of the degree classes (sbp_set_finetune_shape).
+bool set_finetune_shape(sbp_ctx ctx, const int32_t *lanes, const int32_t *waves_per_simd) {
+    for (int b = 0; b < FT_NB - 2; ++b)
+        if (!shape_allowed(b, lanes[b], waves_per_simd[b])) return false;
+    if (lanes[FT_NB - 2] != 128 && lanes[FT_NB - 2] != 256) return false;
+    for (int b = 0; b < FT_NB - 1; ++b) ctx->ft_lanes[b] = lanes[b];
+    for (int b = 0; b < FT_NB - 2; ++b) ctx->ft_occ[b] = waves_per_simd[b];
+    return true;
 }
 
 void ft_evaluate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int64_t n, const int32_t *d_vertices, const int32_t *d_targets,
@@ -902,8 +1241,8 @@ void ft_evaluate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int64_t n, const int32_t *
     }
     a.gcap_log2 = lg;
     const EvalIO io{d_vertices, d_targets, n, d_dS, d_H};
-    if (bm->np) k_ft_hub<true, true, true><<<(int)blocks, HUB_THREADS, 0, st>>>(a, io);
-    else k_ft_hub<true, false, true><<<(int)blocks, HUB_THREADS, 0, st>>>(a, io);
+    if (bm->np) k_ft_hub<true, true, true><<<(int)blocks, HUB_THREADS, 0, st>>>(a, 0, io);
+    else k_ft_hub<true, false, true><<<(int)blocks, HUB_THREADS, 0, st>>>(a, 0, io);
     SBP_HIP(hipGetLastError());
     unsigned long long hb[2] = {0, 0};
     read_small(ctx, {{hb, counters, sizeof(hb)}});

@@ -129,7 +129,7 @@ struct DevicePool {
 enum ScratchSlot {
     S_KEYS_A, S_KEYS_B, S_VALS_A, S_VALS_B, S_UNIQ, S_COUNTS, S_SORT_TMP, S_MISC, S_PARTIAL,
     S_MOVES, S_DS, S_ACCEPT, S_COUNTER, S_HASH, S_RISK, S_EVENTS_A, S_EVENTS_B, S_ORDER, S_BEST,
-    S_ASSIGN_TMP, S_FLAGS, S_NP, S_NUM_SLOTS
+    S_ASSIGN_TMP, S_FLAGS, S_NP, S_BUCKETS, S_NUM_SLOTS
 };
 
 }  // namespace sbpimpl
@@ -142,6 +142,7 @@ struct sbp_ctx_s {
     sbpimpl::Scratch scratch[sbpimpl::S_NUM_SLOTS];
     std::shared_ptr<sbpimpl::DevicePool> pool = std::make_shared<sbpimpl::DevicePool>();
     hipEvent_t ev[8] = {};
+    hipEvent_t ft_ev[8] = {};  // boundaries of the finetune proposal launches, one per degree class
     void *pinned = nullptr;  // pinned staging area for small reads (sbpimpl::PINNED_BYTES)
     double *lgamma = nullptr;  // fastlgamma table for the nonparametric entropy, grown on demand
     int64_t lgamma_n = 0;
@@ -154,6 +155,10 @@ struct sbp_ctx_s {
     std::shared_ptr<sbpcarry::Pool> carry_pool;  // the carry-out's threads, kept between merge phases
     uint64_t peak_bytes = 0;  // largest device footprint seen at the end of a partition (sbp_ctx_memory)
     std::vector<sbp_ctx> slots;  // worker contexts of sbp_partition_batch, created on first use
+    // launch shape of the finetune degree classes 0..4 (k <= 8, 32, 128, 256, 1,024): lanes (threads) per vertex,
+    // and wavefronts per SIMD of the greedy parametric lane-group launches (sbp_set_finetune_shape)
+    int32_t ft_lanes[5] = {8, 8, 32, 64, 256};
+    int32_t ft_occ[4] = {4, 4, 4, 4};
 };
 
 namespace sbpimpl {
@@ -204,6 +209,11 @@ struct sbp_graph_s {
     int32_t *deg_order = nullptr;   // [N] stable descending degree order, computed on first use; published
                                     // under deg_mu once complete (workers on other streams may share the graph)
     std::mutex deg_mu;
+    // every vertex grouped by finetune degree class (ascending id within a class), built on first use and published
+    // under deg_mu once complete: FT_NB counts then FT_NB offsets into ft_blist (device), the counts on the host
+    int32_t *ft_blist = nullptr;
+    int32_t *ft_bmeta = nullptr;
+    int64_t ft_bcount[6] = {};
     int64_t max_degree = 0;         // max over v of |out(v)| + |in(v)|
     int64_t n_high_degree = 0;      // vertices with |out(v)| + |in(v)| > FT_SMALL_DEGREE
     // set when this graph is a sample of `parent` (sbp_sample)
@@ -213,9 +223,13 @@ struct sbp_graph_s {
     int32_t *sample_vertices = nullptr;  // [N] sample vertex -> parent vertex
 };
 
-// Vertices up to this total degree keep their neighbour-block histogram in LDS during finetune proposals;
-// the others use a per-wave table in global memory.
+// Vertices up to this total degree keep their neighbour-block histogram in LDS during finetune proposals and run
+// on lane groups of a wavefront; the others run one workgroup per vertex.
 constexpr int FT_SMALL_DEGREE = 256;
+// Finetune degree classes: four lane-group classes (k <= 8, 32, 128, 256) and two workgroup classes (k <= 1,024,
+// and the rest).
+constexpr int FT_NB = 6;
+
 
 struct sbp_bm_s {
     sbp_ctx ctx = nullptr;
@@ -282,12 +296,15 @@ sbpdev::NpConst np_const(sbp_ctx ctx, sbp_bm bm);
 double np_mdl(sbp_ctx ctx, sbp_graph g, sbp_bm bm, double *parts);
 // finetune.hip
 struct FtLaunchInfo {
-    uint64_t bytes;       // algorithmic bytes of the LDS-table launch (vertices up to FT_SMALL_DEGREE)
-    uint64_t high_bytes;  // ... of the high-degree launch
+    uint64_t bytes;       // algorithmic bytes of the lane-group launches (vertices up to FT_SMALL_DEGREE)
+    uint64_t high_bytes;  // ... of the workgroup-per-vertex launches
     int64_t proposals;
-    float kernel_ms;      // device time of the LDS-table launch
-    float high_kernel_ms; // device time of the high-degree launch (0 when not launched)
+    float kernel_ms;      // device time of the lane-group launches
+    float high_kernel_ms; // device time of the workgroup-per-vertex launches (0 when not launched)
     bool high_launched;
+    uint64_t class_bytes[FT_NB];  // per degree class
+    float class_ms[FT_NB];
+    bool class_launched[FT_NB];
 };
 FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int greedy, const int32_t *d_vertices,
                         int64_t count, int32_t *d_move_s, double *d_dS, int32_t *d_accept);
@@ -296,6 +313,9 @@ int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_
 void ft_evaluate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int64_t n, const int32_t *d_vertices, const int32_t *d_targets,
                  double *d_dS, double *d_H);
 void shuffle_keys(sbp_ctx ctx, const int32_t *d_items, int64_t n, uint32_t epoch, int32_t *d_out);
+// Sets the launch shape of the four lane-group degree classes (lanes per vertex, wavefronts per SIMD of the greedy
+// parametric launch); false (nothing set) when one is not allowed.
+bool set_finetune_shape(sbp_ctx ctx, const int32_t *lanes, const int32_t *waves_per_simd);
 // graph.hip
 void graph_finish(sbp_ctx ctx, sbp_graph g);
 const int32_t *degree_order(sbp_ctx ctx, sbp_graph g);

@@ -112,6 +112,10 @@ int sbp_partition_batch(sbp_ctx ctx, const sbp_graph *graphs, int32_t n, const s
             }
             ctx->slots.push_back(w);
         }
+        for (sbp_ctx w : ctx->slots) {
+            std::copy(ctx->ft_lanes, ctx->ft_lanes + 5, w->ft_lanes);
+            std::copy(ctx->ft_occ, ctx->ft_occ + 4, w->ft_occ);
+        }
         SBP_HIP(hipStreamSynchronize(ctx->stream));  // graphs uploaded on ctx are complete
         // longest first (by edges, then vertices), which bounds the last worker's tail
         std::vector<int32_t> order(n);

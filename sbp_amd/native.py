@@ -25,7 +25,8 @@ EXPORTS = [
     "sbp_ft_propose_all", "sbp_async_gibbs_sweep", "sbp_mh_sweep", "sbp_mcmc", "sbp_finetune_assignment",
     "sbp_merge_propose", "sbp_merge_blocks",
     "sbp_partition",
-    "sbp_ctx_memory", "sbp_ctx_trim", "sbp_set_carry_threads", "sbp_graph_bytes", "sbp_footprint_bytes",
+    "sbp_ctx_memory", "sbp_ctx_trim", "sbp_set_carry_threads", "sbp_set_finetune_shape", "sbp_get_finetune_shape",
+    "sbp_graph_bytes", "sbp_footprint_bytes",
     "sbp_partition_batch",
     "sbp_server_create", "sbp_server_last_error", "sbp_server_submit", "sbp_server_wait", "sbp_server_info",
     "sbp_server_destroy",
@@ -58,10 +59,11 @@ class Stats(C.Structure):
                 ("ft_kernel_ms", C.c_double), ("carry_evaluations", C.c_int64), ("carry_speculative", C.c_int64),
                 ("carry_reused", C.c_int64), ("carry_requeues", C.c_int64), ("carry_load_ms", C.c_double),
                 ("ft_high_bytes", C.c_uint64), ("ft_high_launches", C.c_int64), ("ft_high_kernel_ms", C.c_double),
-                ("mh_bytes", C.c_uint64), ("mh_launches", C.c_int64), ("mh_kernel_ms", C.c_double)]
+                ("mh_bytes", C.c_uint64), ("mh_launches", C.c_int64), ("mh_kernel_ms", C.c_double),
+                ("ft_class_bytes", C.c_uint64 * 6), ("ft_class_launches", C.c_int64 * 6), ("ft_class_ms", C.c_double * 6)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: (list(getattr(self, k)) if k.startswith("ft_class_") else getattr(self, k)) for k, _ in self._fields_}
 
 
 class Evaluation(C.Structure):
@@ -136,6 +138,8 @@ def lib():
             "sbp_ctx_memory": [vp, vp, vp],
             "sbp_ctx_trim": [vp],
             "sbp_set_carry_threads": [vp, i32],
+            "sbp_set_finetune_shape": [vp, vp, vp],
+            "sbp_get_finetune_shape": [vp, vp, vp],
             "sbp_partition_batch": [vp, vp, i32, vp, i32, vp, vp, vp, vp],
             "sbp_server_create": [C.c_int, i32, u64, vp],
             "sbp_server_submit": [vp, i32, i64, vp, vp, vp, vp, P, vp],
@@ -172,6 +176,10 @@ class Context:
         rc = lib().sbp_create(C.byref(self.h), device, seed)
         if rc != 0:
             raise SbpError(f"sbp_create failed with status {rc} (no HIP device?)")
+        from .autotune import tuned_shape
+        shape = tuned_shape()  # the proposal launch shape measured on this device model, if any
+        if shape is not None:
+            self.set_finetune_shape(*shape)
 
     def check(self, rc, what):
         if rc != 0:
@@ -214,6 +222,20 @@ class Context:
 
     def set_carry_threads(self, threads):
         self.check(lib().sbp_set_carry_threads(self.h, threads), "sbp_set_carry_threads")
+
+    def set_finetune_shape(self, lanes, waves_per_simd=(4, 4, 4, 4)):
+        """Lanes (threads) per vertex of the finetune degree classes k <= 8, 32, 128, 256, 1024 and wavefronts per
+        SIMD of the first four (sbp_set_finetune_shape)."""
+        la = np.ascontiguousarray(lanes, np.int32)
+        wa = np.ascontiguousarray(waves_per_simd, np.int32)
+        assert la.shape == (5,) and wa.shape == (4,)
+        self.check(lib().sbp_set_finetune_shape(self.h, _p(la), _p(wa)), "sbp_set_finetune_shape")
+
+    def finetune_shape(self):
+        la = np.zeros(5, np.int32)
+        wa = np.zeros(4, np.int32)
+        self.check(lib().sbp_get_finetune_shape(self.h, _p(la), _p(wa)), "sbp_get_finetune_shape")
+        return la.tolist(), wa.tolist()
 
     def partition_batch(self, graphs, prms, slots=None):
         """sbp_partition_batch over DeviceGraphs with one Params each (slots None: the autotuned count for

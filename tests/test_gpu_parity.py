@@ -117,9 +117,61 @@ def test_propose_all_bit_exact(ctx, greedy, B):
     assert mismatches == 0
 
 
+SHAPES = [((8, 8, 32, 64, 128), (3, 3, 3, 3)), ((16, 16, 32, 64, 256), (4, 4, 4, 4)),
+          ((32, 32, 64, 64, 128), (3, 4, 3, 4)), ((64, 64, 64, 64, 256), (4, 3, 4, 3))]
+
+
+@pytest.mark.parametrize("greedy", [True, False])
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "-".join(map(str, s[0] + s[1])))
+def test_propose_all_every_launch_shape(greedy, shape):
+    """Every allowed lanes-per-vertex / wavefronts-per-SIMD choice of the degree-class launches gives the oracle's
+    proposals: degrees span all four lane-group classes and the workgroup class."""
+    c = native.Context(0, seed=4321)
+    c.set_finetune_shape(*shape)
+    assert c.finetune_shape() == (list(shape[0]), list(shape[1]))
+    B = 40
+    g = with_hubs(small_graph(seed=11, N=2500, kmax=900, mean=14), seed=11, hubs=(0, 1), degree=700)
+    deg = np.diff(g.out_off) + np.diff(g.in_off)
+    assert (deg <= 8).any() and ((deg > 8) & (deg <= 32)).any() and ((deg > 32) & (deg <= 128)).any()
+    assert ((deg > 128) & (deg <= 256)).any() and ((deg > 256) & (deg <= 1024)).any() and (deg > 1024).any()
+    rng = np.random.default_rng(5)
+    assign = rng.integers(0, B, g.N).astype(np.int32)
+    assign[:B] = np.arange(B)
+    og, ob, dg, db = both(c, g, B, assign)
+    s, dS, acc = db.propose_all(epoch=3, greedy=greedy)
+    for v in range(g.N):
+        os_, odS, oacc = ob.ft_propose(4321, 3, v, greedy)
+        assert s[v] == os_, v
+        if s[v] != assign[v]:
+            assert dS[v] == odS, (v, dS[v], odS)
+        assert acc[v] == oacc, v
+    # a sweep in two batches groups its items on the device (not the graph's cached grouping)
+    prm = native.params("async_gibbs", batches=2, seed=4321, greedy=greedy)
+    db.async_gibbs_sweep(prm, 4)
+    ob.async_gibbs_sweep(O.params("async_gibbs", batches=2, seed=4321, greedy=greedy), 4)
+    same_state(ob, db)
+    c.close()
+
+
+def test_launch_shape_rejects_unfit_choices(ctx):
+    before = ctx.finetune_shape()
+    for lanes, occ in [((4, 16, 32, 64, 256), (3, 3, 3, 3)), ((8, 4, 32, 64, 256), (3, 3, 3, 3)),
+                       ((8, 16, 16, 64, 256), (3, 3, 3, 3)), ((8, 16, 32, 32, 256), (3, 3, 3, 3)),
+                       ((8, 16, 32, 64, 256), (2, 3, 3, 3)), ((8, 12, 32, 64, 256), (3, 3, 3, 3)),
+                       ((8, 16, 32, 64, 512), (3, 3, 3, 3)), ((8, 16, 32, 64, 64), (3, 3, 3, 3))]:
+        with pytest.raises(native.SbpError):
+            ctx.set_finetune_shape(lanes, occ)
+        assert ctx.finetune_shape() == before
+
+
 def hub_graph(seed=5, N=8000, hubs=(0, 1, 2), degree=3000):
     """A DCSBM plus a few hub vertices with `degree` out-edges and `degree` in-edges to distinct vertices."""
-    g = dcsbm(N, 8, gamma=2.5, kmin=1, kmax=60, mean_degree=6, p_in=0.7, seed=seed)
+    return with_hubs(dcsbm(N, 8, gamma=2.5, kmin=1, kmax=60, mean_degree=6, p_in=0.7, seed=seed), seed, hubs, degree)
+
+
+def with_hubs(g, seed, hubs, degree):
+    """g plus `degree` out-edges and `degree` in-edges to distinct vertices at each vertex of `hubs`."""
+    N = g.N
     src = np.repeat(np.arange(g.N), np.diff(g.out_off)).astype(np.int64)
     dst = g.out_adj.astype(np.int64)
     rng = np.random.default_rng(seed)
@@ -136,11 +188,13 @@ def hub_graph(seed=5, N=8000, hubs=(0, 1, 2), degree=3000):
 
 
 @pytest.mark.parametrize("greedy", [True, False])
-@pytest.mark.parametrize("B", [1000, 3000])
-def test_propose_all_hubs_bit_exact(ctx, greedy, B):
-    """Vertices of 6,000 edges: at B = 3,000 they can have more distinct neighbour blocks than the high-degree
-    launch's LDS table holds, so they use its global-memory table; at B = 1,000 the LDS table."""
-    g = hub_graph()
+@pytest.mark.parametrize("B,degree", [(1000, 3000), (3000, 3000), (7000, 3000), (7000, 900)])
+def test_propose_all_hubs_bit_exact(ctx, greedy, B, degree):
+    """Hub vertices of 2 * degree edges, above the lane-group classes' 256 and the middle class's 1,024.  Up to
+    6,144 blocks the last class's table is indexed by block id in LDS (B = 1,000 and 3,000); above, a hub of 6,000
+    edges can have more distinct neighbour blocks than the LDS hash table holds and uses a global-memory table
+    (B = 7,000, degree 3,000), and one of 1,800 edges the LDS hash table (degree 900)."""
+    g = hub_graph(degree=degree)
     rng = np.random.default_rng(B)
     assign = rng.integers(0, B, g.N).astype(np.int32)
     assign[:B] = np.arange(B)
