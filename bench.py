@@ -118,9 +118,9 @@ def class_kernels(shape, greedy=True, B=None):
     lanes, occ = shape
     g = "true" if greedy else "false"
     names = [f"k_ft_lanes<{lanes[b]}, {g}, false, {occ[b] if greedy else 4}>" for b in range(4)]
-    names.append(f"k_ft_hub<{g}, false, false, {lanes[4]}, 11, false>")
-    dense = "*" if B is None else ("true" if B <= 6144 else "false")  # a whole partition run passes through both
-    names.append(f"k_ft_hub<{g}, false, false, 512, 12, {dense}>")
+    names.append(f"k_ft_hub<{g}, false, false, {lanes[4]}, 11, 0>")
+    mode = "*" if B is None else ("1" if B <= 6144 else "2")  # a whole partition run passes through both
+    names.append(f"k_ft_hub<{g}, false, false, 512, 12, {mode}>")
     return names
 
 
@@ -136,15 +136,17 @@ def add_stats(tot, d):
 
 def committed_traffic(path):
     """HBM bytes per launch of each degree class's proposal launch from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE scaled by a copy-kernel calibration): {class: (kernel, bytes)}, or {} when absent."""
+    (FETCH_SIZE + WRITE_SIZE scaled by a copy-kernel calibration): {class: (kernel, bytes)}, or {} when absent.
+    Lines read: "class <b> traffic_bytes_per_launch=<x> ... kernel=<name>"."""
     out = {}
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("class "):
-                    parts = dict(x.split("=", 1) for x in line.split()[2:] if "=" in x)
-                    out[int(line.split()[1])] = (parts.get("kernel", "").replace("_", " "),
-                                                 float(parts["traffic_bytes_per_launch"]))
+                if not line.startswith("class ") or " kernel=" not in line:
+                    continue
+                head, kernel = line.rstrip("\n").split(" kernel=", 1)
+                fields = dict(x.split("=", 1) for x in head.split()[2:] if "=" in x)
+                out[int(head.split()[1])] = (kernel.strip(), float(fields["traffic_bytes_per_launch"]))
     except (OSError, KeyError, ValueError):
         return {}
     return out

@@ -780,12 +780,17 @@ __device__ __forceinline__ void hub_finish(const FtArgs &a, MoveCtx &m, const T 
     __syncthreads();
 }
 
-// DENSE: the neighbour-block table is indexed by block id (B <= HUB_DENSE_MAX), in LDS.
+// Where a hub's neighbour-block table lives: the LDS hash table (every vertex of the class fits it), the LDS table
+// indexed by block id (B <= HUB_DENSE_MAX), or per vertex the LDS hash table or a global-memory one.  The first two
+// let the compiler address the table as LDS (ds_ atomics) rather than through flat addressing.
+enum HubTable { HUB_LDS = 0, HUB_DENSE = 1, HUB_LDS_OR_GLOBAL = 2 };
+
 template <bool GREEDY, bool NP, bool EVAL = false, int THREADS = HUB_THREADS, int CAPLG = HUB_CAP_LOG2,
-          bool DENSE = false>
+          int MODE = HUB_LDS_OR_GLOBAL>
 __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4)))
 k_ft_hub(FtArgs a, int bucket, EvalIO io = EvalIO{}) {
     constexpr int CAP = 1 << CAPLG;
+    constexpr bool DENSE = MODE == HUB_DENSE;
     constexpr int64_t LDS_INTS = DENSE ? dense_table_ints(HUB_DENSE_MAX, HUB_DENSE_MAX) : table_ints(CAPLG, CAP / 2);
     __shared__ unsigned long long lds64[(LDS_INTS + 1) / 2];
     int *lds = reinterpret_cast<int *>(lds64);
@@ -847,24 +852,27 @@ k_ft_hub(FtArgs a, int bucket, EvalIO io = EvalIO{}) {
             __syncthreads();  // s_m is written again for the next vertex
             continue;
         }
-        if constexpr (DENSE) {
+        if constexpr (MODE == HUB_DENSE) {
             hub_finish<GREEDY, NP, EVAL, THREADS>(a, m, dt, w, s, v, item, io, grp, bytes);
+        } else if constexpr (MODE == HUB_LDS) {
+            hub_finish<GREEDY, NP, EVAL, THREADS>(a, m, lt, w, s, v, item, io, grp, bytes);
         } else {
             const bool in_lds = 2 * (m.k_all < (long long)a.B ? m.k_all : (long long)a.B) <= CAP;
-            hub_finish<GREEDY, NP, EVAL, THREADS>(a, m, in_lds ? lt : gt, w, s, v, item, io, grp, bytes);
+            if (in_lds) hub_finish<GREEDY, NP, EVAL, THREADS>(a, m, lt, w, s, v, item, io, grp, bytes);
+            else hub_finish<GREEDY, NP, EVAL, THREADS>(a, m, gt, w, s, v, item, io, grp, bytes);
         }
     }
     if (threadIdx.x == 0 && bytes) atomicAdd(a.bytes, bytes);
 }
 
-template <int THREADS, int CAPLG, bool DENSE = false>
+template <int THREADS, int CAPLG, int MODE>
 static void launch_hub_t(int blocks, hipStream_t st, const FtArgs &a, int bucket, bool greedy, bool np) {
     if (np) {
-        if (greedy) k_ft_hub<true, true, false, THREADS, CAPLG, DENSE><<<blocks, THREADS, 0, st>>>(a, bucket);
-        else k_ft_hub<false, true, false, THREADS, CAPLG, DENSE><<<blocks, THREADS, 0, st>>>(a, bucket);
+        if (greedy) k_ft_hub<true, true, false, THREADS, CAPLG, MODE><<<blocks, THREADS, 0, st>>>(a, bucket);
+        else k_ft_hub<false, true, false, THREADS, CAPLG, MODE><<<blocks, THREADS, 0, st>>>(a, bucket);
     } else {
-        if (greedy) k_ft_hub<true, false, false, THREADS, CAPLG, DENSE><<<blocks, THREADS, 0, st>>>(a, bucket);
-        else k_ft_hub<false, false, false, THREADS, CAPLG, DENSE><<<blocks, THREADS, 0, st>>>(a, bucket);
+        if (greedy) k_ft_hub<true, false, false, THREADS, CAPLG, MODE><<<blocks, THREADS, 0, st>>>(a, bucket);
+        else k_ft_hub<false, false, false, THREADS, CAPLG, MODE><<<blocks, THREADS, 0, st>>>(a, bucket);
     }
 }
 
@@ -872,12 +880,12 @@ static void launch_hub_t(int blocks, hipStream_t st, const FtArgs &a, int bucket
 // when the blocks fit (B <= HUB_DENSE_MAX), the table indexed by block id.
 static void launch_hub(int blocks, hipStream_t st, const FtArgs &a, int bucket, int threads, bool greedy, bool np) {
     if (bucket == FT_NB - 1) {
-        if (a.B <= HUB_DENSE_MAX) launch_hub_t<HUB_THREADS, HUB_CAP_LOG2, true>(blocks, st, a, bucket, greedy, np);
-        else launch_hub_t<HUB_THREADS, HUB_CAP_LOG2>(blocks, st, a, bucket, greedy, np);
+        if (a.B <= HUB_DENSE_MAX) launch_hub_t<HUB_THREADS, HUB_CAP_LOG2, HUB_DENSE>(blocks, st, a, bucket, greedy, np);
+        else launch_hub_t<HUB_THREADS, HUB_CAP_LOG2, HUB_LDS_OR_GLOBAL>(blocks, st, a, bucket, greedy, np);
     } else if (threads == 128) {
-        launch_hub_t<128, HUB_MID_CAP_LOG2>(blocks, st, a, bucket, greedy, np);
+        launch_hub_t<128, HUB_MID_CAP_LOG2, HUB_LDS>(blocks, st, a, bucket, greedy, np);
     } else {
-        launch_hub_t<256, HUB_MID_CAP_LOG2>(blocks, st, a, bucket, greedy, np);
+        launch_hub_t<256, HUB_MID_CAP_LOG2, HUB_LDS>(blocks, st, a, bucket, greedy, np);
     }
 }
 

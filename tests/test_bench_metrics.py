@@ -22,3 +22,23 @@ def test_nmi_of_one_block_is_null():
     assert bench.nmi(np.arange(10) % 3, np.zeros(10)) is None
     assert bench.mean_nmi([None, 0.5, 1.0]) == 0.75
     assert bench.mean_nmi([None]) is None
+
+
+def test_roofline_reads_the_committed_per_class_traffic():
+    """The bench's roofline pairs each degree class's launch with the committed rocprofv3 traffic of the same
+    kernel, and picks the class with the largest device time as the headline figure."""
+    traffic = bench.committed_traffic(bench.TRAFFIC_FILE)
+    assert sorted(traffic) == [0, 1, 2, 3, 4, 5]
+    shape = ([8, 8, 32, 64, 256], [4, 4, 4, 4])
+    names = bench.class_kernels(shape, True, B=5000)
+    assert [traffic[b][0] for b in range(6)] == names
+    tot = {"ft_class_launches": [2, 2, 2, 2, 2, 0], "ft_class_ms": [6.0, 8.0, 4.0, 2.0, 3.0, 0.0],
+           "ft_class_bytes": [6e9, 8e10, 4e9, 2e9, 3e9, 0]}
+    r = bench.roofline(tot, shape, traffic_file=bench.TRAFFIC_FILE, B=5000)
+    assert r["kernel"] == names[1] and r["degree_class"] == "8 < k <= 32"
+    assert abs(r["achieved"] - 4e10 / 4e-3 / 1e9) < 1e-6 and abs(r["frac"] - r["achieved"] / 8000.0) < 1e-12
+    assert r["traffic"] == traffic[1][1] and abs(r["hbm_achieved"] - traffic[1][1] / 4e-3 / 1e9) < 1e-6
+    assert set(r["classes"]) == {"0", "1", "2", "3", "4"}
+    # another shape names other kernels: the committed traffic no longer applies to them
+    r2 = bench.roofline(tot, ([16, 16, 32, 64, 128], [4, 4, 4, 4]), traffic_file=bench.TRAFFIC_FILE, B=5000)
+    assert r2["traffic"] is None and r2["hbm_frac"] is None
