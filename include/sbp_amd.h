@@ -338,6 +338,14 @@ int sbp_evaluate(int32_t N, const int32_t *true_assignment, const int32_t *assig
 
 /* Device logf over float bit patterns [first_bits, first_bits + count) (checks the restated glibc logf). */
 int sbp_debug_logf(sbp_ctx ctx, uint32_t first_bits, uint32_t count, float *out);
+/* The library's device primitives on host arrays (checks them; no counterpart in the reference): the stable radix
+ * sort of n 64-bit keys (key_bits 64) or 32-bit keys (key_bits 32, in the low words of keys) by bits
+ * [begin_bit, end_bit), carrying vals when given (NULL: keys only); the int64 prefix sum (inclusive or not);
+ * run-length encoding of n sorted 64-bit keys (uniq and counts hold n entries; *num_runs set). */
+int sbp_debug_sort(sbp_ctx ctx, int64_t n, int32_t key_bits, const uint64_t *keys, const int32_t *vals,
+                   int32_t begin_bit, int32_t end_bit, uint64_t *keys_out, int32_t *vals_out);
+int sbp_debug_scan(sbp_ctx ctx, int64_t n, const int64_t *in, int32_t inclusive, int64_t *out);
+int sbp_debug_rle(sbp_ctx ctx, int64_t n, const uint64_t *keys, uint64_t *uniq, int32_t *counts, int64_t *num_runs);
 
 #ifdef __cplusplus
 }

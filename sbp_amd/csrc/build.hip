@@ -2,15 +2,15 @@
 //
 // Replaces Blockmodel::initialize_edge_counts (src/blockmodel/blockmodel.cpp:193-276), where every
 // OpenMP thread scans all N vertices and inserts into robin_map rows.  Here the edge list is turned
-// into packed (row, col) keys, radix-sorted and run-length encoded; row pointers come from key
-// boundaries, and the transpose from a second sort of the nnz unique cells.  The result is exact
-// integer state, identical to the reference's after any sequence of incremental updates.
-#include <hipcub/hipcub.hpp>
-
+// into packed (row, col) keys, radix-sorted and run-length encoded (the library's own primitives,
+// primitives.hpp); row pointers come from key boundaries, and the transpose from a second sort of the
+// nnz unique cells.  The result is exact integer state, identical to the reference's after any sequence
+// of incremental updates.
 #include <algorithm>
 
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "primitives.hpp"
 
 namespace sbpimpl {
 
@@ -338,12 +338,13 @@ void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm, bool keep_hist) {
     k_make_keys<<<grid_for(E), 256, 0, st>>>(E, g->edge_src, g->out_adj, bm->assign, cbits, keys);
     SBP_HIP(hipGetLastError());
     size_t tmp_a = 0, tmp_b = 0;
-    SBP_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_a, keys, sorted, (int)E, 0, end_bit, st));
-    SBP_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, tmp_b, sorted, uniq, counts, num_runs, (int)E, st));
+    const int32_t *no_vals = nullptr;
+    SBP_HIP(sbpprim::radix_sort(nullptr, tmp_a, keys, sorted, no_vals, (int32_t *)nullptr, E, 0, end_bit, st));
+    SBP_HIP(sbpprim::run_length_encode(nullptr, tmp_b, sorted, uniq, counts, num_runs, E, st));
     size_t tmp = tmp_a > tmp_b ? tmp_a : tmp_b;
     void *tmp_ptr = ctx->scratch[S_SORT_TMP].get(tmp);
-    SBP_HIP(hipcub::DeviceRadixSort::SortKeys(tmp_ptr, tmp_a, keys, sorted, (int)E, 0, end_bit, st));
-    SBP_HIP(hipcub::DeviceRunLengthEncode::Encode(tmp_ptr, tmp_b, sorted, uniq, counts, num_runs, (int)E, st));
+    SBP_HIP(sbpprim::radix_sort(tmp_ptr, tmp_a, keys, sorted, no_vals, (int32_t *)nullptr, E, 0, end_bit, st));
+    SBP_HIP(sbpprim::run_length_encode(tmp_ptr, tmp_b, sorted, uniq, counts, num_runs, E, st));
     int64_t nnz = 0;
     read_small(ctx, {{&nnz, num_runs, sizeof(int64_t)}});
     ensure_cells(bm, nnz);
@@ -356,11 +357,11 @@ void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm, bool keep_hist) {
     auto *tvals_sorted = ctx->scratch[S_VALS_B].as<int32_t>(nnz);
     k_transpose_keys<<<grid_for(nnz), 256, 0, st>>>(nnz, cbits, uniq, tkeys);
     size_t tmp_c = 0;
-    SBP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_c, tkeys, tkeys_sorted, counts, tvals_sorted, (int)nnz, 0,
-                                               end_bit, st));
+    SBP_HIP(sbpprim::radix_sort(nullptr, tmp_c, tkeys, tkeys_sorted, (const int32_t *)counts, tvals_sorted, nnz, 0,
+                                end_bit, st));
     tmp_ptr = ctx->scratch[S_SORT_TMP].get(tmp_c);
-    SBP_HIP(hipcub::DeviceRadixSort::SortPairs(tmp_ptr, tmp_c, tkeys, tkeys_sorted, counts, tvals_sorted, (int)nnz, 0,
-                                               end_bit, st));
+    SBP_HIP(sbpprim::radix_sort(tmp_ptr, tmp_c, tkeys, tkeys_sorted, (const int32_t *)counts, tvals_sorted, nnz, 0,
+                                end_bit, st));
     k_fill_major<<<grid_for(nnz), 256, 0, st>>>(nnz, B, cbits, tkeys_sorted, tvals_sorted, bm->col_ptr, bm->col_cells);
     launch_cum(ctx, bm, bm->d_out, bm->d_in);  // cumulative counts, and the degrees as row / column totals
     k_dtot_sizes<<<grid_for(B), 256, 0, st>>>(B, bm->row_ptr, bm->row_cells, bm->d_out, bm->d_in, bm->d_tot);

@@ -16,6 +16,7 @@
 
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "primitives.hpp"
 #include "carry.hpp"
 #include "host_blockmodel.hpp"
 #include "snowball.hpp"
@@ -991,6 +992,103 @@ int sbp_get_finetune_shape(sbp_ctx ctx, int32_t lanes[5], int32_t waves_per_simd
         std::copy(g_default_occ, g_default_occ + 4, waves_per_simd);
     }
     return SBP_OK;
+}
+
+// Device buffers of one debug call, freed on return.
+struct DebugBufs {
+    std::vector<void *> p;
+    void *get(size_t bytes) {
+        void *x = nullptr;
+        SBP_HIP(hipMalloc(&x, bytes > 0 ? bytes : 1));
+        p.push_back(x);
+        return x;
+    }
+    ~DebugBufs() {
+        for (void *x : p) (void)hipFree(x);
+    }
+};
+
+int sbp_debug_sort(sbp_ctx ctx, int64_t n, int32_t key_bits, const uint64_t *keys, const int32_t *vals,
+                   int32_t begin_bit, int32_t end_bit, uint64_t *keys_out, int32_t *vals_out) {
+    if (!ctx || n < 0 || (n > 0 && (!keys || !keys_out)) || (vals && !vals_out) || (key_bits != 32 && key_bits != 64) ||
+        begin_bit < 0 || end_bit > key_bits || begin_bit > end_bit)
+        return SBP_E_ARG;
+    SBP_API_BEGIN
+    hipStream_t st = ctx->stream;
+    DebugBufs b;
+    const size_t kb = (size_t)n * (key_bits / 8);
+    void *dk = b.get(kb), *dko = b.get(kb);
+    int32_t *dv = vals ? static_cast<int32_t *>(b.get(sizeof(int32_t) * n)) : nullptr;
+    int32_t *dvo = vals ? static_cast<int32_t *>(b.get(sizeof(int32_t) * n)) : nullptr;
+    std::vector<uint32_t> k32;
+    if (key_bits == 32) {
+        k32.resize((size_t)n);
+        for (int64_t i = 0; i < n; ++i) k32[i] = (uint32_t)keys[i];
+        SBP_HIP(hipMemcpy(dk, k32.data(), kb, hipMemcpyHostToDevice));
+    } else if (n > 0) {
+        SBP_HIP(hipMemcpy(dk, keys, kb, hipMemcpyHostToDevice));
+    }
+    if (vals && n > 0) SBP_HIP(hipMemcpy(dv, vals, sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    size_t tmp = 0;
+    if (key_bits == 64) {
+        auto *a = static_cast<const unsigned long long *>(dk);
+        auto *o = static_cast<unsigned long long *>(dko);
+        SBP_HIP(sbpprim::radix_sort(nullptr, tmp, a, o, (const int32_t *)dv, dvo, n, begin_bit, end_bit, st));
+        SBP_HIP(sbpprim::radix_sort(b.get(tmp), tmp, a, o, (const int32_t *)dv, dvo, n, begin_bit, end_bit, st));
+    } else {
+        auto *a = static_cast<const uint32_t *>(dk);
+        auto *o = static_cast<uint32_t *>(dko);
+        SBP_HIP(sbpprim::radix_sort(nullptr, tmp, a, o, (const int32_t *)dv, dvo, n, begin_bit, end_bit, st));
+        SBP_HIP(sbpprim::radix_sort(b.get(tmp), tmp, a, o, (const int32_t *)dv, dvo, n, begin_bit, end_bit, st));
+    }
+    SBP_HIP(hipStreamSynchronize(st));
+    if (key_bits == 32) {
+        SBP_HIP(hipMemcpy(k32.data(), dko, kb, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) keys_out[i] = k32[i];
+    } else if (n > 0) {
+        SBP_HIP(hipMemcpy(keys_out, dko, kb, hipMemcpyDeviceToHost));
+    }
+    if (vals && n > 0) SBP_HIP(hipMemcpy(vals_out, dvo, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    SBP_API_END(ctx)
+}
+
+int sbp_debug_scan(sbp_ctx ctx, int64_t n, const int64_t *in, int32_t inclusive, int64_t *out) {
+    if (!ctx || n < 0 || (n > 0 && (!in || !out))) return SBP_E_ARG;
+    SBP_API_BEGIN
+    hipStream_t st = ctx->stream;
+    DebugBufs b;
+    int64_t *di = static_cast<int64_t *>(b.get(sizeof(int64_t) * n));
+    int64_t *dout = static_cast<int64_t *>(b.get(sizeof(int64_t) * n));
+    if (n > 0) SBP_HIP(hipMemcpy(di, in, sizeof(int64_t) * n, hipMemcpyHostToDevice));
+    size_t tmp = 0;
+    SBP_HIP(sbpprim::scan(nullptr, tmp, (const int64_t *)di, dout, n, inclusive != 0, st));
+    SBP_HIP(sbpprim::scan(b.get(tmp), tmp, (const int64_t *)di, dout, n, inclusive != 0, st));
+    SBP_HIP(hipStreamSynchronize(st));
+    if (n > 0) SBP_HIP(hipMemcpy(out, dout, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
+    SBP_API_END(ctx)
+}
+
+int sbp_debug_rle(sbp_ctx ctx, int64_t n, const uint64_t *keys, uint64_t *uniq, int32_t *counts, int64_t *num_runs) {
+    if (!ctx || n < 0 || n >= (int64_t(1) << 31) || !num_runs || (n > 0 && (!keys || !uniq || !counts)))
+        return SBP_E_ARG;
+    SBP_API_BEGIN
+    hipStream_t st = ctx->stream;
+    DebugBufs b;
+    auto *dk = static_cast<unsigned long long *>(b.get(sizeof(uint64_t) * n));
+    auto *du = static_cast<unsigned long long *>(b.get(sizeof(uint64_t) * n));
+    int32_t *dc = static_cast<int32_t *>(b.get(sizeof(int32_t) * n));
+    int64_t *dn = static_cast<int64_t *>(b.get(sizeof(int64_t)));
+    if (n > 0) SBP_HIP(hipMemcpy(dk, keys, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+    size_t tmp = 0;
+    SBP_HIP(sbpprim::run_length_encode(nullptr, tmp, (const unsigned long long *)dk, du, dc, dn, n, st));
+    SBP_HIP(sbpprim::run_length_encode(b.get(tmp), tmp, (const unsigned long long *)dk, du, dc, dn, n, st));
+    SBP_HIP(hipStreamSynchronize(st));
+    SBP_HIP(hipMemcpy(num_runs, dn, sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (*num_runs > 0) {
+        SBP_HIP(hipMemcpy(uniq, du, sizeof(uint64_t) * *num_runs, hipMemcpyDeviceToHost));
+        SBP_HIP(hipMemcpy(counts, dc, sizeof(int32_t) * *num_runs, hipMemcpyDeviceToHost));
+    }
+    SBP_API_END(ctx)
 }
 
 int sbp_debug_logf(sbp_ctx ctx, uint32_t first_bits, uint32_t count, float *out) {

@@ -26,7 +26,6 @@
 // accepted Deltas in place (Blockmodel::move_vertex(Vertex, const Delta&, ...), blockmodel.cpp:408-435) to
 // a CSR whose rows and columns carry slack.  When a row or column runs out of slack the kernel stops
 // before that vertex; the host re-lays the matrix out with more slack and resumes.
-#include <hipcub/hipcub.hpp>
 #include <math.h>
 
 #include <algorithm>
@@ -36,6 +35,7 @@
 #include "device_table.hpp"
 #include "internal.hpp"
 #include "np_entropy.hpp"
+#include "primitives.hpp"
 
 namespace sbpimpl {
 using namespace sbpdev;
@@ -1389,10 +1389,16 @@ int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_
         k_make_events<<<grid_of(N), 256, 0, st>>>(N, d_move_s, bm->assign, risk, events, counters + 1, acc, acc + N);
         unsigned long long n_ev = 0;
         read_small(ctx, {{&n_ev, counters + 1, sizeof(n_ev)}});
+        // keys (block << 32) | vertex: the bits above 32 + log2(B) are zero
+        int end_bit = 33;
+        while (end_bit < 64 && (int64_t(1) << (end_bit - 32)) < B) ++end_bit;
         size_t tmp = 0;
-        SBP_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, events, events_sorted, (int)n_ev, 0, 64, st));
+        const int32_t *no_vals = nullptr;
+        SBP_HIP(sbpprim::radix_sort(nullptr, tmp, events, events_sorted, no_vals, (int32_t *)nullptr, (int64_t)n_ev, 0,
+                                    end_bit, st));
         void *tp = ctx->scratch[S_SORT_TMP].get(tmp);
-        SBP_HIP(hipcub::DeviceRadixSort::SortKeys(tp, tmp, events, events_sorted, (int)n_ev, 0, 64, st));
+        SBP_HIP(sbpprim::radix_sort(tp, tmp, events, events_sorted, no_vals, (int32_t *)nullptr, (int64_t)n_ev, 0,
+                                    end_bit, st));
         k_segments<<<grid_of((int64_t)n_ev), 256, 0, st>>>((int64_t)n_ev, events_sorted, seg_start, seg_end);
         int32_t *prev = acc, *next = acc + N;
         // monotone iteration; every round fixes at least the earliest undecided move, so it terminates
@@ -1450,9 +1456,10 @@ void shuffle_keys(sbp_ctx ctx, const int32_t *d_items, int64_t n, uint32_t epoch
     auto *kb = ctx->scratch[S_KEYS_B].as<unsigned long long>(n);
     k_shuffle_keys<<<grid_of(n), 256, 0, st>>>(n, d_items, ctx->seed, epoch, ka);
     size_t tmp = 0;
-    SBP_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, ka, kb, (int)n, 0, 64, st));
+    const int32_t *no_vals = nullptr;
+    SBP_HIP(sbpprim::radix_sort(nullptr, tmp, ka, kb, no_vals, (int32_t *)nullptr, n, 0, 64, st));
     void *tp = ctx->scratch[S_SORT_TMP].get(tmp);
-    SBP_HIP(hipcub::DeviceRadixSort::SortKeys(tp, tmp, ka, kb, (int)n, 0, 64, st));
+    SBP_HIP(sbpprim::radix_sort(tp, tmp, ka, kb, no_vals, (int32_t *)nullptr, n, 0, 64, st));
     k_low_word<<<grid_of(n), 256, 0, st>>>(n, kb, d_out);
     SBP_HIP(hipGetLastError());
 }
@@ -2027,13 +2034,13 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         k_slack_caps<<<grid_of(B), 256, 0, st>>>(B, bm->row_ptr, h.rlen, h.rcap, h.rboost);
         k_slack_caps<<<grid_of(B), 256, 0, st>>>(B, bm->col_ptr, h.clen, h.ccap, h.cboost);
         size_t tmp = 0;
-        SBP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, h.rcap, h.rstart, B + 1, st));
+        SBP_HIP(sbpprim::scan(nullptr, tmp, (const int32_t *)h.rcap, h.rstart, (int64_t)B + 1, false, st));
         void *tp = ctx->scratch[S_SORT_TMP].get(tmp);
         // capacities have B entries; the (B+1)-th scan input is the zeroed slot past the end
         SBP_HIP(hipMemsetAsync(h.rcap + B, 0, sizeof(int32_t), st));
         SBP_HIP(hipMemsetAsync(h.ccap + B, 0, sizeof(int32_t), st));
-        SBP_HIP(hipcub::DeviceScan::ExclusiveSum(tp, tmp, h.rcap, h.rstart, B + 1, st));
-        SBP_HIP(hipcub::DeviceScan::ExclusiveSum(tp, tmp, h.ccap, h.cstart, B + 1, st));
+        SBP_HIP(sbpprim::scan(tp, tmp, (const int32_t *)h.rcap, h.rstart, (int64_t)B + 1, false, st));
+        SBP_HIP(sbpprim::scan(tp, tmp, (const int32_t *)h.ccap, h.cstart, (int64_t)B + 1, false, st));
         int32_t tot[2] = {0, 0};
         read_small(ctx, {{&tot[0], h.rstart + B, sizeof(int32_t)}, {&tot[1], h.cstart + B, sizeof(int32_t)}});
         int2 *rc = ctx->scratch[S_KEYS_A].as<int2>((size_t)tot[0] + 1);
@@ -2083,10 +2090,10 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         int32_t *rp = bm->row_ptr, *cp = bm->col_ptr;
         SBP_HIP(hipMemsetAsync(h.rlen + B, 0, sizeof(int32_t), st));
         SBP_HIP(hipMemsetAsync(h.clen + B, 0, sizeof(int32_t), st));
-        SBP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, h.rlen, rp, B + 1, st));
+        SBP_HIP(sbpprim::scan(nullptr, tmp2, (const int32_t *)h.rlen, rp, (int64_t)B + 1, false, st));
         tp = ctx->scratch[S_SORT_TMP].get(tmp2);
-        SBP_HIP(hipcub::DeviceScan::ExclusiveSum(tp, tmp2, h.rlen, rp, B + 1, st));
-        SBP_HIP(hipcub::DeviceScan::ExclusiveSum(tp, tmp2, h.clen, cp, B + 1, st));
+        SBP_HIP(sbpprim::scan(tp, tmp2, (const int32_t *)h.rlen, rp, (int64_t)B + 1, false, st));
+        SBP_HIP(sbpprim::scan(tp, tmp2, (const int32_t *)h.clen, cp, (int64_t)B + 1, false, st));
         int32_t nnz[2] = {0, 0};
         read_small(ctx, {{&nnz[0], rp + B, sizeof(int32_t)}, {&nnz[1], cp + B, sizeof(int32_t)}});
         kernel_ms += (float)elapsed_ms(ctx, ctx->ev[4], ctx->ev[5]);

@@ -15,7 +15,6 @@
 // unsampled neighbours; those reaching zero form the next frontier.  Each vertex is computed exactly once,
 // from exactly the neighbour blocks the sequential loop sees: its lower-id unsampled neighbours are final
 // and its higher-id ones are still unassigned, because they wait for it.
-#include <hipcub/hipcub.hpp>
 #include <math.h>
 
 #include <algorithm>
@@ -25,6 +24,7 @@
 #include "device_common.hpp"
 #include "device_table.hpp"
 #include "internal.hpp"
+#include "primitives.hpp"
 #include "snowball.hpp"
 
 namespace sbpimpl {
@@ -278,9 +278,10 @@ __global__ void __launch_bounds__(EX_WAVES * WAVE) k_extend_level(ExArgs a) {
 // Sorts n 64-bit keys ascending into `out` (stable; keys are unique here).
 void sort_keys(hipStream_t st, const uint64_t *in, uint64_t *out, int64_t n) {
     size_t tmp = 0;
-    SBP_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, in, out, (int)n, 0, 64, st));
+    const int32_t *no_vals = nullptr;
+    SBP_HIP(sbpprim::radix_sort(nullptr, tmp, in, out, no_vals, (int32_t *)nullptr, n, 0, 64, st));
     DevBuf<char> t(tmp);
-    SBP_HIP(hipcub::DeviceRadixSort::SortKeys(t.p, tmp, in, out, (int)n, 0, 64, st));
+    SBP_HIP(sbpprim::radix_sort((void *)t.p, tmp, in, out, no_vals, (int32_t *)nullptr, n, 0, 64, st));
     SBP_HIP(hipStreamSynchronize(st));
 }
 
@@ -390,11 +391,11 @@ sbp_graph sample_graph(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, u
             DevBuf<int32_t> eidx(E), eidx_sorted(E);
             k_product_keys<<<grid_for(E), GB, 0, st>>>(E, g->edge_src, g->out_adj, g->degree, ekeys.p, eidx.p);
             size_t tmp = 0;
-            SBP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ekeys.p, esorted.p, eidx.p, eidx_sorted.p, (int)E, 0,
-                                                       64, st));
+            SBP_HIP(sbpprim::radix_sort(nullptr, tmp, (const uint64_t *)ekeys.p, esorted.p, (const int32_t *)eidx.p,
+                                        eidx_sorted.p, E, 0, 64, st));
             DevBuf<char> t(tmp);
-            SBP_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, ekeys.p, esorted.p, eidx.p, eidx_sorted.p, (int)E, 0, 64,
-                                                       st));
+            SBP_HIP(sbpprim::radix_sort((void *)t.p, tmp, (const uint64_t *)ekeys.p, esorted.p, (const int32_t *)eidx.p,
+                                        eidx_sorted.p, E, 0, 64, st));
             k_first_position<<<grid_for(E), GB, 0, st>>>(E, eidx_sorted.p, g->edge_src, g->out_adj, first.p);
         }
         k_first_keys<<<grid_for(N), GB, 0, st>>>(N, first.p, keys.p);
@@ -448,9 +449,9 @@ sbp_graph sample_graph(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, u
     if (E > 0) {
         k_keep_edges<<<grid_for(E), GB, 0, st>>>(E, g->edge_src, g->out_adj, s->parent_map, keep.p);
         size_t tmp = 0;
-        SBP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, keep.p, pos.p, (int)(E + 0), st));
+        SBP_HIP(sbpprim::scan(nullptr, tmp, (const int32_t *)keep.p, pos.p, E, false, st));
         DevBuf<char> t(tmp);
-        SBP_HIP(hipcub::DeviceScan::ExclusiveSum(t.p, tmp, keep.p, pos.p, (int)E, st));
+        SBP_HIP(sbpprim::scan((void *)t.p, tmp, (const int32_t *)keep.p, pos.p, E, false, st));
         int64_t last_pos = 0;
         int32_t last_keep = 0;
         SBP_HIP(hipMemcpyAsync(&last_pos, pos.p + E - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -472,14 +473,18 @@ sbp_graph sample_graph(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, u
         SBP_HIP(hipMemsetAsync(icnt.p, 0, sizeof(int64_t) * (n + 1), st));
         k_compact_edges<<<grid_for(E), GB, 0, st>>>(E, g->edge_src, g->out_adj, s->parent_map, keep.p, pos.p, ssrc.p,
                                                      sdst.p, ocnt.p, icnt.p);
+        // vertex ids are non-negative, so their int32 bits sort as unsigned
+        auto *usrc = reinterpret_cast<const uint32_t *>(ssrc.p);
+        auto *udst = reinterpret_cast<const uint32_t *>(sdst.p);
+        auto *ukey = reinterpret_cast<uint32_t *>(tmpk.p);
         size_t tmp = 0, tmp2 = 0;
-        SBP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ssrc.p, tmpk.p, sdst.p, s->out_adj, (int)Es, 0, 32, st));
-        SBP_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp2, ocnt.p, s->out_off, (int)(n + 1), st));
+        SBP_HIP(sbpprim::radix_sort(nullptr, tmp, usrc, ukey, (const int32_t *)sdst.p, s->out_adj, Es, 0, 32, st));
+        SBP_HIP(sbpprim::scan(nullptr, tmp2, (const int64_t *)ocnt.p, s->out_off, n + 1, true, st));
         DevBuf<char> t(std::max(tmp, tmp2));
-        SBP_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, ssrc.p, tmpk.p, sdst.p, s->out_adj, (int)Es, 0, 32, st));
-        SBP_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, sdst.p, tmpk.p, ssrc.p, s->in_adj, (int)Es, 0, 32, st));
-        SBP_HIP(hipcub::DeviceScan::InclusiveSum(t.p, tmp2, ocnt.p, s->out_off, (int)(n + 1), st));
-        SBP_HIP(hipcub::DeviceScan::InclusiveSum(t.p, tmp2, icnt.p, s->in_off, (int)(n + 1), st));
+        SBP_HIP(sbpprim::radix_sort((void *)t.p, tmp, usrc, ukey, (const int32_t *)sdst.p, s->out_adj, Es, 0, 32, st));
+        SBP_HIP(sbpprim::radix_sort((void *)t.p, tmp, udst, ukey, (const int32_t *)ssrc.p, s->in_adj, Es, 0, 32, st));
+        SBP_HIP(sbpprim::scan((void *)t.p, tmp2, (const int64_t *)ocnt.p, s->out_off, n + 1, true, st));
+        SBP_HIP(sbpprim::scan((void *)t.p, tmp2, (const int64_t *)icnt.p, s->in_off, n + 1, true, st));
         SBP_HIP(hipStreamSynchronize(st));
     }
     graph_finish(ctx, s.get());

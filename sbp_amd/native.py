@@ -31,7 +31,7 @@ EXPORTS = [
     "sbp_server_create", "sbp_server_last_error", "sbp_server_submit", "sbp_server_wait", "sbp_server_info",
     "sbp_server_destroy",
     "sbp_evaluate_table", "sbp_evaluate",
-    "sbp_debug_logf",
+    "sbp_debug_logf", "sbp_debug_sort", "sbp_debug_scan", "sbp_debug_rle",
 ]
 
 
@@ -135,6 +135,9 @@ def lib():
             "sbp_merge_blocks": [vp, vp, vp, u32, i64, i32, vp, S],
             "sbp_partition": [vp, vp, P, vp, vp, S],
             "sbp_debug_logf": [vp, u32, u32, vp],
+            "sbp_debug_sort": [vp, i64, i32, vp, vp, i32, i32, vp, vp],
+            "sbp_debug_scan": [vp, i64, vp, i32, vp],
+            "sbp_debug_rle": [vp, i64, vp, vp, vp, vp],
             "sbp_ctx_memory": [vp, vp, vp],
             "sbp_ctx_trim": [vp],
             "sbp_set_carry_threads": [vp, i32],
@@ -210,6 +213,31 @@ class Context:
         out = np.zeros(count, np.float32)
         self.check(lib().sbp_debug_logf(self.h, first_bits, count, _p(out)), "sbp_debug_logf")
         return out
+
+    def debug_sort(self, keys, vals=None, begin_bit=0, end_bit=64, key_bits=64):
+        """The library's stable radix sort of keys (uint64 array) by bits [begin_bit, end_bit), carrying vals."""
+        keys = np.ascontiguousarray(keys, np.uint64)
+        n = len(keys)
+        ko = np.zeros(n, np.uint64)
+        v = None if vals is None else np.ascontiguousarray(vals, np.int32)
+        vo = None if vals is None else np.zeros(n, np.int32)
+        self.check(lib().sbp_debug_sort(self.h, n, key_bits, _p(keys), None if v is None else _p(v), begin_bit, end_bit,
+                                        _p(ko), None if vo is None else _p(vo)), "sbp_debug_sort")
+        return ko, vo
+
+    def debug_scan(self, values, inclusive=False):
+        x = np.ascontiguousarray(values, np.int64)
+        out = np.zeros(len(x), np.int64)
+        self.check(lib().sbp_debug_scan(self.h, len(x), _p(x), int(inclusive), _p(out)), "sbp_debug_scan")
+        return out
+
+    def debug_rle(self, keys):
+        k = np.ascontiguousarray(keys, np.uint64)
+        u = np.zeros(len(k), np.uint64)
+        c = np.zeros(len(k), np.int32)
+        m = C.c_int64()
+        self.check(lib().sbp_debug_rle(self.h, len(k), _p(k), _p(u), _p(c), C.byref(m)), "sbp_debug_rle")
+        return u[:m.value], c[:m.value]
 
     def memory(self):
         """(current, peak) device bytes of the context (sbp_ctx_memory)."""
