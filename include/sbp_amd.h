@@ -89,6 +89,8 @@ typedef struct {
     uint64_t ft_class_bytes[6];
     int64_t ft_class_launches[6];
     double ft_class_ms[6];
+    int64_t recorded_sweeps;     /* asynchronous-Gibbs sweeps run as replays of a recorded HIP graph (their
+                                  * per-class launch times are not taken; finetune_ms holds the replays' time) */
 } sbp_stats;
 
 /* ----------------------------------------------------------------- context */
@@ -215,6 +217,13 @@ int sbp_ft_evaluate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int64_t n, const int32_
  * (src/blockmodel/blockmodel.cpp:437-494). */
 int sbp_async_gibbs_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, uint32_t epoch,
                           sbp_stats *stats);
+/* The same sweep followed by the MDL that closes an iteration of finetune::asynchronous_gibbs (src/finetune.cpp:
+ * 50-88), run as a replay of a HIP graph recorded on the first call for this blockmodel and parameters (recorded
+ * again when they change): one graph launch and one read per sweep.  Results are those of sbp_async_gibbs_sweep
+ * then sbp_mdl.  Parametric blockmodels only (SBP_E_ARG otherwise).  The partition driver runs its
+ * asynchronous-Gibbs finetune this way unless SBP_AMD_RECORDED_SWEEPS=0. */
+int sbp_async_gibbs_sweep_recorded(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, uint32_t epoch,
+                                   double *mdl_out, sbp_stats *stats);
 /* One sweep of sequential Metropolis-Hastings (src/finetune.cpp:773-820; propose_move :842-869).
  * ordered = 1 visits vertices in id order (finetune_assignment); sum_dS = sum of accepted delta-MDL. */
 int sbp_mh_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, uint32_t epoch, int32_t ordered,

@@ -31,8 +31,11 @@ __global__ void k_make_keys(int64_t E, const int32_t *__restrict__ src, const in
 // One thread per unique cell: write cells, row pointers at key boundaries, and degree sums.
 // Cells and row pointers from sorted unique (major, minor) keys and their counts.  The degrees are the row
 // totals, taken afterwards by k_cum (same-address atomics here would serialize whole waves when B is small).
-__global__ void k_fill_major(int64_t nnz, int32_t B, int cbits, const unsigned long long *__restrict__ uniq,
-                             const int32_t *__restrict__ counts, int32_t *__restrict__ ptr, int2 *__restrict__ cells) {
+// dnnz: when given, nnz is read there (a recorded rebuild, launched for the largest nnz).
+__global__ void k_fill_major(int64_t nnz, const int64_t *__restrict__ dnnz, int32_t B, int cbits,
+                             const unsigned long long *__restrict__ uniq, const int32_t *__restrict__ counts,
+                             int32_t *__restrict__ ptr, int2 *__restrict__ cells) {
+    if (dnnz) nnz = *dnnz;
     const unsigned long long mask = (1ull << cbits) - 1;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
         int32_t a = (int32_t)(uniq[i] >> cbits), b = (int32_t)(uniq[i] & mask), cnt = counts[i];
@@ -42,10 +45,13 @@ __global__ void k_fill_major(int64_t nnz, int32_t B, int cbits, const unsigned l
         if (i == nnz - 1)
             for (int32_t x = a + 1; x <= B; ++x) ptr[x] = (int32_t)nnz;
     }
+    if (nnz == 0 && blockIdx.x == 0)  // no cells: every row empty
+        for (int32_t x = threadIdx.x; x <= B; x += blockDim.x) ptr[x] = 0;
 }
 
-__global__ void k_transpose_keys(int64_t nnz, int cbits, const unsigned long long *__restrict__ uniq,
-                                 unsigned long long *__restrict__ tkeys) {
+__global__ void k_transpose_keys(int64_t nnz, const int64_t *__restrict__ dnnz, int cbits,
+                                 const unsigned long long *__restrict__ uniq, unsigned long long *__restrict__ tkeys) {
+    if (dnnz) nnz = *dnnz;
     const unsigned long long mask = (1ull << cbits) - 1;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
         unsigned long long r = uniq[i] >> cbits, c = uniq[i] & mask;
@@ -121,7 +127,7 @@ void bm_release(sbp_bm bm) {
         for (void *p : ptrs) bm->pool->release(p);
     } else {
         for (void *p : ptrs)
-            if (p) (void)hipFree(p);
+            if (p) (void)dev_free(p);
     }
     bm->cell_hash = nullptr;
     bm->hash_valid = false;
@@ -215,26 +221,35 @@ __global__ void k_hash_cells(int32_t B, const int32_t *__restrict__ ptr, const i
     }
 }
 
-void bm_ensure_hash(sbp_ctx ctx, sbp_bm bm) {
-    if (bm->hash_valid) return;
-    hipStream_t st = ctx->stream;
+static int hash_log2_for(int64_t nnz) {
     int lg = 10;
-    while ((int64_t(1) << lg) < 2 * bm->nnz + 2) ++lg;
-    if (!bm->cell_hash || bm->hash_log2 < lg) {
+    while ((int64_t(1) << lg) < 2 * nnz + 2) ++lg;
+    return lg;
+}
+
+void bm_ensure_hash(sbp_ctx ctx, sbp_bm bm, bool capture) {
+    if (bm->hash_valid && !capture) return;
+    hipStream_t st = ctx->stream;
+    const int lg_need = hash_log2_for(bm->nnz);
+    if (!bm->cell_hash || bm->hash_log2 < lg_need) {
+        SBP_CHECK(!capture, SBP_E_STATE, "cell hash not reserved for a recorded sweep");
         DevicePool &pool = pool_of(bm);
         pool.release(bm->cell_hash);
-        bm->cell_hash = static_cast<sbpdev::CellSlot *>(pool.alloc(sizeof(sbpdev::CellSlot) << lg));
-        bm->hash_log2 = lg;
+        bm->cell_hash = static_cast<sbpdev::CellSlot *>(pool.alloc(sizeof(sbpdev::CellSlot) << lg_need));
+        bm->hash_log2 = lg_need;
     }
-    lg = bm->hash_log2;
-    SBP_HIP(hipMemsetAsync(bm->cell_hash, 0xff, sizeof(sbpdev::CellSlot) << lg, st));
-    if (bm->nnz > 0) {
+    const int lg = bm->hash_log2;
+    // every key to CELL_EMPTY (all ones), with a kernel: no memset nodes in a recorded graph
+    SBP_HIP(sbpprim::fill<unsigned long long>(reinterpret_cast<unsigned long long *>(bm->cell_hash),
+                                              int64_t(2) << lg, ~0ull, st));
+    if (bm->nnz > 0 || capture) {
         int64_t blocks = std::min<int64_t>((bm->B + 3) / 4, 16384);
         k_hash_cells<<<(int)std::max<int64_t>(blocks, 1), 256, 0, st>>>(bm->B, bm->row_ptr, bm->row_cells, bm->cell_hash,
                                                                        64 - lg, (1u << lg) - 1);
         SBP_HIP(hipGetLastError());
     }
-    bm->hash_valid = true;
+    // a recorded hash is rebuilt by each replay; outside the graph the matrix it describes has changed
+    bm->hash_valid = !capture;
 }
 
 __global__ void k_hist_clear(int64_t n, sbpdev::HistSlot *__restrict__ t) {
@@ -307,9 +322,72 @@ void bm_set_model(sbp_ctx ctx, sbp_graph g, sbp_bm bm, int np, int dc) {
     if (bm->dc && !bm->hist_ready) bm_build_hist(ctx, g, bm);
 }
 
-void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm, bool keep_hist) {
+// Largest block matrix of B blocks over E edges.
+static int64_t max_cells(int64_t B, int64_t E) { return std::min<int64_t>(E, B * B); }
+
+void bm_reserve_cells(sbp_ctx ctx, sbp_graph g, sbp_bm bm) {
+    const int64_t bound = std::max<int64_t>(max_cells(bm->B, g->E), 1);
+    if (bm->cap < bound) {  // grow, keeping the current cells (ensure_cells does not)
+        DevicePool &pool = pool_of(bm);
+        int2 *rc = static_cast<int2 *>(pool.alloc(sizeof(int2) * (size_t)bound));
+        int2 *cc = static_cast<int2 *>(pool.alloc(sizeof(int2) * (size_t)bound));
+        int32_t *rcum = static_cast<int32_t *>(pool.alloc(sizeof(int32_t) * (size_t)bound));
+        int32_t *ccum = static_cast<int32_t *>(pool.alloc(sizeof(int32_t) * (size_t)bound));
+        if (bm->nnz > 0) {
+            hipStream_t st = ctx->stream;
+            SBP_HIP(hipMemcpyAsync(rc, bm->row_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToDevice, st));
+            SBP_HIP(hipMemcpyAsync(cc, bm->col_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToDevice, st));
+            SBP_HIP(hipMemcpyAsync(rcum, bm->row_cum, sizeof(int32_t) * bm->nnz, hipMemcpyDeviceToDevice, st));
+            SBP_HIP(hipMemcpyAsync(ccum, bm->col_cum, sizeof(int32_t) * bm->nnz, hipMemcpyDeviceToDevice, st));
+        }
+        void *old[] = {bm->row_cells, bm->col_cells, bm->row_cum, bm->col_cum};
+        for (void *p : old) pool.release(p);  // reuse is ordered by the stream, after the copies
+        bm->row_cells = rc;
+        bm->col_cells = cc;
+        bm->row_cum = rcum;
+        bm->col_cum = ccum;
+        bm->cap = bound;
+        bm->hash_valid = false;
+    }
+    const int lg = hash_log2_for(bound);
+    if (!bm->cell_hash || bm->hash_log2 < lg) {
+        DevicePool &pool = pool_of(bm);
+        pool.release(bm->cell_hash);
+        bm->cell_hash = static_cast<sbpdev::CellSlot *>(pool.alloc(sizeof(sbpdev::CellSlot) << lg));
+        bm->hash_log2 = lg;
+        bm->hash_valid = false;
+    }
+    // the rebuild's scratch at its largest: keys of every edge, the transposed sort of every cell
+    const int64_t E = std::max<int64_t>(g->E, 1);
+    ctx->scratch[S_KEYS_A].as<unsigned long long>(E);
+    ctx->scratch[S_KEYS_B].as<unsigned long long>(E);
+    ctx->scratch[S_UNIQ].as<unsigned long long>(E);
+    ctx->scratch[S_COUNTS].as<int32_t>(E);
+    ctx->scratch[S_VALS_B].as<int32_t>(bound);
+    ctx->scratch[S_MISC].as<int64_t>(4);
+    const int cbits = bits_for(bm->B);
+    size_t a = 0, b = 0, c = 0;
+    const int32_t *no_vals = nullptr;
+    unsigned long long *k = nullptr;
+    SBP_HIP(sbpprim::radix_sort(nullptr, a, (const unsigned long long *)k, k, no_vals, (int32_t *)nullptr, E, 0,
+                                2 * cbits, ctx->stream));
+    SBP_HIP(sbpprim::run_length_encode(nullptr, b, (const unsigned long long *)k, k, (int32_t *)nullptr,
+                                       (int64_t *)nullptr, E, ctx->stream));
+    SBP_HIP(sbpprim::radix_sort(nullptr, c, (const unsigned long long *)k, k, no_vals, (int32_t *)nullptr, bound, 0,
+                                2 * cbits, ctx->stream));
+    ctx->scratch[S_SORT_TMP].get(std::max(a, std::max(b, c)));
+}
+
+void bm_sync_nnz(sbp_ctx ctx, sbp_bm bm) {
+    int32_t nnz = 0;
+    read_small(ctx, {{&nnz, bm->row_ptr + bm->B, sizeof(int32_t)}});
+    bm->nnz = nnz;
+}
+
+void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm, bool keep_hist, bool capture) {
     hipStream_t st = ctx->stream;
     bm->hash_valid = false;
+    SBP_CHECK(!capture || !bm->dc, SBP_E_STATE, "recorded rebuild: degree-corrected histograms need host bookkeeping");
     if (bm->dc && !keep_hist) bm_build_hist(ctx, g, bm);
     if (!bm->dc) bm->hist_ready = false;
     const int32_t B = bm->B;
@@ -318,15 +396,15 @@ void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm, bool keep_hist) {
     const int cbits = bits_for(B);
     const int end_bit = 2 * cbits;
     SBP_CHECK(end_bit <= 64, SBP_E_ARG, "block count too large for packed keys");
-    SBP_HIP(hipMemsetAsync(bm->sizes, 0, sizeof(int32_t) * B, st));
+    SBP_HIP(sbpprim::fill<int32_t>(bm->sizes, B, 0, st));
     if (B <= SIZES_LDS_MAX) k_sizes_lds<<<std::min(grid_for(bm->N), 512), 256, 0, st>>>(bm->N, B, bm->assign, bm->sizes);
     else k_sizes<<<grid_for(bm->N), 256, 0, st>>>(bm->N, bm->assign, bm->sizes);
     if (E == 0) {
-        SBP_HIP(hipMemsetAsync(bm->d_out, 0, sizeof(int32_t) * B, st));
-        SBP_HIP(hipMemsetAsync(bm->d_in, 0, sizeof(int32_t) * B, st));
-        SBP_HIP(hipMemsetAsync(bm->row_ptr, 0, sizeof(int32_t) * (B + 1), st));
-        SBP_HIP(hipMemsetAsync(bm->col_ptr, 0, sizeof(int32_t) * (B + 1), st));
-        SBP_HIP(hipMemsetAsync(bm->d_tot, 0, sizeof(int32_t) * B, st));
+        SBP_HIP(sbpprim::fill<int32_t>(bm->d_out, B, 0, st));
+        SBP_HIP(sbpprim::fill<int32_t>(bm->d_in, B, 0, st));
+        SBP_HIP(sbpprim::fill<int32_t>(bm->row_ptr, B + 1, 0, st));
+        SBP_HIP(sbpprim::fill<int32_t>(bm->col_ptr, B + 1, 0, st));
+        SBP_HIP(sbpprim::fill<int32_t>(bm->d_tot, B, 0, st));
         bm->nnz = 0;
         return;
     }
@@ -345,24 +423,33 @@ void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm, bool keep_hist) {
     void *tmp_ptr = ctx->scratch[S_SORT_TMP].get(tmp);
     SBP_HIP(sbpprim::radix_sort(tmp_ptr, tmp_a, keys, sorted, no_vals, (int32_t *)nullptr, E, 0, end_bit, st));
     SBP_HIP(sbpprim::run_length_encode(tmp_ptr, tmp_b, sorted, uniq, counts, num_runs, E, st));
+    // nnz: read back (eager), or left on the device with the launches sized for the largest block matrix (recorded)
     int64_t nnz = 0;
-    read_small(ctx, {{&nnz, num_runs, sizeof(int64_t)}});
-    ensure_cells(bm, nnz);
+    const int64_t *dnnz = nullptr;
+    if (capture) {
+        nnz = max_cells(B, E);
+        SBP_CHECK(bm->cap >= nnz, SBP_E_STATE, "cells not reserved for a recorded rebuild");
+        dnnz = num_runs;
+    } else {
+        read_small(ctx, {{&nnz, num_runs, sizeof(int64_t)}});
+        ensure_cells(bm, nnz);
+    }
     bm->nnz = nnz;
-    k_fill_major<<<grid_for(nnz), 256, 0, st>>>(nnz, B, cbits, uniq, counts, bm->row_ptr, bm->row_cells);
+    k_fill_major<<<grid_for(nnz), 256, 0, st>>>(nnz, dnnz, B, cbits, uniq, counts, bm->row_ptr, bm->row_cells);
     SBP_HIP(hipGetLastError());
     // transpose: sort (col, row) keys carrying the counts
     auto *tkeys = keys;  // reuse
     auto *tkeys_sorted = sorted;
     auto *tvals_sorted = ctx->scratch[S_VALS_B].as<int32_t>(nnz);
-    k_transpose_keys<<<grid_for(nnz), 256, 0, st>>>(nnz, cbits, uniq, tkeys);
+    k_transpose_keys<<<grid_for(nnz), 256, 0, st>>>(nnz, dnnz, cbits, uniq, tkeys);
     size_t tmp_c = 0;
     SBP_HIP(sbpprim::radix_sort(nullptr, tmp_c, tkeys, tkeys_sorted, (const int32_t *)counts, tvals_sorted, nnz, 0,
-                                end_bit, st));
+                                end_bit, st, dnnz));
     tmp_ptr = ctx->scratch[S_SORT_TMP].get(tmp_c);
     SBP_HIP(sbpprim::radix_sort(tmp_ptr, tmp_c, tkeys, tkeys_sorted, (const int32_t *)counts, tvals_sorted, nnz, 0,
-                                end_bit, st));
-    k_fill_major<<<grid_for(nnz), 256, 0, st>>>(nnz, B, cbits, tkeys_sorted, tvals_sorted, bm->col_ptr, bm->col_cells);
+                                end_bit, st, dnnz));
+    k_fill_major<<<grid_for(nnz), 256, 0, st>>>(nnz, dnnz, B, cbits, tkeys_sorted, tvals_sorted, bm->col_ptr,
+                                                 bm->col_cells);
     launch_cum(ctx, bm, bm->d_out, bm->d_in);  // cumulative counts, and the degrees as row / column totals
     k_dtot_sizes<<<grid_for(B), 256, 0, st>>>(B, bm->row_ptr, bm->row_cells, bm->d_out, bm->d_in, bm->d_tot);
     SBP_HIP(hipGetLastError());

@@ -85,7 +85,7 @@ static void gibbs_batch(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int greedy, uint32_
                         int64_t count, sbp_stats *st) {
     hipStream_t s = ctx->stream;
     int32_t *move_s = ctx->scratch[S_MOVES].as<int32_t>(g->N);
-    SBP_HIP(hipMemsetAsync(move_s, 0xff, sizeof(int32_t) * g->N, s));
+    SBP_HIP(sbpprim::fill<int32_t>(move_s, g->N, -1, s));
     double t0 = now_ms();
     FtLaunchInfo info = ft_propose(ctx, bm, g, epoch, greedy, d_items, count, move_s, nullptr, nullptr);
     int64_t refused = 0;
@@ -132,6 +132,188 @@ static void async_gibbs_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_par
         gibbs_batch(ctx, bm, g, p.greedy, epoch, order + start, end - start, st);
     }
     if (st) st->sweeps++;
+}
+
+// ------------------------------------------------------------------ recorded sweeps
+
+// One asynchronous-Gibbs iteration (finetune::asynchronous_gibbs' loop body, src/finetune.cpp:50-88: the batches'
+// proposals against the frozen blockmodel, each batch's commit in vertex-id order with the rebuild, then the MDL)
+// recorded once into a HIP graph and replayed every sweep: a replay is one graph launch plus one read of its
+// results, where the sweep run call by call waits on the device five times per batch.  The recorded launches read
+// the epoch from device memory (written before each replay) and size nothing from host-side values of the
+// changing state: the commit's event sort and segment scan read their counts on the device, and the rebuild and
+// the MDL reduction are launched for the largest block matrix of B blocks (min(E, B^2) cells), with the cells,
+// the cell hash and every scratch buffer reserved before recording.  Parametric entropy only (the nonparametric
+// description length takes host-side constants of the state, the degree-corrected histograms host bookkeeping).
+struct SweepResult {
+    unsigned long long class_bytes[FT_NB];  // algorithmic bytes of the proposal launches per degree class
+    unsigned long long err;                 // a proposal found degrees inconsistent with the block matrix
+    unsigned long long moved, proposed;     // commits summed over the batches
+    sbpdev::Fx lp;                          // log posterior, fixed point
+    int32_t lp_err;                         // a non-zero cell with a zero degree
+    uint32_t epoch;                         // written by the host before each replay
+};
+
+__global__ void k_sweep_begin(SweepResult *r) {
+    if (threadIdx.x < FT_NB) r->class_bytes[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        r->err = 0;
+        r->moved = 0;
+        r->proposed = 0;
+    }
+}
+
+__global__ void k_sweep_accumulate(SweepResult *r, const unsigned long long *__restrict__ prop,
+                                   const unsigned long long *__restrict__ commit) {
+    if (threadIdx.x < FT_NB) r->class_bytes[threadIdx.x] += prop[threadIdx.x];
+    if (threadIdx.x == 0) {
+        r->err |= prop[FT_NB];
+        r->moved += commit[0];
+        r->proposed += commit[2];
+    }
+}
+
+struct RecordedSweep {
+    hipGraphExec_t exec = nullptr;
+    SweepResult *d = nullptr;      // device
+    SweepResult *h = nullptr;      // pinned host copy
+    uint32_t *h_epoch = nullptr;   // pinned host epoch for the copy before a replay
+    // what the recording depends on
+    sbp_bm bm = nullptr;
+    sbp_graph g = nullptr;
+    int32_t B = 0, greedy = 0, batches = 0, ordered = 0;
+    int64_t proposals = 0;           // proposals per sweep (the batches' sizes)
+    std::vector<const void *> deps;  // every device buffer the recorded launches address
+    ~RecordedSweep() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (d) (void)dev_free(d);
+        if (h) (void)dev_host_free(h);
+        if (h_epoch) (void)dev_host_free(h_epoch);
+    }
+    static std::vector<const void *> buffers_of(sbp_ctx ctx, sbp_bm b) {
+        std::vector<const void *> v = {b->assign,  b->sizes,   b->d_out,    b->d_in,      b->d_tot,    b->row_ptr,
+                                       b->col_ptr, b->row_cells, b->col_cells, b->row_cum, b->col_cum, b->cell_hash};
+        for (const Scratch &s : ctx->scratch) v.push_back(s.ptr);
+        return v;
+    }
+    bool matches(sbp_ctx ctx, sbp_bm b, sbp_graph gr, const sbp_params &p) const {
+        return exec && b == bm && gr == g && b->B == B && p.greedy == greedy && p.batches == batches &&
+               p.ordered == ordered && b->hash_log2 == hash_log2 && buffers_of(ctx, b) == deps;
+    }
+    int hash_log2 = 0;
+};
+
+// Whether sweeps of this blockmodel run as graph replays (SBP_AMD_RECORDED_SWEEPS=0 turns them off).
+static bool recorded_sweeps_enabled(sbp_bm bm, const sbp_params &p) {
+    static const bool off = [] {
+        const char *e = getenv("SBP_AMD_RECORDED_SWEEPS");
+        return e && e[0] == '0';
+    }();
+    return !off && !bm->np && !bm->dc && p.algorithm == 0 && bm->N > 0 && bm->E > 0;
+}
+
+static void record_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p) {
+    hipStream_t st = ctx->stream;
+    auto rs = std::make_shared<RecordedSweep>();
+    SBP_HIP(dev_malloc(&rs->d, sizeof(SweepResult)));
+    SBP_HIP(hipHostMalloc(&rs->h, sizeof(SweepResult), hipHostMallocDefault));
+    SBP_HIP(hipHostMalloc(&rs->h_epoch, 64, hipHostMallocDefault));
+    const int64_t N = g->N;
+    const int64_t batch_size = (int64_t)ceil((double)N / (double)p.batches);
+    // reserve every buffer at its largest, then record
+    bm_reserve_cells(ctx, g, bm);
+    ft_reserve(ctx, bm, g, batch_size);
+    log_posterior_reserve(ctx, std::min<int64_t>(g->E, (int64_t)bm->B * bm->B));
+    SBP_HIP(hipStreamSynchronize(st));
+    int32_t *ident = iota_of(ctx, g);
+    uint32_t *d_epoch = &rs->d->epoch;
+    const int64_t max_nnz = std::min<int64_t>(g->E, (int64_t)bm->B * bm->B);
+    hipGraph_t graph = nullptr;
+    // no hipFree or hipDeviceSynchronize of another thread may run while the stream is being recorded
+    std::unique_lock<std::shared_mutex> quiet(device_wide_mu());
+    SBP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    g_recording = true;
+    try {
+        k_sweep_begin<<<1, 64, 0, st>>>(rs->d);
+        const int32_t *order = ident;
+        if (!p.ordered && p.batches > 1) {
+            int32_t *sh = ctx->scratch[S_ORDER].as<int32_t>(N);
+            shuffle_keys(ctx, ident, N, 0, sh, d_epoch);
+            order = sh;
+        }
+        int32_t *move_s = ctx->scratch[S_MOVES].as<int32_t>(N);
+        for (int64_t batch = 0; batch_size > 0 && batch < N / batch_size; ++batch) {
+            const int64_t start = batch * batch_size;
+            const int64_t end = std::min<int64_t>(N, (batch + 1) * batch_size);
+            SBP_HIP(sbpprim::fill<int32_t>(move_s, N, -1, st));
+            ft_propose(ctx, bm, g, 0, p.greedy, order + start, end - start, move_s, nullptr, nullptr, d_epoch);
+            ft_commit(ctx, bm, g, move_s, nullptr, true);
+            k_sweep_accumulate<<<1, 64, 0, st>>>(rs->d, ft_propose_counters(ctx), ft_commit_counters(ctx));
+        }
+        log_posterior_device(ctx, bm, max_nnz, &rs->d->lp, &rs->d->lp_err);
+        SBP_HIP(hipGetLastError());
+    } catch (...) {
+        g_recording = false;
+        hipGraph_t aborted = nullptr;
+        (void)hipStreamEndCapture(st, &aborted);
+        if (aborted) (void)hipGraphDestroy(aborted);
+        (void)hipGetLastError();
+        throw;
+    }
+    g_recording = false;
+    SBP_HIP(hipStreamEndCapture(st, &graph));
+    const hipError_t e = hipGraphInstantiate(&rs->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    SBP_HIP(e);
+    rs->bm = bm;
+    rs->g = g;
+    rs->B = bm->B;
+    rs->greedy = p.greedy;
+    rs->batches = p.batches;
+    rs->ordered = p.ordered;
+    rs->hash_log2 = bm->hash_log2;
+    for (int64_t b = 0; batch_size > 0 && b < N / batch_size; ++b)
+        rs->proposals += std::min<int64_t>(N, (b + 1) * batch_size) - b * batch_size;
+    rs->deps = RecordedSweep::buffers_of(ctx, bm);
+    ctx->recorded = std::move(rs);
+}
+
+// One asynchronous-Gibbs iteration as a replay of the recorded graph (recorded first when the blockmodel, its
+// buffers or the parameters changed); returns the log posterior after the sweep.
+static double replay_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t epoch, sbp_stats *st) {
+    if (!ctx->recorded || !ctx->recorded->matches(ctx, bm, g, p)) {
+        ctx->recorded.reset();
+        record_sweep(ctx, bm, g, p);
+    }
+    RecordedSweep &rs = *ctx->recorded;
+    hipStream_t s = ctx->stream;
+    *rs.h_epoch = epoch;
+    SBP_HIP(hipMemcpyAsync(&rs.d->epoch, rs.h_epoch, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    SBP_HIP(hipEventRecord(ctx->ev[0], s));
+    SBP_HIP(hipGraphLaunch(rs.exec, s));
+    SBP_HIP(hipEventRecord(ctx->ev[1], s));
+    int32_t nnz = 0;
+    SBP_HIP(hipMemcpyAsync(rs.h, rs.d, sizeof(SweepResult), hipMemcpyDeviceToHost, s));
+    read_small(ctx, {{&nnz, bm->row_ptr + bm->B, sizeof(int32_t)}});
+    const SweepResult &r = *rs.h;
+    bm->nnz = nnz;
+    bm->hash_valid = false;
+    SBP_CHECK(r.err == 0, SBP_E_STATE, "finetune proposal: block degrees inconsistent with the block matrix");
+    SBP_CHECK(r.lp_err == 0, SBP_E_STATE, "log_posterior: non-zero cell with zero block degree");
+    if (st) {
+        st->proposals += rs.proposals;
+        st->moves += (int64_t)r.moved;
+        st->refused += (int64_t)r.proposed - (int64_t)r.moved;
+        for (int b = 0; b < FT_NB; ++b) {
+            st->ft_class_bytes[b] += r.class_bytes[b];
+            if (b < FT_NB - 2) st->ft_bytes += r.class_bytes[b];
+            else st->ft_high_bytes += r.class_bytes[b];
+        }
+        st->sweeps++;
+        st->recorded_sweeps++;
+        st->finetune_ms += elapsed_ms(ctx, ctx->ev[0], ctx->ev[1]);
+    }
+    return (double)r.lp.hi + (double)r.lp.lo * 0x1p-64;
 }
 
 // finetune::early_stop, 4-argument form (src/finetune.cpp:265-283).
@@ -181,8 +363,18 @@ static void mcmc(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, bool 
             nl = g->N - nh;
             d_l = d_h + nh;
         }
+        const bool replay = p.algorithm == 0 && recorded_sweeps_enabled(bm, p);
         for (int64_t it = 0; it < max_it; ++it) {
             uint32_t ep = (*epoch)++;
+            if (replay) {
+                // the sweep and the MDL that closes the iteration, as one graph launch
+                const double lp = replay_sweep(ctx, bm, g, p, ep, st);
+                const double e = model_term(g->E, g->N, bm->B) - lp;
+                de.push_back(e - last);
+                last = e;
+                if (early_stop4(it, golden_ratio_not_reached, initial, de)) break;
+                continue;
+            }
             if (p.algorithm == 0) {
                 async_gibbs_sweep(ctx, bm, g, p, ep, st);
             } else {
@@ -292,19 +484,19 @@ static sbp_bm merge_apply(sbp_ctx ctx, sbp_bm bm, sbp_graph g, std::vector<int32
     int32_t *d_best_s = ctx->scratch[S_BEST].as<int32_t>(3 * (size_t)B + 4);
     double t0 = now_ms();
     std::vector<int32_t> block_map, sizes(B);
-    SBP_HIP(hipMemcpy(sizes.data(), bm->sizes, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+    SBP_HIP(dev_copy(sizes.data(), bm->sizes, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
     sbpcarry::Model model;
     sbpcarry::Pool *pool = nullptr;
     if (!approximate) {  // the advanced carry-out re-evaluates merges against the changing blockmodel
         std::vector<int32_t> rp(B + 1), cp(B + 1), dout(B), din(B);
         std::vector<int2> rc(bm->nnz), cc(bm->nnz);
-        SBP_HIP(hipMemcpy(rp.data(), bm->row_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
-        SBP_HIP(hipMemcpy(cp.data(), bm->col_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
-        SBP_HIP(hipMemcpy(dout.data(), bm->d_out, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
-        SBP_HIP(hipMemcpy(din.data(), bm->d_in, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(rp.data(), bm->row_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(cp.data(), bm->col_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(dout.data(), bm->d_out, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(din.data(), bm->d_in, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
         if (bm->nnz) {
-            SBP_HIP(hipMemcpy(rc.data(), bm->row_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
-            SBP_HIP(hipMemcpy(cc.data(), bm->col_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
+            SBP_HIP(dev_copy(rc.data(), bm->row_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
+            SBP_HIP(dev_copy(cc.data(), bm->col_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
         }
         int threads = ctx->carry_threads > 0 ? ctx->carry_threads : sbpcarry::carry_threads();
         // the parametric carry-out applies merges on a thread of its own beside the pool: leave it a core
@@ -322,7 +514,7 @@ static sbp_bm merge_apply(sbp_ctx ctx, sbp_bm bm, sbp_graph g, std::vector<int32
             SBP_CHECK(bm->hist_ready, SBP_E_STATE, "degree histograms not built");
             const int64_t n = int64_t(1) << bm->hist_log2;
             std::vector<sbpdev::HistSlot> h((size_t)n);
-            SBP_HIP(hipMemcpy(h.data(), bm->hist, sizeof(sbpdev::HistSlot) * n, hipMemcpyDeviceToHost));
+            SBP_HIP(dev_copy(h.data(), bm->hist, sizeof(sbpdev::HistSlot) * n, hipMemcpyDeviceToHost));
             std::vector<uint64_t> keys((size_t)n);
             std::vector<int32_t> counts((size_t)n);
             for (int64_t i = 0; i < n; ++i) {
@@ -428,6 +620,7 @@ int sbp_create(sbp_ctx *out, int device, uint64_t seed) {
 }
 
 int sbp_destroy(sbp_ctx ctx) {
+    if (ctx) ctx->recorded.reset();
     if (!ctx) return SBP_OK;
     for (sbp_ctx w : ctx->slots) sbp_destroy(w);
     ctx->slots.clear();
@@ -438,12 +631,12 @@ int sbp_destroy(sbp_ctx ctx) {
     for (auto &e : ctx->ft_ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &s : ctx->scratch) {
-        if (s.ptr) (void)hipFree(s.ptr);
+        if (s.ptr) (void)dev_free(s.ptr);
         s.ptr = nullptr;
         s.bytes = 0;
     }
-    if (ctx->lgamma) (void)hipFree(ctx->lgamma);
-    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->lgamma) (void)dev_free(ctx->lgamma);
+    if (ctx->pinned) (void)dev_host_free(ctx->pinned);
     ctx->pool.reset();  // freed here unless blockmodels of this context still hold buffers
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -483,15 +676,15 @@ int sbp_graph_upload(sbp_ctx ctx, int32_t N, int64_t E, const int64_t *out_off, 
     g->ctx = ctx;
     g->N = N;
     g->E = E;
-    SBP_HIP(hipMalloc(&g->out_off, sizeof(int64_t) * (N + 1)));
-    SBP_HIP(hipMalloc(&g->in_off, sizeof(int64_t) * (N + 1)));
-    SBP_HIP(hipMalloc(&g->out_adj, sizeof(int32_t) * (E + 1)));
-    SBP_HIP(hipMalloc(&g->in_adj, sizeof(int32_t) * (E + 1)));
-    SBP_HIP(hipMemcpy(g->out_off, out_off, sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
-    SBP_HIP(hipMemcpy(g->in_off, in_off, sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
+    SBP_HIP(dev_malloc(&g->out_off, sizeof(int64_t) * (N + 1)));
+    SBP_HIP(dev_malloc(&g->in_off, sizeof(int64_t) * (N + 1)));
+    SBP_HIP(dev_malloc(&g->out_adj, sizeof(int32_t) * (E + 1)));
+    SBP_HIP(dev_malloc(&g->in_adj, sizeof(int32_t) * (E + 1)));
+    SBP_HIP(dev_copy(g->out_off, out_off, sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
+    SBP_HIP(dev_copy(g->in_off, in_off, sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice));
     if (E > 0) {
-        SBP_HIP(hipMemcpy(g->out_adj, out_adj, sizeof(int32_t) * E, hipMemcpyHostToDevice));
-        SBP_HIP(hipMemcpy(g->in_adj, in_adj, sizeof(int32_t) * E, hipMemcpyHostToDevice));
+        SBP_HIP(dev_copy(g->out_adj, out_adj, sizeof(int32_t) * E, hipMemcpyHostToDevice));
+        SBP_HIP(dev_copy(g->in_adj, in_adj, sizeof(int32_t) * E, hipMemcpyHostToDevice));
     }
     graph_finish(ctx, g.get());
     *out = g.release();
@@ -503,7 +696,7 @@ int sbp_graph_free(sbp_graph g) {
     void *p[] = {g->out_off, g->in_off,    g->out_adj,    g->in_adj,          g->edge_src, g->degree,
                  g->deg_order, g->parent_map, g->sample_vertices, g->ft_blist, g->ft_bmeta};
     for (void *x : p)
-        if (x) (void)hipFree(x);
+        if (x) (void)dev_free(x);
     delete g;
     return SBP_OK;
 }
@@ -521,10 +714,10 @@ int sbp_graph_download(sbp_ctx ctx, sbp_graph g, int64_t *out_off, int32_t *out_
     SBP_API_BEGIN
     SBP_HIP(hipStreamSynchronize(ctx->stream));
     const size_t no = sizeof(int64_t) * ((size_t)g->N + 1), na = sizeof(int32_t) * (size_t)g->E;
-    if (out_off) SBP_HIP(hipMemcpy(out_off, g->out_off, no, hipMemcpyDeviceToHost));
-    if (in_off) SBP_HIP(hipMemcpy(in_off, g->in_off, no, hipMemcpyDeviceToHost));
-    if (out_adj && na) SBP_HIP(hipMemcpy(out_adj, g->out_adj, na, hipMemcpyDeviceToHost));
-    if (in_adj && na) SBP_HIP(hipMemcpy(in_adj, g->in_adj, na, hipMemcpyDeviceToHost));
+    if (out_off) SBP_HIP(dev_copy(out_off, g->out_off, no, hipMemcpyDeviceToHost));
+    if (in_off) SBP_HIP(dev_copy(in_off, g->in_off, no, hipMemcpyDeviceToHost));
+    if (out_adj && na) SBP_HIP(dev_copy(out_adj, g->out_adj, na, hipMemcpyDeviceToHost));
+    if (in_adj && na) SBP_HIP(dev_copy(in_adj, g->in_adj, na, hipMemcpyDeviceToHost));
     SBP_API_END(ctx)
 }
 
@@ -535,7 +728,7 @@ int sbp_sample(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, uint64_t 
     SBP_API_BEGIN
     sbp_graph s = sample_graph(ctx, g, alg, fraction, seed);
     if (num_sampled) *num_sampled = s->N;
-    if (mapping_out) SBP_HIP(hipMemcpy(mapping_out, s->parent_map, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
+    if (mapping_out) SBP_HIP(dev_copy(mapping_out, s->parent_map, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
     *sample_out = s;
     SBP_API_END(ctx)
 }
@@ -546,7 +739,7 @@ int sbp_detach(sbp_ctx ctx, sbp_graph g, sbp_graph *detached_out, int32_t *num_k
     SBP_API_BEGIN
     sbp_graph s = sample_graph(ctx, g, SAMPLE_DETACH, 1.0, 0);
     if (num_kept) *num_kept = s->N;
-    if (mapping_out) SBP_HIP(hipMemcpy(mapping_out, s->parent_map, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
+    if (mapping_out) SBP_HIP(dev_copy(mapping_out, s->parent_map, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
     *detached_out = s;
     SBP_API_END(ctx)
 }
@@ -579,7 +772,7 @@ int sbp_extend(sbp_ctx ctx, sbp_graph g, sbp_graph sample, sbp_bm sample_bm, uin
         SBP_HIP(hipStreamSynchronize(ctx->stream));
         bm->num_to_merge = (int64_t)((double)bm->B * 0.5);
         if (assignment_out)
-            SBP_HIP(hipMemcpy(assignment_out, bm->assign, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
+            SBP_HIP(dev_copy(assignment_out, bm->assign, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
     } catch (...) {
         bm_release(bm);
         delete bm;
@@ -644,15 +837,15 @@ int sbp_bm_download(sbp_ctx ctx, sbp_bm bm, int32_t *assignment, int32_t *sizes,
     hipStream_t s = ctx->stream;
     SBP_HIP(hipStreamSynchronize(s));
     const int32_t B = bm->B;
-    if (assignment) SBP_HIP(hipMemcpy(assignment, bm->assign, sizeof(int32_t) * bm->N, hipMemcpyDeviceToHost));
-    if (sizes) SBP_HIP(hipMemcpy(sizes, bm->sizes, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
-    if (d_out) SBP_HIP(hipMemcpy(d_out, bm->d_out, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
-    if (d_in) SBP_HIP(hipMemcpy(d_in, bm->d_in, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
-    if (d_tot) SBP_HIP(hipMemcpy(d_tot, bm->d_tot, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
-    if (row_ptr) SBP_HIP(hipMemcpy(row_ptr, bm->row_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
+    if (assignment) SBP_HIP(dev_copy(assignment, bm->assign, sizeof(int32_t) * bm->N, hipMemcpyDeviceToHost));
+    if (sizes) SBP_HIP(dev_copy(sizes, bm->sizes, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+    if (d_out) SBP_HIP(dev_copy(d_out, bm->d_out, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+    if (d_in) SBP_HIP(dev_copy(d_in, bm->d_in, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+    if (d_tot) SBP_HIP(dev_copy(d_tot, bm->d_tot, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+    if (row_ptr) SBP_HIP(dev_copy(row_ptr, bm->row_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
     if ((cols || cnts) && bm->nnz > 0) {
         std::vector<int2> cells(bm->nnz);
-        SBP_HIP(hipMemcpy(cells.data(), bm->row_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(cells.data(), bm->row_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < bm->nnz; ++i) {
             if (cols) cols[i] = cells[i].x;
             if (cnts) cnts[i] = cells[i].y;
@@ -738,9 +931,9 @@ int sbp_ft_propose_all(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int3
     SBP_HIP(hipMemsetAsync(dS, 0, sizeof(double) * N, s));
     ft_propose(ctx, bm, g, epoch, greedy, iota_of(ctx, g), N, move_s, dS, sv);
     std::vector<int32_t> hm(N);
-    SBP_HIP(hipMemcpy(hm.data(), move_s, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
-    if (s_out) SBP_HIP(hipMemcpy(s_out, sv, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
-    if (dS_out) SBP_HIP(hipMemcpy(dS_out, dS, sizeof(double) * N, hipMemcpyDeviceToHost));
+    SBP_HIP(dev_copy(hm.data(), move_s, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    if (s_out) SBP_HIP(dev_copy(s_out, sv, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    if (dS_out) SBP_HIP(dev_copy(dS_out, dS, sizeof(double) * N, hipMemcpyDeviceToHost));
     if (accept_out)
         for (int32_t v = 0; v < N; ++v) accept_out[v] = hm[v] >= 0 ? 1 : 0;
     SBP_API_END(ctx)
@@ -752,6 +945,20 @@ int sbp_async_gibbs_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params 
     SBP_API_BEGIN
     bm_set_model(ctx, g, bm, p->nonparametric, p->nonparametric && p->degreecorrected);
     async_gibbs_sweep(ctx, bm, g, *p, epoch, stats);
+    SBP_API_END(ctx)
+}
+
+int sbp_async_gibbs_sweep_recorded(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params *p, uint32_t epoch,
+                                   double *mdl_out, sbp_stats *stats) {
+    if (!ctx || !bm || !g || !p || p->batches < 1 || p->nonparametric || bm->np || bm->dc || bm->E == 0 || g->E == 0)
+        return SBP_E_ARG;
+    SBP_API_BEGIN
+    sbp_params q = *p;
+    q.algorithm = 0;
+    const double lp = replay_sweep(ctx, bm, g, q, epoch, stats);
+    const double m = model_term(g->E, g->N, bm->B) - lp;
+    bm->overall_entropy = m;
+    if (mdl_out) *mdl_out = m;
     SBP_API_END(ctx)
 }
 
@@ -807,8 +1014,8 @@ int sbp_merge_propose(sbp_ctx ctx, sbp_bm bm, uint32_t epoch, int32_t *best_s, d
     int32_t *d_s = ctx->scratch[S_BEST].as<int32_t>(3 * (size_t)B + 4);
     double *d_dS = ctx->scratch[S_DS].as<double>((size_t)B + 2);
     merge_propose(ctx, bm, epoch, d_s, d_dS);
-    if (best_s) SBP_HIP(hipMemcpy(best_s, d_s, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
-    if (best_dS) SBP_HIP(hipMemcpy(best_dS, d_dS, sizeof(double) * B, hipMemcpyDeviceToHost));
+    if (best_s) SBP_HIP(dev_copy(best_s, d_s, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+    if (best_dS) SBP_HIP(dev_copy(best_dS, d_dS, sizeof(double) * B, hipMemcpyDeviceToHost));
     SBP_API_END(ctx)
 }
 
@@ -886,13 +1093,13 @@ int sbp_bm_move_vertex(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t vertex, int3
     SBP_API_BEGIN
     int32_t r = 0, size_r = 0;
     SBP_HIP(hipStreamSynchronize(ctx->stream));
-    SBP_HIP(hipMemcpy(&r, bm->assign + vertex, sizeof(int32_t), hipMemcpyDeviceToHost));
-    SBP_HIP(hipMemcpy(&size_r, bm->sizes + r, sizeof(int32_t), hipMemcpyDeviceToHost));
+    SBP_HIP(dev_copy(&r, bm->assign + vertex, sizeof(int32_t), hipMemcpyDeviceToHost));
+    SBP_HIP(dev_copy(&size_r, bm->sizes + r, sizeof(int32_t), hipMemcpyDeviceToHost));
     if (size_r == 1) {  // refused: the move would empty its block (src/blockmodel/blockmodel.cpp:439-443)
         if (moved) *moved = 0;
         return SBP_OK;
     }
-    SBP_HIP(hipMemcpy(bm->assign + vertex, &block, sizeof(int32_t), hipMemcpyHostToDevice));
+    SBP_HIP(dev_copy(bm->assign + vertex, &block, sizeof(int32_t), hipMemcpyHostToDevice));
     bm_rebuild(ctx, g, bm);  // the counts and degree histograms of the new assignment, as the moves leave them
     SBP_HIP(hipStreamSynchronize(ctx->stream));
     if (moved) *moved = 1;
@@ -924,7 +1131,7 @@ int sbp_partition(sbp_ctx ctx, sbp_graph g, const sbp_params *p, int32_t *assign
         Blockmodel bm = sbp_amd::drive(ctx, g, g->N, *p, stats);
         SBP_HIP(hipStreamSynchronize(ctx->stream));
         if (assignment_out)
-            SBP_HIP(hipMemcpy(assignment_out, bm.handle()->assign, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
+            SBP_HIP(dev_copy(assignment_out, bm.handle()->assign, sizeof(int32_t) * g->N, hipMemcpyDeviceToHost));
         if (num_blocks_out) *num_blocks_out = bm.handle()->B;
     } catch (...) {
         ctx->seed = saved;
@@ -957,7 +1164,7 @@ int sbp_ctx_trim(sbp_ctx ctx) {
     SBP_HIP(hipStreamSynchronize(ctx->stream));
     if (ctx->pool) ctx->pool->trim();
     for (auto &sc : ctx->scratch) {  // the next use allocates afresh under a new generation
-        if (sc.ptr) SBP_HIP(hipFree(sc.ptr));
+        if (sc.ptr) SBP_HIP(dev_free(sc.ptr));
         sc.ptr = nullptr;
         sc.bytes = 0;
     }
@@ -999,12 +1206,12 @@ struct DebugBufs {
     std::vector<void *> p;
     void *get(size_t bytes) {
         void *x = nullptr;
-        SBP_HIP(hipMalloc(&x, bytes > 0 ? bytes : 1));
+        SBP_HIP(dev_malloc(&x, bytes > 0 ? bytes : 1));
         p.push_back(x);
         return x;
     }
     ~DebugBufs() {
-        for (void *x : p) (void)hipFree(x);
+        for (void *x : p) (void)dev_free(x);
     }
 };
 
@@ -1024,11 +1231,11 @@ int sbp_debug_sort(sbp_ctx ctx, int64_t n, int32_t key_bits, const uint64_t *key
     if (key_bits == 32) {
         k32.resize((size_t)n);
         for (int64_t i = 0; i < n; ++i) k32[i] = (uint32_t)keys[i];
-        SBP_HIP(hipMemcpy(dk, k32.data(), kb, hipMemcpyHostToDevice));
+        SBP_HIP(dev_copy(dk, k32.data(), kb, hipMemcpyHostToDevice));
     } else if (n > 0) {
-        SBP_HIP(hipMemcpy(dk, keys, kb, hipMemcpyHostToDevice));
+        SBP_HIP(dev_copy(dk, keys, kb, hipMemcpyHostToDevice));
     }
-    if (vals && n > 0) SBP_HIP(hipMemcpy(dv, vals, sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    if (vals && n > 0) SBP_HIP(dev_copy(dv, vals, sizeof(int32_t) * n, hipMemcpyHostToDevice));
     size_t tmp = 0;
     if (key_bits == 64) {
         auto *a = static_cast<const unsigned long long *>(dk);
@@ -1043,12 +1250,12 @@ int sbp_debug_sort(sbp_ctx ctx, int64_t n, int32_t key_bits, const uint64_t *key
     }
     SBP_HIP(hipStreamSynchronize(st));
     if (key_bits == 32) {
-        SBP_HIP(hipMemcpy(k32.data(), dko, kb, hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(k32.data(), dko, kb, hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < n; ++i) keys_out[i] = k32[i];
     } else if (n > 0) {
-        SBP_HIP(hipMemcpy(keys_out, dko, kb, hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(keys_out, dko, kb, hipMemcpyDeviceToHost));
     }
-    if (vals && n > 0) SBP_HIP(hipMemcpy(vals_out, dvo, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    if (vals && n > 0) SBP_HIP(dev_copy(vals_out, dvo, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
     SBP_API_END(ctx)
 }
 
@@ -1059,12 +1266,12 @@ int sbp_debug_scan(sbp_ctx ctx, int64_t n, const int64_t *in, int32_t inclusive,
     DebugBufs b;
     int64_t *di = static_cast<int64_t *>(b.get(sizeof(int64_t) * n));
     int64_t *dout = static_cast<int64_t *>(b.get(sizeof(int64_t) * n));
-    if (n > 0) SBP_HIP(hipMemcpy(di, in, sizeof(int64_t) * n, hipMemcpyHostToDevice));
+    if (n > 0) SBP_HIP(dev_copy(di, in, sizeof(int64_t) * n, hipMemcpyHostToDevice));
     size_t tmp = 0;
     SBP_HIP(sbpprim::scan(nullptr, tmp, (const int64_t *)di, dout, n, inclusive != 0, st));
     SBP_HIP(sbpprim::scan(b.get(tmp), tmp, (const int64_t *)di, dout, n, inclusive != 0, st));
     SBP_HIP(hipStreamSynchronize(st));
-    if (n > 0) SBP_HIP(hipMemcpy(out, dout, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
+    if (n > 0) SBP_HIP(dev_copy(out, dout, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
     SBP_API_END(ctx)
 }
 
@@ -1078,15 +1285,15 @@ int sbp_debug_rle(sbp_ctx ctx, int64_t n, const uint64_t *keys, uint64_t *uniq, 
     auto *du = static_cast<unsigned long long *>(b.get(sizeof(uint64_t) * n));
     int32_t *dc = static_cast<int32_t *>(b.get(sizeof(int32_t) * n));
     int64_t *dn = static_cast<int64_t *>(b.get(sizeof(int64_t)));
-    if (n > 0) SBP_HIP(hipMemcpy(dk, keys, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+    if (n > 0) SBP_HIP(dev_copy(dk, keys, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
     size_t tmp = 0;
     SBP_HIP(sbpprim::run_length_encode(nullptr, tmp, (const unsigned long long *)dk, du, dc, dn, n, st));
     SBP_HIP(sbpprim::run_length_encode(b.get(tmp), tmp, (const unsigned long long *)dk, du, dc, dn, n, st));
     SBP_HIP(hipStreamSynchronize(st));
-    SBP_HIP(hipMemcpy(num_runs, dn, sizeof(int64_t), hipMemcpyDeviceToHost));
+    SBP_HIP(dev_copy(num_runs, dn, sizeof(int64_t), hipMemcpyDeviceToHost));
     if (*num_runs > 0) {
-        SBP_HIP(hipMemcpy(uniq, du, sizeof(uint64_t) * *num_runs, hipMemcpyDeviceToHost));
-        SBP_HIP(hipMemcpy(counts, dc, sizeof(int32_t) * *num_runs, hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(uniq, du, sizeof(uint64_t) * *num_runs, hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(counts, dc, sizeof(int32_t) * *num_runs, hipMemcpyDeviceToHost));
     }
     SBP_API_END(ctx)
 }

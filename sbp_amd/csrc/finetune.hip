@@ -89,6 +89,7 @@ struct FtArgs {
     int32_t N, B;
     uint64_t seed;
     uint32_t epoch;
+    const uint32_t *epoch_ptr;  // when set, the epoch is read here (a captured sweep replays with a new epoch)
     int greedy;
     const int64_t *out_off, *in_off;
     const int32_t *out_adj, *in_adj;
@@ -337,7 +338,7 @@ __device__ int propose_block(const FtArgs &a, int v, MoveCtx &m, U4 &w, const G 
     m.k_all = ko + kia;
     m.shift_in = m.k_in + (self_out > 0 ? 1 : 0);
     const long long k = ko + m.k_in;
-    w = philox(a.seed, (uint32_t)v, a.epoch, TAG_FT, 0);
+    w = philox(a.seed, (uint32_t)v, a.epoch_ptr ? *a.epoch_ptr : a.epoch, TAG_FT, 0);
     int s = -1;
     m.t = -1;
     if (k == 0) {
@@ -935,6 +936,7 @@ static FtArgs base_args(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     a.B = bm->B;
     a.seed = ctx->seed;
     a.epoch = epoch;
+    a.epoch_ptr = nullptr;
     a.greedy = greedy;
     a.out_off = g->out_off;
     a.in_off = g->in_off;
@@ -1102,16 +1104,16 @@ static void graph_classes(sbp_ctx ctx, sbp_graph g) {
     for (int b = 0; b < FT_NB; ++b) pos[b] = meta[FT_NB + b];
     for (int32_t v = 0; v < N; ++v) list[(size_t)pos[cls[v]]++] = v;
     int32_t *d_list = nullptr, *d_meta = nullptr;
-    SBP_HIP(hipMalloc(&d_list, sizeof(int32_t) * list.size()));
-    if (hipMalloc(&d_meta, sizeof(int32_t) * meta.size()) != hipSuccess) {
-        (void)hipFree(d_list);
+    SBP_HIP(dev_malloc(&d_list, sizeof(int32_t) * list.size()));
+    if (dev_malloc(&d_meta, sizeof(int32_t) * meta.size()) != hipSuccess) {
+        (void)dev_free(d_list);
         SBP_HIP(hipErrorOutOfMemory);
     }
-    hipError_t e = hipMemcpy(d_list, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_meta, meta.data(), sizeof(int32_t) * meta.size(), hipMemcpyHostToDevice);
+    hipError_t e = dev_copy(d_list, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = dev_copy(d_meta, meta.data(), sizeof(int32_t) * meta.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        (void)hipFree(d_list);
-        (void)hipFree(d_meta);
+        (void)dev_free(d_list);
+        (void)dev_free(d_meta);
         SBP_HIP(e);
     }
     for (int b = 0; b < FT_NB; ++b) g->ft_bcount[b] = cnt[b];
@@ -1120,16 +1122,18 @@ static void graph_classes(sbp_ctx ctx, sbp_graph g) {
 }
 
 FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int greedy, const int32_t *d_vertices,
-                        int64_t count, int32_t *d_move_s, double *d_dS, int32_t *d_s_out) {
+                        int64_t count, int32_t *d_move_s, double *d_dS, int32_t *d_s_out, const uint32_t *epoch_ptr) {
     hipStream_t st = ctx->stream;
+    const bool capture = epoch_ptr != nullptr;  // recorded into a graph: no events, no host reads
     FtLaunchInfo info{};
     info.proposals = count;
     if (count == 0) return info;
     // [0, FT_NB): algorithmic bytes per degree class; [FT_NB]: the error flag
     auto *counters = ctx->scratch[S_COUNTER].as<unsigned long long>(8);
-    SBP_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(unsigned long long), st));
+    SBP_HIP(sbpprim::fill<unsigned long long>(counters, 8, 0ull, st));
     FtArgs a = base_args(ctx, bm, g, epoch, greedy);
-    bm_ensure_hash(ctx, bm);
+    a.epoch_ptr = epoch_ptr;
+    bm_ensure_hash(ctx, bm, capture);
     a.m.hash = bm->cell_hash;
     a.m.hash_shift = 64 - bm->hash_log2;
     a.m.hash_mask = (1u << bm->hash_log2) - 1;
@@ -1148,6 +1152,8 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
         a.bmeta = g->ft_bmeta;
         for (int b = 0; b < FT_NB; ++b) upper[b] = g->ft_bcount[b];
     } else {
+        SBP_CHECK(!capture || count == g->N || ctx->scratch[S_BUCKETS].bytes >= sizeof(int32_t) * ((size_t)count + 4 * FT_NB) + 16,
+                  SBP_E_STATE, "captured proposals: degree-class scratch not reserved");
         int32_t *meta = ctx->scratch[S_BUCKETS].as<int32_t>((size_t)count + 4 * FT_NB);
         int32_t *list = meta + 4 * FT_NB;
         const int grid = (int)std::min<int64_t>((count + 255) / 256, 4096);
@@ -1159,14 +1165,17 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
         a.bmeta = meta;
         for (int b = 0; b < FT_NB; ++b) upper[b] = b < FT_NB - 2 ? count : std::min<int64_t>(count, g->n_high_degree);
     }
-    SBP_HIP(hipEventRecord(ctx->ft_ev[0], st));
+    auto mark = [&](int i) {
+        if (!capture) SBP_HIP(hipEventRecord(ctx->ft_ev[i], st));
+    };
+    mark(0);
     for (int b = 0; b < FT_NB - 2; ++b) {
         info.class_launched[b] = upper[b] > 0;
         if (info.class_launched[b]) {
             a.bytes = counters + b;
             launch_lanes(upper[b], st, a, b, ctx->ft_lanes[b], ctx->ft_occ[b], greedy != 0, bm->np != 0);
         }
-        SBP_HIP(hipEventRecord(ctx->ft_ev[b + 1], st));
+        mark(b + 1);
     }
     SBP_HIP(hipGetLastError());
     // k <= 1,024: one workgroup per vertex, LDS tables only (2 min(k, B) <= 2,048 slots)
@@ -1176,7 +1185,7 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
         launch_hub((int)std::min<int64_t>(upper[FT_NB - 2], 2048), st, a, FT_NB - 2, ctx->ft_lanes[FT_NB - 2],
                    greedy != 0, bm->np != 0);
     }
-    SBP_HIP(hipEventRecord(ctx->ft_ev[FT_NB - 1], st));
+    mark(FT_NB - 1);
     info.class_launched[FT_NB - 1] = upper[FT_NB - 1] > 0;
     if (info.class_launched[FT_NB - 1]) {
         a.bytes = counters + FT_NB - 1;
@@ -1197,7 +1206,8 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
         launch_hub((int)std::max<int64_t>(blocks_h, 1), st, a, FT_NB - 1, HUB_THREADS, greedy != 0, bm->np != 0);
     }
     SBP_HIP(hipGetLastError());
-    SBP_HIP(hipEventRecord(ctx->ft_ev[FT_NB], st));
+    mark(FT_NB);
+    if (capture) return info;  // counters stay on the device (ft_propose_counters)
     unsigned long long hb[8] = {};
     read_small(ctx, {{hb, counters, sizeof(hb)}});
     SBP_CHECK(hb[FT_NB] == 0, SBP_E_STATE, "finetune proposal: block degrees inconsistent with the block matrix");
@@ -1214,6 +1224,46 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     }
     info.high_launched = info.class_launched[FT_NB - 2] || info.class_launched[FT_NB - 1];
     return info;
+}
+
+// Buffers a recorded sweep's proposals and commits need, at their largest, reserved before recording: the graph's
+// degree classes, the per-batch degree-class lists (batch_max items), the commit's event arrays and sort, and the
+// high-degree launch's global tables.
+void ft_reserve(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int64_t batch_max) {
+    const int64_t N = g->N, B = bm->B;
+    graph_classes(ctx, g);
+    ctx->scratch[S_COUNTER].as<unsigned long long>(8);
+    ctx->scratch[S_BUCKETS].as<int32_t>((size_t)batch_max + 4 * FT_NB);
+    ctx->scratch[S_MOVES].as<int32_t>(N);
+    ctx->scratch[S_RISK].as<int32_t>(2 * (size_t)B + 8);
+    ctx->scratch[S_FLAGS].as<int32_t>(16);
+    ctx->scratch[S_EVENTS_A].as<unsigned long long>(2 * (size_t)N + 2);
+    ctx->scratch[S_EVENTS_B].as<unsigned long long>(2 * (size_t)N + 2);
+    ctx->scratch[S_ACCEPT].as<int32_t>(2 * (size_t)N + 2);
+    ctx->scratch[S_MISC].as<int32_t>(3 * (size_t)B + 2);
+    ctx->scratch[S_ORDER].as<int32_t>(N);
+    ctx->scratch[S_KEYS_A].as<unsigned long long>(std::max<int64_t>(N, 1));
+    ctx->scratch[S_KEYS_B].as<unsigned long long>(std::max<int64_t>(N, 1));
+    size_t a = 0, b = 0;
+    const int32_t *no_vals = nullptr;
+    unsigned long long *k = nullptr;
+    SBP_HIP(sbpprim::radix_sort(nullptr, a, (const unsigned long long *)k, k, no_vals, (int32_t *)nullptr, 2 * N, 0, 64,
+                                ctx->stream));
+    SBP_HIP(sbpprim::radix_sort(nullptr, b, (const unsigned long long *)k, k, no_vals, (int32_t *)nullptr, N, 0, 64,
+                                ctx->stream));
+    ctx->scratch[S_SORT_TMP].get(std::max(a, b));
+    const int64_t kmax = std::min<int64_t>(g->max_degree, B);
+    if (g->ft_bcount[FT_NB - 1] > 0 && 2 * kmax > HUB_CAP && B > HUB_DENSE_MAX) {
+        const int lg = std::max(10, log2_ceil(2 * kmax + 2));
+        int64_t blocks_h = std::min<int64_t>(g->ft_bcount[FT_NB - 1], 1024);
+        blocks_h = std::min<int64_t>(blocks_h, (int64_t)((size_t(1) << 30) / ((size_t)table_ints(lg) * sizeof(int))));
+        prepare_tables(ctx, lg, std::max<int64_t>(blocks_h, 1));
+    }
+}
+
+// The counters the proposal launches leave on the device: algorithmic bytes per degree class, then the error flag.
+const unsigned long long *ft_propose_counters(sbp_ctx ctx) {
+    return ctx->scratch[S_COUNTER].as<unsigned long long>(8);
 }
 
 // Launch shape of the degree classes (sbp_set_finetune_shape).
@@ -1295,36 +1345,69 @@ __global__ void k_make_events(int32_t N, const int32_t *__restrict__ move_s, con
     }
 }
 
-__global__ void k_segments(int64_t n, const unsigned long long *__restrict__ ev, int32_t *__restrict__ seg_start,
-                           int32_t *__restrict__ seg_end) {
+// Event range of every block present in the sorted events, and the list of those blocks (n read on the device
+// from dn when given).
+__global__ void k_segments(int64_t n, const unsigned long long *__restrict__ dn, const unsigned long long *__restrict__ ev,
+                           int32_t *__restrict__ seg_start, int32_t *__restrict__ seg_end,
+                           int32_t *__restrict__ seg_list, int32_t *__restrict__ n_seg) {
+    if (dn) n = (int64_t)*dn;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         int b = (int)(ev[i] >> 32);
-        if (i == 0 || (int)(ev[i - 1] >> 32) != b) seg_start[b] = (int)i;
+        if (i == 0 || (int)(ev[i - 1] >> 32) != b) {
+            seg_start[b] = (int)i;
+            seg_list[atomicAdd(n_seg, 1)] = b;
+        }
         if (i == n - 1 || (int)(ev[i + 1] >> 32) != b) seg_end[b] = (int)i + 1;
     }
 }
 
-// One thread per at-risk block walks its events in vertex order with the block's running size.
-__global__ void k_resolve(int32_t B, const int32_t *__restrict__ risk, const int32_t *__restrict__ seg_start,
-                          const int32_t *__restrict__ seg_end, const unsigned long long *__restrict__ ev,
-                          const int32_t *__restrict__ assign, const int32_t *__restrict__ sizes,
-                          const int32_t *__restrict__ acc_prev, int32_t *__restrict__ acc_next,
-                          int32_t *__restrict__ changed) {
-    for (int32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x) {
-        if (!risk[b]) continue;
-        int size = sizes[b];
-        for (int i = seg_start[b]; i < seg_end[b]; ++i) {
-            int v = (int)(ev[i] & 0xffffffffull);
-            if (assign[v] == b) {  // move out of b
-                int ok = size > 1;
-                if (ok) size--;
-                acc_next[v] = ok;
-                if (ok != acc_prev[v]) *changed = 1;
-            } else if (acc_prev[v]) {  // move into b
-                size++;
+// The refusal pattern as the fixed point of per-block scans, in one workgroup: each round, every at-risk block with
+// events walks them in vertex order with its running size, deciding its moves out (a move out is refused when the
+// size has reached 1) from the previous round's decisions of the moves into it.  Moves out of blocks not at risk
+// stay accepted (1 in both buffers, set by k_make_events), so a round writes exactly the decisions that can
+// change; the rounds stop when one changes nothing, and both buffers then hold the fixed point.  The iteration is
+// monotone and each round fixes at least the earliest undecided move, so it ends.
+__global__ void __launch_bounds__(1024) k_resolve_all(const int32_t *__restrict__ seg_list,
+                                                      const int32_t *__restrict__ n_seg,
+                                                      const int32_t *__restrict__ seg_start,
+                                                      const int32_t *__restrict__ seg_end,
+                                                      const unsigned long long *__restrict__ ev,
+                                                      const int32_t *__restrict__ assign,
+                                                      const int32_t *__restrict__ sizes, int32_t *__restrict__ acc_a,
+                                                      int32_t *__restrict__ acc_b, int32_t *__restrict__ rounds_out) {
+    __shared__ int changed;
+    const int m = *n_seg;
+    int32_t *prev = acc_a, *next = acc_b;
+    int rounds = 0;
+    while (true) {
+        if (threadIdx.x == 0) changed = 0;
+        __syncthreads();
+        for (int k = threadIdx.x; k < m; k += blockDim.x) {
+            const int b = seg_list[k];
+            int size = sizes[b];
+            for (int i = seg_start[b]; i < seg_end[b]; ++i) {
+                const int v = (int)(ev[i] & 0xffffffffull);
+                if (assign[v] == b) {  // move out of b
+                    const int ok = size > 1;
+                    if (ok) size--;
+                    next[v] = ok;
+                    if (ok != prev[v]) changed = 1;
+                } else if (prev[v]) {  // move into b
+                    size++;
+                }
             }
         }
+        __threadfence();  // this round's decisions reach the other waves before they read them as prev
+        __syncthreads();
+        ++rounds;
+        const int c = changed;
+        __syncthreads();
+        if (!c) break;
+        int32_t *t = prev;
+        prev = next;
+        next = t;
     }
+    if (threadIdx.x == 0) *rounds_out = rounds;
 }
 
 // Blockmodel::move_vertex(const VertexMove_v3&) (src/blockmodel/blockmodel.cpp:489-492): the committed moves'
@@ -1365,60 +1448,69 @@ static int grid_of(int64_t n) {
     return (int)gsz;
 }
 
-int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_t *refused_out) {
+// Commit of the batch's proposals (d_move_s), then the rebuild.  With capture, nothing is read on the host: the
+// event count sizes nothing (the sort and the segment scan read it on the device, launched for the largest count),
+// the moved and proposed counts stay on the device (ft_commit_counters), so the launches can be recorded into a
+// graph; the moved count returned is then 0.
+int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_t *refused_out, bool capture) {
     hipStream_t st = ctx->stream;
     const int32_t N = bm->N, B = bm->B;
     int32_t *out_count = ctx->scratch[S_RISK].as<int32_t>(2 * (size_t)B + 8);
     int32_t *risk = out_count + B;
     int32_t *flags = ctx->scratch[S_FLAGS].as<int32_t>(16);
-    unsigned long long *counters = reinterpret_cast<unsigned long long *>(flags + 4);  // [0] moved [1] events [2] proposed
-    SBP_HIP(hipMemsetAsync(out_count, 0, sizeof(int32_t) * B, st));
-    SBP_HIP(hipMemsetAsync(flags, 0, 16 * sizeof(int32_t), st));
+    // flags: [0] any block at risk, [1] blocks with events, [2] resolve rounds;
+    // counters: [0] moved [1] events [2] proposed
+    unsigned long long *counters = reinterpret_cast<unsigned long long *>(flags + 4);
+    SBP_HIP(sbpprim::fill<int32_t>(out_count, B, 0, st));
+    SBP_HIP(sbpprim::fill<int32_t>(flags, 16, 0, st));
     k_count_out<<<grid_of(N), 256, 0, st>>>(N, d_move_s, bm->assign, out_count);
     k_mark_risk<<<grid_of(B), 256, 0, st>>>(B, out_count, bm->sizes, risk, flags, counters + 2);
-    int32_t any = 0;
-    read_small(ctx, {{&any, flags, sizeof(int32_t)}});
+    int32_t any = 1;
+    if (!capture) read_small(ctx, {{&any, flags, sizeof(int32_t)}});
     const int32_t *acc_final = nullptr;
     if (any) {
-        auto *events = ctx->scratch[S_EVENTS_A].as<unsigned long long>(2 * (size_t)N + 2);
-        auto *events_sorted = ctx->scratch[S_EVENTS_B].as<unsigned long long>(2 * (size_t)N + 2);
+        const int64_t max_ev = 2 * (int64_t)N;
+        auto *events = ctx->scratch[S_EVENTS_A].as<unsigned long long>((size_t)max_ev + 2);
+        auto *events_sorted = ctx->scratch[S_EVENTS_B].as<unsigned long long>((size_t)max_ev + 2);
         int32_t *acc = ctx->scratch[S_ACCEPT].as<int32_t>(2 * (size_t)N + 2);
-        int32_t *seg = ctx->scratch[S_MISC].as<int32_t>(2 * (size_t)B + 2);
-        int32_t *seg_start = seg, *seg_end = seg + B;
-        SBP_HIP(hipMemsetAsync(seg, 0, sizeof(int32_t) * 2 * (size_t)B, st));
+        int32_t *seg = ctx->scratch[S_MISC].as<int32_t>(3 * (size_t)B + 2);
+        int32_t *seg_start = seg, *seg_end = seg + B, *seg_list = seg + 2 * (size_t)B;
         k_make_events<<<grid_of(N), 256, 0, st>>>(N, d_move_s, bm->assign, risk, events, counters + 1, acc, acc + N);
-        unsigned long long n_ev = 0;
-        read_small(ctx, {{&n_ev, counters + 1, sizeof(n_ev)}});
         // keys (block << 32) | vertex: the bits above 32 + log2(B) are zero
         int end_bit = 33;
         while (end_bit < 64 && (int64_t(1) << (end_bit - 32)) < B) ++end_bit;
-        size_t tmp = 0;
         const int32_t *no_vals = nullptr;
-        SBP_HIP(sbpprim::radix_sort(nullptr, tmp, events, events_sorted, no_vals, (int32_t *)nullptr, (int64_t)n_ev, 0,
-                                    end_bit, st));
-        void *tp = ctx->scratch[S_SORT_TMP].get(tmp);
-        SBP_HIP(sbpprim::radix_sort(tp, tmp, events, events_sorted, no_vals, (int32_t *)nullptr, (int64_t)n_ev, 0,
-                                    end_bit, st));
-        k_segments<<<grid_of((int64_t)n_ev), 256, 0, st>>>((int64_t)n_ev, events_sorted, seg_start, seg_end);
-        int32_t *prev = acc, *next = acc + N;
-        // monotone iteration; every round fixes at least the earliest undecided move, so it terminates
-        for (int64_t it = 0; it <= (int64_t)n_ev + 1; ++it) {
-            SBP_HIP(hipMemsetAsync(flags + 1, 0, sizeof(int32_t), st));
-            SBP_HIP(hipMemcpyAsync(next, prev, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, st));
-            k_resolve<<<grid_of(B), 256, 0, st>>>(B, risk, seg_start, seg_end, events_sorted, bm->assign, bm->sizes,
-                                                   prev, next, flags + 1);
-            int32_t changed = 0;
-            read_small(ctx, {{&changed, flags + 1, sizeof(int32_t)}});
-            std::swap(prev, next);
-            if (!changed) break;
+        int64_t n_sort = max_ev;
+        const int64_t *dn = reinterpret_cast<const int64_t *>(counters + 1);
+        if (!capture) {  // the exact count sizes the launches
+            unsigned long long n_ev = 0;
+            read_small(ctx, {{&n_ev, counters + 1, sizeof(n_ev)}});
+            n_sort = (int64_t)n_ev;
+            dn = nullptr;
         }
-        acc_final = prev;
+        size_t tmp = 0;
+        SBP_HIP(sbpprim::radix_sort(nullptr, tmp, events, events_sorted, no_vals, (int32_t *)nullptr, n_sort, 0, end_bit,
+                                    st, dn));
+        void *tp = ctx->scratch[S_SORT_TMP].get(tmp);
+        SBP_HIP(sbpprim::radix_sort(tp, tmp, events, events_sorted, no_vals, (int32_t *)nullptr, n_sort, 0, end_bit, st,
+                                    dn));
+        if (n_sort > 0)
+            k_segments<<<grid_of(n_sort), 256, 0, st>>>(n_sort, dn ? counters + 1 : nullptr, events_sorted, seg_start,
+                                                        seg_end, seg_list, flags + 1);
+        k_resolve_all<<<1, 1024, 0, st>>>(seg_list, flags + 1, seg_start, seg_end, events_sorted, bm->assign, bm->sizes,
+                                          acc, acc + N, flags + 2);
+        acc_final = acc;
     }
     if (bm->dc)
         k_hist_moves<<<grid_of(N), 256, 0, st>>>(N, d_move_s, acc_final, bm->assign, g->out_off, g->in_off, bm->hist,
                                                  64 - bm->hist_log2, (uint32_t)((1ull << bm->hist_log2) - 1));
     k_apply<<<grid_of(N), 256, 0, st>>>(N, d_move_s, acc_final, bm->assign, counters);
     SBP_HIP(hipGetLastError());
+    if (capture) {
+        SBP_CHECK(!bm->dc, SBP_E_STATE, "captured commit: degree-corrected histograms need host bookkeeping");
+        bm_rebuild(ctx, g, bm, true, true);
+        return 0;
+    }
     unsigned long long c[2] = {0, 0};
     HistSlot book{0, 0, 0};
     if (bm->dc)
@@ -1432,13 +1524,19 @@ int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_
     return (int64_t)c[0];
 }
 
+// The counters a captured commit leaves on the device: moved, events, proposed (unsigned 64-bit each).
+const unsigned long long *ft_commit_counters(sbp_ctx ctx) {
+    return reinterpret_cast<const unsigned long long *>(ctx->scratch[S_FLAGS].as<int32_t>(16) + 4);
+}
+
 // ------------------------------------------------------------------ shuffle
 
 __global__ void k_shuffle_keys(int64_t n, const int32_t *__restrict__ items, uint64_t seed, uint32_t epoch,
+                               const uint32_t *__restrict__ epoch_ptr,
                                unsigned long long *__restrict__ keys) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         int v = items[i];
-        U4 w = philox(seed, (uint32_t)v, epoch, TAG_SHUFFLE, 0);
+        U4 w = philox(seed, (uint32_t)v, epoch_ptr ? *epoch_ptr : epoch, TAG_SHUFFLE, 0);
         keys[i] = ((unsigned long long)w.x << 32) | (unsigned)v;
     }
 }
@@ -1449,12 +1547,13 @@ __global__ void k_low_word(int64_t n, const unsigned long long *__restrict__ key
 }
 
 // Ascending Philox sort key, ties by vertex id.
-void shuffle_keys(sbp_ctx ctx, const int32_t *d_items, int64_t n, uint32_t epoch, int32_t *d_out) {
+void shuffle_keys(sbp_ctx ctx, const int32_t *d_items, int64_t n, uint32_t epoch, int32_t *d_out,
+                  const uint32_t *epoch_ptr) {
     if (n == 0) return;
     hipStream_t st = ctx->stream;
     auto *ka = ctx->scratch[S_KEYS_A].as<unsigned long long>(n);
     auto *kb = ctx->scratch[S_KEYS_B].as<unsigned long long>(n);
-    k_shuffle_keys<<<grid_of(n), 256, 0, st>>>(n, d_items, ctx->seed, epoch, ka);
+    k_shuffle_keys<<<grid_of(n), 256, 0, st>>>(n, d_items, ctx->seed, epoch, epoch_ptr, ka);
     size_t tmp = 0;
     const int32_t *no_vals = nullptr;
     SBP_HIP(sbpprim::radix_sort(nullptr, tmp, ka, kb, no_vals, (int32_t *)nullptr, n, 0, 64, st));

@@ -42,9 +42,9 @@ int grid_for(int64_t n) {
 template <typename T>
 struct DevBuf {  // temporary device buffer for one call
     T *p = nullptr;
-    explicit DevBuf(size_t n) { SBP_HIP(hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1))); }
+    explicit DevBuf(size_t n) { SBP_HIP(dev_malloc(&p, sizeof(T) * std::max<size_t>(n, 1))); }
     ~DevBuf() {
-        if (p) (void)hipFree(p);
+        if (p) (void)dev_free(p);
     }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -293,8 +293,8 @@ void graph_finish(sbp_ctx ctx, sbp_graph g) {
     hipStream_t st = ctx->stream;
     const int32_t N = g->N;
     const int64_t E = g->E;
-    if (!g->edge_src) SBP_HIP(hipMalloc(&g->edge_src, sizeof(int32_t) * (E + N)));
-    if (!g->degree) SBP_HIP(hipMalloc(&g->degree, sizeof(int32_t) * (N + 1)));
+    if (!g->edge_src) SBP_HIP(dev_malloc(&g->edge_src, sizeof(int32_t) * (E + N)));
+    if (!g->degree) SBP_HIP(dev_malloc(&g->degree, sizeof(int32_t) * (N + 1)));
     k_edge_src<<<grid_for(N), GB, 0, st>>>(N, g->out_off, g->edge_src, E);
     DevBuf<int32_t> self(N);
     DevBuf<unsigned long long> stats(2);
@@ -322,12 +322,12 @@ const int32_t *degree_order(sbp_ctx ctx, sbp_graph g) {
     k_degree_keys<<<grid_for(N), GB, 0, st>>>(N, g->degree, keys.p);
     sort_keys(st, keys.p, sorted.p, N);
     int32_t *order = nullptr;
-    SBP_HIP(hipMalloc(&order, sizeof(int32_t) * N));
+    SBP_HIP(dev_malloc(&order, sizeof(int32_t) * N));
     k_low_words<<<grid_for(N), GB, 0, st>>>(N, sorted.p, order);
     SBP_HIP(hipGetLastError());
     const hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
-        (void)hipFree(order);
+        (void)dev_free(order);
         SBP_HIP(e);
     }
     g->deg_order = order;
@@ -350,7 +350,7 @@ sbp_graph sample_graph(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, u
     if (alg == SAMPLE_DETACH) {  // sample::detach (src/sample.cpp:12-26): the vertices of degree > 1, in id order
         std::vector<int32_t> deg(N + 1);
         SBP_HIP(hipStreamSynchronize(st));
-        SBP_HIP(hipMemcpy(deg.data(), g->degree, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(deg.data(), g->degree, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
         host_mapping.assign(N, -1);
         for (int32_t v = 0; v < N; ++v)
             if (deg[v] > 1) {
@@ -363,13 +363,13 @@ sbp_graph sample_graph(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, u
         std::vector<int64_t> oo(N + 1), io(N + 1);
         std::vector<int32_t> oa(E + 1), ia(E + 1), deg(N + 1);
         SBP_HIP(hipStreamSynchronize(st));
-        SBP_HIP(hipMemcpy(oo.data(), g->out_off, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToHost));
-        SBP_HIP(hipMemcpy(io.data(), g->in_off, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(oo.data(), g->out_off, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(io.data(), g->in_off, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToHost));
         if (E > 0) {
-            SBP_HIP(hipMemcpy(oa.data(), g->out_adj, sizeof(int32_t) * E, hipMemcpyDeviceToHost));
-            SBP_HIP(hipMemcpy(ia.data(), g->in_adj, sizeof(int32_t) * E, hipMemcpyDeviceToHost));
+            SBP_HIP(dev_copy(oa.data(), g->out_adj, sizeof(int32_t) * E, hipMemcpyDeviceToHost));
+            SBP_HIP(dev_copy(ia.data(), g->in_adj, sizeof(int32_t) * E, hipMemcpyDeviceToHost));
         }
-        SBP_HIP(hipMemcpy(deg.data(), g->degree, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(deg.data(), g->degree, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
         const sbpsnow::Csr c{N, oo.data(), io.data(), oa.data(), ia.data()};
         host_order = alg == 4 ? sbpsnow::expansion_snowball(c, target, seed, host_mapping)
                               : sbpsnow::degree_product_snowball(c, deg.data(), target, seed, host_mapping);
@@ -403,7 +403,7 @@ sbp_graph sample_graph(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, u
         // vertices that occur, in order of first occurrence; the loop stops after the edge at which the count
         // reaches the target, so it takes every vertex first seen at or before that edge's destination slot
         std::vector<uint64_t> h(N);
-        SBP_HIP(hipMemcpy(h.data(), sorted.p, sizeof(uint64_t) * N, hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(h.data(), sorted.p, sizeof(uint64_t) * N, hipMemcpyDeviceToHost));
         int64_t occurring = 0;
         while (occurring < N && (uint32_t)(h[occurring] >> 32) != (uint32_t)INT32_MAX) ++occurring;
         if (occurring > 0) {
@@ -429,8 +429,8 @@ sbp_graph sample_graph(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, u
     s->N = (int32_t)n;
     s->parent = g;
     s->parent_N = N;
-    SBP_HIP(hipMalloc(&s->parent_map, sizeof(int32_t) * N));
-    SBP_HIP(hipMalloc(&s->sample_vertices, sizeof(int32_t) * std::max<int64_t>(n, 1)));
+    SBP_HIP(dev_malloc(&s->parent_map, sizeof(int32_t) * N));
+    SBP_HIP(dev_malloc(&s->sample_vertices, sizeof(int32_t) * std::max<int64_t>(n, 1)));
     if (!host_order.empty()) {  // as sampled: a vertex appended twice keeps an isolated earlier slot
         SBP_HIP(hipMemcpyAsync(s->parent_map, host_mapping.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
         SBP_HIP(hipMemcpyAsync(s->sample_vertices, host_order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
@@ -460,10 +460,10 @@ sbp_graph sample_graph(sbp_ctx ctx, sbp_graph g, int32_t alg, double fraction, u
         Es = last_pos + last_keep;
     }
     s->E = Es;
-    SBP_HIP(hipMalloc(&s->out_off, sizeof(int64_t) * (n + 1)));
-    SBP_HIP(hipMalloc(&s->in_off, sizeof(int64_t) * (n + 1)));
-    SBP_HIP(hipMalloc(&s->out_adj, sizeof(int32_t) * (Es + 1)));
-    SBP_HIP(hipMalloc(&s->in_adj, sizeof(int32_t) * (Es + 1)));
+    SBP_HIP(dev_malloc(&s->out_off, sizeof(int64_t) * (n + 1)));
+    SBP_HIP(dev_malloc(&s->in_off, sizeof(int64_t) * (n + 1)));
+    SBP_HIP(dev_malloc(&s->out_adj, sizeof(int32_t) * (Es + 1)));
+    SBP_HIP(dev_malloc(&s->in_adj, sizeof(int32_t) * (Es + 1)));
     SBP_HIP(hipMemsetAsync(s->out_off, 0, sizeof(int64_t) * (n + 1), st));
     SBP_HIP(hipMemsetAsync(s->in_off, 0, sizeof(int64_t) * (n + 1), st));
     if (Es > 0) {

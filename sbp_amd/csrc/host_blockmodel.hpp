@@ -21,16 +21,16 @@ struct HostBlockmodel {
         B = bm->B;
         std::vector<int32_t> rp(B + 1), cp(B + 1), a(B), b(B), c(B), sz(B);
         std::vector<int2> rc(bm->nnz), cc(bm->nnz);
-        SBP_HIP(hipMemcpy(rp.data(), bm->row_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
-        SBP_HIP(hipMemcpy(cp.data(), bm->col_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(rp.data(), bm->row_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(cp.data(), bm->col_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
         if (bm->nnz) {
-            SBP_HIP(hipMemcpy(rc.data(), bm->row_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
-            SBP_HIP(hipMemcpy(cc.data(), bm->col_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
+            SBP_HIP(dev_copy(rc.data(), bm->row_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
+            SBP_HIP(dev_copy(cc.data(), bm->col_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
         }
-        SBP_HIP(hipMemcpy(a.data(), bm->d_out, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
-        SBP_HIP(hipMemcpy(b.data(), bm->d_in, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
-        SBP_HIP(hipMemcpy(c.data(), bm->d_tot, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
-        SBP_HIP(hipMemcpy(sz.data(), bm->sizes, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(a.data(), bm->d_out, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(b.data(), bm->d_in, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(c.data(), bm->d_tot, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+        SBP_HIP(dev_copy(sz.data(), bm->sizes, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
         row.assign(B, {});
         col.assign(B, {});
         for (int32_t r = 0; r < B; ++r) {

@@ -9,6 +9,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -20,6 +21,9 @@
 
 namespace sbpcarry {
 class Pool;
+}
+namespace sbpimpl {
+struct RecordedSweep;
 }
 
 namespace sbpimpl {
@@ -42,6 +46,45 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::sbpimpl::Error((code), std::string(msg)); \
     } while (0)
 
+// Set on the thread that records a graph (stream capture): growing a buffer then would synchronize and allocate
+// inside the capture, so every buffer the recorded work needs is reserved beforehand and growth is an error.
+inline thread_local bool g_recording = false;
+
+// hipFree and hipDeviceSynchronize wait for every stream of the device, a stream being recorded by another thread
+// included, which ends that recording with an error.  They take this lock shared; a recording takes it exclusively
+// (record_sweep), so none runs while a graph is being recorded.
+inline std::shared_mutex &device_wide_mu() {
+    static std::shared_mutex m;
+    return m;
+}
+inline hipError_t dev_free(void *p) {
+    if (g_recording) return hipErrorStreamCaptureUnsupported;
+    std::shared_lock<std::shared_mutex> lk(device_wide_mu());
+    return hipFree(p);
+}
+inline hipError_t dev_host_free(void *p) {
+    if (g_recording) return hipErrorStreamCaptureUnsupported;
+    std::shared_lock<std::shared_mutex> lk(device_wide_mu());
+    return hipHostFree(p);
+}
+// Synchronous copies and allocations likewise: they are outside any stream's order.
+inline hipError_t dev_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    if (g_recording) return hipErrorStreamCaptureUnsupported;
+    std::shared_lock<std::shared_mutex> lk(device_wide_mu());
+    return hipMemcpy(dst, src, bytes, kind);
+}
+template <typename T>
+inline hipError_t dev_malloc(T **p, size_t bytes) {
+    if (g_recording) return hipErrorStreamCaptureUnsupported;
+    std::shared_lock<std::shared_mutex> lk(device_wide_mu());
+    return hipMalloc(reinterpret_cast<void **>(p), bytes);
+}
+inline hipError_t dev_sync() {
+    if (g_recording) return hipErrorStreamCaptureUnsupported;
+    std::shared_lock<std::shared_mutex> lk(device_wide_mu());
+    return hipDeviceSynchronize();
+}
+
 // Device buffer that only grows; contents are not preserved across growth.
 struct Scratch {
     void *ptr = nullptr;
@@ -49,15 +92,16 @@ struct Scratch {
     uint64_t generation = 0;  // incremented whenever the buffer is replaced
     void *get(size_t need) {
         if (need > bytes) {
+            if (g_recording) throw Error(SBP_E_STATE, "device scratch growth while recording a graph");
             ++generation;
             // queued kernels may still read the old buffer
             if (ptr) {
-                SBP_HIP(hipDeviceSynchronize());
-                SBP_HIP(hipFree(ptr));
+                SBP_HIP(dev_sync());
+                SBP_HIP(dev_free(ptr));
             }
             ptr = nullptr;
             size_t n = need + need / 4 + 256;
-            SBP_HIP(hipMalloc(&ptr, n));
+            SBP_HIP(dev_malloc(&ptr, n));
             bytes = n;
         }
         return ptr;
@@ -67,7 +111,7 @@ struct Scratch {
         return static_cast<T *>(get(count * sizeof(T) + 16));
     }
     ~Scratch() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr) (void)dev_free(ptr);
     }
 };
 
@@ -83,6 +127,7 @@ struct DevicePool {
     size_t cached_bytes = 0;
     size_t total_bytes = 0;  // every buffer the pool owns, handed out or cached
     void *alloc(size_t bytes) {
+        if (g_recording) throw Error(SBP_E_STATE, "device allocation while recording a graph");
         bytes = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
         auto it = cached.lower_bound(bytes);
         if (it != cached.end() && it->first < 2 * bytes) {
@@ -92,10 +137,10 @@ struct DevicePool {
             return p;
         }
         void *p = nullptr;
-        if (hipMalloc(&p, bytes) != hipSuccess) {
+        if (dev_malloc(&p, bytes) != hipSuccess) {
             (void)hipGetLastError();
             trim();
-            SBP_HIP(hipMalloc(&p, bytes));
+            SBP_HIP(dev_malloc(&p, bytes));
         }
         size_of[p] = bytes;
         total_bytes += bytes;
@@ -105,7 +150,7 @@ struct DevicePool {
         if (!p) return;
         auto it = size_of.find(p);
         if (it == size_of.end()) {
-            (void)hipFree(p);
+            (void)dev_free(p);
             return;
         }
         cached.emplace(it->second, p);
@@ -114,10 +159,10 @@ struct DevicePool {
     }
     void trim() {
         if (cached.empty()) return;
-        (void)hipDeviceSynchronize();
+        (void)dev_sync();
         for (auto &kv : cached) {
             size_of.erase(kv.second);
-            (void)hipFree(kv.second);
+            (void)dev_free(kv.second);
         }
         total_bytes -= cached_bytes;
         cached.clear();
@@ -155,6 +200,7 @@ struct sbp_ctx_s {
     std::shared_ptr<sbpcarry::Pool> carry_pool;  // the carry-out's threads, kept between merge phases
     uint64_t peak_bytes = 0;  // largest device footprint seen at the end of a partition (sbp_ctx_memory)
     std::vector<sbp_ctx> slots;  // worker contexts of sbp_partition_batch, created on first use
+    std::shared_ptr<sbpimpl::RecordedSweep> recorded;  // the asynchronous-Gibbs sweep recorded as a graph (capi.hip)
     // launch shape of the finetune degree classes 0..4 (k <= 8, 32, 128, 256, 1,024): lanes (threads) per vertex,
     // and wavefronts per SIMD of the greedy parametric lane-group launches (sbp_set_finetune_shape)
     int32_t ft_lanes[5] = {8, 8, 32, 64, 256};
@@ -269,7 +315,13 @@ void bm_alloc(sbp_bm bm, int32_t B, int32_t N);
 void bm_release(sbp_bm bm);
 // From bm->assign (device), bm->B set.  The degree histograms of a degree-corrected blockmodel are rebuilt too,
 // unless keep_hist (a finetune commit, which updates them move by move as the reference does).
-void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm, bool keep_hist = false);
+// capture: no host reads and no allocation (the cell arrays must hold the largest block matrix the graph allows,
+// bm_reserve_cells); bm->nnz is then stale until bm_sync_nnz.
+void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm, bool keep_hist = false, bool capture = false);
+// Cell capacity for any block matrix of B blocks over g (nnz <= min(E, B^2)), for captured rebuilds.
+void bm_reserve_cells(sbp_ctx ctx, sbp_graph g, sbp_bm bm);
+// bm->nnz from the device (after captured rebuilds).
+void bm_sync_nnz(sbp_ctx ctx, sbp_bm bm);
 void bm_copy_into(sbp_ctx ctx, sbp_bm src, sbp_bm dst);
 void bm_compute_cum(sbp_ctx ctx, sbp_bm bm);
 // Degree histograms (initialize_edge_counts, src/blockmodel/blockmodel.cpp:234-235) from the assignment and g.
@@ -279,10 +331,17 @@ void bm_build_hist(sbp_ctx ctx, sbp_graph g, sbp_bm bm);
 void bm_hist_after_moves(sbp_ctx ctx, sbp_bm bm, int32_t claimed, int32_t full);
 // Sets the entropy model (args.nonparametric / args.degreecorrected), building the histograms when needed.
 void bm_set_model(sbp_ctx ctx, sbp_graph g, sbp_bm bm, int np, int dc);
-void bm_ensure_hash(sbp_ctx ctx, sbp_bm bm);
+// capture: rebuild the cell hash in place (its table must hold 2 x the largest nnz: bm_reserve_cells), with no
+// host reads; it is then valid only for the graph.
+void bm_ensure_hash(sbp_ctx ctx, sbp_bm bm, bool capture = false);
 void bm_ensure_cells(sbp_bm bm, int64_t nnz);
 // mdl.hip
 double log_posterior(sbp_ctx ctx, sbp_bm bm);
+// The same reduction with nnz read on the device (row_ptr[B]) and the total (fixed point) and error flag left in
+// device memory at out_total / out_err: no host reads, for graph capture.  Work area in slot S_PARTIAL, sized for
+// max_nnz cells.
+void log_posterior_device(sbp_ctx ctx, sbp_bm bm, int64_t max_nnz, sbpdev::Fx *out_total, int32_t *out_err);
+void log_posterior_reserve(sbp_ctx ctx, int64_t max_nnz);
 // The same reduction with its work area in a fixed per-device array (sbp_mdl_noalloc): allocates nothing.
 double log_posterior_noalloc(sbp_ctx ctx, sbp_bm bm);
 // Looks up the fixed array's address on the current device (once per device); sbp_create calls it.
@@ -306,15 +365,24 @@ struct FtLaunchInfo {
     float class_ms[FT_NB];
     bool class_launched[FT_NB];
 };
+// epoch_ptr: the launches read the epoch there and are recorded for a graph (no events, no host reads; the
+// counters stay on the device, ft_propose_counters).
 FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int greedy, const int32_t *d_vertices,
-                        int64_t count, int32_t *d_move_s, double *d_dS, int32_t *d_accept);
-int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_t *refused_out);
+                        int64_t count, int32_t *d_move_s, double *d_dS, int32_t *d_s_out,
+                        const uint32_t *epoch_ptr = nullptr);
+const unsigned long long *ft_propose_counters(sbp_ctx ctx);
+// Reserves every buffer a recorded sweep's proposal and commit launches use (batches of up to batch_max items).
+void ft_reserve(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int64_t batch_max);
+// Commit and rebuild; capture: no host reads (counters left on the device, ft_commit_counters), for graph capture.
+int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_t *refused_out, bool capture = false);
+const unsigned long long *ft_commit_counters(sbp_ctx ctx);
 // delta-MDL and Hastings correction of moving d_vertices[i] to d_targets[i] (device arrays of n), against bm.
 void ft_evaluate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int64_t n, const int32_t *d_vertices, const int32_t *d_targets,
                  double *d_dS, double *d_H);
-void shuffle_keys(sbp_ctx ctx, const int32_t *d_items, int64_t n, uint32_t epoch, int32_t *d_out);
-// Sets the launch shape of the four lane-group degree classes (lanes per vertex, wavefronts per SIMD of the greedy
-// parametric launch); false (nothing set) when one is not allowed.
+void shuffle_keys(sbp_ctx ctx, const int32_t *d_items, int64_t n, uint32_t epoch, int32_t *d_out,
+                  const uint32_t *epoch_ptr = nullptr);
+// Sets the launch shape of the degree classes (lanes or threads per vertex of classes 0-4, wavefronts per SIMD of the
+// greedy parametric lane-group launches); false (nothing set) when one is not allowed.
 bool set_finetune_shape(sbp_ctx ctx, const int32_t *lanes, const int32_t *waves_per_simd);
 // graph.hip
 void graph_finish(sbp_ctx ctx, sbp_graph g);

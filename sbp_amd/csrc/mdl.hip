@@ -21,12 +21,15 @@ using namespace sbpdev;
 
 constexpr int MDL_BLOCK = 256;
 
-// One partial sum and one error flag per workgroup; nothing needs to be cleared before the launch.
-__global__ void __launch_bounds__(MDL_BLOCK) k_log_posterior(int64_t nnz, int32_t B, const int32_t *__restrict__ row_ptr,
+// One partial sum and one error flag per workgroup; nothing needs to be cleared before the launch.  With
+// nnz_on_device, nnz is row_ptr[B] (a recorded reduction, launched for the largest nnz).
+__global__ void __launch_bounds__(MDL_BLOCK) k_log_posterior(int64_t nnz, int nnz_on_device, int32_t B,
+                                                             const int32_t *__restrict__ row_ptr,
                                                              const int2 *__restrict__ cells,
                                                              const int32_t *__restrict__ d_out,
                                                              const int32_t *__restrict__ d_in, Fx *__restrict__ partials,
                                                              int32_t *__restrict__ errs) {
+    if (nnz_on_device) nnz = row_ptr[B];
     __shared__ Fx wave_part[MDL_BLOCK / WAVE];
     __shared__ int32_t block_err;
     const int lane = lane_id(), wid = threadIdx.x / WAVE;
@@ -112,7 +115,7 @@ static double lp_run(sbp_ctx ctx, sbp_bm bm, int64_t blocks, void *work) {
     hipStream_t st = ctx->stream;
     Fx *partials = static_cast<Fx *>(work);
     int32_t *errs = reinterpret_cast<int32_t *>(partials + blocks + 1);
-    k_log_posterior<<<(int)blocks, MDL_BLOCK, 0, st>>>(bm->nnz, bm->B, bm->row_ptr, bm->row_cells, bm->d_out,
+    k_log_posterior<<<(int)blocks, MDL_BLOCK, 0, st>>>(bm->nnz, 0, bm->B, bm->row_ptr, bm->row_cells, bm->d_out,
                                                          bm->d_in, partials, errs);
     k_sum_partials<<<1, MDL_BLOCK, 0, st>>>((int)blocks, partials, errs, partials + blocks, errs + blocks);
     SBP_HIP(hipGetLastError());
@@ -132,6 +135,21 @@ double log_posterior(sbp_ctx ctx, sbp_bm bm) {
     if (bm->nnz == 0) return 0.0;
     const int64_t blocks = lp_blocks(bm->nnz, 4096);
     return lp_run(ctx, bm, blocks, ctx->scratch[S_PARTIAL].get(lp_work_bytes(blocks)));
+}
+
+void log_posterior_reserve(sbp_ctx ctx, int64_t max_nnz) {
+    ctx->scratch[S_PARTIAL].get(lp_work_bytes(lp_blocks(std::max<int64_t>(max_nnz, 1), 4096)));
+}
+
+void log_posterior_device(sbp_ctx ctx, sbp_bm bm, int64_t max_nnz, Fx *out_total, int32_t *out_err) {
+    hipStream_t st = ctx->stream;
+    const int64_t blocks = lp_blocks(std::max<int64_t>(max_nnz, 1), 4096);
+    Fx *partials = static_cast<Fx *>(ctx->scratch[S_PARTIAL].get(lp_work_bytes(blocks)));
+    int32_t *errs = reinterpret_cast<int32_t *>(partials + blocks + 1);
+    k_log_posterior<<<(int)blocks, MDL_BLOCK, 0, st>>>(max_nnz, 1, bm->B, bm->row_ptr, bm->row_cells, bm->d_out,
+                                                         bm->d_in, partials, errs);
+    k_sum_partials<<<1, MDL_BLOCK, 0, st>>>((int)blocks, partials, errs, out_total, out_err);
+    SBP_HIP(hipGetLastError());
 }
 
 // ------------------------------------------------------------------ allocation-free MDL (sbp_mdl_noalloc)
@@ -202,10 +220,10 @@ static const double *lgamma_table(sbp_ctx ctx, int64_t n, int64_t *n_out) {
         std::vector<double> h((size_t)n);
         for (int64_t x = 0; x < n; ++x) h[(size_t)x] = host_lgamma(x);
         SBP_HIP(hipStreamSynchronize(ctx->stream));
-        if (ctx->lgamma) SBP_HIP(hipFree(ctx->lgamma));
+        if (ctx->lgamma) SBP_HIP(dev_free(ctx->lgamma));
         ctx->lgamma = nullptr;
-        SBP_HIP(hipMalloc(&ctx->lgamma, sizeof(double) * (size_t)n));
-        SBP_HIP(hipMemcpy(ctx->lgamma, h.data(), sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+        SBP_HIP(dev_malloc(&ctx->lgamma, sizeof(double) * (size_t)n));
+        SBP_HIP(dev_copy(ctx->lgamma, h.data(), sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
         ctx->lgamma_n = n;
     }
     *n_out = ctx->lgamma_n;

@@ -139,8 +139,10 @@ __device__ __forceinline__ int key_digit(K k, int shift, int bits) {
 }
 
 template <typename K>
-__global__ void __launch_bounds__(PR_THREADS) k_digit_hist(const K *__restrict__ keys, int64_t n, int shift, int bits,
+__global__ void __launch_bounds__(PR_THREADS) k_digit_hist(const K *__restrict__ keys, int64_t n,
+                                                           const int64_t *__restrict__ dn, int shift, int bits,
                                                            int64_t tiles, int32_t *__restrict__ hist) {
+    if (dn) n = *dn;
     __shared__ int32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -169,8 +171,10 @@ __device__ __forceinline__ unsigned long long peers_of(int d) {
 template <typename K, typename V, bool HAS_VALS>
 __global__ void __launch_bounds__(PR_THREADS) k_digit_scatter(const K *__restrict__ keys, const V *__restrict__ vals,
                                                               K *__restrict__ keys_out, V *__restrict__ vals_out,
-                                                              int64_t n, int shift, int bits, int64_t tiles,
+                                                              int64_t n, const int64_t *__restrict__ dn, int shift,
+                                                              int bits, int64_t tiles,
                                                               const int32_t *__restrict__ offs) {
+    if (dn) n = *dn;
     __shared__ int32_t wcnt[PR_WAVES][256];
     __shared__ int32_t goff[256];
     const int lane = pr_lane(), w = (int)threadIdx.x / 64;
@@ -224,10 +228,12 @@ __global__ void __launch_bounds__(PR_THREADS) k_digit_scatter(const K *__restric
 
 // Stable ascending sort of keys by bits [begin_bit, end_bit) (keys agreeing there keep their order), values
 // permuted alike when vals_in is given.  keys_in / vals_in are left unchanged; the results are in keys_out /
-// vals_out.  Temporary: two key buffers (and value buffers), the histogram and its scan.
+// vals_out.  Temporary: two key buffers (and value buffers), the histogram and its scan.  With dn, the key count is
+// read on the device from *dn (at most n, which sizes the launches): no host read, so the sort can be captured in a
+// graph; n = 0 then still launches nothing.
 template <typename K, typename V = int32_t>
 hipError_t radix_sort(void *temp, size_t &temp_bytes, const K *keys_in, K *keys_out, const V *vals_in, V *vals_out,
-                      int64_t n, int begin_bit, int end_bit, hipStream_t st) {
+                      int64_t n, int begin_bit, int end_bit, hipStream_t st, const int64_t *dn = nullptr) {
     static_assert(std::is_unsigned<K>::value && (sizeof(K) == 4 || sizeof(K) == 8), "unsigned 32- or 64-bit keys");
     const int64_t tiles = ceil_div(n > 0 ? n : 1, RS_TILE);
     const int64_t hist_n = 256 * tiles;
@@ -250,6 +256,7 @@ hipError_t radix_sort(void *temp, size_t &temp_bytes, const K *keys_in, K *keys_
     void *scan_tmp = p + kb + vb + hb;
     const int passes = (int)ceil_div(end_bit > begin_bit ? end_bit - begin_bit : 0, 8);
     if (passes == 0) {
+        if (dn) return hipErrorInvalidValue;  // callers sort at least one digit
         hipError_t e = hipMemcpyAsync(keys_out, keys_in, sizeof(K) * (size_t)n, hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess && vals_in)
             e = hipMemcpyAsync(vals_out, vals_in, sizeof(V) * (size_t)n, hipMemcpyDeviceToDevice, st);
@@ -264,22 +271,39 @@ hipError_t radix_sort(void *temp, size_t &temp_bytes, const K *keys_in, K *keys_
         const bool to_out = ((passes - 1 - pass) % 2) == 0;
         K *dst_k = to_out ? keys_out : kt;
         V *dst_v = to_out ? vals_out : vt;
-        k_digit_hist<K><<<(unsigned)tiles, PR_THREADS, 0, st>>>(src_k, n, shift, bits, tiles, hist);
+        k_digit_hist<K><<<(unsigned)tiles, PR_THREADS, 0, st>>>(src_k, n, dn, shift, bits, tiles, hist);
         size_t sb = scan_bytes;
         hipError_t e = scan<int32_t>(scan_tmp, sb, (const int32_t *)hist, hist, hist_n, false, st);
         if (e != hipSuccess) return e;
         if (vals_in)
-            k_digit_scatter<K, V, true><<<(unsigned)tiles, PR_THREADS, 0, st>>>(src_k, src_v, dst_k, dst_v, n, shift,
-                                                                              bits, tiles, hist);
+            k_digit_scatter<K, V, true><<<(unsigned)tiles, PR_THREADS, 0, st>>>(src_k, src_v, dst_k, dst_v, n, dn,
+                                                                              shift, bits, tiles, hist);
         else
             k_digit_scatter<K, V, false><<<(unsigned)tiles, PR_THREADS, 0, st>>>(src_k, nullptr, dst_k, nullptr, n,
-                                                                               shift, bits, tiles, hist);
+                                                                               dn, shift, bits, tiles, hist);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
         src_k = dst_k;
         src_v = dst_v;
     }
     return hipSuccess;
+}
+
+// ---------------------------------------------------------------- fills
+
+// Fills with kernels rather than hipMemsetAsync: memset nodes recorded into a captured graph are avoided.
+template <typename T>
+__global__ void k_fill(T *__restrict__ p, int64_t n, T v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+template <typename T>
+hipError_t fill(T *p, int64_t n, T v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int64_t g = ceil_div(n, 256);
+    k_fill<T><<<(unsigned)(g < 16384 ? g : 16384), 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- run-length encoding
@@ -328,7 +352,7 @@ hipError_t run_length_encode(void *temp, size_t &temp_bytes, const K *keys, K *u
         return hipSuccess;
     }
     if (temp_bytes < need) return hipErrorInvalidValue;
-    if (n <= 0) return hipMemsetAsync(num_runs, 0, sizeof(int64_t), st);
+    if (n <= 0) return fill<int64_t>(num_runs, 1, 0, st);
     char *p = static_cast<char *>(temp);
     int32_t *run = reinterpret_cast<int32_t *>(p);
     int32_t *start = reinterpret_cast<int32_t *>(p + ab);

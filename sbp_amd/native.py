@@ -22,7 +22,7 @@ EXPORTS = [
     "sbp_bm_rebuild", "sbp_bm_move_vertex", "sbp_bm_merge_block", "sbp_ft_evaluate", "sbp_merge_evaluate",
     "sbp_merge_apply",
     "sbp_mdl", "sbp_mdl_noalloc", "sbp_bm_set_model", "sbp_np_description_length",
-    "sbp_ft_propose_all", "sbp_async_gibbs_sweep", "sbp_mh_sweep", "sbp_mcmc", "sbp_finetune_assignment",
+    "sbp_ft_propose_all", "sbp_async_gibbs_sweep", "sbp_async_gibbs_sweep_recorded", "sbp_mh_sweep", "sbp_mcmc", "sbp_finetune_assignment",
     "sbp_merge_propose", "sbp_merge_blocks",
     "sbp_partition",
     "sbp_ctx_memory", "sbp_ctx_trim", "sbp_set_carry_threads", "sbp_set_finetune_shape", "sbp_get_finetune_shape",
@@ -60,7 +60,8 @@ class Stats(C.Structure):
                 ("carry_reused", C.c_int64), ("carry_requeues", C.c_int64), ("carry_load_ms", C.c_double),
                 ("ft_high_bytes", C.c_uint64), ("ft_high_launches", C.c_int64), ("ft_high_kernel_ms", C.c_double),
                 ("mh_bytes", C.c_uint64), ("mh_launches", C.c_int64), ("mh_kernel_ms", C.c_double),
-                ("ft_class_bytes", C.c_uint64 * 6), ("ft_class_launches", C.c_int64 * 6), ("ft_class_ms", C.c_double * 6)]
+                ("ft_class_bytes", C.c_uint64 * 6), ("ft_class_launches", C.c_int64 * 6), ("ft_class_ms", C.c_double * 6),
+                ("recorded_sweeps", C.c_int64)]
 
     def as_dict(self):
         return {k: (list(getattr(self, k)) if k.startswith("ft_class_") else getattr(self, k)) for k, _ in self._fields_}
@@ -128,6 +129,7 @@ def lib():
             "sbp_np_description_length": [vp, vp, vp, vp, vp, vp, vp, vp],
             "sbp_ft_propose_all": [vp, vp, vp, u32, i32, vp, vp, vp],
             "sbp_async_gibbs_sweep": [vp, vp, vp, P, u32, S],
+            "sbp_async_gibbs_sweep_recorded": [vp, vp, vp, P, u32, vp, S],
             "sbp_mh_sweep": [vp, vp, vp, P, u32, i32, vp, S],
             "sbp_mcmc": [vp, vp, vp, P, i32, vp, vp, S],
             "sbp_finetune_assignment": [vp, vp, vp, P, vp, vp, S],
@@ -558,6 +560,14 @@ class DeviceBlockmodel:
         self.ctx.check(lib().sbp_async_gibbs_sweep(self.ctx.h, self.h, self.g.h, C.byref(prm), epoch, C.byref(st)),
                        "sbp_async_gibbs_sweep")
         return st
+
+    def async_gibbs_sweep_recorded(self, prm, epoch):
+        """One sweep plus the MDL as a replay of the recorded graph: (stats, mdl)."""
+        st = Stats()
+        m = C.c_double()
+        self.ctx.check(lib().sbp_async_gibbs_sweep_recorded(self.ctx.h, self.h, self.g.h, C.byref(prm), epoch,
+                                                            C.byref(m), C.byref(st)), "sbp_async_gibbs_sweep_recorded")
+        return st, m.value
 
     def mh_sweep(self, prm, epoch, ordered=False):
         st = Stats()
