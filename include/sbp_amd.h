@@ -289,6 +289,11 @@ int sbp_set_carry_threads(sbp_ctx ctx, int32_t threads);
  * (sbp_amd/autotune.py) sets. */
 int sbp_set_finetune_shape(sbp_ctx ctx, const int32_t lanes[5], const int32_t waves_per_simd[4]);
 int sbp_get_finetune_shape(sbp_ctx ctx, int32_t lanes[5], int32_t waves_per_simd[4]);
+/* Whether the partition driver (sbp_partition, sbp_mcmc, batches) runs asynchronous-Gibbs sweeps as replays of a
+ * recorded graph: -1 for graphs of at least 100,000 vertices (default; SBP_AMD_RECORDED_SWEEPS=0 or 1 overrides
+ * it), 0 never, 1 always.  Batch workers take the calling context's setting.  Results do not depend on it.  No
+ * counterpart in the reference. */
+int sbp_set_recorded_sweeps(sbp_ctx ctx, int32_t mode);
 /* Device bytes of a graph with N vertices and E edges, and an upper bound of the device bytes one partition
  * run of it needs (graph, blockmodels, scratch), given the largest |out(v)| + |in(v)|, the number of vertices
  * where that exceeds 256 and the algorithm (sbp_params.algorithm): the server admits requests by the latter. */

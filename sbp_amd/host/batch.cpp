@@ -115,6 +115,7 @@ int sbp_partition_batch(sbp_ctx ctx, const sbp_graph *graphs, int32_t n, const s
         for (sbp_ctx w : ctx->slots) {
             std::copy(ctx->ft_lanes, ctx->ft_lanes + 5, w->ft_lanes);
             std::copy(ctx->ft_occ, ctx->ft_occ + 4, w->ft_occ);
+            w->record_mode = ctx->record_mode;
         }
         SBP_HIP(hipStreamSynchronize(ctx->stream));  // graphs uploaded on ctx are complete
         // longest first (by edges, then vertices), which bounds the last worker's tail

@@ -26,6 +26,7 @@ EXPORTS = [
     "sbp_merge_propose", "sbp_merge_blocks",
     "sbp_partition",
     "sbp_ctx_memory", "sbp_ctx_trim", "sbp_set_carry_threads", "sbp_set_finetune_shape", "sbp_get_finetune_shape",
+    "sbp_set_recorded_sweeps",
     "sbp_graph_bytes", "sbp_footprint_bytes",
     "sbp_partition_batch",
     "sbp_server_create", "sbp_server_last_error", "sbp_server_submit", "sbp_server_wait", "sbp_server_info",
@@ -145,6 +146,7 @@ def lib():
             "sbp_set_carry_threads": [vp, i32],
             "sbp_set_finetune_shape": [vp, vp, vp],
             "sbp_get_finetune_shape": [vp, vp, vp],
+            "sbp_set_recorded_sweeps": [vp, i32],
             "sbp_partition_batch": [vp, vp, i32, vp, i32, vp, vp, vp, vp],
             "sbp_server_create": [C.c_int, i32, u64, vp],
             "sbp_server_submit": [vp, i32, i64, vp, vp, vp, vp, P, vp],
@@ -260,6 +262,10 @@ class Context:
         wa = np.ascontiguousarray(waves_per_simd, np.int32)
         assert la.shape == (5,) and wa.shape == (4,)
         self.check(lib().sbp_set_finetune_shape(self.h, _p(la), _p(wa)), "sbp_set_finetune_shape")
+
+    def set_recorded_sweeps(self, mode):
+        """-1: graph replays for graphs of >= 100,000 vertices, 0 never, 1 always (sbp_set_recorded_sweeps)."""
+        self.check(lib().sbp_set_recorded_sweeps(self.h, mode), "sbp_set_recorded_sweeps")
 
     def finetune_shape(self):
         la = np.zeros(5, np.int32)

@@ -87,3 +87,25 @@ def test_recording_grows_the_cells_of_a_sparse_block_matrix():
         same(eager.export(), rec.export())
         assert m == eager.mdl()[0]
     ctx.close()
+
+
+def test_partitions_with_replayed_sweeps_match_the_oracle_and_each_other():
+    """Full partitions with every finetune phase replayed from recorded graphs (forced on for small graphs): alone
+    they equal the oracle's partition; as a batch on several worker slots, where one worker records while others
+    allocate, free and copy, each equals the graph alone."""
+    from sbp_amd import workloads
+    graphs = workloads.generate([workloads.batched_spec(i) for i in range(10)])
+    ctx = native.Context(0)
+    ctx.set_recorded_sweeps(1)
+    dgs = [ctx.upload(g) for g in graphs]
+    prms = [native.params("async_gibbs", seed=i) for i in range(len(graphs))]
+    solo = [dg.partition(p) for dg, p in zip(dgs, prms)]
+    assert all(s[2].recorded_sweeps > 0 for s in solo)
+    for i in (2, 8):  # 4,585 and 3,367 vertices
+        a_o, nb_o, st_o = O.sbp(O.OGraph.from_csr(graphs[i]), O.params("async_gibbs", seed=i, threads=8))
+        assert nb_o == solo[i][1] and np.array_equal(a_o, solo[i][0]) and st_o.mdl == solo[i][2].mdl
+    outs, nb, st, _ = ctx.partition_batch(dgs, prms, slots=6)
+    for i in range(len(graphs)):
+        assert int(nb[i]) == solo[i][1] and np.array_equal(outs[i], solo[i][0]) and st[i].mdl == solo[i][2].mdl, i
+        assert st[i].recorded_sweeps > 0
+    ctx.close()
