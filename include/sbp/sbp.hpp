@@ -289,6 +289,13 @@ void carry_out_best_merges_advanced(Blockmodel &blockmodel, const std::vector<do
                                     const std::vector<long> &best_merge_for_each_block, const Graph &graph);
 }  // namespace block_merge
 
+namespace divisive {
+/// src/top_down_sbp.cpp:488-549 with random splits (the default --split) and apply_best_splits (:30-66): the
+/// blockmodel's blocks are split in place, up to target_num_communities blocks, and it is returned.  Consumes one
+/// epoch of the replay counter.  std::runtime_error for a degree-corrected blockmodel.
+Blockmodel split_communities(Blockmodel &blockmodel, const Graph &graph, int target_num_communities);
+}  // namespace divisive
+
 namespace finetune {
 /// src/finetune.cpp:754-771.  Dispatches on args.algorithm (and args.asynciterations).
 Blockmodel &mcmc(int iteration, const Graph &graph, Blockmodel &blockmodel, BlockmodelTriplet &blockmodel_triplet);

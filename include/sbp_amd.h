@@ -258,6 +258,21 @@ int sbp_merge_apply(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const int32_t *best_s, 
 int sbp_merge_blocks(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int64_t num_to_merge,
                      int32_t approximate, sbp_bm *out, sbp_stats *stats);
 
+/* ----------------------------------------------------------------- divisive step */
+
+/* Replaces divisive::split_communities (src/top_down_sbp.cpp:488-549) with random splits (propose_random_split,
+ * :228-240, the default --split) and apply_best_splits (:30-66), in place: blocks of at least 0.001 N vertices
+ * each draw 10 random two-way splits, keep the one of lowest nonparametric::mdl(two-block model of the block's
+ * induced subgraph) - null_mdl_v1(subgraph), and the best splits are applied in ascending order of that value
+ * until the block count reaches target (at least one is applied when any block kept one); the blockmodel is then
+ * rebuilt (initialize_edge_counts).  null_mdl_v1 is the nonparametric one for a nonparametric blockmodel
+ * (args.nonparametric), the closed form otherwise.  Draws are keyed by (seed; block or vertex, epoch), so a
+ * divisive driver passes a new epoch per call.  *num_blocks_out: the new block count.  delta_out (may be NULL):
+ * host double[B] of the old block count, each block's kept value (DBL_MAX for a block not split).
+ * SBP_E_ARG for a degree-corrected blockmodel. */
+int sbp_split_communities(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t target, uint32_t epoch,
+                          int32_t *num_blocks_out, double *delta_out);
+
 /* ----------------------------------------------------------------- driver */
 
 /* Replaces sbp::stochastic_block_partition (src/sbp.cpp:16-72) with the golden-section triplet

@@ -92,6 +92,7 @@ def lib():
             "orc_extend": (C.c_int, [G, vp, vp, i32, u64, vp]),
             "orc_detach": (i64, [G, vp, vp]),
             "orc_reattach": (C.c_int, [G, vp, vp, i32, u64, vp]),
+            "orc_split_communities": (i32, [G, vp, i32, i32, u64, u32, i32, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -352,3 +353,12 @@ def extend(graph, mapping, sample_assignment, sample_blocks, seed=0):
     sa = np.ascontiguousarray(sample_assignment, np.int32)
     lib().orc_extend(graph.ref(), _p(m), _p(sa), sample_blocks, seed, _p(out))
     return out
+
+
+def split_communities(graph, assignment, B, target, seed=0, epoch=0, nonparametric=False):
+    """orc_split_communities (divisive::split_communities): (new assignment, new block count, delta[B])."""
+    a = np.array(assignment, np.int32, copy=True)
+    delta = np.zeros(max(B, 1), np.float64)
+    nb = lib().orc_split_communities(graph.ref(), _p(a), B, target, seed, epoch, int(bool(nonparametric)),
+                                     _p(delta))
+    return a, nb, delta[:B]

@@ -2169,3 +2169,127 @@ int orc_extend(const orc_graph *g, const int32_t *mapping, const int32_t *sample
     free(touched);
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Divisive step: divisive::split_communities (src/top_down_sbp.cpp:488-549) */
+/* with propose_split / propose_random_split (:191-240), extract_subgraph     */
+/* (:98-132) and apply_best_splits (:30-66), literally: each block's induced  */
+/* subgraph is built, each proposal builds the two-block blockmodel of it and */
+/* evaluates nonparametric::mdl on it.  Draws follow the replay contract:     */
+/* proposal p of block b has size lo + uniform_below(w, hi - lo + 1), w word  */
+/* p % 4 of Philox(seed; b, epoch, ORC_TAG_SPLIT_SIZE, p / 4); subgraph       */
+/* vertex i (full id v) is on side 0 when uniform_below(w, n) < size, w word  */
+/* p % 4 of Philox(seed; v, epoch, ORC_TAG_SPLIT_SIDE, p / 4).                */
+/* ------------------------------------------------------------------------- */
+#define ORC_SPLIT_PROPOSALS 10  /* NUM_AGG_PROPOSALS_PER_BLOCK (include/top_down_sbp.hpp:28) */
+
+static void split_sides(uint64_t seed, uint32_t epoch, int32_t b, int32_t p, const int32_t *vertices, int32_t n,
+                        int32_t *side) {
+    uint32_t w[4];
+    orc_philox(seed, (uint32_t)b, epoch, ORC_TAG_SPLIT_SIZE, (uint32_t)(p / 4), w);
+    /* common::random_integer(n * 0.2, n * 0.8): long parameters, so the bounds truncate */
+    long lo = (long)((double)n * 0.2), hi = (long)((double)n * 0.8);
+    long k = lo + (long)orc_uniform_below(w[p % 4], (uint32_t)(hi - lo + 1));
+    for (int32_t i = 0; i < n; ++i) {
+        orc_philox(seed, (uint32_t)vertices[i], epoch, ORC_TAG_SPLIT_SIDE, (uint32_t)(p / 4), w);
+        /* discrete_distribution({k, n - k}): side 0 with probability k / n */
+        side[i] = (long)orc_uniform_below(w[p % 4], (uint32_t)n) < k ? 0 : 1;
+    }
+}
+
+/* A stable sort of block ids by descending value (utils::argsort, include/utils.hpp:165-172). */
+static void argsort_desc(const double *val, int32_t *idx, int32_t *tmp, int32_t n) {
+    for (int32_t w = 1; w < n; w *= 2) {
+        for (int32_t lo = 0; lo < n; lo += 2 * w) {
+            int32_t mid = lo + w < n ? lo + w : n, hi = lo + 2 * w < n ? lo + 2 * w : n;
+            int32_t a = lo, c = mid, o = lo;
+            while (a < mid && c < hi) tmp[o++] = (val[idx[c]] > val[idx[a]]) ? idx[c++] : idx[a++];
+            while (a < mid) tmp[o++] = idx[a++];
+            while (c < hi) tmp[o++] = idx[c++];
+        }
+        memcpy(idx, tmp, sizeof(int32_t) * (size_t)n);
+    }
+}
+
+int32_t orc_split_communities(const orc_graph *g, int32_t *assign, int32_t B, int32_t target, uint64_t seed,
+                              uint32_t epoch, int32_t nonparametric, double *delta_out) {
+    int32_t N = g->N;
+    int32_t *size = (int32_t *)calloc((size_t)B + 1, sizeof(int32_t));
+    for (int32_t v = 0; v < N; ++v) size[assign[v]]++;
+    int32_t *mapping = (int32_t *)malloc(sizeof(int32_t) * (size_t)(N + 1));
+    int32_t *vertices = (int32_t *)malloc(sizeof(int32_t) * (size_t)(N + 1));
+    int32_t *side = (int32_t *)malloc(sizeof(int32_t) * (size_t)(N + 1));
+    int32_t *zeros = (int32_t *)calloc((size_t)N + 1, sizeof(int32_t));
+    int64_t cap = g->E > 0 ? g->E : 1;
+    int64_t *out_off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(N + 2));
+    int64_t *in_off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(N + 2));
+    int32_t *out_adj = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap);
+    int32_t *in_adj = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap);
+    double *delta = (double *)malloc(sizeof(double) * (size_t)(B + 1));
+    int32_t *choice = (int32_t *)malloc(sizeof(int32_t) * (size_t)(B + 1));
+    for (int32_t v = 0; v < N; ++v) mapping[v] = -1;
+    for (int32_t b = 0; b < B; ++b) {
+        delta[b] = DBL_MAX;
+        choice[b] = -1;
+        /* extract_subgraph: the block's vertices ascending, translator[vertex] = index */
+        int32_t n = 0;
+        for (int32_t v = 0; v < N; ++v)
+            if (assign[v] == b) { mapping[v] = n; vertices[n++] = v; }
+        /* Do not attempt to split small clusters (:506) */
+        if ((double)size[b] < 0.001 * (double)N) {
+            for (int32_t i = 0; i < n; ++i) mapping[vertices[i]] = -1;
+            continue;
+        }
+        orc_graph sub = {n, 0, out_off, out_adj, in_off, in_adj};
+        if (n < 2) {  /* a subgraph of fewer than 2 vertices keeps no edges (:114-116) */
+            for (int32_t i = 0; i <= n; ++i) { out_off[i] = 0; in_off[i] = 0; }
+        } else {
+            sub.E = orc_subgraph(g, mapping, vertices, n, out_off, out_adj, in_off, in_adj);
+        }
+        double old_entropy;
+        if (nonparametric) {  /* null_mdl_v1: the one-block nonparametric model of the subgraph */
+            orc_bm *null_model = orc_bm_new(&sub, 1, zeros);
+            old_entropy = orc_np_mdl(null_model, &sub);
+            orc_bm_free(null_model);
+        } else {
+            old_entropy = orc_null_mdl_v1(sub.E);
+        }
+        for (int32_t p = 0; p < ORC_SPLIT_PROPOSALS; ++p) {
+            orc_bm *split;
+            if (n < 2) split = orc_bm_new(&sub, 1, zeros);
+            else {
+                split_sides(seed, epoch, b, p, vertices, n, side);
+                split = orc_bm_new(&sub, 2, side);
+            }
+            double new_entropy = orc_np_mdl(split, &sub);
+            orc_bm_free(split);
+            double d = new_entropy - old_entropy;
+            if (d < delta[b]) { delta[b] = d; choice[b] = p; }
+        }
+        for (int32_t i = 0; i < n; ++i) mapping[vertices[i]] = -1;
+    }
+    if (delta_out) memcpy(delta_out, delta, sizeof(double) * (size_t)B);
+    /* apply_best_splits */
+    int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)(B + 1));
+    int32_t *tmp = (int32_t *)malloc(sizeof(int32_t) * (size_t)(B + 1));
+    for (int32_t b = 0; b < B; ++b) order[b] = b;
+    argsort_desc(delta, order, tmp, B);
+    int32_t num_blocks = B;
+    for (int32_t j = B - 1; j > -1; --j) {
+        int32_t b = order[j];
+        if (choice[b] < 0) continue;  /* the default Split: num_vertices = LONG_MAX > N */
+        int32_t n = 0;
+        for (int32_t v = 0; v < N; ++v)
+            if (assign[v] == b) vertices[n++] = v;
+        if (n >= 2) {
+            split_sides(seed, epoch, b, choice[b], vertices, n, side);
+            for (int32_t i = 0; i < n; ++i)
+                if (side[i] == 1) assign[vertices[i]] = num_blocks;
+        }
+        num_blocks++;
+        if (num_blocks >= target) break;
+    }
+    free(order); free(tmp); free(delta); free(choice); free(size); free(mapping); free(vertices); free(side);
+    free(zeros); free(out_off); free(in_off); free(out_adj); free(in_adj);
+    return num_blocks;
+}

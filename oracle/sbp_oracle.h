@@ -78,6 +78,8 @@ extern "C" {
 #define ORC_TAG_SAMPLE  0x53410001u  /* random sampling key: unit = vertex */
 #define ORC_TAG_SNOWBALL 0x53410002u  /* snowball samplers: unit = draw index */
 #define ORC_TAG_REATTACH 0x52410001u  /* reattach: unit = draw index */
+#define ORC_TAG_SPLIT_SIZE 0x53500001u  /* split size: unit = block, sub = proposal / 4 */
+#define ORC_TAG_SPLIT_SIDE 0x53500002u  /* split side: unit = vertex, sub = proposal / 4 */
 
 typedef struct {
     int32_t N;
@@ -191,6 +193,11 @@ int64_t  orc_subgraph(const orc_graph *g, const int32_t *mapping, const int32_t 
 int      orc_extend(const orc_graph *g, const int32_t *mapping, const int32_t *sample_assignment,
                     int32_t sample_blocks, uint64_t seed, int32_t *assignment_out);
 int64_t  orc_detach(const orc_graph *g, int32_t *mapping, int32_t *vertices);
+/* divisive::split_communities (src/top_down_sbp.cpp:488-549), random splits, in place on assign (N entries,
+ * blocks 0..B-1); nonparametric selects null_mdl_v1's form.  Returns the new block count; delta_out (B
+ * entries, may be NULL) gets each block's kept value (DBL_MAX when not split). */
+int32_t  orc_split_communities(const orc_graph *g, int32_t *assign, int32_t B, int32_t target, uint64_t seed,
+                               uint32_t epoch, int32_t nonparametric, double *delta_out);
 int      orc_reattach(const orc_graph *g, const int32_t *mapping, const int32_t *detached_assignment,
                       int32_t detached_blocks, uint64_t seed, int32_t *assignment_out);
 

@@ -1048,6 +1048,17 @@ int sbp_merge_apply(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const int32_t *best_s, 
     SBP_API_END(ctx)
 }
 
+int sbp_split_communities(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t target, uint32_t epoch,
+                          int32_t *num_blocks_out, double *delta_out) {
+    if (!ctx || !bm || !g || !num_blocks_out) return SBP_E_ARG;
+    SBP_API_BEGIN
+    std::vector<double> delta;
+    const int32_t B = bm->B;
+    *num_blocks_out = split_communities(ctx, g, bm, target, epoch, delta_out ? &delta : nullptr);
+    if (delta_out) std::copy(delta.begin(), delta.begin() + B, delta_out);
+    SBP_API_END(ctx)
+}
+
 int sbp_ft_evaluate(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int64_t n, const int32_t *vertices, const int32_t *targets,
                     double *dS_out, double *hastings_out) {
     if (!ctx || !bm || !g || n < 0 || (n > 0 && (!vertices || !targets))) return SBP_E_ARG;

@@ -21,6 +21,8 @@ constexpr uint32_t TAG_MERGE = 0x4d470001u;
 constexpr uint32_t TAG_SHUFFLE = 0x53480001u;
 constexpr uint32_t TAG_EXTEND = 0x45580001u;
 constexpr uint32_t TAG_SAMPLE = 0x53410001u;
+constexpr uint32_t TAG_SPLIT_SIZE = 0x53500001u;  // split size: unit = block, sub = proposal / 4
+constexpr uint32_t TAG_SPLIT_SIDE = 0x53500002u;  // split side: unit = vertex, sub = proposal / 4
 
 constexpr int WAVE = 64;
 

@@ -354,6 +354,9 @@ double host_edges_dl(int64_t B, int64_t E);
 int64_t count_nonempty(sbp_ctx ctx, sbp_bm bm);
 sbpdev::NpConst np_const(sbp_ctx ctx, sbp_bm bm);
 double np_mdl(sbp_ctx ctx, sbp_graph g, sbp_bm bm, double *parts);
+// split.hip
+int32_t split_communities(sbp_ctx ctx, sbp_graph g, sbp_bm bm, int32_t target, uint32_t epoch,
+                          std::vector<double> *delta_out);
 // finetune.hip
 struct FtLaunchInfo {
     uint64_t bytes;       // algorithmic bytes of the lane-group launches (vertices up to FT_SMALL_DEGREE)

@@ -504,6 +504,20 @@ Blockmodel &merge_blocks(Blockmodel &blockmodel, const Graph &graph, long num_ed
 
 }  // namespace block_merge
 
+namespace divisive {
+
+Blockmodel split_communities(Blockmodel &blockmodel, const Graph &graph, int target_num_communities) {
+    sbp_ctx ctx = blockmodel.ctx();
+    int32_t num_blocks = 0;
+    throw_on(sbp_split_communities(ctx, blockmodel.handle(), graph.device(ctx), (int32_t)target_num_communities,
+                                   sbp_amd::epoch()++, &num_blocks, nullptr),
+             ctx, "sbp_split_communities");
+    blockmodel.set_graph(&graph);
+    return blockmodel;
+}
+
+}  // namespace divisive
+
 namespace finetune {
 
 static Blockmodel &run_mcmc(Blockmodel &bm, const Graph &graph, int algorithm, bool grnr) {

@@ -23,7 +23,7 @@ EXPORTS = [
     "sbp_merge_apply",
     "sbp_mdl", "sbp_mdl_noalloc", "sbp_bm_set_model", "sbp_np_description_length",
     "sbp_ft_propose_all", "sbp_async_gibbs_sweep", "sbp_async_gibbs_sweep_recorded", "sbp_mh_sweep", "sbp_mcmc", "sbp_finetune_assignment",
-    "sbp_merge_propose", "sbp_merge_blocks",
+    "sbp_merge_propose", "sbp_merge_blocks", "sbp_split_communities",
     "sbp_partition",
     "sbp_ctx_memory", "sbp_ctx_trim", "sbp_set_carry_threads", "sbp_set_finetune_shape", "sbp_get_finetune_shape",
     "sbp_set_recorded_sweeps",
@@ -136,6 +136,7 @@ def lib():
             "sbp_finetune_assignment": [vp, vp, vp, P, vp, vp, S],
             "sbp_merge_propose": [vp, vp, u32, vp, vp],
             "sbp_merge_blocks": [vp, vp, vp, u32, i64, i32, vp, S],
+            "sbp_split_communities": [vp, vp, vp, i32, u32, vp, vp],
             "sbp_partition": [vp, vp, P, vp, vp, S],
             "sbp_debug_logf": [vp, u32, u32, vp],
             "sbp_debug_sort": [vp, i64, i32, vp, vp, i32, i32, vp, vp],
@@ -642,6 +643,15 @@ class DeviceBlockmodel:
         dS = np.zeros(B, np.float64)
         self.ctx.check(lib().sbp_merge_propose(self.ctx.h, self.h, epoch, _p(s), _p(dS)), "sbp_merge_propose")
         return s, dS
+
+    def split_communities(self, target, epoch):
+        """divisive::split_communities in place: (new block count, each old block's kept split value)."""
+        B = self.B
+        nb = C.c_int32()
+        delta = np.zeros(max(B, 1), np.float64)
+        self.ctx.check(lib().sbp_split_communities(self.ctx.h, self.h, self.g.h, int(target), int(epoch),
+                                                   C.byref(nb), _p(delta)), "sbp_split_communities")
+        return nb.value, delta[:B]
 
     def merge_blocks(self, epoch, num_to_merge, approximate=False):
         h = C.c_void_p()

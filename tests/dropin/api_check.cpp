@@ -1,6 +1,6 @@
-// Exercises the drop-in C++ surface the way the reference's own callers use it (src/finetune.cpp:842-869 and
-// src/block_merge.cpp:40-117), on a graph given as a TSV path: prints one line per checked value, which the GPU
-// test compares with the oracle.
+// Exercises the drop-in C++ surface the way the reference's own callers use it (src/finetune.cpp:842-869,
+// src/block_merge.cpp:40-117 and src/top_down_sbp.cpp:488-549), on a graph given as a TSV path: prints one line
+// per checked value, which the GPU test compares with the oracle.
 //   api_check <graph base path> <num_blocks> <seed>
 #include <cstdio>
 #include <cstdlib>
@@ -15,6 +15,7 @@
 #include "globals.hpp"
 #include "graph.hpp"
 #include "rng.hpp"
+#include "top_down_sbp.hpp"
 #include "utils.hpp"
 
 int main(int argc, char *argv[]) {
@@ -70,6 +71,12 @@ int main(int argc, char *argv[]) {
     std::vector<long> a = moved.block_assignment();
     printf("assignment");
     for (long x : a) printf(" %ld", x);
+    printf("\n");
+    // the divisive step on the first blockmodel (src/top_down_sbp.cpp:488-549), the first replay epoch
+    Blockmodel split = divisive::split_communities(blockmodel, graph, (int)(B + B / 2));
+    std::vector<long> sa = split.block_assignment();
+    printf("split %ld", split.num_blocks());
+    for (long x : sa) printf(" %ld", x);
     printf("\n");
     return 0;
 }

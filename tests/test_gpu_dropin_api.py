@@ -72,6 +72,11 @@ def test_api_check_program_matches_oracle(tmp_path):
     assert float(after["after_merge"][0]) == O.OBlockmodel.build(og, B, a).mdl()
     assert after["after_merge"][1] == "1"
     assert np.array_equal(np.array(after["assignment"], dtype=np.int32), a)
+    # divisive::split_communities on the first blockmodel, parametric null model, epoch 0
+    split = [l for l in lines if l[0] == "split"][0]
+    sa, snb, _ = O.split_communities(og, assign, B, B + B // 2, seed=0, epoch=0, nonparametric=False)
+    assert int(split[1]) == snb == B + B // 2
+    assert np.array_equal(np.array(split[2:], dtype=np.int32), sa)
 
 
 def hubby(seed=5):
