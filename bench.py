@@ -13,6 +13,8 @@ and a CPU baseline: the oracle's OpenMP asynchronous-Gibbs sweeps on the same gr
 Further legs, each once per run:
 - "sampled": the same graph's whole sampled flow (degree-weighted 50% sample, partition of the sample, extension
   to the full graph, finetune_assignment) with wall time per phase.
+- "split": divisive::split_communities (random splits) on the same graph, from one block (the top-down start,
+  target 2) and from the planted partition (target twice its blocks), with the wall time of each call.
 - "c2": the 200,000-vertex workload (500 lognormal(0.5) blocks, power law 2.1 on [1, 2000], mean 20, p_in 0.6):
   full agglomerative partitions with asynchronous-Gibbs finetune, against the oracle's committed partition.
 - "c1": the 1,000-vertex workload (16 equal blocks, power law 2.5 on [1, 100], mean 10, p_in 0.75): full
@@ -385,6 +387,30 @@ def sampled_leg(ctx, dg, g, w, sampler, algorithm="async_gibbs"):
     }
 
 
+def split_leg(ctx, dg, g, w, calls=3):
+    """divisive::split_communities (src/top_down_sbp.cpp:488-549) on the large graph, from one block (the top-down
+    start, target 2) and from the planted partition (target 2B); each start is rebuilt before every call, which
+    uses its own epoch.  A call includes the vertex and edge counting passes, the host description lengths of
+    10 B two-block models and the blockmodel rebuild."""
+    from sbp_amd import native
+    out = {}
+    for name, B, assign, target in (("one_block", 1, np.zeros(g.N, np.int32), 2),
+                                    ("planted", w["B"], np.asarray(g.truth, np.int32), 2 * w["B"])):
+        ms, nb = [], None
+        for call in range(calls):
+            db = native.DeviceBlockmodel.build(dg, B, assign)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            nb, delta = db.split_communities(target, call)
+            ms.append((time.perf_counter() - t0) * 1e3)
+            del db
+        out[name] = {"blocks": B, "target": target, "blocks_after": nb, "ms_per_call": ms,
+                     "best_ms": min(ms)}
+        print(f"[bench] split {name}: B {B} -> {nb} in {min(ms):.1f} ms (best of {calls})", file=sys.stderr,
+              flush=True)
+    return out
+
+
 def c2_leg(runs):
     """Full agglomerative partitions of the 200,000-vertex graph (asynchronous Gibbs), compared with the oracle's
     partition committed in tests/golden/partition_200k.npz."""
@@ -468,7 +494,7 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_bytes": n, "copies": 3}))
 
 
-LEGS = ("sampled", "c2", "c1", "batched", "served")
+LEGS = ("sampled", "split", "c2", "c1", "batched", "served")
 
 
 def main():
@@ -526,7 +552,7 @@ def main():
 
     out = {}
     line = None
-    if args.only in (None, "headline", "sampled"):
+    if args.only in (None, "headline", "sampled", "split"):
         t0 = time.perf_counter()
         g, w = make_graph("c3", args.n)
         print(f"[bench] graph N={g.N} E={g.E} generated in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
@@ -566,6 +592,8 @@ def main():
                 "finetune_shape": {"lanes": ctx.finetune_shape()[0], "waves_per_simd": ctx.finetune_shape()[1]},
                 "cpu_baseline": cpu,
             }
+        if "split" in legs:
+            out["split"] = split_leg(ctx, dg, g, w)
         if "sampled" in legs:
             out["sampled"] = sampled_leg(ctx, dg, g, w, args.sampler)
         del dg

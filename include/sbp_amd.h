@@ -91,6 +91,8 @@ typedef struct {
     double ft_class_ms[6];
     int64_t recorded_sweeps;     /* asynchronous-Gibbs sweeps run as replays of a recorded HIP graph (their
                                   * per-class launch times are not taken; finetune_ms holds the replays' time) */
+    int64_t cell_updates;        /* asynchronous-Gibbs commits whose block matrix was updated from the moved
+                                  * vertices' edges instead of rebuilt from every edge */
 } sbp_stats;
 
 /* ----------------------------------------------------------------- context */

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsbp_amd.so")
-SOURCES = ["csrc/build.hip", "csrc/mdl.hip", "csrc/finetune.hip", "csrc/merge.hip", "csrc/graph.hip", "csrc/split.hip",
+SOURCES = ["csrc/build.hip", "csrc/mdl.hip", "csrc/finetune.hip", "csrc/merge.hip", "csrc/graph.hip", "csrc/split.hip", "csrc/update.hip",
            "csrc/capi.hip",
            "host/sbp_api.cpp", "host/reference_api.cpp", "host/args.cpp", "host/batch.cpp", "host/evaluate.cpp"]
 INCLUDE_SBP = os.path.join(os.path.dirname(HERE), "include", "sbp")  # the reference's header names

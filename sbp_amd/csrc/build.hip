@@ -19,6 +19,7 @@ static int bits_for(int64_t x) {  // number of bits to represent values in [0, x
     while (b < 62 && (int64_t(1) << b) < x) ++b;
     return b < 1 ? 1 : b;
 }
+int bits_for_blocks(int64_t B) { return bits_for(B); }
 
 __global__ void k_make_keys(int64_t E, const int32_t *__restrict__ src, const int32_t *__restrict__ dst,
                             const int32_t *__restrict__ assign, int cbits, unsigned long long *__restrict__ keys) {
@@ -122,7 +123,7 @@ void bm_alloc(sbp_bm bm, int32_t B, int32_t N) {
 void bm_release(sbp_bm bm) {
     void *ptrs[] = {bm->assign,    bm->sizes,     bm->d_out,   bm->d_in,    bm->d_tot,    bm->row_ptr,
                     bm->col_ptr,   bm->row_cells, bm->col_cells, bm->row_cum, bm->col_cum, bm->cell_hash,
-                    bm->hist};
+                    bm->hist,      bm->row_keys,  bm->col_keys};
     if (bm->pool) {
         for (void *p : ptrs) bm->pool->release(p);
     } else {
@@ -137,6 +138,8 @@ void bm_release(sbp_bm bm) {
     bm->assign = bm->sizes = bm->d_out = bm->d_in = bm->d_tot = bm->row_ptr = bm->col_ptr = nullptr;
     bm->row_cells = bm->col_cells = nullptr;
     bm->row_cum = bm->col_cum = nullptr;
+    bm->row_keys = bm->col_keys = nullptr;
+    bm->keys_valid = false;
     bm->cap = 0;
     bm->bcap = 0;
 }
@@ -144,13 +147,16 @@ void bm_release(sbp_bm bm) {
 void bm_ensure_cells(sbp_bm bm, int64_t nnz) {
     if (nnz > bm->cap) {
         DevicePool &pool = pool_of(bm);
-        void *old[] = {bm->row_cells, bm->col_cells, bm->row_cum, bm->col_cum};
+        void *old[] = {bm->row_cells, bm->col_cells, bm->row_cum, bm->col_cum, bm->row_keys, bm->col_keys};
         for (void *p : old) pool.release(p);
         int64_t cap = nnz + nnz / 8 + 64;
         bm->row_cells = static_cast<int2 *>(pool.alloc(sizeof(int2) * (size_t)cap));
         bm->col_cells = static_cast<int2 *>(pool.alloc(sizeof(int2) * (size_t)cap));
         bm->row_cum = static_cast<int32_t *>(pool.alloc(sizeof(int32_t) * (size_t)cap));
         bm->col_cum = static_cast<int32_t *>(pool.alloc(sizeof(int32_t) * (size_t)cap));
+        bm->row_keys = static_cast<unsigned long long *>(pool.alloc(sizeof(unsigned long long) * (size_t)cap));
+        bm->col_keys = static_cast<unsigned long long *>(pool.alloc(sizeof(unsigned long long) * (size_t)cap));
+        bm->keys_valid = false;
         bm->cap = cap;
     }
 }
@@ -250,6 +256,7 @@ void bm_ensure_hash(sbp_ctx ctx, sbp_bm bm, bool capture) {
     }
     // a recorded hash is rebuilt by each replay; outside the graph the matrix it describes has changed
     bm->hash_valid = !capture;
+    bm->hash_used = bm->nnz;
 }
 
 __global__ void k_hist_clear(int64_t n, sbpdev::HistSlot *__restrict__ t) {
@@ -340,12 +347,15 @@ void bm_reserve_cells(sbp_ctx ctx, sbp_graph g, sbp_bm bm) {
             SBP_HIP(hipMemcpyAsync(rcum, bm->row_cum, sizeof(int32_t) * bm->nnz, hipMemcpyDeviceToDevice, st));
             SBP_HIP(hipMemcpyAsync(ccum, bm->col_cum, sizeof(int32_t) * bm->nnz, hipMemcpyDeviceToDevice, st));
         }
-        void *old[] = {bm->row_cells, bm->col_cells, bm->row_cum, bm->col_cum};
+        void *old[] = {bm->row_cells, bm->col_cells, bm->row_cum, bm->col_cum, bm->row_keys, bm->col_keys};
         for (void *p : old) pool.release(p);  // reuse is ordered by the stream, after the copies
         bm->row_cells = rc;
         bm->col_cells = cc;
         bm->row_cum = rcum;
         bm->col_cum = ccum;
+        bm->row_keys = static_cast<unsigned long long *>(pool.alloc(sizeof(unsigned long long) * (size_t)bound));
+        bm->col_keys = static_cast<unsigned long long *>(pool.alloc(sizeof(unsigned long long) * (size_t)bound));
+        bm->keys_valid = false;
         bm->cap = bound;
         bm->hash_valid = false;
     }
@@ -396,9 +406,8 @@ void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm, bool keep_hist, bool captur
     const int cbits = bits_for(B);
     const int end_bit = 2 * cbits;
     SBP_CHECK(end_bit <= 64, SBP_E_ARG, "block count too large for packed keys");
-    SBP_HIP(sbpprim::fill<int32_t>(bm->sizes, B, 0, st));
-    if (B <= SIZES_LDS_MAX) k_sizes_lds<<<std::min(grid_for(bm->N), 512), 256, 0, st>>>(bm->N, B, bm->assign, bm->sizes);
-    else k_sizes<<<grid_for(bm->N), 256, 0, st>>>(bm->N, bm->assign, bm->sizes);
+    bm->keys_valid = false;
+    bm_count_sizes(ctx, bm);
     if (E == 0) {
         SBP_HIP(sbpprim::fill<int32_t>(bm->d_out, B, 0, st));
         SBP_HIP(sbpprim::fill<int32_t>(bm->d_in, B, 0, st));
@@ -453,6 +462,44 @@ void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm, bool keep_hist, bool captur
     launch_cum(ctx, bm, bm->d_out, bm->d_in);  // cumulative counts, and the degrees as row / column totals
     k_dtot_sizes<<<grid_for(B), 256, 0, st>>>(B, bm->row_ptr, bm->row_cells, bm->d_out, bm->d_in, bm->d_tot);
     SBP_HIP(hipGetLastError());
+    if (!capture && nnz > 0) {  // the sorted keys, for the update after moves
+        SBP_HIP(hipMemcpyAsync(bm->row_keys, uniq, sizeof(unsigned long long) * nnz, hipMemcpyDeviceToDevice, st));
+        SBP_HIP(hipMemcpyAsync(bm->col_keys, tkeys_sorted, sizeof(unsigned long long) * nnz, hipMemcpyDeviceToDevice,
+                               st));
+        bm->keys_cbits = cbits;
+        bm->keys_valid = true;
+    }
+}
+
+void bm_count_sizes(sbp_ctx ctx, sbp_bm bm) {
+    hipStream_t st = ctx->stream;
+    const int32_t B = bm->B;
+    SBP_HIP(sbpprim::fill<int32_t>(bm->sizes, B, 0, st));
+    if (B <= SIZES_LDS_MAX) k_sizes_lds<<<std::min(grid_for(bm->N), 512), 256, 0, st>>>(bm->N, B, bm->assign, bm->sizes);
+    else k_sizes<<<grid_for(bm->N), 256, 0, st>>>(bm->N, bm->assign, bm->sizes);
+    SBP_HIP(hipGetLastError());
+}
+
+// The CSR and CSC, cumulative counts and degrees from sorted row-major and column-major keys with their counts
+// (bm->nnz of each; capacity already ensured); the keys are kept for the next update.
+void bm_fill_cells(sbp_ctx ctx, sbp_bm bm, const unsigned long long *row_keys, const int32_t *row_counts,
+                   const unsigned long long *col_keys, const int32_t *col_counts) {
+    hipStream_t st = ctx->stream;
+    const int32_t B = bm->B;
+    const int64_t nnz = bm->nnz;
+    const int cbits = bm->keys_cbits;
+    k_fill_major<<<grid_for(nnz), 256, 0, st>>>(nnz, nullptr, B, cbits, row_keys, row_counts, bm->row_ptr,
+                                                 bm->row_cells);
+    k_fill_major<<<grid_for(nnz), 256, 0, st>>>(nnz, nullptr, B, cbits, col_keys, col_counts, bm->col_ptr,
+                                                 bm->col_cells);
+    if (nnz > 0) {
+        SBP_HIP(hipMemcpyAsync(bm->row_keys, row_keys, sizeof(unsigned long long) * nnz, hipMemcpyDeviceToDevice, st));
+        SBP_HIP(hipMemcpyAsync(bm->col_keys, col_keys, sizeof(unsigned long long) * nnz, hipMemcpyDeviceToDevice, st));
+    }
+    launch_cum(ctx, bm, bm->d_out, bm->d_in);
+    k_dtot_sizes<<<grid_for(B), 256, 0, st>>>(B, bm->row_ptr, bm->row_cells, bm->d_out, bm->d_in, bm->d_tot);
+    SBP_HIP(hipGetLastError());
+    bm->keys_valid = true;
 }
 
 void bm_copy_into(sbp_ctx ctx, sbp_bm src, sbp_bm dst) {
@@ -477,6 +524,7 @@ void bm_copy_into(sbp_ctx ctx, sbp_bm src, sbp_bm dst) {
     dst->overall_entropy = src->overall_entropy;
     dst->num_to_merge = 0;
     dst->hash_valid = false;
+    dst->keys_valid = false;
     dst->np = src->np;
     dst->dc = src->dc;
     dst->hist_ready = false;

@@ -90,9 +90,11 @@ static void gibbs_batch(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int greedy, uint32_
     FtLaunchInfo info = ft_propose(ctx, bm, g, epoch, greedy, d_items, count, move_s, nullptr, nullptr);
     int64_t refused = 0;
     double t1 = now_ms();
+    const int64_t updates = ctx->cell_updates;
     int64_t moved = ft_commit(ctx, bm, g, move_s, &refused);
     double t2 = now_ms();
     if (st) {
+        st->cell_updates += ctx->cell_updates - updates;
         st->proposals += count;
         st->moves += moved;
         st->refused += refused;

@@ -712,8 +712,14 @@ __global__ void __launch_bounds__(FT_BLOCK) __attribute__((amdgpu_waves_per_eu(O
         bytes += proposal_bytes(a, m, *t.n_used, GREEDY);
         group_table_clear_used(t, grp);
     }
+    // one count per workgroup
+    __shared__ unsigned long long block_bytes;
+    if (threadIdx.x == 0) block_bytes = 0;
+    __syncthreads();
     bytes = wave_sum(grp.rank() == 0 ? bytes : 0ull);
-    if (lane_id() == 0 && bytes) atomicAdd(a.bytes, bytes);
+    if (lane_id() == 0 && bytes) atomicAdd(&block_bytes, bytes);
+    __syncthreads();
+    if (threadIdx.x == 0 && block_bytes) atomicAdd(a.bytes, block_bytes);
 }
 
 // Vertices of total degree > SMALL_K, one workgroup each: THREADS threads split the vertex's edges and its distinct
@@ -1427,8 +1433,13 @@ __global__ void k_hist_moves(int32_t N, const int32_t *__restrict__ move_s, cons
     }
 }
 
-__global__ void k_apply(int32_t N, const int32_t *__restrict__ move_s, const int32_t *__restrict__ acc,
-                        int32_t *__restrict__ assign, unsigned long long *__restrict__ moved) {
+// One count per workgroup: same-address atomics from every wavefront serialise at the L2.
+__global__ void __launch_bounds__(256) k_apply(int32_t N, const int32_t *__restrict__ move_s,
+                                               const int32_t *__restrict__ acc, int32_t *__restrict__ assign,
+                                               unsigned long long *__restrict__ moved) {
+    __shared__ unsigned long long block_total;
+    if (threadIdx.x == 0) block_total = 0;
+    __syncthreads();
     unsigned long long local = 0;
     for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
         int s = move_s[v];
@@ -1438,7 +1449,9 @@ __global__ void k_apply(int32_t N, const int32_t *__restrict__ move_s, const int
         }
     }
     local = wave_sum(local);
-    if (lane_id() == 0 && local) atomicAdd(moved, local);
+    if (lane_id() == 0 && local) atomicAdd(&block_total, local);
+    __syncthreads();
+    if (threadIdx.x == 0 && block_total) atomicAdd(moved, block_total);
 }
 
 static int grid_of(int64_t n) {
@@ -1504,7 +1517,13 @@ int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_
     if (bm->dc)
         k_hist_moves<<<grid_of(N), 256, 0, st>>>(N, d_move_s, acc_final, bm->assign, g->out_off, g->in_off, bm->hist,
                                                  64 - bm->hist_log2, (uint32_t)((1ull << bm->hist_log2) - 1));
-    k_apply<<<grid_of(N), 256, 0, st>>>(N, d_move_s, acc_final, bm->assign, counters);
+    // eager: the assignment before the moves, for the update of the cells in place of a rebuild
+    int32_t *prev = nullptr;
+    if (!capture && bm->keys_valid && !bm->dc && N > 0) {
+        prev = ctx->scratch[S_PREV].as<int32_t>((size_t)N);
+        SBP_HIP(hipMemcpyAsync(prev, bm->assign, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToDevice, st));
+    }
+    k_apply<<<std::min(grid_of(N), 1024), 256, 0, st>>>(N, d_move_s, acc_final, bm->assign, counters);
     SBP_HIP(hipGetLastError());
     if (capture) {
         SBP_CHECK(!bm->dc, SBP_E_STATE, "captured commit: degree-corrected histograms need host bookkeeping");
@@ -1520,7 +1539,7 @@ int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_
         read_small(ctx, {{&c[0], counters, sizeof(unsigned long long)}, {&c[1], counters + 2, sizeof(unsigned long long)}});
     if (bm->dc) bm_hist_after_moves(ctx, bm, book.val, book.pad);
     if (refused_out) *refused_out = (int64_t)c[1] - (int64_t)c[0];
-    bm_rebuild(ctx, g, bm, true);
+    if (!(prev && bm_update_moves(ctx, g, bm, prev, (int64_t)c[0]))) bm_rebuild(ctx, g, bm, true);
     return (int64_t)c[0];
 }
 
@@ -2202,6 +2221,7 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         k_compact_copy<<<grid_of((int64_t)B * 64), 64, 0, st>>>(B, h.cstart, h.clen, cc, cp, bm->col_cells);
         bm->nnz = nnz[0];
         bm->hash_valid = false;
+        bm->keys_valid = false;
         bm_compute_cum(ctx, bm);
         SBP_HIP(hipStreamSynchronize(st));
         if (hs[1] == MH_DONE) break;

@@ -174,7 +174,9 @@ struct DevicePool {
 enum ScratchSlot {
     S_KEYS_A, S_KEYS_B, S_VALS_A, S_VALS_B, S_UNIQ, S_COUNTS, S_SORT_TMP, S_MISC, S_PARTIAL,
     S_MOVES, S_DS, S_ACCEPT, S_COUNTER, S_HASH, S_RISK, S_EVENTS_A, S_EVENTS_B, S_ORDER, S_BEST,
-    S_ASSIGN_TMP, S_FLAGS, S_NP, S_BUCKETS, S_NUM_SLOTS
+    S_ASSIGN_TMP, S_FLAGS, S_NP, S_BUCKETS, S_PREV, S_UPD_CNT, S_UPD_MOVED, S_UPD_KEYS, S_UPD_SORTED, S_UPD_UNIQ,
+    S_UPD_CNTS, S_UPD_NET, S_UPD_RKEYS, S_UPD_CKEYS, S_UPD_RCNT, S_UPD_CCNT, S_UPD_POS, S_UPD_AT, S_UPD_REM,
+    S_NUM_SLOTS
 };
 
 }  // namespace sbpimpl
@@ -201,6 +203,7 @@ struct sbp_ctx_s {
     uint64_t peak_bytes = 0;  // largest device footprint seen at the end of a partition (sbp_ctx_memory)
     std::vector<sbp_ctx> slots;  // worker contexts of sbp_partition_batch, created on first use
     std::shared_ptr<sbpimpl::RecordedSweep> recorded;  // the asynchronous-Gibbs sweep recorded as a graph (capi.hip)
+    int64_t cell_updates = 0;  // commits whose cells were updated from the moves (bm_update_moves)
     int32_t record_mode = -1;  // sbp_set_recorded_sweeps: -1 by size (or SBP_AMD_RECORDED_SWEEPS), 0 off, 1 on
     // launch shape of the finetune degree classes 0..4 (k <= 8, 32, 128, 256, 1,024): lanes (threads) per vertex,
     // and wavefronts per SIMD of the greedy parametric lane-group launches (sbp_set_finetune_shape)
@@ -307,12 +310,24 @@ struct sbp_bm_s {
     sbpdev::CellSlot *cell_hash = nullptr;
     int hash_log2 = 0;
     bool hash_valid = false;
+    int64_t hash_used = 0;  // claimed slots of the hash
+    // sorted cell keys (row << keys_cbits | col and col << keys_cbits | row) of the current cells, kept by eager
+    // rebuilds for the update after moves
+    unsigned long long *row_keys = nullptr, *col_keys = nullptr;  // [cap]
+    int keys_cbits = 0;
+    bool keys_valid = false;
 };
 
 namespace sbpimpl {
 
 // build.hip
 void bm_alloc(sbp_bm bm, int32_t B, int32_t N);
+int bits_for_blocks(int64_t B);
+void bm_fill_cells(sbp_ctx ctx, sbp_bm bm, const unsigned long long *row_keys, const int32_t *row_counts,
+                   const unsigned long long *col_keys, const int32_t *col_counts);
+void bm_count_sizes(sbp_ctx ctx, sbp_bm bm);
+// update.hip: the cells after committed moves (prev: the assignment before them); false when the rebuild must run
+bool bm_update_moves(sbp_ctx ctx, sbp_graph g, sbp_bm bm, const int32_t *prev, int64_t moved_hint);
 void bm_release(sbp_bm bm);
 // From bm->assign (device), bm->B set.  The degree histograms of a degree-corrected blockmodel are rebuilt too,
 // unless keep_hist (a finetune commit, which updates them move by move as the reference does).

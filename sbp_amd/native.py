@@ -62,7 +62,7 @@ class Stats(C.Structure):
                 ("ft_high_bytes", C.c_uint64), ("ft_high_launches", C.c_int64), ("ft_high_kernel_ms", C.c_double),
                 ("mh_bytes", C.c_uint64), ("mh_launches", C.c_int64), ("mh_kernel_ms", C.c_double),
                 ("ft_class_bytes", C.c_uint64 * 6), ("ft_class_launches", C.c_int64 * 6), ("ft_class_ms", C.c_double * 6),
-                ("recorded_sweeps", C.c_int64)]
+                ("recorded_sweeps", C.c_int64), ("cell_updates", C.c_int64)]
 
     def as_dict(self):
         return {k: (list(getattr(self, k)) if k.startswith("ft_class_") else getattr(self, k)) for k, _ in self._fields_}
