@@ -227,9 +227,11 @@ __global__ void k_hash_cells(int32_t B, const int32_t *__restrict__ ptr, const i
     }
 }
 
+// At least three slots per cell: the update after moves inserts new cells in place (and keeps emptied ones) until
+// the load passes 0.7, so the room above the initial load of at most 1/3 sets how many sweeps run between rebuilds.
 static int hash_log2_for(int64_t nnz) {
     int lg = 10;
-    while ((int64_t(1) << lg) < 2 * nnz + 2) ++lg;
+    while ((int64_t(1) << lg) < 3 * nnz + 2) ++lg;
     return lg;
 }
 

@@ -60,7 +60,7 @@ def test_footprint_bound_is_monotone_and_covers_the_graph():
     # the tables are capped at 1 GiB: the largest served request stays a small part of a 200 GB budget (its
     # blockmodels with their sorted cell keys and the cell update's work areas included)
     big = int(L.sbp_footprint_bytes(2_000_000, 32_000_000, 2000, 100_000, 0))
-    assert big < 25e9
+    assert big < 35e9
     assert big - int(L.sbp_footprint_bytes(2_000_000, 32_000_000, 2000, 1, 0)) <= (1 << 30) * 5 // 4 + (1 << 20)
 
 
