@@ -411,9 +411,10 @@ def split_leg(ctx, dg, g, w, calls=3):
     return out
 
 
-def c2_leg(runs):
+def c2_leg(runs, cpu_full=False):
     """Full agglomerative partitions of the 200,000-vertex graph (asynchronous Gibbs), compared with the oracle's
-    partition committed in tests/golden/partition_200k.npz."""
+    partition committed in tests/golden/partition_200k.npz.  With cpu_full, the oracle's full partition of the
+    same graph is timed once on min(nproc, 16) OpenMP threads beside it (a few minutes)."""
     from sbp_amd import native
     g, w = make_graph("c2")
     ctx = native.Context(0, seed=0)
@@ -434,6 +435,16 @@ def c2_leg(runs):
                 st.mdl == float(want["mdl"]))
     print(f"[bench] c2: {runs} runs in {wall:.2f}s, B={nb}, identical to the oracle: {same}", file=sys.stderr,
           flush=True)
+    cpu = None
+    if cpu_full:
+        import oracle as O
+        threads = min(os.cpu_count() or 1, 16)
+        t1 = time.perf_counter()
+        ca, cnb, cst = O.sbp(O.OGraph.from_csr(g), O.params("async_gibbs", seed=0, threads=threads))
+        cdt = time.perf_counter() - t1
+        cpu = {"seconds": cdt, "cores": threads, "kind": "port", "num_blocks": int(cnb), "mdl": cst.mdl,
+               "same_partition": bool(cnb == nb and np.array_equal(ca, a))}
+        print(f"[bench] c2 cpu: oracle full partition {cdt:.1f}s on {threads} threads", file=sys.stderr, flush=True)
     return {
         "workload": w["desc"] + "; full SBP: async_gibbs finetune + block_merge, golden-section search",
         "runs": runs, "seconds_per_run": wall / runs, "graphs_per_s": runs / wall,
@@ -443,6 +454,7 @@ def c2_leg(runs):
         "phase_ms_per_run": {k: tot[k] / runs for k in ("finetune_ms", "merge_ms", "carry_ms", "carry_load_ms",
                                                         "build_ms", "mdl_ms", "ft_kernel_ms", "ft_high_kernel_ms")},
         "roofline": roofline(tot, ctx_shape, greedy=True, traffic_file=C2_TRAFFIC_FILE),
+        "cpu_full_run": cpu,
     }
 
 
@@ -509,6 +521,7 @@ def main():
     ap.add_argument("--only", choices=["headline", *LEGS], help="run only that part and print its JSON object")
     ap.add_argument("--sampler", default="degree_weighted", help="sampler of the sampled leg")
     ap.add_argument("--c2-runs", type=int, default=2)
+    ap.add_argument("--c2-cpu", action="store_true", help="also time the oracle's full partition of the c2 graph")
     ap.add_argument("--c1-runs", type=int, default=3)
     ap.add_argument("--batched-graphs", type=int, default=4096, help="graphs in the batched leg")
     ap.add_argument("--batched-plain-max", type=int, default=128,
@@ -600,7 +613,7 @@ def main():
         ctx.close()
         del g
     if "c2" in legs:
-        out["c2"] = c2_leg(args.c2_runs)
+        out["c2"] = c2_leg(args.c2_runs, args.c2_cpu)
     if "c1" in legs:
         out["c1"] = c1_leg(args.c1_runs)
     if "batched" in legs:
