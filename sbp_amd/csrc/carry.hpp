@@ -975,7 +975,21 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
         init.reserve(B);
         for (int32_t b = 0; b < B; ++b)
             if (best_s[b] >= 0) init.push_back(Item{best_dS[b], b, best_s[b]});
-        std::priority_queue<Item, std::vector<Item>, decltype(worse)> heap(worse, std::move(init));
+        // The queue: the proposals sorted once, merged on the fly with a heap of the re-queued items (the front
+        // is the smaller of the two).  Pops come in the order of one heap over all items, since (dS, r) orders
+        // them totally (each block has one item); the sorted part is read in sequence instead of sifted.
+        std::sort(init.begin(), init.end(), before);
+        size_t q_head = 0;
+        std::priority_queue<Item, std::vector<Item>, decltype(worse)> requeued(worse);
+        auto q_from_requeued = [&] {
+            return q_head == init.size() || (!requeued.empty() && before(requeued.top(), init[q_head]));
+        };
+        auto q_empty = [&] { return q_head == init.size() && requeued.empty(); };
+        auto q_top = [&]() -> const Item & { return q_from_requeued() ? requeued.top() : init[q_head]; };
+        auto q_pop = [&] {
+            if (q_from_requeued()) requeued.pop();
+            else ++q_head;
+        };
         // evaluation cache per source block: target root, merge count when evaluated, value
         std::vector<int32_t> c_to(B, -1);
         std::vector<int64_t> c_time(B, -1), stamp(B, 0), mark(B + 1, 0);  // mark[B]: spare entry
@@ -1033,7 +1047,7 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
         int64_t K = 256, batch_spec = 0, batch_lost = 0, spec_time = -1;
         while (merged < num_to_merge) {
             if (bi == batch.size()) {
-                if (heap.empty()) break;
+                if (q_empty()) break;
                 if (batch_spec >= 32) {  // adapt the batch size to the invalidated fraction of the last batch
                     if (4 * batch_lost > batch_spec) K = std::max<int64_t>(32, K / 2);
                     else if (10 * batch_lost < batch_spec) K = std::min<int64_t>(16384, K * 2);
@@ -1042,9 +1056,9 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
                 bi = 0;
                 batch_spec = batch_lost = 0;
                 const int64_t want = std::min<int64_t>(K, 2 * (num_to_merge - merged) + 32);
-                while ((int64_t)batch.size() < want && !heap.empty()) {
-                    batch.push_back(heap.top());
-                    heap.pop();
+                while ((int64_t)batch.size() < want && !q_empty()) {
+                    batch.push_back(q_top());
+                    q_pop();
                 }
                 if (pool && batch.size() >= 16) {
                     if (applier) applier->drain();
@@ -1063,9 +1077,9 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
                 }
             }
             Item it;
-            if (!heap.empty() && before(heap.top(), batch[bi])) {
-                it = heap.top();
-                heap.pop();
+            if (!q_empty() && before(q_top(), batch[bi])) {
+                it = q_top();
+                q_pop();
             } else {
                 it = batch[bi++];
                 if (m && !applier) {  // the rows and columns of the next items are read when they are merged
@@ -1099,12 +1113,12 @@ inline int32_t carry_out(Model *m, int32_t B, const int32_t *best_s, const doubl
                 st.evaluations++;
             }
             const bool has_b = bi < batch.size();
-            if (has_b || !heap.empty()) {
-                const double next = (has_b && (heap.empty() || before(batch[bi], heap.top()))) ? batch[bi].dS
-                                                                                                : heap.top().dS;
+            if (has_b || !q_empty()) {
+                const double next = (has_b && (q_empty() || before(batch[bi], q_top()))) ? batch[bi].dS
+                                                                                          : q_top().dS;
                 if (actual > next) {
                     it.dS = actual;
-                    heap.push(it);
+                    requeued.push(it);
                     st.requeues++;
                     continue;
                 }
