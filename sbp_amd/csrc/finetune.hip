@@ -1519,7 +1519,7 @@ int64_t ft_commit(sbp_ctx ctx, sbp_bm bm, sbp_graph g, int32_t *d_move_s, int64_
                                                  64 - bm->hist_log2, (uint32_t)((1ull << bm->hist_log2) - 1));
     // eager: the assignment before the moves, for the update of the cells in place of a rebuild
     int32_t *prev = nullptr;
-    if (!capture && bm->keys_valid && !bm->dc && N > 0) {
+    if (!capture && bm->keys_valid && !bm->dc && N > 0 && bm_update_wanted(g)) {
         prev = ctx->scratch[S_PREV].as<int32_t>((size_t)N);
         SBP_HIP(hipMemcpyAsync(prev, bm->assign, sizeof(int32_t) * (size_t)N, hipMemcpyDeviceToDevice, st));
     }

@@ -328,6 +328,7 @@ void bm_fill_cells(sbp_ctx ctx, sbp_bm bm, const unsigned long long *row_keys, c
 void bm_count_sizes(sbp_ctx ctx, sbp_bm bm);
 // update.hip: the cells after committed moves (prev: the assignment before them); false when the rebuild must run
 bool bm_update_moves(sbp_ctx ctx, sbp_graph g, sbp_bm bm, const int32_t *prev, int64_t moved_hint);
+bool bm_update_wanted(sbp_graph g);  // graph large enough for the update (or SBP_AMD_INCREMENTAL forcing it)
 void bm_release(sbp_bm bm);
 // From bm->assign (device), bm->B set.  The degree histograms of a degree-corrected blockmodel are rebuilt too,
 // unless keep_hist (a finetune commit, which updates them move by move as the reference does).

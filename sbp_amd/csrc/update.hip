@@ -12,8 +12,8 @@
 // updated in place (a cell that reaches zero keeps its key with count 0, which reads as absent).
 //
 // The update keeps, per blockmodel, the sorted cell keys (row << cbits | col and col << cbits | row) that the
-// rebuild produces anyway.  It is used by eager sweeps only; when the moved vertices carry more than a quarter of
-// the edges the rebuild is cheaper and runs instead.
+// rebuild produces anyway.  It is used by eager sweeps of graphs of at least 2^22 edges; when the moved vertices
+// carry more than a quarter of the edges, or the graph is smaller, the rebuild is cheaper and runs instead.
 #include <stdlib.h>
 
 #include <algorithm>
@@ -296,8 +296,18 @@ void merge_major(sbp_ctx ctx, sbp_bm bm, int64_t n_max, const int64_t *n_runs, c
 
 }  // namespace
 
+// Graphs below this many edges rebuild after every commit: their rebuild is a few short launches and one read,
+// the update two dozen launches and five reads (measured: the batched workload, graphs of up to 50,000 vertices,
+// 41.3 graphs/s with rebuilds against 36.0 with updates).
+constexpr int64_t UPDATE_MIN_EDGES = int64_t(1) << 22;
+
+bool bm_update_wanted(sbp_graph g) {
+    if (!incremental_allowed()) return false;
+    return incremental_forced() || g->E >= UPDATE_MIN_EDGES;
+}
+
 bool bm_update_moves(sbp_ctx ctx, sbp_graph g, sbp_bm bm, const int32_t *prev, int64_t moved_hint) {
-    if (!bm->keys_valid || bm->dc || !incremental_allowed()) return false;
+    if (!bm->keys_valid || bm->dc || !bm_update_wanted(g)) return false;
     hipStream_t st = ctx->stream;
     const int32_t N = bm->N, B = bm->B;
     const int cbits = bm->keys_cbits;
