@@ -147,10 +147,12 @@ __global__ void k_delta_transpose(const int64_t *__restrict__ n_runs, const unsi
     }
 }
 
-// Inserts key or sets its count in the cell hash.
+// Inserts key or sets its count in the cell hash.  The host keeps the table's load under 0.9 with every insertion
+// counted; the probe is bounded by the table size all the same and reports a full table in *err.
 __device__ __forceinline__ void hash_upsert(CellSlot *table, int shift, unsigned mask, unsigned long long key, int val,
-                                            unsigned long long *inserted) {
-    for (unsigned h = cell_slot(key, shift);; h = (h + 1) & mask) {
+                                            unsigned long long *inserted, int32_t *err) {
+    unsigned h = cell_slot(key, shift);
+    for (unsigned n = 0; n <= mask; ++n, h = (h + 1) & mask) {
         unsigned long long k = table[h].key;
         if (k == CELL_EMPTY) {
             k = atomicCAS(&table[h].key, CELL_EMPTY, key);
@@ -165,6 +167,7 @@ __device__ __forceinline__ void hash_upsert(CellSlot *table, int shift, unsigned
             return;
         }
     }
+    atomicOr(err, 2);
 }
 
 // For each delta: its cell's rank in the old sorted list.  A cell already present takes the delta into its count
@@ -196,16 +199,16 @@ __global__ void k_locate(const int64_t *__restrict__ n_runs, const unsigned long
             c = cells[lo].y + dnet[i];
             cells[lo].y = c;
             ins[i] = 0;
-            if (c < 0) atomicExch(err, 1);
+            if (c < 0) atomicOr(err, 1);
         } else {
             c = dnet[i];
             ins[i] = 1;
             atomicAdd(&at[lo], 1);
-            if (c <= 0) atomicExch(err, 1);
+            if (c <= 0) atomicOr(err, 1);
         }
         if (hash) {
             const unsigned long long mask = (1ull << cbits) - 1;
-            hash_upsert(hash, shift, hmask, cell_key((int)(k >> cbits), (int)(k & mask)), c, inserted);
+            hash_upsert(hash, shift, hmask, cell_key((int)(k >> cbits), (int)(k & mask)), c, inserted, err);
         }
     }
 }
@@ -370,9 +373,9 @@ bool bm_update_moves(sbp_ctx ctx, sbp_graph g, sbp_bm bm, const int32_t *prev, i
     int32_t *tnet = dnet + nk + 1;
     k_delta_net<<<upd_grid(nk), UPD_THREADS, 0, st>>>(n_runs, uniq, cnt, none, dkey, dnet);
     k_delta_transpose<<<upd_grid(nk), UPD_THREADS, 0, st>>>(n_runs, dkey, cbits, none, uniq);
+    int64_t runs = 0;
     {
         // sort the transposed keys with their nets; `none` entries sort last
-        int64_t runs = 0;
         read_small(ctx, {{&runs, n_runs, sizeof(runs)}});
         size_t a = 0;
         SBP_HIP(sbpprim::radix_sort(nullptr, a, uniq, tkey, (const int32_t *)dnet, tnet, runs, 0, end_bit, st));
@@ -386,7 +389,8 @@ bool bm_update_moves(sbp_ctx ctx, sbp_graph g, sbp_bm bm, const int32_t *prev, i
     int32_t *rc = ctx->scratch[S_UPD_RCNT].as<int32_t>((size_t)cap_new);
     int32_t *cc = ctx->scratch[S_UPD_CCNT].as<int32_t>((size_t)cap_new);
     int64_t *new_nnz = reinterpret_cast<int64_t *>(counters + 4);
-    const bool with_hash = bm->hash_valid && bm->cell_hash;
+    // the hash takes the update when even every cell being new keeps its load under 0.9
+    const bool with_hash = bm->hash_valid && bm->cell_hash && (bm->hash_used + runs) * 10 < (int64_t(9) << bm->hash_log2);
     merge_major(ctx, bm, nk, n_runs, dkey, dnet, none, bm->row_keys, bm->row_cells, with_hash, counters + 5, err, rk,
                 rc, new_nnz, S_UPD_POS, S_UPD_AT, S_UPD_REM);
     merge_major(ctx, bm, nk, n_runs, tkey, tnet, none, bm->col_keys, bm->col_cells, false, counters + 5, err, ck, cc,
@@ -396,7 +400,8 @@ bool bm_update_moves(sbp_ctx ctx, sbp_graph g, sbp_bm bm, const int32_t *prev, i
     unsigned long long hash_ins = 0;
     read_small(ctx, {{nn, new_nnz, 2 * sizeof(int64_t)}, {&bad, err, sizeof(int32_t)},
                      {&hash_ins, counters + 5, sizeof(hash_ins)}});
-    SBP_CHECK(!bad && nn[0] == nn[1], SBP_E_STATE, "cell update inconsistent with the block matrix");
+    SBP_CHECK(!(bad & 1) && nn[0] == nn[1], SBP_E_STATE, "cell update inconsistent with the block matrix");
+    SBP_CHECK(!(bad & 2), SBP_E_STATE, "cell hash full during the update");
     const int64_t nnz = nn[0];
     bm_ensure_cells(bm, nnz > 0 ? nnz : 1);
     bm->nnz = nnz;
