@@ -912,7 +912,7 @@ static void launch_lanes_t(int blocks, hipStream_t st, const FtArgs &a, int buck
 // Lanes per vertex that a degree class may use: a power of two from 8 to 64 whose tables fit the LDS budget; and
 // the wavefronts per SIMD of the greedy parametric launch (3 or 4).
 static bool shape_allowed(int bucket, int L, int occ) {
-    return (L == 8 || L == 16 || L == 32 || L == 64) && bucket >= 0 && bucket < FT_NB - 2 && ft_lanes_fit(bucket, L) &&
+    return (L == 4 || L == 8 || L == 16 || L == 32 || L == 64) && bucket >= 0 && bucket < FT_NB - 2 && ft_lanes_fit(bucket, L) &&
            (occ == 3 || occ == 4);
 }
 
@@ -923,6 +923,7 @@ static void launch_lanes(int64_t upper, hipStream_t st, const FtArgs &a, int buc
     int64_t blocks = (upper + per_block - 1) / per_block;
     blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, 4096));
     switch (L) {
+        case 4: launch_lanes_t<4>((int)blocks, st, a, bucket, greedy, np, occ); break;
         case 8: launch_lanes_t<8>((int)blocks, st, a, bucket, greedy, np, occ); break;
         case 16: launch_lanes_t<16>((int)blocks, st, a, bucket, greedy, np, occ); break;
         case 32: launch_lanes_t<32>((int)blocks, st, a, bucket, greedy, np, occ); break;

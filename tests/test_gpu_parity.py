@@ -117,7 +117,8 @@ def test_propose_all_bit_exact(ctx, greedy, B):
     assert mismatches == 0
 
 
-SHAPES = [((8, 8, 32, 64, 128), (3, 3, 3, 3)), ((16, 16, 32, 64, 256), (4, 4, 4, 4)),
+SHAPES = [((4, 8, 32, 64, 256), (4, 4, 4, 4)), ((4, 16, 32, 64, 128), (3, 3, 3, 3)),
+          ((8, 8, 32, 64, 128), (3, 3, 3, 3)), ((16, 16, 32, 64, 256), (4, 4, 4, 4)),
           ((32, 32, 64, 64, 128), (3, 4, 3, 4)), ((64, 64, 64, 64, 256), (4, 3, 4, 3))]
 
 
@@ -155,7 +156,7 @@ def test_propose_all_every_launch_shape(greedy, shape):
 
 def test_launch_shape_rejects_unfit_choices(ctx):
     before = ctx.finetune_shape()
-    for lanes, occ in [((4, 16, 32, 64, 256), (3, 3, 3, 3)), ((8, 4, 32, 64, 256), (3, 3, 3, 3)),
+    for lanes, occ in [((2, 16, 32, 64, 256), (3, 3, 3, 3)), ((8, 4, 32, 64, 256), (3, 3, 3, 3)),
                        ((8, 16, 16, 64, 256), (3, 3, 3, 3)), ((8, 16, 32, 32, 256), (3, 3, 3, 3)),
                        ((8, 16, 32, 64, 256), (2, 3, 3, 3)), ((8, 12, 32, 64, 256), (3, 3, 3, 3)),
                        ((8, 16, 32, 64, 512), (3, 3, 3, 3)), ((8, 16, 32, 64, 64), (3, 3, 3, 3))]:
