@@ -58,8 +58,8 @@ def tuned_shape():
 
 # candidates per degree class: (lanes, waves per SIMD); class 4 varies its threads per vertex only
 SHAPE_CANDIDATES = [
+    [(2, 4), (2, 3), (4, 4), (4, 3), (8, 4), (8, 3), (16, 4), (32, 4)],
     [(4, 4), (4, 3), (8, 4), (8, 3), (16, 4), (16, 3), (32, 4), (64, 4)],
-    [(8, 4), (8, 3), (16, 4), (16, 3), (32, 4), (64, 4)],
     [(32, 4), (32, 3), (64, 4), (64, 3)],
     [(64, 4), (64, 3)],
     [(256, 4), (128, 4)],

@@ -50,9 +50,12 @@ __device__ __forceinline__ unsigned hash_slot(const Table &t, int key) {
     return ((unsigned)key * 0x9E3779B1u) >> t.shift;
 }
 
+// Probes are bounded by the table size: a table may be filled to its last slot (class 1 on 4 lanes: at most 32
+// distinct blocks in 32 slots), where a lookup of an absent key meets no empty slot.  An insertion always finds
+// room, since a table is sized for the most distinct keys an item can have.
 __device__ __forceinline__ void table_add(const Table &t, int key, int dout, int din) {
     unsigned slot = hash_slot(t, key);
-    while (true) {
+    for (int n = 0; n < t.cap; ++n) {
         int prev = atomicCAS(&t.keys[slot], EMPTY, key);
         if (prev == EMPTY) t.used[atomicAdd(t.n_used, 1)] = (int)slot;
         if (prev == EMPTY || prev == key) {
@@ -66,20 +69,18 @@ __device__ __forceinline__ void table_add(const Table &t, int key, int dout, int
 
 __device__ __forceinline__ void table_get(const Table &t, int key, int &o, int &i) {
     unsigned slot = hash_slot(t, key);
-    while (true) {
+    for (int n = 0; n < t.cap; ++n) {
         int k = t.keys[slot];
         if (k == key) {
             o = t.outc[slot];
             i = t.inc[slot];
             return;
         }
-        if (k == EMPTY) {
-            o = 0;
-            i = 0;
-            return;
-        }
+        if (k == EMPTY) break;
         slot = (slot + 1) & t.mask;
     }
+    o = 0;  // absent (an empty slot, or a full table without the key)
+    i = 0;
 }
 
 __device__ __forceinline__ int table_size(const Table &t) { return *t.n_used; }

@@ -56,19 +56,23 @@ static_assert(2 * SMALL_K <= LDS_CAP, "LDS table load factor above 1/2");
 // Degree classes of the proposal launches, by k = |out(v)| + |in(v)|: classes 0..FT_NB-3 hold k up to
 // ft_bucket_max(b) (and above the previous bound) and run on lane groups; classes FT_NB-2 (k <= 1,024) and FT_NB-1
 // (the rest) run one workgroup per vertex.  Each lane-group table has 2 * ft_bucket_max(b)
-// slots (load factor <= 1/2) and lists at most ft_bucket_max(b) used slots.
+// slots (load factor <= 1/2; class 1 on 4 lanes: ft_table_log2) and lists at most ft_bucket_max(b) used slots.
 __host__ __device__ constexpr int ft_bucket_max(int b) {
     return b == 0 ? 8 : b == 1 ? 32 : b == 2 ? 128 : b == 3 ? SMALL_K : 1024;
 }
 __host__ __device__ constexpr int ft_bucket_log2(int b) { return b == 0 ? 4 : b == 1 ? 6 : b == 2 ? 8 : 9; }
+// Table slots (log2) of a lane group of L lanes in class b: class 1 on 4 lanes takes 32 slots (its at most 32
+// distinct blocks fill it at worst; 64 slots for 16 groups would not fit), every other choice 2 * ft_bucket_max.
+__host__ __device__ constexpr int ft_table_log2(int b, int L) { return (b == 1 && L <= 4) ? 5 : ft_bucket_log2(b); }
 static_assert(FT_NB == 6 && ft_bucket_max(FT_NB - 3) == FT_SMALL_DEGREE, "degree classes");
 // LDS ints per wavefront for the lane-group tables: (WAVE / L) tables of class b must fit, which allows
-// L >= 4 for class 0, L >= 8 for class 1, L >= 32 for class 2 and L = 64 for class 3.
-constexpr int LDS_WAVE_INTS = 1808;
+// L >= 2 for class 0, L >= 4 for class 1, L >= 32 for class 2 and L = 64 for class 3.  Four 4-wavefront
+// workgroups per CU (the 4 wavefronts per SIMD the register budget allows) take just under 160 KB.
+constexpr int LDS_WAVE_INTS = 2496;
 __host__ __device__ constexpr bool ft_lanes_fit(int b, int L) {
-    return (WAVE / L) * table_ints(ft_bucket_log2(b), ft_bucket_max(b)) <= LDS_WAVE_INTS;
+    return (WAVE / L) * table_ints(ft_table_log2(b, L), ft_bucket_max(b)) <= LDS_WAVE_INTS;
 }
-static_assert(ft_lanes_fit(0, 4) && ft_lanes_fit(1, 8) && ft_lanes_fit(2, 32) && ft_lanes_fit(3, 64), "LDS budget");
+static_assert(ft_lanes_fit(0, 2) && ft_lanes_fit(1, 4) && ft_lanes_fit(2, 32) && ft_lanes_fit(3, 64), "LDS budget");
 // Read-only view of the block matrix.  Compact CSR: row b = [ptr[b], ptr[b+1]), with per-row inclusive
 // cumulative counts (columns: excluding the diagonal cell) for multinomial selection by binary search.
 // Slack layout (Metropolis-Hastings): row b = [start[b], start[b] + len[b]), no cumulative counts.
@@ -682,7 +686,7 @@ __global__ void __launch_bounds__(FT_BLOCK) __attribute__((amdgpu_waves_per_eu(O
     constexpr int GPW = WAVE / L;  // lane groups per wavefront
     const LaneGroup<L> grp;
     const int wid = threadIdx.x / WAVE, gi = lane_id() / L;
-    const int dmax = ft_bucket_max(bucket), lg = ft_bucket_log2(bucket);
+    const int dmax = ft_bucket_max(bucket), lg = ft_table_log2(bucket, L);
     const Table t = table_at(lds + wid * LDS_WAVE_INTS + gi * (int)table_ints(lg, dmax), lg, dmax);
     group_table_clear_all(t, grp);
     const int64_t count = a.bmeta[bucket];
@@ -912,7 +916,7 @@ static void launch_lanes_t(int blocks, hipStream_t st, const FtArgs &a, int buck
 // Lanes per vertex that a degree class may use: a power of two from 8 to 64 whose tables fit the LDS budget; and
 // the wavefronts per SIMD of the greedy parametric launch (3 or 4).
 static bool shape_allowed(int bucket, int L, int occ) {
-    return (L == 4 || L == 8 || L == 16 || L == 32 || L == 64) && bucket >= 0 && bucket < FT_NB - 2 && ft_lanes_fit(bucket, L) &&
+    return (L == 2 || L == 4 || L == 8 || L == 16 || L == 32 || L == 64) && bucket >= 0 && bucket < FT_NB - 2 && ft_lanes_fit(bucket, L) &&
            (occ == 3 || occ == 4);
 }
 
@@ -923,6 +927,7 @@ static void launch_lanes(int64_t upper, hipStream_t st, const FtArgs &a, int buc
     int64_t blocks = (upper + per_block - 1) / per_block;
     blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, 4096));
     switch (L) {
+        case 2: launch_lanes_t<2>((int)blocks, st, a, bucket, greedy, np, occ); break;
         case 4: launch_lanes_t<4>((int)blocks, st, a, bucket, greedy, np, occ); break;
         case 8: launch_lanes_t<8>((int)blocks, st, a, bucket, greedy, np, occ); break;
         case 16: launch_lanes_t<16>((int)blocks, st, a, bucket, greedy, np, occ); break;

@@ -118,6 +118,7 @@ def test_propose_all_bit_exact(ctx, greedy, B):
 
 
 SHAPES = [((4, 8, 32, 64, 256), (4, 4, 4, 4)), ((4, 16, 32, 64, 128), (3, 3, 3, 3)),
+          ((2, 4, 32, 64, 256), (4, 4, 4, 4)), ((2, 4, 32, 64, 128), (3, 3, 3, 3)),
           ((8, 8, 32, 64, 128), (3, 3, 3, 3)), ((16, 16, 32, 64, 256), (4, 4, 4, 4)),
           ((32, 32, 64, 64, 128), (3, 4, 3, 4)), ((64, 64, 64, 64, 256), (4, 3, 4, 3))]
 
@@ -154,9 +155,45 @@ def test_propose_all_every_launch_shape(greedy, shape):
     c.close()
 
 
+@pytest.mark.parametrize("greedy", [True, False])
+def test_full_lane_group_tables(greedy):
+    """Class-1 vertices on 4 lanes have 32-slot tables: vertices with exactly 32 edges into 32 distinct blocks other
+    than their own fill every slot, so looking up their own block (absent) must end without an empty slot."""
+    c = native.Context(0, seed=99)
+    c.set_finetune_shape((4, 4, 32, 64, 256), (4, 4, 4, 4))
+    base = small_graph(seed=12, N=1200, B=8, kmax=60, mean=6)
+    B = 48
+    rng = np.random.default_rng(12)
+    assign = rng.integers(0, B, base.N).astype(np.int32)
+    assign[:B] = np.arange(B)
+    src = list(np.repeat(np.arange(base.N), np.diff(base.out_off)))
+    dst = list(base.out_adj)
+    full = []
+    for j in range(12):  # new vertices, each with 32 out-edges only, one into each of 32 blocks other than its own
+        v = base.N + j
+        own = int(rng.integers(0, B))
+        others = rng.choice([b for b in range(B) if b != own], 32, replace=False)
+        for b in others:
+            src.append(v)
+            dst.append(int(b))  # vertex b is in block b (assign[:B] = arange)
+        full.append((v, own))
+    g = CSRGraph.from_edges(np.array(src), np.array(dst), base.N + len(full))
+    assign = np.concatenate([assign, np.array([o for _, o in full], np.int32)])
+    og, ob, dg, db = both(c, g, B, assign)
+    for epoch in range(3):
+        s, dS, acc = db.propose_all(epoch=epoch, greedy=greedy)
+        for v in [v for v, _ in full] + list(range(0, base.N, 37)):
+            os_, odS, oacc = ob.ft_propose(99, epoch, v, greedy)
+            assert s[v] == os_, (epoch, v)
+            if s[v] != assign[v]:
+                assert dS[v] == odS, (epoch, v)
+            assert acc[v] == oacc, (epoch, v)
+    c.close()
+
+
 def test_launch_shape_rejects_unfit_choices(ctx):
     before = ctx.finetune_shape()
-    for lanes, occ in [((2, 16, 32, 64, 256), (3, 3, 3, 3)), ((8, 4, 32, 64, 256), (3, 3, 3, 3)),
+    for lanes, occ in [((1, 16, 32, 64, 256), (3, 3, 3, 3)), ((8, 2, 32, 64, 256), (3, 3, 3, 3)),
                        ((8, 16, 16, 64, 256), (3, 3, 3, 3)), ((8, 16, 32, 32, 256), (3, 3, 3, 3)),
                        ((8, 16, 32, 64, 256), (2, 3, 3, 3)), ((8, 12, 32, 64, 256), (3, 3, 3, 3)),
                        ((8, 16, 32, 64, 512), (3, 3, 3, 3)), ((8, 16, 32, 64, 64), (3, 3, 3, 3))]:
