@@ -298,11 +298,11 @@ int sbp_ctx_trim(sbp_ctx ctx);
 int sbp_set_carry_threads(sbp_ctx ctx, int32_t threads);
 /* Launch shape of the finetune proposal launches, per degree class of k = |out(v)| + |in(v)| (b = 0..4 for
  * k <= 8, 32, 128, 256 and 1,024; the rest take 512-thread workgroups).  lanes[b]: lanes (threads) per vertex,
- * allowed 2, 4, 8, 16, 32 or 64 for class 0, 4, 8, 16, 32 or 64 for class 1, 32 or 64 for class 2, 64 for class 3,
- * 128 or 256 for class 4.
+ * allowed 2, 4, 8, 16, 32 or 64 for class 0, 4, 8, 16, 32 or 64 for class 1, 16, 32 or 64 for class 2, 32 or 64
+ * for class 3, 128 or 256 for class 4.
  * waves_per_simd[b] (b = 0..3): the wavefronts per SIMD that the greedy parametric lane-group launch's register
  * budget is sized for, 3 or 4.  SBP_E_ARG (nothing changed) otherwise.  With ctx NULL, sets the default of the
- * contexts created afterwards (lanes 2, 4, 32, 64, 256 at 4 wavefronts per SIMD).  Results do not depend on it.
+ * contexts created afterwards (lanes 2, 4, 16, 64, 256 at 4 wavefronts per SIMD).  Results do not depend on it.
  * No counterpart in the reference: a tuning knob of the launch shape, which the autotuner
  * (sbp_amd/autotune.py) sets. */
 int sbp_set_finetune_shape(sbp_ctx ctx, const int32_t lanes[5], const int32_t waves_per_simd[4]);

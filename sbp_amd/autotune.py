@@ -60,8 +60,8 @@ def tuned_shape():
 SHAPE_CANDIDATES = [
     [(2, 4), (2, 3), (4, 4), (4, 3), (8, 4), (8, 3), (16, 4), (32, 4)],
     [(4, 4), (4, 3), (8, 4), (8, 3), (16, 4), (16, 3), (32, 4), (64, 4)],
+    [(16, 4), (16, 3), (32, 4), (32, 3), (64, 4), (64, 3)],
     [(32, 4), (32, 3), (64, 4), (64, 3)],
-    [(64, 4), (64, 3)],
     [(256, 4), (128, 4)],
 ]
 

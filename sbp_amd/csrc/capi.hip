@@ -598,7 +598,7 @@ extern "C" {
 
 // finetune launch shape that new contexts start with (sbp_set_finetune_shape(NULL, ...))
 static std::mutex g_shape_mu;
-static int32_t g_default_lanes[5] = {2, 4, 32, 64, 256};
+static int32_t g_default_lanes[5] = {2, 4, 16, 64, 256};
 static int32_t g_default_occ[4] = {4, 4, 4, 4};
 
 int sbp_create(sbp_ctx *out, int device, uint64_t seed) {

@@ -56,23 +56,31 @@ static_assert(2 * SMALL_K <= LDS_CAP, "LDS table load factor above 1/2");
 // Degree classes of the proposal launches, by k = |out(v)| + |in(v)|: classes 0..FT_NB-3 hold k up to
 // ft_bucket_max(b) (and above the previous bound) and run on lane groups; classes FT_NB-2 (k <= 1,024) and FT_NB-1
 // (the rest) run one workgroup per vertex.  Each lane-group table has 2 * ft_bucket_max(b)
-// slots (load factor <= 1/2; class 1 on 4 lanes: ft_table_log2) and lists at most ft_bucket_max(b) used slots.
+// slots (load factor <= 1/2; fewer lanes than fit that: ft_table_log2) and lists at most ft_bucket_max(b) used slots.
 __host__ __device__ constexpr int ft_bucket_max(int b) {
     return b == 0 ? 8 : b == 1 ? 32 : b == 2 ? 128 : b == 3 ? SMALL_K : 1024;
 }
 __host__ __device__ constexpr int ft_bucket_log2(int b) { return b == 0 ? 4 : b == 1 ? 6 : b == 2 ? 8 : 9; }
-// Table slots (log2) of a lane group of L lanes in class b: class 1 on 4 lanes takes 32 slots (its at most 32
-// distinct blocks fill it at worst; 64 slots for 16 groups would not fit), every other choice 2 * ft_bucket_max.
-__host__ __device__ constexpr int ft_table_log2(int b, int L) { return (b == 1 && L <= 4) ? 5 : ft_bucket_log2(b); }
-static_assert(FT_NB == 6 && ft_bucket_max(FT_NB - 3) == FT_SMALL_DEGREE, "degree classes");
-// LDS ints per wavefront for the lane-group tables: (WAVE / L) tables of class b must fit, which allows
-// L >= 2 for class 0, L >= 4 for class 1, L >= 32 for class 2 and L = 64 for class 3.  Four 4-wavefront
-// workgroups per CU (the 4 wavefronts per SIMD the register budget allows) take just under 160 KB.
+// LDS ints per wavefront for the lane-group tables.  Four 4-wavefront workgroups per CU (the 4 wavefronts per SIMD
+// the register budget allows) take just under 160 KB.
 constexpr int LDS_WAVE_INTS = 2496;
+// Table slots (log2) of a lane group of L lanes in class b: 2 * ft_bucket_max(b) (load factor <= 1/2) when WAVE / L
+// such tables fit a wavefront's LDS, else ft_bucket_max(b) (the class's most distinct blocks fill it at worst;
+// probes are bounded by the table size).
+__host__ __device__ constexpr int ft_table_log2(int b, int L) {
+    return (b >= 1 && (WAVE / L) * table_ints(ft_bucket_log2(b), ft_bucket_max(b)) > LDS_WAVE_INTS)
+               ? ft_bucket_log2(b) - 1
+               : ft_bucket_log2(b);
+}
+static_assert(FT_NB == 6 && ft_bucket_max(FT_NB - 3) == FT_SMALL_DEGREE, "degree classes");
+// (WAVE / L) tables of class b must fit a wavefront's LDS, which allows L >= 2 for class 0, L >= 4 for class 1,
+// L >= 16 for class 2 and L >= 32 for class 3.
 __host__ __device__ constexpr bool ft_lanes_fit(int b, int L) {
     return (WAVE / L) * table_ints(ft_table_log2(b, L), ft_bucket_max(b)) <= LDS_WAVE_INTS;
 }
-static_assert(ft_lanes_fit(0, 2) && ft_lanes_fit(1, 4) && ft_lanes_fit(2, 32) && ft_lanes_fit(3, 64), "LDS budget");
+static_assert(ft_lanes_fit(0, 2) && ft_lanes_fit(1, 4) && ft_lanes_fit(2, 16) && ft_lanes_fit(3, 32) &&
+                  !ft_lanes_fit(1, 2) && !ft_lanes_fit(2, 8) && !ft_lanes_fit(3, 16),
+              "LDS budget");
 // Read-only view of the block matrix.  Compact CSR: row b = [ptr[b], ptr[b+1]), with per-row inclusive
 // cumulative counts (columns: excluding the diagonal cell) for multinomial selection by binary search.
 // Slack layout (Metropolis-Hastings): row b = [start[b], start[b] + len[b]), no cumulative counts.

@@ -207,7 +207,7 @@ struct sbp_ctx_s {
     int32_t record_mode = -1;  // sbp_set_recorded_sweeps: -1 by size (or SBP_AMD_RECORDED_SWEEPS), 0 off, 1 on
     // launch shape of the finetune degree classes 0..4 (k <= 8, 32, 128, 256, 1,024): lanes (threads) per vertex,
     // and wavefronts per SIMD of the greedy parametric lane-group launches (sbp_set_finetune_shape)
-    int32_t ft_lanes[5] = {2, 4, 32, 64, 256};
+    int32_t ft_lanes[5] = {2, 4, 16, 64, 256};
     int32_t ft_occ[4] = {4, 4, 4, 4};
 };
 

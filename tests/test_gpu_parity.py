@@ -118,7 +118,8 @@ def test_propose_all_bit_exact(ctx, greedy, B):
 
 
 SHAPES = [((4, 8, 32, 64, 256), (4, 4, 4, 4)), ((4, 16, 32, 64, 128), (3, 3, 3, 3)),
-          ((2, 4, 32, 64, 256), (4, 4, 4, 4)), ((2, 4, 32, 64, 128), (3, 3, 3, 3)),
+          ((2, 4, 32, 64, 256), (4, 4, 4, 4)), ((2, 4, 16, 32, 128), (3, 3, 3, 3)),
+          ((4, 8, 16, 32, 256), (4, 4, 4, 4)),
           ((8, 8, 32, 64, 128), (3, 3, 3, 3)), ((16, 16, 32, 64, 256), (4, 4, 4, 4)),
           ((32, 32, 64, 64, 128), (3, 4, 3, 4)), ((64, 64, 64, 64, 256), (4, 3, 4, 3))]
 
@@ -156,23 +157,26 @@ def test_propose_all_every_launch_shape(greedy, shape):
 
 
 @pytest.mark.parametrize("greedy", [True, False])
-def test_full_lane_group_tables(greedy):
-    """Class-1 vertices on 4 lanes have 32-slot tables: vertices with exactly 32 edges into 32 distinct blocks other
-    than their own fill every slot, so looking up their own block (absent) must end without an empty slot."""
+@pytest.mark.parametrize("k,shape,B", [(32, (4, 4, 32, 64, 256), 48), (128, (4, 8, 16, 32, 256), 160),
+                                       (256, (4, 8, 16, 32, 256), 300)])
+def test_full_lane_group_tables(greedy, k, shape, B):
+    """Lane groups with fewer lanes than fit tables of twice the class's degree get tables of the class's degree
+    (class 1 on 4 lanes: 32 slots; class 2 on 16: 128; class 3 on 32: 256): vertices with exactly k edges into k
+    distinct blocks other than their own fill every slot, so looking up their own block (absent) must end without
+    an empty slot."""
     c = native.Context(0, seed=99)
-    c.set_finetune_shape((4, 4, 32, 64, 256), (4, 4, 4, 4))
+    c.set_finetune_shape(shape, (4, 4, 4, 4))
     base = small_graph(seed=12, N=1200, B=8, kmax=60, mean=6)
-    B = 48
     rng = np.random.default_rng(12)
     assign = rng.integers(0, B, base.N).astype(np.int32)
     assign[:B] = np.arange(B)
     src = list(np.repeat(np.arange(base.N), np.diff(base.out_off)))
     dst = list(base.out_adj)
     full = []
-    for j in range(12):  # new vertices, each with 32 out-edges only, one into each of 32 blocks other than its own
+    for j in range(12):  # new vertices, each with k out-edges only, one into each of k blocks other than its own
         v = base.N + j
         own = int(rng.integers(0, B))
-        others = rng.choice([b for b in range(B) if b != own], 32, replace=False)
+        others = rng.choice([b for b in range(B) if b != own], k, replace=False)
         for b in others:
             src.append(v)
             dst.append(int(b))  # vertex b is in block b (assign[:B] = arange)
@@ -194,7 +198,7 @@ def test_full_lane_group_tables(greedy):
 def test_launch_shape_rejects_unfit_choices(ctx):
     before = ctx.finetune_shape()
     for lanes, occ in [((1, 16, 32, 64, 256), (3, 3, 3, 3)), ((8, 2, 32, 64, 256), (3, 3, 3, 3)),
-                       ((8, 16, 16, 64, 256), (3, 3, 3, 3)), ((8, 16, 32, 32, 256), (3, 3, 3, 3)),
+                       ((8, 16, 8, 64, 256), (3, 3, 3, 3)), ((8, 16, 32, 16, 256), (3, 3, 3, 3)),
                        ((8, 16, 32, 64, 256), (2, 3, 3, 3)), ((8, 12, 32, 64, 256), (3, 3, 3, 3)),
                        ((8, 16, 32, 64, 512), (3, 3, 3, 3)), ((8, 16, 32, 64, 64), (3, 3, 3, 3))]:
         with pytest.raises(native.SbpError):
