@@ -377,8 +377,9 @@ inline double np_merge_finish(double entries, int64_t dout_r, int64_t din_r, int
 // Fork-join helper for the speculative evaluations: fn(i) for i in [0, n) on up to `threads` threads.  The
 // carry-out forks every few hundred microseconds, so idle workers spin on the generation counter for about
 // a millisecond before they sleep on the condition variable; a fork then costs a cache-line transfer, not
-// a futex wake-up per worker.  Every worker checks in once per generation, so a slow waker can never run a
-// later generation's indices with an earlier task.
+// a futex wake-up per worker.  A worker takes items of a generation only while its run is open, and the run
+// closes only after the workers inside it have left, so a slow waker can never run a later generation's
+// indices with an earlier task.
 class Pool {
   public:
     explicit Pool(int threads, int spin_us = spin_default()) : nthreads_(std::max(1, threads)), spin_us_(spin_us) {
@@ -413,14 +414,21 @@ class Pool {
         task_ = &f;
         n_ = n;
         next_.store(0, std::memory_order_relaxed);
-        pending_.store(nthreads_ - 1, std::memory_order_relaxed);
-        gen_.fetch_add(1);  // publishes the task (sequentially consistent, paired with sleepers_ below)
+        done_.store(0, std::memory_order_relaxed);
+        const uint64_t g = gen_.load(std::memory_order_relaxed) + 1;
+        open_.store(g);
+        gen_.store(g);  // publishes the task (sequentially consistent, paired with sleepers_ below)
         if (sleepers_.load() > 0) {
             std::lock_guard<std::mutex> lk(mu_);
             cv_.notify_all();
         }
+        // The caller works too, and waits for the items rather than for every worker: a worker that is asleep
+        // or descheduled does not hold up a short run.  Closing the run then waits only for the workers inside
+        // it; one that enters after the close sees it closed and never touches the task.
         drain(f);
-        while (pending_.load(std::memory_order_acquire) != 0) relax();
+        while (done_.load(std::memory_order_acquire) != n) relax();
+        open_.store(0);
+        while (inside_.load() != 0) relax();
         task_ = nullptr;
     }
 
@@ -433,7 +441,10 @@ class Pool {
 #endif
     }
     void drain(const std::function<void(int64_t)> &f) {
-        for (int64_t i; (i = next_.fetch_add(1, std::memory_order_relaxed)) < n_;) f(i);
+        for (int64_t i; (i = next_.fetch_add(1, std::memory_order_relaxed)) < n_;) {
+            f(i);
+            done_.fetch_add(1, std::memory_order_release);
+        }
     }
     void work() {
         uint64_t seen = 0;
@@ -452,16 +463,18 @@ class Pool {
             }
             seen = g;
             if (stop_.load()) return;
-            drain(*task_);
-            pending_.fetch_sub(1, std::memory_order_acq_rel);
+            inside_.fetch_add(1);  // sequentially consistent with the caller's close of the run
+            if (open_.load() == g) drain(*task_);
+            inside_.fetch_sub(1, std::memory_order_release);
         }
     }
     int nthreads_, spin_us_;
     std::vector<std::thread> workers_;
     std::mutex mu_;
     std::condition_variable cv_;
-    std::atomic<uint64_t> gen_{0};
-    std::atomic<int> sleepers_{0}, pending_{0};
+    std::atomic<uint64_t> gen_{0}, open_{0};  // open_: generation of the run that still takes workers, 0 if none
+    std::atomic<int> sleepers_{0}, inside_{0};
+    std::atomic<int64_t> done_{0};
     std::atomic<bool> stop_{false};
     const std::function<void(int64_t)> *task_ = nullptr;
     int64_t n_ = 0;
