@@ -123,7 +123,7 @@ void bm_alloc(sbp_bm bm, int32_t B, int32_t N) {
 void bm_release(sbp_bm bm) {
     void *ptrs[] = {bm->assign,    bm->sizes,     bm->d_out,   bm->d_in,    bm->d_tot,    bm->row_ptr,
                     bm->col_ptr,   bm->row_cells, bm->col_cells, bm->row_cum, bm->col_cum, bm->cell_hash,
-                    bm->hist,      bm->row_keys,  bm->col_keys};
+                    bm->hist,      bm->row_keys,  bm->col_keys, bm->dense};
     if (bm->pool) {
         for (void *p : ptrs) bm->pool->release(p);
     } else {
@@ -131,7 +131,9 @@ void bm_release(sbp_bm bm) {
             if (p) (void)dev_free(p);
     }
     bm->cell_hash = nullptr;
-    bm->hash_valid = false;
+    bm->hash_valid = bm->dense_valid = false;
+    bm->dense = nullptr;
+    bm->dense_B = 0;
     bm->hist = nullptr;
     bm->hist_log2 = 0;
     bm->hist_ready = false;
@@ -261,6 +263,48 @@ void bm_ensure_hash(sbp_ctx ctx, sbp_bm bm, bool capture) {
     bm->hash_used = bm->nnz;
 }
 
+// Dense block matrix from the compact CSR: one wavefront per row writes the row's cells (the array is zeroed first).
+__global__ void k_dense_cells(int32_t B, const int32_t *__restrict__ row_ptr, const int2 *__restrict__ row_cells,
+                              int32_t *__restrict__ dense) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; r < B;
+         r += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+        int32_t *row = dense + r * (int64_t)B;
+        for (int32_t i = row_ptr[r] + lane; i < row_ptr[r + 1]; i += 64) {
+            const int2 c = row_cells[i];
+            row[c.x] = c.y;
+        }
+    }
+}
+
+// Largest block count with a dense copy (256 MB of counts), and the smallest graph for which it pays for its
+// build on each rebuild.
+constexpr int32_t DENSE_MAX_BLOCKS = 8192;
+constexpr int64_t DENSE_MIN_EDGES = int64_t(1) << 22;
+
+const int32_t *bm_ensure_dense(sbp_ctx ctx, sbp_bm bm, sbp_graph g) {
+    const char *e = getenv("SBP_AMD_DENSE");
+    const bool off = e && e[0] == '0', forced = e && e[0] == '1';
+    if (off || bm->B > DENSE_MAX_BLOCKS || bm->B < 1 || (!forced && g->E < DENSE_MIN_EDGES)) return nullptr;
+    if (bm->dense_valid && bm->dense_B == bm->B) return bm->dense;
+    hipStream_t st = ctx->stream;
+    const int64_t n = (int64_t)bm->B * bm->B;
+    if (!bm->dense || (int64_t)bm->dense_B * bm->dense_B < n) {
+        DevicePool &pool = pool_of(bm);
+        pool.release(bm->dense);
+        bm->dense = static_cast<int32_t *>(pool.alloc(sizeof(int32_t) * (size_t)n));
+    }
+    bm->dense_B = bm->B;
+    SBP_HIP(sbpprim::fill<int32_t>(bm->dense, n, 0, st));
+    if (bm->nnz > 0) {
+        const int64_t blocks = std::min<int64_t>((bm->B + 3) / 4, 16384);
+        k_dense_cells<<<(int)std::max<int64_t>(blocks, 1), 256, 0, st>>>(bm->B, bm->row_ptr, bm->row_cells, bm->dense);
+        SBP_HIP(hipGetLastError());
+    }
+    bm->dense_valid = true;
+    return bm->dense;
+}
+
 __global__ void k_hist_clear(int64_t n, sbpdev::HistSlot *__restrict__ t) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         t[i] = sbpdev::HistSlot{sbpdev::HIST_EMPTY, 0, 0};
@@ -359,7 +403,7 @@ void bm_reserve_cells(sbp_ctx ctx, sbp_graph g, sbp_bm bm) {
         bm->col_keys = static_cast<unsigned long long *>(pool.alloc(sizeof(unsigned long long) * (size_t)bound));
         bm->keys_valid = false;
         bm->cap = bound;
-        bm->hash_valid = false;
+        bm->hash_valid = bm->dense_valid = false;
     }
     const int lg = hash_log2_for(bound);
     if (!bm->cell_hash || bm->hash_log2 < lg) {
@@ -367,7 +411,7 @@ void bm_reserve_cells(sbp_ctx ctx, sbp_graph g, sbp_bm bm) {
         pool.release(bm->cell_hash);
         bm->cell_hash = static_cast<sbpdev::CellSlot *>(pool.alloc(sizeof(sbpdev::CellSlot) << lg));
         bm->hash_log2 = lg;
-        bm->hash_valid = false;
+        bm->hash_valid = bm->dense_valid = false;
     }
     // the rebuild's scratch at its largest: keys of every edge, the transposed sort of every cell
     const int64_t E = std::max<int64_t>(g->E, 1);
@@ -398,7 +442,7 @@ void bm_sync_nnz(sbp_ctx ctx, sbp_bm bm) {
 
 void bm_rebuild(sbp_ctx ctx, sbp_graph g, sbp_bm bm, bool keep_hist, bool capture) {
     hipStream_t st = ctx->stream;
-    bm->hash_valid = false;
+    bm->hash_valid = bm->dense_valid = false;
     SBP_CHECK(!capture || !bm->dc, SBP_E_STATE, "recorded rebuild: degree-corrected histograms need host bookkeeping");
     if (bm->dc && !keep_hist) bm_build_hist(ctx, g, bm);
     if (!bm->dc) bm->hist_ready = false;
@@ -525,7 +569,7 @@ void bm_copy_into(sbp_ctx ctx, sbp_bm src, sbp_bm dst) {
     }
     dst->overall_entropy = src->overall_entropy;
     dst->num_to_merge = 0;
-    dst->hash_valid = false;
+    dst->hash_valid = dst->dense_valid = false;
     dst->keys_valid = false;
     dst->np = src->np;
     dst->dc = src->dc;

@@ -309,7 +309,7 @@ static double replay_sweep(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params
     read_small(ctx, {{&nnz, bm->row_ptr + bm->B, sizeof(int32_t)}});
     const SweepResult &r = *rs.h;
     bm->nnz = nnz;
-    bm->hash_valid = false;
+    bm->hash_valid = bm->dense_valid = false;
     SBP_CHECK(r.err == 0, SBP_E_STATE, "finetune proposal: block degrees inconsistent with the block matrix");
     SBP_CHECK(r.lp_err == 0, SBP_E_STATE, "log_posterior: non-zero cell with zero block degree");
     if (st) {

@@ -91,6 +91,8 @@ struct BmView {
     const CellSlot *hash;                      // cell hash of the compact CSR (null for the slack layout)
     int hash_shift;
     unsigned hash_mask;
+    const int32_t *dense;                      // B x B counts, row-major, when kept (looked up before the hash)
+    int64_t dense_stride;
     __device__ __forceinline__ int row_begin(int b) const { return rbeg[b]; }
     __device__ __forceinline__ int row_end(int b) const { return rlen ? rbeg[b] + rlen[b] : rbeg[b + 1]; }
     __device__ __forceinline__ int col_begin(int b) const { return cbeg[b]; }
@@ -123,6 +125,7 @@ struct FtArgs {
 };
 
 __device__ __forceinline__ long long get_cell(const FtArgs &a, int row, int col) {
+    if (a.m.dense) return a.m.dense[row * a.m.dense_stride + col];
     if (a.m.hash) return cell_hash_get(a.m.hash, a.m.hash_shift, a.m.hash_mask, row, col);
     int r0 = a.m.row_begin(row), r1 = a.m.row_end(row);
     int c0 = a.m.col_begin(col), c1 = a.m.col_end(col);
@@ -134,6 +137,11 @@ __device__ __forceinline__ long long get_cell(const FtArgs &a, int row, int col)
 // step issues both loads: the same values as two get_cell calls in half the dependent round trips.
 __device__ __forceinline__ void get_cell_pair(const FtArgs &a, int row1, int col1, int row2, int col2, long long &v1,
                                               long long &v2) {
+    if (a.m.dense) {
+        v1 = a.m.dense[row1 * a.m.dense_stride + col1];
+        v2 = a.m.dense[row2 * a.m.dense_stride + col2];
+        return;
+    }
     if (a.m.hash) {
         v1 = cell_hash_get(a.m.hash, a.m.hash_shift, a.m.hash_mask, row1, col1);
         v2 = cell_hash_get(a.m.hash, a.m.hash_shift, a.m.hash_mask, row2, col2);
@@ -978,6 +986,8 @@ static FtArgs base_args(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     a.m.hash = nullptr;
     a.m.hash_shift = 0;
     a.m.hash_mask = 0;
+    a.m.dense = nullptr;
+    a.m.dense_stride = 0;
     a.items = nullptr;
     a.count = 0;
     a.move_s = nullptr;
@@ -1157,6 +1167,10 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     a.m.hash = bm->cell_hash;
     a.m.hash_shift = 64 - bm->hash_log2;
     a.m.hash_mask = (1u << bm->hash_log2) - 1;
+    if (!capture) {  // a recorded sweep keeps to the hash, which each replay rebuilds
+        a.m.dense = bm_ensure_dense(ctx, bm, g);
+        a.m.dense_stride = bm->B;
+    }
     a.err = reinterpret_cast<int *>(counters + FT_NB);
     a.items = d_vertices;
     a.count = count;
@@ -2234,7 +2248,7 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         k_compact_copy<<<grid_of((int64_t)B * 64), 64, 0, st>>>(B, h.rstart, h.rlen, rc, rp, bm->row_cells);
         k_compact_copy<<<grid_of((int64_t)B * 64), 64, 0, st>>>(B, h.cstart, h.clen, cc, cp, bm->col_cells);
         bm->nnz = nnz[0];
-        bm->hash_valid = false;
+        bm->hash_valid = bm->dense_valid = false;
         bm->keys_valid = false;
         bm_compute_cum(ctx, bm);
         SBP_HIP(hipStreamSynchronize(st));

@@ -311,6 +311,11 @@ struct sbp_bm_s {
     int hash_log2 = 0;
     bool hash_valid = false;
     int64_t hash_used = 0;  // claimed slots of the hash
+    // dense copy of the block matrix (B x B counts, row-major) for lookups when B is small; kept by the in-place
+    // cell update, dropped with the hash by every other change to the cells
+    int32_t *dense = nullptr;
+    int32_t dense_B = 0;
+    bool dense_valid = false;
     // sorted cell keys (row << keys_cbits | col and col << keys_cbits | row) of the current cells, kept by eager
     // rebuilds for the update after moves
     unsigned long long *row_keys = nullptr, *col_keys = nullptr;  // [cap]
@@ -326,6 +331,9 @@ int bits_for_blocks(int64_t B);
 void bm_fill_cells(sbp_ctx ctx, sbp_bm bm, const unsigned long long *row_keys, const int32_t *row_counts,
                    const unsigned long long *col_keys, const int32_t *col_counts);
 void bm_count_sizes(sbp_ctx ctx, sbp_bm bm);
+// the dense block matrix for lookups when it is wanted (B small, large graph; SBP_AMD_DENSE=0/1 turns it off/on
+// for any B up to the bound), built on demand; nullptr when not wanted
+const int32_t *bm_ensure_dense(sbp_ctx ctx, sbp_bm bm, sbp_graph g);
 // update.hip: the cells after committed moves (prev: the assignment before them); false when the rebuild must run
 bool bm_update_moves(sbp_ctx ctx, sbp_graph g, sbp_bm bm, const int32_t *prev, int64_t moved_hint);
 bool bm_update_wanted(sbp_graph g);  // graph large enough for the update (or SBP_AMD_INCREMENTAL forcing it)

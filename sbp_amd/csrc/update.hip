@@ -179,7 +179,8 @@ __global__ void k_locate(const int64_t *__restrict__ n_runs, const unsigned long
                          const unsigned long long *__restrict__ keys, int2 *__restrict__ cells,
                          int32_t *__restrict__ pos, int32_t *__restrict__ ins, int32_t *__restrict__ at,
                          CellSlot *__restrict__ hash, int shift, unsigned hmask, int cbits,
-                         unsigned long long *__restrict__ inserted, int32_t *__restrict__ err) {
+                         unsigned long long *__restrict__ inserted, int32_t *__restrict__ err,
+                         int32_t *__restrict__ dense, int64_t dstride) {
     const int64_t n = *n_runs;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const unsigned long long k = dkey[i];
@@ -210,6 +211,7 @@ __global__ void k_locate(const int64_t *__restrict__ n_runs, const unsigned long
             const unsigned long long mask = (1ull << cbits) - 1;
             hash_upsert(hash, shift, hmask, cell_key((int)(k >> cbits), (int)(k & mask)), c, inserted, err);
         }
+        if (dense) dense[(int64_t)(k >> cbits) * dstride + (int64_t)(k & ((1ull << cbits) - 1))] = c;
     }
 }
 
@@ -273,7 +275,7 @@ void scan_into(sbp_ctx ctx, const TI *in, T *out, int64_t n, bool inclusive) {
 void merge_major(sbp_ctx ctx, sbp_bm bm, int64_t n_max, const int64_t *n_runs, const unsigned long long *dkey,
                  const int32_t *dnet, unsigned long long none, const unsigned long long *keys, int2 *cells,
                  bool with_hash, unsigned long long *inserted, int32_t *err, unsigned long long *nkeys, int32_t *ncnt,
-                 int64_t *new_nnz, ScratchSlot s_pos, ScratchSlot s_at, ScratchSlot s_rem) {
+                 int64_t *new_nnz, ScratchSlot s_pos, ScratchSlot s_at, ScratchSlot s_rem, int32_t *dense = nullptr) {
     hipStream_t st = ctx->stream;
     const int64_t nnz = bm->nnz;
     int32_t *pos = ctx->scratch[s_pos].as<int32_t>(3 * (size_t)n_max + 4);
@@ -286,7 +288,7 @@ void merge_major(sbp_ctx ctx, sbp_bm bm, int64_t n_max, const int64_t *n_runs, c
     const int lg = bm->hash_log2;
     k_locate<<<upd_grid(n_max), UPD_THREADS, 0, st>>>(n_runs, dkey, dnet, none, nnz, keys, cells, pos, ins, at, hash,
                                                        64 - lg, (unsigned)((1ull << lg) - 1), bm->keys_cbits, inserted,
-                                                       err);
+                                                       err, dense, (int64_t)bm->B);
     k_removed<<<upd_grid(nnz + 1), UPD_THREADS, 0, st>>>(nnz, cells, rem);
     scan_into<int32_t>(ctx, rem, rem, nnz + 1, false);  // removed before i (rem[nnz]: all removed)
     scan_into<int32_t>(ctx, at, at, nnz + 1, true);     // insertions at ranks <= i
@@ -391,8 +393,10 @@ bool bm_update_moves(sbp_ctx ctx, sbp_graph g, sbp_bm bm, const int32_t *prev, i
     int64_t *new_nnz = reinterpret_cast<int64_t *>(counters + 4);
     // the hash takes the update when even every cell being new keeps its load under 0.9
     const bool with_hash = bm->hash_valid && bm->cell_hash && (bm->hash_used + runs) * 10 < (int64_t(9) << bm->hash_log2);
+    // the dense copy, when kept, takes every changed count in the same pass
+    int32_t *dense = bm->dense_valid && bm->dense_B == bm->B ? bm->dense : nullptr;
     merge_major(ctx, bm, nk, n_runs, dkey, dnet, none, bm->row_keys, bm->row_cells, with_hash, counters + 5, err, rk,
-                rc, new_nnz, S_UPD_POS, S_UPD_AT, S_UPD_REM);
+                rc, new_nnz, S_UPD_POS, S_UPD_AT, S_UPD_REM, dense);
     merge_major(ctx, bm, nk, n_runs, tkey, tnet, none, bm->col_keys, bm->col_cells, false, counters + 5, err, ck, cc,
                 new_nnz + 1, S_UPD_POS, S_UPD_AT, S_UPD_REM);
     int64_t nn[2] = {0, 0};

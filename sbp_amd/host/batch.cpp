@@ -70,7 +70,13 @@ uint64_t sbp_footprint_bytes(int32_t N, int64_t E, int64_t max_degree, int64_t n
     // keys) at capacity 9/8 nnz, 28 N (assignment and six per-block arrays, B <= N) and a cell hash of up to
     // 96 nnz (16-byte slots, 3 to 6 per cell).  The pool holds at most six at once with the released buffers it
     // keeps for reuse: the triplet's three, the current one, the merged one and a copy.
-    const uint64_t pool = 6 * (141 * e + 28 * n + 4096);
+    uint64_t pool = 6 * (141 * e + 28 * n + 4096);
+    // Graphs of at least 2^22 edges also keep a dense copy of each blockmodel with at most 8,192 blocks: up to
+    // min(N, 8192)^2 counts for each of the six.
+    if (e >= (uint64_t(1) << 22)) {
+        const uint64_t b = std::min<uint64_t>(n, 8192);
+        pool += 6 * 4 * b * b;
+    }
     // Rebuild sort keys and values, move records and per-block arrays; the update after moves (at most E / 2
     // signed deltas: their keys, runs and nets, merged key and count lists of up to 3/2 E cells, ranks).
     uint64_t scratch = 72 * e + 128 * n + (uint64_t(32) << 20) + (68 * e + 8 * n) * 5 / 4;

@@ -3,7 +3,8 @@ into the sorted row-major and column-major cell lists, the cell hash updated in 
 and the oracle: after every sweep the exported state (CSR, CSC, degrees, sizes, assignment) must be bit-exact, and
 the next sweep's proposals, which read the updated hash, must equal the oracle's.  Forced on (SBP_AMD_INCREMENTAL=1)
 and off (0), on a graph with hubs and self-loops, with blocks that empty and blocks that fill, under both
-description lengths, and over enough sweeps for the hash to pass its load limit and be rebuilt."""
+description lengths, and over enough sweeps for the hash to pass its load limit and be rebuilt.  Each case also runs
+with the dense block matrix forced on (SBP_AMD_DENSE=1: lookups read a B x B copy that the update keeps) and off."""
 import numpy as np
 import pytest
 
@@ -22,9 +23,10 @@ def ctx():
     c.close()
 
 
-def run(ctx, g, B, assign, sweeps, mode, nonparametric=False, seed=5, batches=1):
+def run(ctx, g, B, assign, sweeps, mode, nonparametric=False, seed=5, batches=1, dense="0"):
     import os
     os.environ["SBP_AMD_INCREMENTAL"] = mode
+    os.environ["SBP_AMD_DENSE"] = dense
     try:
         og = O.OGraph.from_csr(g)
         ob = O.OBlockmodel.build(og, B, assign, nonparametric=nonparametric)
@@ -59,33 +61,56 @@ def run(ctx, g, B, assign, sweeps, mode, nonparametric=False, seed=5, batches=1)
         return db.export(), moves
     finally:
         os.environ.pop("SBP_AMD_INCREMENTAL", None)
+        os.environ.pop("SBP_AMD_DENSE", None)
 
 
+@pytest.mark.parametrize("dense", ["0", "1"])
 @pytest.mark.parametrize("nonparametric", [False, True])
-def test_update_matches_rebuild_and_oracle(ctx, nonparametric):
+def test_update_matches_rebuild_and_oracle(ctx, nonparametric, dense):
     g = hub_graph(seed=3, N=6000, degree=1500)
     B = 40
     assign = (np.arange(g.N) % B).astype(np.int32)
-    a, moves_a = run(ctx, g, B, assign, 8, "1", nonparametric)
-    b, moves_b = run(ctx, g, B, assign, 8, "0", nonparametric)
+    a, moves_a = run(ctx, g, B, assign, 8, "1", nonparametric, dense=dense)
+    b, moves_b = run(ctx, g, B, assign, 8, "0", nonparametric, dense=dense)
     assert moves_a == moves_b and sum(moves_a) > 0
     for k in a:
         assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
 
 
-def test_update_with_emptying_blocks(ctx):
+@pytest.mark.parametrize("dense", ["0", "1"])
+def test_update_with_emptying_blocks(ctx, dense):
     """Blocks of about two vertices: moves empty blocks (their rows and columns leave the lists) and refill them."""
     g = dcsbm(900, 8, gamma=2.5, kmin=1, kmax=40, mean_degree=6, p_in=0.7, seed=12)
     B = 420
     rng = np.random.default_rng(2)
     assign = np.concatenate([np.arange(B), rng.integers(0, B, g.N - B)]).astype(np.int32)
-    run(ctx, g, B, assign, 6, "1", batches=3)
+    run(ctx, g, B, assign, 6, "1", batches=3, dense=dense)
 
 
-def test_update_many_sweeps_hash_reload(ctx):
+@pytest.mark.parametrize("dense", ["0", "1"])
+def test_update_many_sweeps_hash_reload(ctx, dense):
     """Twenty sweeps from a random partition: the hash takes new cells each sweep until its load limit, then the
-    next proposal launch rebuilds it."""
+    next proposal launch rebuilds it (the dense copy, when kept, stays valid throughout)."""
     g = dcsbm(20000, 30, gamma=2.3, kmin=1, kmax=300, mean_degree=10, p_in=0.6, seed=4)
     B = 2000
     assign = np.random.default_rng(9).integers(0, B, g.N).astype(np.int32)
-    run(ctx, g, B, assign, 20, "1")
+    run(ctx, g, B, assign, 20, "1", dense=dense)
+
+
+@pytest.mark.parametrize("alg", ["async_gibbs", "hybrid_mcmc"])
+def test_partition_dense_bit_exact(ctx, alg):
+    """Full partitions with the dense block matrix forced on: every merge phase with B <= 8192 builds one, and
+    the finetune sweeps read it."""
+    import os
+    g = dcsbm(3000, 16, gamma=2.3, kmin=1, kmax=150, mean_degree=10, p_in=0.7, seed=6)
+    og = O.OGraph.from_csr(g)
+    a_o, nb_o, st_o = O.sbp(og, O.params(alg, seed=2, threads=8))
+    os.environ["SBP_AMD_DENSE"] = "1"
+    try:
+        dg = ctx.upload(g)
+        a_d, nb_d, st_d = dg.partition(native.params(alg, seed=2))
+    finally:
+        os.environ.pop("SBP_AMD_DENSE", None)
+    assert nb_d == nb_o
+    assert np.array_equal(a_d, a_o)
+    assert st_d.mdl == st_o.mdl

@@ -58,9 +58,13 @@ def test_footprint_bound_is_monotone_and_covers_the_graph():
     assert int(L.sbp_footprint_bytes(1000, 20000, 100, 0, 1)) > base
     assert int(L.sbp_footprint_bytes(1000, 20000, 100, 0, 2)) == int(L.sbp_footprint_bytes(1000, 20000, 100, 0, 1))
     # the tables are capped at 1 GiB: the largest served request stays a small part of a 200 GB budget (its
-    # blockmodels with their sorted cell keys and the cell update's work areas included)
+    # blockmodels with their sorted cell keys, their dense copies and the cell update's work areas included)
     big = int(L.sbp_footprint_bytes(2_000_000, 32_000_000, 2000, 100_000, 0))
-    assert big < 35e9
+    assert big < 37e9
+    # graphs of 2^22 edges or more add a dense copy of up to 8,192^2 counts for each of six blockmodels
+    e = 1 << 22
+    assert (int(L.sbp_footprint_bytes(100_000, e, 100, 0, 0)) - int(L.sbp_footprint_bytes(100_000, e - 1, 100, 0, 0))
+            >= 6 * 4 * 8192 ** 2)
     assert big - int(L.sbp_footprint_bytes(2_000_000, 32_000_000, 2000, 1, 0)) <= (1 << 30) * 5 // 4 + (1 << 20)
 
 
