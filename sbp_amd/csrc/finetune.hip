@@ -1163,13 +1163,15 @@ FtLaunchInfo ft_propose(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     SBP_HIP(sbpprim::fill<unsigned long long>(counters, 8, 0ull, st));
     FtArgs a = base_args(ctx, bm, g, epoch, greedy);
     a.epoch_ptr = epoch_ptr;
-    bm_ensure_hash(ctx, bm, capture);
-    a.m.hash = bm->cell_hash;
-    a.m.hash_shift = 64 - bm->hash_log2;
-    a.m.hash_mask = (1u << bm->hash_log2) - 1;
-    if (!capture) {  // a recorded sweep keeps to the hash, which each replay rebuilds
-        a.m.dense = bm_ensure_dense(ctx, bm, g);
-        a.m.dense_stride = bm->B;
+    // lookups read the dense copy when it is kept, else the cell hash (a recorded sweep keeps to the hash, which
+    // each replay rebuilds); the hash is then neither built nor updated, and its other users build it on demand
+    a.m.dense = capture ? nullptr : bm_ensure_dense(ctx, bm, g);
+    a.m.dense_stride = bm->B;
+    if (!a.m.dense) {
+        bm_ensure_hash(ctx, bm, capture);
+        a.m.hash = bm->cell_hash;
+        a.m.hash_shift = 64 - bm->hash_log2;
+        a.m.hash_mask = (1u << bm->hash_log2) - 1;
     }
     a.err = reinterpret_cast<int *>(counters + FT_NB);
     a.items = d_vertices;
