@@ -499,17 +499,32 @@ static sbp_bm merge_apply(sbp_ctx ctx, sbp_bm bm, sbp_graph g, std::vector<int32
     SBP_HIP(dev_copy(sizes.data(), bm->sizes, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
     sbpcarry::Model model;
     sbpcarry::Pool *pool = nullptr;
+    double t_copied = t0;
     if (!approximate) {  // the advanced carry-out re-evaluates merges against the changing blockmodel
-        std::vector<int32_t> rp(B + 1), cp(B + 1), dout(B), din(B);
-        std::vector<int2> rc(bm->nnz), cc(bm->nnz);
-        SBP_HIP(dev_copy(rp.data(), bm->row_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
-        SBP_HIP(dev_copy(cp.data(), bm->col_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost));
-        SBP_HIP(dev_copy(dout.data(), bm->d_out, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
-        SBP_HIP(dev_copy(din.data(), bm->d_in, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
-        if (bm->nnz) {
-            SBP_HIP(dev_copy(rc.data(), bm->row_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
-            SBP_HIP(dev_copy(cc.data(), bm->col_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost));
+        // the CSR, CSC and degrees in one pinned staging area that the context keeps (no pageable copies, no
+        // freshly faulted pages per merge phase); int2 cells start at 16 B + 8 bytes, a multiple of 8
+        const size_t words = 2 * (size_t)(B + 1) + 2 * (size_t)B;
+        const size_t need = sizeof(int32_t) * words + 2 * sizeof(int2) * (size_t)bm->nnz;
+        if (ctx->stage_bytes < need) {
+            if (ctx->stage) SBP_HIP(dev_host_free(ctx->stage));
+            ctx->stage = nullptr;
+            ctx->stage_bytes = 0;
+            const size_t bytes = need + need / 4 + 4096;
+            SBP_HIP(hipHostMalloc(&ctx->stage, bytes, hipHostMallocDefault));
+            ctx->stage_bytes = bytes;
         }
+        int32_t *rp = static_cast<int32_t *>(ctx->stage), *cp = rp + (B + 1), *dout = cp + (B + 1), *din = dout + B;
+        int2 *rc = reinterpret_cast<int2 *>(din + B), *cc = rc + bm->nnz;
+        SBP_HIP(hipMemcpyAsync(rp, bm->row_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost, s));
+        SBP_HIP(hipMemcpyAsync(cp, bm->col_ptr, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost, s));
+        SBP_HIP(hipMemcpyAsync(dout, bm->d_out, sizeof(int32_t) * B, hipMemcpyDeviceToHost, s));
+        SBP_HIP(hipMemcpyAsync(din, bm->d_in, sizeof(int32_t) * B, hipMemcpyDeviceToHost, s));
+        if (bm->nnz) {
+            SBP_HIP(hipMemcpyAsync(rc, bm->row_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost, s));
+            SBP_HIP(hipMemcpyAsync(cc, bm->col_cells, sizeof(int2) * bm->nnz, hipMemcpyDeviceToHost, s));
+        }
+        SBP_HIP(hipStreamSynchronize(s));
+        t_copied = now_ms();
         int threads = ctx->carry_threads > 0 ? ctx->carry_threads : sbpcarry::carry_threads();
         // the parametric carry-out applies merges on a thread of its own beside the pool: leave it a core
         if (!bm->np && B >= sbpcarry::PIPE_MIN_BLOCKS && bm->nnz < sbpcarry::PIPE_MAX_ROW_CELLS * (int64_t)B &&
@@ -518,9 +533,9 @@ static sbp_bm merge_apply(sbp_ctx ctx, sbp_bm bm, sbp_graph g, std::vector<int32
         if (!ctx->carry_pool || ctx->carry_pool->threads() != threads)
             ctx->carry_pool = std::make_shared<sbpcarry::Pool>(threads);
         pool = ctx->carry_pool.get();
-        const int32_t *rcp = reinterpret_cast<const int32_t *>(rc.data());
-        const int32_t *ccp = reinterpret_cast<const int32_t *>(cc.data());
-        model.load(B, rp.data(), rcp, rcp + 1, cp.data(), ccp, ccp + 1, 2, dout.data(), din.data(), pool);
+        const int32_t *rcp = reinterpret_cast<const int32_t *>(rc);
+        const int32_t *ccp = reinterpret_cast<const int32_t *>(cc);
+        model.load(B, rp, rcp, rcp + 1, cp, ccp, ccp + 1, 2, dout, din, pool);
         if (bm->np) model.set_nonparametric(sizes.data(), bm->N);
         if (bm->dc) {
             SBP_CHECK(bm->hist_ready, SBP_E_STATE, "degree histograms not built");
@@ -549,9 +564,10 @@ static sbp_bm merge_apply(sbp_ctx ctx, sbp_bm bm, sbp_graph g, std::vector<int32
         st->carry_requeues += cs.requeues;
     }
     if (getenv("SBP_AMD_TRACE"))
-        fprintf(stderr, "[sbp] merge B=%d -> %d: propose %.1f ms, carry %.1f ms (load %.1f), pops %lld, evals %lld, "
-                "speculative %lld, reused %lld, requeues %lld\n", B, new_B, elapsed_ms(ctx, ctx->ev[2], ctx->ev[3]),
-                now_ms() - t0, t_loaded - t0, (long long)cs.pops, (long long)cs.evaluations,
+        fprintf(stderr, "[sbp] merge B=%d -> %d: propose %.1f ms, carry %.1f ms (load %.1f, copy %.1f), pops %lld, "
+                "evals %lld, speculative %lld, reused %lld, requeues %lld\n", B, new_B,
+                elapsed_ms(ctx, ctx->ev[2], ctx->ev[3]), now_ms() - t0, t_loaded - t0, t_copied - t0,
+                (long long)cs.pops, (long long)cs.evaluations,
                 (long long)cs.speculative, (long long)cs.reused, (long long)cs.requeues);
     double t1 = now_ms();
     int32_t *d_map = d_best_s;  // reuse
@@ -649,6 +665,7 @@ int sbp_destroy(sbp_ctx ctx) {
     }
     if (ctx->lgamma) (void)dev_free(ctx->lgamma);
     if (ctx->pinned) (void)dev_host_free(ctx->pinned);
+    if (ctx->stage) (void)dev_host_free(ctx->stage);
     ctx->pool.reset();  // freed here unless blockmodels of this context still hold buffers
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -191,6 +191,8 @@ struct sbp_ctx_s {
     hipEvent_t ev[8] = {};
     hipEvent_t ft_ev[8] = {};  // boundaries of the finetune proposal launches, one per degree class
     void *pinned = nullptr;  // pinned staging area for small reads (sbpimpl::PINNED_BYTES)
+    void *stage = nullptr;   // pinned staging of the block matrix for the host carry-out, grown on demand
+    size_t stage_bytes = 0;
     double *lgamma = nullptr;  // fastlgamma table for the nonparametric entropy, grown on demand
     int64_t lgamma_n = 0;
     // the global-memory neighbour-block tables in scratch slot S_HASH that are known to be empty (every kernel
