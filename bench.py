@@ -553,23 +553,23 @@ def main():
     if args.served_slots is None:
         # the autotuner measures the batched workload; for Poisson-arriving requests 16 workers measured best
         args.served_slots = 16
-    # the mixed-size graphs are generated in worker processes, forked here before the device is touched; their
-    # work is submitted once the headline and its CPU baseline are done, so those run on an otherwise idle host
-    n_workers = min(8, os.cpu_count() or 1)
-    pool = cf.ProcessPoolExecutor(max_workers=n_workers, mp_context=mp.get_context("fork"))
+    # the mixed-size graphs are generated in worker processes (forked before the device is touched) while the
+    # large graph is generated; every timed leg starts after they are all done, on an otherwise idle host
+    pool = cf.ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1), mp_context=mp.get_context("fork"))
     nb_graphs = args.batched_graphs if "batched" in legs else 0
     ns_graphs = args.served_requests if "served" in legs else 0
-    fut_b, fut_s = [], []
-    if nb_graphs or ns_graphs:
-        for f in [pool.submit(time.sleep, 0.5) for _ in range(n_workers)]:  # busy workers: each submit forks one
-            f.result()
+    fut_b = [pool.submit(workloads.make, workloads.batched_spec(i)) for i in range(nb_graphs)]
+    fut_s = [pool.submit(workloads.make, workloads.served_spec(j, n_max=args.served_max_n)) for j in range(ns_graphs)]
 
-    def submit_workloads():
-        if fut_b or fut_s:
-            return
-        fut_b.extend(pool.submit(workloads.make, workloads.batched_spec(i)) for i in range(nb_graphs))
-        fut_s.extend(pool.submit(workloads.make, workloads.served_spec(j, n_max=args.served_max_n))
-                     for j in range(ns_graphs))
+    waited = []
+
+    def workloads_ready():
+        if (fut_b or fut_s) and not waited:
+            waited.append(True)
+            t = time.perf_counter()
+            cf.wait(fut_b + fut_s)
+            print(f"[bench] batched and served graphs ready ({time.perf_counter() - t:.1f}s wait)", file=sys.stderr,
+                  flush=True)
 
     from sbp_amd import native
 
@@ -583,6 +583,7 @@ def main():
         ctx = native.Context(0, seed=0)
         dg = ctx.upload(g)
         ctx.synchronize()
+        workloads_ready()
         if args.only in (None, "headline"):
             tot, wall, mdl0, mdl, B = headline_leg(ctx, dg, g, w, args.steps, args.warmup)
             cpu = None
@@ -615,7 +616,6 @@ def main():
                 "finetune_shape": {"lanes": ctx.finetune_shape()[0], "waves_per_simd": ctx.finetune_shape()[1]},
                 "cpu_baseline": cpu,
             }
-        submit_workloads()
         if "split" in legs:
             out["split"] = split_leg(ctx, dg, g, w)
         if "sampled" in legs:
@@ -623,7 +623,7 @@ def main():
         del dg
         ctx.close()
         del g
-    submit_workloads()
+    workloads_ready()
     if "c2" in legs:
         out["c2"] = c2_leg(args.c2_runs, args.c2_cpu)
     if "c1" in legs:
