@@ -282,9 +282,9 @@ __global__ void k_dense_cells(int32_t B, const int32_t *__restrict__ row_ptr, co
 constexpr int32_t DENSE_MAX_BLOCKS = 8192;
 constexpr int64_t DENSE_MIN_EDGES = int64_t(1) << 22;
 
-const int32_t *bm_ensure_dense(sbp_ctx ctx, sbp_bm bm, sbp_graph g) {
+const int32_t *bm_ensure_dense(sbp_ctx ctx, sbp_bm bm, sbp_graph g, bool any_graph) {
     const char *e = getenv("SBP_AMD_DENSE");
-    const bool off = e && e[0] == '0', forced = e && e[0] == '1';
+    const bool off = e && e[0] == '0', forced = (e && e[0] == '1') || any_graph;
     if (off || bm->B > DENSE_MAX_BLOCKS || bm->B < 1 || (!forced && g->E < DENSE_MIN_EDGES)) return nullptr;
     if (bm->dense_valid && bm->dense_B == bm->B) return bm->dense;
     hipStream_t st = ctx->stream;

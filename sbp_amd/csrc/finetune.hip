@@ -92,6 +92,7 @@ struct BmView {
     int hash_shift;
     unsigned hash_mask;
     const int32_t *dense;                      // B x B counts, row-major, when kept (looked up before the hash)
+    int32_t *dense_w;                          // the same array, written by the Metropolis-Hastings commits
     int64_t dense_stride;
     __device__ __forceinline__ int row_begin(int b) const { return rbeg[b]; }
     __device__ __forceinline__ int row_end(int b) const { return rlen ? rbeg[b] + rlen[b] : rbeg[b + 1]; }
@@ -987,6 +988,7 @@ static FtArgs base_args(sbp_ctx ctx, sbp_bm bm, sbp_graph g, uint32_t epoch, int
     a.m.hash_shift = 0;
     a.m.hash_mask = 0;
     a.m.dense = nullptr;
+    a.m.dense_w = nullptr;
     a.m.dense_stride = 0;
     a.items = nullptr;
     a.count = 0;
@@ -1692,8 +1694,15 @@ __device__ void slack_add(int2 *cells, int beg, int32_t *len_p, int key, int del
     wave_sync();
 }
 
+// slack_add on row x, and the same change in the dense copy when the walk keeps one.  Each cell lives in one row
+// list and each list is written by one wavefront, so the dense entry has one writer.
+__device__ __forceinline__ void row_add(const FtArgs &a, const MhState &h, int x, int key, int d) {
+    slack_add(a.m.row_cells, h.rstart[x], &h.rlen[x], key, d);
+    if (a.m.dense && lane_id() == 0) a.m.dense_w[x * a.m.dense_stride + key] += d;
+}
+
 __device__ __forceinline__ void apply_cell(const FtArgs &a, const MhState &h, int row, int col, int d) {
-    slack_add(a.m.row_cells, h.rstart[row], &h.rlen[row], col, d);
+    row_add(a, h, row, col, d);
     slack_add(a.m.col_cells, h.cstart[col], &h.clen[col], row, d);
 }
 
@@ -1727,7 +1736,7 @@ __device__ __forceinline__ bool lacks_room(const FtArgs &a, const MhState &h, in
 // passes through the same states and the room checked for that order suffices.
 __device__ void mh_apply_share(const FtArgs &a, const MhState &h, const MoveCtx &m, const Table &t, int nb, int wid) {
     const int r = m.r, s = m.s;
-    int2 *rows = a.m.row_cells, *cols = a.m.col_cells;
+    int2 *cols = a.m.col_cells;
     if (wid == 0 || wid == 1) {
         const int x = wid == 0 ? r : s;
         for (int i = 0; i < nb; ++i) {
@@ -1735,15 +1744,15 @@ __device__ void mh_apply_share(const FtArgs &a, const MhState &h, const MoveCtx 
             const int c = t.keys[slot], oc = t.outc[slot];
             if (oc > 0) {
                 const long long d = x == r ? delta_row_r(m, c, oc) : delta_row_s(m, c, oc);
-                if (d) slack_add(rows, h.rstart[x], &h.rlen[x], c, (int)d);
+                if (d) row_add(a, h, x, c, (int)d);
             }
         }
         if (x == r) {
-            if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0)) slack_add(rows, h.rstart[r], &h.rlen[r], r, -m.selfc - m.in_r);
-            if (m.out_s == 0 && m.in_r > 0) slack_add(rows, h.rstart[r], &h.rlen[r], s, m.in_r);
+            if (m.out_r == 0 && (m.selfc > 0 || m.in_r > 0)) row_add(a, h, r, r, -m.selfc - m.in_r);
+            if (m.out_s == 0 && m.in_r > 0) row_add(a, h, r, s, m.in_r);
         } else {
-            if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) slack_add(rows, h.rstart[s], &h.rlen[s], s, m.selfc + m.in_s);
-            if (m.out_r == 0 && m.in_s > 0) slack_add(rows, h.rstart[s], &h.rlen[s], r, -m.in_s);
+            if (m.out_s == 0 && (m.selfc > 0 || m.in_s > 0)) row_add(a, h, s, s, m.selfc + m.in_s);
+            if (m.out_r == 0 && m.in_s > 0) row_add(a, h, s, r, -m.in_s);
         }
     } else if (wid == 2 || wid == 3) {
         const int x = wid == 2 ? r : s;
@@ -1775,8 +1784,8 @@ __device__ void mh_apply_share(const FtArgs &a, const MhState &h, const MoveCtx 
                 if (ds) slack_add(cols, h.cstart[c], &h.clen[c], s, (int)ds);
             }
             if (ic > 0) {
-                slack_add(rows, h.rstart[c], &h.rlen[c], r, -ic);
-                slack_add(rows, h.rstart[c], &h.rlen[c], s, ic);
+                row_add(a, h, c, r, -ic);
+                row_add(a, h, c, s, ic);
             }
         }
     }
@@ -2196,6 +2205,12 @@ void mh_run(sbp_ctx ctx, sbp_bm bm, sbp_graph g, const sbp_params &p, uint32_t e
         k_slack_copy<<<grid_of((int64_t)B * 64), 64, 0, st>>>(B, bm->row_ptr, bm->row_cells, h.rstart, rc);
         k_slack_copy<<<grid_of((int64_t)B * 64), 64, 0, st>>>(B, bm->col_ptr, bm->col_cells, h.cstart, cc);
         FtArgs a = base_args(ctx, bm, g, epoch, p.greedy);
+        // the walk's lookups read a dense copy of the matrix at this layout's start, which its commits keep
+        // exact (row_add); any B up to the dense bound, as a walk makes many dependent lookups per vertex
+        int32_t *dn = const_cast<int32_t *>(bm_ensure_dense(ctx, bm, g, true));
+        a.m.dense = dn;
+        a.m.dense_w = dn;
+        a.m.dense_stride = B;
         a.m.rbeg = h.rstart;
         a.m.rlen = h.rlen;
         a.m.cbeg = h.cstart;

@@ -333,9 +333,9 @@ int bits_for_blocks(int64_t B);
 void bm_fill_cells(sbp_ctx ctx, sbp_bm bm, const unsigned long long *row_keys, const int32_t *row_counts,
                    const unsigned long long *col_keys, const int32_t *col_counts);
 void bm_count_sizes(sbp_ctx ctx, sbp_bm bm);
-// the dense block matrix for lookups when it is wanted (B small, large graph; SBP_AMD_DENSE=0/1 turns it off/on
-// for any B up to the bound), built on demand; nullptr when not wanted
-const int32_t *bm_ensure_dense(sbp_ctx ctx, sbp_bm bm, sbp_graph g);
+// the dense block matrix for lookups when it is wanted (B small, large graph or any_graph; SBP_AMD_DENSE=0/1
+// turns it off/on for any B up to the bound), built on demand; nullptr when not wanted
+const int32_t *bm_ensure_dense(sbp_ctx ctx, sbp_bm bm, sbp_graph g, bool any_graph = false);
 // update.hip: the cells after committed moves (prev: the assignment before them); false when the rebuild must run
 bool bm_update_moves(sbp_ctx ctx, sbp_graph g, sbp_bm bm, const int32_t *prev, int64_t moved_hint);
 bool bm_update_wanted(sbp_graph g);  // graph large enough for the update (or SBP_AMD_INCREMENTAL forcing it)

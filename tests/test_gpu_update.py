@@ -114,3 +114,21 @@ def test_partition_dense_bit_exact(ctx, alg):
     assert nb_d == nb_o
     assert np.array_equal(a_d, a_o)
     assert st_d.mdl == st_o.mdl
+
+
+def test_mh_partition_without_dense(ctx):
+    """Metropolis-Hastings walks keep a dense copy of the matrix by default (their commits write it); with it
+    turned off (SBP_AMD_DENSE=0) lookups search the slack rows, and the partition is the same."""
+    import os
+    g = dcsbm(1000, 16, gamma=2.5, kmin=1, kmax=100, mean_degree=10, p_in=0.75, seed=0)
+    og = O.OGraph.from_csr(g)
+    a_o, nb_o, st_o = O.sbp(og, O.params("metropolis_hastings", seed=0, threads=8))
+    os.environ["SBP_AMD_DENSE"] = "0"
+    try:
+        dg = ctx.upload(g)
+        a_d, nb_d, st_d = dg.partition(native.params("metropolis_hastings", seed=0))
+    finally:
+        os.environ.pop("SBP_AMD_DENSE", None)
+    assert nb_d == nb_o
+    assert np.array_equal(a_d, a_o)
+    assert st_d.mdl == st_o.mdl
