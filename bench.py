@@ -110,7 +110,6 @@ def cpu_baseline(g, B, seconds, threads):
 
 CLASS_RANGES = ("k <= 8", "8 < k <= 32", "32 < k <= 128", "128 < k <= 256", "256 < k <= 1024", "k > 1024")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ft_propose_c3_traffic.txt")
-C2_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ft_propose_c2_traffic.txt")
 
 
 def class_kernels(shape, greedy=True, B=None):
@@ -453,7 +452,7 @@ def c2_leg(runs, cpu_full=False):
         "num_blocks": nb, "mdl": st.mdl, "nmi": nmi(g.truth, a), "identical_to_oracle": same,
         "phase_ms_per_run": {k: tot[k] / runs for k in ("finetune_ms", "merge_ms", "carry_ms", "carry_load_ms",
                                                         "build_ms", "mdl_ms", "ft_kernel_ms", "ft_high_kernel_ms")},
-        "roofline": roofline(tot, ctx_shape, greedy=True, traffic_file=C2_TRAFFIC_FILE),
+        "roofline": roofline(tot, ctx_shape, greedy=True, traffic_file=None),
         "cpu_full_run": cpu,
     }
 
