@@ -71,9 +71,9 @@ uint64_t sbp_footprint_bytes(int32_t N, int64_t E, int64_t max_degree, int64_t n
     // 96 nnz (16-byte slots, 3 to 6 per cell).  The pool holds at most six at once with the released buffers it
     // keeps for reuse: the triplet's three, the current one, the merged one and a copy.
     uint64_t pool = 6 * (141 * e + 28 * n + 4096);
-    // Graphs of at least 2^22 edges also keep a dense copy of each blockmodel with at most 8,192 blocks: up to
-    // min(N, 8192)^2 counts for each of the six.
-    if (e >= (uint64_t(1) << 22)) {
+    // Graphs of at least 2^22 edges, and every Metropolis-Hastings walk (alone or in hybrid_mcmc), also keep a
+    // dense copy of each blockmodel with at most 8,192 blocks: up to min(N, 8192)^2 counts for each of the six.
+    if (e >= (uint64_t(1) << 22) || algorithm == 1 || algorithm == 2) {
         const uint64_t b = std::min<uint64_t>(n, 8192);
         pool += 6 * 4 * b * b;
     }

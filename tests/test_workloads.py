@@ -65,6 +65,9 @@ def test_footprint_bound_is_monotone_and_covers_the_graph():
     e = 1 << 22
     assert (int(L.sbp_footprint_bytes(100_000, e, 100, 0, 0)) - int(L.sbp_footprint_bytes(100_000, e - 1, 100, 0, 0))
             >= 6 * 4 * 8192 ** 2)
+    # so do Metropolis-Hastings walks on any graph
+    assert (int(L.sbp_footprint_bytes(100_000, e - 1, 100, 0, 1)) - int(L.sbp_footprint_bytes(100_000, e - 1, 100, 0, 0))
+            >= 6 * 4 * 8192 ** 2)
     assert big - int(L.sbp_footprint_bytes(2_000_000, 32_000_000, 2000, 1, 0)) <= (1 << 30) * 5 // 4 + (1 << 20)
 
 
