@@ -526,6 +526,10 @@ static sbp_bm merge_apply(sbp_ctx ctx, sbp_bm bm, sbp_graph g, std::vector<int32
         SBP_HIP(hipStreamSynchronize(s));
         t_copied = now_ms();
         int threads = ctx->carry_threads > 0 ? ctx->carry_threads : sbpcarry::carry_threads();
+        if (ctx->carry_share) {  // the host's carry threads shared among the requests running now
+            const int active = std::max(1, (int)ctx->carry_share->load(std::memory_order_relaxed));
+            threads = std::max(1, sbpcarry::carry_threads() / active);
+        }
         // the parametric carry-out applies merges on a thread of its own beside the pool: leave it a core
         if (!bm->np && B >= sbpcarry::PIPE_MIN_BLOCKS && bm->nnz < sbpcarry::PIPE_MAX_ROW_CELLS * (int64_t)B &&
             threads > 2 && sbpcarry::pipe_enabled())

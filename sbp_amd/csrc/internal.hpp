@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <initializer_list>
 #include <map>
 #include <memory>
@@ -201,6 +202,9 @@ struct sbp_ctx_s {
     int hash_clean_lg = -1;
     int64_t hash_clean_tables = 0;
     int carry_threads = 0;    // host threads of the merge carry-out (0: SBP_AMD_CARRY_THREADS or up to 16)
+    // a server worker's count of the requests running now: each merge phase's carry-out then takes its share of
+    // the host's carry threads (at least one) instead of carry_threads
+    const std::atomic<int32_t> *carry_share = nullptr;
     std::shared_ptr<sbpcarry::Pool> carry_pool;  // the carry-out's threads, kept between merge phases
     uint64_t peak_bytes = 0;  // largest device footprint seen at the end of a partition (sbp_ctx_memory)
     std::vector<sbp_ctx> slots;  // worker contexts of sbp_partition_batch, created on first use

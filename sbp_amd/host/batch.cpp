@@ -225,6 +225,7 @@ struct sbp_server_s {
     int64_t next_ticket = 0;
     uint64_t in_use = 0, peak_in_use = 0;
     int32_t running = 0, max_running = 0;
+    std::atomic<int32_t> active{0};  // running, read by the workers' carry-outs without the lock
     bool stop = false;
     std::string err;
 
@@ -246,6 +247,7 @@ struct sbp_server_s {
             in_use += r.footprint;
             peak_in_use = std::max(peak_in_use, in_use);
             max_running = std::max(max_running, ++running);
+            active.store(running, std::memory_order_relaxed);
             Request job = r;  // the fields read below; the map may rehash while unlocked
             lk.unlock();
             int rc = dev_ok ? SBP_OK : SBP_E_HIP;
@@ -271,6 +273,7 @@ struct sbp_server_s {
             d.state = DONE;
             in_use -= d.footprint;
             --running;
+            active.store(running, std::memory_order_relaxed);
             cv_done.notify_all();
             cv_work.notify_all();
         }
@@ -300,6 +303,7 @@ int sbp_server_create(int device, int32_t slots, uint64_t hbm_budget_bytes, sbp_
             delete s;
             return rc;
         }
+        w->carry_share = &s->active;  // a request's carry-outs share the host with the others running
         s->workers.push_back(w);
     }
     for (sbp_ctx w : s->workers) s->threads.emplace_back([s, w] { s->work(w); });
